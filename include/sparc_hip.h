@@ -1,0 +1,136 @@
+/*
+ * sparc_hip.h -- C ABI of the MI355X (gfx950) implementation of SPARC's 2D
+ * time-domain solver for the linearized Euler equations with background
+ * flows (option 17: MP_QUIET_PML_2D_DISPL).
+ *
+ * The reference has no library interface: its callers (data_forward.py:14-18,
+ * full.py:17-29,44-59, linesearch.py:15-19) start one `sparc SS LL` process
+ * per source and phase and exchange data through files.  Each entry point
+ * below replaces one piece of that program; the comment names the reference
+ * code it stands for.
+ *
+ * Memory: arrays named "host" are read on the host during the call; arrays
+ * named "device" are HIP device pointers (HBM), e.g. from torch tensors.
+ * Layouts are C row-major: [nz][nx] fields are the reference's column-major
+ * (nx,1,nz) arrays.  All floating point is double.
+ *
+ * Errors: every function returns 0 on success or a negative SPARC_E_* code;
+ * sparc_last_error() gives the message.  Calls on one handle are not thread
+ * safe; work is queued on the handle's stream (sparc_set_stream).
+ */
+#ifndef SPARC_HIP_H
+#define SPARC_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SPARC_OK 0
+#define SPARC_E_ARG (-1)      /* invalid argument / unsupported configuration */
+#define SPARC_E_HIP (-2)      /* HIP runtime error */
+#define SPARC_E_STATE (-3)    /* call out of order (e.g. run before sources) */
+#define SPARC_E_NONFINITE (-4) /* non-finite values detected in the state */
+
+/* Runtime values of the params.i constants read by the 2D path
+ * (params.i:25-28 grid/timestep, :37 cadforcing, :66 nupeak/nuwidth,
+ * :89 npmlhor, :122 npmlbot/npmltop, :139 obsheight, :143 outputcad,
+ * :152 excitdep), the background model file rows (solar_data,
+ * initialize.f90:743-747) and the optional 2D sound speed and stream function
+ * (model_c_lsLL.fits / model_psi_lsLL.fits, driver.f90:94-126). */
+typedef struct sparc_config {
+    int nx, nz;               /* nx: power of two in [16, 1024] */
+    int npmlbot, npmltop, npmlhor;
+    int nsrc;                 /* sources advanced together in one batch */
+    int adjoint;              /* 0: forward ('spectral'), 1: adjoint ('adjoint') */
+    double xlength;           /* cm */
+    double timestep;          /* s */
+    double cadforcing;        /* s */
+    double outputcad;         /* s */
+    double obsheight;         /* cm */
+    double excitdep;          /* cm */
+    double nupeak, nuwidth;   /* values as the reference stores them */
+    const double *model;      /* host [nz][6]: z/Rsun, c, rho, p, g, Gamma (cgs) */
+    const double *c2d;        /* host [nz][nx] sound speed (cm/s) or NULL */
+    const double *psi;        /* host [nz][nx] stream function (Mm) or NULL */
+} sparc_config;
+
+typedef struct sparc_handle sparc_handle;
+
+/* Initialize_all + solar_data + mp_Initialize_RHS_2d + INITIALIZE_STEP and
+ * the flow from psi (initialize.f90:178-474, 731-794; physics2d.f90:23-405;
+ * step.f90:24-41; driver.f90:94-147).  Allocates the handle's HBM state. */
+int sparc_create(const sparc_config *cfg, sparc_handle **out);
+/* Frees the handle (end of the `sparc` process, driver.f90:318-334). */
+int sparc_destroy(sparc_handle *h);
+/* Message of the last error on h (h may be NULL for sparc_create errors). */
+const char *sparc_last_error(const sparc_handle *h);
+/* Queue work on an existing hipStream_t (NULL: the handle's own stream). */
+int sparc_set_stream(sparc_handle *h, void *hip_stream);
+/* Integer / double facts of the handle: "e_rad", "o_rad", "steps",
+ * "cadforcing_step", "forcing_length", "nsrc", "nx", "nz", "nzpml", "nph" / "deltat",
+ * "stretchx", "delta_width", "beta2".."beta5". */
+int sparc_query_int(const sparc_handle *h, const char *name, int *value);
+int sparc_query_double(const sparc_handle *h, const char *name, double *value);
+/* Copies a named background array to host memory (z, stretch, unstretch, g,
+ * rho0, rhoinv, c2rho0, gradp0_z, gradrho0_z, v0x, v0z, az, bzpml, axpml,
+ * bxpml, fwdsource); *n gets the element count. */
+int sparc_get_background(const sparc_handle *h, const char *name, double *host_out, int cap,
+                         int *n);
+/* Replace the 2D sound speed and stream function and redo the background
+ * set-up (model_c / model_psi of a new iterate, driver.f90:94-126). */
+int sparc_set_model(sparc_handle *h, const double *c2d_host, const double *psi_host);
+
+/* Forward sources: DETERMINE_STATUS forward branch (driver.f90:521-571) and
+ * FORWARD_SOURCE (driver.f90:1120-1145) for x positions xsrc (Mm, one per
+ * source as in master.pixels), for time indices 0..maxtime. */
+int sparc_set_forward_sources(sparc_handle *h, const double *xsrc_host, int maxtime);
+/* Adjoint sources: DETERMINE_STATUS adjoint branch (driver.f90:584-594).
+ * vr: device [nsrc][nt][nx] (source.fits of each source), nt slices at
+ * cadence cadforcing, interpolated by LAGRANGE_INTERP (physics.f90:678-737). */
+int sparc_set_adjoint_sources(sparc_handle *h, const double *vr_dev, int nt, int maxtime);
+
+/* TIMESTEPPING (driver.f90:337-419) for time indices 0..maxtime of every
+ * source: resets the state, then steps.  Outputs (device, may be NULL):
+ *   vz_out [nsrc][nout][nx], nout = maxtime/steps + 1: a(:,o_rad,v_z) after
+ *          each step whose index is a multiple of steps (vz_cc.bin records);
+ *   snaps  [nsrc][nout][nf][nz][nx]: the partial-state snapshots
+ *          (WRITE_OUT_PARTIAL_STATE, all_modules.f90:750-784): forward
+ *          nf = 6 {xi_x, xi_z, v_x, v_z, acc_x, acc_z}; adjoint nf = 4
+ *          {v_x, v_z, acc_x, acc_z}. */
+int sparc_run(sparc_handle *h, int maxtime, double *vz_out_dev, double *snaps_dev);
+/* Restart the state at zero and set the step counter (time0 = 0 start). */
+int sparc_reset(sparc_handle *h);
+/* One STEP (step.f90:43-119) at time index t of every source. */
+int sparc_step(sparc_handle *h, int t);
+/* State of every source: device [4][nsrc][nz][nx], fields xi_x, xi_z, v_x, v_z
+ * (a(:,:,:,1/3/4/6)). get finalizes the pending z filter of v_x first. */
+int sparc_get_state(sparc_handle *h, double *a_dev);
+int sparc_set_state(sparc_handle *h, const double *a_dev);
+/* Scratch of the last RHS (scr(:,:,:,1/3/4/6)) is not kept; the PML
+ * auxiliaries are: pmlvars device [2][nsrc][nzpml][nx] (psivz, psip),
+ * pmlvarsx device [2][nsrc][nz][2*npmlhor] (psivx, psi_gradp_x). */
+int sparc_get_pml(sparc_handle *h, double *pmlvars_dev, double *pmlvarsx_dev);
+
+/* ddx (derivatives.f90:28-58) of nrows rows of length nx: device in/out. */
+int sparc_ddx(sparc_handle *h, const double *in_dev, double *out_dev, int nrows);
+/* ddz (derivatives.f90:10-24) of nbatch [nz][nx] fields with boundary mode
+ * ibc (dbyd2.f:17-23, 0..3).  For ibc 2 / 3 the first / last row of out is
+ * the caller-set boundary value, as in dbyd2. */
+int sparc_ddz(sparc_handle *h, const double *in_dev, double *out_dev, int nbatch, int ibc);
+/* ddzkern (kernels.f90:648-661): ddz followed by FILTER_VAR_Z. */
+int sparc_ddz_kern(sparc_handle *h, const double *in_dev, double *out_dev, int nbatch, int ibc);
+
+/* PRODUCE_KERNELS (kernels.f90:13-612) for every source of a pair of runs:
+ * fwd_snaps device [nsrc][nt][6][nz][nx] and adj_snaps [nsrc][nt][4][nz][nx]
+ * as written by sparc_run (label order 0, steps, ...); forward label
+ * (k-1)*steps is paired with adjoint label adj_end-(k-1)*steps.  Uses the
+ * kernel-mode background (kernel_back, initialize.f90:937-979).  Outputs
+ * device [nsrc][nz][nx]: kc (kernel_c), kpsi (kernel_psi), kvx, kvz, hess. */
+int sparc_kernels(sparc_handle *h, int nt, const double *fwd_snaps_dev,
+                  const double *adj_snaps_dev, double *kc_dev, double *kpsi_dev, double *kvx_dev,
+                  double *kvz_dev, double *hess_dev);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

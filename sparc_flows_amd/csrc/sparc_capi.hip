@@ -1,0 +1,1012 @@
+// sparc_capi.hip -- C ABI (include/sparc_hip.h): handle, background set-up,
+// time loop and kernel accumulation of the gfx950 solver.
+//
+// The set-up mirrors the reference's initialisation (initialize.f90,
+// physics2d.f90:23-405, driver.f90:94-147, step.f90:24-41) including its
+// float32 literals; its z- and x-derivatives run on the GPU with the same
+// kernels as the time loop.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/sparc_hip.h"
+#include "sparc_device.h"
+#include "sparc_launch.h"
+
+using namespace sparc;
+
+namespace {
+
+constexpr double kRsun = 6.95989467700e10;   // initialize.f90:37
+constexpr double kPi = 3.141592653589793;    // initialize.f90:37
+
+struct DevArena {
+    std::vector<void *> ptrs;
+    hipError_t alloc(double **p, size_t n)
+    {
+        void *q = nullptr;
+        hipError_t e = hipMalloc(&q, (n ? n : 1) * sizeof(double));
+        if (e != hipSuccess) return e;
+        ptrs.push_back(q);
+        *p = (double *)q;
+        return hipSuccess;
+    }
+    void release()
+    {
+        for (void *q : ptrs) (void)hipFree(q);
+        ptrs.clear();
+    }
+};
+
+}  // namespace
+
+struct sparc_handle {
+    // configuration
+    int nx = 0, nz = 0, S = 0, nb = 0, nt = 0, nh = 0, adjoint = 0;
+    double xlength = 0, timestep = 0, cadforcing = 0, outputcad = 0, obsheight = 0, excitdep = 0;
+    double nupeak = 0, nuwidth = 0;
+    std::vector<double> model, c2d, psi;
+    Dims d{};
+    std::string err;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    int device = 0;
+
+    // host background (kept for queries)
+    std::vector<double> x, eyekx, z, stretch, unstretch, g, c2, rho0, p0, rhoinv, c2rho0, gp0, gr0,
+        v0x, v0z, azf, bzf, az, bz, axpml, bxpml, fwdsource;
+    int e_rad = 0, o_rad = 0, steps = 1, cfs = 1;
+    double delta_width = 0, deltat = 0, stretchx = 0, betas[5] = {0, 0, 0, 0, 0};
+    DbyCoef cf{};
+
+    // device memory
+    DevArena bg_mem, st_mem, src_mem, kern_mem;
+    Bg bg{};
+    Tri tri[4]{};
+    State st{};
+    double *d_scratch = nullptr;    // [S][nz][nx] general scratch
+    double *d_scratch2 = nullptr;
+    int *tctr = nullptr;
+
+    // sources
+    Src src{};
+    int maxtime = -1, forcing_length = 0, num_steps = 0;
+    bool have_src = false;
+    bool vx_pending = false;
+
+    // kernel-mode background (kernel_back, initialize.f90:937-979)
+    double *d_krho0 = nullptr, *d_kc2 = nullptr;
+};
+
+static thread_local std::string g_create_err;
+
+#define HCHECK(h, expr)                                                                    \
+    do {                                                                                   \
+        hipError_t e_ = (expr);                                                            \
+        if (e_ != hipSuccess) {                                                            \
+            (h)->err = std::string(#expr) + ": " + hipGetErrorString(e_);                  \
+            return SPARC_E_HIP;                                                            \
+        }                                                                                  \
+    } while (0)
+
+static int fail(sparc_handle *h, int code, const std::string &msg)
+{
+    h->err = msg;
+    return code;
+}
+
+// ---------------------------------------------------------------------------
+// dbyd2 coefficients and matrix (dbyd2.f:85-195), float32 scalars widened.
+static DbyCoef make_coef(int N2)
+{
+    DbyCoef c{};
+    const float d2i = (float)(N2 - 1);
+    c.c1 = (double)(d2i * 7.0f / 9.0f);
+    c.c2 = (double)(d2i / 36.0f);
+    c.b1 = (double)(-(43.f / 96.f * d2i));
+    c.b2 = (double)(-(5.f / 6.f * d2i));
+    c.b3 = (double)(9.f / 8.f * d2i);
+    c.b4 = (double)(1.f / 6.f * d2i);
+    c.b5 = (double)(-(1.f / 96.f * d2i));
+    c.f1 = (double)(-(10.0f / 3.f * d2i));
+    c.f2 = (double)(-(3.0f * d2i));
+    c.f3 = (double)(6.f * d2i);
+    c.f4 = (double)(1.f / 3.f * d2i);
+    const float alpha2 = 3.f / 4.f, gamma2 = 1.f / 8.f, beta1 = 3.f;
+    c.alpha2 = alpha2;
+    c.beta1 = beta1;
+    c.cnst = (double)(1.f / (alpha2 - beta1 * gamma2));
+    return c;
+}
+
+// Thomas factors of the ibc matrix: lowr, gam = uppr(j-1)/betad(j-1), 1/betad.
+static void make_tri(int N2, int ibc, std::vector<double> &lowr, std::vector<double> &gam,
+                     std::vector<double> &rbet)
+{
+    const float alpha = 1.f / 3.f, alpha2 = 3.f / 4.f, gamma2 = 1.f / 8.f;
+    const float alpha1 = 6.0f, beta1 = 3.f;
+    const float cnst = 1.f / (alpha2 - beta1 * gamma2);
+    const float up1 = (alpha1 * alpha2 - beta1) * cnst;
+    std::vector<double> uppr(N2, 0.0), betad(N2, 0.0);
+    lowr.assign(N2, 0.0);
+    gam.assign(N2, 0.0);
+    rbet.assign(N2, 0.0);
+    for (int j = 2; j <= N2 - 3; j++) uppr[j] = alpha;
+    uppr[0] = (ibc % 2 == 0) ? 0.0 : (double)up1;
+    uppr[1] = alpha2;
+    uppr[N2 - 2] = gamma2;
+    for (int j = 0; j < N2; j++) lowr[j] = uppr[N2 - 1 - j];
+    lowr[N2 - 1] = (ibc != 1 && ibc != 2) ? 0.0 : (double)up1;
+    lowr[0] = 0.0;
+    betad[0] = 1.0;
+    for (int j = 1; j < N2; j++) {                 // mtridag.f:96-100
+        gam[j] = uppr[j - 1] / betad[j - 1];
+        betad[j] = 1.0 - lowr[j] * gam[j];
+    }
+    for (int j = 0; j < N2; j++) rbet[j] = 1.0 / betad[j];
+}
+
+// Upload a host vector into a new device buffer of the arena.
+static hipError_t upload(DevArena &ar, double **dst, const std::vector<double> &v, hipStream_t st)
+{
+    hipError_t e = ar.alloc(dst, v.size());
+    if (e != hipSuccess) return e;
+    return hipMemcpyAsync(*dst, v.data(), v.size() * sizeof(double), hipMemcpyHostToDevice, st);
+}
+
+// GPU z-derivative of [nb][nz][ncols] host data (used by the set-up).
+static int host_ddz(sparc_handle *h, const std::vector<double> &in, std::vector<double> &out,
+                    int ncols, int nbatch, int ibc, bool use_stretch, int tri_ibc)
+{
+    const size_t n = (size_t)ncols * h->nz * nbatch;
+    double *di = nullptr, *dout = nullptr, *dw = nullptr;
+    HCHECK(h, hipMalloc(&di, n * sizeof(double)));
+    HCHECK(h, hipMalloc(&dout, n * sizeof(double)));
+    HCHECK(h, hipMalloc(&dw, n * sizeof(double)));
+    HCHECK(h, hipMemcpyAsync(di, in.data(), n * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    HCHECK(h, launch_fill(dout, n, 0.0, h->stream));
+    HCHECK(h, launch_ddz_cols(di, dout, ncols, h->nz, nbatch, ibc,
+                              use_stretch ? h->bg.stretch : nullptr, h->cf, h->tri[tri_ibc], dw, 0,
+                              h->stream));
+    out.resize(n);
+    HCHECK(h, hipMemcpyAsync(out.data(), dout, n * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HCHECK(h, hipStreamSynchronize(h->stream));
+    (void)hipFree(di);
+    (void)hipFree(dout);
+    (void)hipFree(dw);
+    return SPARC_OK;
+}
+
+static int host_ddx(sparc_handle *h, const std::vector<double> &in, std::vector<double> &out,
+                    int nrows)
+{
+    const size_t n = (size_t)nrows * h->nx;
+    double *di = nullptr, *dout = nullptr;
+    HCHECK(h, hipMalloc(&di, n * sizeof(double)));
+    HCHECK(h, hipMalloc(&dout, n * sizeof(double)));
+    HCHECK(h, hipMemcpyAsync(di, in.data(), n * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    HCHECK(h, launch_ddx_rows(di, dout, nrows, h->nx, h->bg.tw, h->bg.eyekx, h->stretchx, h->stream));
+    out.resize(n);
+    HCHECK(h, hipMemcpyAsync(out.data(), dout, n * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HCHECK(h, hipStreamSynchronize(h->stream));
+    (void)hipFree(di);
+    (void)hipFree(dout);
+    return SPARC_OK;
+}
+
+// Grid, derivative tables (initialize.f90:194-225, 466-474).
+static int setup_grid(sparc_handle *h)
+{
+    const int nx = h->nx, nz = h->nz;
+    h->x.resize(nx);
+    h->eyekx.resize(nx / 2 + 1);
+    const double normx = 1.0 / (double)nx;                   // initialize.f90:36
+    for (int i = 0; i < nx; i++) h->x[i] = (double)i / (double)nx;
+    for (int k = 0; k <= nx / 2; k++) h->eyekx[k] = (double)k * normx * 2.0 * kPi;
+    h->stretchx = kRsun / h->xlength;
+    std::vector<double> tw(2 * (size_t)nx);
+    for (int k = 0; k < nx; k++) {
+        // exp(-2 pi i k / nx) from the exact octant values where they exist
+        const double ang = -2.0 * kPi * (double)k / (double)nx;
+        double c = std::cos(ang), s = std::sin(ang);
+        if (4 * k == nx || 4 * k == 3 * nx) c = 0.0;
+        if (2 * k == nx) s = 0.0;
+        tw[2 * k] = c;
+        tw[2 * k + 1] = s;
+    }
+    double *dtw = nullptr;
+    HCHECK(h, upload(h->bg_mem, &dtw, tw, h->stream));
+    h->bg.tw = (const double2 *)dtw;
+    double *p = nullptr;
+    HCHECK(h, upload(h->bg_mem, &p, h->eyekx, h->stream));
+    h->bg.eyekx = p;
+    h->bg.stretchx = h->stretchx;
+
+    h->cf = make_coef(nz);
+    for (int ibc = 0; ibc < 4; ibc++) {
+        std::vector<double> lowr, gam, rbet;
+        make_tri(nz, ibc, lowr, gam, rbet);
+        double *a = nullptr, *b = nullptr, *c = nullptr;
+        HCHECK(h, upload(h->bg_mem, &a, lowr, h->stream));
+        HCHECK(h, upload(h->bg_mem, &b, gam, h->stream));
+        HCHECK(h, upload(h->bg_mem, &c, rbet, h->stream));
+        h->tri[ibc] = Tri{a, b, c};
+    }
+    // solar_data z column and stretch = 1/dbyd2(z) (initialize.f90:466-470)
+    h->z.resize(nz);
+    for (int k = 0; k < nz; k++) h->z[k] = h->model[6 * (size_t)k];
+    std::vector<double> us;
+    int r = host_ddz(h, h->z, us, 1, 1, 1, false, 1);
+    if (r) return r;
+    h->unstretch = us;
+    h->stretch.resize(nz);
+    for (int k = 0; k < nz; k++) h->stretch[k] = 1.0 / h->unstretch[k];
+    HCHECK(h, upload(h->bg_mem, &p, h->stretch, h->stream));
+    h->bg.stretch = p;
+    HCHECK(h, upload(h->bg_mem, &p, h->unstretch, h->stream));
+    h->bg.unstretch = p;
+    h->deltat = h->timestep * 1e6 / kRsun;                 // initialize.f90:474
+    return SPARC_OK;
+}
+
+// solar_data (initialize.f90:731-794), mp_Initialize_RHS_2d
+// (physics2d.f90:23-357), flow from psi (driver.f90:94-147).
+static int setup_background(sparc_handle *h)
+{
+    const int nx = h->nx, nz = h->nz;
+    const size_t n2 = (size_t)nx * nz;
+    const double dimc = 1e6, dimrho = 1e-3, diml = kRsun;
+    const double dimc2 = std::pow(dimc, 2.0);
+    h->c2.assign(n2, 0.0);
+    h->rho0.assign(n2, 0.0);
+    h->p0.assign(n2, 0.0);
+    h->g.assign(nz, 0.0);
+    for (int k = 0; k < nz; k++) {
+        const double *row = &h->model[6 * (size_t)k];
+        const double cs = row[1] / dimc;
+        for (int i = 0; i < nx; i++) {
+            const size_t q = (size_t)k * nx + i;
+            const double c = h->c2d.empty() ? cs : h->c2d[q] / dimc;
+            h->c2[q] = c * c;
+            h->rho0[q] = row[2] / dimrho;
+            h->p0[q] = row[3] / (dimrho * dimc2);
+        }
+        h->g[k] = row[4] * diml / dimc2;
+    }
+    // e_rad, o_rad, delta_width (physics2d.f90:33-57)
+    {
+        const double temp2 = 1.0 + h->excitdep / kRsun;
+        double temp = 1.0;
+        for (int k = 0; k < nz; k++)
+            if (std::fabs(h->z[k] - temp2) <= temp) { temp = std::fabs(h->z[k] - temp2); h->e_rad = k; }
+        const double temp_r = 1.0 + h->obsheight / kRsun;
+        temp = 1.0;
+        for (int k = 0; k < nz; k++)
+            if (std::fabs(h->z[k] - temp_r) <= temp) { temp = std::fabs(h->z[k] - temp_r); h->o_rad = k; }
+        const int r = h->adjoint ? h->o_rad : h->e_rad;
+        if (r < 1 || r > nz - 2) return fail(h, SPARC_E_ARG, "source/observation row at the grid edge");
+        h->delta_width = 2.0 / (h->z[r + 1] - h->z[r - 1]);
+    }
+    h->rhoinv.assign(n2, 0.0);
+    h->c2rho0.assign(n2, 0.0);
+    for (size_t q = 0; q < n2; q++) {                       // physics2d.f90:227-228
+        h->rhoinv[q] = 1.0 / h->rho0[q];
+        h->c2rho0[q] = h->c2[q] * h->rho0[q];
+    }
+    int r = host_ddz(h, h->rho0, h->gr0, nx, 1, 1, true, 1);  // :230
+    if (r) return r;
+    r = host_ddz(h, h->p0, h->gp0, nx, 1, 1, true, 1);        // :231
+    if (r) return r;
+    // vertical PML (physics2d.f90:288-333)
+    const int nb = h->nb, nt = h->nt;
+    h->azf.assign(n2, 0.0);
+    h->bzf.assign(n2, 0.0);
+    const double *z = h->z.data();
+    const double alpha0 = kPi * (double)0.005f * diml / dimc;
+    for (int k = 0; k < nz; k++) {
+        for (int i = 0; i < nx; i++) {
+            double alphaz = alpha0, dampz = 0.0;
+            bool inpml = false;
+            const size_t q = (size_t)k * nx + i;
+            if (k < nb) {
+                alphaz = kPi * (double)0.005f * diml / dimc * (z[k] - z[0]) / (z[nb - 1] - z[0]);
+                const double rr = (z[nb - 1] - z[k]) / (z[nb - 1] - z[0]);
+                dampz = 9.0 * std::sqrt(h->c2[i]) / (z[nb - 1] - z[0]) * (rr * rr);
+                inpml = true;
+            }
+            if (k > nz - nt - 1) {
+                alphaz = kPi * (double)0.005f * diml / dimc * (z[nz - 1] - z[k]) / (z[nz - 1] - z[nz - nt]);
+                const double rr = (z[nz - nt] - z[k]) / (z[nz - 1] - z[nz - nt]);
+                dampz = 9.0 * std::sqrt(h->c2[(size_t)(nz - 6) * nx + i]) / (z[nz - 1] - z[nz - nt]) * (rr * rr);
+                inpml = true;
+            }
+            if (inpml) {
+                const double fd = alphaz / (dampz + alphaz);
+                h->gp0[q] *= fd;
+                h->gr0[q] *= fd;
+                if (i == 0) h->g[k] *= fd;
+            }
+            h->bzf[q] = -alphaz - dampz;
+            h->azf[q] = -dampz;
+        }
+    }
+    const int nzpml = nb + nt;
+    h->az.assign((size_t)nzpml * nx, 0.0);
+    h->bz.assign((size_t)nzpml * nx, 0.0);
+    for (int pj = 0; pj < nzpml; pj++) {
+        const int k = pj < nb ? pj : nz - nt + (pj - nb);
+        for (int i = 0; i < nx; i++) {
+            h->az[(size_t)pj * nx + i] = h->azf[(size_t)k * nx + i];
+            h->bz[(size_t)pj * nx + i] = h->bzf[(size_t)k * nx + i];
+        }
+    }
+    // horizontal PML (physics2d.f90:339-357), c of column x = 1
+    const int nh = h->nh, nph = 2 * nh;
+    h->axpml.assign((size_t)nph * nz, 0.0);
+    h->bxpml.assign((size_t)nph * nz, 0.0);
+    {
+        const double *xx = h->x.data();
+        const double beta0 = (double)0.005f * diml / dimc * kPi;
+        for (int k = 0; k < nz; k++) {
+            const double c1 = std::sqrt(h->c2[(size_t)k * nx]);
+            for (int i = 0; i < nh; i++) {
+                const double rr = (xx[i] - xx[nh - 1]) / (xx[nh - 1] - xx[0]);
+                const double ax = -(2.25 * c1 / (xx[nh - 1] - xx[0]) * (rr * rr));
+                h->axpml[(size_t)k * nph + i] = ax;
+                h->bxpml[(size_t)k * nph + i] = ax - beta0 * (xx[i] - xx[0]) / (xx[nh - 1] - xx[0]);
+            }
+            for (int c = 0; c < nh; c++) {
+                const int i = nx - nh + c;
+                const double rr = (xx[nx - nh] - xx[i]) / (xx[nx - nh] - xx[nx - 1]);
+                const double ax = -(2.25 * c1 / (xx[nx - 1] - xx[nx - nh]) * (rr * rr));
+                h->axpml[(size_t)k * nph + nh + c] = ax;
+                h->bxpml[(size_t)k * nph + nh + c] =
+                    ax - beta0 * (xx[nx - 1] - xx[i]) / (xx[nx - 1] - xx[nx - nh]);
+            }
+        }
+    }
+    // flow from psi (driver.f90:94-126), reversed for the adjoint (:144-147)
+    h->v0x.assign(n2, 0.0);
+    h->v0z.assign(n2, 0.0);
+    if (!h->psi.empty()) {
+        std::vector<double> pv(n2), dz, dx;
+        for (size_t q = 0; q < n2; q++)
+            pv[q] = h->rho0[q] * std::sqrt(h->c2[q]) * (h->psi[q] / (diml / 1.0e8));
+        r = host_ddz(h, pv, dz, nx, 1, 1, true, 1);
+        if (r) return r;
+        r = host_ddx(h, pv, dx, nz);
+        if (r) return r;
+        for (size_t q = 0; q < n2; q++) {
+            h->v0x[q] = -dz[q] / h->rho0[q];
+            h->v0z[q] = dx[q] / h->rho0[q];
+            if (h->adjoint) { h->v0x[q] = -h->v0x[q]; h->v0z[q] = -h->v0z[q]; }
+        }
+    }
+    // INITIALIZE_STEP (step.f90:24-41)
+    {
+        const double o2 = 0.5, o3 = (double)0.1665579725791151184f;
+        const double o4 = (double)0.03950410250586729987f, o5 = (double)0.007810706393337838236f;
+        double *b = h->betas;
+        b[4] = o2;
+        b[3] = o3 / b[4];
+        b[2] = o4 / (b[4] * b[3]);
+        b[1] = o5 / (b[4] * b[3] * b[2]);
+        b[0] = 0.0;
+        for (int s = 0; s < 5; s++) b[s] *= h->deltat;
+    }
+    // upload
+    h->bg_mem.release();
+    return SPARC_OK;
+}
+
+static int upload_background(sparc_handle *h)
+{
+    // grid tables were uploaded into bg_mem by setup_grid and released by
+    // setup_background's caller order; re-upload everything here
+    const int nz = h->nz;
+    double *p = nullptr;
+    std::vector<double> tw(2 * (size_t)h->nx);
+    for (int k = 0; k < h->nx; k++) {
+        const double ang = -2.0 * kPi * (double)k / (double)h->nx;
+        double c = std::cos(ang), s = std::sin(ang);
+        if (4 * k == h->nx || 4 * k == 3 * h->nx) c = 0.0;
+        if (2 * k == h->nx) s = 0.0;
+        tw[2 * k] = c;
+        tw[2 * k + 1] = s;
+    }
+    HCHECK(h, upload(h->bg_mem, &p, tw, h->stream));
+    h->bg.tw = (const double2 *)p;
+    HCHECK(h, upload(h->bg_mem, &p, h->eyekx, h->stream)); h->bg.eyekx = p;
+    HCHECK(h, upload(h->bg_mem, &p, h->stretch, h->stream)); h->bg.stretch = p;
+    HCHECK(h, upload(h->bg_mem, &p, h->unstretch, h->stream)); h->bg.unstretch = p;
+    for (int ibc = 0; ibc < 4; ibc++) {
+        std::vector<double> lowr, gam, rbet;
+        make_tri(nz, ibc, lowr, gam, rbet);
+        double *a = nullptr, *b = nullptr, *c = nullptr;
+        HCHECK(h, upload(h->bg_mem, &a, lowr, h->stream));
+        HCHECK(h, upload(h->bg_mem, &b, gam, h->stream));
+        HCHECK(h, upload(h->bg_mem, &c, rbet, h->stream));
+        h->tri[ibc] = Tri{a, b, c};
+    }
+    HCHECK(h, upload(h->bg_mem, &p, h->rhoinv, h->stream)); h->bg.rhoinv = p;
+    HCHECK(h, upload(h->bg_mem, &p, h->rho0, h->stream)); h->bg.rho0 = p;
+    HCHECK(h, upload(h->bg_mem, &p, h->c2rho0, h->stream)); h->bg.c2rho0 = p;
+    HCHECK(h, upload(h->bg_mem, &p, h->gp0, h->stream)); h->bg.gp0 = p;
+    HCHECK(h, upload(h->bg_mem, &p, h->gr0, h->stream)); h->bg.gr0 = p;
+    HCHECK(h, upload(h->bg_mem, &p, h->v0x, h->stream)); h->bg.v0x = p;
+    HCHECK(h, upload(h->bg_mem, &p, h->v0z, h->stream)); h->bg.v0z = p;
+    HCHECK(h, upload(h->bg_mem, &p, h->g, h->stream)); h->bg.g = p;
+    HCHECK(h, upload(h->bg_mem, &p, h->az, h->stream)); h->bg.az = p;
+    HCHECK(h, upload(h->bg_mem, &p, h->bz, h->stream)); h->bg.bz = p;
+    HCHECK(h, upload(h->bg_mem, &p, h->axpml, h->stream)); h->bg.ax = p;
+    HCHECK(h, upload(h->bg_mem, &p, h->bxpml, h->stream)); h->bg.bx = p;
+    const int r = h->adjoint ? h->o_rad : h->e_rad;
+    std::vector<double> rs(h->rho0.begin() + (size_t)r * h->nx, h->rho0.begin() + (size_t)(r + 1) * h->nx);
+    HCHECK(h, upload(h->bg_mem, &p, rs, h->stream)); h->bg.rho0_src = p;
+    // kernel-mode background: dimc = 10.0**6., dimrho = 10.0**(-3.0) in float32
+    // (kernel_back, initialize.f90:952-975; model_c, kernels.f90:35-42)
+    {
+        const size_t n2 = (size_t)h->nx * nz;
+        const double kdimc = (double)1e6f, kdimrho = (double)1e-3f;
+        std::vector<double> kr(n2), kc(n2);
+        for (int k = 0; k < nz; k++)
+            for (int i = 0; i < h->nx; i++) {
+                const size_t q = (size_t)k * h->nx + i;
+                const double c = (h->c2d.empty() ? h->model[6 * (size_t)k + 1] : h->c2d[q]) / kdimc;
+                kc[q] = std::pow(c, 2.0);
+                kr[q] = h->model[6 * (size_t)k + 2] / kdimrho;
+            }
+        HCHECK(h, upload(h->bg_mem, &h->d_krho0, kr, h->stream));
+        HCHECK(h, upload(h->bg_mem, &h->d_kc2, kc, h->stream));
+    }
+    HCHECK(h, hipStreamSynchronize(h->stream));
+    return SPARC_OK;
+}
+
+static int alloc_state(sparc_handle *h)
+{
+    const size_t f = (size_t)h->S * h->nz * h->nx;
+    const size_t pz = (size_t)2 * h->S * h->d.nzpml * h->nx;
+    const size_t px = (size_t)2 * h->S * h->nz * h->d.nph;
+    DevArena &m = h->st_mem;
+    State &s = h->st;
+    for (int i = 0; i < 4; i++) {
+        HCHECK(h, m.alloc(&s.a[i], f));
+        HCHECK(h, m.alloc(&s.t[i], f));
+    }
+    HCHECK(h, m.alloc(&s.vx_unf, f));
+    HCHECK(h, m.alloc(&s.pz_base, pz));
+    HCHECK(h, m.alloc(&s.pz_tmp, pz));
+    HCHECK(h, m.alloc(&s.px_base, px));
+    HCHECK(h, m.alloc(&s.px_tmp, px));
+    HCHECK(h, m.alloc(&s.dxixdx, f));
+    HCHECK(h, m.alloc(&s.dxvx, f));
+    HCHECK(h, m.alloc(&s.dxvz, f));
+    HCHECK(h, m.alloc(&s.p, f));
+    HCHECK(h, m.alloc(&s.rvx, f));
+    HCHECK(h, m.alloc(&s.topbc, (size_t)h->S * h->nx));
+    HCHECK(h, m.alloc(&s.w_xiz, f));
+    HCHECK(h, m.alloc(&s.w_vx, f));
+    HCHECK(h, m.alloc(&s.w_vz, f));
+    HCHECK(h, m.alloc(&s.w_p, f));
+    HCHECK(h, m.alloc(&s.w_rho, f));
+    HCHECK(h, m.alloc(&h->d_scratch, f));
+    HCHECK(h, m.alloc(&h->d_scratch2, f));
+    double *t = nullptr;
+    HCHECK(h, m.alloc(&t, 1));
+    h->tctr = (int *)t;
+    return SPARC_OK;
+}
+
+static int zero_state(sparc_handle *h)
+{
+    const size_t f = (size_t)h->S * h->nz * h->nx;
+    const size_t pz = (size_t)2 * h->S * h->d.nzpml * h->nx;
+    const size_t px = (size_t)2 * h->S * h->nz * h->d.nph;
+    State &s = h->st;
+    for (int i = 0; i < 4; i++) {
+        HCHECK(h, launch_fill(s.a[i], f, 0.0, h->stream));
+        HCHECK(h, launch_fill(s.t[i], f, 0.0, h->stream));
+    }
+    HCHECK(h, launch_fill(s.vx_unf, f, 0.0, h->stream));
+    HCHECK(h, launch_fill(s.pz_base, pz, 0.0, h->stream));
+    HCHECK(h, launch_fill(s.pz_tmp, pz, 0.0, h->stream));
+    HCHECK(h, launch_fill(s.px_base, px, 0.0, h->stream));
+    HCHECK(h, launch_fill(s.px_tmp, px, 0.0, h->stream));
+    HCHECK(h, launch_fill(s.topbc, (size_t)h->S * h->nx, 0.0, h->stream));
+    h->vx_pending = false;
+    return SPARC_OK;
+}
+
+// ---------------------------------------------------------------------------
+// One step: R1 C1 R C2 R C3 R C4 R C5 R5 (see sparc_kernels.hip).
+static int enqueue_step(sparc_handle *h, int toff, bool filt_vx, const Sink &sk)
+{
+    const Dims &d = h->d;
+    State &s = h->st;
+    Clock clk{h->tctr, toff, h->steps};
+    const double *b = h->betas;   // b[1..4] = betas(2..5)
+    {
+        RowArgs ra{};
+        ra.xi_x = s.a[0];
+        ra.v_z = s.a[3];
+        ra.v_x_out = nullptr;
+        ra.bx1 = b[1];
+        ra.last1 = 0;
+        ra.filt_vx = filt_vx ? 1 : 0;
+        HCHECK(h, launch_row(0, d, h->bg, s, ra, clk, sk, h->stream));
+    }
+    for (int stg = 1; stg <= 5; stg++) {
+        ColArgs ca{};
+        const double *const *T = (stg == 1) ? (const double *const *)s.a : (const double *const *)s.t;
+        for (int f = 0; f < 4; f++) {
+            ca.T[f] = T[f];
+            ca.TN[f] = s.t[f];
+        }
+        ca.last = stg == 5;
+        ca.bnext = stg == 5 ? h->deltat : b[stg];
+        HCHECK(h, launch_col(d, h->bg, h->tri[1], h->tri[2], h->tri[3], h->cf, s, ca, h->src, clk,
+                             sk, h->stream));
+        RowArgs ra{};
+        if (stg < 5) {
+            ra.xi_x = s.t[0];
+            ra.v_z = s.t[3];
+            ra.v_x_out = s.t[2];
+            ra.bx2 = b[stg];
+            ra.last2 = 0;
+            ra.last1 = (stg + 1 == 5);
+            ra.bx1 = ra.last1 ? h->deltat : b[stg + 1];
+            HCHECK(h, launch_row(1, d, h->bg, s, ra, clk, sk, h->stream));
+        } else {
+            ra.v_x_out = s.vx_unf;
+            ra.bx2 = h->deltat;
+            ra.last2 = 1;
+            HCHECK(h, launch_row(2, d, h->bg, s, ra, clk, sk, h->stream));
+        }
+    }
+    return SPARC_OK;
+}
+
+static Sink empty_sink(sparc_handle *h)
+{
+    Sink sk{};
+    sk.orow = h->o_rad;
+    sk.adjoint = h->adjoint;
+    sk.nfield = h->adjoint ? 4 : 6;
+    return sk;
+}
+
+// ---------------------------------------------------------------------------
+extern "C" {
+
+const char *sparc_last_error(const sparc_handle *h)
+{
+    if (!h) return g_create_err.c_str();
+    return h->err.c_str();
+}
+
+int sparc_create(const sparc_config *cfg, sparc_handle **out)
+{
+    if (!cfg || !out || !cfg->model) {
+        g_create_err = "sparc_create: null argument";
+        return SPARC_E_ARG;
+    }
+    const int nx = cfg->nx, nz = cfg->nz;
+    if (nx < 16 || nx > 1024 || (nx & (nx - 1))) {
+        g_create_err = "sparc_create: nx must be a power of two in [16, 1024]";
+        return SPARC_E_ARG;
+    }
+    if (nz < 16 || cfg->npmlbot < 1 || cfg->npmltop < 1 || cfg->npmlbot + cfg->npmltop >= nz ||
+        cfg->npmlhor < 2 || 2 * cfg->npmlhor >= nx || cfg->nsrc < 1 || nz < 6 + 5) {
+        g_create_err = "sparc_create: grid / PML sizes out of range";
+        return SPARC_E_ARG;
+    }
+    if (!(cfg->timestep > 0) || !(cfg->xlength > 0) || cfg->outputcad < cfg->timestep ||
+        cfg->cadforcing < cfg->timestep) {
+        g_create_err = "sparc_create: timestep / cadences out of range";
+        return SPARC_E_ARG;
+    }
+    sparc_handle *h = new sparc_handle();
+    h->nx = nx; h->nz = nz; h->S = cfg->nsrc;
+    h->nb = cfg->npmlbot; h->nt = cfg->npmltop; h->nh = cfg->npmlhor;
+    h->adjoint = cfg->adjoint ? 1 : 0;
+    h->xlength = cfg->xlength; h->timestep = cfg->timestep; h->cadforcing = cfg->cadforcing;
+    h->outputcad = cfg->outputcad; h->obsheight = cfg->obsheight; h->excitdep = cfg->excitdep;
+    h->nupeak = cfg->nupeak; h->nuwidth = cfg->nuwidth;
+    h->model.assign(cfg->model, cfg->model + 6 * (size_t)nz);
+    if (cfg->c2d) h->c2d.assign(cfg->c2d, cfg->c2d + (size_t)nx * nz);
+    if (cfg->psi) h->psi.assign(cfg->psi, cfg->psi + (size_t)nx * nz);
+    h->d = Dims{nx, nz, h->S, h->nb, h->nt, h->nb + h->nt, h->nh, 2 * h->nh};
+    h->steps = (int)std::floor(h->outputcad / h->timestep);      // driver.f90:214
+    h->cfs = (int)std::floor(h->cadforcing / h->timestep);       // initialize.f90:991
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+        g_create_err = "sparc_create: no HIP device";
+        delete h;
+        return SPARC_E_HIP;
+    }
+    (void)hipGetDevice(&h->device);
+    if (hipStreamCreate(&h->stream) != hipSuccess) {
+        g_create_err = "sparc_create: hipStreamCreate failed";
+        delete h;
+        return SPARC_E_HIP;
+    }
+    h->own_stream = true;
+    int r = setup_grid(h);
+    if (!r) r = setup_background(h);
+    if (!r) r = upload_background(h);
+    if (!r) r = alloc_state(h);
+    if (!r) r = zero_state(h);
+    if (!r && hipStreamSynchronize(h->stream) != hipSuccess) r = fail(h, SPARC_E_HIP, "set-up sync");
+    if (r) {
+        g_create_err = "sparc_create: " + h->err;
+        sparc_destroy(h);
+        return r;
+    }
+    *out = h;
+    return SPARC_OK;
+}
+
+int sparc_destroy(sparc_handle *h)
+{
+    if (!h) return SPARC_OK;
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    h->bg_mem.release();
+    h->st_mem.release();
+    h->src_mem.release();
+    h->kern_mem.release();
+    if (h->own_stream && h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+    return SPARC_OK;
+}
+
+int sparc_set_stream(sparc_handle *h, void *stream)
+{
+    if (!h) return SPARC_E_ARG;
+    HCHECK(h, hipStreamSynchronize(h->stream));
+    if (stream) {
+        if (h->own_stream) (void)hipStreamDestroy(h->stream);
+        h->stream = (hipStream_t)stream;
+        h->own_stream = false;
+    } else if (!h->own_stream) {
+        HCHECK(h, hipStreamCreate(&h->stream));
+        h->own_stream = true;
+    }
+    return SPARC_OK;
+}
+
+int sparc_query_int(const sparc_handle *h, const char *name, int *v)
+{
+    if (!h || !name || !v) return SPARC_E_ARG;
+    const std::string n(name);
+    if (n == "e_rad") *v = h->e_rad;
+    else if (n == "o_rad") *v = h->o_rad;
+    else if (n == "steps") *v = h->steps;
+    else if (n == "cadforcing_step") *v = h->cfs;
+    else if (n == "forcing_length") *v = h->forcing_length;
+    else if (n == "num_steps") *v = h->num_steps;
+    else if (n == "nsrc") *v = h->S;
+    else if (n == "nx") *v = h->nx;
+    else if (n == "nz") *v = h->nz;
+    else if (n == "nzpml") *v = h->d.nzpml;
+    else if (n == "nph") *v = h->d.nph;
+    else return SPARC_E_ARG;
+    return SPARC_OK;
+}
+
+int sparc_query_double(const sparc_handle *h, const char *name, double *v)
+{
+    if (!h || !name || !v) return SPARC_E_ARG;
+    const std::string n(name);
+    if (n == "deltat") *v = h->deltat;
+    else if (n == "stretchx") *v = h->stretchx;
+    else if (n == "delta_width") *v = h->delta_width;
+    else if (n.size() == 5 && n.compare(0, 4, "beta") == 0 && n[4] >= '1' && n[4] <= '5') *v = h->betas[n[4] - '1'];
+    else return SPARC_E_ARG;
+    return SPARC_OK;
+}
+
+int sparc_get_background(const sparc_handle *h, const char *name, double *out, int cap, int *n)
+{
+    if (!h || !name) return SPARC_E_ARG;
+    const std::string s(name);
+    const std::vector<double> *v = nullptr;
+    if (s == "z") v = &h->z;
+    else if (s == "stretch") v = &h->stretch;
+    else if (s == "unstretch") v = &h->unstretch;
+    else if (s == "g") v = &h->g;
+    else if (s == "rho0") v = &h->rho0;
+    else if (s == "rhoinv") v = &h->rhoinv;
+    else if (s == "c2rho0") v = &h->c2rho0;
+    else if (s == "c2") v = &h->c2;
+    else if (s == "gradp0_z") v = &h->gp0;
+    else if (s == "gradrho0_z") v = &h->gr0;
+    else if (s == "v0x") v = &h->v0x;
+    else if (s == "v0z") v = &h->v0z;
+    else if (s == "az") v = &h->azf;
+    else if (s == "bzpml") v = &h->bzf;
+    else if (s == "axpml") v = &h->axpml;
+    else if (s == "bxpml") v = &h->bxpml;
+    else if (s == "fwdsource") v = &h->fwdsource;
+    else if (s == "eyekx") v = &h->eyekx;
+    else return SPARC_E_ARG;
+    if (n) *n = (int)v->size();
+    if (out) {
+        if ((size_t)cap < v->size()) return SPARC_E_ARG;
+        std::memcpy(out, v->data(), v->size() * sizeof(double));
+    }
+    return SPARC_OK;
+}
+
+int sparc_set_model(sparc_handle *h, const double *c2d, const double *psi)
+{
+    if (!h) return SPARC_E_ARG;
+    HCHECK(h, hipStreamSynchronize(h->stream));
+    const size_t n2 = (size_t)h->nx * h->nz;
+    if (c2d) h->c2d.assign(c2d, c2d + n2); else h->c2d.clear();
+    if (psi) h->psi.assign(psi, psi + n2); else h->psi.clear();
+    h->bg_mem.release();
+    int r = setup_grid(h);
+    if (!r) r = setup_background(h);
+    if (!r) r = upload_background(h);
+    return r;
+}
+
+static void set_times(sparc_handle *h, int maxtime)
+{
+    h->maxtime = maxtime;
+    const int cfs = h->cfs > 0 ? h->cfs : 1;
+    h->forcing_length = maxtime / cfs + 1;                   // initialize.f90:994
+    const int ns = (int)std::floor((double)maxtime * h->timestep / h->cadforcing) + 2;
+    h->num_steps = 2 * (ns / 2) + 2;                         // driver.f90:217-218
+}
+
+int sparc_set_forward_sources(sparc_handle *h, const double *xsrc, int maxtime)
+{
+    if (!h || !xsrc || maxtime < 0) return SPARC_E_ARG;
+    if (h->adjoint) return fail(h, SPARC_E_STATE, "forward sources on an adjoint handle");
+    HCHECK(h, hipStreamSynchronize(h->stream));
+    set_times(h, maxtime);
+    const int nx = h->nx;
+    // FORWARD_SOURCE (driver.f90:1120-1145)
+    const int nfwd = maxtime + 8;
+    h->fwdsource.assign(nfwd, 0.0);
+    for (int k = 0; k < nfwd; k++) {
+        const double ttime = (double)(float)k * h->timestep;
+        const double dd = ttime - (double)0.5f / h->nupeak;
+        double v = std::cos(2.0 * kPi * h->nupeak * ttime) * std::exp(-(2.0 * (dd * dd) * (h->nuwidth * h->nuwidth)));
+        if (std::fabs(v) < (double)0.00001f) v = 0.0;
+        h->fwdsource[k] = v;
+    }
+    const double sc = (double)1e-5f;
+    for (int k = 0; k < nfwd; k++) h->fwdsource[k] = h->fwdsource[k] * sc;
+    // x profile per source (driver.f90:536-553)
+    std::vector<double> vr((size_t)h->S * nx);
+    const double e8 = (double)1e-8f;
+    const double dx = (h->x[1] - h->x[0]) * h->xlength * e8;
+    const double k43 = (double)0.43f * kPi;
+    std::vector<double> xd(nx);
+    for (int b = 0; b < h->S; b++) {
+        int imin = 0;
+        for (int i = 0; i < nx; i++) {
+            xd[i] = (h->x[i] - 0.5) * h->xlength * e8 - xsrc[b];
+            if (std::fabs(xd[i]) < std::fabs(xd[imin])) imin = i;
+        }
+        for (int i = 0; i < nx; i++) {
+            const double sincx = (i == imin) ? 1.0 : std::sin(k43 * xd[i]) / (k43 * xd[i]);
+            vr[(size_t)b * nx + i] = std::exp(-(xd[i] * xd[i]) / (400.0 * (dx * dx))) * sincx;
+        }
+    }
+    h->src_mem.release();
+    double *dvr = nullptr, *dfw = nullptr;
+    HCHECK(h, upload(h->src_mem, &dvr, vr, h->stream));
+    HCHECK(h, upload(h->src_mem, &dfw, h->fwdsource, h->stream));
+    HCHECK(h, hipStreamSynchronize(h->stream));
+    Src &s = h->src;
+    s.mode = 1;
+    s.row = h->e_rad;
+    s.limit = h->forcing_length * h->cfs;
+    s.delta_width = h->delta_width;
+    s.vr = dvr;
+    s.nvr = 1;
+    s.fwd = dfw;
+    s.nfwd = nfwd;
+    s.timestep = h->timestep;
+    s.cadforcing = h->cadforcing;
+    s.num_steps = h->num_steps;
+    h->have_src = true;
+    return SPARC_OK;
+}
+
+int sparc_set_adjoint_sources(sparc_handle *h, const double *vr_dev, int nt, int maxtime)
+{
+    if (!h || !vr_dev || nt < 1 || maxtime < 0) return SPARC_E_ARG;
+    if (!h->adjoint) return fail(h, SPARC_E_STATE, "adjoint sources on a forward handle");
+    HCHECK(h, hipStreamSynchronize(h->stream));
+    set_times(h, maxtime);
+    const int nx = h->nx, fl = h->forcing_length;
+    const int nvr = fl + 8;                                  // driver.f90:588-591
+    h->src_mem.release();
+    double *dvr = nullptr;
+    HCHECK(h, h->src_mem.alloc(&dvr, (size_t)h->S * nvr * nx));
+    HCHECK(h, launch_fill(dvr, (size_t)h->S * nvr * nx, 0.0, h->stream));
+    const int ncopy = nt < fl ? nt : fl;
+    for (int b = 0; b < h->S; b++)
+        HCHECK(h, hipMemcpyAsync(dvr + (size_t)b * nvr * nx, vr_dev + (size_t)b * nt * nx,
+                                 (size_t)ncopy * nx * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+    HCHECK(h, hipStreamSynchronize(h->stream));
+    Src &s = h->src;
+    s.mode = 2;
+    s.row = h->o_rad;
+    s.limit = h->forcing_length * h->cfs;
+    s.delta_width = h->delta_width;
+    s.vr = dvr;
+    s.nvr = nvr;
+    s.fwd = nullptr;
+    s.nfwd = 0;
+    s.timestep = h->timestep;
+    s.cadforcing = h->cadforcing;
+    s.num_steps = h->num_steps;
+    h->have_src = true;
+    return SPARC_OK;
+}
+
+int sparc_reset(sparc_handle *h)
+{
+    if (!h) return SPARC_E_ARG;
+    int r = zero_state(h);
+    if (r) return r;
+    HCHECK(h, launch_set_int(h->tctr, 0, h->stream));
+    return SPARC_OK;
+}
+
+int sparc_step(sparc_handle *h, int t)
+{
+    if (!h || t < 0) return SPARC_E_ARG;
+    if (!h->have_src) return fail(h, SPARC_E_STATE, "sources not set");
+    Sink sk = empty_sink(h);
+    HCHECK(h, launch_set_int(h->tctr, t, h->stream));
+    int r = enqueue_step(h, 0, false, sk);
+    if (r) return r;
+    HCHECK(h, launch_tick(h->tctr, 1, h->stream));
+    Clock clk{h->tctr, 0, h->steps};
+    HCHECK(h, launch_finalize_vx(h->d, h->st, clk, sk, h->stream));
+    return SPARC_OK;
+}
+
+int sparc_run(sparc_handle *h, int maxtime, double *vz_out, double *snaps)
+{
+    if (!h || maxtime < 0) return SPARC_E_ARG;
+    if (!h->have_src) return fail(h, SPARC_E_STATE, "sources not set");
+    if (maxtime != h->maxtime) return fail(h, SPARC_E_STATE, "maxtime differs from the sources' maxtime");
+    int r = sparc_reset(h);
+    if (r) return r;
+    Sink sk = empty_sink(h);
+    const int nout = maxtime / h->steps + 1;
+    sk.vz_out = vz_out;
+    sk.nout = vz_out ? nout : 0;
+    sk.snap = snaps;
+    sk.nsnap = snaps ? nout : 0;
+    for (int t = 0; t <= maxtime; t++) {
+        r = enqueue_step(h, 0, t > 0, sk);
+        if (r) return r;
+        HCHECK(h, launch_tick(h->tctr, 1, h->stream));
+    }
+    Clock clk{h->tctr, 0, h->steps};
+    HCHECK(h, launch_finalize_vx(h->d, h->st, clk, sk, h->stream));
+    return SPARC_OK;
+}
+
+int sparc_get_state(sparc_handle *h, double *a_dev)
+{
+    if (!h || !a_dev) return SPARC_E_ARG;
+    const size_t f = (size_t)h->S * h->nz * h->nx;
+    for (int i = 0; i < 4; i++)
+        HCHECK(h, hipMemcpyAsync(a_dev + i * f, h->st.a[i], f * sizeof(double), hipMemcpyDeviceToDevice,
+                                 h->stream));
+    return SPARC_OK;
+}
+
+int sparc_set_state(sparc_handle *h, const double *a_dev)
+{
+    if (!h || !a_dev) return SPARC_E_ARG;
+    const size_t f = (size_t)h->S * h->nz * h->nx;
+    for (int i = 0; i < 4; i++)
+        HCHECK(h, hipMemcpyAsync(h->st.a[i], a_dev + i * f, f * sizeof(double), hipMemcpyDeviceToDevice,
+                                 h->stream));
+    return SPARC_OK;
+}
+
+int sparc_get_pml(sparc_handle *h, double *pv, double *px)
+{
+    if (!h) return SPARC_E_ARG;
+    const size_t pz = (size_t)2 * h->S * h->d.nzpml * h->nx;
+    const size_t pxn = (size_t)2 * h->S * h->nz * h->d.nph;
+    if (pv) HCHECK(h, hipMemcpyAsync(pv, h->st.pz_base, pz * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+    if (px) HCHECK(h, hipMemcpyAsync(px, h->st.px_base, pxn * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+    return SPARC_OK;
+}
+
+int sparc_ddx(sparc_handle *h, const double *in, double *out, int nrows)
+{
+    if (!h || !in || !out || nrows < 1) return SPARC_E_ARG;
+    HCHECK(h, launch_ddx_rows(in, out, nrows, h->nx, h->bg.tw, h->bg.eyekx, h->stretchx, h->stream));
+    return SPARC_OK;
+}
+
+static int ddz_impl(sparc_handle *h, const double *in, double *out, int nbatch, int ibc, int filt)
+{
+    if (!h || !in || !out || nbatch < 1 || ibc < 0 || ibc > 3) return SPARC_E_ARG;
+    double *work = nullptr;
+    HCHECK(h, hipMalloc((void **)&work, (size_t)nbatch * h->nz * h->nx * sizeof(double)));
+    HCHECK(h, launch_ddz_cols(in, out, h->nx, h->nz, nbatch, ibc, h->bg.stretch, h->cf, h->tri[ibc], work,
+                              filt, h->stream));
+    HCHECK(h, hipStreamSynchronize(h->stream));
+    HCHECK(h, hipFree(work));
+    return SPARC_OK;
+}
+
+int sparc_ddz(sparc_handle *h, const double *in, double *out, int nbatch, int ibc)
+{
+    return ddz_impl(h, in, out, nbatch, ibc, 0);
+}
+
+int sparc_ddz_kern(sparc_handle *h, const double *in, double *out, int nbatch, int ibc)
+{
+    return ddz_impl(h, in, out, nbatch, ibc, 1);
+}
+
+int sparc_kernels(sparc_handle *h, int nt, const double *fwd, const double *adj, double *kc,
+                  double *kpsi, double *kvx, double *kvz, double *hess)
+{
+    if (!h || nt < 1 || !fwd || !adj || !kc || !kpsi || !kvx || !kvz || !hess) return SPARC_E_ARG;
+    const int S = h->S;
+    const size_t n2 = (size_t)h->nx * h->nz, n = (size_t)S * n2;
+    double *buf = nullptr;
+    // D[8] + KC KX KZ + t1 t2 + ddz work
+    HCHECK(h, hipMalloc((void **)&buf, 14 * n * sizeof(double)));
+    double *D = buf, *KC = buf + 8 * n, *KX = buf + 9 * n, *KZ = buf + 10 * n, *t1 = buf + 11 * n,
+           *t2 = buf + 12 * n, *work = buf + 13 * n;
+    double *g = h->d_scratch, *g2 = h->d_scratch2;
+    HCHECK(h, launch_fill(KC, 3 * n, 0.0, h->stream));
+    HCHECK(h, launch_fill(hess, n, 0.0, h->stream));
+    const Tri &tr1 = h->tri[1];
+    for (int k = 0; k < nt; k++) {
+        const int ka = nt - 1 - k;
+        // forward fields: xi_x 0, xi_z 1, v_x 2, v_z 3; adjoint: v_x 0, v_z 1
+        struct Job { const double *src; int slot, nfield, field, op; };  // op 0 ddx, 1 ddzkern
+        const Job jobs[8] = {{fwd, k, 6, 0, 0}, {fwd, k, 6, 1, 1}, {adj, ka, 4, 0, 0}, {adj, ka, 4, 1, 1},
+                             {fwd, k, 6, 2, 0}, {fwd, k, 6, 3, 0}, {fwd, k, 6, 2, 1}, {fwd, k, 6, 3, 1}};
+        for (int j = 0; j < 8; j++) {
+            const Job &jb = jobs[j];
+            HCHECK(h, launch_gather(jb.src, g, S, nt, jb.slot, jb.nfield, jb.field, n2, h->stream));
+            if (jb.op == 0)
+                HCHECK(h, launch_ddx_rows(g, D + j * n, S * h->nz, h->nx, h->bg.tw, h->bg.eyekx,
+                                          h->stretchx, h->stream));
+            else
+                HCHECK(h, launch_ddz_cols(g, D + j * n, h->nx, h->nz, S, 1, h->bg.stretch, h->cf, tr1,
+                                          work, 1, h->stream));
+        }
+        HCHECK(h, launch_kern_accum(fwd, adj, S, nt, k, n2, D, KC, KX, KZ, hess, h->stream));
+    }
+    (void)g2;
+    // curl (bcz = 0, kernels.f90:832) and unit scalings (:498-579)
+    HCHECK(h, launch_ddz_cols(KX, t1, h->nx, h->nz, S, 0, h->bg.stretch, h->cf, h->tri[0], work, 1,
+                              h->stream));
+    HCHECK(h, launch_ddx_rows(KZ, t2, S * h->nz, h->nx, h->bg.tw, h->bg.eyekx, h->stretchx, h->stream));
+    const double diml = kRsun, dimc = (double)1e6f, e8 = (double)1e-8f, e5 = (double)1e-5f;
+    const double dimen = std::pow(diml * e8 * h->xlength * e8 * dimc * e5, -1.0) * 1.0;
+    const double dimen_c = std::pow(diml * e8 * h->xlength * e8, -1.0) * 1.0;
+    const int stepskern = (int)std::floor(h->outputcad / h->timestep);
+    HCHECK(h, launch_kern_final(S, n2, h->d_krho0, h->d_kc2, (double)stepskern, h->timestep, dimen,
+                                dimen_c, t1, t2, KC, KX, KZ, kc, kpsi, kvx, kvz, h->stream));
+    HCHECK(h, hipStreamSynchronize(h->stream));
+    HCHECK(h, hipFree(buf));
+    return SPARC_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,261 @@
+// sparc_device.h -- device-side data structures and building blocks shared by
+// the solver kernels: the register/LDS FFT used by the x-derivative and the
+// compact-FD (dbyd2) coefficient set used by the z-derivative.
+//
+// Layout in HBM: every field of every source is an [S][nz][nx] array of
+// doubles, x fastest (the reference's column-major a(nx,1,nz) per source).
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace sparc {
+
+// ---------------------------------------------------------------------------
+// Problem geometry.
+struct Dims {
+    int nx, nz, S;            // grid and number of sources in the batch
+    int npmlbot, npmltop, nzpml;
+    int nh, nph;              // horizontal PML width and 2*nh
+};
+
+// Background arrays shared by every source of a handle (device pointers).
+struct Bg {
+    const double *rhoinv, *rho0, *c2rho0, *gp0, *gr0, *v0x, *v0z;  // [nz][nx]
+    const double *g, *stretch, *unstretch;                 // [nz]
+    const double *az, *bz;                                 // [nzpml][nx]
+    const double *ax, *bx;                                 // [nz][nph]
+    const double *rho0_src;                                // [nx] rho0 at the injection row
+    const double *eyekx;                                   // [nx/2+1]
+    const double2 *tw;                                     // [nx] exp(-2 pi i k/nx)
+    double stretchx;
+};
+
+// The dbyd2 right-hand-side coefficients: dbyd2.f keeps them as default
+// REAL (float32); they are stored here already widened to double.
+struct DbyCoef {
+    double c1, c2;                 // interior 6th order (dbyd2.f:85-86)
+    double b1, b2, b3, b4, b5;     // rows 2 and N2-1, 5th order (dbyd2.f:103-107)
+    double f1, f2, f3, f4;         // rows 1 and N2 (dbyd2.f:121-124)
+    double alpha2, beta1, cnst;    // preconditioning (dbyd2.f:156-162)
+};
+
+// Thomas factorization of the constant dbyd2 matrix, one set per ibc.
+// lowr[j]: sub-diagonal, gam[j] = uppr[j-1]/betad[j-1], rbet[j] = 1/betad[j].
+struct Tri {
+    const double *lowr, *gam, *rbet;
+};
+
+// Prognostic state and work arrays of a batch (device pointers).
+struct State {
+    double *a[4];           // xi_x, xi_z, v_x, v_z   [S][nz][nx]
+    double *t[4];           // RK stage state
+    double *vx_unf;         // v_x after the final update, before the z filter
+    double *pz_base, *pz_tmp;  // [2][S][nzpml][nx]  (psivz, psip)
+    double *px_base, *px_tmp;  // [2][S][nz][nph]    (psivx, psi_gradp_x)
+    double *dxixdx, *dxvx, *dxvz, *p, *rvx;  // [S][nz][nx]
+    double *topbc;          // [S][nx]
+    double *w_xiz, *w_vx, *w_vz, *w_p, *w_rho;  // column sweep scratch [S][nz][nx]
+};
+
+// Source term of the RHS (physics2d.f90:1051-1061).
+struct Src {
+    int mode;               // 0 none, 1 forward, 2 adjoint
+    int row;                // e_rad (forward) or o_rad (adjoint), 0-based
+    int limit;              // forcing_length * cadforcing_step
+    double delta_width;
+    const double *vr;       // forward: [S][nx]; adjoint: [S][nvr][nx]
+    int nvr;
+    const double *fwd;      // forward time series [nfwd]
+    int nfwd;
+    double timestep, cadforcing;
+    int num_steps;
+};
+
+// Per-launch time information. t = *tctr + toff.
+struct Clock {
+    const int *tctr;
+    int toff;
+    int steps;              // output cadence in steps
+};
+
+// Snapshot / output sink of a run (device pointers, may be null).
+struct Sink {
+    double *vz_out;         // [S][nout][nx]
+    int nout;
+    double *snap;           // [S][nsnap][nfield][nz][nx]
+    int nsnap, nfield;      // forward: 6 (xix, xiz, vx, vz, accx, accz); adjoint: 4 (vx, vz, accx, accz)
+    int orow;               // o_rad
+    int adjoint;
+};
+
+// Arguments of one row pass (k_row).
+struct RowArgs {
+    const double *xi_x;     // stage s+1 xi_x (MODE 0: a; MODE 1: temp)
+    const double *v_z;      // stage s+1 v_z
+    double *v_x_out;        // MODE 1: temp v_x of stage s+1; MODE 2: vx_unf
+    double bx2;             // beta_{s+1} (or deltat when last2) for the dx p part
+    int last2;
+    double bx1;             // beta_{s+2} (or deltat when last1) for the x-derivative part
+    int last1;
+    int filt_vx;            // MODE 0: apply the pending v_x filter
+};
+
+// Arguments of one column pass (k_col).
+struct ColArgs {
+    const double *T[4];     // stage state (xi_x unused)
+    double *TN[4];          // next stage state (last stage: a_new, unfiltered)
+    double bnext;           // beta_{s+1}, or deltat at the last stage
+    int last;
+};
+
+// ---------------------------------------------------------------------------
+// Twiddles of the 32-point circle, exp(2 pi i k/32) = kC32[k] + i kS32[k].
+__device__ constexpr double kC32[32] = {
+    1.0, 0.98078528040323043, 0.92387953251128674, 0.83146961230254524,
+    0.70710678118654757, 0.55557023301960229, 0.38268343236508984, 0.19509032201612833,
+    0.0, -0.19509032201612833, -0.38268343236508984, -0.55557023301960229,
+    -0.70710678118654757, -0.83146961230254524, -0.92387953251128674, -0.98078528040323043,
+    -1.0, -0.98078528040323043, -0.92387953251128674, -0.83146961230254524,
+    -0.70710678118654757, -0.55557023301960229, -0.38268343236508984, -0.19509032201612833,
+    0.0, 0.19509032201612833, 0.38268343236508984, 0.55557023301960229,
+    0.70710678118654757, 0.83146961230254524, 0.92387953251128674, 0.98078528040323043};
+__device__ constexpr double kS32[32] = {
+    0.0, 0.19509032201612833, 0.38268343236508984, 0.55557023301960229,
+    0.70710678118654757, 0.83146961230254524, 0.92387953251128674, 0.98078528040323043,
+    1.0, 0.98078528040323043, 0.92387953251128674, 0.83146961230254524,
+    0.70710678118654757, 0.55557023301960229, 0.38268343236508984, 0.19509032201612833,
+    0.0, -0.19509032201612833, -0.38268343236508984, -0.55557023301960229,
+    -0.70710678118654757, -0.83146961230254524, -0.92387953251128674, -0.98078528040323043,
+    -1.0, -0.98078528040323043, -0.92387953251128674, -0.83146961230254524,
+    -0.70710678118654757, -0.55557023301960229, -0.38268343236508984, -0.19509032201612833};
+
+__host__ __device__ constexpr int ilog2c(int n) { return n <= 1 ? 0 : 1 + ilog2c(n >> 1); }
+__host__ __device__ constexpr int bitrev(int i, int bits)
+{
+    return bits == 0 ? 0 : ((i & 1) << (bits - 1)) | bitrev(i >> 1, bits - 1);
+}
+
+// In-register N-point DFT (N in {2,4,8,16,32}); sign -1 forward, +1 inverse,
+// unnormalized.  Radix-2 decimation in time; all indices are compile-time.
+template <int N, int SIGN>
+__device__ __forceinline__ void dft_reg(double (&re)[N], double (&im)[N])
+{
+    constexpr int B = ilog2c(N);
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+        const int j = bitrev(i, B);
+        if (i < j) {
+            double t = re[i]; re[i] = re[j]; re[j] = t;
+            t = im[i]; im[i] = im[j]; im[j] = t;
+        }
+    }
+#pragma unroll
+    for (int len = 2; len <= N; len <<= 1) {
+#pragma unroll
+        for (int k = 0; k < len / 2; k++) {
+            const int q = k * (32 / len);
+            const double wr = kC32[q];
+            const double wi = SIGN * kS32[q];
+#pragma unroll
+            for (int s = 0; s < N; s += len) {
+                const int a = s + k, b = s + k + len / 2;
+                double tr, ti;
+                if (k == 0) { tr = re[b]; ti = im[b]; }
+                else { tr = re[b] * wr - im[b] * wi; ti = re[b] * wi + im[b] * wr; }
+                re[b] = re[a] - tr; im[b] = im[a] - ti;
+                re[a] = re[a] + tr; im[a] = im[a] + ti;
+            }
+        }
+    }
+}
+
+// Spectral x-derivative of two real rows at once (z = f + i g), following
+// ddx (derivatives.f90:28-58): forward DFT, multiply bin k by i*eyekx(k)
+// (negative frequencies by the conjugate, the Nyquist bin by zero: FFTW's
+// c2r drops its imaginary part), inverse DFT, times stretchx.
+//
+// nx = N1*N2.  A team of N1 lanes; lane t holds z[t + N1*j2], j2 < N2, in
+// (re, im) and gets D(z) back at the same positions.  lds points at the
+// team's N2*(N1+1) double2 slots.  Every lane of the block must call this
+// (it contains block barriers).
+template <int N1, int N2>
+__device__ __forceinline__ void fft_deriv_team(double (&re)[N2], double (&im)[N2], int t,
+                                               double2 *lds, const double2 *__restrict__ tw,
+                                               const double *__restrict__ eyekx, double stretchx)
+{
+    constexpr int n = N1 * N2;
+    constexpr int LD = N1 + 1;
+    // step 1: DFT over j2, twiddle W_n^(t*k2), transpose through LDS
+    dft_reg<N2, -1>(re, im);
+#pragma unroll
+    for (int k2 = 0; k2 < N2; k2++) {
+        double r = re[k2], i = im[k2];
+        if (k2 != 0) {
+            const double2 w = tw[t * k2];
+            const double rr = r * w.x - i * w.y;
+            i = r * w.y + i * w.x;
+            r = rr;
+        }
+        lds[k2 * LD + t] = make_double2(r, i);
+    }
+    __syncthreads();
+    // step 2 (lanes t < N2): DFT over j1 -> X[t + N2*k1]; derivative multiplier;
+    // inverse DFT over k1; conjugate twiddle; back to LDS
+    if (t < N2) {
+        double ar[N1], ai[N1];
+#pragma unroll
+        for (int j1 = 0; j1 < N1; j1++) {
+            const double2 v = lds[t * LD + j1];
+            ar[j1] = v.x; ai[j1] = v.y;
+        }
+        dft_reg<N1, -1>(ar, ai);
+#pragma unroll
+        for (int k1 = 0; k1 < N1; k1++) {
+            const int k = t + N2 * k1;
+            double w;
+            if (k < n / 2) w = eyekx[k];
+            else if (k == n / 2) w = 0.0;
+            else w = -eyekx[n - k];
+            const double r = ar[k1];
+            ar[k1] = -(ai[k1] * w);
+            ai[k1] = r * w;
+        }
+        dft_reg<N1, +1>(ar, ai);
+#pragma unroll
+        for (int j1 = 0; j1 < N1; j1++) {
+            double r = ar[j1], i = ai[j1];
+            if (j1 != 0 && t != 0) {
+                const double2 w = tw[j1 * t];   // conj: exp(+2 pi i j1 t/n)
+                const double rr = r * w.x + i * w.y;
+                i = i * w.x - r * w.y;
+                r = rr;
+            }
+            lds[t * LD + j1] = make_double2(r, i);
+        }
+    }
+    __syncthreads();
+    // step 3: inverse DFT over k2 -> x[t + N1*j2]
+#pragma unroll
+    for (int k2 = 0; k2 < N2; k2++) {
+        const double2 v = lds[k2 * LD + t];
+        re[k2] = v.x; im[k2] = v.y;
+    }
+    __syncthreads();
+    dft_reg<N2, +1>(re, im);
+#pragma unroll
+    for (int j2 = 0; j2 < N2; j2++) {
+        re[j2] = re[j2] * stretchx;
+        im[j2] = im[j2] * stretchx;
+    }
+}
+
+// Factorization N1 x N2 of the supported row lengths (N1 >= N2).
+template <int NX> struct FftShape;
+template <> struct FftShape<16> { static constexpr int N1 = 4, N2 = 4; };
+template <> struct FftShape<32> { static constexpr int N1 = 8, N2 = 4; };
+template <> struct FftShape<64> { static constexpr int N1 = 8, N2 = 8; };
+template <> struct FftShape<128> { static constexpr int N1 = 16, N2 = 8; };
+template <> struct FftShape<256> { static constexpr int N1 = 16, N2 = 16; };
+template <> struct FftShape<512> { static constexpr int N1 = 32, N2 = 16; };
+template <> struct FftShape<1024> { static constexpr int N1 = 32, N2 = 32; };
+
+}  // namespace sparc

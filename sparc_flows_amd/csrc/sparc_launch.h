@@ -1,0 +1,33 @@
+// sparc_launch.h -- host-side launchers of the solver kernels.
+#pragma once
+#include "sparc_device.h"
+
+namespace sparc {
+
+hipError_t launch_ddz_cols(const double *in, double *out, int ncols, int nz, int nbatch, int ibc,
+                           const double *stretch, const DbyCoef &cf, const Tri &tr, double *work,
+                           int filt, hipStream_t st);
+hipError_t launch_ddx_rows(const double *in, double *out, int nrows, int nx, const double2 *tw,
+                           const double *eyekx, double stretchx, hipStream_t st);
+hipError_t launch_row(int mode, const Dims &d, const Bg &bg, const State &s, const RowArgs &ra,
+                      const Clock &c, const Sink &sk, hipStream_t st);
+hipError_t launch_col(const Dims &d, const Bg &bg, const Tri &t1, const Tri &t2, const Tri &t3,
+                      const DbyCoef &cf, const State &s, const ColArgs &ca, const Src &src,
+                      const Clock &c, const Sink &sk, hipStream_t st);
+hipError_t launch_finalize_vx(const Dims &d, const State &s, const Clock &c, const Sink &sk,
+                              hipStream_t st);
+hipError_t launch_tick(int *tctr, int inc, hipStream_t st);
+hipError_t launch_set_int(int *p, int v, hipStream_t st);
+hipError_t launch_fill(double *p, size_t n, double v, hipStream_t st);
+
+hipError_t launch_gather(const double *src, double *dst, int S, int nslot, int slot, int nfield,
+                         int field, size_t n2, hipStream_t st);
+hipError_t launch_kern_accum(const double *fwd, const double *adj, int S, int nt, int k, size_t n2,
+                             const double *D, double *KC, double *KX, double *KZ, double *H,
+                             hipStream_t st);
+hipError_t launch_kern_final(int S, size_t n2, const double *rho0, const double *c2, double steps,
+                             double dt, double dimen, double dimen_c, const double *t1, const double *t2,
+                             const double *KC, const double *KX, const double *KZ, double *kc,
+                             double *kpsi, double *kvx, double *kvz, hipStream_t st);
+
+}  // namespace sparc

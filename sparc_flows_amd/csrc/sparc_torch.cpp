@@ -1,0 +1,215 @@
+// sparc_torch.cpp -- thin PyTorch-ROCm binding of the C ABI (include/sparc_hip.h).
+//
+// Registers torch.classes.sparc.Solver: one sparc_handle, with float64 device
+// tensors in and out.  Shapes and dtypes are checked here; all work is done
+// by libsparc_hip.so on the stream given by set_stream (the Python layer
+// passes torch.cuda.current_stream()).
+#include <ATen/ATen.h>
+#include <torch/custom_class.h>
+#include <torch/library.h>
+
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../include/sparc_hip.h"
+
+namespace {
+
+void check(int rc, const sparc_handle *h, const char *what)
+{
+    TORCH_CHECK(rc == SPARC_OK, what, " failed (", rc, "): ", sparc_last_error(h));
+}
+
+void check_dev(const at::Tensor &t, const char *name)
+{
+    TORCH_CHECK(t.is_cuda(), name, " must be a GPU tensor");
+    TORCH_CHECK(t.scalar_type() == at::kDouble, name, " must be float64");
+    TORCH_CHECK(t.is_contiguous(), name, " must be contiguous");
+}
+
+at::Tensor cpu_f64(const at::Tensor &t, int64_t numel, const char *name)
+{
+    at::Tensor c = t.to(at::kCPU, at::kDouble).contiguous();
+    TORCH_CHECK(c.numel() == numel, name, " has ", c.numel(), " elements, expected ", numel);
+    return c;
+}
+
+struct Solver : torch::CustomClassHolder {
+    sparc_handle *h = nullptr;
+    int64_t nx = 0, nz = 0, nsrc = 0, adjoint = 0, device = 0;
+
+    // model: [nz][6] (cpu or gpu); c2d, psi: [nz][nx] or empty tensors
+    Solver(int64_t nx_, int64_t nz_, int64_t npmlbot, int64_t npmltop, int64_t npmlhor, int64_t nsrc_,
+           int64_t adjoint_, std::vector<double> scalars, at::Tensor model, at::Tensor c2d, at::Tensor psi,
+           int64_t device_)
+    {
+        TORCH_CHECK(scalars.size() == 8, "scalars: xlength, timestep, cadforcing, outputcad, obsheight, "
+                                         "excitdep, nupeak, nuwidth");
+        nx = nx_; nz = nz_; nsrc = nsrc_; adjoint = adjoint_; device = device_;
+        at::Tensor m = cpu_f64(model, nz * 6, "model");
+        at::Tensor c = c2d.numel() ? cpu_f64(c2d, nx * nz, "c2d") : at::Tensor();
+        at::Tensor p = psi.numel() ? cpu_f64(psi, nx * nz, "psi") : at::Tensor();
+        sparc_config cfg{};
+        cfg.nx = (int)nx; cfg.nz = (int)nz;
+        cfg.npmlbot = (int)npmlbot; cfg.npmltop = (int)npmltop; cfg.npmlhor = (int)npmlhor;
+        cfg.nsrc = (int)nsrc; cfg.adjoint = (int)adjoint;
+        cfg.xlength = scalars[0]; cfg.timestep = scalars[1]; cfg.cadforcing = scalars[2];
+        cfg.outputcad = scalars[3]; cfg.obsheight = scalars[4]; cfg.excitdep = scalars[5];
+        cfg.nupeak = scalars[6]; cfg.nuwidth = scalars[7];
+        cfg.model = m.data_ptr<double>();
+        cfg.c2d = c.defined() ? c.data_ptr<double>() : nullptr;
+        cfg.psi = p.defined() ? p.data_ptr<double>() : nullptr;
+        check(sparc_create(&cfg, &h), nullptr, "sparc_create");
+    }
+    ~Solver() override
+    {
+        if (h) sparc_destroy(h);
+    }
+    at::TensorOptions opts() const { return at::TensorOptions().dtype(at::kDouble).device(at::kCUDA, device); }
+
+    void set_stream(int64_t stream) { check(sparc_set_stream(h, (void *)stream), h, "sparc_set_stream"); }
+
+    int64_t query_int(std::string name)
+    {
+        int v = 0;
+        check(sparc_query_int(h, name.c_str(), &v), h, "sparc_query_int");
+        return v;
+    }
+    double query_double(std::string name)
+    {
+        double v = 0;
+        check(sparc_query_double(h, name.c_str(), &v), h, "sparc_query_double");
+        return v;
+    }
+    at::Tensor background(std::string name)
+    {
+        int n = 0;
+        check(sparc_get_background(h, name.c_str(), nullptr, 0, &n), h, "sparc_get_background");
+        at::Tensor out = at::empty({n}, at::TensorOptions().dtype(at::kDouble));
+        check(sparc_get_background(h, name.c_str(), out.data_ptr<double>(), n, &n), h, "sparc_get_background");
+        return out;
+    }
+    void set_model(at::Tensor c2d, at::Tensor psi)
+    {
+        at::Tensor c = c2d.numel() ? cpu_f64(c2d, nx * nz, "c2d") : at::Tensor();
+        at::Tensor p = psi.numel() ? cpu_f64(psi, nx * nz, "psi") : at::Tensor();
+        check(sparc_set_model(h, c.defined() ? c.data_ptr<double>() : nullptr,
+                              p.defined() ? p.data_ptr<double>() : nullptr),
+              h, "sparc_set_model");
+    }
+    void set_forward_sources(at::Tensor xsrc, int64_t maxtime)
+    {
+        at::Tensor x = cpu_f64(xsrc, nsrc, "xsrc");
+        check(sparc_set_forward_sources(h, x.data_ptr<double>(), (int)maxtime), h, "sparc_set_forward_sources");
+    }
+    void set_adjoint_sources(at::Tensor vr, int64_t maxtime)
+    {
+        check_dev(vr, "vr");
+        TORCH_CHECK(vr.dim() == 3 && vr.size(0) == nsrc && vr.size(2) == nx, "vr must be [nsrc][nt][nx]");
+        check(sparc_set_adjoint_sources(h, vr.data_ptr<double>(), (int)vr.size(1), (int)maxtime), h,
+              "sparc_set_adjoint_sources");
+    }
+    int64_t steps() { return query_int("steps"); }
+
+    // returns (vz [nsrc][nout][nx], snaps [nsrc][nout][nf][nz][nx] or empty)
+    std::tuple<at::Tensor, at::Tensor> run(int64_t maxtime, bool snapshots)
+    {
+        const int64_t nout = maxtime / steps() + 1;
+        at::Tensor vz = at::zeros({nsrc, nout, nx}, opts());
+        at::Tensor sn = snapshots ? at::zeros({nsrc, nout, adjoint ? 4 : 6, nz, nx}, opts())
+                                  : at::empty({0}, opts());
+        check(sparc_run(h, (int)maxtime, vz.data_ptr<double>(), snapshots ? sn.data_ptr<double>() : nullptr),
+              h, "sparc_run");
+        return {vz, sn};
+    }
+    void reset() { check(sparc_reset(h), h, "sparc_reset"); }
+    void step(int64_t t) { check(sparc_step(h, (int)t), h, "sparc_step"); }
+    at::Tensor get_state()
+    {
+        at::Tensor a = at::empty({4, nsrc, nz, nx}, opts());
+        check(sparc_get_state(h, a.data_ptr<double>()), h, "sparc_get_state");
+        return a;
+    }
+    void set_state(at::Tensor a)
+    {
+        check_dev(a, "state");
+        TORCH_CHECK(a.numel() == 4 * nsrc * nz * nx, "state must be [4][nsrc][nz][nx]");
+        check(sparc_set_state(h, a.data_ptr<double>()), h, "sparc_set_state");
+    }
+    std::tuple<at::Tensor, at::Tensor> get_pml()
+    {
+        const int64_t nzpml = query_int("nzpml");
+        const int64_t nph = query_int("nph");
+        at::Tensor pv = at::empty({2, nsrc, nzpml, nx}, opts());
+        at::Tensor px = at::empty({2, nsrc, nz, nph}, opts());
+        check(sparc_get_pml(h, pv.data_ptr<double>(), px.data_ptr<double>()), h, "sparc_get_pml");
+        return {pv, px};
+    }
+    at::Tensor ddx(at::Tensor in)
+    {
+        check_dev(in, "in");
+        TORCH_CHECK(in.numel() % nx == 0 && in.size(-1) == nx, "last dimension must be nx");
+        at::Tensor out = at::empty_like(in);
+        check(sparc_ddx(h, in.data_ptr<double>(), out.data_ptr<double>(), (int)(in.numel() / nx)), h, "sparc_ddx");
+        return out;
+    }
+    at::Tensor ddz_impl(at::Tensor in, int64_t ibc, at::Tensor preset, bool kern)
+    {
+        check_dev(in, "in");
+        TORCH_CHECK(in.dim() >= 2 && in.size(-1) == nx && in.size(-2) == nz, "trailing dims must be [nz][nx]");
+        at::Tensor out = preset.numel() ? preset.to(in.device()).clone().contiguous() : at::zeros_like(in);
+        TORCH_CHECK(out.sizes() == in.sizes(), "preset must match the input shape");
+        const int nb = (int)(in.numel() / (nx * nz));
+        const int rc = kern ? sparc_ddz_kern(h, in.data_ptr<double>(), out.data_ptr<double>(), nb, (int)ibc)
+                            : sparc_ddz(h, in.data_ptr<double>(), out.data_ptr<double>(), nb, (int)ibc);
+        check(rc, h, "sparc_ddz");
+        return out;
+    }
+    at::Tensor ddz(at::Tensor in, int64_t ibc, at::Tensor preset) { return ddz_impl(in, ibc, preset, false); }
+    at::Tensor ddz_kern(at::Tensor in, int64_t ibc) { return ddz_impl(in, ibc, at::Tensor(), true); }
+
+    // returns kc, kpsi, kvx, kvz, hess, each [nsrc][nz][nx]
+    std::vector<at::Tensor> kernels(at::Tensor fwd, at::Tensor adj)
+    {
+        check_dev(fwd, "fwd");
+        check_dev(adj, "adj");
+        TORCH_CHECK(fwd.dim() == 5 && fwd.size(0) == nsrc && fwd.size(2) == 6 && fwd.size(3) == nz &&
+                        fwd.size(4) == nx, "fwd must be [nsrc][nt][6][nz][nx]");
+        TORCH_CHECK(adj.dim() == 5 && adj.size(0) == nsrc && adj.size(1) == fwd.size(1) && adj.size(2) == 4,
+                    "adj must be [nsrc][nt][4][nz][nx]");
+        std::vector<at::Tensor> o;
+        for (int i = 0; i < 5; i++) o.push_back(at::empty({nsrc, nz, nx}, opts()));
+        check(sparc_kernels(h, (int)fwd.size(1), fwd.data_ptr<double>(), adj.data_ptr<double>(),
+                            o[0].data_ptr<double>(), o[1].data_ptr<double>(), o[2].data_ptr<double>(),
+                            o[3].data_ptr<double>(), o[4].data_ptr<double>()),
+              h, "sparc_kernels");
+        return o;
+    }
+};
+
+}  // namespace
+
+TORCH_LIBRARY(sparc, m)
+{
+    m.class_<Solver>("Solver")
+        .def(torch::init<int64_t, int64_t, int64_t, int64_t, int64_t, int64_t, int64_t, std::vector<double>,
+                         at::Tensor, at::Tensor, at::Tensor, int64_t>())
+        .def("set_stream", &Solver::set_stream)
+        .def("query_int", &Solver::query_int)
+        .def("query_double", &Solver::query_double)
+        .def("background", &Solver::background)
+        .def("set_model", &Solver::set_model)
+        .def("set_forward_sources", &Solver::set_forward_sources)
+        .def("set_adjoint_sources", &Solver::set_adjoint_sources)
+        .def("run", &Solver::run)
+        .def("reset", &Solver::reset)
+        .def("step", &Solver::step)
+        .def("get_state", &Solver::get_state)
+        .def("set_state", &Solver::set_state)
+        .def("get_pml", &Solver::get_pml)
+        .def("ddx", &Solver::ddx)
+        .def("ddz", &Solver::ddz)
+        .def("ddz_kern", &Solver::ddz_kern)
+        .def("kernels", &Solver::kernels);
+}

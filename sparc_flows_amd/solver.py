@@ -1,0 +1,164 @@
+"""Host-side API of the gfx950 solver: the counterpart of one `sparc SS LL` run.
+
+The reference drives its solver as a program: ``full.py`` copies an
+instruction file (``Spectral`` or ``Adjoint``) and runs ``sparc`` once per
+source and phase (full.py:24-61); the program reads params.i, the model files
+and master.pixels, and writes vz traces and snapshots to disk.  Here a
+:class:`Sparc` object is one batch of sources sharing a background model, with
+its state and snapshots resident in HBM:
+
+* ``forward(xsrc, maxtime)``   -- COMPUTE_FORWARD (driver.f90:242-277)
+* ``adjoint(vr, maxtime)``     -- COMPUTE_ADJOINT (driver.f90:279-298)
+* ``kernels(fwd, adj)``        -- CONSTRUCT_KERNELS (kernels.f90:13-612)
+
+All compute runs in libsparc_hip.so through the torch binding
+(torch.classes.sparc.Solver); nothing here falls back to another path.
+"""
+from __future__ import annotations
+
+from pathlib import Path
+
+import numpy as np
+import torch
+
+from .params import SparcParams
+
+_LIB_DIR = Path(__file__).resolve().parent / "lib"
+TORCH_SO = _LIB_DIR / "libsparc_torch.so"
+HIP_SO = _LIB_DIR / "libsparc_hip.so"
+_loaded = False
+
+
+def load_native():
+    """Load the torch binding (and with it libsparc_hip.so); fail loudly if absent."""
+    global _loaded
+    if not _loaded:
+        if not TORCH_SO.exists() or not HIP_SO.exists():
+            raise RuntimeError(
+                f"sparc native libraries missing in {_LIB_DIR}; build them with "
+                "`python -m sparc_flows_amd.build`")
+        torch.ops.load_library(str(TORCH_SO))
+        _loaded = True
+    return torch.classes.sparc.Solver
+
+
+def _as_tensor(a, n=None):
+    if a is None:
+        return torch.empty(0, dtype=torch.float64)
+    t = torch.as_tensor(np.ascontiguousarray(a) if isinstance(a, np.ndarray) else a,
+                        dtype=torch.float64).contiguous()
+    return t
+
+
+class Sparc:
+    """A batch of ``nsrc`` sources on one GPU sharing one background model."""
+
+    def __init__(self, params: SparcParams, model: np.ndarray, nsrc: int = 1, adjoint: bool = False,
+                 c2d: np.ndarray | None = None, psi: np.ndarray | None = None, device=None):
+        params.validate()
+        if not torch.cuda.is_available():
+            raise RuntimeError("the sparc solver needs a HIP GPU")
+        self.params = params
+        self.nsrc = int(nsrc)
+        self.is_adjoint = bool(adjoint)
+        self.device = torch.device("cuda", torch.cuda.current_device() if device is None
+                                   else torch.device(device).index or 0)
+        Solver = load_native()
+        p = params
+        scalars = [float(p.xlength), float(p.timestep), float(p.cadforcing), float(p.outputcad),
+                   float(p.obsheight), float(p.excitdep), float(p.nupeak), float(p.nuwidth)]
+        with torch.cuda.device(self.device):
+            self._s = Solver(p.nx, p.nz, p.npmlbot, p.npmltop, p.npmlhor, self.nsrc, int(adjoint), scalars,
+                             _as_tensor(model), _as_tensor(c2d), _as_tensor(psi), self.device.index)
+        self.maxtime = None
+
+    # ---- plumbing ----
+    def _stream(self):
+        self._s.set_stream(torch.cuda.current_stream(self.device).cuda_stream)
+        return self._s
+
+    def query(self, name: str):
+        try:
+            return self._s.query_int(name)
+        except RuntimeError:
+            return self._s.query_double(name)
+
+    def background(self, name: str) -> np.ndarray:
+        return self._s.background(name).numpy()
+
+    @property
+    def steps(self) -> int:
+        return self._s.query_int("steps")
+
+    def set_model(self, c2d=None, psi=None) -> None:
+        torch.cuda.synchronize(self.device)
+        self._s.set_model(_as_tensor(c2d), _as_tensor(psi))
+
+    # ---- runs ----
+    def set_forward_sources(self, xsrc, maxtime: int) -> None:
+        x = np.atleast_1d(np.asarray(xsrc, dtype=np.float64))
+        if x.size != self.nsrc:
+            raise ValueError(f"{x.size} source positions for a batch of {self.nsrc}")
+        self._stream().set_forward_sources(torch.as_tensor(x), int(maxtime))
+        self.maxtime = int(maxtime)
+
+    def set_adjoint_sources(self, vr: torch.Tensor, maxtime: int) -> None:
+        vr = torch.as_tensor(vr, dtype=torch.float64, device=self.device).contiguous()
+        if vr.dim() == 2:
+            vr = vr.unsqueeze(0)
+        self._stream().set_adjoint_sources(vr, int(maxtime))
+        self.maxtime = int(maxtime)
+
+    def run(self, snapshots: bool = False):
+        """TIMESTEPPING for time indices 0..maxtime; returns (vz, snaps)."""
+        if self.maxtime is None:
+            raise RuntimeError("set sources before run()")
+        vz, sn = self._stream().run(self.maxtime, bool(snapshots))
+        return vz, (sn if snapshots else None)
+
+    def forward(self, xsrc, maxtime: int, snapshots: bool = False):
+        if self.is_adjoint:
+            raise RuntimeError("forward() on an adjoint solver")
+        self.set_forward_sources(xsrc, maxtime)
+        return self.run(snapshots)
+
+    def adjoint(self, vr, maxtime: int, snapshots: bool = True):
+        if not self.is_adjoint:
+            raise RuntimeError("adjoint() on a forward solver")
+        self.set_adjoint_sources(vr, maxtime)
+        return self.run(snapshots)
+
+    def reset(self) -> None:
+        self._stream().reset()
+
+    def step(self, t: int) -> None:
+        self._stream().step(int(t))
+
+    def state(self) -> torch.Tensor:
+        """[4][nsrc][nz][nx]: xi_x, xi_z, v_x, v_z."""
+        return self._stream().get_state()
+
+    def set_state(self, a: torch.Tensor) -> None:
+        self._stream().set_state(torch.as_tensor(a, dtype=torch.float64, device=self.device).contiguous())
+
+    def pml(self):
+        return self._stream().get_pml()
+
+    # ---- operators ----
+    def ddx(self, f: torch.Tensor) -> torch.Tensor:
+        return self._stream().ddx(torch.as_tensor(f, dtype=torch.float64, device=self.device).contiguous())
+
+    def ddz(self, f: torch.Tensor, ibc: int, preset: torch.Tensor | None = None) -> torch.Tensor:
+        f = torch.as_tensor(f, dtype=torch.float64, device=self.device).contiguous()
+        pre = torch.empty(0, dtype=torch.float64, device=self.device) if preset is None else \
+            torch.as_tensor(preset, dtype=torch.float64, device=self.device).contiguous()
+        return self._stream().ddz(f, int(ibc), pre)
+
+    def ddz_kern(self, f: torch.Tensor, ibc: int) -> torch.Tensor:
+        f = torch.as_tensor(f, dtype=torch.float64, device=self.device).contiguous()
+        return self._stream().ddz_kern(f, int(ibc))
+
+    def kernels(self, fwd_snaps: torch.Tensor, adj_snaps: torch.Tensor) -> dict:
+        """PRODUCE_KERNELS from in-HBM snapshots; arrays [nsrc][nz][nx]."""
+        o = self._stream().kernels(fwd_snaps.contiguous(), adj_snaps.contiguous())
+        return dict(zip(("kc", "kpsi", "kvx", "kvz", "hess"), o))

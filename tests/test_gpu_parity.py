@@ -1,0 +1,149 @@
+"""GPU parity: the HIP path (through the C ABI via the torch binding) against
+the CPU oracle on the same seeded inputs.
+
+Tolerances: the reference's bar is 1e-9 relative L2 on vz traces after 1000
+steps (floating point, FP64).  Single operators are held to 1e-12, multi-step
+runs to 1e-10.
+"""
+import numpy as np
+import pytest
+
+from tests.cases import adjoint_source, rel_l2, small_case
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+if not torch.cuda.is_available():
+    pytest.skip("no GPU", allow_module_level=True)
+
+import oracle  # noqa: E402
+from sparc_flows_amd.solver import Sparc  # noqa: E402
+
+
+def _dev(a):
+    return torch.as_tensor(np.ascontiguousarray(a), dtype=torch.float64, device="cuda")
+
+
+@pytest.mark.parametrize("nx", [16, 64, 128, 256, 512])
+def test_ddx_matches_oracle(nx):
+    p, bg, _, _ = small_case(nx=nx, nz=40, flow=False, dc=False)
+    s = Sparc(p, bg)
+    o = oracle.Oracle(p, bg)
+    rng = np.random.default_rng(nx)
+    f = rng.standard_normal((3, p.nz, nx))
+    got = s.ddx(_dev(f)).cpu().numpy()
+    ref = np.stack([o.ddx(f[i]) for i in range(3)])
+    print("ddx", nx, rel_l2(got, ref))
+    assert rel_l2(got, ref) < 1e-13
+
+
+@pytest.mark.parametrize("nz", [20, 60, 150, 300])
+@pytest.mark.parametrize("ibc", [0, 1, 2, 3])
+def test_ddz_matches_oracle(nz, ibc):
+    p, bg, _, _ = small_case(nx=64, nz=nz, npml=4, flow=False, dc=False)
+    s = Sparc(p, bg)
+    o = oracle.Oracle(p, bg)
+    rng = np.random.default_rng(nz * 10 + ibc)
+    f = rng.standard_normal((2, nz, 64))
+    pre = rng.standard_normal((2, nz, 64))
+    got = s.ddz(_dev(f), ibc, _dev(pre)).cpu().numpy()
+    ref = np.stack([o.ddz(f[i], ibc, preset=pre[i]) for i in range(2)])
+    print("ddz", nz, ibc, rel_l2(got, ref))
+    assert rel_l2(got, ref) < 1e-13
+
+
+def test_background_matches_oracle():
+    p, bg, c2d, psi = small_case(nx=64, nz=60)
+    s = Sparc(p, bg, c2d=c2d, psi=psi)
+    o = oracle.Oracle(p, bg, c2d=c2d, psi=psi)
+    for name in ("stretch", "unstretch", "g", "rhoinv", "c2rho0", "gradp0_z", "gradrho0_z",
+                 "v0x", "v0z", "az", "bzpml", "axpml", "bxpml"):
+        a, b = s.background(name), o.array(name)
+        assert a.shape == b.shape, name
+        assert rel_l2(a, b) < 1e-13, (name, rel_l2(a, b))
+    for name in ("e_rad", "o_rad", "steps"):
+        assert s.query(name) == o.info(name)
+    for name in ("deltat", "delta_width"):
+        assert abs(s.query(name) - o.info(name)) <= 1e-15 * abs(o.info(name))
+
+
+@pytest.mark.parametrize("flow,dc", [(False, False), (True, True)])
+def test_steps_match_oracle(flow, dc):
+    """Full STEPs (RK + PML + filter + source) against the oracle."""
+    p, bg, c2d, psi = small_case(nx=64, nz=60, flow=flow, dc=dc)
+    maxtime = 60
+    s = Sparc(p, bg, c2d=c2d, psi=psi)
+    o = oracle.Oracle(p, bg, c2d=c2d, psi=psi)
+    s.set_forward_sources([3.0], maxtime)
+    o.set_forward_source(3.0, maxtime)
+    s.reset()
+    for t in range(40):
+        s.step(t)
+        o.step(t)
+    got = s.state()[:, 0].cpu().numpy()
+    ref = o.state()
+    err = rel_l2(got, ref)
+    print("40 steps", flow, dc, err, [rel_l2(got[i], ref[i]) for i in range(4)])
+    assert err < 1e-11
+    pv, px = s.pml()
+    opv, opx = o.pml()
+    assert rel_l2(pv[:, 0].cpu().numpy(), opv) < 1e-10
+    assert rel_l2(px[:, 0].cpu().numpy(), opx) < 1e-10
+
+
+def test_forward_trace_1000_steps():
+    """vz trace and snapshots of a 1000-step forward run with flow and dc."""
+    p, bg, c2d, psi = small_case(nx=64, nz=60, flow=True, dc=True)
+    maxtime = 999
+    s = Sparc(p, bg, c2d=c2d, psi=psi)
+    vz, sn = s.forward([-4.0], maxtime, snapshots=True)
+    o = oracle.Oracle(p, bg, c2d=c2d, psi=psi)
+    o.set_forward_source(-4.0, maxtime)
+    ovz, osn = o.run(maxtime, snapshots=True)
+    e = rel_l2(vz[0].cpu().numpy(), ovz)
+    es = rel_l2(sn[0].cpu().numpy(), osn)
+    print("1000-step vz", e, "snapshots", es)
+    assert np.abs(ovz).max() > 0
+    assert e < 1e-10 and es < 1e-10
+
+
+def test_batched_sources_equal_single_runs():
+    p, bg, c2d, psi = small_case(nx=64, nz=60)
+    xs = [-10.0, 0.0, 7.5]
+    maxtime = 200
+    sb = Sparc(p, bg, nsrc=3, c2d=c2d, psi=psi)
+    vzb, _ = sb.forward(xs, maxtime)
+    for i, x in enumerate(xs):
+        s1 = Sparc(p, bg, c2d=c2d, psi=psi)
+        vz1, _ = s1.forward([x], maxtime)
+        assert torch.equal(vzb[i], vz1[0]), i
+
+
+def test_adjoint_and_kernels_match_oracle():
+    p, bg, c2d, psi = small_case(nx=64, nz=60)
+    maxtime = 4 * p.steps * 5  # multiple of the cadence
+    nt = maxtime // p.cadforcing_step + 1
+    # forward
+    sf = Sparc(p, bg, c2d=c2d, psi=psi)
+    vzf, fsn = sf.forward([2.0], maxtime, snapshots=True)
+    of = oracle.Oracle(p, bg, c2d=c2d, psi=psi)
+    of.set_forward_source(2.0, maxtime)
+    ovz, ofsn = of.run(maxtime, snapshots=True)
+    assert rel_l2(fsn[0].cpu().numpy(), ofsn) < 1e-10
+    # adjoint
+    vr = adjoint_source(p, nt, p.nx)
+    sa = Sparc(p, bg, adjoint=True, c2d=c2d, psi=psi)
+    _, asn = sa.adjoint(_dev(vr[None]), maxtime, snapshots=True)
+    oa = oracle.Oracle(p, bg, c2d=c2d, psi=psi, adjoint=True)
+    oa.set_adjoint_source(vr, maxtime)
+    _, oasn = oa.run(maxtime, snapshots=True)
+    ea = rel_l2(asn[0].cpu().numpy(), oasn)
+    print("adjoint snapshots", ea)
+    assert np.abs(oasn).max() > 0 and ea < 1e-10
+    # kernels
+    K = sa.kernels(fsn, asn)
+    oK = oracle.kernels(p, bg, ofsn, oasn, c2d=c2d)
+    for name in ("kc", "kpsi", "kvx", "kvz", "hess"):
+        e = rel_l2(K[name][0].cpu().numpy(), oK[name])
+        print("kernel", name, e)
+        assert e < 1e-10, name

@@ -1,0 +1,290 @@
+"""Pins of the CPU oracle (oracle/sparc_oracle.c) before it is trusted as the
+GPU checker.
+
+* dbyd2 / mtridag: bit-identical to the reference's own dbyd2.f + mtridag.f,
+  compiled from the reference sources (oracle/_ref) and through the committed
+  golden vectors (tests/golden/dbyd2_ref.npz) made from that build.
+* ddx / FFT: closed forms (Fourier modes, Nyquist mode, constants) and numpy.
+* FILTER_VARS_2D / FILTER_VAR_Z: brute-force convolution and the filter's
+  transfer function (constants kept, the z-Nyquist mode removed).
+* LAGRANGE_INTERP: exact on degree-6 polynomials.
+* STEP: the five-stage update equals I + z + z^2/2 + o3 z^3 + o4 z^4 + o5 z^5
+  (z = dt * RHS operator, built column by column) followed by the filter.
+* RHS: linearity, mirror symmetry about the box centre, and a divergence-free
+  background flow (the reference's CONTINUITY_CHECK, driver.f90:1008-1032).
+* PRODUCE_KERNELS: forward label k is paired with the time-reversed adjoint.
+"""
+import os
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import oracle
+from sparc_flows_amd import background as B
+from sparc_flows_amd.params import SparcParams, f32, read_params_i
+from tests.cases import rel_l2, small_case
+
+GOLDEN = Path(__file__).resolve().parent / "golden" / "dbyd2_ref.npz"
+
+
+# ---------------------------------------------------------------- dbyd2 ----
+@pytest.mark.skipif(not oracle.ref_available(), reason="oracle/_ref not built (no reference tree)")
+@pytest.mark.parametrize("n2", [5, 6, 11, 60, 150, 300])
+@pytest.mark.parametrize("ibc", [0, 1, 2, 3])
+def test_dbyd2_bit_identical_to_reference_build(n2, ibc):
+    rng = np.random.default_rng(n2 * 7 + ibc)
+    Bm = rng.standard_normal((n2, 5))
+    A0 = rng.standard_normal((n2, 5))
+    assert np.array_equal(oracle.dbyd2(Bm, ibc, A0), oracle.ref_dbyd2(Bm, ibc, A0))
+
+
+def test_dbyd2_matches_golden_vectors():
+    g = np.load(GOLDEN)
+    for key in g.files:
+        if not key.startswith("R_"):
+            continue
+        _, n2, ibc = key.split("_")
+        got = oracle.dbyd2(g[f"B_{n2}_{ibc}"], int(ibc), g[f"A_{n2}_{ibc}"])
+        assert np.array_equal(got, g[key]), key
+    assert np.array_equal(oracle.dbyd2(g["z_poly300"], 1)[:, 0], g["dz_poly300"])
+
+
+def test_dbyd2_float32_coefficients():
+    """dbyd2.f's constants are float32, so its boundary closures do not sum to
+    zero: d/ds of a constant is ~5e-4 on the first and last rows, against
+    ~1e-8 in the interior (a double-precision restatement would give ~1e-13
+    everywhere)."""
+    n2 = 300
+    s = np.linspace(0.0, 1.0, n2)
+    for deg in (0, 1, 2):
+        d = oracle.dbyd2(s ** deg, 1)[:, 0]
+        exact = deg * s ** (deg - 1) if deg else np.zeros(n2)
+        err = np.abs(d - exact)
+        assert err[10:-10].max() < 5e-8, deg
+        assert 4e-4 < err[-1] < 6e-4, (deg, err[-1])
+    d0 = oracle.dbyd2(np.ones(n2), 1)[:, 0]
+    assert 4e-4 < abs(d0[0]) < 6e-4
+
+
+def test_dbyd2_ibc2_uses_preset_bottom_row():
+    rng = np.random.default_rng(3)
+    Bm = rng.standard_normal((40, 2))
+    A0 = np.zeros((40, 2))
+    A1 = A0.copy()
+    A1[0] = 1.0
+    d0 = oracle.dbyd2(Bm, 2, A0)
+    d1 = oracle.dbyd2(Bm, 2, A1)
+    assert np.all(d0[0] == 0.0) and np.all(d1[0] == 1.0)
+    assert np.max(np.abs(d1[1] - d0[1])) > 0.0        # couples into row 2 (lowr(2) = 1/8)
+
+
+# ------------------------------------------------------------- ddx / fft ----
+def _model(nx=64, nz=40):
+    p, bg, _, _ = small_case(nx=nx, nz=nz, flow=False, dc=False)
+    return p, bg, oracle.Oracle(p, bg)
+
+
+def test_fft_matches_numpy():
+    rng = np.random.default_rng(0)
+    for n in (8, 64, 256):
+        re, im = rng.standard_normal(n), rng.standard_normal(n)
+        fr, fi = oracle.fft(re, im, -1)
+        ref = np.fft.fft(re + 1j * im)
+        assert np.allclose(fr + 1j * fi, ref, rtol=0, atol=1e-12 * n)
+        br, bi = oracle.fft(re, im, +1)
+        assert np.allclose(br + 1j * bi, np.fft.ifft(re + 1j * im) * n, rtol=0, atol=1e-12 * n)
+
+
+@pytest.mark.parametrize("nx", [16, 64, 256])
+def test_ddx_fourier_modes(nx):
+    p, bg, o = _model(nx=nx)
+    i = np.arange(nx)
+    sx = o.info("stretchx")
+    for m in (1, 3, nx // 2 - 1):
+        f = np.tile(np.sin(2 * np.pi * m * i / nx), (p.nz, 1))
+        d = o.ddx(f)
+        exact = np.tile(2 * np.pi * m * np.cos(2 * np.pi * m * i / nx) * sx, (p.nz, 1))
+        assert rel_l2(d, exact) < 1e-13, m
+    nyq = np.tile((-1.0) ** i, (p.nz, 1))
+    assert np.max(np.abs(o.ddx(nyq))) == 0.0 or np.max(np.abs(o.ddx(nyq))) < 1e-12 * sx
+    assert np.max(np.abs(o.ddx(np.full((p.nz, nx), 3.0)))) < 1e-13 * sx
+
+
+# --------------------------------------------------------------- filters ----
+FILT = np.array([0.001953125, -0.01953125, 0.087890625, -0.234375, 0.41015625])
+
+
+def _kernel11():
+    return np.concatenate([FILT / 2, [0.75390625], FILT[::-1] / 2])
+
+
+def test_filter_transfer_function():
+    k = _kernel11()
+    assert abs(k.sum() - 1.0) == 0.0
+    assert abs(np.sum(k * (-1.0) ** np.arange(11))) == 0.0
+
+
+def test_filter_var_z_brute_force():
+    rng = np.random.default_rng(1)
+    nz, nx = 40, 6
+    v = rng.standard_normal((nz, nx))
+    got = oracle.filter_var_z(v)
+    ref = v.copy()
+    for j in range(1, 5):
+        ref[j] = 0.5 * (v[j + 1] + v[j - 1])
+    for j in range(nz - 5, nz - 1):
+        ref[j] = 0.5 * (v[j + 1] + v[j - 1])
+    kk = _kernel11()
+    for j in range(5, nz - 5):
+        ref[j] = np.tensordot(kk, v[j - 5:j + 6], axes=(0, 0))
+    assert np.allclose(got, ref, rtol=0, atol=1e-14)
+    assert np.array_equal(got[0], v[0]) and np.array_equal(got[-1], v[-1])
+
+
+def test_filter_vars_2d_ranges():
+    p, bg, o = _model(nx=32, nz=30)
+    rng = np.random.default_rng(2)
+    a = rng.standard_normal((4, p.nz, p.nx))
+    f = o.filter_vars_2d(a)
+    nh = p.npmlhor
+    cols = np.arange(p.nx)
+    inside = (cols >= nh - 1) & (cols <= p.nx - nh)
+    # untouched: rows 1..5 and nz (1-based), and the outer horizontal-PML columns
+    assert np.array_equal(f[:, :5], a[:, :5]) and np.array_equal(f[:, -1], a[:, -1])
+    assert np.array_equal(f[:, :, ~inside], a[:, :, ~inside])
+    ref = oracle.filter_var_z(a[0])
+    assert np.allclose(f[0, 5:-1][:, inside], ref[5:-1][:, inside], rtol=0, atol=1e-14)
+
+
+# ------------------------------------------------------------- lagrange ----
+def test_lagrange_exact_for_degree6():
+    p, bg, _, _ = small_case(nx=16, nz=30, flow=False, dc=False)
+    o = oracle.Oracle(p, bg, adjoint=True)
+    maxtime = 40 * p.cadforcing_step
+    nt = maxtime // p.cadforcing_step + 1
+    tn = np.arange(nt) * p.cadforcing
+    coef = np.array([0.3, -1.2e-2, 4e-4, -6e-6, 5e-8, -2e-10, 3e-13])
+    P = lambda t: sum(c * t ** k for k, c in enumerate(coef))  # noqa: E731
+    vr = np.tile(P(tn)[:, None], (1, p.nx))
+    o.set_adjoint_source(vr, maxtime)
+    for t in range(4 * p.cadforcing_step, 30 * p.cadforcing_step, 7):
+        f = o.lagrange(t)
+        exact = P(t * p.timestep)
+        assert abs(f[0] - exact) < 1e-9 * abs(exact) + 1e-12, t
+
+
+# ------------------------------------------------------------------- RK ----
+def _augmented_ops(o, p):
+    nx, nz = p.nx, p.nz
+    nzp = p.npmlbot + p.npmltop
+    nph = 2 * p.npmlhor
+    na, npz, npx = 4 * nz * nx, 2 * nzp * nx, 2 * nz * nph
+
+    def rhs(v):
+        a = v[:na].reshape(4, nz, nx)
+        pz = v[na:na + npz].reshape(2, nzp, nx)
+        px = v[na + npz:].reshape(2, nz, nph)
+        s, spz, spx = o.rhs(5, a, pz, px)
+        return np.concatenate([s.ravel(), spz.ravel(), spx.ravel()])
+
+    return rhs, na, npz, npx
+
+
+def test_step_is_rk_polynomial_of_rhs():
+    p, bg, _, _ = small_case(nx=16, nz=20, npml=3, nph=2, flow=False, dc=False)
+    psi = B.supergranule_psi(16, p.xlength, bg, sigx_mm=30.0)
+    o = oracle.Oracle(p, bg, psi=psi)
+    rhs, na, npz, npx = _augmented_ops(o, p)
+    n = na + npz + npx
+    nzp = p.npmlbot + p.npmltop
+    # psip of PML row 1 is a closed zero subsystem (stale ibc=2 row): never excited
+    skip_idx = set(range(na + nzp * p.nx, na + nzp * p.nx + p.nx))
+    M = np.zeros((n, n))
+    for j in range(n):
+        if j in skip_idx:
+            continue
+        e = np.zeros(n)
+        e[j] = 1.0
+        M[:, j] = rhs(e)
+    dt = o.info("deltat")
+    o3, o4, o5 = f32(0.1665579725791151184), f32(0.03950410250586729987), f32(0.007810706393337838236)
+    Z = dt * M
+    Z2 = Z @ Z
+    Z3 = Z2 @ Z
+    Z4 = Z3 @ Z
+    Z5 = Z4 @ Z
+    P = np.eye(n) + Z + 0.5 * Z2 + o3 * Z3 + o4 * Z4 + o5 * Z5
+    rng = np.random.default_rng(5)
+    v = np.zeros(n)
+    v[:na] = rng.standard_normal(na)                 # PML auxiliaries start at zero
+    pred = P @ v
+    pred_a = o.filter_vars_2d(pred[:na].reshape(4, p.nz, p.nx)).ravel()
+    o.set_state(v[:na].reshape(4, p.nz, p.nx))
+    o.step(5)
+    assert rel_l2(o.state().ravel(), pred_a) < 1e-12
+    opv, opx = o.pml()
+    assert rel_l2(np.concatenate([opv.ravel(), opx.ravel()]), pred[na:]) < 1e-10
+
+
+# ------------------------------------------------------------------ RHS ----
+def test_rhs_linear_and_mirror_symmetric():
+    p, bg, _, _ = small_case(nx=32, nz=30, flow=False, dc=False)
+    o = oracle.Oracle(p, bg)
+    rng = np.random.default_rng(7)
+    u = rng.standard_normal((4, p.nz, p.nx))
+    w = rng.standard_normal((4, p.nz, p.nx))
+    su, _, _ = o.rhs(3, u)
+    sw, _, _ = o.rhs(3, w)
+    suw, _, _ = o.rhs(3, 2.0 * u - 3.0 * w)
+    assert rel_l2(suw, 2.0 * su - 3.0 * sw) < 1e-12
+    # mirror i -> nx-1-i, x-components change sign
+    sign = np.array([-1.0, 1.0, -1.0, 1.0])[:, None, None]
+    um = sign * u[:, :, ::-1]
+    sm, _, _ = o.rhs(3, um)
+    assert rel_l2(sm, sign * su[:, :, ::-1]) < 1e-12
+
+
+def test_background_flow_is_divergence_free():
+    p, bg, _, psi = small_case(nx=64, nz=60, flow=True, dc=False)
+    o = oracle.Oracle(p, bg, psi=psi)
+    rho = o.array("rho0").reshape(p.nz, p.nx)
+    vx = o.array("v0x").reshape(p.nz, p.nx)
+    vz = o.array("v0z").reshape(p.nz, p.nx)
+    dx = o.ddx(rho * vx)
+    dz = o.ddz(rho * vz, 1)
+    assert np.max(np.abs(dx + dz)) < 1e-10 * np.max(np.abs(dx))
+    vmax = np.max(np.hypot(vx, vz)) * 1e6 * 1e-2    # m/s
+    assert 150.0 < vmax < 350.0
+
+
+# -------------------------------------------------------------- kernels ----
+def test_kernel_pairing_is_time_reversed():
+    p, bg, _, _ = small_case(nx=32, nz=30, flow=False, dc=False)
+    rng = np.random.default_rng(11)
+    nt = 3
+    fwd = np.zeros((nt, 6, p.nz, p.nx))
+    adj = np.zeros((nt, 4, p.nz, p.nx))
+    fwd[0] = rng.standard_normal((6, p.nz, p.nx))
+    adj[2] = rng.standard_normal((4, p.nz, p.nx))      # label adj_end pairs with forward label 0
+    k1 = oracle.kernels(p, bg, fwd, adj)
+    assert np.abs(k1["kc"]).max() > 0 and np.abs(k1["kvx"]).max() > 0
+    adj2 = np.zeros_like(adj)
+    adj2[0] = adj[2]
+    k2 = oracle.kernels(p, bg, fwd, adj2)
+    assert np.abs(k2["kc"]).max() == 0 and np.abs(k2["kvx"]).max() == 0
+    # bilinear in (forward, adjoint), hessian quadratic in the forward accelerations
+    k3 = oracle.kernels(p, bg, 2.0 * fwd, adj)
+    assert rel_l2(k3["kc"], 2.0 * k1["kc"]) < 1e-14 and rel_l2(k3["kpsi"], 2.0 * k1["kpsi"]) < 1e-14
+    assert rel_l2(k3["hess"], 4.0 * k1["hess"]) < 1e-14
+
+
+# ---------------------------------------------------------------- params ----
+@pytest.mark.skipif(not os.path.exists("/root/reference/params.i"), reason="reference tree absent")
+def test_read_params_i_of_reference():
+    p = read_params_i("/root/reference/params.i")
+    assert (p.nx, p.nz, p.npmlbot, p.npmltop, p.npmlhor) == (512, 300, 10, 10, 9)
+    assert p.timestep == 2.0 and p.xlength == 800.0e8 and p.outputcad == 30.0
+    assert p.nupeak == f32(0.0017) and p.obsheight == 2.0e7 and p.excitdep == -1.5e7
+    assert p.file_data == "solar_deep"
+    assert p.reference_maxtime() == 7785  # the default run length
