@@ -1,0 +1,167 @@
+"""Benchmark of the 2D sparc solver on one MI355X.
+
+Workload (the configuration the headline metric is quoted on): production-size
+2D forward + adjoint + kernel, FP64, one source, plain method -- nx=256 by
+nz=300 on the adiabatic polytrope with 12-point vertical PMLs, steady
+supergranule flow (peak 240 m/s), 4000 time steps forward and 4000 adjoint,
+windowed vz misfit over receivers 5-60 Mm from the source, kernels for psi and
+c.  One bench "step" = one full gradient (forward + misfit + adjoint + kernel
+accumulation) with all inputs resident in HBM.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1]
+
+Prints one JSON line: value = Mpoint-steps/s (nx*nz*time-steps*(forward +
+adjoint)*sources / wall-s / 1e6), plus the batched 16-source gradient, the
+roofline of the dominant kernel (HIP events) and the CPU baseline (the C
+oracle on the host, one core, bounded sample).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+METRIC = "Mpoint-steps/s FP64 + s per fwd+adjoint gradient, batched vs plain, 1 GPU"
+HBM_PEAK_GBS = 8000.0
+
+
+def make_config(nx=256, nz=300, npml=12):
+    from sparc_flows_amd import background as B
+    from sparc_flows_amd.params import SparcParams
+
+    bg = B.polytrope(nz)
+    dt = B.stable_timestep(bg)
+    p = SparcParams(nx=nx, nz=nz, xlength=200e8, timestep=dt, npmlbot=npml, npmltop=npml, npmlhor=9)
+    psi = B.supergranule_psi(nx, p.xlength, bg)
+    return p, bg, psi
+
+
+def cpu_baseline(p, bg, psi, seconds=12.0):
+    """Time the C oracle (one core) on a bounded sample of the same forward run."""
+    import oracle
+
+    oracle.build()
+    o = oracle.Oracle(p, bg, psi=psi)
+    o.set_forward_source(0.0, 100000)
+    n = 0
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < seconds:
+        o.step(n)
+        n += 1
+    dt = time.perf_counter() - t0
+    return {"value": p.nx * p.nz * n / dt / 1e6, "unit": "Mpoint-steps/s", "cores": 1, "kind": "port",
+            "sample": f"{n} forward time steps of the {p.nx}x{p.nz} workload (one source), "
+                      f"{dt:.1f} s on one host core"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3, help="timed gradients")
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--nsteps", type=int, default=4000, help="time steps per run")
+    ap.add_argument("--batch", type=int, default=16, help="sources of the batched gradient (0: skip)")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    import torch
+    from sparc_flows_amd.gradient import GradientPipeline
+
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    p, bg, psi = make_config()
+    maxtime = args.nsteps - 1
+    pipe = GradientPipeline(p, bg, nsrc=1, psi=psi)
+    xsrc = [0.0]
+
+    def one():
+        return pipe.run(xsrc, maxtime)
+
+    def log(msg):
+        print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+    for i in range(args.warmup):
+        one()
+        torch.cuda.synchronize()
+        log(f"warmup gradient {i + 1}/{args.warmup} done")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        out = one()
+        torch.cuda.synchronize()
+        log(f"timed gradient {i + 1}/{args.steps} done ({time.perf_counter() - t0:.2f} s)")
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    sec = (t1 - t0) / args.steps
+    pts = p.nx * p.nz * (maxtime + 1) * 2
+    value = pts / sec / 1e6
+    finite = bool(torch.isfinite(out["kc"]).all() and torch.isfinite(out["kpsi"]).all())
+
+    # batched gradient of args.batch sources vs the plain one
+    batched = None
+    if args.batch > 0:
+        from sparc_flows_amd import background as B
+        xs = list(B.source_positions(args.batch, seed=0))
+        bpipe = GradientPipeline(p, bg, nsrc=args.batch, psi=psi)
+        bpipe.run(xs, maxtime)
+        torch.cuda.synchronize()
+        log(f"batched warmup ({args.batch} sources) done")
+        tb0 = time.perf_counter()
+        bpipe.run(xs, maxtime)
+        torch.cuda.synchronize()
+        bsec = time.perf_counter() - tb0
+        log(f"batched gradient {bsec:.2f} s")
+        batched = {"sources": args.batch, "s_per_gradient": bsec,
+                   "Mpoint_steps_per_s": pts * args.batch / bsec / 1e6,
+                   "speedup_vs_sequential_plain": (sec * args.batch) / bsec}
+        del bpipe
+
+    # roofline of the dominant kernel: HIP events on the solver's stream
+    ms = pipe.fwd.profile_kernels(20)
+    npts = p.nx * p.nz
+    C = 2  # x-varying coefficient arrays read per point (v0x, v0z)
+    bytes_per_pt = {"col": (14 + C) * 8, "col_last": (14 + C) * 8, "row": 9 * 8, "row_first": 7 * 8}
+    per_step_ms = {"col": 4 * ms["col"], "col_last": ms["col_last"], "row": 4 * ms["row"],
+                   "row_first": ms["row_first"]}
+    dom = max(per_step_ms, key=per_step_ms.get)
+    achieved = bytes_per_pt[dom] * npts / (ms[dom] * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dom,
+                "ms_per_launch": ms, "algorithmic_bytes_per_point": bytes_per_pt[dom]}
+
+    log("kernel profile done")
+    cpu = None if args.no_cpu else cpu_baseline(p, bg, psi, args.cpu_seconds)
+    log("cpu baseline done")
+    line = {
+        "metric": METRIC, "value": value, "unit": "Mpoint-steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": sec * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "2D forward+adjoint+kernel, one source, plain (256x300, 4000 steps)",
+                   "nx": p.nx, "nz": p.nz, "time_steps": maxtime + 1, "sources": 1,
+                   "timestep_s": p.timestep, "flow": "supergranule 240 m/s", "npml": p.npmlbot},
+        "s_per_gradient": sec,
+        "finite": finite,
+        "batched": batched,
+        "roofline": roofline,
+        "column_pass_shape": {k: pipe.fwd.query(k) for k in ("colseg_tc", "colseg_ms", "colseg_p", "colseg_hw")},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

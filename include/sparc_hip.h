@@ -67,7 +67,8 @@ const char *sparc_last_error(const sparc_handle *h);
 /* Queue work on an existing hipStream_t (NULL: the handle's own stream). */
 int sparc_set_stream(sparc_handle *h, void *hip_stream);
 /* Integer / double facts of the handle: "e_rad", "o_rad", "steps",
- * "cadforcing_step", "forcing_length", "nsrc", "nx", "nz", "nzpml", "nph" / "deltat",
+ * "cadforcing_step", "forcing_length", "nsrc", "nx", "nz", "nzpml", "nph",
+ * "colseg_tc", "colseg_ms", "colseg_p", "colseg_hw" (column-pass launch shape) / "deltat",
  * "stretchx", "delta_width", "beta2".."beta5". */
 int sparc_query_int(const sparc_handle *h, const char *name, int *value);
 int sparc_query_double(const sparc_handle *h, const char *name, double *value);
@@ -129,6 +130,14 @@ int sparc_ddz_kern(sparc_handle *h, const double *in_dev, double *out_dev, int n
 int sparc_kernels(sparc_handle *h, int nt, const double *fwd_snaps_dev,
                   const double *adj_snaps_dev, double *kc_dev, double *kpsi_dev, double *kvx_dev,
                   double *kvz_dev, double *hess_dev);
+
+/* Instrumentation (no reference counterpart; the reference prints per-step
+ * MPI_WTIME, driver.f90:355-381): average device time in ms of one launch of
+ * each pass of a step, timed with HIP events on the handle's stream over reps
+ * back-to-back launches: ms_out[0] column pass, [1] last-stage column pass
+ * (with the z filter), [2] row pass, [3] first row pass (with the v_x filter).
+ * Overwrites the state, then resets it. */
+int sparc_profile_kernels(sparc_handle *h, int reps, double *ms_out);
 
 #ifdef __cplusplus
 }

@@ -23,7 +23,7 @@ HIP_SO = LIB / "libsparc_hip.so"
 TORCH_SO = LIB / "libsparc_torch.so"
 ARCH = os.environ.get("SPARC_OFFLOAD_ARCH", "gfx950")
 
-HIP_SOURCES = ["sparc_kernels.hip", "sparc_capi.hip"]
+HIP_SOURCES = ["sparc_kernels.hip", "sparc_colseg.hip", "sparc_capi.hip"]
 HIP_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared",
     # packed FP32 instructions are kept out of generated code on this platform
@@ -39,9 +39,19 @@ def _run(cmd: list[str]) -> None:
 
 
 def build_hip() -> Path:
+    """Compile each translation unit in parallel, then link the shared library."""
+    from concurrent.futures import ThreadPoolExecutor
+
     LIB.mkdir(exist_ok=True)
+    objdir = LIB / "obj"
+    objdir.mkdir(exist_ok=True)
+    cflags = [f for f in HIP_FLAGS if f != "-shared"]
+    objs = [objdir / (Path(s).stem + ".o") for s in HIP_SOURCES]
+    with ThreadPoolExecutor(len(HIP_SOURCES)) as ex:
+        list(ex.map(lambda so: _run(["hipcc", *cflags, "-c", str(CSRC / so[0]), "-o", str(so[1])]),
+                    zip(HIP_SOURCES, objs)))
     tmp = HIP_SO.with_suffix(".so.tmp")
-    _run(["hipcc", *HIP_FLAGS, *[str(CSRC / s) for s in HIP_SOURCES], "-o", str(tmp)])
+    _run(["hipcc", f"--offload-arch={ARCH}", "-shared", "-fPIC", *[str(o) for o in objs], "-o", str(tmp)])
     os.replace(tmp, HIP_SO)
     return HIP_SO
 

@@ -1,0 +1,223 @@
+"""`sparc SS LL` drop-in: the reference program's command line, files and exit code.
+
+    python -m sparc_flows_amd.cli SS LL      (or the `sparc` launcher script)
+
+Behaviour follows the reference driver (driver.f90:40-334, READ_PARAMETERS
+initialize.f90:984-1144, DETERMINE_STATUS driver.f90:480-631):
+
+* ``Instruction_srcSS_lsLL`` in the working directory selects the mode
+  (first line ``spectral`` -> forward, ``adjoint`` -> adjoint, no file ->
+  kernel construction) and is deleted after reading.
+* The flag files ``compute_data``, ``linesearch`` and ``compute_synth`` in the
+  working directory change inputs and outputs as in the reference.
+* ``params.i`` (working directory, or $SPARC_PARAMS) gives the grid and
+  cadences; the data directory is $SPARC_DIRECTORY, else the ``directory``
+  parameter of params.i; the model file is ``file_data``.
+* Forward writes ``forward_srcSS_lsLL/{vz_cc.bin, vz_cc.fits, timeline,
+  kernel_info, rms_hist}`` and the partial snapshots; adjoint writes
+  ``adjoint_srcSS/{timeline, kernel_info}`` and its partial snapshots; the
+  kernel step writes ``kernel/{kernel_c,kernel_psi,kernel_vx,kernel_vz,
+  hessian}_SS.fits`` and removes the partial snapshots (kernels.f90:591-597).
+
+The travel-time adjoint source and misfit the reference computes at the end of
+a forward run (ADJOINT_SOURCE_FILT, MISFIT_ALL) are not part of this program
+yet: a forward run reports that and leaves ``adjoint_srcSS/source.fits`` to the
+caller.  The exit code is 0 on success and 1 on any error, which is what the
+drivers assert (full.py:31).
+"""
+from __future__ import annotations
+
+import os
+import re
+import sys
+from pathlib import Path
+
+import numpy as np
+
+from . import background as B
+from .fitsio import read_fits, write_direct_records, write_fits
+from .params import read_params_i
+
+
+def _directory(params_path: Path) -> Path:
+    env = os.environ.get("SPARC_DIRECTORY")
+    if env:
+        return Path(env)
+    m = re.search(r"directory\s*=\s*'([^']*)'", params_path.read_text())
+    return Path(m.group(1)) if m else Path.cwd()
+
+
+def _maxtime(p) -> int:
+    env = os.environ.get("SPARC_MAXTIME")
+    return int(env) if env else p.reference_maxtime()
+
+
+def _label(t: int) -> str:
+    return f"{int(abs(t)):06d}"
+
+
+FWD_FIELDS = ("xix", "xiz", "vx", "vz", "acc_x", "acc_z")
+ADJ_FIELDS = ("vx", "vz", "acc_x", "acc_z")
+
+
+def main(argv=None) -> int:
+    argv = sys.argv[1:] if argv is None else argv
+    if len(argv) < 2:
+        print("usage: sparc SS LL", file=sys.stderr)
+        return 1
+    contrib, jobno = argv[0].zfill(2), argv[1].zfill(2)
+    cwd = Path.cwd()
+    params_path = Path(os.environ.get("SPARC_PARAMS", cwd / "params.i"))
+    p = read_params_i(params_path)
+    ddir = _directory(params_path)
+    model_path = Path(p.file_data)
+    if not model_path.is_absolute():
+        model_path = cwd / model_path
+    bg = B.read_model_file(model_path, p.nz)
+    maxtime = _maxtime(p)
+    instr = cwd / f"Instruction_src{contrib}_ls{jobno}"
+    compute_data = (cwd / "compute_data").exists()
+    linesearch = (cwd / "linesearch").exists()
+    compute_synth = (cwd / "compute_synth").exists()
+
+    # model_psi / model_c of this iterate (driver.f90:94-126, kernels.f90:35-42)
+    psi_path = ddir / f"model_psi_ls{jobno}.fits"
+    psi = read_fits(psi_path).reshape(p.nz, p.nx) if psi_path.exists() else None
+    c_path = ddir / f"model_c_ls{jobno}.fits"
+
+    if instr.exists():
+        lines = instr.read_text().split()
+        kind = lines[0].strip().lower() if lines else ""
+        instr.unlink()                                  # driver.f90:243
+        if kind == "spectral":
+            return _forward(p, bg, psi, ddir, contrib, jobno, maxtime, compute_data,
+                            linesearch or compute_synth or compute_data)
+        if kind == "adjoint":
+            return _adjoint(p, bg, psi, ddir, contrib, maxtime)
+        print(f"unknown instruction '{kind}'", file=sys.stderr)
+        return 1
+    c2d = None
+    if c_path.exists():
+        cm = read_fits(c_path).reshape(p.nz, p.nx)
+        if cm.sum() != 0:
+            c2d = cm
+    return _kernels(p, bg, c2d, ddir, contrib, jobno)
+
+
+def _xsrc(ddir: Path, contrib: str) -> float:
+    vals = [float(v) for v in (ddir / "master.pixels").read_text().split()]
+    return vals[int(contrib) - 1]
+
+
+def _forward(p, bg, psi, ddir, contrib, jobno, maxtime, compute_data, no_snapshots) -> int:
+    import torch
+
+    from .solver import Sparc
+
+    out = ddir / f"forward_src{contrib}_ls{jobno}"
+    out.mkdir(parents=True, exist_ok=True)
+    v0 = None
+    if compute_data:
+        # true flows in m/s (driver.f90:65-73): build psi-free flows is not
+        # supported by the C ABI, so the stream function of the true model
+        # must be given as model_psi of this iterate instead.
+        if (Path.cwd() / "true_psi.fits").exists():
+            v0 = read_fits(Path.cwd() / "true_psi.fits").reshape(p.nz, p.nx)
+    s = Sparc(p, bg, psi=v0 if v0 is not None else psi)
+    vz, sn = s.forward([_xsrc(ddir, contrib)], maxtime, snapshots=not no_snapshots)
+    vz = vz[0].cpu().numpy()
+    write_direct_records(out / "vz_cc.bin", vz)
+    write_fits(out / "vz_cc.fits", vz[:, None, :], time=maxtime)
+    steps = s.steps
+    timeline = -p.solartime * 3600.0
+    with open(out / "timeline", "w") as f:
+        for t in range(0, maxtime + 1):
+            timeline += p.timestep
+            if t % steps == 0:
+                f.write(f"{t:12d} {timeline:24.15E}\n")
+    with open(out / "kernel_info", "w") as f:
+        f.write(f"{0:12d}\n{maxtime:12d}\n")
+    if sn is not None:
+        a = sn[0].cpu().numpy()
+        for k in range(a.shape[0]):
+            for fi, name in enumerate(FWD_FIELDS):
+                write_fits(out / f"{name}_{_label(k * steps)}_partial.fits", a[k, fi][:, None, :], time=k * steps)
+    if not compute_data:
+        print("forward done; the travel-time adjoint source (ADJOINT_SOURCE_FILT) is computed by the caller")
+    torch.cuda.synchronize()
+    return 0
+
+
+def _adjoint(p, bg, psi, ddir, contrib, maxtime) -> int:
+    import torch
+
+    from .solver import Sparc
+
+    out = ddir / f"adjoint_src{contrib}"
+    out.mkdir(parents=True, exist_ok=True)
+    fl = p.forcing_length(maxtime)
+    vr = read_fits(out / "source.fits").reshape(-1, p.nx)[:fl]
+    s = Sparc(p, bg, adjoint=True, psi=psi)
+    _, sn = s.adjoint(torch.as_tensor(vr[None], dtype=torch.float64, device="cuda"), maxtime,
+                      snapshots=True)
+    steps = s.steps
+    cfs = p.cadforcing_step
+    a = sn[0].cpu().numpy()
+    for k in range(a.shape[0]):
+        for fi, name in enumerate(ADJ_FIELDS):
+            write_fits(out / f"{name}_{_label(k * steps)}_partial.fits", a[k, fi][:, None, :], time=k * steps)
+    final_time = p.solartime * 3600.0
+    timeline = final_time - (fl - 1.0) * cfs * p.timestep
+    with open(out / "timeline", "w") as f:
+        for t in range(0, maxtime + 1):
+            timeline += p.timestep
+            if t % steps == 0:
+                f.write(f"{t:12d} {timeline:24.15E}\n")
+    with open(out / "kernel_info", "w") as f:            # driver.f90:283-292
+        f.write(f"{maxtime // cfs + 1:12d}\n{(maxtime // cfs) * cfs:12d}\n")
+    return 0
+
+
+def _kernels(p, bg, c2d, ddir, contrib, jobno) -> int:
+    import torch
+
+    from .solver import Sparc
+
+    fdir = ddir / f"forward_src{contrib}_ls{jobno}"
+    adir = ddir / f"adjoint_src{contrib}"
+    nt_kern, adj_end = [int(v) for v in (adir / "kernel_info").read_text().split()[:2]]
+    fwd_start = int((fdir / "kernel_info").read_text().split()[0])
+    steps = p.steps
+    fwd = np.zeros((1, nt_kern, 6, p.nz, p.nx))
+    adj = np.zeros((1, nt_kern, 4, p.nz, p.nx))
+    for k in range(nt_kern):
+        lf = _label(k * steps + fwd_start)
+        for fi, name in enumerate(FWD_FIELDS):
+            fwd[0, k, fi] = read_fits(fdir / f"{name}_{lf}_partial.fits").reshape(p.nz, p.nx)
+        la = _label(adj_end - (nt_kern - 1 - k) * steps)
+        for fi, name in enumerate(ADJ_FIELDS):
+            adj[0, k, fi] = read_fits(adir / f"{name}_{la}_partial.fits").reshape(p.nz, p.nx)
+    s = Sparc(p, bg, c2d=c2d)
+    K = s.kernels(torch.as_tensor(fwd, device="cuda"), torch.as_tensor(adj, device="cuda"))
+    kdir = ddir / "kernel"
+    kdir.mkdir(parents=True, exist_ok=True)
+    for key, name in (("kc", "kernel_c"), ("kpsi", "kernel_psi"), ("kvx", "kernel_vx"),
+                      ("kvz", "kernel_vz"), ("hess", "hessian")):
+        write_fits(kdir / f"{name}_{contrib}.fits", K[key][0].cpu().numpy()[:, None, :])
+    for d in (fdir, adir):                               # kernels.f90:591-597
+        for f in d.glob("*partial*"):
+            f.unlink()
+    return 0
+
+
+def entry() -> None:
+    try:
+        rc = main()
+    except Exception as e:  # noqa: BLE001 - the caller only sees the exit code
+        print(f"sparc: {type(e).__name__}: {e}", file=sys.stderr)
+        rc = 1
+    sys.exit(rc)
+
+
+if __name__ == "__main__":
+    entry()

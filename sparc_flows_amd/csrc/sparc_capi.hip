@@ -7,8 +7,10 @@
 // kernels as the time loop.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -80,6 +82,11 @@ struct sparc_handle {
 
     // kernel-mode background (kernel_back, initialize.f90:937-979)
     double *d_krho0 = nullptr, *d_kc2 = nullptr;
+
+    // segmented column pass: tables and launch shape (chosen by autotune)
+    SegTab seg{};
+    int seg_tc = 16, seg_ms = 20;
+    bool tuned = false;
 };
 
 static thread_local std::string g_create_err;
@@ -150,6 +157,55 @@ static void make_tri(int N2, int ibc, std::vector<double> &lowr, std::vector<dou
     for (int j = 0; j < N2; j++) rbet[j] = 1.0 / betad[j];
 }
 
+// The dbyd2 matrix of one ibc: sub-diagonal lowr, unit diagonal, uppr.
+static void make_mat(int N2, int ibc, std::vector<double> &lowr, std::vector<double> &uppr)
+{
+    const float alpha = 1.f / 3.f, alpha2 = 3.f / 4.f, gamma2 = 1.f / 8.f;
+    const float alpha1 = 6.0f, beta1 = 3.f;
+    const float cnst = 1.f / (alpha2 - beta1 * gamma2);
+    const float up1 = (alpha1 * alpha2 - beta1) * cnst;
+    uppr.assign(N2, 0.0);
+    lowr.assign(N2, 0.0);
+    for (int j = 2; j <= N2 - 3; j++) uppr[j] = alpha;
+    uppr[0] = (ibc % 2 == 0) ? 0.0 : (double)up1;
+    uppr[1] = alpha2;
+    uppr[N2 - 2] = gamma2;
+    for (int j = 0; j < N2; j++) lowr[j] = uppr[N2 - 1 - j];
+    lowr[N2 - 1] = (ibc != 1 && ibc != 2) ? 0.0 : (double)up1;
+    lowr[0] = 0.0;
+    uppr[N2 - 1] = 0.0;
+}
+
+// Invert a dense n x n matrix (Gauss-Jordan, partial pivoting).
+static bool invert(std::vector<double> A, int n, std::vector<double> &inv)
+{
+    inv.assign((size_t)n * n, 0.0);
+    for (int i = 0; i < n; i++) inv[(size_t)i * n + i] = 1.0;
+    for (int c = 0; c < n; c++) {
+        int pr = c;
+        for (int r = c + 1; r < n; r++)
+            if (std::fabs(A[(size_t)r * n + c]) > std::fabs(A[(size_t)pr * n + c])) pr = r;
+        if (A[(size_t)pr * n + c] == 0.0) return false;
+        if (pr != c)
+            for (int k = 0; k < n; k++) {
+                std::swap(A[(size_t)pr * n + k], A[(size_t)c * n + k]);
+                std::swap(inv[(size_t)pr * n + k], inv[(size_t)c * n + k]);
+            }
+        const double d = 1.0 / A[(size_t)c * n + c];
+        for (int k = 0; k < n; k++) { A[(size_t)c * n + k] *= d; inv[(size_t)c * n + k] *= d; }
+        for (int r = 0; r < n; r++) {
+            if (r == c) continue;
+            const double f = A[(size_t)r * n + c];
+            if (f == 0.0) continue;
+            for (int k = 0; k < n; k++) {
+                A[(size_t)r * n + k] -= f * A[(size_t)c * n + k];
+                inv[(size_t)r * n + k] -= f * inv[(size_t)c * n + k];
+            }
+        }
+    }
+    return true;
+}
+
 // Upload a host vector into a new device buffer of the arena.
 static hipError_t upload(DevArena &ar, double **dst, const std::vector<double> &v, hipStream_t st)
 {
@@ -189,13 +245,24 @@ static int host_ddx(sparc_handle *h, const std::vector<double> &in, std::vector<
     HCHECK(h, hipMalloc(&di, n * sizeof(double)));
     HCHECK(h, hipMalloc(&dout, n * sizeof(double)));
     HCHECK(h, hipMemcpyAsync(di, in.data(), n * sizeof(double), hipMemcpyHostToDevice, h->stream));
-    HCHECK(h, launch_ddx_rows(di, dout, nrows, h->nx, h->bg.tw, h->bg.eyekx, h->stretchx, h->stream));
+    HCHECK(h, launch_ddx_rows(di, dout, nrows, h->nx, h->bg.tw, h->bg.dmul, h->stretchx, h->stream));
     out.resize(n);
     HCHECK(h, hipMemcpyAsync(out.data(), dout, n * sizeof(double), hipMemcpyDeviceToHost, h->stream));
     HCHECK(h, hipStreamSynchronize(h->stream));
     (void)hipFree(di);
     (void)hipFree(dout);
     return SPARC_OK;
+}
+
+// Per-bin multiplier of the spectral derivative over the full complex
+// spectrum: eyekx(k) below the Nyquist bin, zero at it (FFTW's c2r drops its
+// imaginary part, derivatives.f90:46), -eyekx(nx-k) above.
+static std::vector<double> dmul_table(const sparc_handle *h)
+{
+    const int n = h->nx;
+    std::vector<double> m(n, 0.0);
+    for (int k = 0; k < n; k++) m[k] = k < n / 2 ? h->eyekx[k] : (k == n / 2 ? 0.0 : -h->eyekx[n - k]);
+    return m;
 }
 
 // Grid, derivative tables (initialize.f90:194-225, 466-474).
@@ -224,6 +291,8 @@ static int setup_grid(sparc_handle *h)
     double *p = nullptr;
     HCHECK(h, upload(h->bg_mem, &p, h->eyekx, h->stream));
     h->bg.eyekx = p;
+    HCHECK(h, upload(h->bg_mem, &p, dmul_table(h), h->stream));
+    h->bg.dmul = p;
     h->bg.stretchx = h->stretchx;
 
     h->cf = make_coef(nz);
@@ -403,6 +472,8 @@ static int setup_background(sparc_handle *h)
     return SPARC_OK;
 }
 
+static int setup_seg(sparc_handle *h);
+
 static int upload_background(sparc_handle *h)
 {
     // grid tables were uploaded into bg_mem by setup_grid and released by
@@ -421,6 +492,7 @@ static int upload_background(sparc_handle *h)
     HCHECK(h, upload(h->bg_mem, &p, tw, h->stream));
     h->bg.tw = (const double2 *)p;
     HCHECK(h, upload(h->bg_mem, &p, h->eyekx, h->stream)); h->bg.eyekx = p;
+    HCHECK(h, upload(h->bg_mem, &p, dmul_table(h), h->stream)); h->bg.dmul = p;
     HCHECK(h, upload(h->bg_mem, &p, h->stretch, h->stream)); h->bg.stretch = p;
     HCHECK(h, upload(h->bg_mem, &p, h->unstretch, h->stream)); h->bg.unstretch = p;
     for (int ibc = 0; ibc < 4; ibc++) {
@@ -463,7 +535,105 @@ static int upload_background(sparc_handle *h)
         HCHECK(h, upload(h->bg_mem, &h->d_krho0, kr, h->stream));
         HCHECK(h, upload(h->bg_mem, &h->d_kc2, kc, h->stream));
     }
+    int r2 = setup_seg(h);
+    if (r2) return r2;
     HCHECK(h, hipStreamSynchronize(h->stream));
+    return SPARC_OK;
+}
+
+// Segmented solve tables (SegTab) for the column pass: P segments of m rows
+// (m from SPARC_COLSEG_SIZES, the last segment padded with identity rows and
+// holding at least 3 real rows).
+static int setup_seg(sparc_handle *h)
+{
+    const int N = h->nz;
+    static const int sizes[] = {
+#define X(v) v,
+        SPARC_COLSEG_SIZES(X)
+#undef X
+    };
+    int m = 0, P = 0;
+    for (int v : sizes) {
+        const int p = (N + v - 1) / v;
+        if (p > kSegMax) continue;
+        if (N - (p - 1) * v < 3) continue;
+        m = v;
+        P = p;
+        break;
+    }
+    if (!m) return fail(h, SPARC_E_ARG, "nz not supported by the segmented column pass");
+    h->seg_ms = m;
+    const int nzp = P * m;
+    SegTab &sg = h->seg;
+    sg.P = P;
+    sg.m = m;
+    sg.nzp = nzp;
+    std::vector<double> ztab((size_t)kZT * nzp, 0.0);
+    for (int j = 0; j < N; j++) {
+        ztab[(size_t)0 * nzp + j] = h->stretch[j];
+        ztab[(size_t)1 * nzp + j] = h->g[j];
+    }
+    const int n2 = 2 * P;
+    std::vector<double> minv_all;
+    int hw = 0;
+    const int ibcs[3] = {1, 2, 3};
+    for (int slot = 0; slot < 3; slot++) {
+        std::vector<double> lo0, up0;
+        make_mat(N, ibcs[slot], lo0, up0);
+        std::vector<double> lowr(nzp, 0.0), uppr(nzp, 0.0);
+        for (int j = 0; j < N; j++) { lowr[j] = lo0[j]; uppr[j] = up0[j]; }
+        std::vector<double> rbl(nzp, 1.0), gl(nzp, 0.0), V(nzp, 0.0), W(nzp, 0.0), bet(nzp, 1.0);
+        for (int s = 0; s < P; s++) {
+            const int a0 = s * m;
+            for (int j = a0 + 1; j < a0 + m; j++) {
+                gl[j] = uppr[j - 1] / bet[j - 1];
+                bet[j] = 1.0 - lowr[j] * gl[j];
+                rbl[j] = 1.0 / bet[j];
+            }
+            auto lsolve = [&](std::vector<double> r, double *out) {
+                r[0] = r[0] * rbl[a0];
+                for (int i = 1; i < m; i++) r[i] = (r[i] - lowr[a0 + i] * r[i - 1]) * rbl[a0 + i];
+                for (int i = m - 2; i >= 0; i--) r[i] = r[i] - gl[a0 + i + 1] * r[i + 1];
+                for (int i = 0; i < m; i++) out[a0 + i] = r[i];
+            };
+            std::vector<double> r(m, 0.0);
+            if (s > 0) { r[0] = lowr[a0]; lsolve(r, V.data()); }
+            std::fill(r.begin(), r.end(), 0.0);
+            if (s < P - 1) { r[m - 1] = uppr[a0 + m - 1]; lsolve(r, W.data()); }
+        }
+        std::vector<double> M((size_t)n2 * n2, 0.0), Minv;
+        for (int s = 0; s < P; s++) {
+            const int f = s * m, l = s * m + m - 1;
+            M[(size_t)(2 * s) * n2 + 2 * s] = 1.0;
+            M[(size_t)(2 * s + 1) * n2 + 2 * s + 1] = 1.0;
+            if (s > 0) {
+                M[(size_t)(2 * s) * n2 + 2 * s - 1] += V[f];
+                M[(size_t)(2 * s + 1) * n2 + 2 * s - 1] += V[l];
+            }
+            if (s < P - 1) {
+                M[(size_t)(2 * s) * n2 + 2 * s + 2] += W[f];
+                M[(size_t)(2 * s + 1) * n2 + 2 * s + 2] += W[l];
+            }
+        }
+        if (!invert(M, n2, Minv)) return fail(h, SPARC_E_ARG, "singular interface system");
+        // window half-width: entries below 1e-24 of their row maximum are dropped
+        for (int k = 0; k < n2; k++) {
+            double rm = 0.0;
+            for (int q = 0; q < n2; q++) rm = std::max(rm, std::fabs(Minv[(size_t)k * n2 + q]));
+            for (int q = 0; q < n2; q++)
+                if (std::fabs(Minv[(size_t)k * n2 + q]) > 1e-24 * rm) hw = std::max(hw, std::abs(q / 2 - k / 2));
+        }
+        minv_all.insert(minv_all.end(), Minv.begin(), Minv.end());
+        const std::vector<double> *tabs[5] = {&lowr, &rbl, &gl, &V, &W};
+        for (int k = 0; k < 5; k++)
+            std::copy(tabs[k]->begin(), tabs[k]->end(), ztab.begin() + (size_t)(2 + 5 * slot + k) * nzp);
+    }
+    sg.hw = hw;
+    double *p = nullptr;
+    HCHECK(h, upload(h->bg_mem, &p, ztab, h->stream));
+    sg.ztab = p;
+    HCHECK(h, upload(h->bg_mem, &p, minv_all, h->stream));
+    sg.Minv = p;
     return SPARC_OK;
 }
 
@@ -489,11 +659,8 @@ static int alloc_state(sparc_handle *h)
     HCHECK(h, m.alloc(&s.p, f));
     HCHECK(h, m.alloc(&s.rvx, f));
     HCHECK(h, m.alloc(&s.topbc, (size_t)h->S * h->nx));
-    HCHECK(h, m.alloc(&s.w_xiz, f));
-    HCHECK(h, m.alloc(&s.w_vx, f));
-    HCHECK(h, m.alloc(&s.w_vz, f));
-    HCHECK(h, m.alloc(&s.w_p, f));
     HCHECK(h, m.alloc(&s.w_rho, f));
+    HCHECK(h, m.alloc(&s.discard, 256));
     HCHECK(h, m.alloc(&h->d_scratch, f));
     HCHECK(h, m.alloc(&h->d_scratch2, f));
     double *t = nullptr;
@@ -549,8 +716,8 @@ static int enqueue_step(sparc_handle *h, int toff, bool filt_vx, const Sink &sk)
         }
         ca.last = stg == 5;
         ca.bnext = stg == 5 ? h->deltat : b[stg];
-        HCHECK(h, launch_col(d, h->bg, h->tri[1], h->tri[2], h->tri[3], h->cf, s, ca, h->src, clk,
-                             sk, h->stream));
+        HCHECK(h, launch_colseg(h->seg_tc, h->seg_ms, d, h->bg, h->cf, h->seg, s, ca, h->src, clk, sk,
+                                h->stream));
         RowArgs ra{};
         if (stg < 5) {
             ra.xi_x = s.t[0];
@@ -694,6 +861,10 @@ int sparc_query_int(const sparc_handle *h, const char *name, int *v)
     else if (n == "nz") *v = h->nz;
     else if (n == "nzpml") *v = h->d.nzpml;
     else if (n == "nph") *v = h->d.nph;
+    else if (n == "colseg_tc") *v = h->seg_tc;
+    else if (n == "colseg_ms") *v = h->seg_ms;
+    else if (n == "colseg_p") *v = h->seg.P;
+    else if (n == "colseg_hw") *v = h->seg.hw;
     else return SPARC_E_ARG;
     return SPARC_OK;
 }
@@ -878,12 +1049,61 @@ int sparc_step(sparc_handle *h, int t)
     return SPARC_OK;
 }
 
+// Pick the column-block width of the segmented column pass for this grid and
+// batch by timing each candidate on the handle's stream (state is reset
+// afterwards).  SPARC_COLSEG_TC in the environment overrides the choice.
+static int autotune(sparc_handle *h)
+{
+    if (h->tuned) return SPARC_OK;
+    h->tuned = true;
+    if (const char *e = std::getenv("SPARC_COLSEG_TC")) {
+        const int v = std::atoi(e);
+        if (v == 4 || v == 8 || v == 16) { h->seg_tc = v; return SPARC_OK; }
+    }
+    Sink sk = empty_sink(h);
+    Clock clk{h->tctr, 0, h->steps};
+    State &s = h->st;
+    ColArgs ca{};
+    for (int f = 0; f < 4; f++) { ca.T[f] = s.t[f]; ca.TN[f] = s.t[f]; }
+    ca.last = 0;
+    ca.bnext = h->betas[2];
+    hipEvent_t e0, e1;
+    HCHECK(h, hipEventCreate(&e0));
+    HCHECK(h, hipEventCreate(&e1));
+    const int cands[3] = {4, 8, 16};
+    float best = 1e30f;
+    for (int tc : cands) {
+        if (h->nx % tc) continue;
+        HCHECK(h, launch_colseg(tc, h->seg_ms, h->d, h->bg, h->cf, h->seg, s, ca, h->src, clk, sk, h->stream));
+        HCHECK(h, hipEventRecord(e0, h->stream));
+        for (int r = 0; r < 5; r++)
+            HCHECK(h, launch_colseg(tc, h->seg_ms, h->d, h->bg, h->cf, h->seg, s, ca, h->src, clk, sk, h->stream));
+        HCHECK(h, hipEventRecord(e1, h->stream));
+        HCHECK(h, hipEventSynchronize(e1));
+        float ms = 0.f;
+        HCHECK(h, hipEventElapsedTime(&ms, e0, e1));
+        if (ms < best) { best = ms; h->seg_tc = tc; }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return SPARC_OK;
+}
+
+// Steps per captured graph of the time loop.
+static int graph_steps()
+{
+    if (const char *e = std::getenv("SPARC_GRAPH_STEPS")) return std::max(0, std::atoi(e));
+    return 16;
+}
+
 int sparc_run(sparc_handle *h, int maxtime, double *vz_out, double *snaps)
 {
     if (!h || maxtime < 0) return SPARC_E_ARG;
     if (!h->have_src) return fail(h, SPARC_E_STATE, "sources not set");
     if (maxtime != h->maxtime) return fail(h, SPARC_E_STATE, "maxtime differs from the sources' maxtime");
-    int r = sparc_reset(h);
+    int r = autotune(h);
+    if (r) return r;
+    r = sparc_reset(h);
     if (r) return r;
     Sink sk = empty_sink(h);
     const int nout = maxtime / h->steps + 1;
@@ -891,8 +1111,36 @@ int sparc_run(sparc_handle *h, int maxtime, double *vz_out, double *snaps)
     sk.nout = vz_out ? nout : 0;
     sk.snap = snaps;
     sk.nsnap = snaps ? nout : 0;
-    for (int t = 0; t <= maxtime; t++) {
-        r = enqueue_step(h, 0, t > 0, sk);
+    // step 0 (no pending v_x filter) is launched directly; the rest in
+    // hipGraphs of G steps each, every kernel reading the step counter
+    // (t = counter + offset in the graph) so one graph serves every block.
+    const int nsteps = maxtime + 1;
+    r = enqueue_step(h, 0, false, sk);
+    if (r) return r;
+    HCHECK(h, launch_tick(h->tctr, 1, h->stream));
+    int done = 1;
+    const int G = graph_steps();
+    if (G > 1 && nsteps - done >= G) {
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        HCHECK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+        int rc = SPARC_OK;
+        for (int g = 0; g < G && rc == SPARC_OK; g++) rc = enqueue_step(h, g, true, sk);
+        if (rc == SPARC_OK && launch_tick(h->tctr, G, h->stream) != hipSuccess) rc = SPARC_E_HIP;
+        const hipError_t ce = hipStreamEndCapture(h->stream, &graph);
+        if (rc) return rc;
+        HCHECK(h, ce);
+        HCHECK(h, hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+        while (nsteps - done >= G) {
+            HCHECK(h, hipGraphLaunch(exec, h->stream));
+            done += G;
+        }
+        HCHECK(h, hipStreamSynchronize(h->stream));
+        (void)hipGraphExecDestroy(exec);
+        (void)hipGraphDestroy(graph);
+    }
+    for (; done < nsteps; done++) {
+        r = enqueue_step(h, 0, true, sk);
         if (r) return r;
         HCHECK(h, launch_tick(h->tctr, 1, h->stream));
     }
@@ -934,7 +1182,7 @@ int sparc_get_pml(sparc_handle *h, double *pv, double *px)
 int sparc_ddx(sparc_handle *h, const double *in, double *out, int nrows)
 {
     if (!h || !in || !out || nrows < 1) return SPARC_E_ARG;
-    HCHECK(h, launch_ddx_rows(in, out, nrows, h->nx, h->bg.tw, h->bg.eyekx, h->stretchx, h->stream));
+    HCHECK(h, launch_ddx_rows(in, out, nrows, h->nx, h->bg.tw, h->bg.dmul, h->stretchx, h->stream));
     return SPARC_OK;
 }
 
@@ -985,7 +1233,7 @@ int sparc_kernels(sparc_handle *h, int nt, const double *fwd, const double *adj,
             const Job &jb = jobs[j];
             HCHECK(h, launch_gather(jb.src, g, S, nt, jb.slot, jb.nfield, jb.field, n2, h->stream));
             if (jb.op == 0)
-                HCHECK(h, launch_ddx_rows(g, D + j * n, S * h->nz, h->nx, h->bg.tw, h->bg.eyekx,
+                HCHECK(h, launch_ddx_rows(g, D + j * n, S * h->nz, h->nx, h->bg.tw, h->bg.dmul,
                                           h->stretchx, h->stream));
             else
                 HCHECK(h, launch_ddz_cols(g, D + j * n, h->nx, h->nz, S, 1, h->bg.stretch, h->cf, tr1,
@@ -997,7 +1245,7 @@ int sparc_kernels(sparc_handle *h, int nt, const double *fwd, const double *adj,
     // curl (bcz = 0, kernels.f90:832) and unit scalings (:498-579)
     HCHECK(h, launch_ddz_cols(KX, t1, h->nx, h->nz, S, 0, h->bg.stretch, h->cf, h->tri[0], work, 1,
                               h->stream));
-    HCHECK(h, launch_ddx_rows(KZ, t2, S * h->nz, h->nx, h->bg.tw, h->bg.eyekx, h->stretchx, h->stream));
+    HCHECK(h, launch_ddx_rows(KZ, t2, S * h->nz, h->nx, h->bg.tw, h->bg.dmul, h->stretchx, h->stream));
     const double diml = kRsun, dimc = (double)1e6f, e8 = (double)1e-8f, e5 = (double)1e-5f;
     const double dimen = std::pow(diml * e8 * h->xlength * e8 * dimc * e5, -1.0) * 1.0;
     const double dimen_c = std::pow(diml * e8 * h->xlength * e8, -1.0) * 1.0;
@@ -1007,6 +1255,51 @@ int sparc_kernels(sparc_handle *h, int nt, const double *fwd, const double *adj,
     HCHECK(h, hipStreamSynchronize(h->stream));
     HCHECK(h, hipFree(buf));
     return SPARC_OK;
+}
+
+int sparc_profile_kernels(sparc_handle *h, int reps, double *ms_out)
+{
+    if (!h || reps < 1 || !ms_out) return SPARC_E_ARG;
+    if (!h->have_src) return fail(h, SPARC_E_STATE, "sources not set");
+    Sink sk = empty_sink(h);
+    Clock clk{h->tctr, 0, h->steps};
+    State &s = h->st;
+    hipEvent_t e0, e1;
+    HCHECK(h, hipEventCreate(&e0));
+    HCHECK(h, hipEventCreate(&e1));
+    auto timed = [&](int which, double *ms) -> int {
+        for (int pass = 0; pass < 2; pass++) {   // pass 0 warms up
+            HCHECK(h, hipEventRecord(e0, h->stream));
+            for (int r = 0; r < reps; r++) {
+                if (which == 0 || which == 1) {
+                    ColArgs ca{};
+                    for (int f = 0; f < 4; f++) { ca.T[f] = s.t[f]; ca.TN[f] = s.t[f]; }
+                    ca.last = which == 1;
+                    ca.bnext = which == 1 ? h->deltat : h->betas[2];
+                    HCHECK(h, launch_colseg(h->seg_tc, h->seg_ms, h->d, h->bg, h->cf, h->seg, s, ca, h->src,
+                                            clk, sk, h->stream));
+                } else {
+                    RowArgs ra{};
+                    ra.xi_x = s.t[0]; ra.v_z = s.t[3]; ra.v_x_out = s.t[2];
+                    ra.bx2 = h->betas[2]; ra.bx1 = h->betas[3];
+                    ra.filt_vx = which == 3;
+                    HCHECK(h, launch_row(which == 2 ? 1 : 0, h->d, h->bg, s, ra, clk, sk, h->stream));
+                }
+            }
+            HCHECK(h, hipEventRecord(e1, h->stream));
+            HCHECK(h, hipEventSynchronize(e1));
+        }
+        float t = 0.f;
+        HCHECK(h, hipEventElapsedTime(&t, e0, e1));
+        *ms = (double)t / reps;
+        return SPARC_OK;
+    };
+    int r = SPARC_OK;
+    for (int w = 0; w < 4 && r == SPARC_OK; w++) r = timed(w, &ms_out[w]);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (r) return r;
+    return sparc_reset(h);
 }
 
 }  // extern "C"

@@ -1,0 +1,315 @@
+// sparc_colseg.hip -- segmented column pass of the 2D option-17 solver.
+//
+// Same work as the column half of one RK stage of step.f90:48-106 /
+// physics2d.f90:943-1094: the z-derivatives of xi_z, v_z, v_x and p by dbyd2
+// (dbyd2.f:8-200, mtridag.f:95-112), the z-PML, divergence, pressure and
+// density perturbation, RHS v_z with the source and flow terms, the zeroed
+// rows and columns, the RK stage update and, after the last stage,
+// FILTER_VARS_2D (physics2d.f90:509-556).
+//
+// Every column is split into P <= 16 segments of exactly MS rows, held in
+// registers by one lane each; a block covers TC columns (blockDim = 16*TC).
+// Each z-derivative is: dbyd2 right-hand side from a 7-row window, local
+// Thomas solve of the segment, exchange of the segment's end values through
+// LDS, and the exact correction y - L_{s-1} V - F_{s+1} W.  Loads use clamped
+// row indices and stores go to a discard slot for padding rows, so the
+// unrolled row loops are straight-line code the scheduler can batch.
+#include "sparc_device.h"
+#include "sparc_launch.h"
+
+namespace sparc {
+
+namespace {
+
+enum ZRow { Z_STRETCH = 0, Z_G = 1 };
+__device__ __forceinline__ int zrow(int slot, int k) { return 2 + 5 * slot + k; }  // k: 0 lowr 1 rbl 2 gl 3 V 4 W
+
+// dbyd2 right-hand side (dbyd2.f:91-172) of rows j = a..a+MS-1; ld(k) returns
+// B(a - 3 + k) for any k in [0, MS + 6) (values outside the column are never
+// used by the formula of a real row).  Padding rows (j >= nz) get 0.
+template <int MS, typename Loader>
+__device__ __forceinline__ void seg_rhs(Loader ld, double (&R)[MS], int a, int nz, bool first,
+                                        int ibc, double preset0, double presetN, const DbyCoef &cf)
+{
+    double w0 = ld(0), w1 = ld(1), w2 = ld(2), w3 = ld(3), w4 = ld(4), w5 = ld(5);
+#pragma unroll
+    for (int i = 0; i < MS; i++) {
+        const double w6 = ld(i + 6);  // w0..w6 = B(j-3..j+3), j = a + i
+        const int j = a + i;
+        double r = cf.c1 * (w4 - w2) + cf.c2 * (w5 - w1);
+        if (i == 0 && first) {
+            r = (ibc == 1 || ibc == 3) ? cf.f1 * w3 + cf.f2 * w4 + cf.f3 * w5 + cf.f4 * w6
+                                       : (ibc == 0 ? 0.0 : preset0);
+        }
+        if (i == 1 && first) r = cf.b1 * w2 + cf.b2 * w3 + cf.b3 * w4 + cf.b4 * w5 + cf.b5 * w6;
+        const double rm2 = -(cf.b1 * w4 + cf.b2 * w3 + cf.b3 * w2 + cf.b4 * w1 + cf.b5 * w0);
+        const double rm1 = (ibc == 1 || ibc == 2) ? -(cf.f1 * w3 + cf.f2 * w2 + cf.f3 * w1 + cf.f4 * w0)
+                                                  : (ibc == 0 ? 0.0 : presetN);
+        r = (j == nz - 2) ? rm2 : r;
+        r = (j == nz - 1) ? rm1 : r;
+        r = (j >= nz) ? 0.0 : r;
+        R[i] = r;
+        w0 = w1; w1 = w2; w2 = w3; w3 = w4; w4 = w5; w5 = w6;
+    }
+    if (first && (ibc & 1)) R[0] = (cf.alpha2 * R[0] - cf.beta1 * R[1]) * cf.cnst;  // dbyd2.f:163-167
+    if (ibc == 1 || ibc == 2) {                                                     // dbyd2.f:168-172
+#pragma unroll
+        for (int i = 1; i < MS; i++)
+            if (a + i == nz - 1) R[i] = (cf.alpha2 * R[i] - cf.beta1 * R[i - 1]) * cf.cnst;
+    }
+}
+
+// Local Thomas solve of the segment (factorization restarted at its start).
+template <int MS>
+__device__ __forceinline__ void seg_solve(double (&Y)[MS], const double *__restrict__ lowr,
+                                          const double *__restrict__ rbl, const double *__restrict__ gl)
+{
+    Y[0] = Y[0] * rbl[0];
+#pragma unroll
+    for (int i = 1; i < MS; i++) Y[i] = (Y[i] - lowr[i] * Y[i - 1]) * rbl[i];
+#pragma unroll
+    for (int i = MS - 2; i >= 0; i--) Y[i] = Y[i] - gl[i + 1] * Y[i + 1];
+}
+
+// y <- y - L_{s-1} V - F_{s+1} W; interface values from Minv over a window.
+template <int MS>
+__device__ __forceinline__ void seg_correct(double (&Y)[MS], int s, int P, int hw, const double *y,
+                                            const double *mvt, const double *__restrict__ V,
+                                            const double *__restrict__ W)
+{
+    const int n2P = 2 * P;
+    double Lp = 0.0, Fn = 0.0;
+    if (s > 0) {
+        const double *row = mvt + (2 * s - 1) * n2P;
+        const int q0 = max(0, 2 * (s - 1 - hw)), q1 = min(n2P, 2 * (s - 1 + hw) + 2);
+        for (int q = q0; q < q1; q++) Lp += row[q] * y[q];
+    }
+    if (s < P - 1) {
+        const double *row = mvt + (2 * s + 2) * n2P;
+        const int q0 = max(0, 2 * (s + 1 - hw)), q1 = min(n2P, 2 * (s + 1 + hw) + 2);
+        for (int q = q0; q < q1; q++) Fn += row[q] * y[q];
+    }
+#pragma unroll
+    for (int i = 0; i < MS; i++) Y[i] = Y[i] - Lp * V[i] - Fn * W[i];
+}
+
+}  // namespace
+
+template <int MS, int LAST>
+__global__ __launch_bounds__(256) void k_colseg(Dims d, Bg bg, DbyCoef cf, SegTab sg, State st,
+                                                ColArgs ca, Src src, Clock clk, Sink sink)
+{
+    __shared__ double mv[3 * 4 * kSegMax * kSegMax];
+    __shared__ double yb[4][16 * 2 * kSegMax];
+    const int TC = blockDim.x / kSegMax;
+    const int P = sg.P, n2P = 2 * P, nzp = sg.nzp;
+    const int tid = threadIdx.x;
+    for (int q = tid; q < 3 * n2P * n2P; q += blockDim.x) mv[q] = sg.Minv[q];
+    const int c = tid % TC, s = tid / TC;
+    const int x = blockIdx.x * TC + c, b = blockIdx.y;
+    const int nx = d.nx, nz = d.nz;
+    const bool act = s < P;
+    const int sc = act ? s : 0;
+    const int a = sc * MS;                  // first row of the segment
+    const bool first = act && s == 0;
+    const size_t col = (size_t)b * nz * nx + x;
+    const int time = clock_t_(clk);
+    const int nb = d.npmlbot, ntop = d.npmltop, nzpml = d.nzpml;
+    const size_t pz0 = (size_t)b * nzpml * nx + x;
+    const size_t pz1 = (size_t)(d.S + b) * nzpml * nx + x;
+    const double *zt = sg.ztab + a;         // zt[row * nzp + i] = table(row, a + i)
+    double *const dump = st.discard + tid;  // destination of padding-row stores
+    __syncthreads();
+
+    auto rowq = [&](int i) -> size_t { return col + (size_t)min(a + i, nz - 1) * nx; };
+    auto valid = [&](int i) -> bool { return act && a + i < nz; };
+    auto gload = [&](const double *F) {
+        return [=](int k) -> double {
+            const int j = min(max(a - 3 + k, 0), nz - 1);
+            return F[col + (size_t)j * nx];
+        };
+    };
+    // z-PML row index of row j (clamped into the table for other rows)
+    auto pmlrow = [&](int j) -> int {
+        const int jc = min(j, nz - 1);
+        return jc < nb ? jc : min(nzpml - 1, nb + max(jc - (nz - ntop), 0));
+    };
+    auto is_pml = [&](int j) -> bool { return j < nb || (j >= nz - ntop && j < nz); };
+
+    // source value of this column at its injection row (physics2d.f90:1051-1061)
+    double srcv = 0.0;
+    const int isrc = src.row - a;
+    if (act && isrc >= 0 && isrc < MS && time <= src.limit) {
+        if (src.mode == 1) {
+            const double fs = (time >= 0 && time < src.nfwd) ? src.fwd[time] : 0.0;
+            srcv = src.vr[(size_t)b * nx + x] * fs * src.delta_width / bg.rho0_src[x];
+        } else if (src.mode == 2) {
+            srcv = lagrange_forcing(src, b, x, nx, time) * src.delta_width / bg.rho0_src[x];
+        }
+    }
+
+    double Y[MS];
+    double *const wB = st.w_rho;            // partial RHS v_z between phases
+
+    // --- A1: d xi_z/dz (ibc 3, top row preset, physics2d.f90:954-955), z-PML
+    //         (:957-967), divergence, p and rho' (:990-998, :1043)
+    seg_rhs<MS>(gload(ca.T[1]), Y, a, nz, first, 3, 0.0, st.topbc[(size_t)b * nx + x], cf);
+    seg_solve<MS>(Y, zt + zrow(2, 0) * nzp, zt + zrow(2, 1) * nzp, zt + zrow(2, 2) * nzp);
+    if (act) { yb[0][c * n2P + 2 * s] = Y[0]; yb[0][c * n2P + 2 * s + 1] = Y[MS - 1]; }
+    __syncthreads();
+    if (act) seg_correct<MS>(Y, s, P, sg.hw, yb[0] + c * n2P, mv + 2 * n2P * n2P, zt + zrow(2, 3) * nzp,
+                             zt + zrow(2, 4) * nzp);
+#pragma unroll
+    for (int i = 0; i < MS; i++) {
+        const int j = a + i;
+        const size_t qj = rowq(i), gq = qj - (size_t)b * nz * nx;
+        const int pj = pmlrow(j);
+        const size_t qp = pz0 + (size_t)pj * nx, qb = (size_t)pj * nx + x;
+        const double psi = st.pz_tmp[qp], az = bg.az[qb], bzc = bg.bz[qb], pbase = st.pz_base[qp];
+        const double dxix = st.dxixdx[qj], xiz = ca.T[1][qj];
+        const double c2r = bg.c2rho0[gq], gp0 = bg.gp0[gq], r0 = bg.rho0[gq], gr0 = bg.gr0[gq];
+        const double rinv = bg.rhoinv[gq], gz = zt[Z_G * nzp + i];
+        double dz = Y[i] * zt[Z_STRETCH * nzp + i];
+        const bool pm = is_pml(j);
+        const double prhs = az * dz + bzc * psi;
+        dz = pm ? dz + psi : dz;
+        const double pnew = pbase + ca.bnext * prhs;
+        if (pm && act) {
+            st.pz_tmp[qp] = pnew;
+            if (LAST) st.pz_base[qp] = pnew;
+        }
+        const double div = dxix + dz;
+        const double pv = -(c2r * div) - gp0 * xiz;
+        *(valid(i) ? &st.p[qj] : dump) = pv;
+        const double rho = -(r0 * div) - gr0 * xiz;
+        double bzv = -rinv * (rho * gz);
+        bzv = (i == isrc) ? bzv + srcv : bzv;
+        *(valid(i) ? &wB[qj] : dump) = bzv;
+    }
+
+    // --- A2: d v_z/dz (ibc 1, :1076) and the flow terms of RHS v_z (:1073-1077)
+    seg_rhs<MS>(gload(ca.T[3]), Y, a, nz, first, 1, 0.0, 0.0, cf);
+    seg_solve<MS>(Y, zt + zrow(0, 0) * nzp, zt + zrow(0, 1) * nzp, zt + zrow(0, 2) * nzp);
+    if (act) { yb[1][c * n2P + 2 * s] = Y[0]; yb[1][c * n2P + 2 * s + 1] = Y[MS - 1]; }
+    __syncthreads();
+    if (act) seg_correct<MS>(Y, s, P, sg.hw, yb[1] + c * n2P, mv, zt + zrow(0, 3) * nzp, zt + zrow(0, 4) * nzp);
+#pragma unroll
+    for (int i = 0; i < MS; i++) {
+        const size_t qj = rowq(i), gq = qj - (size_t)b * nz * nx;
+        const double v0x = bg.v0x[gq], v0z = bg.v0z[gq], dxvz = st.dxvz[qj], bz0 = wB[qj];
+        const double dzvz = Y[i] * zt[Z_STRETCH * nzp + i];
+        double bzv = bz0 - 2.0 * v0x * dxvz;
+        bzv = bzv - 2.0 * v0z * dzvz;
+        *(valid(i) ? &wB[qj] : dump) = bzv;
+    }
+
+    // --- A3: d v_x/dz (ibc 1, :1070) and the flow part of RHS v_x (:1067-1071)
+    seg_rhs<MS>(gload(ca.T[2]), Y, a, nz, first, 1, 0.0, 0.0, cf);
+    seg_solve<MS>(Y, zt + zrow(0, 0) * nzp, zt + zrow(0, 1) * nzp, zt + zrow(0, 2) * nzp);
+    if (act) { yb[2][c * n2P + 2 * s] = Y[0]; yb[2][c * n2P + 2 * s + 1] = Y[MS - 1]; }
+    __syncthreads();
+    if (act) seg_correct<MS>(Y, s, P, sg.hw, yb[2] + c * n2P, mv, zt + zrow(0, 3) * nzp, zt + zrow(0, 4) * nzp);
+#pragma unroll
+    for (int i = 0; i < MS; i++) {
+        const int j = a + i;
+        const size_t qj = rowq(i), gq = qj - (size_t)b * nz * nx;
+        const double v0x = bg.v0x[gq], v0z = bg.v0z[gq], dxvx = st.dxvx[qj];
+        double rvx = -(2.0 * v0x * dxvx) - 2.0 * v0z * (Y[i] * zt[Z_STRETCH * nzp + i]);
+        rvx = (j == 0 || x == 0 || x == nx - 1) ? 0.0 : rvx;
+        *(valid(i) ? &st.rvx[qj] : dump) = rvx;
+    }
+
+    // --- A4: d p/dz (ibc 2, bottom row zero, :1023) with the z-PML (:1025-1035),
+    //         RHS v_z, zeroed rows/columns (:1083-1093), RK stage update.
+    // p of the whole column was written in A1; the block barriers since make
+    // the neighbouring segments' rows visible.
+    seg_rhs<MS>(gload(st.p), Y, a, nz, first, 2, 0.0, 0.0, cf);
+    seg_solve<MS>(Y, zt + zrow(1, 0) * nzp, zt + zrow(1, 1) * nzp, zt + zrow(1, 2) * nzp);
+    if (act) { yb[3][c * n2P + 2 * s] = Y[0]; yb[3][c * n2P + 2 * s + 1] = Y[MS - 1]; }
+    __syncthreads();
+    if (act) seg_correct<MS>(Y, s, P, sg.hw, yb[3] + c * n2P, mv + n2P * n2P, zt + zrow(1, 3) * nzp,
+                             zt + zrow(1, 4) * nzp);
+    const bool snap = LAST && sink.snap && time % clk.steps == 0 && time / clk.steps < sink.nsnap;
+    const size_t slot = snap ? ((size_t)b * sink.nsnap + time / clk.steps) * sink.nfield : 0;
+#pragma unroll
+    for (int i = 0; i < MS; i++) {
+        const int j = a + i;
+        const size_t qj = rowq(i), gq = qj - (size_t)b * nz * nx;
+        const int pj = pmlrow(j);
+        const size_t qp = pz1 + (size_t)pj * nx, qb = (size_t)pj * nx + x;
+        const double psi = st.pz_tmp[qp], az = bg.az[qb], bzc = bg.bz[qb], pbase = st.pz_base[qp];
+        const double rinv = bg.rhoinv[gq];
+        const double vx = ca.T[2][qj], vz = ca.T[3][qj];
+        const double a0 = st.a[0][qj], a1 = st.a[1][qj], a3 = st.a[3][qj];
+        double gpz = Y[i] * zt[Z_STRETCH * nzp + i];
+        const bool pm = is_pml(j);
+        const double prhs = az * gpz + bzc * psi;
+        gpz = pm ? gpz + psi : gpz;
+        const double pnew = pbase + ca.bnext * prhs;
+        if (pm && act) {
+            st.pz_tmp[qp] = pnew;
+            if (LAST) st.pz_base[qp] = pnew;
+        }
+        double rvz = -rinv * gpz + wB[qj];
+        double rxix = vx, rxiz = vz;
+        const bool zero = j == 0 || x == 0 || x == nx - 1;
+        rvz = zero ? 0.0 : rvz;
+        rxix = zero ? 0.0 : rxix;
+        rxiz = zero ? 0.0 : rxiz;
+        const bool v = valid(i);
+        *(v ? &ca.TN[0][qj] : dump) = a0 + ca.bnext * rxix;
+        *(v ? &ca.TN[1][qj] : dump) = a1 + ca.bnext * rxiz;
+        *(v ? &ca.TN[3][qj] : dump) = a3 + ca.bnext * rvz;
+        if (snap && v)
+            sink.snap[((slot + (sink.adjoint ? 3 : 5)) * nz + j) * nx + x] = rvz;
+    }
+    if (LAST) {
+        // FILTER_VARS_2D on xi_x, xi_z, v_z of the column; neighbour rows were
+        // written by the other segments of this block
+        __syncthreads();
+        const bool out = sink.vz_out && time % clk.steps == 0 && time / clk.steps < sink.nout;
+#pragma unroll
+        for (int i = 0; i < MS; i++) {
+            const int j = min(a + i, nz - 1);
+            const size_t qj = rowq(i);
+            const double fxix = filter_vars_2d_at(ca.TN[0], qj, j, x, nx, nz, d.nh);
+            const double fxiz = filter_vars_2d_at(ca.TN[1], qj, j, x, nx, nz, d.nh);
+            const double fvz = filter_vars_2d_at(ca.TN[3], qj, j, x, nx, nz, d.nh);
+            const bool v = valid(i);
+            *(v ? &st.a[0][qj] : dump) = fxix;
+            *(v ? &st.a[1][qj] : dump) = fxiz;
+            *(v ? &st.a[3][qj] : dump) = fvz;
+            if (v && out && j == sink.orow)
+                sink.vz_out[((size_t)b * sink.nout + time / clk.steps) * nx + x] = fvz;
+            if (v && snap) {
+                const size_t o = (size_t)j * nx + x;
+                if (!sink.adjoint) {
+                    sink.snap[(slot + 0) * nz * nx + o] = fxix;
+                    sink.snap[(slot + 1) * nz * nx + o] = fxiz;
+                    sink.snap[(slot + 3) * nz * nx + o] = fvz;
+                } else {
+                    sink.snap[(slot + 1) * nz * nx + o] = fvz;
+                }
+            }
+        }
+    }
+}
+
+hipError_t launch_colseg(int tc, int ms, const Dims &d, const Bg &bg, const DbyCoef &cf, const SegTab &sg,
+                         const State &s, const ColArgs &ca, const Src &src, const Clock &c, const Sink &sk,
+                         hipStream_t st)
+{
+    if ((tc != 4 && tc != 8 && tc != 16) || d.nx % tc || sg.m != ms) return hipErrorInvalidValue;
+    dim3 grid(d.nx / tc, d.S), block(tc * kSegMax);
+#define L(MSV)                                                                                         \
+    if (ms == MSV) {                                                                                   \
+        if (ca.last) hipLaunchKernelGGL((k_colseg<MSV, 1>), grid, block, 0, st, d, bg, cf, sg, s, ca, src, c, sk); \
+        else hipLaunchKernelGGL((k_colseg<MSV, 0>), grid, block, 0, st, d, bg, cf, sg, s, ca, src, c, sk);         \
+        return hipGetLastError();                                                                      \
+    }
+    SPARC_COLSEG_SIZES(L)
+#undef L
+    return hipErrorInvalidValue;
+}
+
+}  // namespace sparc

@@ -25,6 +25,7 @@ struct Bg {
     const double *ax, *bx;                                 // [nz][nph]
     const double *rho0_src;                                // [nx] rho0 at the injection row
     const double *eyekx;                                   // [nx/2+1]
+    const double *dmul;                                    // [nx] bin multiplier (see fft_deriv_team)
     const double2 *tw;                                     // [nx] exp(-2 pi i k/nx)
     double stretchx;
 };
@@ -44,6 +45,22 @@ struct Tri {
     const double *lowr, *gam, *rbet;
 };
 
+// Segmented (SPIKE) form of the same matrices for the column pass: the column
+// is cut into P <= 16 segments of exactly m rows (rows >= nz of the last
+// segment are identity padding).  Each segment is solved locally (rbl, gl:
+// Thomas factors restarted at the segment start) and corrected with the spikes
+// V (response to the row above the segment) and W (to the row below), weighted
+// by the interface values from the exact inverse Minv of the 2P x 2P
+// interface system.  ztab rows (each nzp = P*m long): 0 stretch, 1 g, then
+// for slot 0 (ibc 1), 1 (ibc 2), 2 (ibc 3): lowr, rbl, gl, V, W.
+constexpr int kSegMax = 16;
+constexpr int kZT = 17;
+struct SegTab {
+    int P, m, hw, nzp;              // segments, rows per segment, Minv window, P*m
+    const double *ztab;             // [kZT][nzp]
+    const double *Minv;             // [3][2P][2P]
+};
+
 // Prognostic state and work arrays of a batch (device pointers).
 struct State {
     double *a[4];           // xi_x, xi_z, v_x, v_z   [S][nz][nx]
@@ -53,7 +70,8 @@ struct State {
     double *px_base, *px_tmp;  // [2][S][nz][nph]    (psivx, psi_gradp_x)
     double *dxixdx, *dxvx, *dxvz, *p, *rvx;  // [S][nz][nx]
     double *topbc;          // [S][nx]
-    double *w_xiz, *w_vx, *w_vz, *w_p, *w_rho;  // column sweep scratch [S][nz][nx]
+    double *w_rho;          // partial RHS v_z between column-pass phases [S][nz][nx]
+    double *discard;        // >= 256 doubles: target of stores from padding rows
 };
 
 // Source term of the RHS (physics2d.f90:1051-1061).
@@ -180,27 +198,23 @@ __device__ __forceinline__ void dft_reg(double (&re)[N], double (&im)[N])
 template <int N1, int N2>
 __device__ __forceinline__ void fft_deriv_team(double (&re)[N2], double (&im)[N2], int t,
                                                double2 *lds, const double2 *__restrict__ tw,
-                                               const double *__restrict__ eyekx, double stretchx)
+                                               const double *__restrict__ dmul, double stretchx)
 {
-    constexpr int n = N1 * N2;
     constexpr int LD = N1 + 1;
     // step 1: DFT over j2, twiddle W_n^(t*k2), transpose through LDS
     dft_reg<N2, -1>(re, im);
 #pragma unroll
     for (int k2 = 0; k2 < N2; k2++) {
-        double r = re[k2], i = im[k2];
-        if (k2 != 0) {
-            const double2 w = tw[t * k2];
-            const double rr = r * w.x - i * w.y;
-            i = r * w.y + i * w.x;
-            r = rr;
-        }
+        const double2 w = tw[t * k2];
+        const double r = re[k2] * w.x - im[k2] * w.y;
+        const double i = re[k2] * w.y + im[k2] * w.x;
         lds[k2 * LD + t] = make_double2(r, i);
     }
     __syncthreads();
-    // step 2 (lanes t < N2): DFT over j1 -> X[t + N2*k1]; derivative multiplier;
+    // step 2 (lanes t < N2): DFT over j1 -> X[t + N2*k1]; times i*dmul[k]
+    // (dmul[k] = eyekx(k) below the Nyquist bin, 0 at it, -eyekx(nx-k) above);
     // inverse DFT over k1; conjugate twiddle; back to LDS
-    if (t < N2) {
+    if (N1 == N2 || t < N2) {
         double ar[N1], ai[N1];
 #pragma unroll
         for (int j1 = 0; j1 < N1; j1++) {
@@ -210,11 +224,7 @@ __device__ __forceinline__ void fft_deriv_team(double (&re)[N2], double (&im)[N2
         dft_reg<N1, -1>(ar, ai);
 #pragma unroll
         for (int k1 = 0; k1 < N1; k1++) {
-            const int k = t + N2 * k1;
-            double w;
-            if (k < n / 2) w = eyekx[k];
-            else if (k == n / 2) w = 0.0;
-            else w = -eyekx[n - k];
+            const double w = dmul[t + N2 * k1];
             const double r = ar[k1];
             ar[k1] = -(ai[k1] * w);
             ai[k1] = r * w;
@@ -222,13 +232,9 @@ __device__ __forceinline__ void fft_deriv_team(double (&re)[N2], double (&im)[N2
         dft_reg<N1, +1>(ar, ai);
 #pragma unroll
         for (int j1 = 0; j1 < N1; j1++) {
-            double r = ar[j1], i = ai[j1];
-            if (j1 != 0 && t != 0) {
-                const double2 w = tw[j1 * t];   // conj: exp(+2 pi i j1 t/n)
-                const double rr = r * w.x + i * w.y;
-                i = i * w.x - r * w.y;
-                r = rr;
-            }
+            const double2 w = tw[j1 * t];   // conj: exp(+2 pi i j1 t/n)
+            const double r = ar[j1] * w.x + ai[j1] * w.y;
+            const double i = ai[j1] * w.x - ar[j1] * w.y;
             lds[t * LD + j1] = make_double2(r, i);
         }
     }
@@ -257,5 +263,72 @@ template <> struct FftShape<128> { static constexpr int N1 = 16, N2 = 8; };
 template <> struct FftShape<256> { static constexpr int N1 = 16, N2 = 16; };
 template <> struct FftShape<512> { static constexpr int N1 = 32, N2 = 16; };
 template <> struct FftShape<1024> { static constexpr int N1 = 32, N2 = 32; };
+
+// ---------------------------------------------------------------------------
+// Helpers shared by the pass kernels.
+
+// 11-point z filter (physics2d.f90:517-533, kernels.f90:911-918) at row k of
+// a column whose row stride is s.
+__device__ __forceinline__ double filt11(const double *__restrict__ a, size_t q, size_t s)
+{
+    return 0.75390625 * a[q] + 0.5 * 0.41015625 * (a[q - s] + a[q + s]) +
+           0.5 * -0.234375 * (a[q - 2 * s] + a[q + 2 * s]) +
+           0.5 * 0.087890625 * (a[q - 3 * s] + a[q + 3 * s]) +
+           0.5 * -0.01953125 * (a[q - 4 * s] + a[q + 4 * s]) +
+           0.5 * 0.001953125 * (a[q - 5 * s] + a[q + 5 * s]);
+}
+
+// FILTER_VARS_2D value of row k, column x (HORIZONTAL_PMLS: columns
+// npmlhor..nx-npmlhor+1 and rows 6..nz-1, 1-based, are rewritten).
+__device__ __forceinline__ double filter_vars_2d_at(const double *__restrict__ a, size_t q, int k,
+                                                    int x, int nx, int nz, int nh)
+{
+    if (k < 5 || k > nz - 2 || x < nh - 1 || x > nx - nh) return a[q];
+    if (k <= nz - 6) return filt11(a, q, (size_t)nx);
+    return 0.5 * (a[q + nx] + a[q - nx]);
+}
+
+__device__ __forceinline__ int clock_t_(const Clock &c) { return c.tctr[0] + c.toff; }
+
+// LAGRANGE_INTERP (physics.f90:678-737) for column x of source b.
+__device__ __forceinline__ double lagrange_forcing(const Src &src, int b, int x, int nx, int time)
+{
+    const double cad = src.cadforcing;
+    double xt = (double)time * src.timestep / cad;
+    const int time1 = (int)floor(xt) + 1;
+    xt = time * src.timestep;
+    double xs[7];
+#pragma unroll
+    for (int q = 0; q < 7; q++) xs[q] = (double)(time1 - 4 + q) * cad;
+    const double *vr = src.vr + (size_t)b * src.nvr * nx + x;
+    double f = 0.0;
+#pragma unroll
+    for (int q = 0; q < 7; q++) {
+        bool ok = true;
+        if (q <= 3) ok = xs[q] >= 0.0;
+        if (q == 5) ok = time1 < src.num_steps;
+        if (q == 6) ok = time1 < src.num_steps - 1;
+        double LC = 0.0;
+        if (ok) {
+            const int slice = time1 - 3 + q;  // 1-based vr slice
+            double den = 1.0;
+            bool first = true;
+#pragma unroll
+            for (int r = 0; r < 7; r++) {
+                if (r == q) continue;
+                if (first) { den = xs[q] - xs[r]; first = false; }
+                else den = den * (xs[q] - xs[r]);
+            }
+            const double v = (slice >= 1 && slice <= src.nvr) ? vr[(size_t)(slice - 1) * nx] : 0.0;
+            LC = v / den;
+        }
+        double tq = LC;
+#pragma unroll
+        for (int r = 0; r < 7; r++)
+            if (r != q) tq = tq * (xt - xs[r]);
+        f = (q == 0) ? tq : f + tq;
+    }
+    return f;
+}
 
 }  // namespace sparc

@@ -7,11 +7,11 @@
 //               of grad_x p, RHS v_x and the v_x stage update of stage s, then
 //               the x-derivatives of xi_x, v_x, v_z of stage s+1 and the x-PML
 //               of d xi_x / dx.
-//   column pass C (one lane per (source, x) column): the compact-FD z
-//               derivatives of xi_z, p, v_x, v_z (Thomas sweeps), the z-PML,
-//               divergence, pressure, RHS v_z with the source, the flow terms
-//               and the stage update of xi_x, xi_z, v_z; after the last stage
-//               also the z filter FILTER_VARS_2D (physics2d.f90:509-556).
+//   column pass C (sparc_colseg.hip): the compact-FD z derivatives of xi_z,
+//               p, v_x, v_z, the z-PML, divergence, pressure, RHS v_z with the
+//               source, the flow terms and the stage update of xi_x, xi_z, v_z;
+//               after the last stage also the z filter FILTER_VARS_2D
+//               (physics2d.f90:509-556).
 //
 // A step is R1 C1 R(1|2) C2 R(2|3) C3 R(3|4) C4 R(4|5) C5 R5; the v_x filter of
 // the previous step is applied on the fly by the next R1 (or by k_finalize_vx).
@@ -26,70 +26,6 @@ namespace sparc {
 __device__ __forceinline__ size_t fidx(const Dims &d, int b, int k, int x)
 {
     return ((size_t)b * d.nz + k) * d.nx + x;
-}
-
-// 11-point z filter (physics2d.f90:517-533, kernels.f90:911-918) at row k of
-// a column whose row stride is s.
-__device__ __forceinline__ double filt11(const double *__restrict__ a, size_t q, size_t s)
-{
-    return 0.75390625 * a[q] + 0.5 * 0.41015625 * (a[q - s] + a[q + s]) +
-           0.5 * -0.234375 * (a[q - 2 * s] + a[q + 2 * s]) +
-           0.5 * 0.087890625 * (a[q - 3 * s] + a[q + 3 * s]) +
-           0.5 * -0.01953125 * (a[q - 4 * s] + a[q + 4 * s]) +
-           0.5 * 0.001953125 * (a[q - 5 * s] + a[q + 5 * s]);
-}
-
-// FILTER_VARS_2D value of row k, column x (HORIZONTAL_PMLS: columns
-// npmlhor..nx-npmlhor+1 and rows 6..nz-1, 1-based, are rewritten).
-__device__ __forceinline__ double filter_vars_2d_at(const double *__restrict__ a, size_t q, int k,
-                                                    int x, int nx, int nz, int nh)
-{
-    if (k < 5 || k > nz - 2 || x < nh - 1 || x > nx - nh) return a[q];
-    if (k <= nz - 6) return filt11(a, q, (size_t)nx);
-    return 0.5 * (a[q + nx] + a[q - nx]);
-}
-
-__device__ __forceinline__ int clock_t_(const Clock &c) { return c.tctr[0] + c.toff; }
-
-// LAGRANGE_INTERP (physics.f90:678-737) for column x of source b.
-__device__ double lagrange_forcing(const Src &src, int b, int x, int nx, int time)
-{
-    const double cad = src.cadforcing;
-    double xt = (double)time * src.timestep / cad;
-    const int time1 = (int)floor(xt) + 1;
-    xt = time * src.timestep;
-    double xs[7];
-#pragma unroll
-    for (int q = 0; q < 7; q++) xs[q] = (double)(time1 - 4 + q) * cad;
-    const double *vr = src.vr + (size_t)b * src.nvr * nx + x;
-    double f = 0.0;
-#pragma unroll
-    for (int q = 0; q < 7; q++) {
-        bool ok = true;
-        if (q <= 3) ok = xs[q] >= 0.0;
-        if (q == 5) ok = time1 < src.num_steps;
-        if (q == 6) ok = time1 < src.num_steps - 1;
-        double LC = 0.0;
-        if (ok) {
-            const int slice = time1 - 3 + q;  // 1-based vr slice
-            double den = 1.0;
-            bool first = true;
-#pragma unroll
-            for (int r = 0; r < 7; r++) {
-                if (r == q) continue;
-                if (first) { den = xs[q] - xs[r]; first = false; }
-                else den = den * (xs[q] - xs[r]);
-            }
-            const double v = (slice >= 1 && slice <= src.nvr) ? vr[(size_t)(slice - 1) * nx] : 0.0;
-            LC = v / den;
-        }
-        double tq = LC;
-#pragma unroll
-        for (int r = 0; r < 7; r++)
-            if (r != q) tq = tq * (xt - xs[r]);
-        f = (q == 0) ? tq : f + tq;
-    }
-    return f;
 }
 
 // ---------------------------------------------------------------------------
@@ -203,7 +139,7 @@ template <int NX>
 __global__ __launch_bounds__(64) void k_ddx_rows(const double *__restrict__ in,
                                                  double *__restrict__ out, int nrows,
                                                  const double2 *__restrict__ tw,
-                                                 const double *__restrict__ eyekx, double stretchx)
+                                                 const double *__restrict__ dmul, double stretchx)
 {
     constexpr int N1 = FftShape<NX>::N1, N2 = FftShape<NX>::N2, TEAMS = 64 / N1;
     __shared__ double2 lds[TEAMS][N2 * (N1 + 1)];
@@ -216,7 +152,7 @@ __global__ __launch_bounds__(64) void k_ddx_rows(const double *__restrict__ in,
         re[j2] = r0 < nrows ? in[(size_t)r0 * NX + x] : 0.0;
         im[j2] = r1 < nrows ? in[(size_t)r1 * NX + x] : 0.0;
     }
-    fft_deriv_team<N1, N2>(re, im, t, lds[team], tw, eyekx, stretchx);
+    fft_deriv_team<N1, N2>(re, im, t, lds[team], tw, dmul, stretchx);
 #pragma unroll
     for (int j2 = 0; j2 < N2; j2++) {
         const int x = t + N1 * j2;
@@ -239,310 +175,142 @@ __global__ __launch_bounds__(64) void k_row(Dims d, Bg bg, State st, RowArgs ra,
     const int team = threadIdx.x / N1, t = threadIdx.x % N1;
     const int r = blockIdx.x * TEAMS + team;
     const bool valid = r < d.S * d.nz;
-    const int b = valid ? r / d.nz : 0, k = valid ? r % d.nz : 0;
+    const int rc = valid ? r : d.S * d.nz - 1;     // clamped row: loads stay in bounds
+    const int b = rc / d.nz, k = rc % d.nz;
     const int nz = d.nz, nh = d.nh, nph = d.nph;
     const int time = clock_t_(clk);
     const size_t rowq = ((size_t)b * nz + k) * NX;
     const size_t pxq0 = ((size_t)b * nz + k) * nph;             // psivx row
     const size_t pxq1 = ((size_t)(d.S + b) * nz + k) * nph;     // psi_gradp_x row
-    double re[N2], im[N2];
-    double vx[N2];
+    double *const dump = st.discard + threadIdx.x;              // target of invalid-row stores
+    double re[N2], im[N2], vx[N2];
 
     if (MODE == 0) {
-        // v_x of stage 1 (= a_vx after the pending filter)
+        // v_x of stage 1 (= a_vx after the pending filter), xi_x, v_z
 #pragma unroll
         for (int j2 = 0; j2 < N2; j2++) {
             const int x = t + N1 * j2;
-            double v;
-            if (!valid) v = 0.0;
-            else if (ra.filt_vx) v = filter_vars_2d_at(st.vx_unf, rowq + x, k, x, NX, nz, nh);
-            else v = st.a[2][rowq + x];
+            const double v = ra.filt_vx ? filter_vars_2d_at(st.vx_unf, rowq + x, k, x, NX, nz, nh)
+                                        : st.a[2][rowq + x];
             vx[j2] = v;
-            re[j2] = valid ? ra.xi_x[rowq + x] : 0.0;
+            re[j2] = ra.xi_x[rowq + x];
             im[j2] = v;
         }
-        if (valid && ra.filt_vx) {
+        if (ra.filt_vx) {
             const int tp = time - 1;
             const bool snap = sink.snap && tp >= 0 && tp % clk.steps == 0 && tp / clk.steps < sink.nsnap;
             const int fslot = sink.adjoint ? 0 : 2;
 #pragma unroll
             for (int j2 = 0; j2 < N2; j2++) {
                 const int x = t + N1 * j2;
-                st.a[2][rowq + x] = vx[j2];
-                if (snap)
+                *(valid ? &st.a[2][rowq + x] : dump) = vx[j2];
+                if (snap && valid)
                     sink.snap[((((size_t)b * sink.nsnap + tp / clk.steps) * sink.nfield + fslot) * nz + k) * NX + x] = vx[j2];
             }
         }
-        fft_deriv_team<N1, N2>(re, im, t, lds[team], bg.tw, bg.eyekx, bg.stretchx);
+        fft_deriv_team<N1, N2>(re, im, t, lds[team], bg.tw, bg.dmul, bg.stretchx);
         // re = dxi_x/dx, im = dv_x/dx
-        double dvz_re[N2], dvz_im[N2];
+        double zr[N2], zi[N2];
+#pragma unroll
+        for (int j2 = 0; j2 < N2; j2++) {
+            zr[j2] = ra.v_z[rowq + t + N1 * j2];
+            zi[j2] = 0.0;
+        }
+        fft_deriv_team<N1, N2>(zr, zi, t, lds[team], bg.tw, bg.dmul, bg.stretchx);
 #pragma unroll
         for (int j2 = 0; j2 < N2; j2++) {
             const int x = t + N1 * j2;
-            dvz_re[j2] = valid ? ra.v_z[rowq + x] : 0.0;
-            dvz_im[j2] = 0.0;
-        }
-        fft_deriv_team<N1, N2>(dvz_re, dvz_im, t, lds[team], bg.tw, bg.eyekx, bg.stretchx);
-        if (valid) {
-#pragma unroll
-            for (int j2 = 0; j2 < N2; j2++) {
-                const int x = t + N1 * j2;
-                st.dxvx[rowq + x] = im[j2];
-                st.dxvz[rowq + x] = dvz_re[j2];
-            }
+            *(valid ? &st.dxvx[rowq + x] : dump) = im[j2];
+            *(valid ? &st.dxvz[rowq + x] : dump) = zr[j2];
         }
     } else {
         // dx p (stage s) paired with dx xi_x (stage s+1, or nothing in MODE 2)
 #pragma unroll
         for (int j2 = 0; j2 < N2; j2++) {
             const int x = t + N1 * j2;
-            re[j2] = valid ? st.p[rowq + x] : 0.0;
-            im[j2] = (valid && MODE == 1) ? ra.xi_x[rowq + x] : 0.0;
+            re[j2] = st.p[rowq + x];
+            im[j2] = MODE == 1 ? ra.xi_x[rowq + x] : 0.0;
         }
-        fft_deriv_team<N1, N2>(re, im, t, lds[team], bg.tw, bg.eyekx, bg.stretchx);
+        double rvx0[N2], avx[N2], rinv[N2];
+#pragma unroll
+        for (int j2 = 0; j2 < N2; j2++) {
+            const int x = t + N1 * j2;
+            rvx0[j2] = st.rvx[rowq + x];
+            avx[j2] = st.a[2][rowq + x];
+            rinv[j2] = bg.rhoinv[(size_t)k * NX + x];
+        }
+        fft_deriv_team<N1, N2>(re, im, t, lds[team], bg.tw, bg.dmul, bg.stretchx);
         // x-PML on grad_x p (physics2d.f90:1003-1020), RHS v_x (:1039 + flows)
         const bool snap = MODE == 2 && sink.snap && time % clk.steps == 0 && time / clk.steps < sink.nsnap;
 #pragma unroll
         for (int j2 = 0; j2 < N2; j2++) {
             const int x = t + N1 * j2;
             double gpx = re[j2];
-            if (valid && (x < nh || x >= NX - nh)) {
-                const int pi = x < nh ? x : x - NX + 2 * nh;
-                const double psi = st.px_tmp[pxq1 + pi];
-                gpx = gpx + psi;
-                const double rhs = bg.ax[(size_t)k * nph + pi] * gpx + bg.bx[(size_t)k * nph + pi] * psi;
-                if (ra.last2) {
-                    const double nb = st.px_base[pxq1 + pi] + ra.bx2 * rhs;
-                    st.px_base[pxq1 + pi] = nb;
+            if (N1 * j2 < nh || N1 * (j2 + 1) > NX - nh) {     // team-uniform: this j2 can hold PML columns
+                const bool pml = x < nh || x >= NX - nh;
+                const int pi = x < nh ? x : min(max(x - NX + 2 * nh, 0), nph - 1);
+                const double psi = st.px_tmp[pxq1 + pi], pb = st.px_base[pxq1 + pi];
+                const double axv = bg.ax[(size_t)k * nph + pi], bxv = bg.bx[(size_t)k * nph + pi];
+                const double g2 = gpx + psi;
+                const double rhs = axv * g2 + bxv * psi;
+                const double nb = pb + ra.bx2 * rhs;
+                if (pml && valid) {
                     st.px_tmp[pxq1 + pi] = nb;
-                } else {
-                    st.px_tmp[pxq1 + pi] = st.px_base[pxq1 + pi] + ra.bx2 * rhs;
+                    if (ra.last2) st.px_base[pxq1 + pi] = nb;
                 }
+                gpx = pml ? g2 : gpx;
             }
-            double rvx = 0.0;
-            if (valid) {
-                rvx = -bg.rhoinv[(size_t)k * NX + x] * gpx + st.rvx[rowq + x];
-                if (k == 0 || x == 0 || x == NX - 1) rvx = 0.0;
-            }
-            const double v = valid ? st.a[2][rowq + x] + ra.bx2 * rvx : 0.0;
+            double rvx = -rinv[j2] * gpx + rvx0[j2];
+            rvx = (k == 0 || x == 0 || x == NX - 1) ? 0.0 : rvx;
+            const double v = avx[j2] + ra.bx2 * rvx;
             vx[j2] = v;
-            if (valid) {
-                ra.v_x_out[rowq + x] = v;
-                if (snap)
-                    sink.snap[((((size_t)b * sink.nsnap + time / clk.steps) * sink.nfield +
-                                (sink.adjoint ? 2 : 4)) * nz + k) * NX + x] = rvx;
-            }
+            *(valid ? &ra.v_x_out[rowq + x] : dump) = v;
+            if (snap && valid)
+                sink.snap[((((size_t)b * sink.nsnap + time / clk.steps) * sink.nfield +
+                            (sink.adjoint ? 2 : 4)) * nz + k) * NX + x] = rvx;
         }
         if (MODE == 1) {
             // im holds dxi_x/dx of stage s+1; now dv_x/dx, dv_z/dx of stage s+1
             double wr[N2], wi[N2];
 #pragma unroll
             for (int j2 = 0; j2 < N2; j2++) {
-                const int x = t + N1 * j2;
                 wr[j2] = vx[j2];
-                wi[j2] = valid ? ra.v_z[rowq + x] : 0.0;
+                wi[j2] = ra.v_z[rowq + t + N1 * j2];
             }
-            fft_deriv_team<N1, N2>(wr, wi, t, lds[team], bg.tw, bg.eyekx, bg.stretchx);
-            if (valid) {
+            fft_deriv_team<N1, N2>(wr, wi, t, lds[team], bg.tw, bg.dmul, bg.stretchx);
 #pragma unroll
-                for (int j2 = 0; j2 < N2; j2++) {
-                    const int x = t + N1 * j2;
-                    st.dxvx[rowq + x] = wr[j2];
-                    st.dxvz[rowq + x] = wi[j2];
-                }
+            for (int j2 = 0; j2 < N2; j2++) {
+                const int x = t + N1 * j2;
+                *(valid ? &st.dxvx[rowq + x] : dump) = wr[j2];
+                *(valid ? &st.dxvz[rowq + x] : dump) = wi[j2];
             }
 #pragma unroll
             for (int j2 = 0; j2 < N2; j2++) re[j2] = im[j2];
         }
     }
-    if (MODE == 2 || !valid) return;
+    if (MODE == 2) return;
     // re = dxi_x/dx of the next stage: top boundary value for d xi_z/dz
     // (physics2d.f90:954, before the x-PML), then the x-PML (:970-986)
 #pragma unroll
     for (int j2 = 0; j2 < N2; j2++) {
         const int x = t + N1 * j2;
         double dx = re[j2];
-        if (k == nz - 1) st.topbc[(size_t)b * NX + x] = -dx * bg.unstretch[nz - 1];
-        if (x < nh || x >= NX - nh) {
-            const int pi = x < nh ? x : x - NX + 2 * nh;
-            const double psi = st.px_tmp[pxq0 + pi];
-            dx = dx + psi;
-            const double rhs = bg.ax[(size_t)k * nph + pi] * dx + bg.bx[(size_t)k * nph + pi] * psi;
-            if (ra.last1) {
-                const double nb = st.px_base[pxq0 + pi] + ra.bx1 * rhs;
-                st.px_base[pxq0 + pi] = nb;
+        if (k == nz - 1 && valid) st.topbc[(size_t)b * NX + x] = -dx * bg.unstretch[nz - 1];
+        if (N1 * j2 < nh || N1 * (j2 + 1) > NX - nh) {
+            const bool pml = x < nh || x >= NX - nh;
+            const int pi = x < nh ? x : min(max(x - NX + 2 * nh, 0), nph - 1);
+            const double psi = st.px_tmp[pxq0 + pi], pb = st.px_base[pxq0 + pi];
+            const double axv = bg.ax[(size_t)k * nph + pi], bxv = bg.bx[(size_t)k * nph + pi];
+            const double d2 = dx + psi;
+            const double rhs = axv * d2 + bxv * psi;
+            const double nb = pb + ra.bx1 * rhs;
+            if (pml && valid) {
                 st.px_tmp[pxq0 + pi] = nb;
-            } else {
-                st.px_tmp[pxq0 + pi] = st.px_base[pxq0 + pi] + ra.bx1 * rhs;
+                if (ra.last1) st.px_base[pxq0 + pi] = nb;
             }
+            dx = pml ? d2 : dx;
         }
-        st.dxixdx[rowq + x] = dx;
-    }
-}
-
-// ---------------------------------------------------------------------------
-// Column pass of stage s.
-
-
-template <int LAST>
-__global__ __launch_bounds__(64) void k_col(Dims d, Bg bg, Tri t1, Tri t2, Tri t3, DbyCoef cf,
-                                            State st, ColArgs ca, Src src, Clock clk, Sink sink)
-{
-    const int x = blockIdx.x * 64 + threadIdx.x;
-    const int b = blockIdx.y;
-    if (x >= d.nx) return;
-    const int nx = d.nx, nz = d.nz;
-    const size_t s = (size_t)nx;
-    const size_t q0 = (size_t)b * nz * nx + x;
-    const int time = clock_t_(clk);
-    const double *XIZ = ca.T[1], *VX = ca.T[2], *VZ = ca.T[3];
-
-    // Loop A: forward sweeps of d/dz for xi_z (ibc 3, top row preset by the
-    // caller, physics2d.f90:954-955), v_x and v_z (ibc 1, :1070, :1076)
-    FwdSweep fz, fx, fv;
-    const double topv = st.topbc[(size_t)b * nx + x];
-    fz.start(XIZ, q0, s, 3, 0.0, cf, t3, st.w_xiz);
-    fx.start(VX, q0, s, 1, 0.0, cf, t1, st.w_vx);
-    fv.start(VZ, q0, s, 1, 0.0, cf, t1, st.w_vz);
-    for (int j = 2; j <= nz - 3; j++) {
-        const size_t qj = q0 + (size_t)j * s;
-        const bool in = j + 3 < nz;
-        const double bz = in ? XIZ[qj + 3 * s] : 0.0;
-        const double bx = in ? VX[qj + 3 * s] : 0.0;
-        const double bv = in ? VZ[qj + 3 * s] : 0.0;
-        fz.step(j, qj, bz, cf, t3, st.w_xiz);
-        fx.step(j, qj, bx, cf, t1, st.w_vx);
-        fv.step(j, qj, bv, cf, t1, st.w_vz);
-    }
-    fz.finish(XIZ, q0, s, nz, 3, topv, cf, t3, st.w_xiz);
-    fx.finish(VX, q0, s, nz, 1, 0.0, cf, t1, st.w_vx);
-    fv.finish(VZ, q0, s, nz, 1, 0.0, cf, t1, st.w_vz);
-
-    // Loop B (descending): back substitution; z-PML on d xi_z/dz; divergence,
-    // pressure and density perturbation (physics2d.f90:957-967, 990-998, 1043)
-    const int nb = d.npmlbot, ntop = d.npmltop;
-    const size_t pz0 = (size_t)b * d.nzpml * nx + x;            // psivz
-    const size_t pz1 = (size_t)(d.S + b) * d.nzpml * nx + x;    // psip
-    double dz = fz.ud, dvx = fx.ud, dvz = fv.ud;
-    for (int j = nz - 1; j >= 0; j--) {
-        const size_t qj = q0 + (size_t)j * s;
-        if (j < nz - 1) {
-            dz = st.w_xiz[qj] - t3.gam[j + 1] * dz;
-            dvx = st.w_vx[qj] - t1.gam[j + 1] * dvx;
-            dvz = st.w_vz[qj] - t1.gam[j + 1] * dvz;
-        }
-        const double sj = bg.stretch[j];
-        double dxizdz = dz * sj;
-        st.w_vx[qj] = dvx * sj;
-        st.w_vz[qj] = dvz * sj;
-        if (j < nb || j >= nz - ntop) {
-            const int pj = j < nb ? j : nb + (j - (nz - ntop));
-            const size_t qp = pz0 + (size_t)pj * nx;
-            const size_t qb = (size_t)pj * nx + x;
-            const double psi = st.pz_tmp[qp];
-            const double rhs = bg.az[qb] * dxizdz + bg.bz[qb] * psi;
-            dxizdz = dxizdz + psi;
-            if (LAST) {
-                const double nbv = st.pz_base[qp] + ca.bnext * rhs;
-                st.pz_base[qp] = nbv;
-                st.pz_tmp[qp] = nbv;
-            } else {
-                st.pz_tmp[qp] = st.pz_base[qp] + ca.bnext * rhs;
-            }
-        }
-        const size_t gq = (size_t)j * nx + x;
-        const double div = st.dxixdx[qj] + dxizdz;
-        const double xiz = XIZ[qj];
-        st.p[qj] = -(bg.c2rho0[gq] * div) - bg.gp0[gq] * xiz;
-        st.w_rho[qj] = -(bg.rho0[gq] * div) - bg.gr0[gq] * xiz;
-    }
-
-    // Loop C: forward sweep of d p/dz with ibc 2; the bottom row value is the
-    // stale output row of the reference (physics2d.f90:1023), zero here
-    FwdSweep fp;
-    fp.start(st.p, q0, s, 2, 0.0, cf, t2, st.w_p);
-    for (int j = 2; j <= nz - 3; j++) {
-        const size_t qj = q0 + (size_t)j * s;
-        const double bp = (j + 3 < nz) ? st.p[qj + 3 * s] : 0.0;
-        fp.step(j, qj, bp, cf, t2, st.w_p);
-    }
-    fp.finish(st.p, q0, s, nz, 2, 0.0, cf, t2, st.w_p);
-
-    // Loop D (descending): grad_z p with the z-PML (:1025-1035), RHS v_z
-    // (:1044-1046), source (:1051-1061), flows (:1065-1079), zeroed rows and
-    // columns (:1083-1093), RK stage update (step.f90:59 / :107)
-    const bool snap = LAST && sink.snap && time % clk.steps == 0 && time / clk.steps < sink.nsnap;
-    double gp = fp.ud;
-    for (int j = nz - 1; j >= 0; j--) {
-        const size_t qj = q0 + (size_t)j * s;
-        const size_t gq = (size_t)j * nx + x;
-        if (j < nz - 1) gp = st.w_p[qj] - t2.gam[j + 1] * gp;
-        double gpz = gp * bg.stretch[j];
-        if (j < nb || j >= nz - ntop) {
-            const int pj = j < nb ? j : nb + (j - (nz - ntop));
-            const size_t qp = pz1 + (size_t)pj * nx;
-            const size_t qb = (size_t)pj * nx + x;
-            const double psi = st.pz_tmp[qp];
-            const double rhs = bg.az[qb] * gpz + bg.bz[qb] * psi;
-            gpz = gpz + psi;
-            if (LAST) {
-                const double nbv = st.pz_base[qp] + ca.bnext * rhs;
-                st.pz_base[qp] = nbv;
-                st.pz_tmp[qp] = nbv;
-            } else {
-                st.pz_tmp[qp] = st.pz_base[qp] + ca.bnext * rhs;
-            }
-        }
-        double rvz = -bg.rhoinv[gq] * (gpz + st.w_rho[qj] * bg.g[j]);
-        if (j == src.row && time <= src.limit) {
-            if (src.mode == 1) {
-                const double fs = (time >= 0 && time < src.nfwd) ? src.fwd[time] : 0.0;
-                rvz = rvz + src.vr[(size_t)b * nx + x] * fs * src.delta_width / bg.rho0_src[x];
-            } else if (src.mode == 2) {
-                const double f = lagrange_forcing(src, b, x, nx, time);
-                rvz = rvz + f * src.delta_width / bg.rho0_src[x];
-            }
-        }
-        const double v0x = bg.v0x[gq], v0z = bg.v0z[gq];
-        rvz = rvz - 2.0 * v0x * st.dxvz[qj];
-        rvz = rvz - 2.0 * v0z * st.w_vz[qj];
-        double rvx = -(2.0 * v0x * st.dxvx[qj]) - 2.0 * v0z * st.w_vx[qj];
-        double rxix = VX[qj], rxiz = VZ[qj];
-        if (j == 0 || x == 0 || x == nx - 1) { rvz = 0.0; rxix = 0.0; rxiz = 0.0; rvx = 0.0; }
-        st.rvx[qj] = rvx;
-        ca.TN[0][qj] = st.a[0][qj] + ca.bnext * rxix;
-        ca.TN[1][qj] = st.a[1][qj] + ca.bnext * rxiz;
-        ca.TN[3][qj] = st.a[3][qj] + ca.bnext * rvz;
-        if (snap)
-            sink.snap[((((size_t)b * sink.nsnap + time / clk.steps) * sink.nfield +
-                        (sink.adjoint ? 3 : 5)) * nz + j) * nx + x] = rvz;
-    }
-
-    if (LAST) {
-        // FILTER_VARS_2D on xi_x, xi_z, v_z (v_x is filtered by the next row pass)
-        const bool out = sink.vz_out && time % clk.steps == 0 && time / clk.steps < sink.nout;
-        const size_t sbase = snap ? (((size_t)b * sink.nsnap + time / clk.steps) * sink.nfield) : 0;
-        for (int j = 0; j < nz; j++) {
-            const size_t qj = q0 + (size_t)j * s;
-            const double fxix = filter_vars_2d_at(ca.TN[0], qj, j, x, nx, nz, d.nh);
-            const double fxiz = filter_vars_2d_at(ca.TN[1], qj, j, x, nx, nz, d.nh);
-            const double fvz = filter_vars_2d_at(ca.TN[3], qj, j, x, nx, nz, d.nh);
-            st.a[0][qj] = fxix;
-            st.a[1][qj] = fxiz;
-            st.a[3][qj] = fvz;
-            if (out && j == sink.orow)
-                sink.vz_out[((size_t)b * sink.nout + time / clk.steps) * nx + x] = fvz;
-            if (snap) {
-                const size_t o = (size_t)j * nx + x;
-                if (!sink.adjoint) {
-                    sink.snap[(sbase + 0) * nz * nx + o] = fxix;
-                    sink.snap[(sbase + 1) * nz * nx + o] = fxiz;
-                    sink.snap[(sbase + 3) * nz * nx + o] = fvz;
-                } else {
-                    sink.snap[(sbase + 1) * nz * nx + o] = fvz;
-                }
-            }
-        }
+        *(valid ? &st.dxixdx[rowq + x] : dump) = dx;
     }
 }
 
@@ -670,14 +438,14 @@ hipError_t launch_ddz_cols(const double *in, double *out, int ncols, int nz, int
 }
 
 hipError_t launch_ddx_rows(const double *in, double *out, int nrows, int nx, const double2 *tw,
-                           const double *eyekx, double stretchx, hipStream_t st)
+                           const double *dmul, double stretchx, hipStream_t st)
 {
 #define L(NXV)                                                                              \
     {                                                                                       \
         constexpr int TEAMS = 64 / FftShape<NXV>::N1;                                      \
         const int npairs = (nrows + 1) / 2;                                                 \
         dim3 grid((npairs + TEAMS - 1) / TEAMS);                                            \
-        hipLaunchKernelGGL(k_ddx_rows<NXV>, grid, dim3(64), 0, st, in, out, nrows, tw, eyekx, \
+        hipLaunchKernelGGL(k_ddx_rows<NXV>, grid, dim3(64), 0, st, in, out, nrows, tw, dmul, \
                            stretchx);                                                       \
     }
     SPARC_DISPATCH_NX(nx, L)
@@ -698,18 +466,6 @@ hipError_t launch_row(int mode, const Dims &d, const Bg &bg, const State &s, con
     }
     SPARC_DISPATCH_NX(d.nx, L)
 #undef L
-    return hipGetLastError();
-}
-
-hipError_t launch_col(const Dims &d, const Bg &bg, const Tri &t1, const Tri &t2, const Tri &t3,
-                      const DbyCoef &cf, const State &s, const ColArgs &ca, const Src &src,
-                      const Clock &c, const Sink &sk, hipStream_t st)
-{
-    dim3 grid((d.nx + 63) / 64, d.S);
-    if (ca.last)
-        hipLaunchKernelGGL(k_col<1>, grid, dim3(64), 0, st, d, bg, t1, t2, t3, cf, s, ca, src, c, sk);
-    else
-        hipLaunchKernelGGL(k_col<0>, grid, dim3(64), 0, st, d, bg, t1, t2, t3, cf, s, ca, src, c, sk);
     return hipGetLastError();
 }
 

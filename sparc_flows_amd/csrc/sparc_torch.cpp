@@ -169,6 +169,13 @@ struct Solver : torch::CustomClassHolder {
     at::Tensor ddz(at::Tensor in, int64_t ibc, at::Tensor preset) { return ddz_impl(in, ibc, preset, false); }
     at::Tensor ddz_kern(at::Tensor in, int64_t ibc) { return ddz_impl(in, ibc, at::Tensor(), true); }
 
+    std::vector<double> profile_kernels(int64_t reps)
+    {
+        std::vector<double> ms(4, 0.0);
+        check(sparc_profile_kernels(h, (int)reps, ms.data()), h, "sparc_profile_kernels");
+        return ms;
+    }
+
     // returns kc, kpsi, kvx, kvz, hess, each [nsrc][nz][nx]
     std::vector<at::Tensor> kernels(at::Tensor fwd, at::Tensor adj)
     {
@@ -211,5 +218,6 @@ TORCH_LIBRARY(sparc, m)
         .def("ddx", &Solver::ddx)
         .def("ddz", &Solver::ddz)
         .def("ddz_kern", &Solver::ddz_kern)
-        .def("kernels", &Solver::kernels);
+        .def("kernels", &Solver::kernels)
+        .def("profile_kernels", &Solver::profile_kernels);
 }

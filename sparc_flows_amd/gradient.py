@@ -1,0 +1,75 @@
+"""Forward + adjoint + kernel pipeline: the in-HBM counterpart of full.py.
+
+full.py:24-61 runs three `sparc` processes per source -- forward ('Spectral'
+instruction), adjoint ('Adjoint') and kernel (no instruction) -- exchanging
+vz traces, adjoint sources and ~1 GB of snapshot FITS files on disk.  Here one
+:class:`GradientPipeline` keeps a forward and an adjoint batch on the GPU and
+passes snapshots between them in HBM.
+
+The adjoint source of the reference is ADJOINT_SOURCE_FILT (driver.f90:
+1148-1754: ridge filters, travel-time windows, time reversal).  Until that is
+built, :func:`waveform_adjoint_source` supplies the windowed, time-reversed vz
+residual of a waveform misfit over the same receiver band (5-60 Mm from the
+source, params.0 of the reference) and time window.
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from .params import SparcParams
+from .solver import Sparc
+
+
+def receiver_mask(p: SparcParams, xsrc, dmin_mm: float = 5.0, dmax_mm: float = 60.0) -> np.ndarray:
+    """[nsrc][nx] 1 where the horizontal distance to the source is within [dmin, dmax] Mm."""
+    x = (np.arange(p.nx) / p.nx - 0.5) * p.xlength * 1e-8
+    xs = np.atleast_1d(np.asarray(xsrc, dtype=np.float64))
+    d = np.abs(x[None, :] - xs[:, None])
+    return ((d >= dmin_mm) & (d <= dmax_mm)).astype(np.float64)
+
+
+def waveform_adjoint_source(vz: torch.Tensor, vz_obs: torch.Tensor | None, mask: np.ndarray,
+                            window_s: float, cadence_s: float):
+    """Misfit 0.5*sum(w (vz - obs)^2) and its time-reversed adjoint source.
+
+    vz: [nsrc][nt][nx].  The window keeps the first window_s seconds of the
+    record, tapered with a cosine over its last 10 %.
+    Returns (misfit [nsrc], vr [nsrc][nt][nx]).
+    """
+    nsrc, nt, nx = vz.shape
+    t = torch.arange(nt, dtype=torch.float64, device=vz.device) * cadence_s
+    w = torch.ones(nt, dtype=torch.float64, device=vz.device)
+    taper = 0.1 * window_s
+    w = torch.where(t > window_s, torch.zeros_like(w), w)
+    ramp = (t > window_s - taper) & (t <= window_s)
+    w = torch.where(ramp, 0.5 * (1 + torch.cos(np.pi * (t - (window_s - taper)) / taper)), w)
+    m = torch.as_tensor(mask, dtype=torch.float64, device=vz.device)[:, None, :]
+    res = vz if vz_obs is None else vz - vz_obs
+    wres = res * w[None, :, None] * m
+    misfit = 0.5 * (wres * res).sum(dim=(1, 2))
+    vr = torch.flip(wres, dims=[1]).contiguous()
+    return misfit, vr
+
+
+class GradientPipeline:
+    """Forward + misfit + adjoint + kernels for a batch of sources."""
+
+    def __init__(self, params: SparcParams, model: np.ndarray, nsrc: int = 1, c2d=None, psi=None,
+                 window_s: float = 1800.0):
+        self.p = params
+        self.nsrc = nsrc
+        self.window_s = window_s
+        self.fwd = Sparc(params, model, nsrc=nsrc, adjoint=False, c2d=c2d, psi=psi)
+        self.adj = Sparc(params, model, nsrc=nsrc, adjoint=True, c2d=c2d, psi=psi)
+
+    def run(self, xsrc, maxtime: int, vz_obs: torch.Tensor | None = None) -> dict:
+        vz, fsn = self.fwd.forward(xsrc, maxtime, snapshots=True)
+        mask = receiver_mask(self.p, xsrc)
+        cad = self.fwd.steps * self.p.timestep
+        misfit, vr = waveform_adjoint_source(vz, vz_obs, mask, self.window_s, cad)
+        _, asn = self.adj.adjoint(vr, maxtime, snapshots=True)
+        K = self.adj.kernels(fsn, asn)
+        K["vz"] = vz
+        K["misfit"] = misfit
+        return K

@@ -158,6 +158,11 @@ class Sparc:
         f = torch.as_tensor(f, dtype=torch.float64, device=self.device).contiguous()
         return self._stream().ddz_kern(f, int(ibc))
 
+    def profile_kernels(self, reps: int = 20) -> dict:
+        """Average ms per launch of each pass (HIP events on this stream); resets the state."""
+        ms = self._stream().profile_kernels(int(reps))
+        return dict(zip(("col", "col_last", "row", "row_first"), ms))
+
     def kernels(self, fwd_snaps: torch.Tensor, adj_snaps: torch.Tensor) -> dict:
         """PRODUCE_KERNELS from in-HBM snapshots; arrays [nsrc][nz][nx]."""
         o = self._stream().kernels(fwd_snaps.contiguous(), adj_snaps.contiguous())
