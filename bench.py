@@ -130,12 +130,13 @@ def main():
         del bpipe
 
     # roofline of the dominant kernel: HIP events on the solver's stream
-    ms = pipe.fwd.profile_kernels(20)
+    ms = pipe.fwd.profile_kernels(200)
     npts = p.nx * p.nz
     C = 2  # x-varying coefficient arrays read per point (v0x, v0z)
-    bytes_per_pt = {"col": (14 + C) * 8, "col_last": (14 + C) * 8, "row": 9 * 8, "row_first": 7 * 8}
+    bytes_per_pt = {"col": (14 + C) * 8, "col_last": (14 + C) * 8, "row": 9 * 8, "row_first": 7 * 8,
+                    "row_last": 4 * 8}
     per_step_ms = {"col": 4 * ms["col"], "col_last": ms["col_last"], "row": 4 * ms["row"],
-                   "row_first": ms["row_first"]}
+                   "row_first": ms["row_first"], "row_last": ms["row_last"]}
     dom = max(per_step_ms, key=per_step_ms.get)
     achieved = bytes_per_pt[dom] * npts / (ms[dom] * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

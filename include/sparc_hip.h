@@ -80,6 +80,12 @@ int sparc_get_background(const sparc_handle *h, const char *name, double *host_o
 /* Replace the 2D sound speed and stream function and redo the background
  * set-up (model_c / model_psi of a new iterate, driver.f90:94-126). */
 int sparc_set_model(sparc_handle *h, const double *c2d_host, const double *psi_host);
+/* Set the background flow directly instead of from psi: host [nz][nx]
+ * v0_x, v0_z in m/s, as the compute_data branch reads true_vx.fits and
+ * true_vz.fits and scales them by 1e2/dimc (driver.f90:65-73).  An adjoint
+ * handle reverses the flow (driver.f90:144-147).  Both NULL restores the
+ * psi flow of the last sparc_set_model/sparc_create. */
+int sparc_set_flow(sparc_handle *h, const double *vx_host, const double *vz_host);
 
 /* Forward sources: DETERMINE_STATUS forward branch (driver.f90:521-571) and
  * FORWARD_SOURCE (driver.f90:1120-1145) for x positions xsrc (Mm, one per
@@ -133,10 +139,12 @@ int sparc_kernels(sparc_handle *h, int nt, const double *fwd_snaps_dev,
 
 /* Instrumentation (no reference counterpart; the reference prints per-step
  * MPI_WTIME, driver.f90:355-381): average device time in ms of one launch of
- * each pass of a step, timed with HIP events on the handle's stream over reps
- * back-to-back launches: ms_out[0] column pass, [1] last-stage column pass
- * (with the z filter), [2] row pass, [3] first row pass (with the v_x filter).
- * Overwrites the state, then resets it. */
+ * each pass, timed with HIP events recorded around every launch of reps real
+ * time steps (t = 0..reps-1 from a zero state, sources as set) on the
+ * handle's stream: ms_out[0] column pass, [1] last-stage column pass (with
+ * the z filter), [2] row pass, [3] first row pass (with the pending v_x
+ * filter), [4] last-stage row pass.  ms_out holds 5 values.  Resets the state
+ * before and after. */
 int sparc_profile_kernels(sparc_handle *h, int reps, double *ms_out);
 
 #ifdef __cplusplus

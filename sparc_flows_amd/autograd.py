@@ -1,0 +1,180 @@
+"""Differentiable 2D solve: (psi, c) -> vz traces as a torch.autograd.Function.
+
+Forward: the background is rebuilt from the stream function psi (model_psi,
+driver.f90:94-126) and the 2D sound speed c (model_c, cm/s), and every source
+of the batch is run to maxtime with its snapshots kept in HBM
+(TIMESTEPPING, driver.f90:337-419; WRITE_OUT_PARTIAL_STATE,
+all_modules.f90:750-784).
+
+Backward: the reference's adjoint recipe.  The gradient of the loss with
+respect to the vz traces, time reversed, becomes the adjoint source at the
+observation row (as ADJOINT_SOURCE writes adj(i, nt-j+1) from sample j and
+scales it by nx, driver.f90:805-812); the adjoint batch runs with the flow
+reversed (driver.f90:144-147) and PRODUCE_KERNELS (kernels.f90:13-612)
+correlates the two.  The kernels come in the reference's physical units
+(kernels.f90:495-580); :func:`kernel_to_gradient` turns them into the
+gradient with respect to the grid values of psi and c.
+
+Why the recipe gives the gradient.  Write the solver as u' = M u + f with
+u = (xi, v) and observations d_k = v_z(x, z_o, t_k).  In the inner product
+<a, b> = sum rho0 a.b dx dz the wave operator is symmetric and the flow
+operator -2 v0.grad is antisymmetric, so the Lagrange multiplier of the v
+equation at time t is -v_a(T - t), where v_a solves the same equations with
+v0 -> -v0, forced at row z_o by g(x, T - t) / (rho0 dx dz) and g is the time
+density of dL/dd.  The injection term vr * delta_width / rho0
+(physics2d.f90:1052-1054) has delta_width = 1/dz, so the adjoint source is
+vr = g / dx, with g = (dL/dd_k) / (steps * deltat) between samples.  Then
+
+    dL/dv0x = -2 rho0 dx dz dT sum_k sum_j d_x v_j v_a,j      (Kx of kernels.f90:340-355)
+    dL/d(rho0 c^2) = -dx dz dT sum_k div xi . div v_a        (Kc of kernels.f90:303-306)
+
+with dT = steps * deltat, and psi enters through rho0 v0 = curl(rho0 c psi / (diml/1e8) y).
+These are the continuous-adjoint identities; the discrete solver matches them
+up to its boundary closures, the z filter and the sampling of the time
+integral at the output cadence, so finite differences agree to about a
+percent, not to rounding.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import torch
+
+from .params import SparcParams
+from .solver import Sparc
+
+RSUN = 6.95989467700e10          # diml, initialize.f90:37
+DIMC = 1e6                        # initialize.f90:753-776
+
+
+def _f32(v: float) -> float:
+    return float(np.float32(v))
+
+
+def grid_cells(p: SparcParams, model: np.ndarray):
+    """Non-dimensional cell sizes: dx (scalar) and dz per row ([nz])."""
+    z = np.asarray(model, dtype=np.float64)[: p.nz, 0]
+    dx = p.xlength / RSUN / p.nx
+    dz = np.gradient(z)
+    return dx, dz
+
+
+def kernel_to_gradient(p: SparcParams, model: np.ndarray, K: dict, psi: np.ndarray | torch.Tensor,
+                       c: np.ndarray | torch.Tensor) -> tuple[torch.Tensor, torch.Tensor]:
+    """Gradient of the loss with respect to psi (model_psi units) and c (cm/s)
+    on the grid, from the reference-scaled kernels of one adjoint run.
+
+    K["kpsi"], K["kc"]: [nsrc][nz][nx] as PRODUCE_KERNELS writes them
+    (kernels.f90:510-513, 579); sources are summed.
+    """
+    dev = K["kpsi"].device
+    dx, dz = grid_cells(p, model)
+    dz = torch.as_tensor(dz, dtype=torch.float64, device=dev)[:, None]
+    e8, e5 = _f32(1e-8), _f32(1e-5)
+    dimc32 = _f32(1e6)
+    dimen = 1.0 / (RSUN * e8 * p.xlength * e8 * dimc32 * e5)        # kernels.f90:503
+    dimen_c = 1.0 / (RSUN * e8 * p.xlength * e8)                     # kernels.f90:563
+    t_scale = DIMC / RSUN                                            # deltat / timestep
+    s = 1e8 / RSUN                                                   # psi' = rho0 c psi s
+    psi = torch.as_tensor(psi, dtype=torch.float64, device=dev)
+    c = torch.as_tensor(c, dtype=torch.float64, device=dev)
+    c_nd = c / DIMC
+    # Kpsi = 2 dimen^2 rho0 c steps timestep curl(K); dL/dpsi = -2 dx dz dT rho0 c s curl(K)
+    gpsi = -K["kpsi"].sum(0) * (dx * dz * t_scale * s / dimen ** 2)
+    # Kc = -2 rho0 c^2 steps timestep dimen_c Kc_raw; dL/dc = -2 rho0 c dx dz dT Kc_raw / dimc
+    gc = K["kc"].sum(0) * (dx * dz * t_scale / (dimen_c * c_nd * DIMC))
+    # c also scales the flow through psi' = rho0 c psi s: dL/dc += dL/dpsi * psi / c
+    gc = gc + gpsi * psi / c
+    return gpsi, gc
+
+
+class _Solve(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, psi, c, pipe, xsrc, maxtime):
+        pipe.set_model(c, psi)
+        vz, fsn = pipe.fwd.forward(xsrc, maxtime, snapshots=True)
+        ctx.pipe, ctx.maxtime = pipe, maxtime
+        ctx.fsn = fsn
+        ctx.save_for_backward(psi, c)
+        return vz
+
+    @staticmethod
+    def backward(ctx, gvz):
+        psi, c = ctx.saved_tensors
+        pipe = ctx.pipe
+        vr = pipe.adjoint_source_from_grad(gvz)
+        _, asn = pipe.adj.adjoint(vr, ctx.maxtime, snapshots=True)
+        K = pipe.adj.kernels(ctx.fsn, asn)
+        ctx.fsn = None
+        gpsi, gc = kernel_to_gradient(pipe.p, pipe.model, K, psi, c)
+        return gpsi.to(psi.device), gc.to(c.device), None, None, None
+
+
+class DifferentiableSolver:
+    """Forward and adjoint batches sharing one background, for autograd.
+
+    ``solve(psi, c)`` returns vz [nsrc][nout][nx] (non-dimensional, the
+    vz_cc records) and is differentiable with respect to psi [nz][nx]
+    (model_psi) and c [nz][nx] (cm/s).
+    """
+
+    def __init__(self, params: SparcParams, model: np.ndarray, xsrc, maxtime: int):
+        if params.steps != params.cadforcing_step:
+            raise ValueError("the adjoint source is sampled at the output cadence: "
+                             "outputcad and cadforcing must give the same step count")
+        self.p = params
+        self.model = np.asarray(model, dtype=np.float64)
+        self.xsrc = list(np.atleast_1d(np.asarray(xsrc, dtype=np.float64)))
+        self.maxtime = int(maxtime)
+        n = len(self.xsrc)
+        self.fwd = Sparc(params, model, nsrc=n, adjoint=False)
+        self.adj = Sparc(params, model, nsrc=n, adjoint=True)
+        self.dx, _ = grid_cells(params, model)
+        self.dT = params.steps * params.timestep * DIMC / RSUN
+
+    def set_model(self, c, psi) -> None:
+        cn = c.detach().cpu().numpy() if torch.is_tensor(c) else np.asarray(c)
+        pn = psi.detach().cpu().numpy() if torch.is_tensor(psi) else np.asarray(psi)
+        self.fwd.set_model(cn, pn)
+        self.adj.set_model(cn, pn)
+
+    def adjoint_source_from_grad(self, gvz: torch.Tensor) -> torch.Tensor:
+        """vr [nsrc][nt][nx]: time-reversed dL/dvz as a force density."""
+        g = torch.flip(gvz.to(torch.float64), dims=[1])
+        return (g / (self.dT * self.dx)).contiguous()
+
+    def solve(self, psi: torch.Tensor, c: torch.Tensor) -> torch.Tensor:
+        return _Solve.apply(psi, c, self, self.xsrc, self.maxtime)
+
+
+def window_weights(nt: int, cadence_s: float, window_s: float, device=None) -> torch.Tensor:
+    """1 over the first window_s seconds, cosine taper over its last 10 %."""
+    t = torch.arange(nt, dtype=torch.float64, device=device) * cadence_s
+    w = torch.ones(nt, dtype=torch.float64, device=device)
+    taper = 0.1 * window_s
+    w = torch.where(t > window_s, torch.zeros_like(w), w)
+    ramp = (t > window_s - taper) & (t <= window_s)
+    return torch.where(ramp, 0.5 * (1 + torch.cos(math.pi * (t - (window_s - taper)) / taper)), w)
+
+
+def travel_times(vz: torch.Tensor, vz_ref: torch.Tensor, mask: torch.Tensor, w: torch.Tensor,
+                 cadence_s: float, floor: float = 1e-2) -> tuple[torch.Tensor, torch.Tensor]:
+    """Linearised cross-correlation travel times (Gizon & Birch 2002, eq. 13):
+    tau = -int W dref/dt (vz - vref) dt / int W (dref/dt)^2 dt, so that
+    vz(t) = vref(t - tau) gives tau > 0.  vz, vz_ref: [nsrc][nt][nx];
+    mask: [nsrc][nx] receivers; w: [nt] window.  Receivers whose windowed
+    reference energy is below `floor` times the source's largest are dropped
+    (tau is ill-conditioned where the wave has not arrived).
+    Returns (tau [nsrc][nx], the receiver mask used)."""
+    dref = torch.gradient(vz_ref, spacing=cadence_s, dim=1)[0]
+    num = (w[None, :, None] * dref * (vz - vz_ref)).sum(1)
+    den = (w[None, :, None] * dref * dref).sum(1)
+    m = mask * (den * mask > floor * (den * mask).amax(dim=1, keepdim=True))
+    return -num / torch.where(m > 0, den, torch.ones_like(den)) * m, m
+
+
+def travel_time_misfit(vz, vz_ref, tau_obs, mask, w, cadence_s, floor: float = 1e-2):
+    """0.5 * sum over sources and receivers of (tau - tau_obs)^2."""
+    tau, m = travel_times(vz, vz_ref, mask, w, cadence_s, floor)
+    return 0.5 * ((tau - tau_obs) * m).pow(2).sum()

@@ -69,6 +69,18 @@ def stable_timestep(bg: np.ndarray, courant: float = 0.25) -> float:
     return courant * float(np.min(np.diff(zc))) / float(np.max(bg[:, 1]))
 
 
+def cadence_aligned_timestep(dt_max: float, cadence: float = 30.0) -> float:
+    """Largest dt <= dt_max for which floor(cadence/dt) * dt equals cadence to
+    rounding, so that the output samples (every floor(outputcad/dt) steps,
+    initialize.f90:998-1001) and the adjoint-source nodes (multiples of
+    cadforcing, physics.f90:684-697) fall on the same times."""
+    n = int(np.ceil(cadence / dt_max - 1e-12))
+    dt = cadence / n
+    while int(np.floor(cadence / dt)) != n:     # keep floor() at n despite rounding
+        dt = np.nextafter(dt, 0.0)
+    return float(dt)
+
+
 def grid_x_mm(nx: int, xlength_cm: float) -> np.ndarray:
     """Horizontal positions (Mm) relative to the box centre, x = (i-1)/nx (initialize.f90:194-198)."""
     return (np.arange(nx) / nx - 0.5) * xlength_cm * 1e-8

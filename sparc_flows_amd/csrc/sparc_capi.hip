@@ -52,6 +52,7 @@ struct sparc_handle {
     double xlength = 0, timestep = 0, cadforcing = 0, outputcad = 0, obsheight = 0, excitdep = 0;
     double nupeak = 0, nuwidth = 0;
     std::vector<double> model, c2d, psi;
+    std::vector<double> flowx, flowz;   // explicit flow (non-dimensional), replaces psi when set
     Dims d{};
     std::string err;
     hipStream_t stream = nullptr;
@@ -78,6 +79,11 @@ struct sparc_handle {
     Src src{};
     int maxtime = -1, forcing_length = 0, num_steps = 0;
     bool have_src = false;
+    // launch timing (sparc_profile_kernels): events recorded around each pass
+    struct Mark { hipEvent_t a, b; int kind; };
+    std::vector<hipEvent_t> evpool;
+    std::vector<Mark> marks;
+    bool profiling = false;
     bool vx_pending = false;
 
     // kernel-mode background (kernel_back, initialize.f90:937-979)
@@ -441,7 +447,13 @@ static int setup_background(sparc_handle *h)
     // flow from psi (driver.f90:94-126), reversed for the adjoint (:144-147)
     h->v0x.assign(n2, 0.0);
     h->v0z.assign(n2, 0.0);
-    if (!h->psi.empty()) {
+    if (!h->flowx.empty()) {
+        // compute_data: flows given directly (driver.f90:65-73)
+        for (size_t q = 0; q < n2; q++) {
+            h->v0x[q] = h->adjoint ? -h->flowx[q] : h->flowx[q];
+            h->v0z[q] = h->adjoint ? -h->flowz[q] : h->flowz[q];
+        }
+    } else if (!h->psi.empty()) {
         std::vector<double> pv(n2), dz, dx;
         for (size_t q = 0; q < n2; q++)
             pv[q] = h->rho0[q] * std::sqrt(h->c2[q]) * (h->psi[q] / (diml / 1.0e8));
@@ -691,6 +703,23 @@ static int zero_state(sparc_handle *h)
 
 // ---------------------------------------------------------------------------
 // One step: R1 C1 R C2 R C3 R C4 R C5 R5 (see sparc_kernels.hip).
+// Runs launch(); while profiling, brackets it with events of the given pass
+// kind (0 column, 1 last column, 2 row, 3 first row, 4 last row).
+template <typename F>
+static int timed_launch(sparc_handle *h, int kind, F launch)
+{
+    if (!h->profiling) return launch();
+    const size_t k = 2 * h->marks.size();
+    if (k + 2 > h->evpool.size()) return fail(h, SPARC_E_STATE, "profiling event pool exhausted");
+    sparc_handle::Mark m{h->evpool[k], h->evpool[k + 1], kind};
+    HCHECK(h, hipEventRecord(m.a, h->stream));
+    const int r = launch();
+    if (r) return r;
+    HCHECK(h, hipEventRecord(m.b, h->stream));
+    h->marks.push_back(m);
+    return SPARC_OK;
+}
+
 static int enqueue_step(sparc_handle *h, int toff, bool filt_vx, const Sink &sk)
 {
     const Dims &d = h->d;
@@ -705,7 +734,11 @@ static int enqueue_step(sparc_handle *h, int toff, bool filt_vx, const Sink &sk)
         ra.bx1 = b[1];
         ra.last1 = 0;
         ra.filt_vx = filt_vx ? 1 : 0;
-        HCHECK(h, launch_row(0, d, h->bg, s, ra, clk, sk, h->stream));
+        const int r = timed_launch(h, 3, [&] {
+            HCHECK(h, launch_row(0, d, h->bg, s, ra, clk, sk, h->stream));
+            return SPARC_OK;
+        });
+        if (r) return r;
     }
     for (int stg = 1; stg <= 5; stg++) {
         ColArgs ca{};
@@ -716,8 +749,12 @@ static int enqueue_step(sparc_handle *h, int toff, bool filt_vx, const Sink &sk)
         }
         ca.last = stg == 5;
         ca.bnext = stg == 5 ? h->deltat : b[stg];
-        HCHECK(h, launch_colseg(h->seg_tc, h->seg_ms, d, h->bg, h->cf, h->seg, s, ca, h->src, clk, sk,
-                                h->stream));
+        int r = timed_launch(h, stg == 5 ? 1 : 0, [&] {
+            HCHECK(h, launch_colseg(h->seg_tc, h->seg_ms, d, h->bg, h->cf, h->seg, s, ca, h->src, clk, sk,
+                                    h->stream));
+            return SPARC_OK;
+        });
+        if (r) return r;
         RowArgs ra{};
         if (stg < 5) {
             ra.xi_x = s.t[0];
@@ -727,13 +764,20 @@ static int enqueue_step(sparc_handle *h, int toff, bool filt_vx, const Sink &sk)
             ra.last2 = 0;
             ra.last1 = (stg + 1 == 5);
             ra.bx1 = ra.last1 ? h->deltat : b[stg + 1];
-            HCHECK(h, launch_row(1, d, h->bg, s, ra, clk, sk, h->stream));
+            r = timed_launch(h, 2, [&] {
+                HCHECK(h, launch_row(1, d, h->bg, s, ra, clk, sk, h->stream));
+                return SPARC_OK;
+            });
         } else {
             ra.v_x_out = s.vx_unf;
             ra.bx2 = h->deltat;
             ra.last2 = 1;
-            HCHECK(h, launch_row(2, d, h->bg, s, ra, clk, sk, h->stream));
+            r = timed_launch(h, 4, [&] {
+                HCHECK(h, launch_row(2, d, h->bg, s, ra, clk, sk, h->stream));
+                return SPARC_OK;
+            });
         }
+        if (r) return r;
     }
     return SPARC_OK;
 }
@@ -920,6 +964,34 @@ int sparc_set_model(sparc_handle *h, const double *c2d, const double *psi)
     const size_t n2 = (size_t)h->nx * h->nz;
     if (c2d) h->c2d.assign(c2d, c2d + n2); else h->c2d.clear();
     if (psi) h->psi.assign(psi, psi + n2); else h->psi.clear();
+    h->flowx.clear();
+    h->flowz.clear();
+    h->bg_mem.release();
+    int r = setup_grid(h);
+    if (!r) r = setup_background(h);
+    if (!r) r = upload_background(h);
+    return r;
+}
+
+int sparc_set_flow(sparc_handle *h, const double *vx, const double *vz)
+{
+    if (!h) return SPARC_E_ARG;
+    if (!vx != !vz) return fail(h, SPARC_E_ARG, "sparc_set_flow needs both components or neither");
+    HCHECK(h, hipStreamSynchronize(h->stream));
+    const size_t n2 = (size_t)h->nx * h->nz;
+    h->flowx.clear();
+    h->flowz.clear();
+    if (vx) {
+        const double dimc = 1e6;
+        h->flowx.resize(n2);
+        h->flowz.resize(n2);
+        for (size_t q = 0; q < n2; q++) {
+            if (!std::isfinite(vx[q]) || !std::isfinite(vz[q]))
+                return fail(h, SPARC_E_ARG, "sparc_set_flow: non-finite flow value");
+            h->flowx[q] = vx[q] * 1e2 / dimc;       // m/s -> cm/s -> units of dimc
+            h->flowz[q] = vz[q] * 1e2 / dimc;
+        }
+    }
     h->bg_mem.release();
     int r = setup_grid(h);
     if (!r) r = setup_background(h);
@@ -1261,43 +1333,36 @@ int sparc_profile_kernels(sparc_handle *h, int reps, double *ms_out)
 {
     if (!h || reps < 1 || !ms_out) return SPARC_E_ARG;
     if (!h->have_src) return fail(h, SPARC_E_STATE, "sources not set");
+    int r = sparc_reset(h);
+    if (r) return r;
+    const size_t nev = (size_t)2 * 11 * (size_t)reps;
+    for (size_t i = 0; i < nev; i++) {
+        hipEvent_t e;
+        HCHECK(h, hipEventCreate(&e));
+        h->evpool.push_back(e);
+    }
+    h->marks.clear();
+    h->profiling = true;
     Sink sk = empty_sink(h);
-    Clock clk{h->tctr, 0, h->steps};
-    State &s = h->st;
-    hipEvent_t e0, e1;
-    HCHECK(h, hipEventCreate(&e0));
-    HCHECK(h, hipEventCreate(&e1));
-    auto timed = [&](int which, double *ms) -> int {
-        for (int pass = 0; pass < 2; pass++) {   // pass 0 warms up
-            HCHECK(h, hipEventRecord(e0, h->stream));
-            for (int r = 0; r < reps; r++) {
-                if (which == 0 || which == 1) {
-                    ColArgs ca{};
-                    for (int f = 0; f < 4; f++) { ca.T[f] = s.t[f]; ca.TN[f] = s.t[f]; }
-                    ca.last = which == 1;
-                    ca.bnext = which == 1 ? h->deltat : h->betas[2];
-                    HCHECK(h, launch_colseg(h->seg_tc, h->seg_ms, h->d, h->bg, h->cf, h->seg, s, ca, h->src,
-                                            clk, sk, h->stream));
-                } else {
-                    RowArgs ra{};
-                    ra.xi_x = s.t[0]; ra.v_z = s.t[3]; ra.v_x_out = s.t[2];
-                    ra.bx2 = h->betas[2]; ra.bx1 = h->betas[3];
-                    ra.filt_vx = which == 3;
-                    HCHECK(h, launch_row(which == 2 ? 1 : 0, h->d, h->bg, s, ra, clk, sk, h->stream));
-                }
-            }
-            HCHECK(h, hipEventRecord(e1, h->stream));
-            HCHECK(h, hipEventSynchronize(e1));
+    for (int t = 0; t < reps && r == SPARC_OK; t++) {
+        r = launch_set_int(h->tctr, t, h->stream) == hipSuccess ? SPARC_OK : SPARC_E_HIP;
+        if (!r) r = enqueue_step(h, 0, t > 0, sk);
+    }
+    h->profiling = false;
+    if (!r) r = hipStreamSynchronize(h->stream) == hipSuccess ? SPARC_OK : SPARC_E_HIP;
+    double sum[5] = {0, 0, 0, 0, 0};
+    int cnt[5] = {0, 0, 0, 0, 0};
+    for (const auto &m : h->marks) {
+        float ms = 0.f;
+        if (!r && hipEventElapsedTime(&ms, m.a, m.b) == hipSuccess) {
+            sum[m.kind] += ms;
+            cnt[m.kind]++;
         }
-        float t = 0.f;
-        HCHECK(h, hipEventElapsedTime(&t, e0, e1));
-        *ms = (double)t / reps;
-        return SPARC_OK;
-    };
-    int r = SPARC_OK;
-    for (int w = 0; w < 4 && r == SPARC_OK; w++) r = timed(w, &ms_out[w]);
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
+    }
+    for (int k = 0; k < 5; k++) ms_out[k] = cnt[k] ? sum[k] / cnt[k] : 0.0;
+    h->marks.clear();
+    for (hipEvent_t e : h->evpool) (void)hipEventDestroy(e);
+    h->evpool.clear();
     if (r) return r;
     return sparc_reset(h);
 }

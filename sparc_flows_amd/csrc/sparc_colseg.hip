@@ -78,17 +78,22 @@ __device__ __forceinline__ void seg_correct(double (&Y)[MS], int s, int P, int h
                                             const double *__restrict__ W)
 {
     const int n2P = 2 * P;
-    double Lp = 0.0, Fn = 0.0;
-    if (s > 0) {
-        const double *row = mvt + (2 * s - 1) * n2P;
-        const int q0 = max(0, 2 * (s - 1 - hw)), q1 = min(n2P, 2 * (s - 1 + hw) + 2);
-        for (int q = q0; q < q1; q++) Lp += row[q] * y[q];
-    }
-    if (s < P - 1) {
-        const double *row = mvt + (2 * s + 2) * n2P;
-        const int q0 = max(0, 2 * (s + 1 - hw)), q1 = min(n2P, 2 * (s + 1 + hw) + 2);
-        for (int q = q0; q < q1; q++) Fn += row[q] * y[q];
-    }
+    // interface values L_{s-1} (row 2s-1 of Minv) and F_{s+1} (row 2s+2),
+    // each a dot product over the window of segments that can contribute
+    auto dot = [&](int k, int centre) -> double {
+        const double *row = mvt + k * n2P;
+        const int q0 = max(0, 2 * (centre - hw)), q1 = min(n2P, 2 * (centre + hw) + 2);
+        double acc0 = 0.0, acc1 = 0.0;
+        int q = q0;
+        for (; q + 1 < q1; q += 2) {
+            acc0 += row[q] * y[q];
+            acc1 += row[q + 1] * y[q + 1];
+        }
+        if (q < q1) acc0 += row[q] * y[q];
+        return acc0 + acc1;
+    };
+    const double Lp = s > 0 ? dot(2 * s - 1, s - 1) : 0.0;
+    const double Fn = s < P - 1 ? dot(2 * s + 2, s + 1) : 0.0;
 #pragma unroll
     for (int i = 0; i < MS; i++) Y[i] = Y[i] - Lp * V[i] - Fn * W[i];
 }
@@ -264,33 +269,46 @@ __global__ __launch_bounds__(256) void k_colseg(Dims d, Bg bg, DbyCoef cf, SegTa
             sink.snap[((slot + (sink.adjoint ? 3 : 5)) * nz + j) * nx + x] = rvz;
     }
     if (LAST) {
-        // FILTER_VARS_2D on xi_x, xi_z, v_z of the column; neighbour rows were
-        // written by the other segments of this block
+        // FILTER_VARS_2D (physics2d.f90:509-556) on xi_x, xi_z, v_z of the
+        // column: each field's rows a-5 .. a+MS+4 are read once (neighbour
+        // segments' rows were written before the barrier) and filtered in
+        // registers.
         __syncthreads();
         const bool out = sink.vz_out && time % clk.steps == 0 && time / clk.steps < sink.nout;
+        const int nh = d.nh;
+        const bool xin = x >= nh - 1 && x <= nx - nh;
 #pragma unroll
-        for (int i = 0; i < MS; i++) {
-            const int j = min(a + i, nz - 1);
-            const size_t qj = rowq(i);
-            const double fxix = filter_vars_2d_at(ca.TN[0], qj, j, x, nx, nz, d.nh);
-            const double fxiz = filter_vars_2d_at(ca.TN[1], qj, j, x, nx, nz, d.nh);
-            const double fvz = filter_vars_2d_at(ca.TN[3], qj, j, x, nx, nz, d.nh);
-            const bool v = valid(i);
-            *(v ? &st.a[0][qj] : dump) = fxix;
-            *(v ? &st.a[1][qj] : dump) = fxiz;
-            *(v ? &st.a[3][qj] : dump) = fvz;
-            if (v && out && j == sink.orow)
-                sink.vz_out[((size_t)b * sink.nout + time / clk.steps) * nx + x] = fvz;
-            if (v && snap) {
-                const size_t o = (size_t)j * nx + x;
-                if (!sink.adjoint) {
-                    sink.snap[(slot + 0) * nz * nx + o] = fxix;
-                    sink.snap[(slot + 1) * nz * nx + o] = fxiz;
-                    sink.snap[(slot + 3) * nz * nx + o] = fvz;
-                } else {
-                    sink.snap[(slot + 1) * nz * nx + o] = fvz;
+        for (int f = 0; f < 3; f++) {
+            const int fi = f == 2 ? 3 : f;
+            const double *F = ca.TN[fi];
+            double w[MS + 10];
+#pragma unroll
+            for (int q = 0; q < MS + 10; q++) w[q] = F[col + (size_t)min(max(a - 5 + q, 0), nz - 1) * nx];
+            asm volatile("" ::: "memory");
+#pragma unroll
+            for (int i = 0; i < MS; i++) {
+                const int j = a + i;
+                const int q = i + 5;   // w[q] = row j
+                const double f11 = 0.75390625 * w[q] + 0.5 * 0.41015625 * (w[q - 1] + w[q + 1]) +
+                                   0.5 * -0.234375 * (w[q - 2] + w[q + 2]) +
+                                   0.5 * 0.087890625 * (w[q - 3] + w[q + 3]) +
+                                   0.5 * -0.01953125 * (w[q - 4] + w[q + 4]) +
+                                   0.5 * 0.001953125 * (w[q - 5] + w[q + 5]);
+                const double f2 = 0.5 * (w[q + 1] + w[q - 1]);
+                double v = w[q];
+                if (xin && j >= 5 && j <= nz - 2) v = (j <= nz - 6) ? f11 : f2;
+                const bool ok = valid(i);
+                const size_t qj = rowq(i);
+                *(ok ? &st.a[fi][qj] : dump) = v;
+                if (f == 2 && ok && out && j == sink.orow)
+                    sink.vz_out[((size_t)b * sink.nout + time / clk.steps) * nx + x] = v;
+                if (ok && snap) {
+                    const size_t o = (size_t)j * nx + x;
+                    const int fs = sink.adjoint ? (f == 2 ? 1 : -1) : (f == 2 ? 3 : f);
+                    if (fs >= 0) sink.snap[(slot + fs) * nz * nx + o] = v;
                 }
             }
+            asm volatile("" ::: "memory");
         }
     }
 }

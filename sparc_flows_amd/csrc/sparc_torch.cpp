@@ -98,6 +98,14 @@ struct Solver : torch::CustomClassHolder {
                               p.defined() ? p.data_ptr<double>() : nullptr),
               h, "sparc_set_model");
     }
+    void set_flow(at::Tensor vx, at::Tensor vz)
+    {
+        at::Tensor a = vx.numel() ? cpu_f64(vx, nx * nz, "vx") : at::Tensor();
+        at::Tensor b = vz.numel() ? cpu_f64(vz, nx * nz, "vz") : at::Tensor();
+        check(sparc_set_flow(h, a.defined() ? a.data_ptr<double>() : nullptr,
+                             b.defined() ? b.data_ptr<double>() : nullptr),
+              h, "sparc_set_flow");
+    }
     void set_forward_sources(at::Tensor xsrc, int64_t maxtime)
     {
         at::Tensor x = cpu_f64(xsrc, nsrc, "xsrc");
@@ -171,7 +179,7 @@ struct Solver : torch::CustomClassHolder {
 
     std::vector<double> profile_kernels(int64_t reps)
     {
-        std::vector<double> ms(4, 0.0);
+        std::vector<double> ms(5, 0.0);
         check(sparc_profile_kernels(h, (int)reps, ms.data()), h, "sparc_profile_kernels");
         return ms;
     }
@@ -207,6 +215,7 @@ TORCH_LIBRARY(sparc, m)
         .def("query_double", &Solver::query_double)
         .def("background", &Solver::background)
         .def("set_model", &Solver::set_model)
+        .def("set_flow", &Solver::set_flow)
         .def("set_forward_sources", &Solver::set_forward_sources)
         .def("set_adjoint_sources", &Solver::set_adjoint_sources)
         .def("run", &Solver::run)

@@ -94,6 +94,12 @@ class Sparc:
         torch.cuda.synchronize(self.device)
         self._s.set_model(_as_tensor(c2d), _as_tensor(psi))
 
+    def set_flow(self, vx_ms=None, vz_ms=None) -> None:
+        """Background flow given directly in m/s, [nz][nx] (the compute_data
+        branch, driver.f90:65-73); None restores the flow from psi."""
+        torch.cuda.synchronize(self.device)
+        self._s.set_flow(_as_tensor(vx_ms), _as_tensor(vz_ms))
+
     # ---- runs ----
     def set_forward_sources(self, xsrc, maxtime: int) -> None:
         x = np.atleast_1d(np.asarray(xsrc, dtype=np.float64))
@@ -158,10 +164,11 @@ class Sparc:
         f = torch.as_tensor(f, dtype=torch.float64, device=self.device).contiguous()
         return self._stream().ddz_kern(f, int(ibc))
 
-    def profile_kernels(self, reps: int = 20) -> dict:
-        """Average ms per launch of each pass (HIP events on this stream); resets the state."""
+    def profile_kernels(self, reps: int = 200) -> dict:
+        """Average ms per launch of each pass over `reps` real time steps (HIP
+        events around every launch on this stream); resets the state."""
         ms = self._stream().profile_kernels(int(reps))
-        return dict(zip(("col", "col_last", "row", "row_first"), ms))
+        return dict(zip(("col", "col_last", "row", "row_first", "row_last"), ms))
 
     def kernels(self, fwd_snaps: torch.Tensor, adj_snaps: torch.Tensor) -> dict:
         """PRODUCE_KERNELS from in-HBM snapshots; arrays [nsrc][nz][nx]."""

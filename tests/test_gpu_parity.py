@@ -170,3 +170,25 @@ def test_adjoint_and_kernels_match_oracle():
         e = rel_l2(K[name][0].cpu().numpy(), oK[name])
         print("kernel", name, e)
         assert e < 1e-10, name
+
+
+def test_explicit_flow_equals_psi_flow():
+    """sparc_set_flow (flow in m/s, the compute_data branch) given the flow the
+    psi path derives reproduces that background and that run; an adjoint
+    handle reverses it."""
+    p, bg, c2d, psi = small_case(nx=64, nz=60)
+    s = Sparc(p, bg, c2d=c2d, psi=psi)
+    v0x, v0z = s.background("v0x"), s.background("v0z")
+    to_ms = 1e6 / 1e2                       # units of dimc -> cm/s -> m/s
+    se = Sparc(p, bg, c2d=c2d)
+    se.set_flow((v0x * to_ms).reshape(p.nz, p.nx), (v0z * to_ms).reshape(p.nz, p.nx))
+    assert rel_l2(se.background("v0x"), v0x) < 1e-15
+    assert rel_l2(se.background("v0z"), v0z) < 1e-15
+    vz1, _ = s.forward([1.0], 120)
+    vz2, _ = se.forward([1.0], 120)
+    assert rel_l2(vz2.cpu().numpy(), vz1.cpu().numpy()) < 1e-13
+    sa = Sparc(p, bg, adjoint=True, c2d=c2d)
+    sa.set_flow((v0x * to_ms).reshape(p.nz, p.nx), (v0z * to_ms).reshape(p.nz, p.nx))
+    assert rel_l2(sa.background("v0x"), -v0x) < 1e-15
+    se.set_flow(None, None)                 # back to the psi flow of construction (none here)
+    assert np.abs(se.background("v0x")).max() == 0.0
