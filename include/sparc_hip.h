@@ -107,6 +107,23 @@ int sparc_set_adjoint_sources(sparc_handle *h, const double *vr_dev, int nt, int
 int sparc_run(sparc_handle *h, int maxtime, double *vz_out_dev, double *snaps_dev);
 /* Restart the state at zero and set the step counter (time0 = 0 start). */
 int sparc_reset(sparc_handle *h);
+/* Part of TIMESTEPPING (driver.f90:337-419): steps t0..t1 (inclusive) from
+ * the current state, which must be the state at time index t0 (after
+ * sparc_reset for t0 = 0, sparc_load_checkpoint, or a previous call that
+ * ended at t0 - 1).  Outputs as sparc_run, but slot j of vz_out_dev / snaps_dev
+ * (nslots slots each, may be NULL) holds output index k0 + j, i.e. time index
+ * (k0 + j) * steps; other outputs are not written.  The reference restarts a
+ * run from its dumped state the same way (init = time0 + 1, driver.f90:236-237,
+ * 350). */
+int sparc_run_range(sparc_handle *h, int t0, int t1, double *vz_out_dev, double *snaps_dev, int k0,
+                    int nslots);
+/* Checkpoints of the full solver state (fields, z- and x-PML variables) for
+ * checkpoint-and-recompute; they replace the reference's full-state dumps
+ * (WRITE_OUT_FULL_STATE, all_modules.f90:623-694).  A checkpoint taken after
+ * sparc_run / sparc_run_range ended at time index t - 1 restarts at t. */
+int sparc_checkpoint_size(const sparc_handle *h, size_t *bytes);
+int sparc_save_checkpoint(sparc_handle *h, void *dev);
+int sparc_load_checkpoint(sparc_handle *h, const void *dev);
 /* One STEP (step.f90:43-119) at time index t of every source. */
 int sparc_step(sparc_handle *h, int t);
 /* State of every source: device [4][nsrc][nz][nx], fields xi_x, xi_z, v_x, v_z
@@ -136,6 +153,15 @@ int sparc_ddz_kern(sparc_handle *h, const double *in_dev, double *out_dev, int n
 int sparc_kernels(sparc_handle *h, int nt, const double *fwd_snaps_dev,
                   const double *adj_snaps_dev, double *kc_dev, double *kpsi_dev, double *kvx_dev,
                   double *kvz_dev, double *hess_dev);
+/* The same accumulation in pieces (PRODUCE_KERNELS' loop over snapshot
+ * pairs, kernels.f90:200-360, then its scalings, :495-580): begin zeroes the
+ * accumulators; add accumulates n pairs from fwd [nsrc][n][6][nz][nx] and
+ * adj [nsrc][n][4][nz][nx], pairing fwd slot i with adj slot n-1-i; end
+ * writes the scaled kernels as sparc_kernels does. */
+int sparc_kernels_begin(sparc_handle *h);
+int sparc_kernels_add(sparc_handle *h, int n, const double *fwd_snaps_dev, const double *adj_snaps_dev);
+int sparc_kernels_end(sparc_handle *h, double *kc_dev, double *kpsi_dev, double *kvx_dev,
+                      double *kvz_dev, double *hess_dev);
 
 /* Instrumentation (no reference counterpart; the reference prints per-step
  * MPI_WTIME, driver.f90:355-381): average device time in ms of one launch of

@@ -72,6 +72,8 @@ struct sparc_handle {
     Tri tri[4]{};
     State st{};
     double *d_scratch = nullptr;    // [S][nz][nx] general scratch
+    double *kbuf = nullptr;         // kernel accumulation: D[8], KC, KX, KZ, H, t1, t2, work
+    long kern_pairs = 0;
     double *d_scratch2 = nullptr;
     int *tctr = nullptr;
 
@@ -79,6 +81,8 @@ struct sparc_handle {
     Src src{};
     int maxtime = -1, forcing_length = 0, num_steps = 0;
     bool have_src = false;
+    bool seg_forced = false;             // segment length set by SPARC_COLSEG_MS
+    std::vector<SegTab> seg_cands;       // column-pass tables built so far
     // launch timing (sparc_profile_kernels): events recorded around each pass
     struct Mark { hipEvent_t a, b; int kind; };
     std::vector<hipEvent_t> evpool;
@@ -517,6 +521,11 @@ static int upload_background(sparc_handle *h)
         h->tri[ibc] = Tri{a, b, c};
     }
     HCHECK(h, upload(h->bg_mem, &p, h->rhoinv, h->stream)); h->bg.rhoinv = p;
+    {
+        std::vector<double> rz(nz);
+        for (int k = 0; k < nz; k++) rz[k] = h->rhoinv[(size_t)k * h->nx];
+        HCHECK(h, upload(h->bg_mem, &p, rz, h->stream)); h->bg.rhoinv_z = p;
+    }
     HCHECK(h, upload(h->bg_mem, &p, h->rho0, h->stream)); h->bg.rho0 = p;
     HCHECK(h, upload(h->bg_mem, &p, h->c2rho0, h->stream)); h->bg.c2rho0 = p;
     HCHECK(h, upload(h->bg_mem, &p, h->gp0, h->stream)); h->bg.gp0 = p;
@@ -556,34 +565,48 @@ static int upload_background(sparc_handle *h)
 // Segmented solve tables (SegTab) for the column pass: P segments of m rows
 // (m from SPARC_COLSEG_SIZES, the last segment padded with identity rows and
 // holding at least 3 real rows).
-static int setup_seg(sparc_handle *h)
+// Segment lengths of the column pass usable for nz rows: the segment count
+// fits kSegMax and the last (padded) segment holds at least 3 real rows.
+static std::vector<int> seg_lengths(int N)
 {
-    const int N = h->nz;
     static const int sizes[] = {
 #define X(v) v,
         SPARC_COLSEG_SIZES(X)
 #undef X
     };
-    int m = 0, P = 0;
+    std::vector<int> out;
     for (int v : sizes) {
         const int p = (N + v - 1) / v;
-        if (p > kSegMax) continue;
-        if (N - (p - 1) * v < 3) continue;
-        m = v;
-        P = p;
-        break;
+        if (p <= kSegMax && N - (p - 1) * v >= 3) out.push_back(v);
     }
-    if (!m) return fail(h, SPARC_E_ARG, "nz not supported by the segmented column pass");
-    h->seg_ms = m;
+    return out;
+}
+
+// Segmented solve tables (SegTab) of segment length m for the column pass:
+// P segments of m rows, the last padded with identity rows.
+static int build_seg(sparc_handle *h, int m, SegTab &sg)
+{
+    const int N = h->nz;
+    const int P = (N + m - 1) / m;
     const int nzp = P * m;
-    SegTab &sg = h->seg;
     sg.P = P;
     sg.m = m;
     sg.nzp = nzp;
     std::vector<double> ztab((size_t)kZT * nzp, 0.0);
+    // z profiles; rho0, d rho0/dz, d p0/dz and 1/rho0 come from the 1D model
+    // and are the same in every column
     for (int j = 0; j < N; j++) {
+        const size_t q = (size_t)j * h->nx;
         ztab[(size_t)0 * nzp + j] = h->stretch[j];
         ztab[(size_t)1 * nzp + j] = h->g[j];
+        ztab[(size_t)17 * nzp + j] = h->rho0[q];
+        ztab[(size_t)18 * nzp + j] = h->gr0[q];
+        ztab[(size_t)19 * nzp + j] = h->gp0[q];
+        ztab[(size_t)20 * nzp + j] = h->rhoinv[q];
+        for (int i = 1; i < h->nx; i++)
+            if (h->rho0[q + i] != h->rho0[q] || h->gr0[q + i] != h->gr0[q] || h->gp0[q + i] != h->gp0[q] ||
+                h->rhoinv[q + i] != h->rhoinv[q])
+                return fail(h, SPARC_E_ARG, "background density / pressure vary along x");
     }
     const int n2 = 2 * P;
     std::vector<double> minv_all;
@@ -646,6 +669,31 @@ static int setup_seg(sparc_handle *h)
     sg.ztab = p;
     HCHECK(h, upload(h->bg_mem, &p, minv_all, h->stream));
     sg.Minv = p;
+    return SPARC_OK;
+}
+
+// Default column-pass tables: the segment length SPARC_COLSEG_MS when it fits
+// this nz, else the shortest usable one (the autotuner may pick another).
+static int setup_seg(sparc_handle *h)
+{
+    HCHECK(h, colseg_init());
+    const std::vector<int> ms = seg_lengths(h->nz);
+    if (ms.empty()) return fail(h, SPARC_E_ARG, "nz not supported by the segmented column pass");
+    int m = ms[0];
+    h->seg_forced = false;
+    if (const char *e = std::getenv("SPARC_COLSEG_MS")) {
+        const int v = std::atoi(e);
+        for (int w : ms)
+            if (w == v) { m = v; h->seg_forced = true; }
+    }
+    h->seg_cands.clear();
+    h->tuned = false;
+    SegTab sg{};
+    const int r = build_seg(h, m, sg);
+    if (r) return r;
+    h->seg = sg;
+    h->seg_ms = m;
+    h->seg_cands.push_back(sg);
     return SPARC_OK;
 }
 
@@ -1128,9 +1176,16 @@ static int autotune(sparc_handle *h)
 {
     if (h->tuned) return SPARC_OK;
     h->tuned = true;
+    int force_tc = 0;
     if (const char *e = std::getenv("SPARC_COLSEG_TC")) {
         const int v = std::atoi(e);
-        if (v == 4 || v == 8 || v == 16) { h->seg_tc = v; return SPARC_OK; }
+        if (v == 4 || v == 8 || v == 16) force_tc = v;
+    }
+    // candidate segment lengths (only the forced one under SPARC_COLSEG_MS)
+    std::vector<int> ms = h->seg_forced ? std::vector<int>{h->seg.m} : seg_lengths(h->nz);
+    if (force_tc && ms.size() == 1) {
+        if (h->nx % force_tc == 0 && force_tc * h->seg.P <= 256) h->seg_tc = force_tc;
+        return SPARC_OK;
     }
     Sink sk = empty_sink(h);
     Clock clk{h->tctr, 0, h->steps};
@@ -1142,22 +1197,43 @@ static int autotune(sparc_handle *h)
     hipEvent_t e0, e1;
     HCHECK(h, hipEventCreate(&e0));
     HCHECK(h, hipEventCreate(&e1));
-    const int cands[3] = {4, 8, 16};
     float best = 1e30f;
-    for (int tc : cands) {
-        if (h->nx % tc) continue;
-        HCHECK(h, launch_colseg(tc, h->seg_ms, h->d, h->bg, h->cf, h->seg, s, ca, h->src, clk, sk, h->stream));
-        HCHECK(h, hipEventRecord(e0, h->stream));
-        for (int r = 0; r < 5; r++)
-            HCHECK(h, launch_colseg(tc, h->seg_ms, h->d, h->bg, h->cf, h->seg, s, ca, h->src, clk, sk, h->stream));
-        HCHECK(h, hipEventRecord(e1, h->stream));
-        HCHECK(h, hipEventSynchronize(e1));
-        float ms = 0.f;
-        HCHECK(h, hipEventElapsedTime(&ms, e0, e1));
-        if (ms < best) { best = ms; h->seg_tc = tc; }
+    SegTab best_sg = h->seg;
+    int best_tc = 0;
+    for (int m : ms) {
+        SegTab sg{};
+        bool have = false;
+        for (const SegTab &c : h->seg_cands)
+            if (c.m == m) { sg = c; have = true; }
+        if (!have) {
+            const int r = build_seg(h, m, sg);
+            if (r) return r;
+            h->seg_cands.push_back(sg);
+        }
+        for (int tc : {4, 8, 16}) {
+            if (force_tc && tc != force_tc) continue;
+            if (h->nx % tc || tc * sg.P > 256) continue;
+            HCHECK(h, launch_colseg(tc, m, h->d, h->bg, h->cf, sg, s, ca, h->src, clk, sk, h->stream));
+            HCHECK(h, hipEventRecord(e0, h->stream));
+            for (int r = 0; r < 10; r++)
+                HCHECK(h, launch_colseg(tc, m, h->d, h->bg, h->cf, sg, s, ca, h->src, clk, sk, h->stream));
+            HCHECK(h, hipEventRecord(e1, h->stream));
+            HCHECK(h, hipEventSynchronize(e1));
+            float t = 0.f;
+            HCHECK(h, hipEventElapsedTime(&t, e0, e1));
+            if (t < best) { best = t; best_sg = sg; best_tc = tc; }
+        }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
+    if (!best_tc) return fail(h, SPARC_E_ARG, "no column-pass block shape fits this grid");
+    h->seg = best_sg;
+    h->seg_ms = best_sg.m;
+    h->seg_tc = best_tc;
+    // the timed launches wrote only stage temporaries: restore the z-PML
+    // stage copy, which equals its base between steps
+    const size_t pz = (size_t)2 * h->S * h->d.nzpml * h->nx;
+    HCHECK(h, hipMemcpyAsync(s.pz_tmp, s.pz_base, pz * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
     return SPARC_OK;
 }
 
@@ -1168,28 +1244,21 @@ static int graph_steps()
     return 16;
 }
 
-int sparc_run(sparc_handle *h, int maxtime, double *vz_out, double *snaps)
+}  // extern "C"
+
+// Steps t0..t1 (inclusive) of every source from the current state, which is
+// at time index t0 with the v_x filter applied; leaves it at t1 + 1, filtered.
+// Step t0 is launched directly, the rest in hipGraphs of G steps each, every
+// kernel reading the step counter (t = counter + offset in the graph) so one
+// graph serves every block of G steps.
+static int run_steps(sparc_handle *h, int t0, int t1, const Sink &sk)
 {
-    if (!h || maxtime < 0) return SPARC_E_ARG;
-    if (!h->have_src) return fail(h, SPARC_E_STATE, "sources not set");
-    if (maxtime != h->maxtime) return fail(h, SPARC_E_STATE, "maxtime differs from the sources' maxtime");
-    int r = autotune(h);
-    if (r) return r;
-    r = sparc_reset(h);
-    if (r) return r;
-    Sink sk = empty_sink(h);
-    const int nout = maxtime / h->steps + 1;
-    sk.vz_out = vz_out;
-    sk.nout = vz_out ? nout : 0;
-    sk.snap = snaps;
-    sk.nsnap = snaps ? nout : 0;
-    // step 0 (no pending v_x filter) is launched directly; the rest in
-    // hipGraphs of G steps each, every kernel reading the step counter
-    // (t = counter + offset in the graph) so one graph serves every block.
-    const int nsteps = maxtime + 1;
-    r = enqueue_step(h, 0, false, sk);
+    if (t1 < t0) return SPARC_OK;
+    HCHECK(h, launch_set_int(h->tctr, t0, h->stream));
+    int r = enqueue_step(h, 0, false, sk);
     if (r) return r;
     HCHECK(h, launch_tick(h->tctr, 1, h->stream));
+    const int nsteps = t1 - t0 + 1;
     int done = 1;
     const int G = graph_steps();
     if (G > 1 && nsteps - done >= G) {
@@ -1218,6 +1287,81 @@ int sparc_run(sparc_handle *h, int maxtime, double *vz_out, double *snaps)
     }
     Clock clk{h->tctr, 0, h->steps};
     HCHECK(h, launch_finalize_vx(h->d, h->st, clk, sk, h->stream));
+    return SPARC_OK;
+}
+
+extern "C" {
+
+int sparc_run(sparc_handle *h, int maxtime, double *vz_out, double *snaps)
+{
+    if (!h || maxtime < 0) return SPARC_E_ARG;
+    if (!h->have_src) return fail(h, SPARC_E_STATE, "sources not set");
+    if (maxtime != h->maxtime) return fail(h, SPARC_E_STATE, "maxtime differs from the sources' maxtime");
+    int r = autotune(h);
+    if (r) return r;
+    r = sparc_reset(h);
+    if (r) return r;
+    Sink sk = empty_sink(h);
+    const int nout = maxtime / h->steps + 1;
+    sk.vz_out = vz_out;
+    sk.nout = vz_out ? nout : 0;
+    sk.snap = snaps;
+    sk.nsnap = snaps ? nout : 0;
+    return run_steps(h, 0, maxtime, sk);
+}
+
+int sparc_run_range(sparc_handle *h, int t0, int t1, double *vz_out, double *snaps, int k0, int nslots)
+{
+    if (!h || t0 < 0 || t1 < t0 - 1 || k0 < 0 || nslots < 0) return SPARC_E_ARG;
+    if (!h->have_src) return fail(h, SPARC_E_STATE, "sources not set");
+    if (t1 > h->maxtime) return fail(h, SPARC_E_ARG, "sparc_run_range beyond the sources' maxtime");
+    int r = autotune(h);
+    if (r) return r;
+    Sink sk = empty_sink(h);
+    sk.k0 = k0;
+    sk.vz_out = vz_out;
+    sk.nout = vz_out ? nslots : 0;
+    sk.snap = snaps;
+    sk.nsnap = snaps ? nslots : 0;
+    return run_steps(h, t0, t1, sk);
+}
+
+int sparc_checkpoint_size(const sparc_handle *h, size_t *bytes)
+{
+    if (!h || !bytes) return SPARC_E_ARG;
+    const size_t f = (size_t)h->S * h->nz * h->nx;
+    const size_t pz = (size_t)2 * h->S * h->d.nzpml * h->nx, px = (size_t)2 * h->S * h->nz * h->d.nph;
+    *bytes = (4 * f + pz + px) * sizeof(double);
+    return SPARC_OK;
+}
+
+int sparc_save_checkpoint(sparc_handle *h, void *dev)
+{
+    if (!h || !dev) return SPARC_E_ARG;
+    const size_t f = (size_t)h->S * h->nz * h->nx;
+    const size_t pz = (size_t)2 * h->S * h->d.nzpml * h->nx, px = (size_t)2 * h->S * h->nz * h->d.nph;
+    double *o = (double *)dev;
+    for (int i = 0; i < 4; i++)
+        HCHECK(h, hipMemcpyAsync(o + i * f, h->st.a[i], f * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+    HCHECK(h, hipMemcpyAsync(o + 4 * f, h->st.pz_base, pz * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+    HCHECK(h, hipMemcpyAsync(o + 4 * f + pz, h->st.px_base, px * sizeof(double), hipMemcpyDeviceToDevice,
+                             h->stream));
+    return SPARC_OK;
+}
+
+int sparc_load_checkpoint(sparc_handle *h, const void *dev)
+{
+    if (!h || !dev) return SPARC_E_ARG;
+    const size_t f = (size_t)h->S * h->nz * h->nx;
+    const size_t pz = (size_t)2 * h->S * h->d.nzpml * h->nx, px = (size_t)2 * h->S * h->nz * h->d.nph;
+    const double *o = (const double *)dev;
+    for (int i = 0; i < 4; i++)
+        HCHECK(h, hipMemcpyAsync(h->st.a[i], o + i * f, f * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+    // the stage copies of the PML variables equal their bases between steps
+    for (double *dst : {h->st.pz_base, h->st.pz_tmp})
+        HCHECK(h, hipMemcpyAsync(dst, o + 4 * f, pz * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+    for (double *dst : {h->st.px_base, h->st.px_tmp})
+        HCHECK(h, hipMemcpyAsync(dst, o + 4 * f + pz, px * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
     return SPARC_OK;
 }
 
@@ -1280,20 +1424,28 @@ int sparc_ddz_kern(sparc_handle *h, const double *in, double *out, int nbatch, i
     return ddz_impl(h, in, out, nbatch, ibc, 1);
 }
 
-int sparc_kernels(sparc_handle *h, int nt, const double *fwd, const double *adj, double *kc,
-                  double *kpsi, double *kvx, double *kvz, double *hess)
+int sparc_kernels_begin(sparc_handle *h)
 {
-    if (!h || nt < 1 || !fwd || !adj || !kc || !kpsi || !kvx || !kvz || !hess) return SPARC_E_ARG;
-    const int S = h->S;
+    if (!h) return SPARC_E_ARG;
+    const size_t n = (size_t)h->S * h->nx * h->nz;
+    if (!h->kbuf) {
+        // D[8] + KC KX KZ H + t1 t2 + ddz work
+        HCHECK(h, h->kern_mem.alloc(&h->kbuf, 15 * n));
+    }
+    HCHECK(h, launch_fill(h->kbuf + 8 * n, 4 * n, 0.0, h->stream));
+    h->kern_pairs = 0;
+    return SPARC_OK;
+}
+
+int sparc_kernels_add(sparc_handle *h, int n_pairs, const double *fwd, const double *adj)
+{
+    if (!h || n_pairs < 1 || !fwd || !adj) return SPARC_E_ARG;
+    if (!h->kbuf) return fail(h, SPARC_E_STATE, "sparc_kernels_add before sparc_kernels_begin");
+    const int S = h->S, nt = n_pairs;
     const size_t n2 = (size_t)h->nx * h->nz, n = (size_t)S * n2;
-    double *buf = nullptr;
-    // D[8] + KC KX KZ + t1 t2 + ddz work
-    HCHECK(h, hipMalloc((void **)&buf, 14 * n * sizeof(double)));
-    double *D = buf, *KC = buf + 8 * n, *KX = buf + 9 * n, *KZ = buf + 10 * n, *t1 = buf + 11 * n,
-           *t2 = buf + 12 * n, *work = buf + 13 * n;
-    double *g = h->d_scratch, *g2 = h->d_scratch2;
-    HCHECK(h, launch_fill(KC, 3 * n, 0.0, h->stream));
-    HCHECK(h, launch_fill(hess, n, 0.0, h->stream));
+    double *D = h->kbuf, *KC = h->kbuf + 8 * n, *KX = KC + n, *KZ = KC + 2 * n, *H = KC + 3 * n;
+    double *work = h->kbuf + 14 * n;
+    double *g = h->d_scratch;
     const Tri &tr1 = h->tri[1];
     for (int k = 0; k < nt; k++) {
         const int ka = nt - 1 - k;
@@ -1311,9 +1463,20 @@ int sparc_kernels(sparc_handle *h, int nt, const double *fwd, const double *adj,
                 HCHECK(h, launch_ddz_cols(g, D + j * n, h->nx, h->nz, S, 1, h->bg.stretch, h->cf, tr1,
                                           work, 1, h->stream));
         }
-        HCHECK(h, launch_kern_accum(fwd, adj, S, nt, k, n2, D, KC, KX, KZ, hess, h->stream));
+        HCHECK(h, launch_kern_accum(fwd, adj, S, nt, k, n2, D, KC, KX, KZ, H, h->stream));
     }
-    (void)g2;
+    h->kern_pairs += nt;
+    return SPARC_OK;
+}
+
+int sparc_kernels_end(sparc_handle *h, double *kc, double *kpsi, double *kvx, double *kvz, double *hess)
+{
+    if (!h || !kc || !kpsi || !kvx || !kvz || !hess) return SPARC_E_ARG;
+    if (!h->kbuf) return fail(h, SPARC_E_STATE, "sparc_kernels_end before sparc_kernels_begin");
+    const int S = h->S;
+    const size_t n2 = (size_t)h->nx * h->nz, n = (size_t)S * n2;
+    double *KC = h->kbuf + 8 * n, *KX = KC + n, *KZ = KC + 2 * n, *H = KC + 3 * n;
+    double *t1 = h->kbuf + 12 * n, *t2 = h->kbuf + 13 * n, *work = h->kbuf + 14 * n;
     // curl (bcz = 0, kernels.f90:832) and unit scalings (:498-579)
     HCHECK(h, launch_ddz_cols(KX, t1, h->nx, h->nz, S, 0, h->bg.stretch, h->cf, h->tri[0], work, 1,
                               h->stream));
@@ -1324,9 +1487,19 @@ int sparc_kernels(sparc_handle *h, int nt, const double *fwd, const double *adj,
     const int stepskern = (int)std::floor(h->outputcad / h->timestep);
     HCHECK(h, launch_kern_final(S, n2, h->d_krho0, h->d_kc2, (double)stepskern, h->timestep, dimen,
                                 dimen_c, t1, t2, KC, KX, KZ, kc, kpsi, kvx, kvz, h->stream));
+    HCHECK(h, hipMemcpyAsync(hess, H, n * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
     HCHECK(h, hipStreamSynchronize(h->stream));
-    HCHECK(h, hipFree(buf));
     return SPARC_OK;
+}
+
+int sparc_kernels(sparc_handle *h, int nt, const double *fwd, const double *adj, double *kc,
+                  double *kpsi, double *kvx, double *kvz, double *hess)
+{
+    if (!h || nt < 1 || !fwd || !adj || !kc || !kpsi || !kvx || !kvz || !hess) return SPARC_E_ARG;
+    int r = sparc_kernels_begin(h);
+    if (!r) r = sparc_kernels_add(h, nt, fwd, adj);
+    if (!r) r = sparc_kernels_end(h, kc, kpsi, kvx, kvz, hess);
+    return r;
 }
 
 int sparc_profile_kernels(sparc_handle *h, int reps, double *ms_out)

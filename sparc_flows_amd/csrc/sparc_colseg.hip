@@ -7,8 +7,8 @@
 // rows and columns, the RK stage update and, after the last stage,
 // FILTER_VARS_2D (physics2d.f90:509-556).
 //
-// Every column is split into P <= 16 segments of exactly MS rows, held in
-// registers by one lane each; a block covers TC columns (blockDim = 16*TC).
+// Every column is split into P <= kSegMax segments of exactly MS rows, held in
+// registers by one lane each; a block covers TC columns (blockDim = P*TC).
 // Each z-derivative is: dbyd2 right-hand side from a 7-row window, local
 // Thomas solve of the segment, exchange of the segment's end values through
 // LDS, and the exact correction y - L_{s-1} V - F_{s+1} W.  Loads use clamped
@@ -21,7 +21,7 @@ namespace sparc {
 
 namespace {
 
-enum ZRow { Z_STRETCH = 0, Z_G = 1 };
+enum ZRow { Z_STRETCH = 0, Z_G = 1, Z_RHO = 17, Z_GRHO = 18, Z_GP = 19, Z_RINV = 20 };
 __device__ __forceinline__ int zrow(int slot, int k) { return 2 + 5 * slot + k; }  // k: 0 lowr 1 rbl 2 gl 3 V 4 W
 
 // dbyd2 right-hand side (dbyd2.f:91-172) of rows j = a..a+MS-1; ld(k) returns
@@ -100,14 +100,20 @@ __device__ __forceinline__ void seg_correct(double (&Y)[MS], int s, int P, int h
 
 }  // namespace
 
+// Dynamic LDS of one block: the three interface inverses Minv [3][2P][2P],
+// then the four end-value exchange buffers [4][TC][2P].
+inline size_t colseg_lds_bytes(int tc, int P) { return sizeof(double) * (size_t)(3 * 4 * P * P + 4 * tc * 2 * P); }
+
 template <int MS, int LAST>
 __global__ __launch_bounds__(256) void k_colseg(Dims d, Bg bg, DbyCoef cf, SegTab sg, State st,
                                                 ColArgs ca, Src src, Clock clk, Sink sink)
 {
-    __shared__ double mv[3 * 4 * kSegMax * kSegMax];
-    __shared__ double yb[4][16 * 2 * kSegMax];
-    const int TC = blockDim.x / kSegMax;
+    extern __shared__ double lds_dyn[];
     const int P = sg.P, n2P = 2 * P, nzp = sg.nzp;
+    const int TC = blockDim.x / P;
+    double *const mv = lds_dyn;
+    double *const yb0 = lds_dyn + 3 * n2P * n2P;
+    double *const yb[4] = {yb0, yb0 + TC * n2P, yb0 + 2 * TC * n2P, yb0 + 3 * TC * n2P};
     const int tid = threadIdx.x;
     for (int q = tid; q < 3 * n2P * n2P; q += blockDim.x) mv[q] = sg.Minv[q];
     const int c = tid % TC, s = tid / TC;
@@ -172,8 +178,8 @@ __global__ __launch_bounds__(256) void k_colseg(Dims d, Bg bg, DbyCoef cf, SegTa
         const size_t qp = pz0 + (size_t)pj * nx, qb = (size_t)pj * nx + x;
         const double psi = st.pz_tmp[qp], az = bg.az[qb], bzc = bg.bz[qb], pbase = st.pz_base[qp];
         const double dxix = st.dxixdx[qj], xiz = ca.T[1][qj];
-        const double c2r = bg.c2rho0[gq], gp0 = bg.gp0[gq], r0 = bg.rho0[gq], gr0 = bg.gr0[gq];
-        const double rinv = bg.rhoinv[gq], gz = zt[Z_G * nzp + i];
+        const double c2r = bg.c2rho0[gq], gp0 = zt[Z_GP * nzp + i], r0 = zt[Z_RHO * nzp + i];
+        const double gr0 = zt[Z_GRHO * nzp + i], rinv = zt[Z_RINV * nzp + i], gz = zt[Z_G * nzp + i];
         double dz = Y[i] * zt[Z_STRETCH * nzp + i];
         const bool pm = is_pml(j);
         const double prhs = az * dz + bzc * psi;
@@ -234,8 +240,9 @@ __global__ __launch_bounds__(256) void k_colseg(Dims d, Bg bg, DbyCoef cf, SegTa
     __syncthreads();
     if (act) seg_correct<MS>(Y, s, P, sg.hw, yb[3] + c * n2P, mv + n2P * n2P, zt + zrow(1, 3) * nzp,
                              zt + zrow(1, 4) * nzp);
-    const bool snap = LAST && sink.snap && time % clk.steps == 0 && time / clk.steps < sink.nsnap;
-    const size_t slot = snap ? ((size_t)b * sink.nsnap + time / clk.steps) * sink.nfield : 0;
+    const int sslot = (LAST && sink.snap) ? out_slot(time, clk.steps, sink.k0, sink.nsnap) : -1;
+    const bool snap = sslot >= 0;
+    const size_t slot = snap ? ((size_t)b * sink.nsnap + sslot) * sink.nfield : 0;
 #pragma unroll
     for (int i = 0; i < MS; i++) {
         const int j = a + i;
@@ -243,7 +250,7 @@ __global__ __launch_bounds__(256) void k_colseg(Dims d, Bg bg, DbyCoef cf, SegTa
         const int pj = pmlrow(j);
         const size_t qp = pz1 + (size_t)pj * nx, qb = (size_t)pj * nx + x;
         const double psi = st.pz_tmp[qp], az = bg.az[qb], bzc = bg.bz[qb], pbase = st.pz_base[qp];
-        const double rinv = bg.rhoinv[gq];
+        const double rinv = zt[Z_RINV * nzp + i];
         const double vx = ca.T[2][qj], vz = ca.T[3][qj];
         const double a0 = st.a[0][qj], a1 = st.a[1][qj], a3 = st.a[3][qj];
         double gpz = Y[i] * zt[Z_STRETCH * nzp + i];
@@ -274,7 +281,8 @@ __global__ __launch_bounds__(256) void k_colseg(Dims d, Bg bg, DbyCoef cf, SegTa
         // segments' rows were written before the barrier) and filtered in
         // registers.
         __syncthreads();
-        const bool out = sink.vz_out && time % clk.steps == 0 && time / clk.steps < sink.nout;
+        const int oslot = sink.vz_out ? out_slot(time, clk.steps, sink.k0, sink.nout) : -1;
+        const bool out = oslot >= 0;
         const int nh = d.nh;
         const bool xin = x >= nh - 1 && x <= nx - nh;
 #pragma unroll
@@ -301,7 +309,7 @@ __global__ __launch_bounds__(256) void k_colseg(Dims d, Bg bg, DbyCoef cf, SegTa
                 const size_t qj = rowq(i);
                 *(ok ? &st.a[fi][qj] : dump) = v;
                 if (f == 2 && ok && out && j == sink.orow)
-                    sink.vz_out[((size_t)b * sink.nout + time / clk.steps) * nx + x] = v;
+                    sink.vz_out[((size_t)b * sink.nout + oslot) * nx + x] = v;
                 if (ok && snap) {
                     const size_t o = (size_t)j * nx + x;
                     const int fs = sink.adjoint ? (f == 2 ? 1 : -1) : (f == 2 ? 3 : f);
@@ -313,16 +321,37 @@ __global__ __launch_bounds__(256) void k_colseg(Dims d, Bg bg, DbyCoef cf, SegTa
     }
 }
 
+// Allows each instantiation the dynamic LDS of the largest block shape
+// (16 columns, kSegMax segments).
+hipError_t colseg_init()
+{
+    const int bytes = (int)colseg_lds_bytes(16, kSegMax);
+#define A(MSV)                                                                                          \
+    {                                                                                                   \
+        hipError_t e = hipFuncSetAttribute((const void *)k_colseg<MSV, 0>,                              \
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, bytes);          \
+        if (e != hipSuccess) return e;                                                                  \
+        e = hipFuncSetAttribute((const void *)k_colseg<MSV, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, \
+                                bytes);                                                                 \
+        if (e != hipSuccess) return e;                                                                  \
+    }
+    SPARC_COLSEG_SIZES(A)
+#undef A
+    return hipSuccess;
+}
+
 hipError_t launch_colseg(int tc, int ms, const Dims &d, const Bg &bg, const DbyCoef &cf, const SegTab &sg,
                          const State &s, const ColArgs &ca, const Src &src, const Clock &c, const Sink &sk,
                          hipStream_t st)
 {
-    if ((tc != 4 && tc != 8 && tc != 16) || d.nx % tc || sg.m != ms) return hipErrorInvalidValue;
-    dim3 grid(d.nx / tc, d.S), block(tc * kSegMax);
+    if ((tc != 4 && tc != 8 && tc != 16) || d.nx % tc || sg.m != ms || sg.P > kSegMax || tc * sg.P > 256)
+        return hipErrorInvalidValue;
+    dim3 grid(d.nx / tc, d.S), block(tc * sg.P);
+    const size_t shm = colseg_lds_bytes(tc, sg.P);
 #define L(MSV)                                                                                         \
     if (ms == MSV) {                                                                                   \
-        if (ca.last) hipLaunchKernelGGL((k_colseg<MSV, 1>), grid, block, 0, st, d, bg, cf, sg, s, ca, src, c, sk); \
-        else hipLaunchKernelGGL((k_colseg<MSV, 0>), grid, block, 0, st, d, bg, cf, sg, s, ca, src, c, sk);         \
+        if (ca.last) hipLaunchKernelGGL((k_colseg<MSV, 1>), grid, block, shm, st, d, bg, cf, sg, s, ca, src, c, sk); \
+        else hipLaunchKernelGGL((k_colseg<MSV, 0>), grid, block, shm, st, d, bg, cf, sg, s, ca, src, c, sk);         \
         return hipGetLastError();                                                                      \
     }
     SPARC_COLSEG_SIZES(L)

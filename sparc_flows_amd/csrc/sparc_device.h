@@ -20,6 +20,7 @@ struct Dims {
 // Background arrays shared by every source of a handle (device pointers).
 struct Bg {
     const double *rhoinv, *rho0, *c2rho0, *gp0, *gr0, *v0x, *v0z;  // [nz][nx]
+    const double *rhoinv_z;                                        // [nz]: 1/rho0 is x-independent
     const double *g, *stretch, *unstretch;                 // [nz]
     const double *az, *bz;                                 // [nzpml][nx]
     const double *ax, *bx;                                 // [nz][nph]
@@ -46,15 +47,16 @@ struct Tri {
 };
 
 // Segmented (SPIKE) form of the same matrices for the column pass: the column
-// is cut into P <= 16 segments of exactly m rows (rows >= nz of the last
+// is cut into P <= kSegMax segments of exactly m rows (rows >= nz of the last
 // segment are identity padding).  Each segment is solved locally (rbl, gl:
 // Thomas factors restarted at the segment start) and corrected with the spikes
 // V (response to the row above the segment) and W (to the row below), weighted
 // by the interface values from the exact inverse Minv of the 2P x 2P
 // interface system.  ztab rows (each nzp = P*m long): 0 stretch, 1 g, then
-// for slot 0 (ibc 1), 1 (ibc 2), 2 (ibc 3): lowr, rbl, gl, V, W.
-constexpr int kSegMax = 16;
-constexpr int kZT = 17;
+// for slot 0 (ibc 1), 1 (ibc 2), 2 (ibc 3): lowr, rbl, gl, V, W (rows 2-16),
+// then the x-independent profiles rho0, d rho0/dz, d p0/dz, 1/rho0 (17-20).
+constexpr int kSegMax = 32;
+constexpr int kZT = 21;
 struct SegTab {
     int P, m, hw, nzp;              // segments, rows per segment, Minv window, P*m
     const double *ztab;             // [kZT][nzp]
@@ -103,7 +105,17 @@ struct Sink {
     int nsnap, nfield;      // forward: 6 (xix, xiz, vx, vz, accx, accz); adjoint: 4 (vx, vz, accx, accz)
     int orow;               // o_rad
     int adjoint;
+    int k0;                 // output index held by slot 0 of vz_out / snap
 };
+
+// Slot of time index t in an output buffer of n slots starting at output
+// index k0 (outputs are the multiples of steps), or -1.
+__device__ __forceinline__ int out_slot(int t, int steps, int k0, int n)
+{
+    if (t < 0 || t % steps) return -1;
+    const int k = t / steps - k0;
+    return (k >= 0 && k < n) ? k : -1;
+}
 
 // Arguments of one row pass (k_row).
 struct RowArgs {

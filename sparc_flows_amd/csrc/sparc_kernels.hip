@@ -198,14 +198,15 @@ __global__ __launch_bounds__(64) void k_row(Dims d, Bg bg, State st, RowArgs ra,
         }
         if (ra.filt_vx) {
             const int tp = time - 1;
-            const bool snap = sink.snap && tp >= 0 && tp % clk.steps == 0 && tp / clk.steps < sink.nsnap;
+            const int sslot = sink.snap ? out_slot(tp, clk.steps, sink.k0, sink.nsnap) : -1;
+            const bool snap = sslot >= 0;
             const int fslot = sink.adjoint ? 0 : 2;
 #pragma unroll
             for (int j2 = 0; j2 < N2; j2++) {
                 const int x = t + N1 * j2;
                 *(valid ? &st.a[2][rowq + x] : dump) = vx[j2];
                 if (snap && valid)
-                    sink.snap[((((size_t)b * sink.nsnap + tp / clk.steps) * sink.nfield + fslot) * nz + k) * NX + x] = vx[j2];
+                    sink.snap[((((size_t)b * sink.nsnap + sslot) * sink.nfield + fslot) * nz + k) * NX + x] = vx[j2];
             }
         }
         fft_deriv_team<N1, N2>(re, im, t, lds[team], bg.tw, bg.dmul, bg.stretchx);
@@ -237,11 +238,12 @@ __global__ __launch_bounds__(64) void k_row(Dims d, Bg bg, State st, RowArgs ra,
             const int x = t + N1 * j2;
             rvx0[j2] = st.rvx[rowq + x];
             avx[j2] = st.a[2][rowq + x];
-            rinv[j2] = bg.rhoinv[(size_t)k * NX + x];
+            rinv[j2] = bg.rhoinv_z[k];
         }
         fft_deriv_team<N1, N2>(re, im, t, lds[team], bg.tw, bg.dmul, bg.stretchx);
         // x-PML on grad_x p (physics2d.f90:1003-1020), RHS v_x (:1039 + flows)
-        const bool snap = MODE == 2 && sink.snap && time % clk.steps == 0 && time / clk.steps < sink.nsnap;
+        const int sslot = (MODE == 2 && sink.snap) ? out_slot(time, clk.steps, sink.k0, sink.nsnap) : -1;
+        const bool snap = sslot >= 0;
 #pragma unroll
         for (int j2 = 0; j2 < N2; j2++) {
             const int x = t + N1 * j2;
@@ -266,7 +268,7 @@ __global__ __launch_bounds__(64) void k_row(Dims d, Bg bg, State st, RowArgs ra,
             vx[j2] = v;
             *(valid ? &ra.v_x_out[rowq + x] : dump) = v;
             if (snap && valid)
-                sink.snap[((((size_t)b * sink.nsnap + time / clk.steps) * sink.nfield +
+                sink.snap[((((size_t)b * sink.nsnap + sslot) * sink.nfield +
                             (sink.adjoint ? 2 : 4)) * nz + k) * NX + x] = rvx;
         }
         if (MODE == 1) {
@@ -322,13 +324,14 @@ __global__ __launch_bounds__(64) void k_finalize_vx(Dims d, State st, Clock clk,
     if (x >= d.nx) return;
     const int nx = d.nx, nz = d.nz;
     const int tp = clock_t_(clk) - 1;
-    const bool snap = sink.snap && tp >= 0 && tp % clk.steps == 0 && tp / clk.steps < sink.nsnap;
+    const int sslot = sink.snap ? out_slot(tp, clk.steps, sink.k0, sink.nsnap) : -1;
+    const bool snap = sslot >= 0;
     for (int j = 0; j < nz; j++) {
         const size_t q = ((size_t)b * nz + j) * nx + x;
         const double v = filter_vars_2d_at(st.vx_unf, q, j, x, nx, nz, d.nh);
         st.a[2][q] = v;
         if (snap)
-            sink.snap[((((size_t)b * sink.nsnap + tp / clk.steps) * sink.nfield +
+            sink.snap[((((size_t)b * sink.nsnap + sslot) * sink.nfield +
                         (sink.adjoint ? 0 : 2)) * nz + j) * nx + x] = v;
     }
 }

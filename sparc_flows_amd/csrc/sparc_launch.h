@@ -14,6 +14,7 @@ hipError_t launch_row(int mode, const Dims &d, const Bg &bg, const State &s, con
 // Segmented column pass; tc in {4, 8, 16} columns per block, ms = rows per
 // segment, one of SPARC_COLSEG_SIZES.
 #define SPARC_COLSEG_SIZES(X) X(4) X(6) X(8) X(10) X(12) X(16) X(19) X(24) X(32)
+hipError_t colseg_init();
 hipError_t launch_colseg(int tc, int ms, const Dims &d, const Bg &bg, const DbyCoef &cf, const SegTab &sg,
                          const State &s, const ColArgs &ca, const Src &src, const Clock &c, const Sink &sk,
                          hipStream_t st);

@@ -131,6 +131,66 @@ struct Solver : torch::CustomClassHolder {
               h, "sparc_run");
         return {vz, sn};
     }
+    // steps t0..t1 into caller-allocated outputs: vz [nsrc][nslots][nx],
+    // snaps [nsrc][nslots][nf][nz][nx] (either may be empty), slot j = output k0 + j
+    void run_range(int64_t t0, int64_t t1, at::Tensor vz, at::Tensor sn, int64_t k0)
+    {
+        int64_t nslots = -1;
+        if (vz.numel()) {
+            check_dev(vz, "vz");
+            TORCH_CHECK(vz.dim() == 3 && vz.size(0) == nsrc && vz.size(2) == nx, "vz must be [nsrc][nslots][nx]");
+            nslots = vz.size(1);
+        }
+        if (sn.numel()) {
+            check_dev(sn, "snaps");
+            TORCH_CHECK(sn.dim() == 5 && sn.size(0) == nsrc && sn.size(2) == (adjoint ? 4 : 6) &&
+                            sn.size(3) == nz && sn.size(4) == nx, "snaps must be [nsrc][nslots][nf][nz][nx]");
+            TORCH_CHECK(nslots < 0 || nslots == sn.size(1), "vz and snaps must have the same slot count");
+            nslots = sn.size(1);
+        }
+        check(sparc_run_range(h, (int)t0, (int)t1, vz.numel() ? vz.data_ptr<double>() : nullptr,
+                              sn.numel() ? sn.data_ptr<double>() : nullptr, (int)k0, (int)std::max<int64_t>(nslots, 0)),
+              h, "sparc_run_range");
+    }
+    int64_t checkpoint_numel()
+    {
+        size_t b = 0;
+        check(sparc_checkpoint_size(h, &b), h, "sparc_checkpoint_size");
+        return (int64_t)(b / sizeof(double));
+    }
+    void save_checkpoint(at::Tensor buf)
+    {
+        check_dev(buf, "checkpoint");
+        TORCH_CHECK(buf.numel() >= checkpoint_numel(), "checkpoint buffer too small");
+        check(sparc_save_checkpoint(h, buf.data_ptr<double>()), h, "sparc_save_checkpoint");
+    }
+    void load_checkpoint(at::Tensor buf)
+    {
+        check_dev(buf, "checkpoint");
+        TORCH_CHECK(buf.numel() >= checkpoint_numel(), "checkpoint buffer too small");
+        check(sparc_load_checkpoint(h, buf.data_ptr<double>()), h, "sparc_load_checkpoint");
+    }
+    void kernels_begin() { check(sparc_kernels_begin(h), h, "sparc_kernels_begin"); }
+    void kernels_add(at::Tensor fwd, at::Tensor adj)
+    {
+        check_dev(fwd, "fwd");
+        check_dev(adj, "adj");
+        TORCH_CHECK(fwd.dim() == 5 && fwd.size(0) == nsrc && fwd.size(2) == 6 && fwd.size(3) == nz &&
+                        fwd.size(4) == nx, "fwd must be [nsrc][n][6][nz][nx]");
+        TORCH_CHECK(adj.dim() == 5 && adj.size(0) == nsrc && adj.size(1) == fwd.size(1) && adj.size(2) == 4,
+                    "adj must be [nsrc][n][4][nz][nx]");
+        check(sparc_kernels_add(h, (int)fwd.size(1), fwd.data_ptr<double>(), adj.data_ptr<double>()), h,
+              "sparc_kernels_add");
+    }
+    std::vector<at::Tensor> kernels_end()
+    {
+        std::vector<at::Tensor> o;
+        for (int i = 0; i < 5; i++) o.push_back(at::empty({nsrc, nz, nx}, opts()));
+        check(sparc_kernels_end(h, o[0].data_ptr<double>(), o[1].data_ptr<double>(), o[2].data_ptr<double>(),
+                                o[3].data_ptr<double>(), o[4].data_ptr<double>()),
+              h, "sparc_kernels_end");
+        return o;
+    }
     void reset() { check(sparc_reset(h), h, "sparc_reset"); }
     void step(int64_t t) { check(sparc_step(h, (int)t), h, "sparc_step"); }
     at::Tensor get_state()
@@ -216,6 +276,13 @@ TORCH_LIBRARY(sparc, m)
         .def("background", &Solver::background)
         .def("set_model", &Solver::set_model)
         .def("set_flow", &Solver::set_flow)
+        .def("run_range", &Solver::run_range)
+        .def("checkpoint_numel", &Solver::checkpoint_numel)
+        .def("save_checkpoint", &Solver::save_checkpoint)
+        .def("load_checkpoint", &Solver::load_checkpoint)
+        .def("kernels_begin", &Solver::kernels_begin)
+        .def("kernels_add", &Solver::kernels_add)
+        .def("kernels_end", &Solver::kernels_end)
         .def("set_forward_sources", &Solver::set_forward_sources)
         .def("set_adjoint_sources", &Solver::set_adjoint_sources)
         .def("run", &Solver::run)

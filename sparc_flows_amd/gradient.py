@@ -14,6 +14,8 @@ source, params.0 of the reference) and time window.
 """
 from __future__ import annotations
 
+import math
+
 import numpy as np
 import torch
 
@@ -70,6 +72,71 @@ class GradientPipeline:
         misfit, vr = waveform_adjoint_source(vz, vz_obs, mask, self.window_s, cad)
         _, asn = self.adj.adjoint(vr, maxtime, snapshots=True)
         K = self.adj.kernels(fsn, asn)
+        K["vz"] = vz
+        K["misfit"] = misfit
+        return K
+
+
+class CheckpointedGradient(GradientPipeline):
+    """The same gradient with fixed-interval checkpoint-and-recompute instead
+    of keeping every forward snapshot.
+
+    The reference dumps the forward snapshots to disk and re-reads them in the
+    kernel step (WRITE_OUT_PARTIAL_STATE, all_modules.f90:750-784;
+    kernels.f90:200-280).  Here the forward run keeps only a full-state
+    checkpoint every `interval` steps (plus the vz traces); the adjoint then
+    advances in its own time while the forward segments are recomputed from
+    their checkpoints in reverse order, and each recomputed segment's
+    snapshots are correlated with the matching adjoint snapshots
+    (forward output k with adjoint output nt-1-k) as soon as both exist.
+    Forward snapshot memory drops from nt to one segment's worth.
+    """
+
+    def __init__(self, params: SparcParams, model: np.ndarray, nsrc: int = 1, c2d=None, psi=None,
+                 window_s: float = 1800.0, interval: int = 100):
+        super().__init__(params, model, nsrc, c2d, psi, window_s)
+        if interval < 1:
+            raise ValueError("checkpoint interval must be positive")
+        self.interval = int(interval)
+
+    def run(self, xsrc, maxtime: int, vz_obs: torch.Tensor | None = None) -> dict:
+        p, C, fwd, adj = self.p, self.interval, self.fwd, self.adj
+        steps = fwd.steps
+        nt = maxtime // steps + 1
+        nseg = (maxtime + 1 + C - 1) // C
+        dev = fwd.device
+        fwd.set_forward_sources(xsrc, maxtime)
+        fwd.reset()
+        ck = fwd.checkpoint_buffer(nseg)
+        vz = torch.zeros(self.nsrc, nt, p.nx, dtype=torch.float64, device=dev)
+        for j in range(nseg):
+            t0, t1 = j * C, min(maxtime, j * C + C - 1)
+            fwd.save_checkpoint(ck[j])
+            fwd.run_range(t0, t1, vz=vz, k0=0)
+        mask = receiver_mask(self.p, xsrc)
+        misfit, vr = waveform_adjoint_source(vz, vz_obs, mask, self.window_s, steps * p.timestep)
+        adj.set_adjoint_sources(vr, maxtime)
+        adj.reset()
+        adj.kernels_begin()
+        ta = 0
+        kmax = (C + steps - 1) // steps + 1
+        fbuf = torch.empty(self.nsrc, kmax, 6, p.nz, p.nx, dtype=torch.float64, device=dev)
+        abuf = torch.empty(self.nsrc, kmax, 4, p.nz, p.nx, dtype=torch.float64, device=dev)
+        for j in reversed(range(nseg)):
+            t0, t1 = j * C, min(maxtime, j * C + C - 1)
+            ka, kb = math.ceil(t0 / steps), t1 // steps + 1      # forward outputs in this segment
+            n = kb - ka
+            if n <= 0:
+                continue
+            fs = fbuf[:, :n].contiguous() if n < kmax else fbuf
+            fwd.load_checkpoint(ck[j])
+            fwd.run_range(t0, t1, snaps=fs, k0=ka)
+            tend = (nt - 1 - ka) * steps                        # adjoint outputs nt-kb .. nt-1-ka
+            as_ = abuf[:, :n].contiguous() if n < kmax else abuf
+            adj.run_range(ta, tend, snaps=as_, k0=nt - kb)
+            ta = tend + 1
+            adj.kernels_add(fs, as_)
+        K = adj.kernels_end()
         K["vz"] = vz
         K["misfit"] = misfit
         return K

@@ -134,6 +134,33 @@ class Sparc:
         self.set_adjoint_sources(vr, maxtime)
         return self.run(snapshots)
 
+    def run_range(self, t0: int, t1: int, vz: torch.Tensor | None = None, snaps: torch.Tensor | None = None,
+                  k0: int = 0) -> None:
+        """Steps t0..t1 from the current state (which must be at t0) into
+        caller-allocated slot buffers; slot j holds output index k0 + j."""
+        empty = torch.empty(0, dtype=torch.float64, device=self.device)
+        self._stream().run_range(int(t0), int(t1), empty if vz is None else vz,
+                                 empty if snaps is None else snaps, int(k0))
+
+    def checkpoint_buffer(self, n: int = 1) -> torch.Tensor:
+        """Device storage for n full-state checkpoints, [n][numel]."""
+        return torch.empty(n, self._stream().checkpoint_numel(), dtype=torch.float64, device=self.device)
+
+    def save_checkpoint(self, buf: torch.Tensor) -> None:
+        self._stream().save_checkpoint(buf)
+
+    def load_checkpoint(self, buf: torch.Tensor) -> None:
+        self._stream().load_checkpoint(buf)
+
+    def kernels_begin(self) -> None:
+        self._stream().kernels_begin()
+
+    def kernels_add(self, fwd_snaps: torch.Tensor, adj_snaps: torch.Tensor) -> None:
+        self._stream().kernels_add(fwd_snaps.contiguous(), adj_snaps.contiguous())
+
+    def kernels_end(self) -> dict:
+        return dict(zip(("kc", "kpsi", "kvx", "kvz", "hess"), self._stream().kernels_end()))
+
     def reset(self) -> None:
         self._stream().reset()
 
