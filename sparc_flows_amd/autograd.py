@@ -41,6 +41,7 @@ import math
 import numpy as np
 import torch
 
+from .gradient import forward_with_checkpoints, kernels_from_checkpoints
 from .params import SparcParams
 from .solver import Sparc
 
@@ -93,9 +94,13 @@ class _Solve(torch.autograd.Function):
     @staticmethod
     def forward(ctx, psi, c, pipe, xsrc, maxtime):
         pipe.set_model(c, psi)
-        vz, fsn = pipe.fwd.forward(xsrc, maxtime, snapshots=True)
-        ctx.pipe, ctx.maxtime = pipe, maxtime
-        ctx.fsn = fsn
+        if pipe.interval:
+            vz, ctx.ck = forward_with_checkpoints(pipe.fwd, xsrc, maxtime, pipe.interval)
+            ctx.fsn = None
+        else:
+            vz, ctx.fsn = pipe.fwd.forward(xsrc, maxtime, snapshots=True)
+            ctx.ck = None
+        ctx.pipe, ctx.maxtime, ctx.model_id = pipe, maxtime, pipe.model_id
         ctx.save_for_backward(psi, c)
         return vz
 
@@ -103,10 +108,16 @@ class _Solve(torch.autograd.Function):
     def backward(ctx, gvz):
         psi, c = ctx.saved_tensors
         pipe = ctx.pipe
+        if pipe.model_id != ctx.model_id:        # another solve changed the model since
+            pipe.set_model(c, psi)
         vr = pipe.adjoint_source_from_grad(gvz)
-        _, asn = pipe.adj.adjoint(vr, ctx.maxtime, snapshots=True)
-        K = pipe.adj.kernels(ctx.fsn, asn)
-        ctx.fsn = None
+        if ctx.ck is not None:
+            # the forward model of this solve: checkpoints were taken with it
+            K = kernels_from_checkpoints(pipe.fwd, pipe.adj, ctx.ck, vr, ctx.maxtime, pipe.interval)
+        else:
+            _, asn = pipe.adj.adjoint(vr, ctx.maxtime, snapshots=True)
+            K = pipe.adj.kernels(ctx.fsn, asn)
+        ctx.fsn = ctx.ck = None
         gpsi, gc = kernel_to_gradient(pipe.p, pipe.model, K, psi, c)
         return gpsi.to(psi.device), gc.to(c.device), None, None, None
 
@@ -116,10 +127,13 @@ class DifferentiableSolver:
 
     ``solve(psi, c)`` returns vz [nsrc][nout][nx] (non-dimensional, the
     vz_cc records) and is differentiable with respect to psi [nz][nx]
-    (model_psi) and c [nz][nx] (cm/s).
+    (model_psi) and c [nz][nx] (cm/s).  With ``checkpoint_interval`` n > 0
+    the forward pass keeps a full-state checkpoint every n steps instead of
+    the snapshots, and the backward pass recomputes them segment by segment.
     """
 
-    def __init__(self, params: SparcParams, model: np.ndarray, xsrc, maxtime: int):
+    def __init__(self, params: SparcParams, model: np.ndarray, xsrc, maxtime: int,
+                 checkpoint_interval: int = 0):
         if params.steps != params.cadforcing_step:
             raise ValueError("the adjoint source is sampled at the output cadence: "
                              "outputcad and cadforcing must give the same step count")
@@ -132,12 +146,17 @@ class DifferentiableSolver:
         self.adj = Sparc(params, model, nsrc=n, adjoint=True)
         self.dx, _ = grid_cells(params, model)
         self.dT = params.steps * params.timestep * DIMC / RSUN
+        # 0: keep every forward snapshot; n > 0: checkpoint every n steps and
+        # recompute the snapshots during the backward pass
+        self.interval = int(checkpoint_interval)
+        self.model_id = 0
 
     def set_model(self, c, psi) -> None:
         cn = c.detach().cpu().numpy() if torch.is_tensor(c) else np.asarray(c)
         pn = psi.detach().cpu().numpy() if torch.is_tensor(psi) else np.asarray(psi)
         self.fwd.set_model(cn, pn)
         self.adj.set_model(cn, pn)
+        self.model_id += 1
 
     def adjoint_source_from_grad(self, gvz: torch.Tensor) -> torch.Tensor:
         """vr [nsrc][nt][nx]: time-reversed dL/dvz as a force density."""

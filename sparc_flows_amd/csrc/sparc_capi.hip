@@ -62,6 +62,7 @@ struct sparc_handle {
     // host background (kept for queries)
     std::vector<double> x, eyekx, z, stretch, unstretch, g, c2, rho0, p0, rhoinv, c2rho0, gp0, gr0,
         v0x, v0z, azf, bzf, az, bz, axpml, bxpml, fwdsource;
+    std::vector<double> gp0z, gr0z, fdec, fz;   // unscaled d p0/dz, d rho0/dz; PML factor (full, PML rows)
     int e_rad = 0, o_rad = 0, steps = 1, cfs = 1;
     double delta_width = 0, deltat = 0, stretchx = 0, betas[5] = {0, 0, 0, 0, 0};
     DbyCoef cf{};
@@ -380,6 +381,15 @@ static int setup_background(sparc_handle *h)
     if (r) return r;
     r = host_ddz(h, h->p0, h->gp0, nx, 1, 1, true, 1);        // :231
     if (r) return r;
+    // x-independent profiles before the PML scales them (the column pass
+    // applies the PML factor functiondecay(x, z) itself)
+    h->gp0z.assign(nz, 0.0);
+    h->gr0z.assign(nz, 0.0);
+    for (int k = 0; k < nz; k++) {
+        h->gp0z[k] = h->gp0[(size_t)k * nx];
+        h->gr0z[k] = h->gr0[(size_t)k * nx];
+    }
+    h->fdec.assign(n2, 1.0);
     // vertical PML (physics2d.f90:288-333)
     const int nb = h->nb, nt = h->nt;
     h->azf.assign(n2, 0.0);
@@ -405,6 +415,7 @@ static int setup_background(sparc_handle *h)
             }
             if (inpml) {
                 const double fd = alphaz / (dampz + alphaz);
+                h->fdec[q] = fd;
                 h->gp0[q] *= fd;
                 h->gr0[q] *= fd;
                 if (i == 0) h->g[k] *= fd;
@@ -416,11 +427,13 @@ static int setup_background(sparc_handle *h)
     const int nzpml = nb + nt;
     h->az.assign((size_t)nzpml * nx, 0.0);
     h->bz.assign((size_t)nzpml * nx, 0.0);
+    h->fz.assign((size_t)nzpml * nx, 1.0);
     for (int pj = 0; pj < nzpml; pj++) {
         const int k = pj < nb ? pj : nz - nt + (pj - nb);
         for (int i = 0; i < nx; i++) {
             h->az[(size_t)pj * nx + i] = h->azf[(size_t)k * nx + i];
             h->bz[(size_t)pj * nx + i] = h->bzf[(size_t)k * nx + i];
+            h->fz[(size_t)pj * nx + i] = h->fdec[(size_t)k * nx + i];
         }
     }
     // horizontal PML (physics2d.f90:339-357), c of column x = 1
@@ -535,6 +548,7 @@ static int upload_background(sparc_handle *h)
     HCHECK(h, upload(h->bg_mem, &p, h->g, h->stream)); h->bg.g = p;
     HCHECK(h, upload(h->bg_mem, &p, h->az, h->stream)); h->bg.az = p;
     HCHECK(h, upload(h->bg_mem, &p, h->bz, h->stream)); h->bg.bz = p;
+    HCHECK(h, upload(h->bg_mem, &p, h->fz, h->stream)); h->bg.fz = p;
     HCHECK(h, upload(h->bg_mem, &p, h->axpml, h->stream)); h->bg.ax = p;
     HCHECK(h, upload(h->bg_mem, &p, h->bxpml, h->stream)); h->bg.bx = p;
     const int r = h->adjoint ? h->o_rad : h->e_rad;
@@ -600,12 +614,12 @@ static int build_seg(sparc_handle *h, int m, SegTab &sg)
         ztab[(size_t)0 * nzp + j] = h->stretch[j];
         ztab[(size_t)1 * nzp + j] = h->g[j];
         ztab[(size_t)17 * nzp + j] = h->rho0[q];
-        ztab[(size_t)18 * nzp + j] = h->gr0[q];
-        ztab[(size_t)19 * nzp + j] = h->gp0[q];
+        ztab[(size_t)18 * nzp + j] = h->gr0z[j];
+        ztab[(size_t)19 * nzp + j] = h->gp0z[j];
         ztab[(size_t)20 * nzp + j] = h->rhoinv[q];
         for (int i = 1; i < h->nx; i++)
-            if (h->rho0[q + i] != h->rho0[q] || h->gr0[q + i] != h->gr0[q] || h->gp0[q + i] != h->gp0[q] ||
-                h->rhoinv[q + i] != h->rhoinv[q])
+            if (h->rho0[q + i] != h->rho0[q] || h->rhoinv[q + i] != h->rhoinv[q] ||
+                h->gr0[q + i] != h->gr0z[j] * h->fdec[q + i] || h->gp0[q + i] != h->gp0z[j] * h->fdec[q + i])
                 return fail(h, SPARC_E_ARG, "background density / pressure vary along x");
     }
     const int n2 = 2 * P;
@@ -658,16 +672,27 @@ static int build_seg(sparc_handle *h, int m, SegTab &sg)
             for (int q = 0; q < n2; q++)
                 if (std::fabs(Minv[(size_t)k * n2 + q]) > 1e-24 * rm) hw = std::max(hw, std::abs(q / 2 - k / 2));
         }
-        minv_all.insert(minv_all.end(), Minv.begin(), Minv.end());
+        minv_all.insert(minv_all.end(), Minv.begin(), Minv.end());   // banded below, once hw is known
         const std::vector<double> *tabs[5] = {&lowr, &rbl, &gl, &V, &W};
         for (int k = 0; k < 5; k++)
             std::copy(tabs[k]->begin(), tabs[k]->end(), ztab.begin() + (size_t)(2 + 5 * slot + k) * nzp);
     }
     sg.hw = hw;
+    // band storage: row k keeps the entries q = 2 (k/2 - hw) + w, w < 4 hw + 4
+    // (the window around its own segment), zero outside 0..2P-1
+    const int W = 4 * hw + 4;
+    std::vector<double> band((size_t)3 * n2 * W, 0.0);
+    for (int slot = 0; slot < 3; slot++)
+        for (int k = 0; k < n2; k++)
+            for (int w = 0; w < W; w++) {
+                const int q = 2 * (k / 2 - hw) + w;
+                if (q >= 0 && q < n2)
+                    band[((size_t)slot * n2 + k) * W + w] = minv_all[((size_t)slot * n2 + k) * n2 + q];
+            }
     double *p = nullptr;
     HCHECK(h, upload(h->bg_mem, &p, ztab, h->stream));
     sg.ztab = p;
-    HCHECK(h, upload(h->bg_mem, &p, minv_all, h->stream));
+    HCHECK(h, upload(h->bg_mem, &p, band, h->stream));
     sg.Minv = p;
     return SPARC_OK;
 }
@@ -694,6 +719,10 @@ static int setup_seg(sparc_handle *h)
     h->seg = sg;
     h->seg_ms = m;
     h->seg_cands.push_back(sg);
+    // untuned block width: the widest that fits
+    h->seg_tc = 4;
+    for (int tc : {8, 16})
+        if (h->nx % tc == 0 && tc * sg.P <= 256) h->seg_tc = tc;
     return SPARC_OK;
 }
 
@@ -1213,7 +1242,10 @@ static int autotune(sparc_handle *h)
         for (int tc : {4, 8, 16}) {
             if (force_tc && tc != force_tc) continue;
             if (h->nx % tc || tc * sg.P > 256) continue;
-            HCHECK(h, launch_colseg(tc, m, h->d, h->bg, h->cf, sg, s, ca, h->src, clk, sk, h->stream));
+            if (launch_colseg(tc, m, h->d, h->bg, h->cf, sg, s, ca, h->src, clk, sk, h->stream) != hipSuccess) {
+                (void)hipGetLastError();     // shape not launchable here (LDS): skip it
+                continue;
+            }
             HCHECK(h, hipEventRecord(e0, h->stream));
             for (int r = 0; r < 10; r++)
                 HCHECK(h, launch_colseg(tc, m, h->d, h->bg, h->cf, sg, s, ca, h->src, clk, sk, h->stream));

@@ -71,38 +71,39 @@ __device__ __forceinline__ void seg_solve(double (&Y)[MS], const double *__restr
     for (int i = MS - 2; i >= 0; i--) Y[i] = Y[i] - gl[i + 1] * Y[i + 1];
 }
 
-// y <- y - L_{s-1} V - F_{s+1} W; interface values from Minv over a window.
+// y <- y - L_{s-1} V - F_{s+1} W; the interface values are rows of the
+// banded inverse (row k holds entries q = 2 (k/2 - hw) + w, w < 4 hw + 4)
+// against the segment end values y.
 template <int MS>
 __device__ __forceinline__ void seg_correct(double (&Y)[MS], int s, int P, int hw, const double *y,
-                                            const double *mvt, const double *__restrict__ V,
+                                            const double *band, const double *__restrict__ V,
                                             const double *__restrict__ W)
 {
-    const int n2P = 2 * P;
-    // interface values L_{s-1} (row 2s-1 of Minv) and F_{s+1} (row 2s+2),
-    // each a dot product over the window of segments that can contribute
-    auto dot = [&](int k, int centre) -> double {
-        const double *row = mvt + k * n2P;
-        const int q0 = max(0, 2 * (centre - hw)), q1 = min(n2P, 2 * (centre + hw) + 2);
+    const int n2P = 2 * P, nw = 4 * hw + 4;
+    auto dot = [&](int k) -> double {
+        const double *row = band + k * nw;
+        const int qb = 2 * (k / 2 - hw);
         double acc0 = 0.0, acc1 = 0.0;
-        int q = q0;
-        for (; q + 1 < q1; q += 2) {
-            acc0 += row[q] * y[q];
-            acc1 += row[q + 1] * y[q + 1];
+        for (int w = 0; w < nw; w += 2) {   // nw is even; entries outside 0..2P-1 are zero
+            acc0 += row[w] * y[min(max(qb + w, 0), n2P - 1)];
+            acc1 += row[w + 1] * y[min(max(qb + w + 1, 0), n2P - 1)];
         }
-        if (q < q1) acc0 += row[q] * y[q];
         return acc0 + acc1;
     };
-    const double Lp = s > 0 ? dot(2 * s - 1, s - 1) : 0.0;
-    const double Fn = s < P - 1 ? dot(2 * s + 2, s + 1) : 0.0;
+    const double Lp = s > 0 ? dot(2 * s - 1) : 0.0;
+    const double Fn = s < P - 1 ? dot(2 * s + 2) : 0.0;
 #pragma unroll
     for (int i = 0; i < MS; i++) Y[i] = Y[i] - Lp * V[i] - Fn * W[i];
 }
 
 }  // namespace
 
-// Dynamic LDS of one block: the three interface inverses Minv [3][2P][2P],
-// then the four end-value exchange buffers [4][TC][2P].
-inline size_t colseg_lds_bytes(int tc, int P) { return sizeof(double) * (size_t)(3 * 4 * P * P + 4 * tc * 2 * P); }
+// Dynamic LDS of one block: the three banded interface inverses
+// [3][2P][4hw+4], then the four end-value exchange buffers [4][TC][2P].
+inline size_t colseg_lds_bytes(int tc, int P, int hw)
+{
+    return sizeof(double) * (size_t)(3 * 2 * P * (4 * hw + 4) + 4 * tc * 2 * P);
+}
 
 template <int MS, int LAST>
 __global__ __launch_bounds__(256) void k_colseg(Dims d, Bg bg, DbyCoef cf, SegTab sg, State st,
@@ -111,11 +112,23 @@ __global__ __launch_bounds__(256) void k_colseg(Dims d, Bg bg, DbyCoef cf, SegTa
     extern __shared__ double lds_dyn[];
     const int P = sg.P, n2P = 2 * P, nzp = sg.nzp;
     const int TC = blockDim.x / P;
+    const int nband = n2P * (4 * sg.hw + 4);    // one banded inverse
     double *const mv = lds_dyn;
-    double *const yb0 = lds_dyn + 3 * n2P * n2P;
+    double *const yb0 = lds_dyn + 3 * nband;
     double *const yb[4] = {yb0, yb0 + TC * n2P, yb0 + 2 * TC * n2P, yb0 + 3 * TC * n2P};
     const int tid = threadIdx.x;
-    for (int q = tid; q < 3 * n2P * n2P; q += blockDim.x) mv[q] = sg.Minv[q];
+    {
+        // eight loads in flight per thread and round
+        const int nb3 = 3 * nband, stride = blockDim.x;
+        for (int q0 = tid; q0 < nb3; q0 += 8 * stride) {
+            double v[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) v[u] = sg.Minv[min(q0 + u * stride, nb3 - 1)];
+#pragma unroll
+            for (int u = 0; u < 8; u++)
+                if (q0 + u * stride < nb3) mv[q0 + u * stride] = v[u];
+        }
+    }
     const int c = tid % TC, s = tid / TC;
     const int x = blockIdx.x * TC + c, b = blockIdx.y;
     const int nx = d.nx, nz = d.nz;
@@ -168,7 +181,7 @@ __global__ __launch_bounds__(256) void k_colseg(Dims d, Bg bg, DbyCoef cf, SegTa
     seg_solve<MS>(Y, zt + zrow(2, 0) * nzp, zt + zrow(2, 1) * nzp, zt + zrow(2, 2) * nzp);
     if (act) { yb[0][c * n2P + 2 * s] = Y[0]; yb[0][c * n2P + 2 * s + 1] = Y[MS - 1]; }
     __syncthreads();
-    if (act) seg_correct<MS>(Y, s, P, sg.hw, yb[0] + c * n2P, mv + 2 * n2P * n2P, zt + zrow(2, 3) * nzp,
+    if (act) seg_correct<MS>(Y, s, P, sg.hw, yb[0] + c * n2P, mv + 2 * nband, zt + zrow(2, 3) * nzp,
                              zt + zrow(2, 4) * nzp);
 #pragma unroll
     for (int i = 0; i < MS; i++) {
@@ -178,10 +191,13 @@ __global__ __launch_bounds__(256) void k_colseg(Dims d, Bg bg, DbyCoef cf, SegTa
         const size_t qp = pz0 + (size_t)pj * nx, qb = (size_t)pj * nx + x;
         const double psi = st.pz_tmp[qp], az = bg.az[qb], bzc = bg.bz[qb], pbase = st.pz_base[qp];
         const double dxix = st.dxixdx[qj], xiz = ca.T[1][qj];
-        const double c2r = bg.c2rho0[gq], gp0 = zt[Z_GP * nzp + i], r0 = zt[Z_RHO * nzp + i];
-        const double gr0 = zt[Z_GRHO * nzp + i], rinv = zt[Z_RINV * nzp + i], gz = zt[Z_G * nzp + i];
-        double dz = Y[i] * zt[Z_STRETCH * nzp + i];
+        const double c2r = bg.c2rho0[gq], r0 = zt[Z_RHO * nzp + i];
+        const double rinv = zt[Z_RINV * nzp + i], gz = zt[Z_G * nzp + i];
         const bool pm = is_pml(j);
+        // functiondecay scales d p0/dz and d rho0/dz in the PML rows (physics2d.f90:303-318)
+        const double fdec = pm ? bg.fz[qb] : 1.0;
+        const double gp0 = zt[Z_GP * nzp + i] * fdec, gr0 = zt[Z_GRHO * nzp + i] * fdec;
+        double dz = Y[i] * zt[Z_STRETCH * nzp + i];
         const double prhs = az * dz + bzc * psi;
         dz = pm ? dz + psi : dz;
         const double pnew = pbase + ca.bnext * prhs;
@@ -238,7 +254,7 @@ __global__ __launch_bounds__(256) void k_colseg(Dims d, Bg bg, DbyCoef cf, SegTa
     seg_solve<MS>(Y, zt + zrow(1, 0) * nzp, zt + zrow(1, 1) * nzp, zt + zrow(1, 2) * nzp);
     if (act) { yb[3][c * n2P + 2 * s] = Y[0]; yb[3][c * n2P + 2 * s + 1] = Y[MS - 1]; }
     __syncthreads();
-    if (act) seg_correct<MS>(Y, s, P, sg.hw, yb[3] + c * n2P, mv + n2P * n2P, zt + zrow(1, 3) * nzp,
+    if (act) seg_correct<MS>(Y, s, P, sg.hw, yb[3] + c * n2P, mv + nband, zt + zrow(1, 3) * nzp,
                              zt + zrow(1, 4) * nzp);
     const int sslot = (LAST && sink.snap) ? out_slot(time, clk.steps, sink.k0, sink.nsnap) : -1;
     const bool snap = sslot >= 0;
@@ -321,19 +337,26 @@ __global__ __launch_bounds__(256) void k_colseg(Dims d, Bg bg, DbyCoef cf, SegTa
     }
 }
 
-// Allows each instantiation the dynamic LDS of the largest block shape
-// (16 columns, kSegMax segments).
+namespace {
+int g_lds_max = 64 * 1024;   // dynamic LDS a block may use on this device
+}
+
+// Raises every instantiation's dynamic-LDS limit to the device's maximum per
+// block (gfx950: 160 KB).
 hipError_t colseg_init()
 {
-    const int bytes = (int)colseg_lds_bytes(16, kSegMax);
+    int dev = 0, v = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && v > 0)
+        g_lds_max = v;
 #define A(MSV)                                                                                          \
-    {                                                                                                   \
-        hipError_t e = hipFuncSetAttribute((const void *)k_colseg<MSV, 0>,                              \
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, bytes);          \
-        if (e != hipSuccess) return e;                                                                  \
-        e = hipFuncSetAttribute((const void *)k_colseg<MSV, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, \
-                                bytes);                                                                 \
-        if (e != hipSuccess) return e;                                                                  \
+    if (hipFuncSetAttribute((const void *)k_colseg<MSV, 0>, hipFuncAttributeMaxDynamicSharedMemorySize, \
+                            g_lds_max) != hipSuccess ||                                                 \
+        hipFuncSetAttribute((const void *)k_colseg<MSV, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, \
+                            g_lds_max) != hipSuccess) {                                                 \
+        (void)hipGetLastError();                                                                        \
+        g_lds_max = 64 * 1024;                                                                          \
     }
     SPARC_COLSEG_SIZES(A)
 #undef A
@@ -347,7 +370,8 @@ hipError_t launch_colseg(int tc, int ms, const Dims &d, const Bg &bg, const DbyC
     if ((tc != 4 && tc != 8 && tc != 16) || d.nx % tc || sg.m != ms || sg.P > kSegMax || tc * sg.P > 256)
         return hipErrorInvalidValue;
     dim3 grid(d.nx / tc, d.S), block(tc * sg.P);
-    const size_t shm = colseg_lds_bytes(tc, sg.P);
+    const size_t shm = colseg_lds_bytes(tc, sg.P, sg.hw);
+    if (shm > (size_t)g_lds_max) return hipErrorInvalidValue;
 #define L(MSV)                                                                                         \
     if (ms == MSV) {                                                                                   \
         if (ca.last) hipLaunchKernelGGL((k_colseg<MSV, 1>), grid, block, shm, st, d, bg, cf, sg, s, ca, src, c, sk); \

@@ -23,6 +23,7 @@ struct Bg {
     const double *rhoinv_z;                                        // [nz]: 1/rho0 is x-independent
     const double *g, *stretch, *unstretch;                 // [nz]
     const double *az, *bz;                                 // [nzpml][nx]
+    const double *fz;                                      // [nzpml][nx] PML factor of d p0/dz, d rho0/dz
     const double *ax, *bx;                                 // [nz][nph]
     const double *rho0_src;                                // [nx] rho0 at the injection row
     const double *eyekx;                                   // [nx/2+1]
@@ -60,7 +61,7 @@ constexpr int kZT = 21;
 struct SegTab {
     int P, m, hw, nzp;              // segments, rows per segment, Minv window, P*m
     const double *ztab;             // [kZT][nzp]
-    const double *Minv;             // [3][2P][2P]
+    const double *Minv;             // [3][2P][4hw+4]: row k of each inverse, entries 2(k/2-hw)+w
 };
 
 // Prognostic state and work arrays of a batch (device pointers).

@@ -77,6 +77,57 @@ class GradientPipeline:
         return K
 
 
+def forward_with_checkpoints(fwd: Sparc, xsrc, maxtime: int, interval: int):
+    """Forward run keeping the vz traces and a full-state checkpoint every
+    `interval` steps (checkpoint j holds the state at time index j*interval).
+    Returns (vz [nsrc][nt][nx], checkpoints [nseg][numel])."""
+    steps = fwd.steps
+    nt = maxtime // steps + 1
+    nseg = (maxtime + 1 + interval - 1) // interval
+    fwd.set_forward_sources(xsrc, maxtime)
+    fwd.reset()
+    ck = fwd.checkpoint_buffer(nseg)
+    vz = torch.zeros(fwd.nsrc, nt, fwd.p.nx, dtype=torch.float64, device=fwd.device)
+    for j in range(nseg):
+        fwd.save_checkpoint(ck[j])
+        fwd.run_range(j * interval, min(maxtime, j * interval + interval - 1), vz=vz, k0=0)
+    return vz, ck
+
+
+def kernels_from_checkpoints(fwd: Sparc, adj: Sparc, ck: torch.Tensor, vr: torch.Tensor, maxtime: int,
+                             interval: int) -> dict:
+    """Adjoint run driven by vr with the forward snapshots recomputed segment
+    by segment, in reverse, from the checkpoints; forward output k is
+    correlated with adjoint output nt-1-k (kernels.f90:200-360) as soon as
+    both exist.  Returns the kernels of sparc_kernels_end."""
+    p, C, steps = fwd.p, interval, fwd.steps
+    nt = maxtime // steps + 1
+    nseg = ck.shape[0]
+    dev = fwd.device
+    adj.set_adjoint_sources(vr, maxtime)
+    adj.reset()
+    adj.kernels_begin()
+    ta = 0
+    kmax = (C + steps - 1) // steps + 1
+    fbuf = torch.empty(fwd.nsrc, kmax, 6, p.nz, p.nx, dtype=torch.float64, device=dev)
+    abuf = torch.empty(fwd.nsrc, kmax, 4, p.nz, p.nx, dtype=torch.float64, device=dev)
+    for j in reversed(range(nseg)):
+        t0, t1 = j * C, min(maxtime, j * C + C - 1)
+        ka, kb = math.ceil(t0 / steps), t1 // steps + 1      # forward outputs in this segment
+        n = kb - ka
+        if n <= 0:
+            continue
+        fs = fbuf[:, :n].contiguous() if n < kmax else fbuf
+        fwd.load_checkpoint(ck[j])
+        fwd.run_range(t0, t1, snaps=fs, k0=ka)
+        tend = (nt - 1 - ka) * steps                        # adjoint outputs nt-kb .. nt-1-ka
+        as_ = abuf[:, :n].contiguous() if n < kmax else abuf
+        adj.run_range(ta, tend, snaps=as_, k0=nt - kb)
+        ta = tend + 1
+        adj.kernels_add(fs, as_)
+    return adj.kernels_end()
+
+
 class CheckpointedGradient(GradientPipeline):
     """The same gradient with fixed-interval checkpoint-and-recompute instead
     of keeping every forward snapshot.
@@ -100,43 +151,10 @@ class CheckpointedGradient(GradientPipeline):
         self.interval = int(interval)
 
     def run(self, xsrc, maxtime: int, vz_obs: torch.Tensor | None = None) -> dict:
-        p, C, fwd, adj = self.p, self.interval, self.fwd, self.adj
-        steps = fwd.steps
-        nt = maxtime // steps + 1
-        nseg = (maxtime + 1 + C - 1) // C
-        dev = fwd.device
-        fwd.set_forward_sources(xsrc, maxtime)
-        fwd.reset()
-        ck = fwd.checkpoint_buffer(nseg)
-        vz = torch.zeros(self.nsrc, nt, p.nx, dtype=torch.float64, device=dev)
-        for j in range(nseg):
-            t0, t1 = j * C, min(maxtime, j * C + C - 1)
-            fwd.save_checkpoint(ck[j])
-            fwd.run_range(t0, t1, vz=vz, k0=0)
+        vz, ck = forward_with_checkpoints(self.fwd, xsrc, maxtime, self.interval)
         mask = receiver_mask(self.p, xsrc)
-        misfit, vr = waveform_adjoint_source(vz, vz_obs, mask, self.window_s, steps * p.timestep)
-        adj.set_adjoint_sources(vr, maxtime)
-        adj.reset()
-        adj.kernels_begin()
-        ta = 0
-        kmax = (C + steps - 1) // steps + 1
-        fbuf = torch.empty(self.nsrc, kmax, 6, p.nz, p.nx, dtype=torch.float64, device=dev)
-        abuf = torch.empty(self.nsrc, kmax, 4, p.nz, p.nx, dtype=torch.float64, device=dev)
-        for j in reversed(range(nseg)):
-            t0, t1 = j * C, min(maxtime, j * C + C - 1)
-            ka, kb = math.ceil(t0 / steps), t1 // steps + 1      # forward outputs in this segment
-            n = kb - ka
-            if n <= 0:
-                continue
-            fs = fbuf[:, :n].contiguous() if n < kmax else fbuf
-            fwd.load_checkpoint(ck[j])
-            fwd.run_range(t0, t1, snaps=fs, k0=ka)
-            tend = (nt - 1 - ka) * steps                        # adjoint outputs nt-kb .. nt-1-ka
-            as_ = abuf[:, :n].contiguous() if n < kmax else abuf
-            adj.run_range(ta, tend, snaps=as_, k0=nt - kb)
-            ta = tend + 1
-            adj.kernels_add(fs, as_)
-        K = adj.kernels_end()
+        misfit, vr = waveform_adjoint_source(vz, vz_obs, mask, self.window_s, self.fwd.steps * self.p.timestep)
+        K = kernels_from_checkpoints(self.fwd, self.adj, ck, vr, maxtime, self.interval)
         K["vz"] = vz
         K["misfit"] = misfit
         return K

@@ -124,3 +124,26 @@ def read_params_i(path: str | Path) -> SparcParams:
     if fd:
         p.file_data = fd.group(1)
     return p
+
+
+def write_params_i(p: SparcParams, path: str | Path, directory: str = "") -> None:
+    """Write a minimal params.i with the reference's PARAMETER statement forms
+    (params.i:26-152) for the values :func:`read_params_i` reads."""
+    def d(v: float) -> str:
+        r = repr(float(v))
+        return r.replace("e", "d") if "e" in r else r + "d0"
+    lines = [
+        f"character (LEN = *), PARAMETER :: directory = '{directory}'",
+        f"character (LEN = *), PARAMETER :: file_data = '{p.file_data}'",
+        f"parameter (nx = {p.nx}, ny = 1, nz = {p.nz})",
+        f"parameter (xlength = {d(p.xlength)}, ylength = xlength, timestep = {d(p.timestep)})",
+        f"parameter( cadforcing = {d(p.cadforcing)})",
+        f"parameter(nupeak={repr(float(p.nupeak))}, nuwidth = {repr(float(p.nuwidth))})",
+        f"parameter (npmlhor = {p.npmlhor})",
+        f"parameter (npmlbot = {p.npmlbot}, npmltop = {p.npmltop})",
+        f"parameter (solartime = {d(p.solartime)})",
+        f"parameter (obsheight = {d(p.obsheight)})",
+        f"parameter (outputcad = {d(p.outputcad)})",
+        f"parameter (excitdep = {d(p.excitdep)})",
+    ]
+    Path(path).write_text("\n".join(lines) + "\n")

@@ -1,0 +1,103 @@
+"""The `sparc SS LL` drop-in (sparc_flows_amd.cli): the reference's command
+line, instruction and flag files, FITS outputs and exit code.
+
+The CPU tests cover the parameter file and the argument/instruction
+handling; the GPU test runs the three phases of full.py:17-61 (forward,
+adjoint, kernel) as child processes in a scratch directory and compares the
+files they write with the same computation through the Python API.
+"""
+import os
+import subprocess
+import sys
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from sparc_flows_amd import background as B
+from sparc_flows_amd.cli import main
+from sparc_flows_amd.fitsio import read_fits, write_fits
+from sparc_flows_amd.params import SparcParams, read_params_i, write_params_i
+
+ROOT = Path(__file__).resolve().parents[1]
+
+
+def _setup(tmp: Path, nx=64, nz=60):
+    bg = B.polytrope(nz)
+    p = SparcParams(nx=nx, nz=nz, xlength=200e8, timestep=B.stable_timestep(bg), npmlbot=6, npmltop=6,
+                    npmlhor=5, file_data="poly.dat")
+    data = tmp / "data"
+    data.mkdir()
+    B.write_model_file(tmp / "poly.dat", bg)
+    write_params_i(p, tmp / "params.i", str(data) + "/")
+    (data / "master.pixels").write_text("-12.0\n7.5\n")
+    psi = B.supergranule_psi(nx, p.xlength, bg)
+    write_fits(data / "model_psi_ls00.fits", psi[:, None, :])
+    return p, bg, psi, data
+
+
+def test_params_file_round_trip(tmp_path):
+    p, _, _, _ = _setup(tmp_path)
+    assert read_params_i(tmp_path / "params.i") == p
+
+
+def test_usage_and_unknown_instruction(tmp_path, monkeypatch, capsys):
+    assert main([]) == 1
+    _setup(tmp_path)
+    monkeypatch.chdir(tmp_path)
+    (tmp_path / "Instruction_src01_ls00").write_text("bogus\n")
+    assert main(["01", "00"]) == 1
+    assert not (tmp_path / "Instruction_src01_ls00").exists()   # read and removed (driver.f90:243)
+
+
+def _cli(tmp: Path, data: Path, maxtime: int, *args):
+    env = dict(os.environ, SPARC_DIRECTORY=str(data) + "/", SPARC_MAXTIME=str(maxtime),
+               PYTHONPATH=str(ROOT))
+    r = subprocess.run([sys.executable, "-m", "sparc_flows_amd.cli", *args], cwd=tmp, env=env,
+                       capture_output=True, text=True, timeout=300)
+    print(r.stdout, r.stderr)
+    return r.returncode
+
+
+@pytest.mark.gpu
+def test_forward_adjoint_kernel_phases(tmp_path):
+    torch = pytest.importorskip("torch")
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from sparc_flows_amd.solver import Sparc
+    p, bg, psi, data = _setup(tmp_path)
+    maxtime = 8 * p.steps
+    # forward (instruction 'spectral')
+    (tmp_path / "Instruction_src01_ls00").write_text("spectral\n")
+    assert _cli(tmp_path, data, maxtime, "01", "00") == 0
+    fdir = data / "forward_src01_ls00"
+    vz = read_fits(fdir / "vz_cc.fits")[:, 0, :]
+    s = Sparc(p, bg, psi=psi)
+    vz_ref, _ = s.forward([-12.0], maxtime)
+    assert np.array_equal(vz, vz_ref[0].cpu().numpy())
+    assert np.array_equal(np.fromfile(fdir / "vz_cc.bin", dtype="<f8").reshape(vz.shape), vz)
+    assert (fdir / "kernel_info").exists() and (fdir / f"xix_{0:06d}_partial.fits").exists()
+    # adjoint (instruction 'adjoint') from a caller-written source.fits
+    adir = data / "adjoint_src01"
+    adir.mkdir(exist_ok=True)
+    nt = p.forcing_length(maxtime)
+    t = np.arange(nt)[:, None]
+    vr = 1e-6 * np.sin(2 * np.pi * t / nt) * np.exp(-((np.arange(p.nx)[None, :] - 40) / 5.0) ** 2)
+    write_fits(adir / "source.fits", vr[:, None, :])
+    (tmp_path / "Instruction_src01_ls00").write_text("adjoint\n")
+    assert _cli(tmp_path, data, maxtime, "01", "00") == 0
+    assert (adir / "kernel_info").exists()
+    # kernels (no instruction file)
+    assert _cli(tmp_path, data, maxtime, "01", "00") == 0
+    kc = read_fits(data / "kernel" / "kernel_c_01.fits")[:, 0, :]
+    kpsi = read_fits(data / "kernel" / "kernel_psi_01.fits")[:, 0, :]
+    assert not list(fdir.glob("*partial*"))                        # kernels.f90:591-597
+    # the same three phases through the Python API
+    _, fsn = s.forward([-12.0], maxtime, snapshots=True)
+    sa = Sparc(p, bg, adjoint=True, psi=psi)
+    _, asn = sa.adjoint(torch.as_tensor(vr[None], device="cuda"), maxtime, snapshots=True)
+    K = Sparc(p, bg).kernels(fsn, asn)
+    for got, ref in ((kc, K["kc"]), (kpsi, K["kpsi"])):
+        ref = ref[0].cpu().numpy()
+        assert np.abs(ref).max() > 0
+        assert np.linalg.norm(got - ref) <= 1e-12 * np.linalg.norm(ref)
