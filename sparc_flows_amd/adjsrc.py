@@ -1,0 +1,275 @@
+"""Travel-time adjoint source and misfit: ADJOINT_SOURCE_FILT (driver.f90:1758-2238).
+
+After a forward run the reference filters the synthetic vz traces and the
+observed ones with ridge filters in (k, nu) (f mode, p1..p7, all p modes;
+FMODE_FILTER .. ALL_PMODE_FILTER, driver.f90:1148-1718) and optional
+frequency filters (FREQ_FILTER, :1721-1733), picks a window around the
+largest synthetic arrival of each receiver, measures the Gizon & Birch travel
+time (COMPUTE_TT_GIZONBIRCH, :894-940) and builds the adjoint source
+
+    adj(x, nt+1-j) += Re[ F^-1( filter * FFT_t(ccdot_i * window) * e^{-i k x_i} ) ](x, j) * con,
+    con = -tau / (sum_window ccdot_i^2 dt),    adj *= nt / 60,
+
+with ccdot the time derivative of the filtered synthetic.  The misfit is
+0.5 sum tau^2 over all measurements (minutes^2).
+
+This module restates that arithmetic with numpy FFTs in FFTW's conventions
+(forward e^{-i...} unnormalised, backward unnormalised).  Arrays are
+[nt][nx] (time, x), the transpose of the reference's (nx, 1, nt).  REAL
+literals of the reference are float32 and are widened the same way here.
+
+Deviations, each where the reference's behaviour is undefined:
+* SIMPSON_REGULAR (integrals.f90:107-131) accumulates into an uninitialised
+  variable; the sums start at 0 here.
+* the search range timest..timefin of a receiver is clipped to the record,
+  and a receiver whose range lies wholly outside it is not measured (the
+  reference indexes outside the array in both cases).
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+from pathlib import Path
+
+import numpy as np
+
+from .params import f32
+
+PI = math.pi
+
+
+def distmat(n: int) -> np.ndarray:
+    """distmat(n, 1, f) (initialize.f90:1181-1198): FFT index order with the
+    Nyquist index positive."""
+    i = np.arange(n)
+    return np.where(i > n - i, -(n - i), i).astype(np.float64)
+
+
+def _kk(nx: int, xlength_cm: float) -> np.ndarray:
+    # k = abs(k) * 2 pi / (xlength * 10.**(-8.) * nx / (nx - 1.))  (Mm^-1)
+    return np.abs(distmat(nx)) * 2.0 * PI / (xlength_cm * f32(1e-8) * nx / (nx - 1.0))
+
+
+def _ff(nt: int, outputcad: float) -> np.ndarray:
+    # w = abs(w) * 2 pi / (nt dt); f = w / (2 pi) * 1e3  (mHz)
+    w = np.abs(distmat(nt)) * 2.0 * PI / (nt * outputcad)
+    return w / (2.0 * PI) * 1e3
+
+
+def _low_cut(filt: np.ndarray, f: np.ndarray, f_low: float, df: float) -> np.ndarray:
+    """Zero below f_low, raised-cosine taper up to f_low + df (f per row)."""
+    out = filt.copy()
+    out[f < f_low, :] = 0.0
+    taper = f < f_low + df
+    out[taper, :] *= (0.5 * (1.0 + np.cos(PI * (f[taper] - (f_low + df)) / df)))[:, None]
+    return out
+
+
+def _band(f: np.ndarray, f0: np.ndarray, f1: np.ndarray) -> np.ndarray:
+    """Raised cosine over 0 < f - f0 < f1 - f0 for every (f, k)."""
+    delta = (f1 - f0)[None, :]
+    d = f[:, None] - f0[None, :]
+    inside = (d < delta) & (d > 0)
+    with np.errstate(invalid="ignore", divide="ignore"):
+        v = 0.5 * (1.0 + np.cos(PI * (np.abs(d) - np.abs(delta) * 0.5) / (np.abs(delta) * 0.5)))
+    return np.where(inside, v, 0.0)
+
+
+# (Polylow, Poly) quadratic ridge boundaries in mHz against k in Mm^-1
+# (driver.f90:1148-1718); p2 uses f0 = const * sqrt(k).
+_RIDGES = {
+    "f": ((0.7, 1.7, -0.2), (1.1, 1.9, -0.2), 1.1),
+    "p1": ((1.1, 2.4, -0.3), (1.4, 3.0, -0.5), 1.6),
+    "p2": (4.1004, (1.6, 3.8, -0.65), 1.6),
+    "p3": ((2.0, 3.55, -0.7), (2.0, 4.1, -0.8), 1.6),
+    "p4": ((2.15, 4.0, -0.7), (2.3, 4.4, -0.7), 1.6),
+    "p5": ((2.2, 4.7, -1.0), (2.35, 5.6, -1.1), 1.6),
+    "p6": ((1.80035417224, 7.42939105658, -2.84595764385), (1.98567115899, 8.09108986838, -3.20316331815), 1.6),
+    "p7": ((1.98297334673, 7.94931076885, -3.00725356897), (2.18544600032, 8.68183289647, -3.84478880142), 1.6),
+}
+
+
+def ridge_filter(name: str, nt: int, nx: int, xlength_cm: float, outputcad: float) -> np.ndarray:
+    """FMODE_FILTER / P1..P7MODE_FILTER / ALL_PMODE_FILTER as [nt][nx]."""
+    k = _kk(nx, xlength_cm)
+    f = _ff(nt, outputcad)
+    df = f32(0.5)
+    if name == "allp":
+        # ALL_PMODE_FILTER starts from 1 everywhere and sets the band to 1
+        filt = np.ones((nt, nx))
+        return _low_cut(filt, f, f32(1.6), df)
+    low, high, f_low = _RIDGES[name]
+    poly = [f32(c) for c in high]
+    f1 = poly[0] + poly[1] * k + poly[2] * k ** 2.0
+    if name == "p2":
+        f0 = f32(low) * np.abs(k) ** 0.5
+    else:
+        pl = [f32(c) for c in low]
+        f0 = pl[0] + pl[1] * k + pl[2] * k ** 2.0
+    return _low_cut(_band(f, f0, f1), f, f32(f_low), df)
+
+
+def freq_filter(f1: float, f2: float, nt: int, outputcad: float) -> np.ndarray:
+    """FREQ_FILTER (driver.f90:1721-1733): band pass between f1 and f2 mHz, [nt]."""
+    w = np.abs(distmat(nt)) * 1e3 / (nt * outputcad)
+    wid = f32(0.4) * (w[1] - w[0])
+    return 1.0 / (1.0 + np.exp((f1 - w) / wid)) - 1.0 / (1.0 + np.exp((f2 - w) / wid))
+
+
+def simpson_regular(fv: np.ndarray, dvar: float) -> float:
+    """SIMPSON_REGULAR (integrals.f90:107-131): weights 1, 4, 2, 4, ..., 1 by
+    1-based index parity, times dvar/3 (the sum starts at 0)."""
+    n = fv.shape[0]
+    wts = np.where(np.arange(1, n + 1) % 2 == 0, 4.0, 2.0)
+    wts[0] = wts[-1] = 1.0
+    return float((wts * fv).sum() * dvar / 3.0)
+
+
+def tt_gizon_birch(u0: np.ndarray, u: np.ndarray, dt: float, lef: int, rig: int) -> float:
+    """COMPUTE_TT_GIZONBIRCH (driver.f90:894-940); lef, rig 0-based inclusive."""
+    nt = u0.shape[0]
+    window = np.zeros(nt)
+    window[lef:rig + 1] = 1.0
+    u0w = np.fft.rfft(u0)
+    kk = np.arange(nt // 2)
+    u0w[: nt // 2] = u0w[: nt // 2] * (1j * 2 * PI * kk / (nt * dt))
+    u0w[nt // 2] = 0.0
+    u0dot = np.fft.irfft(u0w, n=nt) * nt / nt          # c2r unnormalised, then / nt
+    num = simpson_regular(window * u0dot * (u0 - u), dt)
+    den = simpson_regular(window * u0dot ** 2, dt)
+    return num / den
+
+
+@dataclass
+class Measurement:
+    mode: int           # 0 f, 1..7 p1..p7, 8 first-bounce p modes
+    pixel: int          # 0-based receiver column
+    tau_s: float        # travel time, s
+    lef: int            # window, 1-based inclusive (as windows.<mode> holds it)
+    rig: int
+    loc: int
+    timest: int
+    timefin: int
+
+
+@dataclass
+class AdjointSource:
+    adj: np.ndarray                      # [nt][nx], written as source.fits
+    misfit: float                        # 0.5 sum tau^2 (minutes^2)
+    nmeas: int
+    measurements: list = field(default_factory=list)
+
+
+_MODE_NAMES = {0: "f", 1: "p1", 2: "p2", 3: "p3", 4: "p4", 5: "p5", 6: "p6", 7: "p7", 8: "allp"}
+_VEL = [0.44, 0.60, 0.75, 0.9, 1.2, 1.4, 1.7, 1.9]          # Mm/min (driver.f90:1946-1954)
+
+
+def _fftw_fwd(a):
+    return np.fft.fft2(a)
+
+
+def _fftw_bwd(a):
+    return np.fft.ifft2(a) * a.shape[0] * a.shape[1]
+
+
+def adjoint_source_filt(acc: np.ndarray, dat: np.ndarray, x: np.ndarray, x00: float, xlength_cm: float,
+                        outputcad: float, dt_min: float, mode_params: dict,
+                        freq_params: dict | None = None, windows: dict | None = None) -> AdjointSource:
+    """ADJOINT_SOURCE_FILT for one source.
+
+    acc, dat: [nt][nx] synthetic and observed vz (vz_cc.fits, data/SS.fits);
+    x: grid x in [0, 1) (x = (i-1)/nx); x00: source position (Mm);
+    dt_min: sample spacing in minutes (from the timeline file);
+    mode_params: {mode: (mindist, maxdist, window_min)} for the params.<mode>
+    files present; freq_params: {"allmodes"|0|1|2: (f1, f2)} for the
+    filter.params.* files present; windows: {mode: [(lef, rig), ...]} to reuse
+    (the linesearch branch reads them from windows.<mode>).
+    """
+    nt, nx = acc.shape
+    freq_params = freq_params or {}
+    distances = np.abs((x - 0.5) * xlength_cm * f32(1e-8) - x00)
+    eyekh = 1j * distmat(nx) * 2.0 * PI
+    freqnu = distmat(nt) / (nt * dt_min * 60.0)
+    # frequency filters as the reference applies them (driver.f90:1862-1935):
+    # one running filter, replaced by each filter.params.* present, multiplies
+    # every ridge filter once and then the f, p1, p2 filters again in turn
+    filt = np.ones(nt)
+    if "allmodes" in freq_params:
+        filt = freq_filter(*freq_params["allmodes"], nt, outputcad)
+    filters = {m: ridge_filter(name, nt, nx, xlength_cm, outputcad) * filt[:, None]
+               for m, name in _MODE_NAMES.items()}
+    for m in (0, 1, 2):
+        if m in freq_params:
+            filt = freq_filter(*freq_params[m], nt, outputcad)
+        filters[m] = filters[m] * filt[:, None]
+    tempout = _fftw_fwd(acc.astype(np.complex128))
+    tempdat = _fftw_fwd(dat.astype(np.complex128))
+    adj = np.zeros((nt, nx))
+    misfit = 0.0
+    nmeas = 0
+    meas = []
+    for pord in range(9):
+        if pord not in mode_params:
+            continue
+        mindist, maxdist, window = mode_params[pord]
+        halftime = int(math.floor(window / (2.0 * dt_min) + 0.5))      # nint of a positive value
+        filt = filters[pord]
+        accf = _fftw_bwd(tempout * filt)
+        datf = _fftw_bwd(tempdat * filt)
+        ccdot = _fftw_bwd(tempout * filt * (1j * freqnu * 2.0 * PI)[:, None])
+        wlist = list(windows.get(pord, [])) if windows else None
+        for i in range(nx):
+            if not (distances[i] > mindist and distances[i] < maxdist):
+                continue
+            if wlist is None:
+                if pord != 8:
+                    timest = int(math.floor(distances[i] / f32(_VEL[pord]) * 1.0 / dt_min))
+                    timefin = timest + 40
+                else:
+                    dist = distances[i]
+                    timest = int(math.floor((-0.000417147774671 * dist ** 2 + 0.313350998096 * dist
+                                             + 17.9609631186) * 1.0 / dt_min))
+                    timefin = int(math.floor((-0.00028361249034 * dist ** 2 + 0.29270337114 * dist
+                                              + 36.4398734578) * 1.0 / dt_min))
+                # 1-based timest..timefin, clipped to the record; a receiver
+                # whose search range lies outside the record is not measured
+                t0, t1 = max(timest, 1), min(timefin, nt)
+                if t0 > t1:
+                    continue
+                seg = np.abs(accf[t0 - 1:t1, i].real)
+                loc = int(np.argmax(seg)) + t0              # 1-based
+                lef, rig = max(loc - halftime, 1), min(loc + halftime, nt)
+            else:
+                lef, rig = wlist.pop(0)
+                loc, timest, timefin = 0, 0, 0
+            tau = tt_gizon_birch(accf[:, i].real, datf[:, i].real, dt_min, lef - 1, rig - 1)
+            meas.append(Measurement(pord, i, tau * 60.0, lef, rig, loc, timest, timefin))
+            win = np.zeros(nt)
+            win[lef - 1:rig] = 1.0
+            ccdotone = np.fft.fft(ccdot[:, i] * win)
+            spec = filt * ccdotone[:, None] * np.exp(-eyekh * x[i])[None, :]
+            filtex = _fftw_bwd(spec)
+            misfit += tau ** 2.0
+            nmeas += 1
+            con = -tau / (np.sum(ccdot[lef - 1:rig, i] ** 2.0) * dt_min)
+            adj[::-1, :] += (filtex * con).real
+    adj = adj * nt / 60.0
+    return AdjointSource(adj=adj, misfit=0.5 * misfit, nmeas=nmeas, measurements=meas)
+
+
+def read_mode_params(directory: Path) -> tuple[dict, dict]:
+    """params.0 .. params.8 (mindist, maxdist, window) and filter.params.*
+    (f1, f2 in mHz) present in the data directory."""
+    directory = Path(directory)
+    modes, freqs = {}, {}
+    for m in range(9):
+        f = directory / f"params.{m}"
+        if f.exists():
+            v = [float(t) for t in f.read_text().split()[:3]]
+            modes[m] = tuple(v)
+    for key, name in (("allmodes", "filter.params.allmodes"), (0, "filter.params.0"), (1, "filter.params.1"),
+                      (2, "filter.params.2")):
+        f = directory / name
+        if f.exists():
+            freqs[key] = tuple(float(t) for t in f.read_text().split()[:2])
+    return modes, freqs

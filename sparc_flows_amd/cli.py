@@ -19,11 +19,11 @@ initialize.f90:984-1144, DETERMINE_STATUS driver.f90:480-631):
   kernel step writes ``kernel/{kernel_c,kernel_psi,kernel_vx,kernel_vz,
   hessian}_SS.fits`` and removes the partial snapshots (kernels.f90:591-597).
 
-The travel-time adjoint source and misfit the reference computes at the end of
-a forward run (ADJOINT_SOURCE_FILT, MISFIT_ALL) are not part of this program
-yet: a forward run reports that and leaves ``adjoint_srcSS/source.fits`` to the
-caller.  The exit code is 0 on success and 1 on any error, which is what the
-drivers assert (full.py:31).
+At the end of a forward run without ``compute_data`` the travel-time adjoint
+source and misfit are computed as ADJOINT_SOURCE_FILT does (sparc_flows_amd.
+adjsrc) when ``data/SS.fits`` holds the observed traces; MISFIT_ALL's
+per-mode summary files are not written.  The exit code is 0 on success and 1
+on any error, which is what the drivers assert (full.py:31).
 """
 from __future__ import annotations
 
@@ -142,10 +142,62 @@ def _forward(p, bg, psi, ddir, contrib, jobno, maxtime, compute_data, no_snapsho
         for k in range(a.shape[0]):
             for fi, name in enumerate(FWD_FIELDS):
                 write_fits(out / f"{name}_{_label(k * steps)}_partial.fits", a[k, fi][:, None, :], time=k * steps)
-    if not compute_data:
-        print("forward done; the travel-time adjoint source (ADJOINT_SOURCE_FILT) is computed by the caller")
     torch.cuda.synchronize()
+    if not compute_data:
+        _adjoint_source(p, ddir, contrib, jobno, vz, steps * p.timestep, linesearch, compute_synth)
     return 0
+
+
+def _fortran_tt_line(m) -> str:
+    # format (I3,1X,F14.8,1X,I4,1X,I4,1X,I4,1X,I4,1X,I4), driver.f90:2081
+    return (f"{m.pixel + 1:3d} {m.tau_s:14.8f} {m.lef:4d} {m.rig:4d} {m.loc:4d} {m.timest:4d} "
+            f"{m.timefin:4d}\n")
+
+
+def _adjoint_source(p, ddir, contrib, jobno, vz, cadence_s, linesearch, compute_synth) -> None:
+    """ADJOINT_SOURCE_FILT after a forward run (driver.f90:271-276): travel
+    times against data/SS.fits, ttdiff.<mode>, windows.<mode>, the adjoint
+    source and the misfit file."""
+    from .adjsrc import adjoint_source_filt, read_mode_params
+
+    data = ddir / "data" / f"{contrib}.fits"
+    if not data.exists():
+        print(f"no observed traces {data}: adjoint source not computed")
+        return
+    nt = vz.shape[0]
+    dat = read_fits(data).reshape(-1, p.nx)[:nt]
+    modes, freqs = read_mode_params(ddir)
+    win_dir = ddir / f"forward_src{contrib}_ls00"
+    windows = None
+    if linesearch:
+        windows = {}
+        for m in modes:
+            f = win_dir / f"windows.{m}"
+            if f.exists():
+                v = [int(t) for t in f.read_text().split()]
+                windows[m] = list(zip(v[0::2], v[1::2]))
+    r = adjoint_source_filt(vz, dat, np.arange(p.nx) / p.nx, _xsrc(ddir, contrib), p.xlength, p.outputcad,
+                            cadence_s / 60.0, modes, freqs, windows)
+    out = ddir / f"forward_src{contrib}_ls{jobno}"
+    for m in modes:
+        rows = [x for x in r.measurements if x.mode == m]
+        (out / f"ttdiff.{m}").write_text("".join(_fortran_tt_line(x) for x in rows))
+        if not linesearch:
+            win_dir.mkdir(parents=True, exist_ok=True)
+            (win_dir / f"windows.{m}").write_text("".join(f"{x.lef:12d}{x.rig:12d}\n" for x in rows))
+    adir = ddir / f"adjoint_src{contrib}"
+    adir.mkdir(parents=True, exist_ok=True)
+    if not (linesearch or compute_synth):
+        write_fits(adir / "source.fits", r.adj[:, None, :])
+    if not linesearch:
+        # t = -timeline / 60 in reversed order; written as t(1)*60, t(nt)*60 (driver.f90:1807-1812, 2211-2216)
+        tl = [float(line.split()[1]) for line in (out / "timeline").read_text().splitlines()[:nt]]
+        with open(adir / "information", "w") as f:
+            f.write(f"{nt:12d}\n{-tl[-1]:24.15E}\n{-tl[0]:24.15E}\n 2D\n")
+    kdir = ddir / "kernel"
+    kdir.mkdir(parents=True, exist_ok=True)
+    (kdir / f"misfit_{contrib}_{jobno}").write_text(f"{int(contrib):12d}{r.nmeas:12d} {r.misfit:24.15E}\n")
+    print(f"adjoint source: {r.nmeas} measurements, misfit {r.misfit:.6e}")
 
 
 def _adjoint(p, bg, psi, ddir, contrib, maxtime) -> int:

@@ -14,6 +14,8 @@
 // LDS, and the exact correction y - L_{s-1} V - F_{s+1} W.  Loads use clamped
 // row indices and stores go to a discard slot for padding rows, so the
 // unrolled row loops are straight-line code the scheduler can batch.
+#include <type_traits>
+
 #include "sparc_device.h"
 #include "sparc_launch.h"
 
@@ -172,85 +174,101 @@ __global__ __launch_bounds__(256) void k_colseg(Dims d, Bg bg, DbyCoef cf, SegTa
         }
     }
 
-    double Y[MS];
-    double *const wB = st.w_rho;            // partial RHS v_z between phases
-
-    // --- A1: d xi_z/dz (ibc 3, top row preset, physics2d.f90:954-955), z-PML
-    //         (:957-967), divergence, p and rho' (:990-998, :1043)
-    seg_rhs<MS>(gload(ca.T[1]), Y, a, nz, first, 3, 0.0, st.topbc[(size_t)b * nx + x], cf);
-    seg_solve<MS>(Y, zt + zrow(2, 0) * nzp, zt + zrow(2, 1) * nzp, zt + zrow(2, 2) * nzp);
-    if (act) { yb[0][c * n2P + 2 * s] = Y[0]; yb[0][c * n2P + 2 * s + 1] = Y[MS - 1]; }
-    __syncthreads();
-    if (act) seg_correct<MS>(Y, s, P, sg.hw, yb[0] + c * n2P, mv + 2 * nband, zt + zrow(2, 3) * nzp,
-                             zt + zrow(2, 4) * nzp);
+    // Own rows of xi_z, v_z, v_x are kept from the right-hand-side windows for
+    // short segments; long segments reload them (register budget).
+    constexpr bool KEEP = MS <= 12;
+    constexpr int KO = KEEP ? MS : 1;
+    double Yz[MS], Yv[MS], Yx[MS];          // d xi_z/dz, d v_z/dz, d v_x/dz (unstretched)
+    double oz[KO], ov[KO], ox[KO];          // own rows of xi_z, v_z, v_x
+    double w[MS];                           // RHS v_z without the pressure gradient
+    const bool segpml = a < nb || a + MS > nz - ntop;   // segment touches PML rows
+    auto window_rhs = [&](const double *F, double(&R)[MS], double *own, int ibc, double pre0, double preN) {
+        double win[MS + 6];
 #pragma unroll
-    for (int i = 0; i < MS; i++) {
-        const int j = a + i;
-        const size_t qj = rowq(i), gq = qj - (size_t)b * nz * nx;
-        const int pj = pmlrow(j);
-        const size_t qp = pz0 + (size_t)pj * nx, qb = (size_t)pj * nx + x;
-        const double psi = st.pz_tmp[qp], az = bg.az[qb], bzc = bg.bz[qb], pbase = st.pz_base[qp];
-        const double dxix = st.dxixdx[qj], xiz = ca.T[1][qj];
-        const double c2r = bg.c2rho0[gq], r0 = zt[Z_RHO * nzp + i];
-        const double rinv = zt[Z_RINV * nzp + i], gz = zt[Z_G * nzp + i];
-        const bool pm = is_pml(j);
-        // functiondecay scales d p0/dz and d rho0/dz in the PML rows (physics2d.f90:303-318)
-        const double fdec = pm ? bg.fz[qb] : 1.0;
-        const double gp0 = zt[Z_GP * nzp + i] * fdec, gr0 = zt[Z_GRHO * nzp + i] * fdec;
-        double dz = Y[i] * zt[Z_STRETCH * nzp + i];
-        const double prhs = az * dz + bzc * psi;
-        dz = pm ? dz + psi : dz;
-        const double pnew = pbase + ca.bnext * prhs;
-        if (pm && act) {
-            st.pz_tmp[qp] = pnew;
-            if (LAST) st.pz_base[qp] = pnew;
+        for (int k = 0; k < MS + 6; k++) win[k] = F[col + (size_t)min(max(a - 3 + k, 0), nz - 1) * nx];
+        seg_rhs<MS>([&](int k) { return win[k]; }, R, a, nz, first, ibc, pre0, preN, cf);
+        if (KEEP && own) {
+#pragma unroll
+            for (int i = 0; i < MS; i++) own[i] = win[3 + i];
         }
-        const double div = dxix + dz;
-        const double pv = -(c2r * div) - gp0 * xiz;
-        *(valid(i) ? &st.p[qj] : dump) = pv;
-        const double rho = -(r0 * div) - gr0 * xiz;
-        double bzv = -rinv * (rho * gz);
-        bzv = (i == isrc) ? bzv + srcv : bzv;
-        *(valid(i) ? &wB[qj] : dump) = bzv;
-    }
+    };
 
-    // --- A2: d v_z/dz (ibc 1, :1076) and the flow terms of RHS v_z (:1073-1077)
-    seg_rhs<MS>(gload(ca.T[3]), Y, a, nz, first, 1, 0.0, 0.0, cf);
-    seg_solve<MS>(Y, zt + zrow(0, 0) * nzp, zt + zrow(0, 1) * nzp, zt + zrow(0, 2) * nzp);
-    if (act) { yb[1][c * n2P + 2 * s] = Y[0]; yb[1][c * n2P + 2 * s + 1] = Y[MS - 1]; }
-    __syncthreads();
-    if (act) seg_correct<MS>(Y, s, P, sg.hw, yb[1] + c * n2P, mv, zt + zrow(0, 3) * nzp, zt + zrow(0, 4) * nzp);
-#pragma unroll
-    for (int i = 0; i < MS; i++) {
-        const size_t qj = rowq(i), gq = qj - (size_t)b * nz * nx;
-        const double v0x = bg.v0x[gq], v0z = bg.v0z[gq], dxvz = st.dxvz[qj], bz0 = wB[qj];
-        const double dzvz = Y[i] * zt[Z_STRETCH * nzp + i];
-        double bzv = bz0 - 2.0 * v0x * dxvz;
-        bzv = bzv - 2.0 * v0z * dzvz;
-        *(valid(i) ? &wB[qj] : dump) = bzv;
+    // --- B: d xi_z/dz (ibc 3, top row preset, physics2d.f90:954-955),
+    //        d v_z/dz (ibc 1, :1076) and d v_x/dz (ibc 1, :1070): three
+    //        independent solves, one exchange of segment end values
+    window_rhs(ca.T[1], Yz, oz, 3, 0.0, st.topbc[(size_t)b * nx + x]);
+    seg_solve<MS>(Yz, zt + zrow(2, 0) * nzp, zt + zrow(2, 1) * nzp, zt + zrow(2, 2) * nzp);
+    window_rhs(ca.T[3], Yv, ov, 1, 0.0, 0.0);
+    seg_solve<MS>(Yv, zt + zrow(0, 0) * nzp, zt + zrow(0, 1) * nzp, zt + zrow(0, 2) * nzp);
+    window_rhs(ca.T[2], Yx, ox, 1, 0.0, 0.0);
+    seg_solve<MS>(Yx, zt + zrow(0, 0) * nzp, zt + zrow(0, 1) * nzp, zt + zrow(0, 2) * nzp);
+    if (act) {
+        yb[0][c * n2P + 2 * s] = Yz[0]; yb[0][c * n2P + 2 * s + 1] = Yz[MS - 1];
+        yb[1][c * n2P + 2 * s] = Yv[0]; yb[1][c * n2P + 2 * s + 1] = Yv[MS - 1];
+        yb[2][c * n2P + 2 * s] = Yx[0]; yb[2][c * n2P + 2 * s + 1] = Yx[MS - 1];
     }
-
-    // --- A3: d v_x/dz (ibc 1, :1070) and the flow part of RHS v_x (:1067-1071)
-    seg_rhs<MS>(gload(ca.T[2]), Y, a, nz, first, 1, 0.0, 0.0, cf);
-    seg_solve<MS>(Y, zt + zrow(0, 0) * nzp, zt + zrow(0, 1) * nzp, zt + zrow(0, 2) * nzp);
-    if (act) { yb[2][c * n2P + 2 * s] = Y[0]; yb[2][c * n2P + 2 * s + 1] = Y[MS - 1]; }
     __syncthreads();
-    if (act) seg_correct<MS>(Y, s, P, sg.hw, yb[2] + c * n2P, mv, zt + zrow(0, 3) * nzp, zt + zrow(0, 4) * nzp);
-#pragma unroll
-    for (int i = 0; i < MS; i++) {
-        const int j = a + i;
-        const size_t qj = rowq(i), gq = qj - (size_t)b * nz * nx;
-        const double v0x = bg.v0x[gq], v0z = bg.v0z[gq], dxvx = st.dxvx[qj];
-        double rvx = -(2.0 * v0x * dxvx) - 2.0 * v0z * (Y[i] * zt[Z_STRETCH * nzp + i]);
-        rvx = (j == 0 || x == 0 || x == nx - 1) ? 0.0 : rvx;
-        *(valid(i) ? &st.rvx[qj] : dump) = rvx;
+    if (act) {
+        seg_correct<MS>(Yz, s, P, sg.hw, yb[0] + c * n2P, mv + 2 * nband, zt + zrow(2, 3) * nzp,
+                        zt + zrow(2, 4) * nzp);
+        seg_correct<MS>(Yv, s, P, sg.hw, yb[1] + c * n2P, mv, zt + zrow(0, 3) * nzp, zt + zrow(0, 4) * nzp);
+        seg_correct<MS>(Yx, s, P, sg.hw, yb[2] + c * n2P, mv, zt + zrow(0, 3) * nzp, zt + zrow(0, 4) * nzp);
     }
+    // z-PML (:957-967), divergence, p and rho' (:990-998, :1043), the flow
+    // terms of RHS v_z (:1073-1077) and the flow part of RHS v_x (:1067-1071)
+    auto epi_b = [&](auto pml_c) {
+        constexpr bool PML = decltype(pml_c)::value;
+#pragma unroll
+        for (int i = 0; i < MS; i++) {
+            const int j = a + i;
+            const size_t qj = rowq(i), gq = qj - (size_t)b * nz * nx;
+            const double dxix = st.dxixdx[qj], dxvz = st.dxvz[qj], dxvx = st.dxvx[qj];
+            const double xiz = KEEP ? oz[KEEP ? i : 0] : ca.T[1][qj];
+            const double c2r = bg.c2rho0[gq], v0x = bg.v0x[gq], v0z = bg.v0z[gq];
+            const double str = zt[Z_STRETCH * nzp + i], r0 = zt[Z_RHO * nzp + i];
+            const double rinv = zt[Z_RINV * nzp + i], gz = zt[Z_G * nzp + i];
+            double dz = Yz[i] * str;
+            double fdec = 1.0;
+            if (PML) {
+                const int pj = pmlrow(j);
+                const size_t qp = pz0 + (size_t)pj * nx, qb = (size_t)pj * nx + x;
+                const double psi = st.pz_tmp[qp], az = bg.az[qb], bzc = bg.bz[qb], pbase = st.pz_base[qp];
+                const bool pm = is_pml(j);
+                // functiondecay scales d p0/dz and d rho0/dz in the PML rows (physics2d.f90:303-318)
+                fdec = pm ? bg.fz[qb] : 1.0;
+                const double prhs = az * dz + bzc * psi;
+                dz = pm ? dz + psi : dz;
+                const double pnew = pbase + ca.bnext * prhs;
+                if (pm && act) {
+                    st.pz_tmp[qp] = pnew;
+                    if (LAST) st.pz_base[qp] = pnew;
+                }
+            }
+            const double gp0 = zt[Z_GP * nzp + i] * fdec, gr0 = zt[Z_GRHO * nzp + i] * fdec;
+            const double div = dxix + dz;
+            const double pv = -(c2r * div) - gp0 * xiz;
+            *(valid(i) ? &st.p[qj] : dump) = pv;
+            const double rho = -(r0 * div) - gr0 * xiz;
+            double bzv = -rinv * (rho * gz);
+            bzv = (i == isrc) ? bzv + srcv : bzv;
+            const double dzvz = Yv[i] * str;
+            bzv = bzv - 2.0 * v0x * dxvz;
+            w[i] = bzv - 2.0 * v0z * dzvz;
+            double rvx = -(2.0 * v0x * dxvx) - 2.0 * v0z * (Yx[i] * str);
+            rvx = (j == 0 || x == 0 || x == nx - 1) ? 0.0 : rvx;
+            *(valid(i) ? &st.rvx[qj] : dump) = rvx;
+        }
+    };
+    if (segpml) epi_b(std::true_type{});
+    else epi_b(std::false_type{});
 
     // --- A4: d p/dz (ibc 2, bottom row zero, :1023) with the z-PML (:1025-1035),
     //         RHS v_z, zeroed rows/columns (:1083-1093), RK stage update.
-    // p of the whole column was written in A1; the block barriers since make
-    // the neighbouring segments' rows visible.
-    seg_rhs<MS>(gload(st.p), Y, a, nz, first, 2, 0.0, 0.0, cf);
+    // p of the whole column was written above; the block barrier makes the
+    // neighbouring segments' rows visible.
+    __syncthreads();
+    double (&Y)[MS] = Yz;
+    window_rhs(st.p, Y, nullptr, 2, 0.0, 0.0);
     seg_solve<MS>(Y, zt + zrow(1, 0) * nzp, zt + zrow(1, 1) * nzp, zt + zrow(1, 2) * nzp);
     if (act) { yb[3][c * n2P + 2 * s] = Y[0]; yb[3][c * n2P + 2 * s + 1] = Y[MS - 1]; }
     __syncthreads();
@@ -259,38 +277,46 @@ __global__ __launch_bounds__(256) void k_colseg(Dims d, Bg bg, DbyCoef cf, SegTa
     const int sslot = (LAST && sink.snap) ? out_slot(time, clk.steps, sink.k0, sink.nsnap) : -1;
     const bool snap = sslot >= 0;
     const size_t slot = snap ? ((size_t)b * sink.nsnap + sslot) * sink.nfield : 0;
+    auto epi_4 = [&](auto pml_c) {
+        constexpr bool PML = decltype(pml_c)::value;
 #pragma unroll
-    for (int i = 0; i < MS; i++) {
-        const int j = a + i;
-        const size_t qj = rowq(i), gq = qj - (size_t)b * nz * nx;
-        const int pj = pmlrow(j);
-        const size_t qp = pz1 + (size_t)pj * nx, qb = (size_t)pj * nx + x;
-        const double psi = st.pz_tmp[qp], az = bg.az[qb], bzc = bg.bz[qb], pbase = st.pz_base[qp];
-        const double rinv = zt[Z_RINV * nzp + i];
-        const double vx = ca.T[2][qj], vz = ca.T[3][qj];
-        const double a0 = st.a[0][qj], a1 = st.a[1][qj], a3 = st.a[3][qj];
-        double gpz = Y[i] * zt[Z_STRETCH * nzp + i];
-        const bool pm = is_pml(j);
-        const double prhs = az * gpz + bzc * psi;
-        gpz = pm ? gpz + psi : gpz;
-        const double pnew = pbase + ca.bnext * prhs;
-        if (pm && act) {
-            st.pz_tmp[qp] = pnew;
-            if (LAST) st.pz_base[qp] = pnew;
+        for (int i = 0; i < MS; i++) {
+            const int j = a + i;
+            const size_t qj = rowq(i);
+            const double rinv = zt[Z_RINV * nzp + i];
+            const double vx = KEEP ? ox[KEEP ? i : 0] : ca.T[2][qj];
+            const double vz = KEEP ? ov[KEEP ? i : 0] : ca.T[3][qj];
+            const double a0 = st.a[0][qj], a1 = st.a[1][qj], a3 = st.a[3][qj];
+            double gpz = Y[i] * zt[Z_STRETCH * nzp + i];
+            if (PML) {
+                const int pj = pmlrow(j);
+                const size_t qp = pz1 + (size_t)pj * nx, qb = (size_t)pj * nx + x;
+                const double psi = st.pz_tmp[qp], az = bg.az[qb], bzc = bg.bz[qb], pbase = st.pz_base[qp];
+                const bool pm = is_pml(j);
+                const double prhs = az * gpz + bzc * psi;
+                gpz = pm ? gpz + psi : gpz;
+                const double pnew = pbase + ca.bnext * prhs;
+                if (pm && act) {
+                    st.pz_tmp[qp] = pnew;
+                    if (LAST) st.pz_base[qp] = pnew;
+                }
+            }
+            double rvz = -rinv * gpz + w[i];
+            double rxix = vx, rxiz = vz;
+            const bool zero = j == 0 || x == 0 || x == nx - 1;
+            rvz = zero ? 0.0 : rvz;
+            rxix = zero ? 0.0 : rxix;
+            rxiz = zero ? 0.0 : rxiz;
+            const bool v = valid(i);
+            *(v ? &ca.TN[0][qj] : dump) = a0 + ca.bnext * rxix;
+            *(v ? &ca.TN[1][qj] : dump) = a1 + ca.bnext * rxiz;
+            *(v ? &ca.TN[3][qj] : dump) = a3 + ca.bnext * rvz;
+            if (snap && v)
+                sink.snap[((slot + (sink.adjoint ? 3 : 5)) * nz + j) * nx + x] = rvz;
         }
-        double rvz = -rinv * gpz + wB[qj];
-        double rxix = vx, rxiz = vz;
-        const bool zero = j == 0 || x == 0 || x == nx - 1;
-        rvz = zero ? 0.0 : rvz;
-        rxix = zero ? 0.0 : rxix;
-        rxiz = zero ? 0.0 : rxiz;
-        const bool v = valid(i);
-        *(v ? &ca.TN[0][qj] : dump) = a0 + ca.bnext * rxix;
-        *(v ? &ca.TN[1][qj] : dump) = a1 + ca.bnext * rxiz;
-        *(v ? &ca.TN[3][qj] : dump) = a3 + ca.bnext * rvz;
-        if (snap && v)
-            sink.snap[((slot + (sink.adjoint ? 3 : 5)) * nz + j) * nx + x] = rvz;
-    }
+    };
+    if (segpml) epi_4(std::true_type{});
+    else epi_4(std::false_type{});
     if (LAST) {
         // FILTER_VARS_2D (physics2d.f90:509-556) on xi_x, xi_z, v_z of the
         // column: each field's rows a-5 .. a+MS+4 are read once (neighbour

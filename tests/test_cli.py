@@ -66,8 +66,15 @@ def test_forward_adjoint_kernel_phases(tmp_path):
         pytest.skip("no GPU")
     from sparc_flows_amd.solver import Sparc
     p, bg, psi, data = _setup(tmp_path)
-    maxtime = 8 * p.steps
-    # forward (instruction 'spectral')
+    maxtime = 60 * p.steps
+    # observed traces: the same source without the flow; the all-p-mode
+    # measurement (params.8) between 3 and 40 Mm with a 10 min window
+    s0 = Sparc(p, bg)
+    vz_obs, _ = s0.forward([-12.0], maxtime)
+    (data / "data").mkdir(exist_ok=True)
+    write_fits(data / "data" / "01.fits", vz_obs[0].cpu().numpy()[:, None, :])
+    (data / "params.8").write_text("3.0\n40.0\n10.0\n")
+    # forward (instruction 'spectral'): traces, snapshots, adjoint source
     (tmp_path / "Instruction_src01_ls00").write_text("spectral\n")
     assert _cli(tmp_path, data, maxtime, "01", "00") == 0
     fdir = data / "forward_src01_ls00"
@@ -77,13 +84,16 @@ def test_forward_adjoint_kernel_phases(tmp_path):
     assert np.array_equal(vz, vz_ref[0].cpu().numpy())
     assert np.array_equal(np.fromfile(fdir / "vz_cc.bin", dtype="<f8").reshape(vz.shape), vz)
     assert (fdir / "kernel_info").exists() and (fdir / f"xix_{0:06d}_partial.fits").exists()
-    # adjoint (instruction 'adjoint') from a caller-written source.fits
+    from sparc_flows_amd.adjsrc import adjoint_source_filt
+    r = adjoint_source_filt(vz, vz_obs[0].cpu().numpy(), np.arange(p.nx) / p.nx, -12.0, p.xlength,
+                            p.outputcad, p.steps * p.timestep / 60.0, {8: (3.0, 40.0, 10.0)})
+    assert r.nmeas > 0
     adir = data / "adjoint_src01"
-    adir.mkdir(exist_ok=True)
-    nt = p.forcing_length(maxtime)
-    t = np.arange(nt)[:, None]
-    vr = 1e-6 * np.sin(2 * np.pi * t / nt) * np.exp(-((np.arange(p.nx)[None, :] - 40) / 5.0) ** 2)
-    write_fits(adir / "source.fits", vr[:, None, :])
+    vr = read_fits(adir / "source.fits")[:, 0, :]
+    assert np.array_equal(vr, r.adj)
+    assert (fdir / "ttdiff.8").read_text().count("\n") == r.nmeas
+    assert (data / "kernel" / "misfit_01_00").exists() and (adir / "information").exists()
+    # adjoint (instruction 'adjoint') from that source.fits
     (tmp_path / "Instruction_src01_ls00").write_text("adjoint\n")
     assert _cli(tmp_path, data, maxtime, "01", "00") == 0
     assert (adir / "kernel_info").exists()
