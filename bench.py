@@ -44,6 +44,29 @@ def make_config(nx=256, nz=300, npml=12):
     return p, bg, psi
 
 
+def forward_rate(nx, nz, nsrc, nsteps, flow, dc, reps=1):
+    """Forward-only Mpoint-steps/s of a batch (the plain-forward and batched
+    multi-source configurations: 128x150 polytrope, 12-point PML)."""
+    import torch
+    from sparc_flows_amd import background as B
+    from sparc_flows_amd.solver import Sparc
+
+    p, bg, psi = make_config(nx, nz)
+    c2d = B.sound_speed_bump(nx, p.xlength, bg) if dc else None
+    s = Sparc(p, bg, nsrc=nsrc, c2d=c2d, psi=psi if flow else None)
+    xs = list(B.source_positions(nsrc, seed=0)) if nsrc > 1 else [0.0]
+    s.forward(xs, nsteps - 1)                          # warm-up (and autotune)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        s.forward(xs, nsteps - 1)
+    torch.cuda.synchronize()
+    sec = (time.perf_counter() - t0) / reps
+    return {"Mpoint_steps_per_s": nx * nz * nsteps * nsrc / sec / 1e6, "s_per_run": sec,
+            "grid": [nx, nz], "sources": nsrc, "time_steps": nsteps, "flow": flow, "dc": dc,
+            "column_pass_shape": {k: s.query(k) for k in ("colseg_tc", "colseg_ms", "colseg_p")}}
+
+
 def cpu_baseline(p, bg, psi, seconds=12.0):
     """Time the C oracle (one core) on a bounded sample of the same forward run."""
     import oracle
@@ -71,6 +94,9 @@ def main():
     ap.add_argument("--batch", type=int, default=16, help="sources of the batched gradient (0: skip)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the plain-forward (128x150, 500 steps) and batched 16-source forward "
+                         "(128x150 with flow and sound-speed bump, 4000 steps) lines")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a plain device-to-device copy of this many MiB ten times "
                          "(calibrates the FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -156,6 +182,11 @@ def main():
                 "ms_per_launch": ms, "algorithmic_bytes_per_point": bytes_per_pt[dom]}
 
     log("kernel profile done")
+    others = None
+    if not args.no_configs:
+        others = {"plain_forward_128x150": forward_rate(128, 150, 1, 500, False, False, reps=3),
+                  "batched16_forward_128x150": forward_rate(128, 150, 16, 4000, True, True)}
+        log("forward configurations done")
     cpu = None if args.no_cpu else cpu_baseline(p, bg, psi, args.cpu_seconds)
     log("cpu baseline done")
     line = {
@@ -172,6 +203,7 @@ def main():
         "roofline": roofline,
         "column_pass_shape": {k: pipe.fwd.query(k) for k in ("colseg_tc", "colseg_ms", "colseg_p", "colseg_hw")},
         "cpu_baseline": cpu,
+        "other_configurations": others,
     }
     print(json.dumps(line))
 

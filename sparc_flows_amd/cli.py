@@ -90,8 +90,7 @@ def main(argv=None) -> int:
         kind = lines[0].strip().lower() if lines else ""
         instr.unlink()                                  # driver.f90:243
         if kind == "spectral":
-            return _forward(p, bg, psi, ddir, contrib, jobno, maxtime, compute_data,
-                            linesearch or compute_synth or compute_data)
+            return _forward(p, bg, psi, ddir, contrib, jobno, maxtime, compute_data, linesearch, compute_synth)
         if kind == "adjoint":
             return _adjoint(p, bg, psi, ddir, contrib, maxtime)
         print(f"unknown instruction '{kind}'", file=sys.stderr)
@@ -109,7 +108,7 @@ def _xsrc(ddir: Path, contrib: str) -> float:
     return vals[int(contrib) - 1]
 
 
-def _forward(p, bg, psi, ddir, contrib, jobno, maxtime, compute_data, no_snapshots) -> int:
+def _forward(p, bg, psi, ddir, contrib, jobno, maxtime, compute_data, linesearch, compute_synth) -> int:
     import torch
 
     from .solver import Sparc
@@ -124,6 +123,7 @@ def _forward(p, bg, psi, ddir, contrib, jobno, maxtime, compute_data, no_snapsho
         if (Path.cwd() / "true_psi.fits").exists():
             v0 = read_fits(Path.cwd() / "true_psi.fits").reshape(p.nz, p.nx)
     s = Sparc(p, bg, psi=v0 if v0 is not None else psi)
+    no_snapshots = linesearch or compute_synth or compute_data
     vz, sn = s.forward([_xsrc(ddir, contrib)], maxtime, snapshots=not no_snapshots)
     vz = vz[0].cpu().numpy()
     write_direct_records(out / "vz_cc.bin", vz)

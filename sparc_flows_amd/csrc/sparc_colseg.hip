@@ -197,22 +197,35 @@ __global__ __launch_bounds__(256) void k_colseg(Dims d, Bg bg, DbyCoef cf, SegTa
     //        d v_z/dz (ibc 1, :1076) and d v_x/dz (ibc 1, :1070): three
     //        independent solves, one exchange of segment end values
     window_rhs(ca.T[1], Yz, oz, 3, 0.0, st.topbc[(size_t)b * nx + x]);
-    seg_solve<MS>(Yz, zt + zrow(2, 0) * nzp, zt + zrow(2, 1) * nzp, zt + zrow(2, 2) * nzp);
     window_rhs(ca.T[3], Yv, ov, 1, 0.0, 0.0);
-    seg_solve<MS>(Yv, zt + zrow(0, 0) * nzp, zt + zrow(0, 1) * nzp, zt + zrow(0, 2) * nzp);
     window_rhs(ca.T[2], Yx, ox, 1, 0.0, 0.0);
-    seg_solve<MS>(Yx, zt + zrow(0, 0) * nzp, zt + zrow(0, 1) * nzp, zt + zrow(0, 2) * nzp);
+    // factor rows: from the shared interior table when every lane of the wave
+    // is in an interior segment (wave-uniform loads), else from the per-row table
+    const bool interior = act && ((sg.imask >> s) & 1u);
+    const bool wave_interior = __all(interior);
+    auto zrow_ptr = [&](int r) -> const double * { return zt + r * nzp; };
+    auto irow_ptr = [&](int r) -> const double * { return sg.itab + (r - 2) * MS; };
+    auto solves_b = [&](auto tab) {
+        seg_solve<MS>(Yz, tab(zrow(2, 0)), tab(zrow(2, 1)), tab(zrow(2, 2)));
+        seg_solve<MS>(Yv, tab(zrow(0, 0)), tab(zrow(0, 1)), tab(zrow(0, 2)));
+        seg_solve<MS>(Yx, tab(zrow(0, 0)), tab(zrow(0, 1)), tab(zrow(0, 2)));
+    };
+    if (wave_interior) solves_b(irow_ptr);
+    else solves_b(zrow_ptr);
     if (act) {
         yb[0][c * n2P + 2 * s] = Yz[0]; yb[0][c * n2P + 2 * s + 1] = Yz[MS - 1];
         yb[1][c * n2P + 2 * s] = Yv[0]; yb[1][c * n2P + 2 * s + 1] = Yv[MS - 1];
         yb[2][c * n2P + 2 * s] = Yx[0]; yb[2][c * n2P + 2 * s + 1] = Yx[MS - 1];
     }
     __syncthreads();
+    auto corrects_b = [&](auto tab) {
+        seg_correct<MS>(Yz, s, P, sg.hw, yb[0] + c * n2P, mv + 2 * nband, tab(zrow(2, 3)), tab(zrow(2, 4)));
+        seg_correct<MS>(Yv, s, P, sg.hw, yb[1] + c * n2P, mv, tab(zrow(0, 3)), tab(zrow(0, 4)));
+        seg_correct<MS>(Yx, s, P, sg.hw, yb[2] + c * n2P, mv, tab(zrow(0, 3)), tab(zrow(0, 4)));
+    };
     if (act) {
-        seg_correct<MS>(Yz, s, P, sg.hw, yb[0] + c * n2P, mv + 2 * nband, zt + zrow(2, 3) * nzp,
-                        zt + zrow(2, 4) * nzp);
-        seg_correct<MS>(Yv, s, P, sg.hw, yb[1] + c * n2P, mv, zt + zrow(0, 3) * nzp, zt + zrow(0, 4) * nzp);
-        seg_correct<MS>(Yx, s, P, sg.hw, yb[2] + c * n2P, mv, zt + zrow(0, 3) * nzp, zt + zrow(0, 4) * nzp);
+        if (wave_interior) corrects_b(irow_ptr);
+        else corrects_b(zrow_ptr);
     }
     // z-PML (:957-967), divergence, p and rho' (:990-998, :1043), the flow
     // terms of RHS v_z (:1073-1077) and the flow part of RHS v_x (:1067-1071)
@@ -269,11 +282,16 @@ __global__ __launch_bounds__(256) void k_colseg(Dims d, Bg bg, DbyCoef cf, SegTa
     __syncthreads();
     double (&Y)[MS] = Yz;
     window_rhs(st.p, Y, nullptr, 2, 0.0, 0.0);
-    seg_solve<MS>(Y, zt + zrow(1, 0) * nzp, zt + zrow(1, 1) * nzp, zt + zrow(1, 2) * nzp);
+    if (wave_interior) seg_solve<MS>(Y, irow_ptr(zrow(1, 0)), irow_ptr(zrow(1, 1)), irow_ptr(zrow(1, 2)));
+    else seg_solve<MS>(Y, zrow_ptr(zrow(1, 0)), zrow_ptr(zrow(1, 1)), zrow_ptr(zrow(1, 2)));
     if (act) { yb[3][c * n2P + 2 * s] = Y[0]; yb[3][c * n2P + 2 * s + 1] = Y[MS - 1]; }
     __syncthreads();
-    if (act) seg_correct<MS>(Y, s, P, sg.hw, yb[3] + c * n2P, mv + nband, zt + zrow(1, 3) * nzp,
-                             zt + zrow(1, 4) * nzp);
+    if (act) {
+        if (wave_interior)
+            seg_correct<MS>(Y, s, P, sg.hw, yb[3] + c * n2P, mv + nband, irow_ptr(zrow(1, 3)), irow_ptr(zrow(1, 4)));
+        else
+            seg_correct<MS>(Y, s, P, sg.hw, yb[3] + c * n2P, mv + nband, zrow_ptr(zrow(1, 3)), zrow_ptr(zrow(1, 4)));
+    }
     const int sslot = (LAST && sink.snap) ? out_slot(time, clk.steps, sink.k0, sink.nsnap) : -1;
     const bool snap = sslot >= 0;
     const size_t slot = snap ? ((size_t)b * sink.nsnap + sslot) * sink.nfield : 0;

@@ -87,7 +87,7 @@ def forward_with_checkpoints(fwd: Sparc, xsrc, maxtime: int, interval: int):
     fwd.set_forward_sources(xsrc, maxtime)
     fwd.reset()
     ck = fwd.checkpoint_buffer(nseg)
-    vz = torch.zeros(fwd.nsrc, nt, fwd.p.nx, dtype=torch.float64, device=fwd.device)
+    vz = torch.zeros(fwd.nsrc, nt, fwd.params.nx, dtype=torch.float64, device=fwd.device)
     for j in range(nseg):
         fwd.save_checkpoint(ck[j])
         fwd.run_range(j * interval, min(maxtime, j * interval + interval - 1), vz=vz, k0=0)
@@ -100,7 +100,7 @@ def kernels_from_checkpoints(fwd: Sparc, adj: Sparc, ck: torch.Tensor, vr: torch
     by segment, in reverse, from the checkpoints; forward output k is
     correlated with adjoint output nt-1-k (kernels.f90:200-360) as soon as
     both exist.  Returns the kernels of sparc_kernels_end."""
-    p, C, steps = fwd.p, interval, fwd.steps
+    p, C, steps = fwd.params, interval, fwd.steps
     nt = maxtime // steps + 1
     nseg = ck.shape[0]
     dev = fwd.device
