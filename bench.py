@@ -31,6 +31,22 @@ if ROOT not in sys.path:
 
 METRIC = "Mpoint-steps/s FP64 + s per fwd+adjoint gradient, batched vs plain, 1 GPU"
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "colseg_traffic_256x300.md")
+
+
+def measured_traffic(ms: int, nseg: int):
+    """HBM bytes per launch of the column pass from the committed rocprofv3
+    --pmc FETCH_SIZE / WRITE_SIZE summary, when it was taken with this
+    segmentation (the line `traffic: k_colseg<MS,0> P=.. fetch_kib=..
+    write_kib=.. fetch_scale=.. write_scale=..`); None otherwise."""
+    import re
+    if not os.path.exists(TRAFFIC_FILE):
+        return None
+    m = re.search(r"^traffic: k_colseg<(\d+),0> P=(\d+) fetch_kib=([\d.]+) write_kib=([\d.]+) "
+                  r"fetch_scale=([\d.]+) write_scale=([\d.]+)", open(TRAFFIC_FILE).read(), re.M)
+    if not m or int(m.group(1)) != ms or int(m.group(2)) != nseg:
+        return None
+    return (float(m.group(3)) * float(m.group(5)) + float(m.group(4)) * float(m.group(6))) * 1024.0
 
 
 def make_config(nx=256, nz=300, npml=12):
@@ -177,8 +193,9 @@ def main():
                    "row_first": ms["row_first"], "row_last": ms["row_last"]}
     dom = max(per_step_ms, key=per_step_ms.get)
     achieved = bytes_per_pt[dom] * npts / (ms[dom] * 1e-3) / 1e9
+    traffic = measured_traffic(pipe.fwd.query("colseg_ms"), pipe.fwd.query("colseg_p")) if dom == "col" else None
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dom,
+                "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": dom,
                 "ms_per_launch": ms, "algorithmic_bytes_per_point": bytes_per_pt[dom]}
 
     log("kernel profile done")

@@ -1,4 +1,5 @@
-"""Travel-time adjoint source and misfit: ADJOINT_SOURCE_FILT (driver.f90:1758-2238).
+"""Travel-time adjoint source and misfit: ADJOINT_SOURCE_FILT (driver.f90:1758-2238)
+and the per-mode, per-band misfit summary MISFIT_ALL (driver.f90:2294-2651).
 
 After a forward run the reference filters the synthetic vz traces and the
 observed ones with ridge filters in (k, nu) (f mode, p1..p7, all p modes;
@@ -23,7 +24,10 @@ Deviations, each where the reference's behaviour is undefined:
   variable; the sums start at 0 here.
 * the search range timest..timefin of a receiver is clipped to the record,
   and a receiver whose range lies wholly outside it is not measured (the
-  reference indexes outside the array in both cases).
+  reference indexes outside the array in both cases); windows are clipped
+  to the record the same way.
+* a line search whose stored window list is shorter than its receivers
+  stops measuring that mode (the reference reuses the last window read).
 """
 from __future__ import annotations
 
@@ -227,10 +231,11 @@ def adjoint_source_filt(acc: np.ndarray, dat: np.ndarray, x: np.ndarray, x00: fl
                     timefin = timest + 40
                 else:
                     dist = distances[i]
-                    timest = int(math.floor((-0.000417147774671 * dist ** 2 + 0.313350998096 * dist
-                                             + 17.9609631186) * 1.0 / dt_min))
-                    timefin = int(math.floor((-0.00028361249034 * dist ** 2 + 0.29270337114 * dist
-                                              + 36.4398734578) * 1.0 / dt_min))
+                    # default-REAL literals: float32 constants (driver.f90:2046-2049)
+                    timest = int(math.floor((f32(-0.000417147774671) * dist ** 2 + f32(0.313350998096) * dist
+                                             + f32(17.9609631186)) * 1.0 / dt_min))
+                    timefin = int(math.floor((f32(-0.00028361249034) * dist ** 2 + f32(0.29270337114) * dist
+                                              + f32(36.4398734578)) * 1.0 / dt_min))
                 # 1-based timest..timefin, clipped to the record; a receiver
                 # whose search range lies outside the record is not measured
                 t0, t1 = max(timest, 1), min(timefin, nt)
@@ -255,6 +260,89 @@ def adjoint_source_filt(acc: np.ndarray, dat: np.ndarray, x: np.ndarray, x00: fl
             adj[::-1, :] += (filtex * con).real
     adj = adj * nt / 60.0
     return AdjointSource(adj=adj, misfit=0.5 * misfit, nmeas=nmeas, measurements=meas)
+
+
+# MISFIT_ALL (driver.f90:2294-2651): the misfit of every mode in three
+# frequency bands, written to kernel/misfit_all_SS_LL.
+MISFIT_BANDS = ((2.0, 3.5), (3.5, 4.5), (4.5, 6.0))      # mHz (driver.f90:2400-2411)
+
+
+@dataclass
+class MisfitAll:
+    bands: list          # per band: (left_mHz, right_mHz, nmeas[0..10], 0.5 * sum tau^2 [0..10])
+    windows: dict        # {(mode, band 1..3): [(lef, rig), ...]}, as windows.all.MM.B holds them
+
+    def text(self, index: int) -> str:
+        """The file misfit_all_SS_LL: per band a '#' line with the source
+        index and band edges, a '#' line with the 11 measurement counts and a
+        line with the 11 half misfits (driver.f90:2627-2632; list-directed
+        output, read back with numpy.loadtxt(comments='#'))."""
+        out = []
+        for f1, f2, n, m in self.bands:
+            out.append(f" # {index:d} {f1:.16E} {f2:.16E}")
+            out.append(" # " + " ".join(f"{int(v):d}" for v in n))
+            out.append(" " + " ".join(f"{v:.16E}" for v in m))
+        return "\n".join(out) + "\n"
+
+
+def misfit_all(acc: np.ndarray, dat: np.ndarray, x: np.ndarray, x00: float, xlength_cm: float,
+               outputcad: float, dt_min: float, mode_params: dict, windows: dict | None = None) -> MisfitAll:
+    """MISFIT_ALL for one source: for each band of MISFIT_BANDS, every ridge
+    filter with a params.<mode> file times the band's FREQ_FILTER (no
+    chaining between filters here), windows picked as in
+    ADJOINT_SOURCE_FILT except the all-p-mode polynomials
+    (driver.f90:2503-2512), the Gizon-Birch travel time per receiver, and
+    sums of tau^2 per mode.  `windows` ({(mode, band): [(lef, rig)]}) replaces
+    the picking, as the linesearch branch reads windows.all.MM.B."""
+    nt, nx = acc.shape
+    distances = np.abs((x - 0.5) * xlength_cm * f32(1e-8) - x00)
+    tempout = _fftw_fwd(acc.astype(np.complex128))
+    tempdat = _fftw_fwd(dat.astype(np.complex128))
+    ridges = {m: ridge_filter(name, nt, nx, xlength_cm, outputcad) for m, name in _MODE_NAMES.items()}
+    bands, wins = [], {}
+    for band, (f1, f2) in enumerate(MISFIT_BANDS, start=1):
+        filt = freq_filter(f1, f2, nt, outputcad)
+        mis = np.zeros(11)
+        nmeas = np.zeros(11, dtype=np.int64)
+        for pord in range(9):
+            if pord not in mode_params:
+                continue
+            mindist, maxdist, window = mode_params[pord]
+            halftime = int(math.floor(window / (2.0 * dt_min) + 0.5))
+            F = ridges[pord] * filt[:, None]
+            accf = _fftw_bwd(tempout * F).real
+            datf = _fftw_bwd(tempdat * F).real
+            given = list(windows.get((pord, band), [])) if windows else None
+            picked = []
+            for i in range(nx):
+                if not (distances[i] > mindist and distances[i] < maxdist):
+                    continue
+                if given is None:
+                    if pord != 8:
+                        timest = int(math.floor(distances[i] / f32(_VEL[pord]) * 1.0 / dt_min))
+                        timefin = timest + 40
+                    else:
+                        d_i = distances[i]
+                        timest = int(math.floor((f32(-1.45511095e-04) * d_i ** 2 + f32(2.72140632e-01) * d_i
+                                                 + f32(2.16331969e+01)) * 1.0 / dt_min))
+                        timefin = int(math.floor((f32(3.85746516e-08) * d_i ** 2 + f32(2.21324786e-01) * d_i
+                                                  + f32(4.36702155e+01)) * 1.0 / dt_min))
+                    t0, t1 = max(timest, 1), min(timefin, nt)
+                    if t0 > t1:
+                        continue
+                    loc = int(np.argmax(np.abs(accf[t0 - 1:t1, i]))) + t0
+                    lef, rig = max(loc - halftime, 1), min(loc + halftime, nt)
+                else:
+                    if not given:
+                        break
+                    lef, rig = given.pop(0)
+                picked.append((lef, rig))
+                tau = tt_gizon_birch(accf[:, i], datf[:, i], dt_min, lef - 1, rig - 1)
+                mis[pord] += tau ** 2
+                nmeas[pord] += 1
+            wins[(pord, band)] = picked
+        bands.append((f1, f2, nmeas, 0.5 * mis))
+    return MisfitAll(bands=bands, windows=wins)
 
 
 def read_mode_params(directory: Path) -> tuple[dict, dict]:

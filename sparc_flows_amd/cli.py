@@ -155,10 +155,11 @@ def _fortran_tt_line(m) -> str:
 
 
 def _adjoint_source(p, ddir, contrib, jobno, vz, cadence_s, linesearch, compute_synth) -> None:
-    """ADJOINT_SOURCE_FILT after a forward run (driver.f90:271-276): travel
-    times against data/SS.fits, ttdiff.<mode>, windows.<mode>, the adjoint
-    source and the misfit file."""
-    from .adjsrc import adjoint_source_filt, read_mode_params
+    """ADJOINT_SOURCE_FILT and MISFIT_ALL after a forward run
+    (driver.f90:271-276): travel times against data/SS.fits, ttdiff.<mode>,
+    windows.<mode>, the adjoint source, the misfit file and the per-mode,
+    per-band misfit_all file with its windows.all.MM.B."""
+    from .adjsrc import MISFIT_BANDS, adjoint_source_filt, misfit_all, read_mode_params
 
     data = ddir / "data" / f"{contrib}.fits"
     if not data.exists():
@@ -198,6 +199,24 @@ def _adjoint_source(p, ddir, contrib, jobno, vz, cadence_s, linesearch, compute_
     kdir.mkdir(parents=True, exist_ok=True)
     (kdir / f"misfit_{contrib}_{jobno}").write_text(f"{int(contrib):12d}{r.nmeas:12d} {r.misfit:24.15E}\n")
     print(f"adjoint source: {r.nmeas} measurements, misfit {r.misfit:.6e}")
+    # MISFIT_ALL (driver.f90:276): per-mode misfits in three bands; windows.all.MM.B
+    # written by a plain forward run, read back by a line search
+    wall = None
+    if linesearch:
+        wall = {}
+        for m in modes:
+            for band in range(1, len(MISFIT_BANDS) + 1):
+                f = win_dir / f"windows.all.{m:02d}.{band}"
+                if f.exists():
+                    v = [int(t) for t in f.read_text().split()]
+                    wall[(m, band)] = list(zip(v[0::2], v[1::2]))
+    ma = misfit_all(vz, dat, np.arange(p.nx) / p.nx, _xsrc(ddir, contrib), p.xlength, p.outputcad,
+                    cadence_s / 60.0, modes, wall)
+    if not linesearch:
+        win_dir.mkdir(parents=True, exist_ok=True)
+        for (m, band), rows in ma.windows.items():
+            (win_dir / f"windows.all.{m:02d}.{band}").write_text("".join(f"{a:12d}{b:12d}\n" for a, b in rows))
+    (kdir / f"misfit_all_{contrib}_{jobno}").write_text(ma.text(int(contrib)))
 
 
 def _adjoint(p, bg, psi, ddir, contrib, maxtime) -> int:

@@ -689,27 +689,11 @@ static int build_seg(sparc_handle *h, int m, SegTab &sg)
                 if (q >= 0 && q < n2)
                     band[((size_t)slot * n2 + k) * W + w] = minv_all[((size_t)slot * n2 + k) * n2 + q];
             }
-    // segments whose factor rows (ztab rows 2-16) equal those of the middle
-    // segment share one table, which the kernel reads with wave-uniform loads
-    std::vector<double> itab((size_t)15 * m);
-    const int sm = P / 2;
-    for (int r = 0; r < 15; r++)
-        for (int i = 0; i < m; i++) itab[(size_t)r * m + i] = ztab[(size_t)(2 + r) * nzp + sm * m + i];
-    sg.imask = 0;
-    for (int sgi = 0; sgi < P; sgi++) {
-        bool same = true;
-        for (int r = 0; r < 15 && same; r++)
-            for (int i = 0; i < m && same; i++)
-                same = ztab[(size_t)(2 + r) * nzp + sgi * m + i] == itab[(size_t)r * m + i];
-        if (same) sg.imask |= 1u << sgi;
-    }
     double *p = nullptr;
     HCHECK(h, upload(h->bg_mem, &p, ztab, h->stream));
     sg.ztab = p;
     HCHECK(h, upload(h->bg_mem, &p, band, h->stream));
     sg.Minv = p;
-    HCHECK(h, upload(h->bg_mem, &p, itab, h->stream));
-    sg.itab = p;
     return SPARC_OK;
 }
 
@@ -1224,7 +1208,7 @@ static int autotune(sparc_handle *h)
     int force_tc = 0;
     if (const char *e = std::getenv("SPARC_COLSEG_TC")) {
         const int v = std::atoi(e);
-        if (v == 4 || v == 8 || v == 16) force_tc = v;
+        if (v == 1 || v == 2 || v == 4 || v == 8 || v == 16) force_tc = v;
     }
     // candidate segment lengths (only the forced one under SPARC_COLSEG_MS)
     std::vector<int> ms = h->seg_forced ? std::vector<int>{h->seg.m} : seg_lengths(h->nz);
@@ -1255,7 +1239,7 @@ static int autotune(sparc_handle *h)
             if (r) return r;
             h->seg_cands.push_back(sg);
         }
-        for (int tc : {4, 8, 16}) {
+        for (int tc : {1, 2, 4, 8, 16}) {
             if (force_tc && tc != force_tc) continue;
             if (h->nx % tc || tc * sg.P > 256) continue;
             if (launch_colseg(tc, m, h->d, h->bg, h->cf, sg, s, ca, h->src, clk, sk, h->stream) != hipSuccess) {

@@ -62,8 +62,6 @@ struct SegTab {
     int P, m, hw, nzp;              // segments, rows per segment, Minv window, P*m
     const double *ztab;             // [kZT][nzp]
     const double *Minv;             // [3][2P][4hw+4]: row k of each inverse, entries 2(k/2-hw)+w
-    const double *itab;             // [15][m]: solve rows 2-16 of ztab shared by the interior segments
-    unsigned imask;                 // bit s set: segment s uses itab (its rows 2-16 equal it)
 };
 
 // Prognostic state and work arrays of a batch (device pointers).

@@ -172,6 +172,15 @@ __global__ __launch_bounds__(64) void k_row(Dims d, Bg bg, State st, RowArgs ra,
 {
     constexpr int N1 = FftShape<NX>::N1, N2 = FftShape<NX>::N2, TEAMS = 64 / N1;
     __shared__ double2 lds[TEAMS][N2 * (N1 + 1)];
+    // twiddles and bin multipliers staged in LDS: their reads then never
+    // wait behind this pass's global stores
+    __shared__ double2 stw[NX];
+    __shared__ double sdm[NX];
+#pragma unroll
+    for (int q = threadIdx.x; q < NX; q += 64) {
+        stw[q] = bg.tw[q];
+        sdm[q] = bg.dmul[q];
+    }
     const int team = threadIdx.x / N1, t = threadIdx.x % N1;
     const int r = blockIdx.x * TEAMS + team;
     const bool valid = r < d.S * d.nz;
@@ -209,7 +218,8 @@ __global__ __launch_bounds__(64) void k_row(Dims d, Bg bg, State st, RowArgs ra,
                     sink.snap[((((size_t)b * sink.nsnap + sslot) * sink.nfield + fslot) * nz + k) * NX + x] = vx[j2];
             }
         }
-        fft_deriv_team<N1, N2>(re, im, t, lds[team], bg.tw, bg.dmul, bg.stretchx);
+        __syncthreads();   // staged twiddles visible
+        fft_deriv_team<N1, N2>(re, im, t, lds[team], stw, sdm, bg.stretchx);
         // re = dxi_x/dx, im = dv_x/dx
         double zr[N2], zi[N2];
 #pragma unroll
@@ -217,7 +227,7 @@ __global__ __launch_bounds__(64) void k_row(Dims d, Bg bg, State st, RowArgs ra,
             zr[j2] = ra.v_z[rowq + t + N1 * j2];
             zi[j2] = 0.0;
         }
-        fft_deriv_team<N1, N2>(zr, zi, t, lds[team], bg.tw, bg.dmul, bg.stretchx);
+        fft_deriv_team<N1, N2>(zr, zi, t, lds[team], stw, sdm, bg.stretchx);
 #pragma unroll
         for (int j2 = 0; j2 < N2; j2++) {
             const int x = t + N1 * j2;
@@ -240,7 +250,8 @@ __global__ __launch_bounds__(64) void k_row(Dims d, Bg bg, State st, RowArgs ra,
             avx[j2] = st.a[2][rowq + x];
             rinv[j2] = bg.rhoinv_z[k];
         }
-        fft_deriv_team<N1, N2>(re, im, t, lds[team], bg.tw, bg.dmul, bg.stretchx);
+        __syncthreads();   // staged twiddles visible
+        fft_deriv_team<N1, N2>(re, im, t, lds[team], stw, sdm, bg.stretchx);
         // x-PML on grad_x p (physics2d.f90:1003-1020), RHS v_x (:1039 + flows)
         const int sslot = (MODE == 2 && sink.snap) ? out_slot(time, clk.steps, sink.k0, sink.nsnap) : -1;
         const bool snap = sslot >= 0;
@@ -279,7 +290,7 @@ __global__ __launch_bounds__(64) void k_row(Dims d, Bg bg, State st, RowArgs ra,
                 wr[j2] = vx[j2];
                 wi[j2] = ra.v_z[rowq + t + N1 * j2];
             }
-            fft_deriv_team<N1, N2>(wr, wi, t, lds[team], bg.tw, bg.dmul, bg.stretchx);
+            fft_deriv_team<N1, N2>(wr, wi, t, lds[team], stw, sdm, bg.stretchx);
 #pragma unroll
             for (int j2 = 0; j2 < N2; j2++) {
                 const int x = t + N1 * j2;

@@ -136,3 +136,37 @@ def test_adjoint_source_is_the_misfit_gradient():
         if abs(ad - fd) < 0.1 * abs(fd):
             ok += 1
     assert ok >= 3
+
+
+def test_misfit_all_matches_adjoint_source_measurements():
+    """MISFIT_ALL's band-1 all-p-mode misfit equals ADJOINT_SOURCE_FILT's
+    misfit with the same band as its all-modes frequency filter and the same
+    windows (both are 0.5 sum tau^2 of the same filtered traces)."""
+    dt_min = 0.5
+    acc, dat, x, t = _traces(dt_min=dt_min)
+    modes = {8: (5.0, 60.0, 30.0)}
+    ma = A.misfit_all(acc, dat, x, 0.0, 200e8, dt_min * 60.0, dt_min, modes)
+    wins = ma.windows[(8, 1)]
+    assert len(wins) > 0
+    band = A.MISFIT_BANDS[0]
+    r = A.adjoint_source_filt(acc, dat, x, 0.0, 200e8, dt_min * 60.0, dt_min, modes,
+                              freq_params={"allmodes": band}, windows={8: list(wins)})
+    f1, f2, nmeas, half = ma.bands[0]
+    assert (f1, f2) == band and nmeas[8] == r.nmeas == len(wins)
+    assert abs(half[8] - r.misfit) <= 1e-12 * abs(r.misfit)
+    # reusing the picked windows (the linesearch branch) reproduces every band
+    again = A.misfit_all(acc, dat, x, 0.0, 200e8, dt_min * 60.0, dt_min, modes, windows=ma.windows)
+    for (_, _, n0, m0), (_, _, n1, m1) in zip(ma.bands, again.bands):
+        assert np.array_equal(n0, n1) and np.allclose(m0, m1, rtol=1e-14, atol=0)
+
+
+def test_misfit_all_file_reads_with_loadtxt(tmp_path):
+    acc, dat, x, t = _traces()
+    ma = A.misfit_all(acc, dat, x, 0.0, 200e8, 30.0, 0.5, {8: (5.0, 60.0, 30.0), 0: (3.0, 40.0, 20.0)})
+    f = tmp_path / "misfit_all_01_00"
+    f.write_text(ma.text(1))
+    v = np.loadtxt(f, comments="#")
+    assert v.shape == (3, 11)
+    for b in range(3):
+        assert np.allclose(v[b], ma.bands[b][3], rtol=1e-15)
+    assert np.all(v[:, 1:8] == 0) and np.all(v[:, 9:] == 0)

@@ -93,6 +93,13 @@ def test_forward_adjoint_kernel_phases(tmp_path):
     assert np.array_equal(vr, r.adj)
     assert (fdir / "ttdiff.8").read_text().count("\n") == r.nmeas
     assert (data / "kernel" / "misfit_01_00").exists() and (adir / "information").exists()
+    # MISFIT_ALL: three bands of 11 half misfits, and the band windows it picked
+    from sparc_flows_amd.adjsrc import misfit_all
+    ma = misfit_all(vz, vz_obs[0].cpu().numpy(), np.arange(p.nx) / p.nx, -12.0, p.xlength, p.outputcad,
+                    p.steps * p.timestep / 60.0, {8: (3.0, 40.0, 10.0)})
+    v = np.loadtxt(data / "kernel" / "misfit_all_01_00", comments="#")
+    assert v.shape == (3, 11) and np.allclose(v, [b[3] for b in ma.bands], rtol=1e-15, atol=0)
+    assert (fdir / "windows.all.08.1").read_text().count("\n") == len(ma.windows[(8, 1)])
     # adjoint (instruction 'adjoint') from that source.fits
     (tmp_path / "Instruction_src01_ls00").write_text("adjoint\n")
     assert _cli(tmp_path, data, maxtime, "01", "00") == 0
