@@ -34,19 +34,19 @@ HBM_PEAK_GBS = 8000.0
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "colseg_traffic_256x300.md")
 
 
-def measured_traffic(ms: int, nseg: int):
+def measured_traffic(kernel: str, ms: int, nseg: int):
     """HBM bytes per launch of the column pass from the committed rocprofv3
-    --pmc FETCH_SIZE / WRITE_SIZE summary, when it was taken with this
-    segmentation (the line `traffic: k_colseg<MS,0> P=.. fetch_kib=..
+    --pmc FETCH_SIZE / WRITE_SIZE summary, when it was taken with this kernel
+    and segmentation (the line `traffic: <kernel><MS,0> P=.. fetch_kib=..
     write_kib=.. fetch_scale=.. write_scale=..`); None otherwise."""
     import re
     if not os.path.exists(TRAFFIC_FILE):
         return None
-    m = re.search(r"^traffic: k_colseg<(\d+),0> P=(\d+) fetch_kib=([\d.]+) write_kib=([\d.]+) "
+    m = re.search(r"^traffic: (k_col\w+)<(\d+),0> P=(\d+) fetch_kib=([\d.]+) write_kib=([\d.]+) "
                   r"fetch_scale=([\d.]+) write_scale=([\d.]+)", open(TRAFFIC_FILE).read(), re.M)
-    if not m or int(m.group(1)) != ms or int(m.group(2)) != nseg:
+    if not m or m.group(1) != kernel or int(m.group(2)) != ms or int(m.group(3)) != nseg:
         return None
-    return (float(m.group(3)) * float(m.group(5)) + float(m.group(4)) * float(m.group(6))) * 1024.0
+    return (float(m.group(4)) * float(m.group(6)) + float(m.group(5)) * float(m.group(7))) * 1024.0
 
 
 def make_config(nx=256, nz=300, npml=12):
@@ -80,7 +80,7 @@ def forward_rate(nx, nz, nsrc, nsteps, flow, dc, reps=1):
     sec = (time.perf_counter() - t0) / reps
     return {"Mpoint_steps_per_s": nx * nz * nsteps * nsrc / sec / 1e6, "s_per_run": sec,
             "grid": [nx, nz], "sources": nsrc, "time_steps": nsteps, "flow": flow, "dc": dc,
-            "column_pass_shape": {k: s.query(k) for k in ("colseg_tc", "colseg_ms", "colseg_p")}}
+            "column_pass_shape": {k: s.query(k) for k in ("colseg_tc", "colseg_ms", "colseg_p", "colseg_split")}}
 
 
 def cpu_baseline(p, bg, psi, seconds=12.0):
@@ -164,6 +164,11 @@ def main():
     value = pts / sec / 1e6
     finite = bool(torch.isfinite(out["kc"]).all() and torch.isfinite(out["kpsi"]).all())
 
+    npts = p.nx * p.nz
+    C = 2  # x-varying coefficient arrays read per point (v0x, v0z)
+    bytes_per_pt = {"col": (14 + C) * 8, "col_last": (14 + C) * 8, "row": 9 * 8, "row_first": 7 * 8,
+                    "row_last": 4 * 8}
+
     # batched gradient of args.batch sources vs the plain one
     batched = None
     if args.batch > 0:
@@ -178,22 +183,26 @@ def main():
         torch.cuda.synchronize()
         bsec = time.perf_counter() - tb0
         log(f"batched gradient {bsec:.2f} s")
+        # the batch's column pass against HBM: every launch moves args.batch
+        # times the single-source bytes, more than the 256 MB MALL holds
+        bms = bpipe.fwd.profile_kernels(200)
+        bach = bytes_per_pt["col"] * npts * args.batch / (bms["col"] * 1e-3) / 1e9
         batched = {"sources": args.batch, "s_per_gradient": bsec,
                    "Mpoint_steps_per_s": pts * args.batch / bsec / 1e6,
-                   "speedup_vs_sequential_plain": (sec * args.batch) / bsec}
+                   "speedup_vs_sequential_plain": (sec * args.batch) / bsec,
+                   "roofline_col": {"achieved": bach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                    "frac": bach / HBM_PEAK_GBS, "ms_per_launch": bms}}
         del bpipe
 
     # roofline of the dominant kernel: HIP events on the solver's stream
     ms = pipe.fwd.profile_kernels(200)
-    npts = p.nx * p.nz
-    C = 2  # x-varying coefficient arrays read per point (v0x, v0z)
-    bytes_per_pt = {"col": (14 + C) * 8, "col_last": (14 + C) * 8, "row": 9 * 8, "row_first": 7 * 8,
-                    "row_last": 4 * 8}
     per_step_ms = {"col": 4 * ms["col"], "col_last": ms["col_last"], "row": 4 * ms["row"],
                    "row_first": ms["row_first"], "row_last": ms["row_last"]}
     dom = max(per_step_ms, key=per_step_ms.get)
     achieved = bytes_per_pt[dom] * npts / (ms[dom] * 1e-3) / 1e9
-    traffic = measured_traffic(pipe.fwd.query("colseg_ms"), pipe.fwd.query("colseg_p")) if dom == "col" else None
+    colk = "k_colsplit" if pipe.fwd.query("colseg_split") else "k_colseg"
+    traffic = (measured_traffic(colk, pipe.fwd.query("colseg_ms"), pipe.fwd.query("colseg_p"))
+               if dom == "col" else None)
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": dom,
                 "ms_per_launch": ms, "algorithmic_bytes_per_point": bytes_per_pt[dom]}
@@ -218,7 +227,8 @@ def main():
         "finite": finite,
         "batched": batched,
         "roofline": roofline,
-        "column_pass_shape": {k: pipe.fwd.query(k) for k in ("colseg_tc", "colseg_ms", "colseg_p", "colseg_hw")},
+        "column_pass_shape": {k: pipe.fwd.query(k) for k in ("colseg_tc", "colseg_ms", "colseg_p", "colseg_hw",
+                                                             "colseg_split")},
         "cpu_baseline": cpu,
         "other_configurations": others,
     }

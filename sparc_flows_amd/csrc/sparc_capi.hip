@@ -97,8 +97,17 @@ struct sparc_handle {
     // segmented column pass: tables and launch shape (chosen by autotune)
     SegTab seg{};
     int seg_tc = 16, seg_ms = 20;
+    int seg_split = 0;                   // 1: one lane per (column, segment, field)
     bool tuned = false;
 };
+
+// One launch of the column pass in the handle's chosen form.
+static hipError_t launch_col(const sparc_handle *h, int split, int tc, int ms, const SegTab &sg,
+                             const ColArgs &ca, const Clock &clk, const Sink &sk)
+{
+    return split ? launch_colsplit(tc, ms, h->d, h->bg, h->cf, sg, h->st, ca, h->src, clk, sk, h->stream)
+                 : launch_colseg(tc, ms, h->d, h->bg, h->cf, sg, h->st, ca, h->src, clk, sk, h->stream);
+}
 
 static thread_local std::string g_create_err;
 
@@ -719,10 +728,15 @@ static int setup_seg(sparc_handle *h)
     h->seg = sg;
     h->seg_ms = m;
     h->seg_cands.push_back(sg);
-    // untuned block width: the widest that fits
-    h->seg_tc = 4;
-    for (int tc : {8, 16})
-        if (h->nx % tc == 0 && tc * sg.P <= 256) h->seg_tc = tc;
+    // untuned form and block width: SPARC_COLSEG_SPLIT=1 selects one lane
+    // per (column, segment, field); the widest block that fits
+    h->seg_split = 0;
+    if (const char *e = std::getenv("SPARC_COLSEG_SPLIT")) h->seg_split = std::atoi(e) == 1;
+    h->seg_tc = 1;
+    for (int tc : {2, 4, 8, 16}) {
+        const bool fits = h->seg_split ? (tc * sg.P + 63) / 64 <= 2 : tc * sg.P <= 256;
+        if (h->nx % tc == 0 && fits) h->seg_tc = tc;
+    }
     return SPARC_OK;
 }
 
@@ -827,8 +841,7 @@ static int enqueue_step(sparc_handle *h, int toff, bool filt_vx, const Sink &sk)
         ca.last = stg == 5;
         ca.bnext = stg == 5 ? h->deltat : b[stg];
         int r = timed_launch(h, stg == 5 ? 1 : 0, [&] {
-            HCHECK(h, launch_colseg(h->seg_tc, h->seg_ms, d, h->bg, h->cf, h->seg, s, ca, h->src, clk, sk,
-                                    h->stream));
+            HCHECK(h, launch_col(h, h->seg_split, h->seg_tc, h->seg_ms, h->seg, ca, clk, sk));
             return SPARC_OK;
         });
         if (r) return r;
@@ -983,6 +996,7 @@ int sparc_query_int(const sparc_handle *h, const char *name, int *v)
     else if (n == "nzpml") *v = h->d.nzpml;
     else if (n == "nph") *v = h->d.nph;
     else if (n == "colseg_tc") *v = h->seg_tc;
+    else if (n == "colseg_split") *v = h->seg_split;
     else if (n == "colseg_ms") *v = h->seg_ms;
     else if (n == "colseg_p") *v = h->seg.P;
     else if (n == "colseg_hw") *v = h->seg.hw;
@@ -1198,22 +1212,29 @@ int sparc_step(sparc_handle *h, int t)
     return SPARC_OK;
 }
 
-// Pick the column-block width of the segmented column pass for this grid and
-// batch by timing each candidate on the handle's stream (state is reset
-// afterwards).  SPARC_COLSEG_TC in the environment overrides the choice.
+// Pick the segment length, column-block width and form (one lane per column
+// and segment, or per column, segment and field) of the column pass for this
+// grid and batch by timing each candidate on the handle's stream (state is
+// reset afterwards).  SPARC_COLSEG_MS, SPARC_COLSEG_TC and SPARC_COLSEG_SPLIT
+// (0 or 1) in the environment force a choice.
 static int autotune(sparc_handle *h)
 {
     if (h->tuned) return SPARC_OK;
     h->tuned = true;
-    int force_tc = 0;
+    int force_tc = 0, force_split = -1;
     if (const char *e = std::getenv("SPARC_COLSEG_TC")) {
         const int v = std::atoi(e);
         if (v == 1 || v == 2 || v == 4 || v == 8 || v == 16) force_tc = v;
     }
+    if (const char *e = std::getenv("SPARC_COLSEG_SPLIT")) {
+        const int v = std::atoi(e);
+        if (v == 0 || v == 1) force_split = v;
+    }
     // candidate segment lengths (only the forced one under SPARC_COLSEG_MS)
     std::vector<int> ms = h->seg_forced ? std::vector<int>{h->seg.m} : seg_lengths(h->nz);
-    if (force_tc && ms.size() == 1) {
+    if (force_tc && force_split >= 0 && ms.size() == 1) {
         if (h->nx % force_tc == 0 && force_tc * h->seg.P <= 256) h->seg_tc = force_tc;
+        h->seg_split = force_split;
         return SPARC_OK;
     }
     Sink sk = empty_sink(h);
@@ -1228,7 +1249,7 @@ static int autotune(sparc_handle *h)
     HCHECK(h, hipEventCreate(&e1));
     float best = 1e30f;
     SegTab best_sg = h->seg;
-    int best_tc = 0;
+    int best_tc = 0, best_split = 0;
     for (int m : ms) {
         SegTab sg{};
         bool have = false;
@@ -1239,21 +1260,22 @@ static int autotune(sparc_handle *h)
             if (r) return r;
             h->seg_cands.push_back(sg);
         }
+        for (int split : {0, 1})
         for (int tc : {1, 2, 4, 8, 16}) {
             if (force_tc && tc != force_tc) continue;
+            if (force_split >= 0 && split != force_split) continue;
             if (h->nx % tc || tc * sg.P > 256) continue;
-            if (launch_colseg(tc, m, h->d, h->bg, h->cf, sg, s, ca, h->src, clk, sk, h->stream) != hipSuccess) {
+            if (launch_col(h, split, tc, m, sg, ca, clk, sk) != hipSuccess) {
                 (void)hipGetLastError();     // shape not launchable here (LDS): skip it
                 continue;
             }
             HCHECK(h, hipEventRecord(e0, h->stream));
-            for (int r = 0; r < 10; r++)
-                HCHECK(h, launch_colseg(tc, m, h->d, h->bg, h->cf, sg, s, ca, h->src, clk, sk, h->stream));
+            for (int r = 0; r < 10; r++) HCHECK(h, launch_col(h, split, tc, m, sg, ca, clk, sk));
             HCHECK(h, hipEventRecord(e1, h->stream));
             HCHECK(h, hipEventSynchronize(e1));
             float t = 0.f;
             HCHECK(h, hipEventElapsedTime(&t, e0, e1));
-            if (t < best) { best = t; best_sg = sg; best_tc = tc; }
+            if (t < best) { best = t; best_sg = sg; best_tc = tc; best_split = split; }
         }
     }
     (void)hipEventDestroy(e0);
@@ -1262,6 +1284,7 @@ static int autotune(sparc_handle *h)
     h->seg = best_sg;
     h->seg_ms = best_sg.m;
     h->seg_tc = best_tc;
+    h->seg_split = best_split;
     // the timed launches wrote only stage temporaries: restore the z-PML
     // stage copy, which equals its base between steps
     const size_t pz = (size_t)2 * h->S * h->d.nzpml * h->nx;

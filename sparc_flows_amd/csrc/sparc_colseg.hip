@@ -159,7 +159,20 @@ __global__ __launch_bounds__(256) void k_colseg(Dims d, Bg bg, DbyCoef cf, SegTa
         }
     };
     const int c = tid % TC, s = tid / TC;
-    const int x = blockIdx.x * TC + c, b = blockIdx.y;
+    // XCD-aware block order: workgroups are dealt to the 8 XCDs in turn, so
+    // workgroup id is remapped to give each XCD a contiguous range of column
+    // groups; neighbouring groups share 128-byte lines, now in one L2.
+    int bx = blockIdx.x, by = blockIdx.y;
+    {
+        const int n = gridDim.x * gridDim.y;
+        if ((n & 7) == 0) {
+            const int id = blockIdx.x + gridDim.x * blockIdx.y;
+            const int l = (id & 7) * (n >> 3) + (id >> 3);
+            bx = l % gridDim.x;
+            by = l / gridDim.x;
+        }
+    }
+    const int x = bx * TC + c, b = by;
     const int nx = d.nx, nz = d.nz;
     const bool act = s < P;
     const int sc = act ? s : 0;
@@ -463,6 +476,326 @@ __global__ __launch_bounds__(256) void k_colseg(Dims d, Bg bg, DbyCoef cf, SegTa
     }
 }
 
+// ---------------------------------------------------------------------------
+// The same column pass with one lane per (column, segment, field): the three
+// independent first derivatives (xi_z, v_z, v_x) run in three lane groups of
+// G = roundup(TC*P, 64) <= 128 threads each (blockDim = 3G, so every wave
+// holds one field), which splits the window loads, solves, corrections and epilogue
+// work of the first half three ways.  Group 0 (xi_z) then does d p/dz and
+// the v_z update; group 1 writes the xi_z update and passes its part of RHS
+// v_z through LDS; group 2 writes RHS v_x and the xi_x update.  After the
+// last stage each group filters one field.  Store/load order across groups:
+// xi_z (TN[1]) is written only after the first barrier, when every lane's
+// window reads of T[1] are complete (T and TN are one buffer after the first
+// stage); v_z (TN[3]) after the third.
+//
+// Dynamic LDS: as k_colseg, plus the group-1 part of RHS v_z [TC][nzp].
+inline size_t colsplit_lds_bytes(int tc, int P, int hw, int nzp)
+{
+    return colseg_lds_bytes(tc, P, hw, nzp) + sizeof(double) * (size_t)tc * nzp;
+}
+
+template <int MS, int LAST>
+__global__ __launch_bounds__(384) void k_colsplit(Dims d, Bg bg, DbyCoef cf, SegTab sg, State st,
+                                                  ColArgs ca, Src src, Clock clk, Sink sink, int TC)
+{
+    extern __shared__ double lds_dyn[];
+    const int P = sg.P, n2P = 2 * P, nzp = sg.nzp;
+    const int nband = n2P * (4 * sg.hw + 4);
+    double *const mv = lds_dyn;
+    double *const yb0 = lds_dyn + 3 * nband;
+    double *const yb[4] = {yb0, yb0 + TC * n2P, yb0 + 2 * TC * n2P, yb0 + 3 * TC * n2P};
+    double *const zl = yb0 + 4 * TC * n2P;
+    double *const wl = zl + kZL * nzp;                 // [TC][nzp] group-1 part of RHS v_z
+    const int tid = threadIdx.x, G = blockDim.x / 3;
+    const int f = tid / G, l = tid - f * G;            // f is wave-uniform
+    const int c = l % TC, s = l / TC;
+    const bool act = s < P;
+    const int sc = act ? s : 0;
+    const int a = sc * MS;
+    const bool first = act && s == 0;
+    int bx = blockIdx.x, by = blockIdx.y;
+    {
+        const int n = gridDim.x * gridDim.y;
+        if ((n & 7) == 0) {
+            const int id = blockIdx.x + gridDim.x * blockIdx.y;
+            const int lg = (id & 7) * (n >> 3) + (id >> 3);
+            bx = lg % gridDim.x;
+            by = lg / gridDim.x;
+        }
+    }
+    const int x = bx * TC + c, b = by;
+    const int nx = d.nx, nz = d.nz;
+    const size_t col = (size_t)b * nz * nx + x;
+    const int time = clock_t_(clk);
+    const int nb = d.npmlbot, ntop = d.npmltop, nzpml = d.nzpml;
+    const size_t pz0 = (size_t)b * nzpml * nx + x;
+    const size_t pz1 = (size_t)(d.S + b) * nzpml * nx + x;
+    const double *zt = sg.ztab + a;
+    const double *zs = zl + a;
+    double *const dump = st.discard + (tid & 255);
+
+    auto rowq = [&](int i) -> size_t { return col + (size_t)min(a + i, nz - 1) * nx; };
+    auto valid = [&](int i) -> bool { return act && a + i < nz; };
+    auto pmlrow = [&](int j) -> int {
+        const int jc = min(j, nz - 1);
+        return jc < nb ? jc : min(nzpml - 1, nb + max(jc - (nz - ntop), 0));
+    };
+    auto is_pml = [&](int j) -> bool { return j < nb || (j >= nz - ntop && j < nz); };
+    const bool segpml = a < nb || a + MS > nz - ntop;
+
+    // --- first derivatives: group 0 d xi_z/dz (ibc 3, top row preset),
+    //     group 1 d v_z/dz (ibc 1), group 2 d v_x/dz (ibc 1)
+    const double *F = f == 0 ? ca.T[1] : (f == 1 ? ca.T[3] : ca.T[2]);
+    double win[MS + 6];
+#pragma unroll
+    for (int k = 0; k < MS + 6; k++) win[k] = F[col + (size_t)min(max(a - 3 + k, 0), nz - 1) * nx];
+    const double topv = f == 0 ? st.topbc[(size_t)b * nx + x] : 0.0;
+    {
+        const int nb3 = 3 * nband, stride = blockDim.x;
+        for (int q0 = tid; q0 < nb3; q0 += 8 * stride) {
+            double v[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) v[u] = sg.Minv[min(q0 + u * stride, nb3 - 1)];
+#pragma unroll
+            for (int u = 0; u < 8; u++)
+                if (q0 + u * stride < nb3) mv[q0 + u * stride] = v[u];
+        }
+        const int nzl = kZL * nzp;
+        for (int q0 = tid; q0 < nzl; q0 += 8 * stride) {
+            double v[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                const int q = min(q0 + u * stride, nzl - 1), r = q / nzp;
+                v[u] = sg.ztab[(size_t)lrow_src(r) * nzp + (q - r * nzp)];
+            }
+#pragma unroll
+            for (int u = 0; u < 8; u++)
+                if (q0 + u * stride < nzl) zl[q0 + u * stride] = v[u];
+        }
+    }
+    // source value of this column at its injection row (physics2d.f90:1051-1061)
+    double srcv = 0.0;
+    const int isrc = src.row - a;
+    if (f == 0 && act && isrc >= 0 && isrc < MS && time <= src.limit) {
+        if (src.mode == 1) {
+            const double fs = (time >= 0 && time < src.nfwd) ? src.fwd[time] : 0.0;
+            srcv = src.vr[(size_t)b * nx + x] * fs * src.delta_width / bg.rho0_src[x];
+        } else if (src.mode == 2) {
+            srcv = lagrange_forcing(src, b, x, nx, time) * src.delta_width / bg.rho0_src[x];
+        }
+    }
+    const int slot_f = f == 0 ? 2 : 0;                 // dbyd2 table set: ibc 3 or ibc 1
+    double Y[MS], own[MS];
+    seg_rhs<MS>([&](int k) { return win[k]; }, Y, a, nz, first, f == 0 ? 3 : 1, 0.0, topv, cf);
+#pragma unroll
+    for (int i = 0; i < MS; i++) own[i] = win[3 + i];
+    seg_solve<MS>(Y, zt + zrow(slot_f, 0) * nzp, zt + zrow(slot_f, 1) * nzp, zt + zrow(slot_f, 2) * nzp);
+    if (act) { yb[f][c * n2P + 2 * s] = Y[0]; yb[f][c * n2P + 2 * s + 1] = Y[MS - 1]; }
+    __syncthreads();
+    if (act)
+        seg_correct<MS>(Y, s, P, sg.hw, yb[f] + c * n2P, mv + (f == 0 ? 2 * nband : 0),
+                        zt + zrow(slot_f, 3) * nzp, zt + zrow(slot_f, 4) * nzp);
+
+    double w0[MS];                                     // group 0: -rho'/rho0 g + source
+    if (f == 0) {
+        // z-PML (:957-967), divergence, p and rho' (:990-998, :1043)
+        auto epi = [&](auto pml_c) {
+            constexpr bool PML = decltype(pml_c)::value;
+            constexpr int CH = PML ? (MS + 1) / 2 : MS;   // rows whose loads go out together
+#pragma unroll
+            for (int c0 = 0; c0 < MS; c0 += CH) {
+            double ldix[CH], lc2r[CH], lpsi[CH], laz[CH], lbz[CH], lpb[CH], lfz[CH];
+#pragma unroll
+            for (int k = 0; k < CH; k++) {
+                const int i = c0 + k;
+                if (i >= MS) break;
+                const size_t qj = rowq(i), gq = qj - (size_t)b * nz * nx;
+                ldix[k] = st.dxixdx[qj];
+                lc2r[k] = bg.c2rho0[gq];
+                if (PML) {
+                    const int pj = pmlrow(a + i);
+                    const size_t qp = pz0 + (size_t)pj * nx, qb = (size_t)pj * nx + x;
+                    lpsi[k] = st.pz_tmp[qp];
+                    lpb[k] = st.pz_base[qp];
+                    laz[k] = bg.az[qb];
+                    lbz[k] = bg.bz[qb];
+                    lfz[k] = bg.fz[qb];
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < CH; k++) {
+                const int i = c0 + k;
+                if (i >= MS) break;
+                const int j = a + i;
+                const size_t qj = rowq(i);
+                const double str = zs[L_STR * nzp + i], r0 = zs[L_RHO * nzp + i];
+                const double rinv = zs[L_RINV * nzp + i], gz = zs[L_G * nzp + i];
+                double dz = Y[i] * str;
+                double fdec = 1.0;
+                if (PML) {
+                    const bool pm = is_pml(j);
+                    fdec = pm ? lfz[k] : 1.0;
+                    const double prhs = laz[k] * dz + lbz[k] * lpsi[k];
+                    dz = pm ? dz + lpsi[k] : dz;
+                    const double pnew = lpb[k] + ca.bnext * prhs;
+                    if (pm && act) {
+                        const size_t qp = pz0 + (size_t)pmlrow(j) * nx;
+                        st.pz_tmp[qp] = pnew;
+                        if (LAST) st.pz_base[qp] = pnew;
+                    }
+                }
+                const double gp0 = zs[L_GP * nzp + i] * fdec, gr0 = zs[L_GRHO * nzp + i] * fdec;
+                const double xiz = own[i];
+                const double div = ldix[k] + dz;
+                const double pv = -(lc2r[k] * div) - gp0 * xiz;
+                *(valid(i) ? &st.p[qj] : dump) = pv;
+                const double rho = -(r0 * div) - gr0 * xiz;
+                const double bzv = -rinv * (rho * gz);
+                w0[i] = (i == isrc) ? bzv + srcv : bzv;
+            }
+            }
+        };
+        if (segpml) epi(std::true_type{});
+        else epi(std::false_type{});
+    } else {
+        // group 1: flow part of RHS v_z (:1073-1077) to LDS, xi_z update;
+        // group 2: RHS v_x flow part (:1067-1071), xi_x update
+        double l1[MS], l2[MS], l3[MS], l4[MS];
+#pragma unroll
+        for (int i = 0; i < MS; i++) {
+            const size_t qj = rowq(i), gq = qj - (size_t)b * nz * nx;
+            l1[i] = (f == 1 ? st.dxvz : st.dxvx)[qj];
+            l2[i] = bg.v0x[gq];
+            l3[i] = bg.v0z[gq];
+            l4[i] = st.a[f == 1 ? 1 : 0][qj];
+        }
+#pragma unroll
+        for (int i = 0; i < MS; i++) {
+            const int j = a + i;
+            const size_t qj = rowq(i);
+            const bool zero = j == 0 || x == 0 || x == nx - 1;
+            const double dzf = Y[i] * zs[L_STR * nzp + i];
+            const bool v = valid(i);
+            if (f == 1) {
+                if (act) wl[c * nzp + a + i] = -(2.0 * l2[i] * l1[i]) - 2.0 * l3[i] * dzf;
+                *(v ? &ca.TN[1][qj] : dump) = l4[i] + ca.bnext * (zero ? 0.0 : own[i]);
+            } else {
+                double rvx = -(2.0 * l2[i] * l1[i]) - 2.0 * l3[i] * dzf;
+                rvx = zero ? 0.0 : rvx;
+                *(v ? &st.rvx[qj] : dump) = rvx;
+                *(v ? &ca.TN[0][qj] : dump) = l4[i] + ca.bnext * (zero ? 0.0 : own[i]);
+            }
+        }
+    }
+
+    // --- d p/dz (ibc 2, :1023) with the z-PML (:1025-1035), RHS v_z, the
+    //     zeroed rows/columns (:1083-1093) and the v_z update: group 0
+    __syncthreads();
+    if (f == 0) {
+        double wp[MS + 6];
+#pragma unroll
+        for (int k = 0; k < MS + 6; k++) wp[k] = st.p[col + (size_t)min(max(a - 3 + k, 0), nz - 1) * nx];
+        seg_rhs<MS>([&](int k) { return wp[k]; }, Y, a, nz, first, 2, 0.0, 0.0, cf);
+        seg_solve<MS>(Y, zs, zs + nzp, zs + 2 * nzp);
+        if (act) { yb[3][c * n2P + 2 * s] = Y[0]; yb[3][c * n2P + 2 * s + 1] = Y[MS - 1]; }
+    }
+    __syncthreads();
+    if (f == 0) {
+        if (act) seg_correct<MS>(Y, s, P, sg.hw, yb[3] + c * n2P, mv + nband, zs + 3 * nzp, zs + 4 * nzp);
+        const int sslot = (LAST && sink.snap) ? out_slot(time, clk.steps, sink.k0, sink.nsnap) : -1;
+        const bool snap = sslot >= 0;
+        const size_t slot = snap ? ((size_t)b * sink.nsnap + sslot) * sink.nfield : 0;
+        auto epi = [&](auto pml_c) {
+            constexpr bool PML = decltype(pml_c)::value;
+            constexpr int CH = PML ? (MS + 1) / 2 : MS;
+#pragma unroll
+            for (int c0 = 0; c0 < MS; c0 += CH) {
+            double la3[CH], lpsi[CH], laz[CH], lbz[CH], lpb[CH];
+#pragma unroll
+            for (int k = 0; k < CH; k++) {
+                const int i = c0 + k;
+                if (i >= MS) break;
+                la3[k] = st.a[3][rowq(i)];
+                if (PML) {
+                    const int pj = pmlrow(a + i);
+                    const size_t qp = pz1 + (size_t)pj * nx, qb = (size_t)pj * nx + x;
+                    lpsi[k] = st.pz_tmp[qp];
+                    lpb[k] = st.pz_base[qp];
+                    laz[k] = bg.az[qb];
+                    lbz[k] = bg.bz[qb];
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < CH; k++) {
+                const int i = c0 + k;
+                if (i >= MS) break;
+                const int j = a + i;
+                const size_t qj = rowq(i);
+                double gpz = Y[i] * zs[L_STR * nzp + i];
+                if (PML) {
+                    const bool pm = is_pml(j);
+                    const double prhs = laz[k] * gpz + lbz[k] * lpsi[k];
+                    gpz = pm ? gpz + lpsi[k] : gpz;
+                    const double pnew = lpb[k] + ca.bnext * prhs;
+                    if (pm && act) {
+                        const size_t qp = pz1 + (size_t)pmlrow(j) * nx;
+                        st.pz_tmp[qp] = pnew;
+                        if (LAST) st.pz_base[qp] = pnew;
+                    }
+                }
+                double rvz = -zs[L_RINV * nzp + i] * gpz + (w0[i] + wl[c * nzp + a + i]);
+                rvz = (j == 0 || x == 0 || x == nx - 1) ? 0.0 : rvz;
+                const bool v = valid(i);
+                *(v ? &ca.TN[3][qj] : dump) = la3[k] + ca.bnext * rvz;
+                if (snap && v)
+                    sink.snap[((slot + (sink.adjoint ? 3 : 5)) * nz + j) * nx + x] = rvz;
+            }
+            }
+        };
+        if (segpml) epi(std::true_type{});
+        else epi(std::false_type{});
+    }
+    if (LAST) {
+        // FILTER_VARS_2D (physics2d.f90:509-556): group 0 filters v_z, group 1
+        // xi_z, group 2 xi_x, each from a window of rows a-5 .. a+MS+4
+        __syncthreads();
+        const int fi = f == 0 ? 3 : (f == 1 ? 1 : 0);
+        const int sslot = sink.snap ? out_slot(time, clk.steps, sink.k0, sink.nsnap) : -1;
+        const bool snap = sslot >= 0;
+        const size_t slot = snap ? ((size_t)b * sink.nsnap + sslot) * sink.nfield : 0;
+        const int oslot = sink.vz_out ? out_slot(time, clk.steps, sink.k0, sink.nout) : -1;
+        const bool out = oslot >= 0;
+        const int nh = d.nh;
+        const bool xin = x >= nh - 1 && x <= nx - nh;
+        const double *Fw = ca.TN[fi];
+        double w[MS + 10];
+#pragma unroll
+        for (int q = 0; q < MS + 10; q++) w[q] = Fw[col + (size_t)min(max(a - 5 + q, 0), nz - 1) * nx];
+        // snapshot field: forward xi_x 0, xi_z 1, v_z 3; adjoint v_z 1
+        const int fs = sink.adjoint ? (fi == 3 ? 1 : -1) : fi;
+#pragma unroll
+        for (int i = 0; i < MS; i++) {
+            const int j = a + i;
+            const int q = i + 5;
+            const double f11 = 0.75390625 * w[q] + 0.5 * 0.41015625 * (w[q - 1] + w[q + 1]) +
+                               0.5 * -0.234375 * (w[q - 2] + w[q + 2]) +
+                               0.5 * 0.087890625 * (w[q - 3] + w[q + 3]) +
+                               0.5 * -0.01953125 * (w[q - 4] + w[q + 4]) +
+                               0.5 * 0.001953125 * (w[q - 5] + w[q + 5]);
+            const double f2 = 0.5 * (w[q + 1] + w[q - 1]);
+            double v = w[q];
+            if (xin && j >= 5 && j <= nz - 2) v = (j <= nz - 6) ? f11 : f2;
+            const bool ok = valid(i);
+            const size_t qj = rowq(i);
+            *(ok ? &st.a[fi][qj] : dump) = v;
+            if (fi == 3 && ok && out && j == sink.orow)
+                sink.vz_out[((size_t)b * sink.nout + oslot) * nx + x] = v;
+            if (ok && snap && fs >= 0) sink.snap[(slot + fs) * nz * nx + (size_t)j * nx + x] = v;
+        }
+    }
+}
+
 namespace {
 int g_lds_max = 64 * 1024;   // dynamic LDS a block may use on this device
 }
@@ -480,6 +813,10 @@ hipError_t colseg_init()
     if (hipFuncSetAttribute((const void *)k_colseg<MSV, 0>, hipFuncAttributeMaxDynamicSharedMemorySize, \
                             g_lds_max) != hipSuccess ||                                                 \
         hipFuncSetAttribute((const void *)k_colseg<MSV, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, \
+                            g_lds_max) != hipSuccess ||                                                 \
+        hipFuncSetAttribute((const void *)k_colsplit<MSV, 0>, hipFuncAttributeMaxDynamicSharedMemorySize, \
+                            g_lds_max) != hipSuccess ||                                                 \
+        hipFuncSetAttribute((const void *)k_colsplit<MSV, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, \
                             g_lds_max) != hipSuccess) {                                                 \
         (void)hipGetLastError();                                                                        \
         g_lds_max = 64 * 1024;                                                                          \
@@ -503,6 +840,32 @@ hipError_t launch_colseg(int tc, int ms, const Dims &d, const Bg &bg, const DbyC
         if (ca.last) hipLaunchKernelGGL((k_colseg<MSV, 1>), grid, block, shm, st, d, bg, cf, sg, s, ca, src, c, sk); \
         else hipLaunchKernelGGL((k_colseg<MSV, 0>), grid, block, shm, st, d, bg, cf, sg, s, ca, src, c, sk);         \
         return hipGetLastError();                                                                      \
+    }
+    SPARC_COLSEG_SIZES(L)
+#undef L
+    return hipErrorInvalidValue;
+}
+
+}  // namespace sparc
+
+namespace sparc {
+
+hipError_t launch_colsplit(int tc, int ms, const Dims &d, const Bg &bg, const DbyCoef &cf, const SegTab &sg,
+                           const State &s, const ColArgs &ca, const Src &src, const Clock &c, const Sink &sk,
+                           hipStream_t st)
+{
+    const int G = (tc * sg.P + 63) / 64 * 64;
+    if ((tc != 1 && tc != 2 && tc != 4 && tc != 8 && tc != 16) || d.nx % tc || sg.m != ms ||
+        sg.P > kSegMax || 3 * G > 384)
+        return hipErrorInvalidValue;
+    dim3 grid(d.nx / tc, d.S), block(3 * G);
+    const size_t shm = colsplit_lds_bytes(tc, sg.P, sg.hw, sg.nzp);
+    if (shm > (size_t)g_lds_max) return hipErrorInvalidValue;
+#define L(MSV)                                                                                          \
+    if (ms == MSV) {                                                                                    \
+        if (ca.last) hipLaunchKernelGGL((k_colsplit<MSV, 1>), grid, block, shm, st, d, bg, cf, sg, s, ca, src, c, sk, tc); \
+        else hipLaunchKernelGGL((k_colsplit<MSV, 0>), grid, block, shm, st, d, bg, cf, sg, s, ca, src, c, sk, tc);         \
+        return hipGetLastError();                                                                       \
     }
     SPARC_COLSEG_SIZES(L)
 #undef L

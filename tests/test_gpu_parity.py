@@ -90,12 +90,17 @@ def test_background_matches_oracle():
         assert abs(s.query(name) - o.info(name)) <= 1e-15 * abs(o.info(name))
 
 
+@pytest.mark.parametrize("split", ["0", "1"])
 @pytest.mark.parametrize("flow,dc", [(False, False), (True, True)])
-def test_steps_match_oracle(flow, dc):
-    """Full STEPs (RK + PML + filter + source) against the oracle."""
+def test_steps_match_oracle(flow, dc, split, monkeypatch):
+    """Full STEPs (RK + PML + filter + source) against the oracle, with both
+    forms of the column pass (one lane per column and segment, or per
+    column, segment and field)."""
+    monkeypatch.setenv("SPARC_COLSEG_SPLIT", split)
     p, bg, c2d, psi = small_case(nx=64, nz=60, flow=flow, dc=dc)
     maxtime = 60
     s = Sparc(p, bg, c2d=c2d, psi=psi)
+    assert s.query("colseg_split") == int(split)
     o = oracle.Oracle(p, bg, c2d=c2d, psi=psi)
     s.set_forward_sources([3.0], maxtime)
     o.set_forward_source(3.0, maxtime)
@@ -130,13 +135,16 @@ def test_forward_trace_1000_steps():
     assert e < 1e-10 and es < 1e-10
 
 
-def test_batched_sources_equal_single_runs(monkeypatch):
+@pytest.mark.parametrize("split", ["0", "1"])
+def test_batched_sources_equal_single_runs(split, monkeypatch):
     p, bg, c2d, psi = small_case(nx=64, nz=60)
     xs = [-10.0, 0.0, 7.5]
     maxtime = 200
-    # with the same column segmentation for every batch size, a batch is bit
-    # for bit the single runs
+    # with the same column segmentation and form for every batch size, a
+    # batch is bit for bit the single runs
     monkeypatch.setenv("SPARC_COLSEG_MS", "6")
+    monkeypatch.setenv("SPARC_COLSEG_TC", "2")
+    monkeypatch.setenv("SPARC_COLSEG_SPLIT", split)
     sb = Sparc(p, bg, nsrc=3, c2d=c2d, psi=psi)
     vzb, _ = sb.forward(xs, maxtime)
     for i, x in enumerate(xs):
@@ -145,6 +153,8 @@ def test_batched_sources_equal_single_runs(monkeypatch):
         assert torch.equal(vzb[i], vz1[0]), i
     # the autotuner may segment a batch differently: equal to rounding
     monkeypatch.delenv("SPARC_COLSEG_MS")
+    monkeypatch.delenv("SPARC_COLSEG_TC")
+    monkeypatch.delenv("SPARC_COLSEG_SPLIT")
     sb = Sparc(p, bg, nsrc=3, c2d=c2d, psi=psi)
     vzb, _ = sb.forward(xs, maxtime)
     for i, x in enumerate(xs):
@@ -153,7 +163,9 @@ def test_batched_sources_equal_single_runs(monkeypatch):
         assert rel_l2(vzb[i].cpu().numpy(), vz1[0].cpu().numpy()) < 1e-13, i
 
 
-def test_adjoint_and_kernels_match_oracle():
+@pytest.mark.parametrize("split", ["0", "1"])
+def test_adjoint_and_kernels_match_oracle(split, monkeypatch):
+    monkeypatch.setenv("SPARC_COLSEG_SPLIT", split)
     p, bg, c2d, psi = small_case(nx=64, nz=60)
     maxtime = 4 * p.steps * 5  # multiple of the cadence
     nt = maxtime // p.cadforcing_step + 1
