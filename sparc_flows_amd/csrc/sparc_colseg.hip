@@ -492,7 +492,7 @@ __global__ __launch_bounds__(256) void k_colseg(Dims d, Bg bg, DbyCoef cf, SegTa
 // Dynamic LDS: as k_colseg, plus the group-1 part of RHS v_z [TC][nzp].
 inline size_t colsplit_lds_bytes(int tc, int P, int hw, int nzp)
 {
-    return colseg_lds_bytes(tc, P, hw, nzp) + sizeof(double) * (size_t)tc * nzp;
+    return colseg_lds_bytes(tc, P, hw, nzp) + sizeof(double) * (size_t)2 * tc * nzp;
 }
 
 template <int MS, int LAST>
@@ -592,12 +592,26 @@ __global__ __launch_bounds__(384) void k_colsplit(Dims d, Bg bg, DbyCoef cf, Seg
     for (int i = 0; i < MS; i++) own[i] = win[3 + i];
     seg_solve<MS>(Y, zt + zrow(slot_f, 0) * nzp, zt + zrow(slot_f, 1) * nzp, zt + zrow(slot_f, 2) * nzp);
     if (act) { yb[f][c * n2P + 2 * s] = Y[0]; yb[f][c * n2P + 2 * s + 1] = Y[MS - 1]; }
+    // the epilogue's inputs that no lane of this kernel writes, loaded
+    // before the barrier so that they arrive during the exchange and the
+    // correction: group 0 d xi_x/dx, c^2 rho0 and the v_z base; groups 1, 2
+    // their x-derivative, the flow and the base of their update
+    double pa[MS], pb[MS], pc[MS];
+    if (f == 0) {
+#pragma unroll
+        for (int i = 0; i < MS; i++) {
+            const size_t qj = rowq(i), gq = qj - (size_t)b * nz * nx;
+            pa[i] = st.dxixdx[qj];
+            pb[i] = bg.c2rho0[gq];
+            pc[i] = st.a[3][qj];
+        }
+    }
     __syncthreads();
     if (act)
         seg_correct<MS>(Y, s, P, sg.hw, yb[f] + c * n2P, mv + (f == 0 ? 2 * nband : 0),
                         zt + zrow(slot_f, 3) * nzp, zt + zrow(slot_f, 4) * nzp);
 
-    double w0[MS];                                     // group 0: -rho'/rho0 g + source
+    double *const w0 = wl + (TC + c) * nzp + a;       // group 0: -rho'/rho0 g + source
     if (f == 0) {
         // z-PML (:957-967), divergence, p and rho' (:990-998, :1043)
         auto epi = [&](auto pml_c) {
@@ -605,14 +619,11 @@ __global__ __launch_bounds__(384) void k_colsplit(Dims d, Bg bg, DbyCoef cf, Seg
             constexpr int CH = PML ? (MS + 1) / 2 : MS;   // rows whose loads go out together
 #pragma unroll
             for (int c0 = 0; c0 < MS; c0 += CH) {
-            double ldix[CH], lc2r[CH], lpsi[CH], laz[CH], lbz[CH], lpb[CH], lfz[CH];
+            double lpsi[CH], laz[CH], lbz[CH], lpb[CH], lfz[CH];
 #pragma unroll
             for (int k = 0; k < CH; k++) {
                 const int i = c0 + k;
                 if (i >= MS) break;
-                const size_t qj = rowq(i), gq = qj - (size_t)b * nz * nx;
-                ldix[k] = st.dxixdx[qj];
-                lc2r[k] = bg.c2rho0[gq];
                 if (PML) {
                     const int pj = pmlrow(a + i);
                     const size_t qp = pz0 + (size_t)pj * nx, qb = (size_t)pj * nx + x;
@@ -647,12 +658,12 @@ __global__ __launch_bounds__(384) void k_colsplit(Dims d, Bg bg, DbyCoef cf, Seg
                 }
                 const double gp0 = zs[L_GP * nzp + i] * fdec, gr0 = zs[L_GRHO * nzp + i] * fdec;
                 const double xiz = own[i];
-                const double div = ldix[k] + dz;
-                const double pv = -(lc2r[k] * div) - gp0 * xiz;
+                const double div = pa[i] + dz;
+                const double pv = -(pb[i] * div) - gp0 * xiz;
                 *(valid(i) ? &st.p[qj] : dump) = pv;
                 const double rho = -(r0 * div) - gr0 * xiz;
                 const double bzv = -rinv * (rho * gz);
-                w0[i] = (i == isrc) ? bzv + srcv : bzv;
+                if (act) w0[i] = (i == isrc) ? bzv + srcv : bzv;
             }
             }
         };
@@ -711,12 +722,11 @@ __global__ __launch_bounds__(384) void k_colsplit(Dims d, Bg bg, DbyCoef cf, Seg
             constexpr int CH = PML ? (MS + 1) / 2 : MS;
 #pragma unroll
             for (int c0 = 0; c0 < MS; c0 += CH) {
-            double la3[CH], lpsi[CH], laz[CH], lbz[CH], lpb[CH];
+            double lpsi[CH], laz[CH], lbz[CH], lpb[CH];
 #pragma unroll
             for (int k = 0; k < CH; k++) {
                 const int i = c0 + k;
                 if (i >= MS) break;
-                la3[k] = st.a[3][rowq(i)];
                 if (PML) {
                     const int pj = pmlrow(a + i);
                     const size_t qp = pz1 + (size_t)pj * nx, qb = (size_t)pj * nx + x;
@@ -747,7 +757,7 @@ __global__ __launch_bounds__(384) void k_colsplit(Dims d, Bg bg, DbyCoef cf, Seg
                 double rvz = -zs[L_RINV * nzp + i] * gpz + (w0[i] + wl[c * nzp + a + i]);
                 rvz = (j == 0 || x == 0 || x == nx - 1) ? 0.0 : rvz;
                 const bool v = valid(i);
-                *(v ? &ca.TN[3][qj] : dump) = la3[k] + ca.bnext * rvz;
+                *(v ? &ca.TN[3][qj] : dump) = pc[i] + ca.bnext * rvz;
                 if (snap && v)
                     sink.snap[((slot + (sink.adjoint ? 3 : 5)) * nz + j) * nx + x] = rvz;
             }
