@@ -6,11 +6,16 @@
  * float32 literals that the reference widens to real64.  A C float literal
  * written as 0.005f here stands for the Fortran default-REAL literal 0.005.
  *
- * Parity pins (tests/test_oracle_*.py):
+ * Parity pins (tests/test_oracle.py):
  *   orc_dbyd2 / mtridag  - bit-compared against the reference dbyd2.f +
  *                          mtridag.f compiled from source (oracle/_ref);
  *   orc_ddx / orc_fft    - closed forms (Fourier modes, Nyquist, constants);
- *   filters, RK, Lagrange, sources - brute force / closed forms;
+ *   filters, RK, Lagrange - brute force / closed forms;
+ *   orc_forward_source_series - the series' spectrum against the closed-form
+ *                          transform of the continuous pulse;
+ *   orc_create           - flows against the closed form of a separable
+ *                          stream function; PML coefficient structure and
+ *                          the absorption of an outgoing pulse;
  *   orc_rhs / orc_step   - linearity, symmetry and energy invariants.
  * End-to-end traces against the full Fortran build are "parity unpinned":
  * the reference needs FFTW3, which this image does not have.

@@ -13,6 +13,14 @@ GPU checker.
 * RHS: linearity, mirror symmetry about the box centre, and a divergence-free
   background flow (the reference's CONTINUITY_CHECK, driver.f90:1008-1032).
 * PRODUCE_KERNELS: forward label k is paired with the time-reversed adjoint.
+* FORWARD_SOURCE: the series' Fourier amplitude equals the closed-form
+  transform of cos(2 pi nu0 t) exp(-2 w^2 (t - 1/(2 nu0))^2) * 1e-5.
+* Flow from psi: for rho0 c psi / (R / 1e8) = sin(2 pi x) G(z) the flow is
+  rho0 v0x = -G'(z) sin(2 pi x) and rho0 v0z = (R / Lx) 2 pi cos(2 pi x) G(z),
+  negated for the adjoint, with and without a 2D sound speed.
+* PML coefficients: no damping and the constant frequency shift outside the
+  layers, damping that grows quadratically from 0 at the inner edge, and a
+  pulse whose energy leaves the box through the layers without growth.
 """
 import os
 from pathlib import Path
@@ -288,3 +296,117 @@ def test_read_params_i_of_reference():
     assert p.nupeak == f32(0.0017) and p.obsheight == 2.0e7 and p.excitdep == -1.5e7
     assert p.file_data == "solar_deep"
     assert p.reference_maxtime() == 7785  # the default run length
+
+
+# ------------------------------------------------------- forward source ----
+@pytest.mark.parametrize("dt", [1.0, 0.28215680745308463])
+def test_forward_source_spectrum(dt):
+    """|S(nu)| of the sampled series against the transform of the continuous
+    pulse: with a = 2 w^2 and the shift 1/(2 nu0) both images carry the same
+    phase, |S(nu)| = 1e-5 sqrt(pi/a)/2 (exp(-pi^2 (nu-nu0)^2/a) +
+    exp(-pi^2 (nu+nu0)^2/a)).  Cutting the series at t = 0 and the 1e-5
+    threshold leave < 1e-3; a width factor of w^2 instead of 2 w^2 is off by
+    20 %."""
+    nu0, w = f32(0.0017), f32(0.0055)
+    nt = int(1500 / dt)
+    s = oracle.forward_source_series(dt, nu0, w, nt)
+    t = np.arange(nt) * dt
+    nu = np.linspace(0.0, 0.008, 81)
+    S = np.abs(np.exp(-2j * np.pi * nu[:, None] * t[None, :]) @ s) * dt
+    a = 2.0 * w * w
+    ref = f32(1e-5) * 0.5 * np.sqrt(np.pi / a) * (np.exp(-np.pi ** 2 * (nu - nu0) ** 2 / a)
+                                                   + np.exp(-np.pi ** 2 * (nu + nu0) ** 2 / a))
+    assert np.max(np.abs(S - ref)) < 2e-3 * ref.max()
+    # envelope maximum at 1/(2 nu0), where the carrier is -1
+    k = int(np.argmax(np.abs(s)))
+    assert abs(t[k] - 0.5 / nu0) <= dt and s[k] < 0
+
+
+# ---------------------------------------------------------------- flows ----
+@pytest.mark.parametrize("dc,adjoint", [(False, False), (True, False), (True, True)])
+def test_flow_from_psi_closed_form(dc, adjoint):
+    """v0 = curl(rho0 c psi y) / rho0 (driver.f90:94-126, reversed for the
+    adjoint at :144-147) against the closed form of a separable stream
+    function; x derivatives are spectral (1e-12), z derivatives the compact
+    scheme with float32 coefficients (2e-5)."""
+    p, bg, c2d, _ = small_case(nx=64, nz=60, dc=dc)
+    nz, nx = p.nz, p.nx
+    o0 = oracle.Oracle(p, bg, c2d=c2d)
+    z = o0.array("z")
+    rho = o0.array("rho0").reshape(nz, nx)
+    c = np.sqrt(o0.array("c2").reshape(nz, nx))
+    x = np.arange(nx) / nx
+    zc, wz = 0.5 * (z[0] + z[-1]), 0.15 * (z[-1] - z[0])
+    G = np.exp(-((z - zc) / wz) ** 2)
+    dG = -2.0 * (z - zc) / wz ** 2 * G
+    F = np.sin(2 * np.pi * x)[None, :] * G[:, None]
+    rsun = 6.95989467700e10
+    o = oracle.Oracle(p, bg, c2d=c2d, psi=F * (rsun / 1e8) / (rho * c), adjoint=adjoint)
+    sgn = -1.0 if adjoint else 1.0
+    ex = -sgn * np.sin(2 * np.pi * x)[None, :] * dG[:, None]
+    ez = sgn * o0.info("stretchx") * 2 * np.pi * np.cos(2 * np.pi * x)[None, :] * G[:, None]
+    vx = o.array("v0x").reshape(nz, nx)
+    vz = o.array("v0z").reshape(nz, nx)
+    assert np.max(np.abs(rho * vx - ex)) < 2e-5 * np.max(np.abs(ex))
+    assert np.max(np.abs(rho * vz - ez)) < 1e-12 * np.max(np.abs(ez))
+
+
+# ------------------------------------------------------------------ PML ----
+def test_pml_coefficient_profiles():
+    """z-PML (initialize.f90:288-333): az = -d, bz = -(alpha + d); outside the
+    layers d = 0 and alpha is the constant shift; inside, d > 0 grows from
+    the inner edge to the outer one.  x-PML (:339-357): ax < 0 grows towards
+    the box edge and bx - ax = -beta ramps linearly."""
+    p, bg, c2d, _ = small_case(nx=32, nz=60, npml=8, nph=6, flow=False)
+    o = oracle.Oracle(p, bg, c2d=c2d)
+    nz, nx, nb, nt, nh = p.nz, p.nx, p.npmlbot, p.npmltop, p.npmlhor
+    az = o.array("az").reshape(nz, nx)
+    bz = o.array("bzpml").reshape(nz, nx)
+    inner = slice(nb, nz - nt)
+    assert np.all(az[inner] == 0.0)
+    alpha0 = bz[nb + 2, 0]
+    assert alpha0 < 0 and np.all(bz[inner] == alpha0)
+    d = -az
+    bot, top = d[:nb], d[nz - nt:]
+    assert np.all(bot >= 0) and np.all(top >= 0)
+    assert np.all(np.diff(bot, axis=0) <= 0)       # largest at the bottom edge
+    assert np.all(np.diff(top, axis=0) >= 0)       # largest at the top edge
+    assert np.max(np.abs(bot[nb - 1])) < 1e-12 * np.max(bot) and np.max(np.abs(top[0])) < 1e-12 * np.max(top)
+    alpha = -(bz - az)
+    assert np.all(alpha[:nb] <= -alpha0 + 1e-15) and np.all(alpha[:nb] >= 0)
+    ax = o.array("axpml").reshape(nz, 2 * nh)
+    bx = o.array("bxpml").reshape(nz, 2 * nh)
+    assert np.all(ax <= 0) and np.all(np.diff(ax[:, :nh], axis=1) >= 0)
+    assert np.all(np.diff(ax[:, nh:], axis=1) <= 0)
+    beta = -(bx - ax)[:, :nh]
+    assert np.allclose(np.diff(beta, axis=1), np.diff(beta, axis=1)[:, :1], rtol=1e-12, atol=0)
+
+
+def test_pml_absorbs_an_outgoing_pulse():
+    """A v_z pulse at mid-depth of a 64 x 150 box with 12-row layers: within
+    800 steps more than 85 % of its kinetic energy has left through the top
+    layer, and the energy never grows.  (With the layers' damping removed
+    the pulse would bounce off the closed boundary rows and stay.)"""
+    bg = B.polytrope(150)
+    p = SparcParams(nx=64, nz=150, xlength=200e8, timestep=B.stable_timestep(bg), npmlbot=12,
+                    npmltop=12, npmlhor=5)
+    o = oracle.Oracle(p, bg)
+    nz, nx = p.nz, p.nx
+    rho = o.array("rho0").reshape(nz, nx)
+    kk, ii = np.meshgrid(np.arange(nz), np.arange(nx), indexing="ij")
+    a = np.zeros((4, nz, nx))
+    a[3] = 1e-3 * np.exp(-((kk - nz // 2) / 3.0) ** 2 - ((ii - nx // 2) / 3.0) ** 2)
+    o.set_state(a)
+
+    def energy():
+        st = o.state()
+        return float(np.sum(rho * (st[2] ** 2 + st[3] ** 2)))
+
+    e0 = energy()
+    peak = 0.0
+    for t in range(800):
+        o.step(t)
+        if t % 100 == 99:
+            peak = max(peak, energy())
+    assert peak < 1.02 * e0
+    assert energy() < 0.15 * e0
