@@ -68,7 +68,9 @@ struct sparc_handle {
     DbyCoef cf{};
 
     // device memory
-    DevArena bg_mem, st_mem, src_mem, kern_mem;
+    DevArena bg_mem, st_mem, src_mem, kern_mem, kpair_mem;
+    double *kpair = nullptr;        // many-pair accumulation: G[8][Kc], Dx[4][Kc], Dz[4][Kc], ddz work[4][Kc]
+    int kpair_cap = 0;              // Kc the buffer holds
     Bg bg{};
     Tri tri[4]{};
     State st{};
@@ -960,6 +962,7 @@ int sparc_destroy(sparc_handle *h)
     h->st_mem.release();
     h->src_mem.release();
     h->kern_mem.release();
+    h->kpair_mem.release();
     if (h->own_stream && h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
     return SPARC_OK;
@@ -1498,27 +1501,33 @@ int sparc_kernels_add(sparc_handle *h, int n_pairs, const double *fwd, const dou
     if (!h->kbuf) return fail(h, SPARC_E_STATE, "sparc_kernels_add before sparc_kernels_begin");
     const int S = h->S, nt = n_pairs;
     const size_t n2 = (size_t)h->nx * h->nz, n = (size_t)S * n2;
-    double *D = h->kbuf, *KC = h->kbuf + 8 * n, *KX = KC + n, *KZ = KC + 2 * n, *H = KC + 3 * n;
-    double *work = h->kbuf + 14 * n;
-    double *g = h->d_scratch;
+    double *KC = h->kbuf + 8 * n, *KX = KC + n, *KZ = KC + 2 * n, *H = KC + 3 * n;
     const Tri &tr1 = h->tri[1];
-    for (int k = 0; k < nt; k++) {
-        const int ka = nt - 1 - k;
-        // forward fields: xi_x 0, xi_z 1, v_x 2, v_z 3; adjoint: v_x 0, v_z 1
-        struct Job { const double *src; int slot, nfield, field, op; };  // op 0 ddx, 1 ddzkern
-        const Job jobs[8] = {{fwd, k, 6, 0, 0}, {fwd, k, 6, 1, 1}, {adj, ka, 4, 0, 0}, {adj, ka, 4, 1, 1},
-                             {fwd, k, 6, 2, 0}, {fwd, k, 6, 3, 0}, {fwd, k, 6, 2, 1}, {fwd, k, 6, 3, 1}};
-        for (int j = 0; j < 8; j++) {
-            const Job &jb = jobs[j];
-            HCHECK(h, launch_gather(jb.src, g, S, nt, jb.slot, jb.nfield, jb.field, n2, h->stream));
-            if (jb.op == 0)
-                HCHECK(h, launch_ddx_rows(g, D + j * n, S * h->nz, h->nx, h->bg.tw, h->bg.dmul,
-                                          h->stretchx, h->stream));
-            else
-                HCHECK(h, launch_ddz_cols(g, D + j * n, h->nx, h->nz, S, 1, h->bg.stretch, h->cf, tr1,
-                                          work, 1, h->stream));
-        }
-        HCHECK(h, launch_kern_accum(fwd, adj, S, nt, k, n2, D, KC, KX, KZ, H, h->stream));
+    // pairs per chunk: all of them, within 2 GiB of work buffers (20 fields
+    // per pair); one gather, one ddx, one ddz and one accumulation launch
+    // per chunk instead of 17 launches per pair
+    const size_t per_pair = 20 * n * sizeof(double);
+    const int kmax = (int)std::max<size_t>(1, std::min<size_t>(((size_t)2 << 30) / per_pair,
+                                                               65535 / (4 * (size_t)S)));   // ddz grid.y
+    const int want = std::min(nt, kmax);
+    if (h->kpair_cap < want) {
+        if (h->stream) HCHECK(h, hipStreamSynchronize(h->stream));
+        h->kpair_mem.release();
+        h->kpair = nullptr;
+        h->kpair_cap = 0;
+        HCHECK(h, h->kpair_mem.alloc(&h->kpair, (size_t)20 * want * n));
+        h->kpair_cap = want;
+    }
+    for (int k0 = 0; k0 < nt; k0 += want) {
+        const int Kc = std::min(want, nt - k0);
+        const size_t per = (size_t)Kc * n;
+        double *G = h->kpair, *Dx = G + 8 * per, *Dz = Dx + 4 * per, *work = Dz + 4 * per;
+        HCHECK(h, launch_gather_pairs(fwd, adj, S, nt, k0, Kc, n2, G, h->stream));
+        HCHECK(h, launch_ddx_rows(G, Dx, 4 * Kc * S * h->nz, h->nx, h->bg.tw, h->bg.dmul, h->stretchx,
+                                  h->stream));
+        HCHECK(h, launch_ddz_cols(G + 4 * per, Dz, h->nx, h->nz, 4 * Kc * S, 1, h->bg.stretch, h->cf, tr1,
+                                  work, 1, h->stream));
+        HCHECK(h, launch_kern_accum_pairs(fwd, adj, S, nt, k0, Kc, n2, Dx, Dz, KC, KX, KZ, H, h->stream));
     }
     h->kern_pairs += nt;
     return SPARC_OK;

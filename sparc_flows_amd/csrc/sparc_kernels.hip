@@ -367,40 +367,67 @@ __global__ void k_fill(double *p, size_t n, double v)
 // ---------------------------------------------------------------------------
 // Kernel accumulation (PRODUCE_KERNELS, kernels.f90:200-360).
 
-// dst[b][q] = src[((b*nslot + slot)*nfield + field)*n2 + q]
-__global__ void k_gather(const double *__restrict__ src, double *__restrict__ dst, int S, int nslot,
-                         int slot, int nfield, int field, size_t n2)
+// Many snapshot pairs at once.  k_gather_pairs lays out the derivative inputs
+// of pairs k0 .. k0+Kc-1 as G[8][Kc][S][n2]: the x-derivative group first
+// (forward xi_x, adjoint v_x, forward v_x, forward v_z), then the
+// z-derivative group (forward xi_z, adjoint v_z, forward v_x, forward v_z),
+// so that each group is one ddx / ddz launch.
+__global__ void k_gather_pairs(const double *__restrict__ fwd, const double *__restrict__ adj, int S, int nt,
+                               int k0, int Kc, size_t n2, double *__restrict__ G)
 {
-    const size_t n = (size_t)S * n2;
+    const size_t per = (size_t)Kc * S * n2, n = 8 * per;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (size_t)gridDim.x * blockDim.x) {
-        const size_t b = i / n2, q = i % n2;
-        dst[i] = src[((b * nslot + slot) * nfield + field) * n2 + q];
+        const int g = (int)(i / per);
+        const size_t r = i - (size_t)g * per;
+        const int kk = (int)(r / ((size_t)S * n2));
+        const size_t r2 = r - (size_t)kk * S * n2;
+        const size_t b = r2 / n2, q = r2 - b * n2;
+        const int k = k0 + kk;
+        // group member: source, field
+        const bool isadj = (g == 1 || g == 5);
+        const int field = g == 0 ? 0 : g == 4 ? 1 : (g == 1 || g == 5) ? (g == 1 ? 0 : 1)
+                                                : (g == 2 || g == 6) ? 2 : 3;
+        const double *src = isadj ? adj + ((b * nt + (nt - 1 - k)) * 4 + field) * n2
+                                  : fwd + ((b * nt + k) * 6 + field) * n2;
+        G[i] = src[q];
     }
 }
 
-// One snapshot pair: hessian (:293-294), sound-speed kernel (:301-306), flow
-// kernel x / z components (:338-355; the y terms vanish in 2D).
-// D: [8][S][n2] = ddx f_xix, ddz f_xiz, ddx a_vx, ddz a_vz, ddx f_vx, ddx f_vz,
-// ddz f_vx, ddz f_vz (ddz with FILTER_VAR_Z).
-__global__ void k_kern_accum(const double *__restrict__ fwd, const double *__restrict__ adj, int S,
-                             int nt, int k, size_t n2, const double *__restrict__ D, double *KC,
-                             double *KX, double *KZ, double *H)
+// Accumulation over snapshot pairs k0 .. k0+Kc-1, in order, of the hessian
+// (kernels.f90:293-294), the sound-speed kernel (:301-306) and the x / z
+// components of the flow kernel (:338-355; the y terms vanish in 2D), from
+// the derivatives laid out by k_gather_pairs (Dx, Dz: [4][Kc][S][n2]).
+// Forward pair k meets adjoint snapshot nt-1-k.  The sums stay in registers
+// across the pairs; the order of the additions is that of one pair at a time.
+__global__ void k_kern_accum_pairs(const double *__restrict__ fwd, const double *__restrict__ adj, int S,
+                                   int nt, int k0, int Kc, size_t n2, const double *__restrict__ Dx,
+                                   const double *__restrict__ Dz, double *KC, double *KX, double *KZ,
+                                   double *H)
 {
-    const size_t n = (size_t)S * n2;
+    const size_t n = (size_t)S * n2, per = (size_t)Kc * n;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (size_t)gridDim.x * blockDim.x) {
         const size_t b = i / n2, q = i % n2;
-        const double *f = fwd + (b * nt + k) * 6 * n2;
-        const double *a = adj + (b * nt + (nt - 1 - k)) * 4 * n2;
-        const double faccx = f[4 * n2 + q], faccz = f[5 * n2 + q];
-        H[i] = H[i] + faccz * faccz + faccx * faccx + 0.0 * 0.0;
-        const double divf = D[0 * n + i] + D[1 * n + i];
-        const double diva = D[2 * n + i] + D[3 * n + i];
-        KC[i] = KC[i] + divf * diva;
-        const double avx = a[q], avz = a[n2 + q];
-        KX[i] = D[4 * n + i] * avx + 0.0 * 0.0 + D[5 * n + i] * avz + KX[i];
-        KZ[i] = D[6 * n + i] * avx + 0.0 * 0.0 + D[7 * n + i] * avz + KZ[i];
+        double h = H[i], kc = KC[i], kx = KX[i], kz = KZ[i];
+        for (int kk = 0; kk < Kc; kk++) {
+            const int k = k0 + kk;
+            const double *f = fwd + (b * nt + k) * 6 * n2;
+            const double *a = adj + (b * nt + (nt - 1 - k)) * 4 * n2;
+            const size_t o = (size_t)kk * n + i;
+            const double faccx = f[4 * n2 + q], faccz = f[5 * n2 + q];
+            h = h + faccz * faccz + faccx * faccx + 0.0 * 0.0;
+            const double divf = Dx[o] + Dz[o];                    // D0 + D1
+            const double diva = Dx[per + o] + Dz[per + o];        // D2 + D3
+            kc = kc + divf * diva;
+            const double avx = a[q], avz = a[n2 + q];
+            kx = Dx[2 * per + o] * avx + 0.0 * 0.0 + Dx[3 * per + o] * avz + kx;   // D4, D5
+            kz = Dz[2 * per + o] * avx + 0.0 * 0.0 + Dz[3 * per + o] * avz + kz;   // D6, D7
+        }
+        H[i] = h;
+        KC[i] = kc;
+        KX[i] = kx;
+        KZ[i] = kz;
     }
 }
 
@@ -518,20 +545,20 @@ static unsigned grid_for(size_t n)
     return (unsigned)(b > 8192 ? 8192 : (b ? b : 1));
 }
 
-hipError_t launch_gather(const double *src, double *dst, int S, int nslot, int slot, int nfield,
-                         int field, size_t n2, hipStream_t st)
+hipError_t launch_gather_pairs(const double *fwd, const double *adj, int S, int nt, int k0, int Kc, size_t n2,
+                               double *G, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_gather, dim3(grid_for((size_t)S * n2)), dim3(256), 0, st, src, dst, S,
-                       nslot, slot, nfield, field, n2);
+    hipLaunchKernelGGL(k_gather_pairs, dim3(grid_for((size_t)8 * Kc * S * n2)), dim3(256), 0, st, fwd, adj, S,
+                       nt, k0, Kc, n2, G);
     return hipGetLastError();
 }
 
-hipError_t launch_kern_accum(const double *fwd, const double *adj, int S, int nt, int k, size_t n2,
-                             const double *D, double *KC, double *KX, double *KZ, double *H,
-                             hipStream_t st)
+hipError_t launch_kern_accum_pairs(const double *fwd, const double *adj, int S, int nt, int k0, int Kc,
+                                   size_t n2, const double *Dx, const double *Dz, double *KC, double *KX,
+                                   double *KZ, double *H, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_kern_accum, dim3(grid_for((size_t)S * n2)), dim3(256), 0, st, fwd, adj, S,
-                       nt, k, n2, D, KC, KX, KZ, H);
+    hipLaunchKernelGGL(k_kern_accum_pairs, dim3(grid_for((size_t)S * n2)), dim3(256), 0, st, fwd, adj, S, nt,
+                       k0, Kc, n2, Dx, Dz, KC, KX, KZ, H);
     return hipGetLastError();
 }
 

@@ -28,11 +28,11 @@ hipError_t launch_tick(int *tctr, int inc, hipStream_t st);
 hipError_t launch_set_int(int *p, int v, hipStream_t st);
 hipError_t launch_fill(double *p, size_t n, double v, hipStream_t st);
 
-hipError_t launch_gather(const double *src, double *dst, int S, int nslot, int slot, int nfield,
-                         int field, size_t n2, hipStream_t st);
-hipError_t launch_kern_accum(const double *fwd, const double *adj, int S, int nt, int k, size_t n2,
-                             const double *D, double *KC, double *KX, double *KZ, double *H,
-                             hipStream_t st);
+hipError_t launch_gather_pairs(const double *fwd, const double *adj, int S, int nt, int k0, int Kc, size_t n2,
+                               double *G, hipStream_t st);
+hipError_t launch_kern_accum_pairs(const double *fwd, const double *adj, int S, int nt, int k0, int Kc,
+                                   size_t n2, const double *Dx, const double *Dz, double *KC, double *KX,
+                                   double *KZ, double *H, hipStream_t st);
 hipError_t launch_kern_final(int S, size_t n2, const double *rho0, const double *c2, double steps,
                              double dt, double dimen, double dimen_c, const double *t1, const double *t2,
                              const double *KC, const double *KX, const double *KZ, double *kc,
