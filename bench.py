@@ -191,7 +191,9 @@ def main():
                    "Mpoint_steps_per_s": pts * args.batch / bsec / 1e6,
                    "speedup_vs_sequential_plain": (sec * args.batch) / bsec,
                    "roofline_col": {"achieved": bach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                                    "frac": bach / HBM_PEAK_GBS, "ms_per_launch": bms}}
+                                    "frac": bach / HBM_PEAK_GBS, "ms_per_launch": bms},
+                   "shape": {k: bpipe.fwd.query(k) for k in ("colseg_tc", "colseg_ms", "colseg_split",
+                                                             "row_wide")}}
         del bpipe
 
     # roofline of the dominant kernel: HIP events on the solver's stream
@@ -228,7 +230,7 @@ def main():
         "batched": batched,
         "roofline": roofline,
         "column_pass_shape": {k: pipe.fwd.query(k) for k in ("colseg_tc", "colseg_ms", "colseg_p", "colseg_hw",
-                                                             "colseg_split")},
+                                                             "colseg_split", "row_wide")},
         "cpu_baseline": cpu,
         "other_configurations": others,
     }

@@ -100,6 +100,7 @@ struct sparc_handle {
     SegTab seg{};
     int seg_tc = 16, seg_ms = 20;
     int seg_split = 0;                   // 1: one lane per (column, segment, field)
+    int row_wide = 0;                    // 1: row pass on 32 lanes per row (nx = 256)
     bool tuned = false;
 };
 
@@ -734,6 +735,8 @@ static int setup_seg(sparc_handle *h)
     // per (column, segment, field); the widest block that fits
     h->seg_split = 0;
     if (const char *e = std::getenv("SPARC_COLSEG_SPLIT")) h->seg_split = std::atoi(e) == 1;
+    h->row_wide = 0;
+    if (const char *e = std::getenv("SPARC_ROW_WIDE")) h->row_wide = std::atoi(e) == 1 && h->nx == 256;
     h->seg_tc = 1;
     for (int tc : {2, 4, 8, 16}) {
         const bool fits = h->seg_split ? (tc * sg.P + 63) / 64 <= 2 : tc * sg.P <= 256;
@@ -828,7 +831,7 @@ static int enqueue_step(sparc_handle *h, int toff, bool filt_vx, const Sink &sk)
         ra.last1 = 0;
         ra.filt_vx = filt_vx ? 1 : 0;
         const int r = timed_launch(h, 3, [&] {
-            HCHECK(h, launch_row(0, d, h->bg, s, ra, clk, sk, h->stream));
+            HCHECK(h, launch_row(0, d, h->bg, s, ra, clk, sk, h->stream, h->row_wide));
             return SPARC_OK;
         });
         if (r) return r;
@@ -857,7 +860,7 @@ static int enqueue_step(sparc_handle *h, int toff, bool filt_vx, const Sink &sk)
             ra.last1 = (stg + 1 == 5);
             ra.bx1 = ra.last1 ? h->deltat : b[stg + 1];
             r = timed_launch(h, 2, [&] {
-                HCHECK(h, launch_row(1, d, h->bg, s, ra, clk, sk, h->stream));
+                HCHECK(h, launch_row(1, d, h->bg, s, ra, clk, sk, h->stream, h->row_wide));
                 return SPARC_OK;
             });
         } else {
@@ -865,7 +868,7 @@ static int enqueue_step(sparc_handle *h, int toff, bool filt_vx, const Sink &sk)
             ra.bx2 = h->deltat;
             ra.last2 = 1;
             r = timed_launch(h, 4, [&] {
-                HCHECK(h, launch_row(2, d, h->bg, s, ra, clk, sk, h->stream));
+                HCHECK(h, launch_row(2, d, h->bg, s, ra, clk, sk, h->stream, h->row_wide));
                 return SPARC_OK;
             });
         }
@@ -1000,6 +1003,7 @@ int sparc_query_int(const sparc_handle *h, const char *name, int *v)
     else if (n == "nph") *v = h->d.nph;
     else if (n == "colseg_tc") *v = h->seg_tc;
     else if (n == "colseg_split") *v = h->seg_split;
+    else if (n == "row_wide") *v = h->row_wide;
     else if (n == "colseg_ms") *v = h->seg_ms;
     else if (n == "colseg_p") *v = h->seg.P;
     else if (n == "colseg_hw") *v = h->seg.hw;
@@ -1215,6 +1219,62 @@ int sparc_step(sparc_handle *h, int t)
     return SPARC_OK;
 }
 
+// Row-pass team width for nx = 256 (16 or 32 lanes per row): four whole
+// steps are timed twice with each width on the handle's stream from the current
+// state, which is saved before and put back after (the row passes run at
+// three widths of work per step and their cache state depends on the column
+// passes around them, so the step is what is timed); SPARC_ROW_WIDE (0 or 1)
+// forces the choice.
+static int enqueue_step(sparc_handle *h, int toff, bool filt_vx, const Sink &sk);
+static Sink empty_sink(sparc_handle *h);
+static int tune_rows(sparc_handle *h)
+{
+    h->row_wide = 0;
+    if (h->nx != 256) return SPARC_OK;
+    if (const char *e = std::getenv("SPARC_ROW_WIDE")) {
+        const int v = std::atoi(e);
+        if (v == 0 || v == 1) { h->row_wide = v; return SPARC_OK; }
+    }
+    size_t bytes = 0;
+    int r = sparc_checkpoint_size(h, &bytes);
+    if (r) return r;
+    void *ck = nullptr;
+    HCHECK(h, hipMalloc(&ck, bytes));
+    r = sparc_save_checkpoint(h, ck);
+    Sink sk = empty_sink(h);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    // widths in the order 0 1 1 0, best of two each, so that clock ramp-up
+    // during the first timings favours neither
+    float t[2] = {1e30f, 1e30f};
+    if (!r && hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess) {
+        for (int w : {0, 1, 1, 0}) {
+            if (r) break;
+            h->row_wide = w;
+            if (launch_set_int(h->tctr, 0, h->stream) != hipSuccess) { r = SPARC_E_HIP; break; }
+            r = enqueue_step(h, 0, false, sk);                         // warm-up
+            if (!r && hipEventRecord(e0, h->stream) != hipSuccess) r = SPARC_E_HIP;
+            for (int k = 0; k < 4 && !r; k++) r = enqueue_step(h, 0, true, sk);
+            if (!r && hipEventRecord(e1, h->stream) != hipSuccess) r = SPARC_E_HIP;
+            if (!r && hipEventSynchronize(e1) != hipSuccess) r = SPARC_E_HIP;
+            float ms = 0.f;
+            if (!r && hipEventElapsedTime(&ms, e0, e1) != hipSuccess) r = SPARC_E_HIP;
+            if (!r) t[w] = std::min(t[w], ms);
+        }
+    } else if (!r) {
+        r = SPARC_E_HIP;
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    h->row_wide = (!r && t[1] < t[0]) ? 1 : 0;
+    const int rl = sparc_load_checkpoint(h, ck);
+    const hipError_t es = hipStreamSynchronize(h->stream);
+    (void)hipFree(ck);
+    if (r) return r;
+    if (rl) return rl;
+    HCHECK(h, es);
+    return SPARC_OK;
+}
+
 // Pick the segment length, column-block width and form (one lane per column
 // and segment, or per column, segment and field) of the column pass for this
 // grid and batch by timing each candidate on the handle's stream (state is
@@ -1238,7 +1298,7 @@ static int autotune(sparc_handle *h)
     if (force_tc && force_split >= 0 && ms.size() == 1) {
         if (h->nx % force_tc == 0 && force_tc * h->seg.P <= 256) h->seg_tc = force_tc;
         h->seg_split = force_split;
-        return SPARC_OK;
+        return tune_rows(h);
     }
     Sink sk = empty_sink(h);
     Clock clk{h->tctr, 0, h->steps};
@@ -1292,7 +1352,7 @@ static int autotune(sparc_handle *h)
     // stage copy, which equals its base between steps
     const size_t pz = (size_t)2 * h->S * h->d.nzpml * h->nx;
     HCHECK(h, hipMemcpyAsync(s.pz_tmp, s.pz_base, pz * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
-    return SPARC_OK;
+    return tune_rows(h);
 }
 
 // Steps per captured graph of the time loop.

@@ -267,6 +267,119 @@ __device__ __forceinline__ void fft_deriv_team(double (&re)[N2], double (&im)[N2
     }
 }
 
+// The same derivative for nx = 256 on a team of 32 lanes (two rows per
+// wave), lane t holding z[t + 32 m], m < 8.  Three-stage Cooley-Tukey,
+// 256 = 8 x (4 x 8) with n = t + 32 m, t = a + 8 b, k = k2 + 8 (c + 4 d):
+// an 8-point DFT over m in each lane; two 4-point DFTs over b per lane; an
+// 8-point DFT over a, the bin multiplier, and the inverse 8-point DFT over
+// d in each lane; then the inverse stages in reverse order.  Every lane
+// does the same work in every stage (the 16 x 16 form leaves no lane idle
+// either, but runs one row per 16 lanes).  A and B are the team's two LDS
+// buffers of 8 x 33 slots, used alternately so that each exchange needs
+// one barrier.
+__device__ __forceinline__ void fft_deriv_team_256x32(double (&re)[8], double (&im)[8], int t, double2 *A,
+                                                      double2 *B, const double2 *__restrict__ tw,
+                                                      const double *__restrict__ dmul, double stretchx)
+{
+    constexpr int LD = 33;
+    // stage 1: DFT over m, twiddle W_256^(t k2) -> A[k2][t]
+    dft_reg<8, -1>(re, im);
+#pragma unroll
+    for (int k2 = 0; k2 < 8; k2++) {
+        const double2 w = tw[t * k2];
+        A[k2 * LD + t] = make_double2(re[k2] * w.x - im[k2] * w.y, re[k2] * w.y + im[k2] * w.x);
+    }
+    __syncthreads();
+    // stage 2: for (k2, a) = (p / 8, p % 8), p = 2t, 2t+1: DFT over b of
+    // A[k2][a + 8b], twiddle W_32^(a c) -> B[k2][8c + a]
+#pragma unroll
+    for (int r = 0; r < 2; r++) {
+        const int p = 2 * t + r, k2 = p >> 3, a = p & 7;
+        double yr[4], yi[4];
+#pragma unroll
+        for (int b = 0; b < 4; b++) {
+            const double2 v = A[k2 * LD + a + 8 * b];
+            yr[b] = v.x; yi[b] = v.y;
+        }
+        dft_reg<4, -1>(yr, yi);
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            const double2 w = tw[8 * a * c];
+            B[k2 * LD + 8 * c + a] = make_double2(yr[c] * w.x - yi[c] * w.y, yr[c] * w.y + yi[c] * w.x);
+        }
+    }
+    __syncthreads();
+    // stage 3 at (k2, c) = (t / 4, t % 4): DFT over a -> X[k2 + 8c + 32d];
+    // times i*dmul[k]; inverse DFT over d -> a; twiddle W_32^(-a c) -> A[k2][8c + a]
+    {
+        const int k2 = t >> 2, c = t & 3;
+        double zr[8], zi[8];
+#pragma unroll
+        for (int a = 0; a < 8; a++) {
+            const double2 v = B[k2 * LD + 8 * c + a];
+            zr[a] = v.x; zi[a] = v.y;
+        }
+        dft_reg<8, -1>(zr, zi);
+#pragma unroll
+        for (int d = 0; d < 8; d++) {
+            const double w = dmul[k2 + 8 * c + 32 * d];
+            const double r = zr[d];
+            zr[d] = -(zi[d] * w);
+            zi[d] = r * w;
+        }
+        dft_reg<8, +1>(zr, zi);
+#pragma unroll
+        for (int a = 0; a < 8; a++) {
+            const double2 w = tw[8 * a * c];   // conj: exp(+2 pi i a c / 32)
+            A[k2 * LD + 8 * c + a] = make_double2(zr[a] * w.x + zi[a] * w.y, zi[a] * w.x - zr[a] * w.y);
+        }
+    }
+    __syncthreads();
+    // stage 2 inverse: for (k2, a): inverse DFT over c of A[k2][8c + a] -> B[k2][a + 8b]
+#pragma unroll
+    for (int r = 0; r < 2; r++) {
+        const int p = 2 * t + r, k2 = p >> 3, a = p & 7;
+        double yr[4], yi[4];
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            const double2 v = A[k2 * LD + 8 * c + a];
+            yr[c] = v.x; yi[c] = v.y;
+        }
+        dft_reg<4, +1>(yr, yi);
+#pragma unroll
+        for (int b = 0; b < 4; b++) B[k2 * LD + a + 8 * b] = make_double2(yr[b], yi[b]);
+    }
+    __syncthreads();
+    // stage 1 inverse: twiddle W_256^(-t k2), inverse DFT over k2 -> m
+#pragma unroll
+    for (int k2 = 0; k2 < 8; k2++) {
+        const double2 v = B[k2 * LD + t];
+        const double2 w = tw[t * k2];
+        re[k2] = v.x * w.x + v.y * w.y;
+        im[k2] = v.y * w.x - v.x * w.y;
+    }
+    dft_reg<8, +1>(re, im);
+#pragma unroll
+    for (int m = 0; m < 8; m++) {
+        re[m] = re[m] * stretchx;
+        im[m] = im[m] * stretchx;
+    }
+}
+
+// LDS slots (double2) a team needs for the derivative of an nx-point row.
+template <int N1, int N2> struct FftLds { static constexpr int n = N2 * (N1 + 1); };
+template <> struct FftLds<32, 8> { static constexpr int n = 2 * 8 * 33; };
+
+// The team derivative for the factorization N1 x N2.
+template <int N1, int N2>
+__device__ __forceinline__ void fft_deriv(double (&re)[N2], double (&im)[N2], int t, double2 *lds,
+                                          const double2 *__restrict__ tw, const double *__restrict__ dmul,
+                                          double stretchx)
+{
+    if constexpr (N1 == 32 && N2 == 8) fft_deriv_team_256x32(re, im, t, lds, lds + 8 * 33, tw, dmul, stretchx);
+    else fft_deriv_team<N1, N2>(re, im, t, lds, tw, dmul, stretchx);
+}
+
 // Factorization N1 x N2 of the supported row lengths (N1 >= N2).
 template <int NX> struct FftShape;
 template <> struct FftShape<16> { static constexpr int N1 = 4, N2 = 4; };

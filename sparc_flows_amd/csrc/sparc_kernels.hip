@@ -142,7 +142,7 @@ __global__ __launch_bounds__(64) void k_ddx_rows(const double *__restrict__ in,
                                                  const double *__restrict__ dmul, double stretchx)
 {
     constexpr int N1 = FftShape<NX>::N1, N2 = FftShape<NX>::N2, TEAMS = 64 / N1;
-    __shared__ double2 lds[TEAMS][N2 * (N1 + 1)];
+    __shared__ double2 lds[TEAMS][FftLds<N1, N2>::n];
     const int team = threadIdx.x / N1, t = threadIdx.x % N1;
     const int r0 = 2 * (blockIdx.x * TEAMS + team), r1 = r0 + 1;
     double re[N2], im[N2];
@@ -152,7 +152,7 @@ __global__ __launch_bounds__(64) void k_ddx_rows(const double *__restrict__ in,
         re[j2] = r0 < nrows ? in[(size_t)r0 * NX + x] : 0.0;
         im[j2] = r1 < nrows ? in[(size_t)r1 * NX + x] : 0.0;
     }
-    fft_deriv_team<N1, N2>(re, im, t, lds[team], tw, dmul, stretchx);
+    fft_deriv<N1, N2>(re, im, t, lds[team], tw, dmul, stretchx);
 #pragma unroll
     for (int j2 = 0; j2 < N2; j2++) {
         const int x = t + N1 * j2;
@@ -167,11 +167,14 @@ __global__ __launch_bounds__(64) void k_ddx_rows(const double *__restrict__ in,
 // dx p / v_x update + stage s+1 x-derivatives.  MODE 2: stage 5 dx p / v_x.
 
 
-template <int NX, int MODE>
+// N1 lanes per row: FftShape's by default; nx = 256 also runs on 32 lanes
+// (two rows per wave, three-stage transform), which is faster when the
+// pass is latency-bound (one source) and slower when it is bandwidth-bound.
+template <int NX, int MODE, int N1 = FftShape<NX>::N1>
 __global__ __launch_bounds__(64) void k_row(Dims d, Bg bg, State st, RowArgs ra, Clock clk, Sink sink)
 {
-    constexpr int N1 = FftShape<NX>::N1, N2 = FftShape<NX>::N2, TEAMS = 64 / N1;
-    __shared__ double2 lds[TEAMS][N2 * (N1 + 1)];
+    constexpr int N2 = NX / N1, TEAMS = 64 / N1;
+    __shared__ double2 lds[TEAMS][FftLds<N1, N2>::n];
     // twiddles and bin multipliers staged in LDS: their reads then never
     // wait behind this pass's global stores
     __shared__ double2 stw[NX];
@@ -219,7 +222,7 @@ __global__ __launch_bounds__(64) void k_row(Dims d, Bg bg, State st, RowArgs ra,
             }
         }
         __syncthreads();   // staged twiddles visible
-        fft_deriv_team<N1, N2>(re, im, t, lds[team], stw, sdm, bg.stretchx);
+        fft_deriv<N1, N2>(re, im, t, lds[team], stw, sdm, bg.stretchx);
         // re = dxi_x/dx, im = dv_x/dx
         double zr[N2], zi[N2];
 #pragma unroll
@@ -227,7 +230,7 @@ __global__ __launch_bounds__(64) void k_row(Dims d, Bg bg, State st, RowArgs ra,
             zr[j2] = ra.v_z[rowq + t + N1 * j2];
             zi[j2] = 0.0;
         }
-        fft_deriv_team<N1, N2>(zr, zi, t, lds[team], stw, sdm, bg.stretchx);
+        fft_deriv<N1, N2>(zr, zi, t, lds[team], stw, sdm, bg.stretchx);
 #pragma unroll
         for (int j2 = 0; j2 < N2; j2++) {
             const int x = t + N1 * j2;
@@ -251,7 +254,7 @@ __global__ __launch_bounds__(64) void k_row(Dims d, Bg bg, State st, RowArgs ra,
             rinv[j2] = bg.rhoinv_z[k];
         }
         __syncthreads();   // staged twiddles visible
-        fft_deriv_team<N1, N2>(re, im, t, lds[team], stw, sdm, bg.stretchx);
+        fft_deriv<N1, N2>(re, im, t, lds[team], stw, sdm, bg.stretchx);
         // x-PML on grad_x p (physics2d.f90:1003-1020), RHS v_x (:1039 + flows)
         const int sslot = (MODE == 2 && sink.snap) ? out_slot(time, clk.steps, sink.k0, sink.nsnap) : -1;
         const bool snap = sslot >= 0;
@@ -290,7 +293,7 @@ __global__ __launch_bounds__(64) void k_row(Dims d, Bg bg, State st, RowArgs ra,
                 wr[j2] = vx[j2];
                 wi[j2] = ra.v_z[rowq + t + N1 * j2];
             }
-            fft_deriv_team<N1, N2>(wr, wi, t, lds[team], stw, sdm, bg.stretchx);
+            fft_deriv<N1, N2>(wr, wi, t, lds[team], stw, sdm, bg.stretchx);
 #pragma unroll
             for (int j2 = 0; j2 < N2; j2++) {
                 const int x = t + N1 * j2;
@@ -495,8 +498,15 @@ hipError_t launch_ddx_rows(const double *in, double *out, int nrows, int nx, con
 }
 
 hipError_t launch_row(int mode, const Dims &d, const Bg &bg, const State &s, const RowArgs &ra,
-                      const Clock &c, const Sink &sk, hipStream_t st)
+                      const Clock &c, const Sink &sk, hipStream_t st, int wide)
 {
+    if (wide && d.nx == 256) {
+        dim3 grid((d.S * d.nz + 1) / 2);
+        if (mode == 0) hipLaunchKernelGGL((k_row<256, 0, 32>), grid, dim3(64), 0, st, d, bg, s, ra, c, sk);
+        else if (mode == 1) hipLaunchKernelGGL((k_row<256, 1, 32>), grid, dim3(64), 0, st, d, bg, s, ra, c, sk);
+        else hipLaunchKernelGGL((k_row<256, 2, 32>), grid, dim3(64), 0, st, d, bg, s, ra, c, sk);
+        return hipGetLastError();
+    }
 #define L(NXV)                                                                                   \
     {                                                                                            \
         constexpr int TEAMS = 64 / FftShape<NXV>::N1;                                           \

@@ -119,6 +119,30 @@ def test_steps_match_oracle(flow, dc, split, monkeypatch):
     assert rel_l2(px[:, 0].cpu().numpy(), opx) < 1e-10
 
 
+@pytest.mark.parametrize("wide", ["0", "1"])
+def test_steps_match_oracle_256_columns(wide, monkeypatch):
+    """40 full steps at nx = 256 with both row-pass team widths (16 lanes per
+    row, or 32 lanes and the three-stage transform)."""
+    monkeypatch.setenv("SPARC_ROW_WIDE", wide)
+    p, bg, c2d, psi = small_case(nx=256, nz=60, flow=True, dc=True)
+    maxtime = 60
+    s = Sparc(p, bg, c2d=c2d, psi=psi)
+    assert s.query("row_wide") == int(wide)
+    o = oracle.Oracle(p, bg, c2d=c2d, psi=psi)
+    s.set_forward_sources([3.0], maxtime)
+    o.set_forward_source(3.0, maxtime)
+    s.reset()
+    for t in range(40):
+        s.step(t)
+        o.step(t)
+    err = rel_l2(s.state()[:, 0].cpu().numpy(), o.state())
+    print("40 steps nx=256 wide", wide, err)
+    assert err < 1e-11
+    pv, px = s.pml()
+    opv, opx = o.pml()
+    assert rel_l2(px[:, 0].cpu().numpy(), opx) < 1e-10
+
+
 def test_forward_trace_1000_steps():
     """vz trace and snapshots of a 1000-step forward run with flow and dc."""
     p, bg, c2d, psi = small_case(nx=64, nz=60, flow=True, dc=True)
