@@ -370,16 +370,6 @@ __device__ __forceinline__ void fft_deriv_team_256x32(double (&re)[8], double (&
 template <int N1, int N2> struct FftLds { static constexpr int n = N2 * (N1 + 1); };
 template <> struct FftLds<32, 8> { static constexpr int n = 2 * 8 * 33; };
 
-// The team derivative for the factorization N1 x N2.
-template <int N1, int N2>
-__device__ __forceinline__ void fft_deriv(double (&re)[N2], double (&im)[N2], int t, double2 *lds,
-                                          const double2 *__restrict__ tw, const double *__restrict__ dmul,
-                                          double stretchx)
-{
-    if constexpr (N1 == 32 && N2 == 8) fft_deriv_team_256x32(re, im, t, lds, lds + 8 * 33, tw, dmul, stretchx);
-    else fft_deriv_team<N1, N2>(re, im, t, lds, tw, dmul, stretchx);
-}
-
 // Factorization N1 x N2 of the supported row lengths (N1 >= N2).
 template <int NX> struct FftShape;
 template <> struct FftShape<16> { static constexpr int N1 = 4, N2 = 4; };

@@ -133,6 +133,17 @@ __global__ __launch_bounds__(64) void k_ddz_cols(const double *in, double *out,
     }
 }
 
+// The team derivative for the factorization N1 x N2, dispatched in place
+// (a wrapper function around the two forms costs the 16 x 16 row kernels
+// some 30 registers).
+#define SPARC_FFT_DERIV(RE, IM, T, LDS, TW, DM, SX)                                            \
+    do {                                                                                       \
+        if constexpr (N1 == 32 && N2 == 8)                                                     \
+            fft_deriv_team_256x32(RE, IM, T, LDS, (LDS) + 8 * 33, TW, DM, SX);                 \
+        else                                                                                   \
+            fft_deriv_team<N1, N2>(RE, IM, T, LDS, TW, DM, SX);                                \
+    } while (0)
+
 // Generic x-derivative (ddx) of nrows real rows of length NX; two rows per
 // FFT team.
 template <int NX>
@@ -152,7 +163,7 @@ __global__ __launch_bounds__(64) void k_ddx_rows(const double *__restrict__ in,
         re[j2] = r0 < nrows ? in[(size_t)r0 * NX + x] : 0.0;
         im[j2] = r1 < nrows ? in[(size_t)r1 * NX + x] : 0.0;
     }
-    fft_deriv<N1, N2>(re, im, t, lds[team], tw, dmul, stretchx);
+    SPARC_FFT_DERIV(re, im, t, lds[team], tw, dmul, stretchx);
 #pragma unroll
     for (int j2 = 0; j2 < N2; j2++) {
         const int x = t + N1 * j2;
@@ -222,7 +233,7 @@ __global__ __launch_bounds__(64) void k_row(Dims d, Bg bg, State st, RowArgs ra,
             }
         }
         __syncthreads();   // staged twiddles visible
-        fft_deriv<N1, N2>(re, im, t, lds[team], stw, sdm, bg.stretchx);
+        SPARC_FFT_DERIV(re, im, t, lds[team], stw, sdm, bg.stretchx);
         // re = dxi_x/dx, im = dv_x/dx
         double zr[N2], zi[N2];
 #pragma unroll
@@ -230,7 +241,7 @@ __global__ __launch_bounds__(64) void k_row(Dims d, Bg bg, State st, RowArgs ra,
             zr[j2] = ra.v_z[rowq + t + N1 * j2];
             zi[j2] = 0.0;
         }
-        fft_deriv<N1, N2>(zr, zi, t, lds[team], stw, sdm, bg.stretchx);
+        SPARC_FFT_DERIV(zr, zi, t, lds[team], stw, sdm, bg.stretchx);
 #pragma unroll
         for (int j2 = 0; j2 < N2; j2++) {
             const int x = t + N1 * j2;
@@ -254,7 +265,7 @@ __global__ __launch_bounds__(64) void k_row(Dims d, Bg bg, State st, RowArgs ra,
             rinv[j2] = bg.rhoinv_z[k];
         }
         __syncthreads();   // staged twiddles visible
-        fft_deriv<N1, N2>(re, im, t, lds[team], stw, sdm, bg.stretchx);
+        SPARC_FFT_DERIV(re, im, t, lds[team], stw, sdm, bg.stretchx);
         // x-PML on grad_x p (physics2d.f90:1003-1020), RHS v_x (:1039 + flows)
         const int sslot = (MODE == 2 && sink.snap) ? out_slot(time, clk.steps, sink.k0, sink.nsnap) : -1;
         const bool snap = sslot >= 0;
@@ -293,7 +304,7 @@ __global__ __launch_bounds__(64) void k_row(Dims d, Bg bg, State st, RowArgs ra,
                 wr[j2] = vx[j2];
                 wi[j2] = ra.v_z[rowq + t + N1 * j2];
             }
-            fft_deriv<N1, N2>(wr, wi, t, lds[team], stw, sdm, bg.stretchx);
+            SPARC_FFT_DERIV(wr, wi, t, lds[team], stw, sdm, bg.stretchx);
 #pragma unroll
             for (int j2 = 0; j2 < N2; j2++) {
                 const int x = t + N1 * j2;

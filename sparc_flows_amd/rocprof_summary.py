@@ -5,7 +5,8 @@
 
 ``stats`` prints the per-kernel rows of ``*kernel_stats.csv`` (calls, total,
 average, min and max duration) and, when a ``*kernel_trace.csv`` is present,
-the number of dispatches, the summed kernel time and the time span they cover.
+the number of dispatches, the summed kernel time, the time span they cover and
+the idle gaps between back-to-back dispatches.
 ``pmc`` averages every counter of ``*counter_collection.csv`` per kernel over
 its dispatches (the value is per dispatch, e.g. KiB for FETCH_SIZE and
 WRITE_SIZE).  ``--prune`` deletes the per-dispatch CSV files and database files
@@ -45,6 +46,12 @@ def stats(d: str, out=sys.stdout, top: int = 16):
             busy = sum(e - s for s, e in st)
             print(f"dispatches {len(st)} busy_ms {busy / 1e6:.1f} span_ms {(st[-1][1] - st[0][0]) / 1e6:.1f}",
                   file=out)
+            # idle time between consecutive dispatches that follow each other
+            # within 100 us (back-to-back launches, e.g. inside a graph)
+            gaps = sorted(b[0] - a[1] for a, b in zip(st, st[1:]) if 0 <= b[0] - a[1] < 100_000)
+            if gaps:
+                print(f"back-to-back gaps {len(gaps)} median_us {gaps[len(gaps) // 2] / 1e3:.2f} "
+                      f"mean_us {sum(gaps) / len(gaps) / 1e3:.2f}", file=out)
     return t
 
 
