@@ -5,6 +5,8 @@ Tolerances: the reference's bar is 1e-9 relative L2 on vz traces after 1000
 steps (floating point, FP64).  Single operators are held to 1e-11 or tighter,
 multi-step runs to 1e-10.
 """
+import dataclasses
+
 import numpy as np
 import pytest
 
@@ -17,6 +19,7 @@ if not torch.cuda.is_available():
     pytest.skip("no GPU", allow_module_level=True)
 
 import oracle  # noqa: E402
+from sparc_flows_amd import background as B  # noqa: E402
 from sparc_flows_amd.solver import Sparc  # noqa: E402
 
 
@@ -239,3 +242,14 @@ def test_explicit_flow_equals_psi_flow():
     assert rel_l2(sa.background("v0x"), -v0x) < 1e-15
     se.set_flow(None, None)                 # back to the psi flow of construction (none here)
     assert np.abs(se.background("v0x")).max() == 0.0
+
+
+def test_unsupported_sizes_are_rejected():
+    """nx must be a power of two in [16, 1024]; nz needs a segmentation of
+    the column pass (673 has none); both fail at creation with an error."""
+    for nx, nz in ((100, 60), (64, 673)):
+        p, bg, _, _ = small_case(nx=64, nz=min(nz, 300), flow=False, dc=False)
+        bg = B.polytrope(nz)
+        p = dataclasses.replace(p, nx=nx, nz=nz, timestep=B.stable_timestep(bg))
+        with pytest.raises(RuntimeError):
+            Sparc(p, bg)
