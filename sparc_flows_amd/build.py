@@ -23,7 +23,7 @@ HIP_SO = LIB / "libsparc_hip.so"
 TORCH_SO = LIB / "libsparc_torch.so"
 ARCH = os.environ.get("SPARC_OFFLOAD_ARCH", "gfx950")
 
-HIP_SOURCES = ["sparc_kernels.hip", "sparc_colseg.hip", "sparc_capi.hip"]
+HIP_SOURCES = ["sparc_kernels.hip", "sparc_colseg.hip", "sparc_capi.hip", "sparc_3d.hip"]
 HIP_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared",
     # packed FP32 instructions are kept out of generated code on this platform

@@ -1,0 +1,1044 @@
+// sparc_3d.hip -- gfx950 kernels and C ABI (include/sparc_hip3d.h) of the 3D
+// solver with flows, option 16 (MP_FLOWS_PML_3D_DISPL, displacement.f90:377-477).
+//
+// One RK stage (step.f90:48-106) runs as
+//   stage input  T = a + beta_s scr, psi = pmlvars + beta_s scrpml (s > 1)
+//   x pass       d/dx of xi_x, v_x, v_y, v_z (FFT rows; x contiguous)
+//   y pass       d/dy of xi_y, v_x, v_y, v_z (FFT lines of stride nx)
+//   z pass Zp    d xi_z/dz (ibc 1) with the z-PML, div, p and rho' g / rho0
+//   x, y passes  d p/dx, d p/dy
+//   z pass Zv    (x3) d v_i/dz (ibc 1), RHS v_i = -2 v0 . grad v_i - d_i p / rho0,
+//                RHS xi_i = v_i
+//   z pass Zpz   d p/dz (ibc 0) with the z-PML; RHS v_z gets - d p/dz / rho0,
+//                - rho' g / rho0 and the source
+// and after the fifth stage one pass adds deltat * scr to a and applies
+// FILTER_VARS_3D_HORIZ_PML (physics.f90:836-882) on every other step.
+//
+// Layout: fields [nz][ny][nx], x fastest.  A z pass gives each lane one (x, y)
+// column and streams it bottom to top and back (the Thomas sweeps of dbyd2 /
+// mtridag), so the 64 lanes of a wave touch 64 consecutive doubles per row.
+// At nx = ny = 256 a pass has 65536 columns, 1024 waves: the chip is full
+// without the segmented solve the 2D column pass needs.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/sparc_hip3d.h"
+#include "sparc_device.h"
+#include "sparc_host.h"
+#include "sparc_launch.h"
+
+namespace sparc {
+namespace d3 {
+
+// out = a + beta * b over n doubles (n even; pointers 16-byte aligned).
+__global__ __launch_bounds__(256) void k3_axpy(double *out, const double *a, const double *b, double beta,
+                                               size_t n2)
+{
+    const size_t stride = (size_t)gridDim.x * 256;
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n2; i += stride) {
+        const double2 x = ((const double2 *)a)[i], y = ((const double2 *)b)[i];
+        ((double2 *)out)[i] = make_double2(x.x + beta * y.x, x.y + beta * y.y);
+    }
+}
+
+// d/dy (ddxyz, derivatives.f90:224-247) of nplanes [NY][nx] planes: a team of
+// N1 lanes transforms two neighbouring columns (x0, x0 + 1) as one complex
+// line, so each lane moves 16 bytes per row and a wave covers 2 * TEAMS
+// consecutive columns.
+template <int NY>
+__global__ __launch_bounds__(64) void k3_ddy(const double *__restrict__ in, double *__restrict__ out, int nplanes,
+                                             int nx, const double2 *__restrict__ tw,
+                                             const double *__restrict__ dmul, double stretch)
+{
+    constexpr int N1 = FftShape<NY>::N1, N2 = FftShape<NY>::N2, TEAMS = 64 / N1;
+    __shared__ double2 lds[TEAMS][FftLds<N1, N2>::n];
+    const int team = threadIdx.x / N1, t = threadIdx.x % N1;
+    const int hx = nx >> 1;
+    const long pair = (long)blockIdx.x * TEAMS + team;
+    const bool ok = pair < (long)nplanes * hx;
+    const long pl = ok ? pair / hx : 0;
+    const int x0 = ok ? 2 * (int)(pair - pl * hx) : 0;
+    const size_t base = (size_t)pl * NY * nx + x0;
+    double re[N2], im[N2];
+#pragma unroll
+    for (int j2 = 0; j2 < N2; j2++) {
+        const double2 v = *(const double2 *)(in + base + (size_t)(t + N1 * j2) * nx);
+        re[j2] = v.x;
+        im[j2] = v.y;
+    }
+    fft_deriv_team<N1, N2>(re, im, t, lds[team], tw, dmul, stretch);
+    if (ok) {
+#pragma unroll
+        for (int j2 = 0; j2 < N2; j2++)
+            *(double2 *)(out + base + (size_t)(t + N1 * j2) * nx) = make_double2(re[j2], im[j2]);
+    }
+}
+
+// Arguments of a z pass.
+struct Z3 {
+    const double *B;            // the column field [nz][nxy]
+    double *W;                  // forward-sweep values [nz][nxy]
+    int nxy, nz, nb, nt, ibc;
+    const double *stretch;      // [nz]
+    DbyCoef cf;
+    Tri tr;
+    const double *rhoinv;       // [nz] 1/rho0
+    const double *az, *bz;      // [npml][nxy]
+    const double *psi;          // the z-PML variable of this derivative [npml][nxy]
+    double *rpsi;               // its RHS
+    // Zp
+    const double *dxi, *dyi;    // d xi_x/dx, d xi_y/dy
+    const double *c2rho0, *gp0, *gr0;   // [nz][nxy], PML factor applied
+    const double *rho0, *g;     // [nz]
+    double *P, *RG;             // p; rho' g / rho0
+    // Zv
+    const double *dxv, *dyv;    // d v_i/dx, d v_i/dy
+    const double *v0x, *v0y, *v0z;
+    const double *gpi;          // d p/dx or d p/dy (i = 0, 1), unused for i = 2
+    double *rv, *rxi;           // RHS v_i, RHS xi_i
+    int comp;
+    // Zpz
+    const double *srcdep;       // [nz]
+    Src src;
+    int time, e_rad;
+};
+
+enum { ZP = 0, ZV = 1, ZPZ = 2 };
+
+__device__ __forceinline__ int pml_index(int j, int nz, int nb, int nt)
+{
+    return j < nb ? j : (j >= nz - nt ? nb + (j - (nz - nt)) : -1);
+}
+
+// One z pass: dbyd2 of the column (dbyd2.f + mtridag.f:96-112, ibc 0 or 1)
+// times stretch, then the pass's pointwise algebra per row.
+template <int EPI>
+__global__ __launch_bounds__(256) void k3_zpass(Z3 z)
+{
+    const int c = blockIdx.x * 256 + threadIdx.x;
+    if (c >= z.nxy) return;
+    const size_t s = (size_t)z.nxy;
+    const int nz = z.nz;
+    const double *__restrict__ B = z.B;
+    double *__restrict__ W = z.W;
+    FwdSweep fw;
+    fw.start(B, c, s, z.ibc, 0.0, z.cf, z.tr, W);
+    for (int j = 2; j <= nz - 3; j++) {
+        const double Bn = (j + 3 < nz) ? B[c + (size_t)(j + 3) * s] : 0.0;
+        fw.step(j, c + (size_t)j * s, Bn, z.cf, z.tr, W);
+    }
+    fw.finish(B, c, s, nz, z.ibc, 0.0, z.cf, z.tr, W);
+    // forcing of this column (physics.f90:678-737 for the adjoint)
+    double forcing = 0.0;
+    if (EPI == ZPZ && z.time <= z.src.limit) {
+        if (z.src.mode == 1)
+            forcing = (z.time >= 0 && z.time < z.src.nfwd) ? z.src.vr[c] * z.src.fwd[z.time] : 0.0;
+        else if (z.src.mode == 2)
+            forcing = lagrange_forcing(z.src, 0, c, z.nxy, z.time);
+    }
+    double d = fw.ud;
+    for (int j = nz - 1; j >= 0; j--) {
+        const size_t q = c + (size_t)j * s;
+        if (j < nz - 1) d = W[q] - z.tr.gam[j + 1] * d;
+        double dz = d * z.stretch[j];
+        const bool edge = j == 0 || j == nz - 1;
+        if (EPI == ZP || EPI == ZPZ) {
+            // z-PML (displacement.f90:400-409, 422-431)
+            const int pj = pml_index(j, nz, z.nb, z.nt);
+            if (pj >= 0) {
+                const size_t qp = c + (size_t)pj * s;
+                const double ps = z.psi[qp];
+                z.rpsi[qp] = z.az[qp] * dz + z.bz[qp] * ps;
+                dz = dz + ps;
+            }
+        }
+        if (EPI == ZP) {
+            // div, p, rho (displacement.f90:411-415); rho' g / rho0 for :458
+            const double div = (z.dxi[q] + z.dyi[q]) + dz;
+            const double xiz = B[q];
+            z.P[q] = -(z.c2rho0[q] * div) - xiz * z.gp0[q];
+            const double rho = -(z.rho0[j] * div) - xiz * z.gr0[q];
+            z.RG[q] = z.rhoinv[j] * rho * z.g[j];
+        } else if (EPI == ZV) {
+            // advection and pressure gradient (displacement.f90:436-452), RHS xi (:466-468),
+            // rows 1 and nz zeroed (:470-473); RHS v_z is completed by Zpz
+            const double adv = -2.0 * ((z.v0x[q] * z.dxv[q] + z.v0y[q] * z.dyv[q]) + z.v0z[q] * dz);
+            if (z.comp < 2) z.rv[q] = edge ? 0.0 : adv - z.rhoinv[j] * z.gpi[q];
+            else z.rv[q] = adv;
+            z.rxi[q] = edge ? 0.0 : B[q];
+        } else {
+            // RHS v_z: - d p/dz / rho0 (:452), - rho' g / rho0 (:458), source (:460-462)
+            double r = z.rv[q] - z.rhoinv[j] * dz;
+            r = r - z.RG[q];
+            const int dk = j - z.e_rad;
+            if (dk >= -4 && dk <= 4) r = r + forcing * z.srcdep[j];
+            z.rv[q] = edge ? 0.0 : r;
+        }
+    }
+}
+
+// a += deltat * scr (step.f90:107) for the six fields, then, when filt is set,
+// FILTER_VARS_3D_HORIZ_PML (physics.f90:855-878) on rows npmlbot+1 ..
+// nz-npmltop from a register window of the updated, unfiltered column.
+__global__ __launch_bounds__(256) void k3_update(double *a, const double *scr, int nxy, int nz, int nb, int nt,
+                                                 double dt, int filt)
+{
+    const long id = (long)blockIdx.x * 256 + threadIdx.x;
+    if (id >= 6L * nxy) return;
+    const int f = (int)(id / nxy), c = (int)(id - (long)f * nxy);
+    const size_t s = (size_t)nxy;
+    double *A = a + (size_t)f * nz * s + c;
+    const double *S = scr + (size_t)f * nz * s + c;
+    if (!filt) {
+        for (int j = 0; j < nz; j++) A[j * s] = A[j * s] + dt * S[j * s];
+        return;
+    }
+    for (int j = 0; j < nb - 5; j++) A[j * s] = A[j * s] + dt * S[j * s];
+    double w[11];                       // w[k] = u(j - 5 + k) at filtered row j
+#pragma unroll
+    for (int k = 0; k < 11; k++) {
+        const size_t q = (size_t)(nb - 5 + k) * s;
+        w[k] = A[q] + dt * S[q];
+    }
+#pragma unroll
+    for (int k = 0; k < 5; k++) A[(size_t)(nb - 5 + k) * s] = w[k];
+    const double c0 = 0.75390625, c1 = 0.41015625, c2 = -0.234375, c3 = 0.087890625,
+                 c4 = -0.01953125, c5 = 0.001953125;
+    for (int j = nb; j < nz - nt; j++) {
+        double v = c0 * w[5] + 0.5 * c1 * (w[4] + w[6]);
+        v = v + 0.5 * c2 * (w[3] + w[7]);
+        v = v + 0.5 * c3 * (w[2] + w[8]);
+        v = v + 0.5 * c4 * (w[1] + w[9]);
+        v = v + 0.5 * c5 * (w[0] + w[10]);
+        const int jn = j + 6;
+        const double un = jn < nz ? A[(size_t)jn * s] + dt * S[(size_t)jn * s] : 0.0;
+        A[(size_t)j * s] = v;
+#pragma unroll
+        for (int k = 0; k < 10; k++) w[k] = w[k + 1];
+        w[10] = un;
+    }
+#pragma unroll
+    for (int k = 0; k < 5; k++) A[(size_t)(nz - nt + k) * s] = w[5 + k];
+    for (int j = nz - nt + 5; j < nz; j++) A[j * s] = A[j * s] + dt * S[j * s];
+}
+
+#define SPARC3D_DISPATCH(N, MACRO) \
+    switch (N) {                   \
+    case 16: MACRO(16); break;     \
+    case 32: MACRO(32); break;     \
+    case 64: MACRO(64); break;     \
+    case 128: MACRO(128); break;   \
+    case 256: MACRO(256); break;   \
+    case 512: MACRO(512); break;   \
+    case 1024: MACRO(1024); break; \
+    default: return hipErrorInvalidValue; \
+    }
+
+static hipError_t launch_axpy(double *out, const double *a, const double *b, double beta, size_t n,
+                              hipStream_t st)
+{
+    const size_t n2 = n / 2;
+    const unsigned blocks = (unsigned)std::min<size_t>((n2 + 255) / 256, 8192);
+    hipLaunchKernelGGL(k3_axpy, dim3(blocks), dim3(256), 0, st, out, a, b, beta, n2);
+    return hipGetLastError();
+}
+
+static hipError_t launch_ddy(const double *in, double *out, int nplanes, int nx, int ny, const double2 *tw,
+                             const double *dmul, double stretch, hipStream_t st)
+{
+#define L(NYV)                                                                                     \
+    {                                                                                              \
+        constexpr int TEAMS = 64 / FftShape<NYV>::N1;                                              \
+        const long pairs = (long)nplanes * (nx / 2);                                               \
+        hipLaunchKernelGGL(k3_ddy<NYV>, dim3((unsigned)((pairs + TEAMS - 1) / TEAMS)), dim3(64), 0, \
+                           st, in, out, nplanes, nx, tw, dmul, stretch);                           \
+    }
+    SPARC3D_DISPATCH(ny, L)
+#undef L
+    return hipGetLastError();
+}
+
+static hipError_t launch_zpass(int epi, const Z3 &z, hipStream_t st)
+{
+    const dim3 grid((z.nxy + 255) / 256);
+    if (epi == ZP) hipLaunchKernelGGL(k3_zpass<ZP>, grid, dim3(256), 0, st, z);
+    else if (epi == ZV) hipLaunchKernelGGL(k3_zpass<ZV>, grid, dim3(256), 0, st, z);
+    else hipLaunchKernelGGL(k3_zpass<ZPZ>, grid, dim3(256), 0, st, z);
+    return hipGetLastError();
+}
+
+static hipError_t launch_update(double *a, const double *scr, int nxy, int nz, int nb, int nt, double dt,
+                                int filt, hipStream_t st)
+{
+    const long n = 6L * nxy;
+    hipLaunchKernelGGL(k3_update, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a, scr, nxy, nz, nb, nt,
+                       dt, filt);
+    return hipGetLastError();
+}
+
+}  // namespace d3
+}  // namespace sparc
+
+using namespace sparc;
+using namespace sparc::d3;
+
+namespace {
+constexpr double kRsun3 = 6.95989467700e10;   // initialize.f90:37
+constexpr double kPi3 = 3.141592653589793;
+}  // namespace
+
+struct sparc3d_handle {
+    int nx = 0, ny = 0, nz = 0, nb = 0, nt = 0, adjoint = 0;
+    double xlength = 0, ylength = 0, timestep = 0, cadforcing = 0, outputcad = 0, obsheight = 0, excitdep = 0;
+    double nupeak = 0, nuwidth = 0;
+    std::vector<double> model, c3d;
+    std::string err;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    size_t nxy = 0, n = 0, np = 0;      // columns, points per field, PML points per variable
+    std::vector<double> z, stretch, g, rho0z, rhoinvz, srcdep;
+    int e_rad = 0, o_rad = 0, steps = 1, cfs = 1, maxtime = -1;
+    double deltat = 0, stretchx = 0, stretchy = 0, betas[5] = {0, 0, 0, 0, 0};
+    DbyCoef cf{};
+    Tri tri[4]{};
+    DevArena bg_mem, st_mem, src_mem;
+    const double2 *twx = nullptr, *twy = nullptr;
+    const double *dmx = nullptr, *dmy = nullptr, *d_stretch = nullptr, *d_rhoinv = nullptr, *d_rho0 = nullptr,
+                 *d_g = nullptr, *d_srcdep = nullptr;
+    double *c2rho0 = nullptr, *gp0 = nullptr, *gr0 = nullptr, *az = nullptr, *bz = nullptr;
+    double *v0x = nullptr, *v0y = nullptr, *v0z = nullptr;
+    double *a = nullptr, *scr = nullptr, *tmp = nullptr, *pml = nullptr, *scrpml = nullptr, *psit = nullptr;
+    double *DXI = nullptr, *DYI = nullptr, *DX3 = nullptr, *DY3 = nullptr, *P = nullptr, *RG = nullptr,
+           *DXP = nullptr, *DYP = nullptr, *W = nullptr;
+    Src src{};
+    bool have_src = false;
+    // instrumentation
+    struct Mark { hipEvent_t a, b; int kind; };
+    std::vector<hipEvent_t> evpool;
+    std::vector<Mark> marks;
+    bool profiling = false;
+};
+
+#define HCHECK3(h, expr)                                                                   \
+    do {                                                                                   \
+        hipError_t e_ = (expr);                                                            \
+        if (e_ != hipSuccess) {                                                            \
+            (h)->err = std::string(#expr) + ": " + hipGetErrorString(e_);                  \
+            return SPARC_E_HIP;                                                            \
+        }                                                                                  \
+    } while (0)
+
+static thread_local std::string g_create_err3;
+
+static int fail3(sparc3d_handle *h, int code, const std::string &msg)
+{
+    h->err = msg;
+    return code;
+}
+
+static bool pow2_in(int n, int lo, int hi) { return n >= lo && n <= hi && (n & (n - 1)) == 0; }
+
+// Launch with HIP events around it when profiling (kinds as in sparc3d_profile).
+template <class F>
+static int timed3(sparc3d_handle *h, int kind, F launch)
+{
+    if (!h->profiling) {
+        HCHECK3(h, launch());
+        return SPARC_OK;
+    }
+    hipEvent_t e0, e1;
+    HCHECK3(h, hipEventCreate(&e0));
+    HCHECK3(h, hipEventCreate(&e1));
+    HCHECK3(h, hipEventRecord(e0, h->stream));
+    HCHECK3(h, launch());
+    HCHECK3(h, hipEventRecord(e1, h->stream));
+    h->marks.push_back({e0, e1, kind});
+    return SPARC_OK;
+}
+
+#define RUN3(kind, expr)                                       \
+    do {                                                       \
+        int r_ = timed3(h, kind, [&]() { return (expr); });    \
+        if (r_) return r_;                                     \
+    } while (0)
+
+// ddx of nrows rows, ddy of nplanes planes (device).
+static int ddx3(sparc3d_handle *h, const double *in, double *out, int nrows)
+{
+    RUN3(1, launch_ddx_rows(in, out, nrows, h->nx, h->twx, h->dmx, h->stretchx, h->stream));
+    return SPARC_OK;
+}
+
+static int ddy3(sparc3d_handle *h, const double *in, double *out, int nplanes)
+{
+    RUN3(2, launch_ddy(in, out, nplanes, h->nx, h->ny, h->twy, h->dmy, h->stretchy, h->stream));
+    return SPARC_OK;
+}
+
+static Z3 zargs(sparc3d_handle *h, const double *B, int ibc)
+{
+    Z3 z{};
+    z.B = B;
+    z.W = h->W;
+    z.nxy = (int)h->nxy;
+    z.nz = h->nz;
+    z.nb = h->nb;
+    z.nt = h->nt;
+    z.ibc = ibc;
+    z.stretch = h->d_stretch;
+    z.cf = h->cf;
+    z.tr = h->tri[ibc];
+    z.rhoinv = h->d_rhoinv;
+    z.az = h->az;
+    z.bz = h->bz;
+    return z;
+}
+
+// One RK stage s (0-based) of MP_FLOWS_PML_3D_DISPL at time index t.
+static int enqueue_stage(sparc3d_handle *h, int s, int t)
+{
+    const size_t N = h->n;
+    const int nyz = h->ny * h->nz, nz = h->nz;
+    const double *T = h->a, *PS = h->pml;
+    if (s > 0) {                                                  // step.f90:60-62
+        RUN3(0, launch_axpy(h->tmp, h->a, h->scr, h->betas[s], 6 * N, h->stream));
+        RUN3(0, launch_axpy(h->psit, h->pml, h->scrpml, h->betas[s], 2 * h->np, h->stream));
+        T = h->tmp;
+        PS = h->psit;
+    }
+    int r;
+    // ddxyz(xi_x, xi_y, xi_z) and the first derivatives of the velocity
+    if ((r = ddx3(h, T, h->DXI, nyz))) return r;
+    if ((r = ddx3(h, T + 3 * N, h->DX3, 3 * nyz))) return r;
+    if ((r = ddy3(h, T + N, h->DYI, nz))) return r;
+    if ((r = ddy3(h, T + 3 * N, h->DY3, 3 * nz))) return r;
+    {
+        Z3 z = zargs(h, T + 2 * N, 1);
+        z.psi = PS;
+        z.rpsi = h->scrpml;
+        z.dxi = h->DXI;
+        z.dyi = h->DYI;
+        z.c2rho0 = h->c2rho0;
+        z.gp0 = h->gp0;
+        z.gr0 = h->gr0;
+        z.rho0 = h->d_rho0;
+        z.g = h->d_g;
+        z.P = h->P;
+        z.RG = h->RG;
+        RUN3(3, launch_zpass(ZP, z, h->stream));
+    }
+    if ((r = ddx3(h, h->P, h->DXP, nyz))) return r;
+    if ((r = ddy3(h, h->P, h->DYP, nz))) return r;
+    for (int i = 0; i < 3; i++) {
+        Z3 z = zargs(h, T + (3 + i) * N, 1);
+        z.dxv = h->DX3 + i * N;
+        z.dyv = h->DY3 + i * N;
+        z.v0x = h->v0x;
+        z.v0y = h->v0y;
+        z.v0z = h->v0z;
+        z.gpi = i == 0 ? h->DXP : (i == 1 ? h->DYP : nullptr);
+        z.rv = h->scr + (3 + i) * N;
+        z.rxi = h->scr + i * N;
+        z.comp = i;
+        RUN3(5, launch_zpass(ZV, z, h->stream));
+    }
+    {
+        Z3 z = zargs(h, h->P, 0);
+        z.psi = PS + h->np;
+        z.rpsi = h->scrpml + h->np;
+        z.RG = h->RG;
+        z.rv = h->scr + 5 * N;
+        z.srcdep = h->d_srcdep;
+        z.src = h->src;
+        z.time = t;
+        z.e_rad = h->e_rad;
+        RUN3(4, launch_zpass(ZPZ, z, h->stream));
+    }
+    return SPARC_OK;
+}
+
+static int enqueue_step3(sparc3d_handle *h, int t)
+{
+    for (int s = 0; s < 5; s++) {
+        int r = enqueue_stage(h, s, t);
+        if (r) return r;
+    }
+    const int filt = (t + 1) % 2 == 0;                           // step.f90:114
+    RUN3(6, launch_update(h->a, h->scr, (int)h->nxy, h->nz, h->nb, h->nt, h->deltat, filt, h->stream));
+    RUN3(0, launch_axpy(h->pml, h->pml, h->scrpml, h->deltat, 2 * h->np, h->stream));
+    return SPARC_OK;
+}
+
+// d/dz of a host column (set-up): dbyd2 with ibc 1, times stretch when given.
+static int host_ddz3(sparc3d_handle *h, const std::vector<double> &in, std::vector<double> &out,
+                     const double *stretch_dev)
+{
+    const int nz = h->nz;
+    double *di = nullptr, *dout = nullptr, *w = nullptr;
+    DevArena tmp;
+    HCHECK3(h, upload(tmp, &di, in, h->stream));
+    HCHECK3(h, tmp.alloc(&dout, nz));
+    HCHECK3(h, tmp.alloc(&w, nz));
+    HCHECK3(h, launch_ddz_cols(di, dout, 1, nz, 1, 1, stretch_dev, h->cf, h->tri[1], w, 0, h->stream));
+    out.assign(nz, 0.0);
+    HCHECK3(h, hipMemcpyAsync(out.data(), dout, nz * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HCHECK3(h, hipStreamSynchronize(h->stream));
+    tmp.release();
+    return SPARC_OK;
+}
+
+static int setup3(sparc3d_handle *h)
+{
+    const int nx = h->nx, ny = h->ny, nz = h->nz;
+    const size_t nxy = h->nxy, N = h->n;
+    const double dimc = 1e6, dimrho = 1e-3, diml = kRsun3, dimc2 = std::pow(dimc, 2.0);
+    // x, y spectral tables (initialize.f90:194-225): exp(-2 pi i k/n) and i k
+    auto tables = [&](int n, double length, const double2 **tw, const double **dm) -> int {
+        std::vector<double> t(2 * (size_t)n), m(n);
+        for (int k = 0; k < n; k++) {
+            const double ang = -2.0 * kPi3 * (double)k / (double)n;
+            double c = std::cos(ang), s = std::sin(ang);
+            if (4 * k == n || 4 * k == 3 * n) c = 0.0;
+            if (2 * k == n) s = 0.0;
+            t[2 * k] = c;
+            t[2 * k + 1] = s;
+        }
+        const double norm = 1.0 / (double)n;
+        for (int k = 0; k < n; k++) {
+            const int kk = k < n / 2 ? k : (k == n / 2 ? -1 : n - k);
+            const double e = kk < 0 ? 0.0 : (double)kk * norm * 2.0 * kPi3;
+            m[k] = k <= n / 2 ? e : -e;
+        }
+        double *p = nullptr;
+        HCHECK3(h, upload(h->bg_mem, &p, t, h->stream));
+        *tw = (const double2 *)p;
+        HCHECK3(h, upload(h->bg_mem, &p, m, h->stream));
+        *dm = p;
+        (void)length;
+        return SPARC_OK;
+    };
+    int r;
+    if ((r = tables(nx, h->xlength, &h->twx, &h->dmx))) return r;
+    if ((r = tables(ny, h->ylength, &h->twy, &h->dmy))) return r;
+    h->stretchx = kRsun3 / h->xlength;
+    h->stretchy = kRsun3 / h->ylength;
+    h->cf = make_coef(nz);
+    for (int ibc = 0; ibc < 4; ibc++) {
+        std::vector<double> lowr, gam, rbet;
+        make_tri(nz, ibc, lowr, gam, rbet);
+        double *pa = nullptr, *pb = nullptr, *pc = nullptr;
+        HCHECK3(h, upload(h->bg_mem, &pa, lowr, h->stream));
+        HCHECK3(h, upload(h->bg_mem, &pb, gam, h->stream));
+        HCHECK3(h, upload(h->bg_mem, &pc, rbet, h->stream));
+        h->tri[ibc] = Tri{pa, pb, pc};
+    }
+    // z and stretch = 1/dbyd2(z) (initialize.f90:466-470); deltat (:474)
+    h->z.resize(nz);
+    for (int k = 0; k < nz; k++) h->z[k] = h->model[6 * (size_t)k];
+    std::vector<double> us;
+    if ((r = host_ddz3(h, h->z, us, nullptr))) return r;
+    h->stretch.resize(nz);
+    for (int k = 0; k < nz; k++) h->stretch[k] = 1.0 / us[k];
+    double *p = nullptr;
+    HCHECK3(h, upload(h->bg_mem, &p, h->stretch, h->stream));
+    h->d_stretch = p;
+    h->deltat = h->timestep * dimc / kRsun3;
+    // solar_data (initialize.f90:731-794)
+    std::vector<double> c2(N), p0z(nz);
+    h->rho0z.resize(nz);
+    h->g.resize(nz);
+    for (int k = 0; k < nz; k++) {
+        const double *row = &h->model[6 * (size_t)k];
+        h->rho0z[k] = row[2] / dimrho;
+        p0z[k] = row[3] / (dimrho * dimc2);
+        h->g[k] = row[4] * diml / dimc2;
+        for (size_t q = 0; q < nxy; q++) {
+            const double c = (h->c3d.empty() ? row[1] : h->c3d[(size_t)k * nxy + q]) / dimc;
+            c2[(size_t)k * nxy + q] = c * c;
+        }
+    }
+    // e_rad, o_rad, source_dep (physics.f90:44-60, 105)
+    {
+        const double t2 = 1.0 + h->excitdep / kRsun3;
+        double t = 1.0;
+        for (int k = 0; k < nz; k++)
+            if (std::fabs(h->z[k] - t2) <= t) { t = std::fabs(h->z[k] - t2); h->e_rad = k; }
+        const double tr = 1.0 + h->obsheight / kRsun3;
+        t = 1.0;
+        for (int k = 0; k < nz; k++)
+            if (std::fabs(h->z[k] - tr) <= t) { t = std::fabs(h->z[k] - tr); h->o_rad = k; }
+        h->srcdep.resize(nz);
+        const double k4 = (double)4.0e7f;
+        for (int k = 0; k < nz; k++) {
+            const double dz = h->z[k] - h->z[h->e_rad];
+            h->srcdep[k] = std::exp(-k4 * (dz * dz));
+        }
+    }
+    // rhoinv, c2rho0, d rho0/dz, d p0/dz (physics.f90:203-208)
+    h->rhoinvz.resize(nz);
+    for (int k = 0; k < nz; k++) h->rhoinvz[k] = 1.0 / h->rho0z[k];
+    std::vector<double> gr0z, gp0z;
+    if ((r = host_ddz3(h, h->rho0z, gr0z, h->d_stretch))) return r;
+    if ((r = host_ddz3(h, p0z, gp0z, h->d_stretch))) return r;
+    std::vector<double> c2rho0(N), gp0(N), gr0(N);
+    for (int k = 0; k < nz; k++)
+        for (size_t q = 0; q < nxy; q++) {
+            const size_t i = (size_t)k * nxy + q;
+            c2rho0[i] = c2[i] * h->rho0z[k];
+            gp0[i] = gp0z[k];
+            gr0[i] = gr0z[k];
+        }
+    // z-PML (physics.f90:250-296)
+    const int nb = h->nb, nt = h->nt, npml = nb + nt;
+    std::vector<double> az((size_t)npml * nxy, 0.0), bz((size_t)npml * nxy, 0.0);
+    const double *zz = h->z.data();
+    const double a0 = kPi3 * (double)0.005f * diml / dimc;
+    for (int k = 0; k < nz; k++) {
+        const bool bot = k < nb, top = k > nz - nt - 1;
+        if (!bot && !top) continue;
+        const int pj = bot ? k : nb + (k - (nz - nt));
+        double alphaz = a0;
+        if (bot) alphaz = a0 * (zz[k] - zz[0]) / (zz[nb - 1] - zz[0]);
+        if (top) alphaz = a0 * (zz[nz - 1] - zz[k]) / (zz[nz - 1] - zz[nz - nt]);
+        for (size_t q = 0; q < nxy; q++) {
+            double dampz = 0.0;
+            if (bot) {
+                const double rr = (zz[nb - 1] - zz[k]) / (zz[nb - 1] - zz[0]);
+                dampz = 18.0 * std::sqrt(c2[q]) / (zz[nb - 1] - zz[0]) * (rr * rr);
+            }
+            if (top) {
+                const double rr = (zz[nz - nt] - zz[k]) / (zz[nz - 1] - zz[nz - nt]);
+                dampz = 9.0 * std::sqrt(c2[(size_t)(nz - 1) * nxy + q]) / (zz[nz - 1] - zz[nz - nt]) * (rr * rr);
+            }
+            const double fd = alphaz / (dampz + alphaz);
+            const size_t i = (size_t)k * nxy + q;
+            gp0[i] *= fd;
+            gr0[i] *= fd;
+            if (q == 0) h->g[k] *= fd;
+            bz[(size_t)pj * nxy + q] = -alphaz - dampz;
+            az[(size_t)pj * nxy + q] = -dampz;
+        }
+    }
+    HCHECK3(h, upload(h->bg_mem, &h->c2rho0, c2rho0, h->stream));
+    HCHECK3(h, upload(h->bg_mem, &h->gp0, gp0, h->stream));
+    HCHECK3(h, upload(h->bg_mem, &h->gr0, gr0, h->stream));
+    HCHECK3(h, upload(h->bg_mem, &h->az, az, h->stream));
+    HCHECK3(h, upload(h->bg_mem, &h->bz, bz, h->stream));
+    HCHECK3(h, upload(h->bg_mem, &p, h->rhoinvz, h->stream));
+    h->d_rhoinv = p;
+    HCHECK3(h, upload(h->bg_mem, &p, h->rho0z, h->stream));
+    h->d_rho0 = p;
+    HCHECK3(h, upload(h->bg_mem, &p, h->g, h->stream));
+    h->d_g = p;
+    HCHECK3(h, upload(h->bg_mem, &p, h->srcdep, h->stream));
+    h->d_srcdep = p;
+    HCHECK3(h, h->bg_mem.alloc(&h->v0x, N));
+    HCHECK3(h, h->bg_mem.alloc(&h->v0y, N));
+    HCHECK3(h, h->bg_mem.alloc(&h->v0z, N));
+    HCHECK3(h, hipMemsetAsync(h->v0x, 0, N * sizeof(double), h->stream));
+    HCHECK3(h, hipMemsetAsync(h->v0y, 0, N * sizeof(double), h->stream));
+    HCHECK3(h, hipMemsetAsync(h->v0z, 0, N * sizeof(double), h->stream));
+    // INITIALIZE_STEP (step.f90:24-41)
+    {
+        const double o2 = 0.5, o3 = (double)0.1665579725791151184f;
+        const double o4 = (double)0.03950410250586729987f, o5 = (double)0.007810706393337838236f;
+        double *b = h->betas;
+        b[4] = o2;
+        b[3] = o3 / b[4];
+        b[2] = o4 / (b[4] * b[3]);
+        b[1] = o5 / (b[4] * b[3] * b[2]);
+        b[0] = 0.0;
+        for (int s = 0; s < 5; s++) b[s] *= h->deltat;
+    }
+    // state and work buffers
+    DevArena &m = h->st_mem;
+    HCHECK3(h, m.alloc(&h->a, 6 * N));
+    HCHECK3(h, m.alloc(&h->scr, 6 * N));
+    HCHECK3(h, m.alloc(&h->tmp, 6 * N));
+    HCHECK3(h, m.alloc(&h->pml, 2 * h->np));
+    HCHECK3(h, m.alloc(&h->scrpml, 2 * h->np));
+    HCHECK3(h, m.alloc(&h->psit, 2 * h->np));
+    HCHECK3(h, m.alloc(&h->DXI, N));
+    HCHECK3(h, m.alloc(&h->DYI, N));
+    HCHECK3(h, m.alloc(&h->DX3, 3 * N));
+    HCHECK3(h, m.alloc(&h->DY3, 3 * N));
+    HCHECK3(h, m.alloc(&h->P, N));
+    HCHECK3(h, m.alloc(&h->RG, N));
+    HCHECK3(h, m.alloc(&h->DXP, N));
+    HCHECK3(h, m.alloc(&h->DYP, N));
+    HCHECK3(h, m.alloc(&h->W, N));
+    HCHECK3(h, hipMemsetAsync(h->a, 0, 6 * N * sizeof(double), h->stream));
+    HCHECK3(h, hipMemsetAsync(h->scr, 0, 6 * N * sizeof(double), h->stream));
+    HCHECK3(h, hipMemsetAsync(h->pml, 0, 2 * h->np * sizeof(double), h->stream));
+    HCHECK3(h, hipMemsetAsync(h->scrpml, 0, 2 * h->np * sizeof(double), h->stream));
+    HCHECK3(h, hipStreamSynchronize(h->stream));
+    (void)nx;
+    (void)ny;
+    return SPARC_OK;
+}
+
+static void set_times3(sparc3d_handle *h, int maxtime)
+{
+    h->maxtime = maxtime;
+    const int cfs = h->cfs > 0 ? h->cfs : 1;
+    const int fl = maxtime / cfs + 1;                           // initialize.f90:994
+    const int ns = (int)std::floor((double)maxtime * h->timestep / h->cadforcing) + 2;
+    h->src.limit = fl * cfs;
+    h->src.num_steps = 2 * (ns / 2) + 2;                        // driver.f90:217-218
+    h->src.timestep = h->timestep;
+    h->src.cadforcing = h->cadforcing;
+}
+
+extern "C" {
+
+const char *sparc3d_last_error(const sparc3d_handle *h)
+{
+    if (!h) return g_create_err3.c_str();
+    return h->err.c_str();
+}
+
+int sparc3d_create(const sparc3d_config *cfg, sparc3d_handle **out)
+{
+    if (!cfg || !out || !cfg->model) {
+        g_create_err3 = "null argument";
+        return SPARC_E_ARG;
+    }
+    *out = nullptr;
+    if (!pow2_in(cfg->nx, 16, 1024) || !pow2_in(cfg->ny, 16, 1024)) {
+        g_create_err3 = "nx and ny must be powers of two in [16, 1024]";
+        return SPARC_E_ARG;
+    }
+    if (cfg->nz < 16 || cfg->npmlbot < 5 || cfg->npmltop < 5 || cfg->npmlbot + cfg->npmltop + 1 > cfg->nz) {
+        g_create_err3 = "nz >= 16 and z-PML rows of at least 5 each (the z filter reaches 5 rows) are needed";
+        return SPARC_E_ARG;
+    }
+    if (cfg->timestep <= 0 || cfg->xlength <= 0 || cfg->ylength <= 0 || cfg->outputcad < cfg->timestep ||
+        cfg->cadforcing < cfg->timestep) {
+        g_create_err3 = "timestep, lengths or cadences out of range";
+        return SPARC_E_ARG;
+    }
+    sparc3d_handle *h = new sparc3d_handle();
+    h->nx = cfg->nx;
+    h->ny = cfg->ny;
+    h->nz = cfg->nz;
+    h->nb = cfg->npmlbot;
+    h->nt = cfg->npmltop;
+    h->adjoint = cfg->adjoint ? 1 : 0;
+    h->xlength = cfg->xlength;
+    h->ylength = cfg->ylength;
+    h->timestep = cfg->timestep;
+    h->cadforcing = cfg->cadforcing;
+    h->outputcad = cfg->outputcad;
+    h->obsheight = cfg->obsheight;
+    h->excitdep = cfg->excitdep;
+    h->nupeak = cfg->nupeak;
+    h->nuwidth = cfg->nuwidth;
+    h->nxy = (size_t)h->nx * h->ny;
+    h->n = h->nxy * h->nz;
+    h->np = (size_t)(h->nb + h->nt) * h->nxy;
+    h->steps = (int)std::floor(h->outputcad / h->timestep);     // driver.f90:214
+    h->cfs = (int)std::floor(h->cadforcing / h->timestep);
+    h->model.assign(cfg->model, cfg->model + 6 * (size_t)h->nz);
+    if (cfg->c3d) h->c3d.assign(cfg->c3d, cfg->c3d + h->n);
+    if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
+        g_create_err3 = "hipStreamCreate failed";
+        delete h;
+        return SPARC_E_HIP;
+    }
+    h->own_stream = true;
+    const int r = setup3(h);
+    if (r) {
+        g_create_err3 = h->err;
+        sparc3d_destroy(h);
+        return r;
+    }
+    *out = h;
+    return SPARC_OK;
+}
+
+int sparc3d_destroy(sparc3d_handle *h)
+{
+    if (!h) return SPARC_OK;
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    for (auto &mk : h->marks) {
+        (void)hipEventDestroy(mk.a);
+        (void)hipEventDestroy(mk.b);
+    }
+    h->bg_mem.release();
+    h->st_mem.release();
+    h->src_mem.release();
+    if (h->own_stream && h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+    return SPARC_OK;
+}
+
+int sparc3d_set_stream(sparc3d_handle *h, void *stream)
+{
+    if (!h) return SPARC_E_ARG;
+    HCHECK3(h, hipStreamSynchronize(h->stream));
+    if (h->own_stream) (void)hipStreamDestroy(h->stream);
+    h->stream = (hipStream_t)stream;
+    h->own_stream = false;
+    return SPARC_OK;
+}
+
+int sparc3d_set_flow(sparc3d_handle *h, const double *vx, const double *vy, const double *vz)
+{
+    if (!h) return SPARC_E_ARG;
+    if ((vx == nullptr) != (vy == nullptr) || (vx == nullptr) != (vz == nullptr))
+        return fail3(h, SPARC_E_ARG, "give all three flow components or none");
+    const size_t N = h->n;
+    HCHECK3(h, hipStreamSynchronize(h->stream));
+    if (!vx) {
+        HCHECK3(h, hipMemsetAsync(h->v0x, 0, N * sizeof(double), h->stream));
+        HCHECK3(h, hipMemsetAsync(h->v0y, 0, N * sizeof(double), h->stream));
+        HCHECK3(h, hipMemsetAsync(h->v0z, 0, N * sizeof(double), h->stream));
+        HCHECK3(h, hipStreamSynchronize(h->stream));
+        return SPARC_OK;
+    }
+    const double sgn = h->adjoint ? -1.0 : 1.0;                 // driver.f90:144-147
+    std::vector<double> v(N);
+    const double *src[3] = {vx, vy, vz};
+    double *dst[3] = {h->v0x, h->v0y, h->v0z};
+    for (int c = 0; c < 3; c++) {
+        for (size_t q = 0; q < N; q++) v[q] = sgn * (src[c][q] / 1e6);
+        HCHECK3(h, hipMemcpyAsync(dst[c], v.data(), N * sizeof(double), hipMemcpyHostToDevice, h->stream));
+        HCHECK3(h, hipStreamSynchronize(h->stream));
+    }
+    return SPARC_OK;
+}
+
+int sparc3d_set_forward_source(sparc3d_handle *h, double xsrc, double ysrc, int maxtime)
+{
+    if (!h || maxtime < 0) return SPARC_E_ARG;
+    if (h->adjoint) return fail3(h, SPARC_E_STATE, "forward source on an adjoint handle");
+    HCHECK3(h, hipStreamSynchronize(h->stream));
+    set_times3(h, maxtime);
+    // FORWARD_SOURCE (driver.f90:1120-1145)
+    const int nfwd = maxtime + 8;
+    std::vector<double> fw(nfwd);
+    for (int k = 0; k < nfwd; k++) {
+        const double ttime = (double)(float)k * h->timestep;
+        const double dd = ttime - (double)0.5f / h->nupeak;
+        double v = std::cos(2.0 * kPi3 * h->nupeak * ttime) * std::exp(-(2.0 * (dd * dd) * (h->nuwidth * h->nuwidth)));
+        if (std::fabs(v) < (double)0.00001f) v = 0.0;
+        fw[k] = v * (double)1e-5f;
+    }
+    // radial form of the x profile (driver.f90:536-553)
+    const int nx = h->nx, ny = h->ny;
+    const double e8 = (double)1e-8f, k43 = (double)0.43f * kPi3;
+    const double dx = (1.0 / (double)nx) * h->xlength * e8;
+    std::vector<double> rd(h->nxy), vr(h->nxy);
+    size_t imin = 0;
+    for (int j = 0; j < ny; j++) {
+        const double yd = ((double)j / (double)ny - 0.5) * h->ylength * e8 - ysrc;
+        for (int i = 0; i < nx; i++) {
+            const double xd = ((double)i / (double)nx - 0.5) * h->xlength * e8 - xsrc;
+            const size_t q = (size_t)j * nx + i;
+            rd[q] = std::sqrt(yd * yd + xd * xd);
+            if (rd[q] < rd[imin]) imin = q;
+        }
+    }
+    for (size_t q = 0; q < h->nxy; q++) {
+        const double sinc = q == imin ? 1.0 : std::sin(k43 * rd[q]) / (k43 * rd[q]);
+        vr[q] = std::exp(-(rd[q] * rd[q]) / (400.0 * (dx * dx))) * sinc;
+    }
+    h->src_mem.release();
+    double *dvr = nullptr, *dfw = nullptr;
+    HCHECK3(h, upload(h->src_mem, &dvr, vr, h->stream));
+    HCHECK3(h, upload(h->src_mem, &dfw, fw, h->stream));
+    HCHECK3(h, hipStreamSynchronize(h->stream));
+    h->src.mode = 1;
+    h->src.row = h->e_rad;
+    h->src.vr = dvr;
+    h->src.nvr = 1;
+    h->src.fwd = dfw;
+    h->src.nfwd = nfwd;
+    h->have_src = true;
+    return SPARC_OK;
+}
+
+int sparc3d_set_adjoint_source(sparc3d_handle *h, const double *vr_dev, int nt, int maxtime)
+{
+    if (!h || !vr_dev || nt < 1 || maxtime < 0) return SPARC_E_ARG;
+    if (!h->adjoint) return fail3(h, SPARC_E_STATE, "adjoint source on a forward handle");
+    HCHECK3(h, hipStreamSynchronize(h->stream));
+    set_times3(h, maxtime);
+    const int fl = maxtime / (h->cfs > 0 ? h->cfs : 1) + 1;
+    const int nvr = fl + 8;                                     // driver.f90:588-591
+    h->src_mem.release();
+    double *dvr = nullptr;
+    HCHECK3(h, h->src_mem.alloc(&dvr, (size_t)nvr * h->nxy));
+    HCHECK3(h, hipMemsetAsync(dvr, 0, (size_t)nvr * h->nxy * sizeof(double), h->stream));
+    const int ncopy = nt < fl ? nt : fl;
+    HCHECK3(h, hipMemcpyAsync(dvr, vr_dev, (size_t)ncopy * h->nxy * sizeof(double), hipMemcpyDeviceToDevice,
+                              h->stream));
+    HCHECK3(h, hipStreamSynchronize(h->stream));
+    h->src.mode = 2;
+    h->src.row = h->o_rad;
+    h->src.vr = dvr;
+    h->src.nvr = nvr;
+    h->src.fwd = nullptr;
+    h->src.nfwd = 0;
+    h->have_src = true;
+    return SPARC_OK;
+}
+
+int sparc3d_reset(sparc3d_handle *h)
+{
+    if (!h) return SPARC_E_ARG;
+    HCHECK3(h, hipMemsetAsync(h->a, 0, 6 * h->n * sizeof(double), h->stream));
+    HCHECK3(h, hipMemsetAsync(h->scr, 0, 6 * h->n * sizeof(double), h->stream));
+    HCHECK3(h, hipMemsetAsync(h->pml, 0, 2 * h->np * sizeof(double), h->stream));
+    HCHECK3(h, hipMemsetAsync(h->scrpml, 0, 2 * h->np * sizeof(double), h->stream));
+    HCHECK3(h, hipStreamSynchronize(h->stream));
+    return SPARC_OK;
+}
+
+int sparc3d_step(sparc3d_handle *h, int t)
+{
+    if (!h || t < 0) return SPARC_E_ARG;
+    if (!h->have_src) return fail3(h, SPARC_E_STATE, "no source set");
+    int r = enqueue_step3(h, t);
+    if (r) return r;
+    HCHECK3(h, hipStreamSynchronize(h->stream));
+    return SPARC_OK;
+}
+
+int sparc3d_run(sparc3d_handle *h, int maxtime, double *vz_out)
+{
+    if (!h || maxtime < 0) return SPARC_E_ARG;
+    if (!h->have_src) return fail3(h, SPARC_E_STATE, "no source set");
+    const size_t plane = h->nxy;
+    int nout = 0;
+    for (int t = 0; t <= maxtime; t++) {
+        int r = enqueue_step3(h, t);
+        if (r) return r;
+        if (t % h->steps == 0) {                                 // driver.f90:388-419
+            if (vz_out)
+                HCHECK3(h, hipMemcpyAsync(vz_out + (size_t)nout * plane, h->a + 5 * h->n + (size_t)h->o_rad * plane,
+                                          plane * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+            nout++;
+        }
+    }
+    HCHECK3(h, hipStreamSynchronize(h->stream));
+    return SPARC_OK;
+}
+
+int sparc3d_get_state(sparc3d_handle *h, double *a_dev)
+{
+    if (!h || !a_dev) return SPARC_E_ARG;
+    HCHECK3(h, hipMemcpyAsync(a_dev, h->a, 6 * h->n * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+    HCHECK3(h, hipStreamSynchronize(h->stream));
+    return SPARC_OK;
+}
+
+int sparc3d_set_state(sparc3d_handle *h, const double *a_dev)
+{
+    if (!h || !a_dev) return SPARC_E_ARG;
+    HCHECK3(h, hipMemcpyAsync(h->a, a_dev, 6 * h->n * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+    HCHECK3(h, hipStreamSynchronize(h->stream));
+    return SPARC_OK;
+}
+
+int sparc3d_get_pml(sparc3d_handle *h, double *pml_dev)
+{
+    if (!h || !pml_dev) return SPARC_E_ARG;
+    HCHECK3(h, hipMemcpyAsync(pml_dev, h->pml, 2 * h->np * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+    HCHECK3(h, hipStreamSynchronize(h->stream));
+    return SPARC_OK;
+}
+
+int sparc3d_get_scr(sparc3d_handle *h, double *scr_dev)
+{
+    if (!h || !scr_dev) return SPARC_E_ARG;
+    HCHECK3(h, hipMemcpyAsync(scr_dev, h->scr, 6 * h->n * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+    HCHECK3(h, hipStreamSynchronize(h->stream));
+    return SPARC_OK;
+}
+
+int sparc3d_ddx(sparc3d_handle *h, const double *in, double *out, int nplanes)
+{
+    if (!h || !in || !out || nplanes < 1) return SPARC_E_ARG;
+    int r = ddx3(h, in, out, nplanes * h->ny);
+    if (r) return r;
+    HCHECK3(h, hipStreamSynchronize(h->stream));
+    return SPARC_OK;
+}
+
+int sparc3d_ddy(sparc3d_handle *h, const double *in, double *out, int nplanes)
+{
+    if (!h || !in || !out || nplanes < 1) return SPARC_E_ARG;
+    int r = ddy3(h, in, out, nplanes);
+    if (r) return r;
+    HCHECK3(h, hipStreamSynchronize(h->stream));
+    return SPARC_OK;
+}
+
+int sparc3d_ddz(sparc3d_handle *h, const double *in, double *out, int nfields, int ibc)
+{
+    if (!h || !in || !out || nfields < 1 || ibc < 0 || ibc > 3) return SPARC_E_ARG;
+    for (int f = 0; f < nfields; f++)   // the work buffer holds one field
+        HCHECK3(h, launch_ddz_cols(in + f * h->n, out + f * h->n, (int)h->nxy, h->nz, 1, ibc, h->d_stretch, h->cf,
+                                   h->tri[ibc], h->W, 0, h->stream));
+    HCHECK3(h, hipStreamSynchronize(h->stream));
+    return SPARC_OK;
+}
+
+int sparc3d_query_int(const sparc3d_handle *h, const char *name, int *v)
+{
+    if (!h || !name || !v) return SPARC_E_ARG;
+    const std::string n = name;
+    if (n == "nx") *v = h->nx;
+    else if (n == "ny") *v = h->ny;
+    else if (n == "nz") *v = h->nz;
+    else if (n == "e_rad") *v = h->e_rad;
+    else if (n == "o_rad") *v = h->o_rad;
+    else if (n == "steps") *v = h->steps;
+    else if (n == "npml") *v = h->nb + h->nt;
+    else if (n == "maxtime") *v = h->maxtime;
+    else return SPARC_E_ARG;
+    return SPARC_OK;
+}
+
+int sparc3d_query_double(const sparc3d_handle *h, const char *name, double *v)
+{
+    if (!h || !name || !v) return SPARC_E_ARG;
+    const std::string n = name;
+    if (n == "deltat") *v = h->deltat;
+    else if (n == "stretchx") *v = h->stretchx;
+    else if (n == "stretchy") *v = h->stretchy;
+    else return SPARC_E_ARG;
+    return SPARC_OK;
+}
+
+int sparc3d_profile(sparc3d_handle *h, int reps, double *ms_out)
+{
+    if (!h || reps < 1 || !ms_out) return SPARC_E_ARG;
+    if (!h->have_src) return fail3(h, SPARC_E_STATE, "no source set");
+    HCHECK3(h, hipStreamSynchronize(h->stream));
+    h->profiling = true;
+    int r = SPARC_OK;
+    for (int t = 0; t < reps && !r; t++) r = enqueue_step3(h, t);
+    h->profiling = false;
+    if (r) return r;
+    HCHECK3(h, hipStreamSynchronize(h->stream));
+    double sum[8] = {0}, cnt[8] = {0};
+    for (auto &mk : h->marks) {
+        float ms = 0.f;
+        HCHECK3(h, hipEventElapsedTime(&ms, mk.a, mk.b));
+        sum[mk.kind] += ms;
+        cnt[mk.kind] += 1;
+        (void)hipEventDestroy(mk.a);
+        (void)hipEventDestroy(mk.b);
+    }
+    h->marks.clear();
+    for (int k = 0; k < 8; k++) ms_out[k] = cnt[k] > 0 ? sum[k] / cnt[k] : 0.0;
+    return SPARC_OK;
+}
+
+}  // extern "C"

@@ -18,6 +18,7 @@
 #include "../../include/sparc_hip.h"
 #include "sparc_device.h"
 #include "sparc_launch.h"
+#include "sparc_host.h"
 
 using namespace sparc;
 
@@ -25,24 +26,6 @@ namespace {
 
 constexpr double kRsun = 6.95989467700e10;   // initialize.f90:37
 constexpr double kPi = 3.141592653589793;    // initialize.f90:37
-
-struct DevArena {
-    std::vector<void *> ptrs;
-    hipError_t alloc(double **p, size_t n)
-    {
-        void *q = nullptr;
-        hipError_t e = hipMalloc(&q, (n ? n : 1) * sizeof(double));
-        if (e != hipSuccess) return e;
-        ptrs.push_back(q);
-        *p = (double *)q;
-        return hipSuccess;
-    }
-    void release()
-    {
-        for (void *q : ptrs) (void)hipFree(q);
-        ptrs.clear();
-    }
-};
 
 }  // namespace
 
@@ -131,7 +114,7 @@ static int fail(sparc_handle *h, int code, const std::string &msg)
 
 // ---------------------------------------------------------------------------
 // dbyd2 coefficients and matrix (dbyd2.f:85-195), float32 scalars widened.
-static DbyCoef make_coef(int N2)
+DbyCoef make_coef(int N2)
 {
     DbyCoef c{};
     const float d2i = (float)(N2 - 1);
@@ -154,7 +137,7 @@ static DbyCoef make_coef(int N2)
 }
 
 // Thomas factors of the ibc matrix: lowr, gam = uppr(j-1)/betad(j-1), 1/betad.
-static void make_tri(int N2, int ibc, std::vector<double> &lowr, std::vector<double> &gam,
+void make_tri(int N2, int ibc, std::vector<double> &lowr, std::vector<double> &gam,
                      std::vector<double> &rbet)
 {
     const float alpha = 1.f / 3.f, alpha2 = 3.f / 4.f, gamma2 = 1.f / 8.f;
@@ -230,7 +213,7 @@ static bool invert(std::vector<double> A, int n, std::vector<double> &inv)
 }
 
 // Upload a host vector into a new device buffer of the arena.
-static hipError_t upload(DevArena &ar, double **dst, const std::vector<double> &v, hipStream_t st)
+hipError_t upload(DevArena &ar, double **dst, const std::vector<double> &v, hipStream_t st)
 {
     hipError_t e = ar.alloc(dst, v.size());
     if (e != hipSuccess) return e;

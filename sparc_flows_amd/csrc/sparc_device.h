@@ -381,6 +381,64 @@ template <> struct FftShape<512> { static constexpr int N1 = 32, N2 = 16; };
 template <> struct FftShape<1024> { static constexpr int N1 = 32, N2 = 32; };
 
 // ---------------------------------------------------------------------------
+// Compact-FD forward elimination (dbyd2.f + mtridag.f:96-105) of one column,
+// streaming the right-hand side; the window holds B(j-2..j+2).
+struct FwdSweep {
+    double m2, m1, c0, p1, p2, ud;
+
+    __device__ __forceinline__ void start(const double *__restrict__ B, size_t q0, size_t s, int ibc,
+                                          double preset0, const DbyCoef &cf, const Tri &tr,
+                                          double *__restrict__ W)
+    {
+        const double B0 = B[q0], B1 = B[q0 + s], B2 = B[q0 + 2 * s], B3 = B[q0 + 3 * s],
+                     B4 = B[q0 + 4 * s];
+        const double R1 = cf.b1 * B0 + cf.b2 * B1 + cf.b3 * B2 + cf.b4 * B3 + cf.b5 * B4;
+        double R0;
+        if (ibc == 1 || ibc == 3) R0 = cf.f1 * B0 + cf.f2 * B1 + cf.f3 * B2 + cf.f4 * B3;
+        else if (ibc == 0) R0 = 0.0;
+        else R0 = preset0;
+        if (ibc & 1) R0 = (cf.alpha2 * R0 - cf.beta1 * R1) * cf.cnst;
+        const double u0 = R0 * tr.rbet[0];
+        W[q0] = u0;
+        ud = (R1 - tr.lowr[1] * u0) * tr.rbet[1];
+        W[q0 + s] = ud;
+        m2 = B0; m1 = B1; c0 = B2; p1 = B3; p2 = B4;
+    }
+    // interior row j (2 <= j <= N-3); Bn = B(j+3) (unused past the end)
+    __device__ __forceinline__ void step(int j, size_t qj, double Bn, const DbyCoef &cf,
+                                         const Tri &tr, double *__restrict__ W)
+    {
+        const double R = cf.c1 * (p1 - m1) + cf.c2 * (p2 - m2);
+        ud = (R - tr.lowr[j] * ud) * tr.rbet[j];
+        W[qj] = ud;
+        m2 = m1; m1 = c0; c0 = p1; p1 = p2; p2 = Bn;
+    }
+    // rows N-2 and N-1
+    __device__ __forceinline__ void finish(const double *__restrict__ B, size_t q0, size_t s, int N,
+                                           int ibc, double presetN, const DbyCoef &cf,
+                                           const Tri &tr, double *__restrict__ W)
+    {
+        const double Bl = B[q0 + (size_t)(N - 1) * s], Bl1 = B[q0 + (size_t)(N - 2) * s],
+                     Bl2 = B[q0 + (size_t)(N - 3) * s], Bl3 = B[q0 + (size_t)(N - 4) * s],
+                     Bl4 = B[q0 + (size_t)(N - 5) * s];
+        const double Rm = -(cf.b1 * Bl + cf.b2 * Bl1 + cf.b3 * Bl2 + cf.b4 * Bl3 + cf.b5 * Bl4);
+        double RN;
+        if (ibc == 1 || ibc == 2) {
+            RN = -(cf.f1 * Bl + cf.f2 * Bl1 + cf.f3 * Bl2 + cf.f4 * Bl3);
+            RN = (cf.alpha2 * RN - cf.beta1 * Rm) * cf.cnst;
+        } else if (ibc == 0) {
+            RN = 0.0;
+        } else {
+            RN = presetN;
+        }
+        ud = (Rm - tr.lowr[N - 2] * ud) * tr.rbet[N - 2];
+        W[q0 + (size_t)(N - 2) * s] = ud;
+        ud = (RN - tr.lowr[N - 1] * ud) * tr.rbet[N - 1];
+        W[q0 + (size_t)(N - 1) * s] = ud;
+    }
+};
+
+// ---------------------------------------------------------------------------
 // Helpers shared by the pass kernels.
 
 // 11-point z filter (physics2d.f90:517-533, kernels.f90:911-918) at row k of

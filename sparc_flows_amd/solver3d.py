@@ -1,0 +1,268 @@
+"""Host-side API of the 3D solver with flows (option 16, MP_FLOWS_PML_3D_DISPL,
+displacement.f90:377-477): ctypes over the C ABI of include/sparc_hip3d.h in
+libsparc_hip.so, with torch tensors as the device buffers.
+
+* ``Sparc3D(params, model, c3d=None, adjoint=False)``  -- sparc3d_create
+* ``set_flow(vx, vy, vz)``      -- the background flow, cm/s [nz][ny][nx]
+* ``forward(xsrc, ysrc, maxtime)`` -- forward source + TIMESTEPPING; vz traces
+* ``adjoint(vr, maxtime)``      -- adjoint source (LAGRANGE_INTERP) + TIMESTEPPING
+* ``step(t)``, ``state()``, ``ddx/ddy/ddz`` -- single steps and operators
+
+All compute runs in libsparc_hip.so; nothing here falls back to another path.
+"""
+from __future__ import annotations
+
+import ctypes
+import dataclasses
+import math
+
+import numpy as np
+import torch
+
+from .params import f32
+from .solver import HIP_SO
+
+
+@dataclasses.dataclass
+class Params3D:
+    """params.i values of a 3D run (runtime here)."""
+    nx: int = 256
+    ny: int = 256
+    nz: int = 300
+    xlength: float = 200.0e8          # cm
+    ylength: float = 200.0e8          # cm
+    timestep: float = 2.0             # s
+    cadforcing: float = 30.0          # s
+    outputcad: float = 30.0           # s
+    obsheight: float = 2.0e7          # cm
+    excitdep: float = -1.5e7          # cm
+    npmlbot: int = 12
+    npmltop: int = 12
+    nupeak: float = f32(0.0017)
+    nuwidth: float = f32(0.0055)
+
+    @property
+    def steps(self) -> int:
+        return int(math.floor(self.outputcad / self.timestep))
+
+    def nt_out(self, maxtime: int) -> int:
+        return maxtime // self.steps + 1
+
+    def validate(self) -> None:
+        for n, v in (("nx", self.nx), ("ny", self.ny)):
+            if v < 16 or v > 1024 or v & (v - 1):
+                raise ValueError(f"{n}={v}: the FFT derivatives need a power of two in [16, 1024]")
+        if self.npmlbot < 5 or self.npmltop < 5 or self.npmlbot + self.npmltop + 1 > self.nz:
+            raise ValueError("z-PML rows of at least 5 each inside nz are needed")
+
+
+class _Config(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int),
+                ("npmlbot", ctypes.c_int), ("npmltop", ctypes.c_int), ("adjoint", ctypes.c_int),
+                ("xlength", ctypes.c_double), ("ylength", ctypes.c_double),
+                ("timestep", ctypes.c_double), ("cadforcing", ctypes.c_double),
+                ("outputcad", ctypes.c_double), ("obsheight", ctypes.c_double),
+                ("excitdep", ctypes.c_double), ("nupeak", ctypes.c_double),
+                ("nuwidth", ctypes.c_double), ("model", ctypes.c_void_p), ("c3d", ctypes.c_void_p)]
+
+
+_lib = None
+
+
+def lib() -> ctypes.CDLL:
+    """libsparc_hip.so with the 3D entry points typed; raises if it is not built."""
+    global _lib
+    if _lib is None:
+        if not HIP_SO.exists():
+            raise RuntimeError(f"{HIP_SO} missing; build it with `python -m sparc_flows_amd.build`")
+        L = ctypes.CDLL(str(HIP_SO))
+        vp, ip, dp = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
+        sig = {
+            "sparc3d_create": [ctypes.POINTER(_Config), ctypes.POINTER(vp)],
+            "sparc3d_destroy": [vp],
+            "sparc3d_set_stream": [vp, vp],
+            "sparc3d_set_flow": [vp, vp, vp, vp],
+            "sparc3d_set_forward_source": [vp, dp, dp, ip],
+            "sparc3d_set_adjoint_source": [vp, vp, ip, ip],
+            "sparc3d_reset": [vp],
+            "sparc3d_step": [vp, ip],
+            "sparc3d_run": [vp, ip, vp],
+            "sparc3d_get_state": [vp, vp],
+            "sparc3d_set_state": [vp, vp],
+            "sparc3d_get_pml": [vp, vp],
+            "sparc3d_get_scr": [vp, vp],
+            "sparc3d_ddx": [vp, vp, vp, ip],
+            "sparc3d_ddy": [vp, vp, vp, ip],
+            "sparc3d_ddz": [vp, vp, vp, ip, ip],
+            "sparc3d_query_int": [vp, ctypes.c_char_p, ctypes.POINTER(ip)],
+            "sparc3d_query_double": [vp, ctypes.c_char_p, ctypes.POINTER(dp)],
+            "sparc3d_profile": [vp, ip, ctypes.POINTER(dp)],
+        }
+        for name, args in sig.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = ctypes.c_int
+        L.sparc3d_last_error.argtypes = [vp]
+        L.sparc3d_last_error.restype = ctypes.c_char_p
+        _lib = L
+    return _lib
+
+
+def _ptr(t: torch.Tensor | None):
+    return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+
+class Sparc3D:
+    """One 3D source on one GPU; state [6][nz][ny][nx] in HBM."""
+
+    def __init__(self, params: Params3D, model: np.ndarray, c3d: np.ndarray | None = None,
+                 adjoint: bool = False, device=None):
+        params.validate()
+        if not torch.cuda.is_available():
+            raise RuntimeError("the sparc 3D solver needs a HIP GPU")
+        self.p = params
+        self.device = torch.device(device or "cuda")
+        self.adjoint = bool(adjoint)
+        nz = params.nz
+        self._model = np.ascontiguousarray(np.asarray(model, dtype=np.float64)[:nz])
+        if self._model.shape != (nz, 6):
+            raise ValueError("model must be [nz][6]")
+        self._c3d = None
+        if c3d is not None:
+            self._c3d = np.ascontiguousarray(np.asarray(c3d, dtype=np.float64).reshape(nz, params.ny, params.nx))
+        cfg = _Config(params.nx, params.ny, nz, params.npmlbot, params.npmltop, int(self.adjoint),
+                      params.xlength, params.ylength, params.timestep, params.cadforcing,
+                      params.outputcad, params.obsheight, params.excitdep, params.nupeak, params.nuwidth,
+                      self._model.ctypes.data, None if self._c3d is None else self._c3d.ctypes.data)
+        h = ctypes.c_void_p()
+        with torch.cuda.device(self.device):
+            self._check(lib().sparc3d_create(ctypes.byref(cfg), ctypes.byref(h)), None)
+        self.h = h
+        # queue on torch's stream, so that torch's own writes to the buffers
+        # passed in (zero fills, copies) are ordered with the solver's work
+        self._call("sparc3d_set_stream", ctypes.c_void_p(torch.cuda.current_stream(self.device).cuda_stream))
+        self.maxtime = -1
+
+    # ---- plumbing ----
+    def _check(self, rc: int, h) -> None:
+        if rc != 0:
+            msg = lib().sparc3d_last_error(h).decode(errors="replace")
+            exc = ValueError if rc == -1 else RuntimeError
+            raise exc(f"sparc3d error {rc}: {msg}")
+
+    def _call(self, name: str, *args) -> None:
+        self._check(getattr(lib(), name)(self.h, *args), self.h)
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            lib().sparc3d_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _zeros(self, *shape) -> torch.Tensor:
+        return torch.zeros(shape, dtype=torch.float64, device=self.device)
+
+    def query(self, name: str):
+        v = ctypes.c_int()
+        if lib().sparc3d_query_int(self.h, name.encode(), ctypes.byref(v)) == 0:
+            return v.value
+        d = ctypes.c_double()
+        self._check(lib().sparc3d_query_double(self.h, name.encode(), ctypes.byref(d)), self.h)
+        return d.value
+
+    # ---- set-up ----
+    def set_flow(self, vx=None, vy=None, vz=None) -> None:
+        """Background flow in cm/s, [nz][ny][nx] each (None: no flow)."""
+        if vx is None:
+            self._call("sparc3d_set_flow", None, None, None)
+            return
+        arrs = [np.ascontiguousarray(np.asarray(v, dtype=np.float64).reshape(self.p.nz, self.p.ny, self.p.nx))
+                for v in (vx, vy, vz)]
+        self._call("sparc3d_set_flow", *[a.ctypes.data for a in arrs])
+
+    def set_forward_source(self, xsrc_mm: float, ysrc_mm: float, maxtime: int) -> None:
+        self._call("sparc3d_set_forward_source", float(xsrc_mm), float(ysrc_mm), int(maxtime))
+        self.maxtime = int(maxtime)
+
+    def set_adjoint_source(self, vr: torch.Tensor, maxtime: int) -> None:
+        """vr: [nt][ny][nx] adjoint source slices at the forcing cadence."""
+        vr = torch.as_tensor(vr, dtype=torch.float64, device=self.device).contiguous()
+        self._vr = vr
+        self._call("sparc3d_set_adjoint_source", _ptr(vr), int(vr.shape[0]), int(maxtime))
+        self.maxtime = int(maxtime)
+
+    # ---- time stepping ----
+    def reset(self) -> None:
+        self._call("sparc3d_reset")
+
+    def step(self, t: int) -> None:
+        self._call("sparc3d_step", int(t))
+
+    def run(self, maxtime: int | None = None) -> torch.Tensor:
+        """Steps 0..maxtime from the current state; vz at o_rad [nout][ny][nx]."""
+        maxtime = self.maxtime if maxtime is None else int(maxtime)
+        vz = self._zeros(self.p.nt_out(maxtime), self.p.ny, self.p.nx)
+        self._call("sparc3d_run", maxtime, _ptr(vz))
+        return vz
+
+    def forward(self, xsrc_mm: float, ysrc_mm: float, maxtime: int) -> torch.Tensor:
+        self.set_forward_source(xsrc_mm, ysrc_mm, maxtime)
+        self.reset()
+        return self.run(maxtime)
+
+    def adjoint_run(self, vr, maxtime: int) -> torch.Tensor:
+        self.set_adjoint_source(vr, maxtime)
+        self.reset()
+        return self.run(maxtime)
+
+    # ---- state and operators ----
+    def state(self) -> torch.Tensor:
+        a = self._zeros(6, self.p.nz, self.p.ny, self.p.nx)
+        self._call("sparc3d_get_state", _ptr(a))
+        return a
+
+    def set_state(self, a: torch.Tensor) -> None:
+        a = torch.as_tensor(a, dtype=torch.float64, device=self.device).contiguous()
+        self._call("sparc3d_set_state", _ptr(a))
+
+    def pml(self) -> torch.Tensor:
+        npml = self.query("npml")
+        t = self._zeros(2, npml, self.p.ny, self.p.nx)
+        self._call("sparc3d_get_pml", _ptr(t))
+        return t
+
+    def scr(self) -> torch.Tensor:
+        t = self._zeros(6, self.p.nz, self.p.ny, self.p.nx)
+        self._call("sparc3d_get_scr", _ptr(t))
+        return t
+
+    def ddx(self, f: torch.Tensor) -> torch.Tensor:
+        f = torch.as_tensor(f, dtype=torch.float64, device=self.device).contiguous()
+        out = torch.empty_like(f)
+        self._call("sparc3d_ddx", _ptr(f), _ptr(out), f.numel() // (self.p.ny * self.p.nx))
+        return out
+
+    def ddy(self, f: torch.Tensor) -> torch.Tensor:
+        f = torch.as_tensor(f, dtype=torch.float64, device=self.device).contiguous()
+        out = torch.empty_like(f)
+        self._call("sparc3d_ddy", _ptr(f), _ptr(out), f.numel() // (self.p.ny * self.p.nx))
+        return out
+
+    def ddz(self, f: torch.Tensor, ibc: int) -> torch.Tensor:
+        f = torch.as_tensor(f, dtype=torch.float64, device=self.device).contiguous()
+        out = torch.zeros_like(f)
+        n = f.numel() // (self.p.nz * self.p.ny * self.p.nx)
+        self._call("sparc3d_ddz", _ptr(f), _ptr(out), n, int(ibc))
+        return out
+
+    def profile(self, reps: int = 2) -> dict:
+        """Mean ms per launch of each pass kind over `reps` steps (HIP events)."""
+        ms = (ctypes.c_double * 8)()
+        self._check(lib().sparc3d_profile(self.h, int(reps), ms), self.h)
+        names = ["stage", "ddx", "ddy", "z_p", "z_dp", "z_v", "update", "other"]
+        return {n: ms[i] for i, n in enumerate(names)}

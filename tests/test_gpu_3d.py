@@ -1,0 +1,147 @@
+"""GPU parity of the 3D solver (option 16) against the 3D oracle on the same
+seeded inputs, through the C ABI (include/sparc_hip3d.h, ctypes).
+
+Tolerances: single operators 1e-12 relative, one step 1e-11, 200-step vz
+traces and states 1e-10 relative L2 (FP64; the reference's 1e-9 bar)."""
+import dataclasses
+
+import numpy as np
+import pytest
+
+from tests.cases import rel_l2
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+if not torch.cuda.is_available():
+    pytest.skip("no GPU", allow_module_level=True)
+
+from oracle.oracle3d import Oracle3D, Params3D as OParams  # noqa: E402
+from sparc_flows_amd import background as B  # noqa: E402
+from sparc_flows_amd.solver3d import Params3D, Sparc3D  # noqa: E402
+
+
+def case(nx=32, ny=16, nz=40, flow=True, dc=True, npml=6, seed=0):
+    bg = B.polytrope(nz)
+    p = Params3D(nx=nx, ny=ny, nz=nz, timestep=B.stable_timestep(bg), npmlbot=npml, npmltop=npml,
+                 xlength=100e8, ylength=100e8 * ny / nx)
+    x = (np.arange(nx) / nx - 0.5)[None, None, :]
+    y = (np.arange(ny) / ny - 0.5)[None, :, None]
+    zz = (np.arange(nz) / nz)[:, None, None]
+    c3d = None
+    if dc:
+        c3d = bg[:, 1][:, None, None] * (1 + 0.05 * np.exp(-(x ** 2 + y ** 2) / 0.03 - (zz - 0.7) ** 2 / 0.01))
+    fl = None
+    if flow:
+        rng = np.random.default_rng(seed)
+        amp = rng.uniform(0.5, 1.5, 3) * 2.4e4       # 240 m/s scale
+        env = np.exp(-((x - 0.1) ** 2 + (y + 0.05) ** 2) / 0.02 - (zz - 0.75) ** 2 / 0.005)
+        fl = np.stack([amp[0] * env * np.cos(2 * np.pi * y), amp[1] * env * np.sin(2 * np.pi * x),
+                       amp[2] * env * 0.3]).astype(np.float64)
+        fl = np.broadcast_to(fl, (3, nz, ny, nx)).copy()
+    return p, bg, c3d, fl
+
+
+def pair(p, bg, c3d, fl, adjoint=False):
+    o = Oracle3D(OParams(**dataclasses.asdict(p)), bg, c3d=c3d, flow=fl, adjoint=adjoint)
+    s = Sparc3D(p, bg, c3d=c3d, adjoint=adjoint)
+    if fl is not None:
+        s.set_flow(*fl)
+    return o, s
+
+
+def test_derivatives_match_oracle():
+    p, bg, c3d, fl = case()
+    o, s = pair(p, bg, c3d, fl)
+    rng = np.random.default_rng(1)
+    f = rng.standard_normal((2, p.nz, p.ny, p.nx))
+    ft = torch.as_tensor(f, device="cuda")
+    assert rel_l2(s.ddx(ft).cpu().numpy(), o.ddx(f)) < 1e-12
+    assert rel_l2(s.ddy(ft).cpu().numpy(), o.ddy(f)) < 1e-12
+    for ibc in (0, 1):
+        got = s.ddz(ft, ibc).cpu().numpy()
+        assert rel_l2(got[0], o.ddz(f[0], ibc)) < 1e-12 and rel_l2(got[1], o.ddz(f[1], ibc)) < 1e-12
+
+
+def test_background_facts_match_oracle():
+    p, bg, c3d, fl = case()
+    o, s = pair(p, bg, c3d, fl)
+    assert s.query("e_rad") == o.e_rad and s.query("o_rad") == o.o_rad
+    assert s.query("deltat") == o.deltat
+    assert s.query("npml") == len(o.pml_rows)
+
+
+@pytest.mark.parametrize("t", [0, 1])
+def test_one_step_matches_oracle(t):
+    """One STEP from a seeded state (odd t: the z filter runs)."""
+    p, bg, c3d, fl = case()
+    o, s = pair(p, bg, c3d, fl)
+    o.set_forward_source(3.0, -2.0, 50)
+    s.set_forward_source(3.0, -2.0, 50)
+    rng = np.random.default_rng(2)
+    a0 = 1e-8 * rng.standard_normal((6, p.nz, p.ny, p.nx))
+    o.a = a0.copy()
+    s.set_state(torch.as_tensor(a0, device="cuda"))
+    o.step(t)
+    s.step(t)
+    got = s.state().cpu().numpy()
+    for f in range(6):
+        assert rel_l2(got[f], o.a[f]) < 1e-11, f
+    assert rel_l2(s.pml().cpu().numpy(), o.pmlvars) < 1e-11
+
+
+@pytest.mark.parametrize("flow,dc", [(False, False), (True, True)])
+def test_forward_run_matches_oracle(flow, dc):
+    p, bg, c3d, fl = case(flow=flow, dc=dc)
+    o, s = pair(p, bg, c3d, fl)
+    maxtime = 200
+    o.set_forward_source(5.0, 0.0, maxtime)
+    ovz = o.run(maxtime)
+    vz = s.forward(5.0, 0.0, maxtime).cpu().numpy()
+    e = rel_l2(vz, ovz)
+    es = rel_l2(s.state().cpu().numpy(), o.a)
+    print("3D forward vz", e, "state", es)
+    assert np.abs(ovz).max() > 0
+    assert e < 1e-10 and es < 1e-10
+
+
+def test_adjoint_run_matches_oracle():
+    p, bg, c3d, fl = case()
+    o, s = pair(p, bg, c3d, fl, adjoint=True)
+    maxtime = 120
+    nt = maxtime // int(p.cadforcing // p.timestep) + 1
+    rng = np.random.default_rng(4)
+    vr = 1e-6 * rng.standard_normal((nt, p.ny, p.nx))
+    o.set_adjoint_source(vr, maxtime)
+    ovz = o.run(maxtime)
+    vz = s.adjoint_run(torch.as_tensor(vr, device="cuda"), maxtime).cpu().numpy()
+    e = rel_l2(vz, ovz)
+    es = rel_l2(s.state().cpu().numpy(), o.a)
+    print("3D adjoint vz", e, "state", es)
+    assert e < 1e-10 and es < 1e-10
+
+
+def test_full_size_exchange_symmetry():
+    """256 x 256 x 300: a centred source in an x <-> y symmetric medium and flow
+    keeps xi_x(x, y) = xi_y(y, x), v_x(x, y) = v_y(y, x) and xi_z, v_z symmetric."""
+    nx = ny = 256
+    nz = 300
+    bg = B.polytrope(nz)
+    p = Params3D(nx=nx, ny=ny, nz=nz, timestep=B.stable_timestep(bg), npmlbot=12, npmltop=12)
+    x = (np.arange(nx) / nx - 0.5)[None, None, :]
+    y = (np.arange(ny) / ny - 0.5)[None, :, None]
+    zz = (np.arange(nz) / nz)[:, None, None]
+    env = np.exp(-(x ** 2 + y ** 2) / 0.01 - (zz - 0.85) ** 2 / 0.002)
+    fl = np.stack([2e4 * env * x, 2e4 * env * y, 5e3 * env + 0 * x * y])
+    fl = np.broadcast_to(fl, (3, nz, ny, nx))
+    s = Sparc3D(p, bg)
+    s.set_flow(*fl)
+    vz = s.forward(0.0, 0.0, 12)
+    a = s.state()
+    assert torch.isfinite(a).all() and torch.isfinite(vz).all()
+    sw = lambda t: t.transpose(-1, -2)
+    scale = a.abs().max().item()
+    assert scale > 0
+    for i, j in ((0, 1), (3, 4), (2, 2), (5, 5)):
+        d = (a[i] - sw(a[j])).abs().max().item()
+        assert d <= 1e-11 * scale, (i, j, d, scale)
