@@ -83,6 +83,50 @@ def forward_rate(nx, nz, nsrc, nsteps, flow, dc, reps=1):
             "column_pass_shape": {k: s.query(k) for k in ("colseg_tc", "colseg_ms", "colseg_p", "colseg_split")}}
 
 
+def forward3d_rate(nsteps=20, warm=2, nx=256, ny=256, nz=300):
+    """3D forward (option 16) at 256x256x300 with the axisymmetric supergranule
+    flow and the radially symmetric sound-speed bump: Mpoint-steps/s, the
+    per-pass launch times (HIP events on the solver's stream) and the roofline
+    of the dominant pass.  Algorithmic bytes per point of each pass: what it
+    must read and write once (the Thomas sweep's scratch round trip excluded)."""
+    import torch
+    from sparc_flows_amd import background as B
+    from sparc_flows_amd.solver3d import Params3D, Sparc3D
+
+    bg = B.polytrope(nz)
+    p = Params3D(nx=nx, ny=ny, nz=nz, timestep=B.stable_timestep(bg), npmlbot=12, npmltop=12)
+    c3d = B.sound_speed_bump_3d(nx, ny, p.xlength, p.ylength, bg)
+    s = Sparc3D(p, bg, c3d=c3d)
+    s.set_flow(*B.supergranule_flow_3d(nx, ny, p.xlength, p.ylength, bg))
+    s.set_forward_source(0.0, 0.0, warm + nsteps)
+    s.reset()
+    for t in range(warm):
+        s.step(t)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for t in range(warm, warm + nsteps):
+        s.step(t)
+    torch.cuda.synchronize()
+    sec = (time.perf_counter() - t0) / nsteps
+    ms = s.profile(2)
+    # launches per step: 4 x 2 stage inputs + 1 PML update, 5 x 3 ddx, 5 x 3 ddy, 5 Zp, 5 Zv, 1 update
+    per_step = {"stage": 9 * ms["stage"], "ddx": 15 * ms["ddx"], "ddy": 15 * ms["ddy"],
+                "z_p": 5 * ms["z_p"], "z_v": 5 * ms["z_v"], "update": ms["update"]}
+    # Zv: v_x, v_y, v_z, p, d v_i/dx, d v_i/dy, v0 (3), d p/dx, d p/dy, rho' g/rho0
+    # read, six RHS fields written: 22 doubles; Zp: xi_z, d xi_x/dx, d xi_y/dy,
+    # c2 rho0, d p0/dz, d rho0/dz read, p and rho' g/rho0 written: 8
+    bpp = {"z_v": 22 * 8, "z_p": 8 * 8, "ddx": 16, "ddy": 16, "stage": 24, "update": 6 * 24}
+    dom = max(per_step, key=per_step.get)
+    npts = nx * ny * nz
+    achieved = bpp[dom] * npts / (ms[dom] * 1e-3) / 1e9
+    return {"Mpoint_steps_per_s": npts / sec / 1e6, "ms_per_step": sec * 1e3, "grid": [nx, ny, nz],
+            "time_steps_timed": nsteps, "flow": "axisymmetric supergranule 240 m/s",
+            "dc": "radial bump 5%", "ms_per_launch": ms, "ms_per_step_by_pass": per_step,
+            "roofline": {"kernel": dom, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                         "algorithmic_bytes_per_point": bpp[dom]}}
+
+
 def cpu_baseline(p, bg, psi, seconds=12.0):
     """Time the C oracle (one core) on a bounded sample of the same forward run."""
     import oracle
@@ -113,6 +157,8 @@ def main():
     ap.add_argument("--no-configs", action="store_true",
                     help="skip the plain-forward (128x150, 500 steps) and batched 16-source forward "
                          "(128x150 with flow and sound-speed bump, 4000 steps) lines")
+    ap.add_argument("--only-3d", action="store_true",
+                    help="only run the 3D forward line (256x256x300) and print it")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a plain device-to-device copy of this many MiB ten times "
                          "(calibrates the FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -135,6 +181,9 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         print(json.dumps({"copy_probe_bytes": n * 8, "launches": 10}))
+        return
+    if args.only_3d:
+        print(json.dumps({"forward3d_256x256x300": forward3d_rate()}))
         return
     p, bg, psi = make_config()
     maxtime = args.nsteps - 1
@@ -213,7 +262,8 @@ def main():
     others = None
     if not args.no_configs:
         others = {"plain_forward_128x150": forward_rate(128, 150, 1, 500, False, False, reps=3),
-                  "batched16_forward_128x150": forward_rate(128, 150, 16, 4000, True, True)}
+                  "batched16_forward_128x150": forward_rate(128, 150, 16, 4000, True, True),
+                  "forward3d_256x256x300": forward3d_rate()}
         log("forward configurations done")
     cpu = None if args.no_cpu else cpu_baseline(p, bg, psi, args.cpu_seconds)
     log("cpu baseline done")

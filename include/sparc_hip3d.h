@@ -104,8 +104,9 @@ int sparc3d_query_double(const sparc3d_handle *h, const char *name, double *v);
 
 /* Instrumentation: run `reps` steps with HIP events around every kernel and
  * return the mean milliseconds per launch of each pass kind in ms_out[8]:
- * stage update, x-derivative, y-derivative, z pass of xi_z (p, rho), z pass
- * of p, z pass of a velocity component, step update + filter, other. */
+ * stage input (a + beta scr), x-derivative, y-derivative, z pass of xi_z
+ * (div, p, rho'), z pass of v_x, v_y, v_z and p (the RHS), step update + z
+ * filter, and two unused slots. */
 int sparc3d_profile(sparc3d_handle *h, int reps, double *ms_out);
 
 #ifdef __cplusplus

@@ -128,6 +128,45 @@ def sound_speed_bump(nx: int, xlength_cm: float, bg: np.ndarray, amp: float = 0.
     return bg[:, 1][:, None] * (1.0 + bump)
 
 
+def supergranule_flow_3d(nx: int, ny: int, xlength_cm: float, ylength_cm: float, bg: np.ndarray,
+                          vmax_ms: float = 240.0, sigr_mm: float = 15.0, sigz_mm: float = 0.9,
+                          z0_mm: float = -2.3):
+    """Axisymmetric supergranule: the 2D stream function rotated about the z axis,
+    rho v = curl(psi phi_hat) with psi = A exp(-r^2/2sr^2) exp(-(z-z0)^2/2sz^2);
+    returns (vx, vy, vz) in cm/s, [nz][ny][nx] each, peak |v| = vmax."""
+    x = grid_x_mm(nx, xlength_cm) * 1e8
+    y = grid_x_mm(ny, ylength_cm) * 1e8
+    z = grid_z_mm(bg) * 1e8
+    rho = bg[:, 2][:, None, None]
+    X, Y = np.meshgrid(x, y)
+    r = np.sqrt(X ** 2 + Y ** 2)
+    sr, sz, z0 = sigr_mm * 1e8, sigz_mm * 1e8, z0_mm * 1e8
+    gz = np.exp(-(z - z0) ** 2 / (2 * sz ** 2))[:, None, None]
+    dgz = (-(z - z0) / sz ** 2)[:, None, None] * gz
+    gr = np.exp(-r ** 2 / (2 * sr ** 2))[None]
+    # v_r = -(1/rho) d psi/dz, v_z = (1/rho) (1/r) d(r psi)/dr
+    vr = -(gr * dgz) / rho
+    vz = (gz * gr * (2.0 - r[None] ** 2 / sr ** 2)) / rho
+    with np.errstate(invalid="ignore", divide="ignore"):
+        cx = np.where(r > 0, X / r, 0.0)[None]
+        cy = np.where(r > 0, Y / r, 0.0)[None]
+    vx, vy = vr * cx, vr * cy
+    amp = vmax_ms * 1e2 / float(np.sqrt(vx ** 2 + vy ** 2 + vz ** 2).max())
+    return amp * vx, amp * vy, amp * vz
+
+
+def sound_speed_bump_3d(nx: int, ny: int, xlength_cm: float, ylength_cm: float, bg: np.ndarray,
+                        amp: float = 0.05, sigr_mm: float = 10.0, sigz_mm: float = 1.5,
+                        z0_mm: float = -3.0) -> np.ndarray:
+    """c(x, y, z) in cm/s with the radially symmetric bump dc/c = amp exp(-r^2/2sr^2) exp(-(z-z0)^2/2sz^2)."""
+    x = grid_x_mm(nx, xlength_cm)
+    y = grid_x_mm(ny, ylength_cm)
+    z = grid_z_mm(bg)
+    r2 = x[None, None, :] ** 2 + y[None, :, None] ** 2
+    bump = amp * np.exp(-r2 / (2 * sigr_mm ** 2)) * np.exp(-(z[:, None, None] - z0_mm) ** 2 / (2 * sigz_mm ** 2))
+    return bg[:, 1][:, None, None] * (1.0 + bump)
+
+
 def source_positions(nsrc: int, seed: int = 0, lo: float = -60.0, hi: float = 60.0) -> np.ndarray:
     """Source x positions (Mm) drawn uniformly and rounded to %.1f like master.pixels
     (generate_master_pixels.py:8-14)."""

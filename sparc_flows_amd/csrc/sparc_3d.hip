@@ -7,18 +7,18 @@
 //   y pass       d/dy of xi_y, v_x, v_y, v_z (FFT lines of stride nx)
 //   z pass Zp    d xi_z/dz (ibc 1) with the z-PML, div, p and rho' g / rho0
 //   x, y passes  d p/dx, d p/dy
-//   z pass Zv    (x3) d v_i/dz (ibc 1), RHS v_i = -2 v0 . grad v_i - d_i p / rho0,
-//                RHS xi_i = v_i
-//   z pass Zpz   d p/dz (ibc 0) with the z-PML; RHS v_z gets - d p/dz / rho0,
-//                - rho' g / rho0 and the source
+//   z pass Zv    d v_x/dz, d v_y/dz, d v_z/dz (ibc 1) and d p/dz (ibc 0, z-PML)
+//                as four chains of one lane; RHS v_i = -2 v0 . grad v_i
+//                - d_i p / rho0, RHS v_z - rho' g / rho0 + source, RHS xi_i = v_i
 // and after the fifth stage one pass adds deltat * scr to a and applies
 // FILTER_VARS_3D_HORIZ_PML (physics.f90:836-882) on every other step.
 //
 // Layout: fields [nz][ny][nx], x fastest.  A z pass gives each lane one (x, y)
 // column and streams it bottom to top and back (the Thomas sweeps of dbyd2 /
 // mtridag), so the 64 lanes of a wave touch 64 consecutive doubles per row.
-// At nx = ny = 256 a pass has 65536 columns, 1024 waves: the chip is full
-// without the segmented solve the 2D column pass needs.
+// At nx = ny = 256 a pass has 65536 columns, 1024 waves, one per SIMD: the
+// sweeps issue their loads in chunks of rows (and Zv runs four independent
+// chains) so that each lane keeps tens of loads in flight.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -78,76 +78,107 @@ __global__ __launch_bounds__(64) void k3_ddy(const double *__restrict__ in, doub
     }
 }
 
-// Arguments of a z pass.
+// Arguments of the two z passes.
 struct Z3 {
-    const double *B;            // the column field [nz][nxy]
-    double *W;                  // forward-sweep values [nz][nxy]
-    int nxy, nz, nb, nt, ibc;
+    const double *B;            // Zp: xi_z; Zv: v_x (v_y, v_z follow at +N, +2N)
+    double *W;                  // forward-sweep values, one [nz][nxy] block per chain
+    size_t N;                   // points per field
+    int nxy, nz, nb, nt;
     const double *stretch;      // [nz]
     DbyCoef cf;
-    Tri tr;
+    Tri tr1, tr0;               // dbyd2 factors for ibc 1 and ibc 0
     const double *rhoinv;       // [nz] 1/rho0
     const double *az, *bz;      // [npml][nxy]
-    const double *psi;          // the z-PML variable of this derivative [npml][nxy]
-    double *rpsi;               // its RHS
+    const double *psi;          // z-PML variables [2][npml][nxy]: d xi_z/dz, d p/dz
+    double *rpsi;               // their RHS
     // Zp
     const double *dxi, *dyi;    // d xi_x/dx, d xi_y/dy
     const double *c2rho0, *gp0, *gr0;   // [nz][nxy], PML factor applied
     const double *rho0, *g;     // [nz]
     double *P, *RG;             // p; rho' g / rho0
     // Zv
-    const double *dxv, *dyv;    // d v_i/dx, d v_i/dy
+    const double *dxv, *dyv;    // d v_i/dx, d v_i/dy, [3][nz][nxy]
     const double *v0x, *v0y, *v0z;
-    const double *gpi;          // d p/dx or d p/dy (i = 0, 1), unused for i = 2
-    double *rv, *rxi;           // RHS v_i, RHS xi_i
-    int comp;
-    // Zpz
+    const double *dxp, *dyp;    // d p/dx, d p/dy
+    double *rhs;                // scr [6][nz][nxy]
     const double *srcdep;       // [nz]
     Src src;
     int time, e_rad;
 };
-
-enum { ZP = 0, ZV = 1, ZPZ = 2 };
 
 __device__ __forceinline__ int pml_index(int j, int nz, int nb, int nt)
 {
     return j < nb ? j : (j >= nz - nt ? nb + (j - (nz - nt)) : -1);
 }
 
-// One z pass: dbyd2 of the column (dbyd2.f + mtridag.f:96-112, ibc 0 or 1)
-// times stretch, then the pass's pointwise algebra per row.
-template <int EPI>
-__global__ __launch_bounds__(256) void k3_zpass(Z3 z)
+// Forward elimination (dbyd2.f + mtridag.f:96-105) of NC columns of one lane
+// at once, rows 2..nz-3 in chunks of U whose loads are issued together: a
+// lane then has NC * U loads in flight instead of one per dependent step.
+template <int NC, int U>
+__device__ __forceinline__ void sweep_fwd(FwdSweep (&fw)[NC], const double *const (&B)[NC],
+                                          double *const (&W)[NC], const int (&ibc)[NC], const Tri *const (&tr)[NC],
+                                          size_t c, size_t s, int nz, const DbyCoef &cf)
 {
+#pragma unroll
+    for (int k = 0; k < NC; k++) fw[k].start(B[k], c, s, ibc[k], 0.0, cf, *tr[k], W[k]);
+    for (int j0 = 2; j0 <= nz - 3; j0 += U) {
+        double bn[NC][U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const size_t qn = c + (size_t)min(j0 + u + 3, nz - 1) * s;
+#pragma unroll
+            for (int k = 0; k < NC; k++) bn[k][u] = B[k][qn];
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int j = j0 + u;
+            if (j <= nz - 3) {
+#pragma unroll
+                for (int k = 0; k < NC; k++)
+                    fw[k].step(j, c + (size_t)j * s, j + 3 < nz ? bn[k][u] : 0.0, cf, *tr[k], W[k]);
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < NC; k++) fw[k].finish(B[k], c, s, nz, ibc[k], 0.0, cf, *tr[k], W[k]);
+}
+
+// Zp: d xi_z/dz (ibc 1) with the z-PML (displacement.f90:400-409), div, p and
+// rho' (:411-415), and rho' g / rho0 for RHS v_z (:458).  One lane per column.
+__global__ __launch_bounds__(256) void k3_zp(Z3 z)
+{
+    constexpr int UF = 16, UB = 4;
     const int c = blockIdx.x * 256 + threadIdx.x;
     if (c >= z.nxy) return;
     const size_t s = (size_t)z.nxy;
     const int nz = z.nz;
-    const double *__restrict__ B = z.B;
-    double *__restrict__ W = z.W;
-    FwdSweep fw;
-    fw.start(B, c, s, z.ibc, 0.0, z.cf, z.tr, W);
-    for (int j = 2; j <= nz - 3; j++) {
-        const double Bn = (j + 3 < nz) ? B[c + (size_t)(j + 3) * s] : 0.0;
-        fw.step(j, c + (size_t)j * s, Bn, z.cf, z.tr, W);
-    }
-    fw.finish(B, c, s, nz, z.ibc, 0.0, z.cf, z.tr, W);
-    // forcing of this column (physics.f90:678-737 for the adjoint)
-    double forcing = 0.0;
-    if (EPI == ZPZ && z.time <= z.src.limit) {
-        if (z.src.mode == 1)
-            forcing = (z.time >= 0 && z.time < z.src.nfwd) ? z.src.vr[c] * z.src.fwd[z.time] : 0.0;
-        else if (z.src.mode == 2)
-            forcing = lagrange_forcing(z.src, 0, c, z.nxy, z.time);
-    }
-    double d = fw.ud;
-    for (int j = nz - 1; j >= 0; j--) {
-        const size_t q = c + (size_t)j * s;
-        if (j < nz - 1) d = W[q] - z.tr.gam[j + 1] * d;
-        double dz = d * z.stretch[j];
-        const bool edge = j == 0 || j == nz - 1;
-        if (EPI == ZP || EPI == ZPZ) {
-            // z-PML (displacement.f90:400-409, 422-431)
+    FwdSweep fw[1];
+    const double *const Bs[1] = {z.B};
+    double *const Ws[1] = {z.W};
+    const int ibc[1] = {1};
+    const Tri *const trs[1] = {&z.tr1};
+    sweep_fwd<1, UF>(fw, Bs, Ws, ibc, trs, c, s, nz, z.cf);
+    double d = fw[0].ud;
+    for (int j0 = nz - 1; j0 >= 0; j0 -= UB) {
+        double w[UB], dxi[UB], dyi[UB], xiz[UB], c2r[UB], gp[UB], gr[UB];
+#pragma unroll
+        for (int u = 0; u < UB; u++) {
+            const size_t q = c + (size_t)max(j0 - u, 0) * s;
+            w[u] = z.W[q];
+            dxi[u] = z.dxi[q];
+            dyi[u] = z.dyi[q];
+            xiz[u] = z.B[q];
+            c2r[u] = z.c2rho0[q];
+            gp[u] = z.gp0[q];
+            gr[u] = z.gr0[q];
+        }
+#pragma unroll
+        for (int u = 0; u < UB; u++) {
+            const int j = j0 - u;
+            if (j < 0) break;
+            const size_t q = c + (size_t)j * s;
+            if (j < nz - 1) d = w[u] - z.tr1.gam[j + 1] * d;
+            double dz = d * z.stretch[j];
             const int pj = pml_index(j, nz, z.nb, z.nt);
             if (pj >= 0) {
                 const size_t qp = c + (size_t)pj * s;
@@ -155,28 +186,96 @@ __global__ __launch_bounds__(256) void k3_zpass(Z3 z)
                 z.rpsi[qp] = z.az[qp] * dz + z.bz[qp] * ps;
                 dz = dz + ps;
             }
-        }
-        if (EPI == ZP) {
-            // div, p, rho (displacement.f90:411-415); rho' g / rho0 for :458
-            const double div = (z.dxi[q] + z.dyi[q]) + dz;
-            const double xiz = B[q];
-            z.P[q] = -(z.c2rho0[q] * div) - xiz * z.gp0[q];
-            const double rho = -(z.rho0[j] * div) - xiz * z.gr0[q];
+            const double div = (dxi[u] + dyi[u]) + dz;
+            z.P[q] = -(c2r[u] * div) - xiz[u] * gp[u];
+            const double rho = -(z.rho0[j] * div) - xiz[u] * gr[u];
             z.RG[q] = z.rhoinv[j] * rho * z.g[j];
-        } else if (EPI == ZV) {
-            // advection and pressure gradient (displacement.f90:436-452), RHS xi (:466-468),
-            // rows 1 and nz zeroed (:470-473); RHS v_z is completed by Zpz
-            const double adv = -2.0 * ((z.v0x[q] * z.dxv[q] + z.v0y[q] * z.dyv[q]) + z.v0z[q] * dz);
-            if (z.comp < 2) z.rv[q] = edge ? 0.0 : adv - z.rhoinv[j] * z.gpi[q];
-            else z.rv[q] = adv;
-            z.rxi[q] = edge ? 0.0 : B[q];
-        } else {
-            // RHS v_z: - d p/dz / rho0 (:452), - rho' g / rho0 (:458), source (:460-462)
-            double r = z.rv[q] - z.rhoinv[j] * dz;
-            r = r - z.RG[q];
+        }
+    }
+}
+
+// Zv: the z-derivatives of v_x, v_y, v_z (ibc 1) and of p (ibc 0, with the
+// z-PML, :422-431) as four chains of one lane, then RHS v_i = -2 v0 . grad v_i
+// - d_i p / rho0 (:436-452), RHS v_z - rho' g / rho0 (:458) + forcing *
+// source_dep (:460-462), RHS xi_i = v_i (:466-468), rows 1 and nz zeroed
+// (:470-473).
+__global__ __launch_bounds__(256) void k3_zv(Z3 z)
+{
+    constexpr int UF = 8, UB = 2;
+    const int c = blockIdx.x * 256 + threadIdx.x;
+    if (c >= z.nxy) return;
+    const size_t s = (size_t)z.nxy, N = z.N;
+    const int nz = z.nz;
+    FwdSweep fw[4];
+    const double *const Bs[4] = {z.B, z.B + N, z.B + 2 * N, z.P};
+    double *const Ws[4] = {z.W, z.W + N, z.W + 2 * N, z.W + 3 * N};
+    const int ibc[4] = {1, 1, 1, 0};
+    const Tri *const trs[4] = {&z.tr1, &z.tr1, &z.tr1, &z.tr0};
+    sweep_fwd<4, UF>(fw, Bs, Ws, ibc, trs, c, s, nz, z.cf);
+    // forcing of this column (physics.f90:678-737 for the adjoint)
+    double forcing = 0.0;
+    if (z.time <= z.src.limit) {
+        if (z.src.mode == 1)
+            forcing = (z.time >= 0 && z.time < z.src.nfwd) ? z.src.vr[c] * z.src.fwd[z.time] : 0.0;
+        else if (z.src.mode == 2)
+            forcing = lagrange_forcing(z.src, 0, c, z.nxy, z.time);
+    }
+    double d[4] = {fw[0].ud, fw[1].ud, fw[2].ud, fw[3].ud};
+    double *const r = z.rhs;
+    for (int j0 = nz - 1; j0 >= 0; j0 -= UB) {
+        double w[4][UB], dx[3][UB], dy[3][UB], v[3][UB], u0[3][UB], dxp[UB], dyp[UB], rg[UB];
+#pragma unroll
+        for (int u = 0; u < UB; u++) {
+            const size_t q = c + (size_t)max(j0 - u, 0) * s;
+#pragma unroll
+            for (int k = 0; k < 4; k++) w[k][u] = z.W[q + k * N];
+#pragma unroll
+            for (int i = 0; i < 3; i++) {
+                dx[i][u] = z.dxv[q + i * N];
+                dy[i][u] = z.dyv[q + i * N];
+                v[i][u] = z.B[q + i * N];
+            }
+            u0[0][u] = z.v0x[q];
+            u0[1][u] = z.v0y[q];
+            u0[2][u] = z.v0z[q];
+            dxp[u] = z.dxp[q];
+            dyp[u] = z.dyp[q];
+            rg[u] = z.RG[q];
+        }
+#pragma unroll
+        for (int u = 0; u < UB; u++) {
+            const int j = j0 - u;
+            if (j < 0) break;
+            const size_t q = c + (size_t)j * s;
+            if (j < nz - 1) {
+#pragma unroll
+                for (int k = 0; k < 3; k++) d[k] = w[k][u] - z.tr1.gam[j + 1] * d[k];
+                d[3] = w[3][u] - z.tr0.gam[j + 1] * d[3];
+            }
+            const double st = z.stretch[j];
+            double gpz = d[3] * st;
+            const int pj = pml_index(j, nz, z.nb, z.nt);
+            if (pj >= 0) {
+                const size_t qa = c + (size_t)pj * s, qp = qa + (size_t)(z.nb + z.nt) * s;
+                const double ps = z.psi[qp];
+                z.rpsi[qp] = z.az[qa] * gpz + z.bz[qa] * ps;
+                gpz = gpz + ps;
+            }
+            const bool edge = j == 0 || j == nz - 1;
+            const double ri = z.rhoinv[j];
+            double adv[3];
+#pragma unroll
+            for (int i = 0; i < 3; i++)
+                adv[i] = -2.0 * ((u0[0][u] * dx[i][u] + u0[1][u] * dy[i][u]) + u0[2][u] * (d[i] * st));
+            r[q + 3 * N] = edge ? 0.0 : adv[0] - ri * dxp[u];
+            r[q + 4 * N] = edge ? 0.0 : adv[1] - ri * dyp[u];
+            double rz = adv[2] - ri * gpz;
+            rz = rz - rg[u];
             const int dk = j - z.e_rad;
-            if (dk >= -4 && dk <= 4) r = r + forcing * z.srcdep[j];
-            z.rv[q] = edge ? 0.0 : r;
+            if (dk >= -4 && dk <= 4) rz = rz + forcing * z.srcdep[j];
+            r[q + 5 * N] = edge ? 0.0 : rz;
+#pragma unroll
+            for (int i = 0; i < 3; i++) r[q + i * N] = edge ? 0.0 : v[i][u];
         }
     }
 }
@@ -262,12 +361,15 @@ static hipError_t launch_ddy(const double *in, double *out, int nplanes, int nx,
     return hipGetLastError();
 }
 
-static hipError_t launch_zpass(int epi, const Z3 &z, hipStream_t st)
+static hipError_t launch_zp(const Z3 &z, hipStream_t st)
 {
-    const dim3 grid((z.nxy + 255) / 256);
-    if (epi == ZP) hipLaunchKernelGGL(k3_zpass<ZP>, grid, dim3(256), 0, st, z);
-    else if (epi == ZV) hipLaunchKernelGGL(k3_zpass<ZV>, grid, dim3(256), 0, st, z);
-    else hipLaunchKernelGGL(k3_zpass<ZPZ>, grid, dim3(256), 0, st, z);
+    hipLaunchKernelGGL(k3_zp, dim3((z.nxy + 255) / 256), dim3(256), 0, st, z);
+    return hipGetLastError();
+}
+
+static hipError_t launch_zv(const Z3 &z, hipStream_t st)
+{
+    hipLaunchKernelGGL(k3_zv, dim3((z.nxy + 255) / 256), dim3(256), 0, st, z);
     return hipGetLastError();
 }
 
@@ -379,19 +481,20 @@ static int ddy3(sparc3d_handle *h, const double *in, double *out, int nplanes)
     return SPARC_OK;
 }
 
-static Z3 zargs(sparc3d_handle *h, const double *B, int ibc)
+static Z3 zargs(sparc3d_handle *h, const double *B)
 {
     Z3 z{};
     z.B = B;
     z.W = h->W;
+    z.N = h->n;
     z.nxy = (int)h->nxy;
     z.nz = h->nz;
     z.nb = h->nb;
     z.nt = h->nt;
-    z.ibc = ibc;
     z.stretch = h->d_stretch;
     z.cf = h->cf;
-    z.tr = h->tri[ibc];
+    z.tr1 = h->tri[1];
+    z.tr0 = h->tri[0];
     z.rhoinv = h->d_rhoinv;
     z.az = h->az;
     z.bz = h->bz;
@@ -411,13 +514,13 @@ static int enqueue_stage(sparc3d_handle *h, int s, int t)
         PS = h->psit;
     }
     int r;
-    // ddxyz(xi_x, xi_y, xi_z) and the first derivatives of the velocity
+    // ddxyz(xi_x, xi_y, xi_z) and the x, y derivatives of the velocity
     if ((r = ddx3(h, T, h->DXI, nyz))) return r;
     if ((r = ddx3(h, T + 3 * N, h->DX3, 3 * nyz))) return r;
     if ((r = ddy3(h, T + N, h->DYI, nz))) return r;
     if ((r = ddy3(h, T + 3 * N, h->DY3, 3 * nz))) return r;
     {
-        Z3 z = zargs(h, T + 2 * N, 1);
+        Z3 z = zargs(h, T + 2 * N);
         z.psi = PS;
         z.rpsi = h->scrpml;
         z.dxi = h->DXI;
@@ -429,34 +532,29 @@ static int enqueue_stage(sparc3d_handle *h, int s, int t)
         z.g = h->d_g;
         z.P = h->P;
         z.RG = h->RG;
-        RUN3(3, launch_zpass(ZP, z, h->stream));
+        RUN3(3, launch_zp(z, h->stream));
     }
     if ((r = ddx3(h, h->P, h->DXP, nyz))) return r;
     if ((r = ddy3(h, h->P, h->DYP, nz))) return r;
-    for (int i = 0; i < 3; i++) {
-        Z3 z = zargs(h, T + (3 + i) * N, 1);
-        z.dxv = h->DX3 + i * N;
-        z.dyv = h->DY3 + i * N;
+    {
+        Z3 z = zargs(h, T + 3 * N);
+        z.P = h->P;
+        z.RG = h->RG;
+        z.psi = PS;
+        z.rpsi = h->scrpml;
+        z.dxv = h->DX3;
+        z.dyv = h->DY3;
         z.v0x = h->v0x;
         z.v0y = h->v0y;
         z.v0z = h->v0z;
-        z.gpi = i == 0 ? h->DXP : (i == 1 ? h->DYP : nullptr);
-        z.rv = h->scr + (3 + i) * N;
-        z.rxi = h->scr + i * N;
-        z.comp = i;
-        RUN3(5, launch_zpass(ZV, z, h->stream));
-    }
-    {
-        Z3 z = zargs(h, h->P, 0);
-        z.psi = PS + h->np;
-        z.rpsi = h->scrpml + h->np;
-        z.RG = h->RG;
-        z.rv = h->scr + 5 * N;
+        z.dxp = h->DXP;
+        z.dyp = h->DYP;
+        z.rhs = h->scr;
         z.srcdep = h->d_srcdep;
         z.src = h->src;
         z.time = t;
         z.e_rad = h->e_rad;
-        RUN3(4, launch_zpass(ZPZ, z, h->stream));
+        RUN3(4, launch_zv(z, h->stream));
     }
     return SPARC_OK;
 }
@@ -468,7 +566,7 @@ static int enqueue_step3(sparc3d_handle *h, int t)
         if (r) return r;
     }
     const int filt = (t + 1) % 2 == 0;                           // step.f90:114
-    RUN3(6, launch_update(h->a, h->scr, (int)h->nxy, h->nz, h->nb, h->nt, h->deltat, filt, h->stream));
+    RUN3(5, launch_update(h->a, h->scr, (int)h->nxy, h->nz, h->nb, h->nt, h->deltat, filt, h->stream));
     RUN3(0, launch_axpy(h->pml, h->pml, h->scrpml, h->deltat, 2 * h->np, h->stream));
     return SPARC_OK;
 }
@@ -670,7 +768,7 @@ static int setup3(sparc3d_handle *h)
     HCHECK3(h, m.alloc(&h->RG, N));
     HCHECK3(h, m.alloc(&h->DXP, N));
     HCHECK3(h, m.alloc(&h->DYP, N));
-    HCHECK3(h, m.alloc(&h->W, N));
+    HCHECK3(h, m.alloc(&h->W, 4 * N));   // one block per chain of the Zv pass
     HCHECK3(h, hipMemsetAsync(h->a, 0, 6 * N * sizeof(double), h->stream));
     HCHECK3(h, hipMemsetAsync(h->scr, 0, 6 * N * sizeof(double), h->stream));
     HCHECK3(h, hipMemsetAsync(h->pml, 0, 2 * h->np * sizeof(double), h->stream));
