@@ -21,6 +21,8 @@
 #ifndef SPARC_HIP_H
 #define SPARC_HIP_H
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif

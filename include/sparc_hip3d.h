@@ -82,6 +82,33 @@ int sparc3d_step(sparc3d_handle *h, int t);
  * vz_out [nout][ny][nx] (nout = maxtime/steps + 1; may be NULL). */
 int sparc3d_run(sparc3d_handle *h, int maxtime, double *vz_out);
 
+/* Time indices t0..t1 from the current state (a restart from a checkpoint,
+ * driver.f90:236-237); the vz output of time t (t % steps == 0) goes to slot
+ * t/steps - k0 of device vz_out [nslots][ny][nx] when it exists. */
+int sparc3d_run_range(sparc3d_handle *h, int t0, int t1, double *vz_out, int k0, int nslots);
+/* Full solver state (a and the z-PML variables) to / from a device buffer of
+ * sparc3d_checkpoint_size bytes: in-HBM checkpoints instead of the
+ * reference's dumped full state. */
+int sparc3d_checkpoint_size(const sparc3d_handle *h, size_t *bytes);
+int sparc3d_save_checkpoint(sparc3d_handle *h, void *dev);
+int sparc3d_load_checkpoint(sparc3d_handle *h, const void *dev);
+/* The partial snapshot of the current time (WRITE_OUT_PARTIAL_STATE,
+ * all_modules.f90:750-784) to device memory: forward [9][nz][ny][nx] xi_x,
+ * xi_y, xi_z, v_x, v_y, v_z and the last stage's RHS of v_x, v_y, v_z
+ * (acc); adjoint [3][nz][ny][nx] v_x, v_y, v_z. */
+int sparc3d_get_snapshot(sparc3d_handle *h, double *dev);
+
+/* PRODUCE_KERNELS with its y terms (kernels.f90:13-612 with test_in_2d
+ * false): begin zeroes the sums; add accumulates one forward snapshot (9
+ * fields, device) with the adjoint snapshot (3 fields, device) it pairs with
+ * (the caller pairs forward output k with adjoint output nt-1-k, :215, :256);
+ * end writes Kc, Kvx, Kvy, Kvz and the hessian, [nz][ny][nx] device, in the
+ * reference's units (:498-522, 563, 579) on kernel_back's background
+ * (initialize.f90:937-979).  Uses the handle's 3D sound speed for c2. */
+int sparc3d_kernels_begin(sparc3d_handle *h);
+int sparc3d_kernels_add(sparc3d_handle *h, const double *fwd, const double *adj);
+int sparc3d_kernels_end(sparc3d_handle *h, double *kc, double *kvx, double *kvy, double *kvz, double *hess);
+
 /* State a [6][nz][ny][nx] (xi_x, xi_y, xi_z, v_x, v_y, v_z), device. */
 int sparc3d_get_state(sparc3d_handle *h, double *a_dev);
 int sparc3d_set_state(sparc3d_handle *h, const double *a_dev);

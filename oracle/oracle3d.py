@@ -46,6 +46,7 @@ import struct
 import numpy as np
 
 from . import dbyd2 as _dbyd2
+from . import filter_var_z as _filter_var_z
 from . import forward_source_series
 
 RSUN = 6.95989467700e10      # initialize.f90:37
@@ -330,3 +331,59 @@ class Oracle3D:
             if t % self.p.steps == 0:
                 out.append(self.a[5, self.o_rad].copy())
         return np.array(out)
+
+
+def kernels3d(p: Params3D, model: np.ndarray, fwd: np.ndarray, adj: np.ndarray, c3d=None) -> dict:
+    """PRODUCE_KERNELS with its y terms (kernels.f90:13-612 with test_in_2d
+    false, the 3D branches of :289-355, :498-522, :563, :579) on in-memory
+    snapshots.  fwd [nt][9][nz][ny][nx]: xi_x, xi_y, xi_z, v_x, v_y, v_z,
+    acc_x, acc_y, acc_z; adj [nt][3][nz][ny][nx]: v_x, v_y, v_z.  Forward
+    snapshot k meets adjoint snapshot nt-1-k.  The background is kernel_back's
+    (initialize.f90:937-979): rho0 = rho / 10.0**(-3.0), c2 = (c / 10.0**6.)**2.0,
+    with c from c3d when given (model_c, kernels.f90:35-42).  ddzkern is dbyd2
+    (ibc 1) times stretch followed by FILTER_VAR_Z (:648-661)."""
+    nx, ny, nz = p.nx, p.ny, p.nz
+    m = np.asarray(model, dtype=np.float64)[:nz]
+    dimc, dimrho, diml = f32(1e6), f32(1e-3), RSUN
+    z = m[:, 0]
+    stretch = 1.0 / _dbyd2(z, 1)[:, 0]
+    c = (m[:, 1][:, None, None] * np.ones((1, ny, nx))) if c3d is None else np.asarray(c3d).reshape(nz, ny, nx)
+    c2 = (c / dimc) ** 2.0
+    rho0 = (m[:, 2] / dimrho)[:, None, None]
+    sx, sy = RSUN / p.xlength, RSUN / p.ylength
+
+    def ddzk(f):
+        d = _dbyd2(f.reshape(nz, -1), 1) * stretch[:, None]
+        return _filter_var_z(d).reshape(f.shape)
+
+    nt = fwd.shape[0]
+    H = np.zeros((nz, ny, nx))
+    KC = np.zeros_like(H)
+    KV = np.zeros((3, nz, ny, nx))
+    for k in range(nt):
+        f = fwd[k]
+        a = adj[nt - 1 - k]
+        H = H + f[8] * f[8] + f[6] * f[6] + f[7] * f[7]                  # :293-294
+        divf = _deriv_fft(f[0], -1, sx)                                    # compute_div :843-861
+        if ny > 4:
+            divf = divf + _deriv_fft(f[1], -2, sy)
+        divf = divf + ddzk(f[2])
+        diva = _deriv_fft(a[0], -1, sx)
+        if ny > 4:
+            diva = diva + _deriv_fft(a[1], -2, sy)
+        diva = diva + ddzk(a[2])
+        KC = KC + divf * diva                                              # :303-306
+        d = [[_deriv_fft(f[3 + j], -1, sx) for j in range(3)],            # compute_del :866-887
+             [_deriv_fft(f[3 + j], -2, sy) for j in range(3)],
+             [ddzk(f[3 + j]) for j in range(3)]]
+        for i in range(3):                                                 # :338-355
+            KV[i] = d[i][0] * a[0] + d[i][1] * a[1] + d[i][2] * a[2] + KV[i]
+    steps = int(math.floor(p.outputcad / p.timestep))
+    e8, e5 = f32(1e-8), f32(1e-5)
+    dimen = (diml * e8) ** (-3.0) * (dimc * e5) ** (-1.0)                  # :503
+    out = {"hess": H}
+    for i, name in enumerate(("kvx", "kvy", "kvz")):                       # :520-522
+        out[name] = -2.0 * KV[i] * rho0 * steps * p.timestep * dimen
+    dimen_c = (diml * e8) ** (-3.0)                                        # :563
+    out["kc"] = -KC * rho0 * c2 * steps * p.timestep * dimen_c * 2.0       # :579
+    return out

@@ -21,6 +21,7 @@
 // chains) so that each lane keeps tens of loads in flight.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -325,6 +326,47 @@ __global__ __launch_bounds__(256) void k3_update(double *a, const double *scr, i
     for (int j = nz - nt + 5; j < nz; j++) A[j * s] = A[j * s] + dt * S[j * s];
 }
 
+// One snapshot pair of PRODUCE_KERNELS with its y terms (kernels.f90:
+// 289-355): DX, DY, DZ hold the x, y and z (ddzkern) derivatives of forward
+// xi_x / xi_y / xi_z (slot 0), forward v_x, v_y, v_z (slots 1-3) and adjoint
+// v_x / v_y / v_z (slot 4); the sums keep the reference's order of additions.
+__global__ __launch_bounds__(256) void k3_kern_accum(const double *__restrict__ f, const double *__restrict__ a,
+                                                     const double *__restrict__ DX, const double *__restrict__ DY,
+                                                     const double *__restrict__ DZ, double *K, size_t N)
+{
+    const size_t stride = (size_t)gridDim.x * 256;
+    for (size_t q = (size_t)blockIdx.x * 256 + threadIdx.x; q < N; q += stride) {
+        const double fax = f[q + 6 * N], fay = f[q + 7 * N], faz = f[q + 8 * N];
+        K[q + 4 * N] = K[q + 4 * N] + faz * faz + fax * fax + fay * fay;           // hessian :293-294
+        const double divf = (DX[q] + DY[q]) + DZ[q];                             // compute_div :843-861
+        const double diva = (DX[q + 4 * N] + DY[q + 4 * N]) + DZ[q + 4 * N];
+        K[q] = K[q] + divf * diva;                                               // :303-306
+        const double avx = a[q], avy = a[q + N], avz = a[q + 2 * N];
+        const double *D[3] = {DX, DY, DZ};
+#pragma unroll
+        for (int i = 0; i < 3; i++)                                              // :338-355
+            K[q + (1 + i) * N] = D[i][q + N] * avx + D[i][q + 2 * N] * avy + D[i][q + 3 * N] * avz +
+                                 K[q + (1 + i) * N];
+    }
+}
+
+// Units (kernels.f90:498-522, 563, 579): K = [Kc, Kvx, Kvy, Kvz, H].
+__global__ __launch_bounds__(256) void k3_kern_final(const double *__restrict__ K, const double *__restrict__ rho0z,
+                                                     const double *__restrict__ c2, size_t N, size_t nxy,
+                                                     double steps, double dt, double dimen, double dimen_c,
+                                                     double *kc, double *kvx, double *kvy, double *kvz, double *hess)
+{
+    const size_t stride = (size_t)gridDim.x * 256;
+    for (size_t q = (size_t)blockIdx.x * 256 + threadIdx.x; q < N; q += stride) {
+        const double r = rho0z[q / nxy];
+        kvx[q] = -2.0 * K[q + N] * r * steps * dt * dimen;
+        kvy[q] = -2.0 * K[q + 2 * N] * r * steps * dt * dimen;
+        kvz[q] = -2.0 * K[q + 3 * N] * r * steps * dt * dimen;
+        kc[q] = -K[q] * r * c2[q] * steps * dt * dimen_c * 2.0;
+        hess[q] = K[q + 4 * N];
+    }
+}
+
 #define SPARC3D_DISPATCH(N, MACRO) \
     switch (N) {                   \
     case 16: MACRO(16); break;     \
@@ -418,6 +460,11 @@ struct sparc3d_handle {
            *DXP = nullptr, *DYP = nullptr, *W = nullptr;
     Src src{};
     bool have_src = false;
+    // kernel accumulation: K [5][N] (Kc, Kvx, Kvy, Kvz, H), derivatives D [3][5][N],
+    // kernel_back's rho0 [nz] and c2 [N] (initialize.f90:937-979)
+    DevArena kern_mem;
+    double *K = nullptr, *KD = nullptr, *krho0 = nullptr, *kc2 = nullptr;
+    long kern_pairs = 0;
     // instrumentation
     struct Mark { hipEvent_t a, b; int kind; };
     std::vector<hipEvent_t> evpool;
@@ -869,6 +916,7 @@ int sparc3d_destroy(sparc3d_handle *h)
     h->bg_mem.release();
     h->st_mem.release();
     h->src_mem.release();
+    h->kern_mem.release();
     if (h->own_stream && h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
     return SPARC_OK;
@@ -1007,22 +1055,139 @@ int sparc3d_step(sparc3d_handle *h, int t)
     return SPARC_OK;
 }
 
+int sparc3d_run_range(sparc3d_handle *h, int t0, int t1, double *vz_out, int k0, int nslots)
+{
+    if (!h || t0 < 0 || t1 < t0 || (vz_out && nslots < 1)) return SPARC_E_ARG;
+    if (!h->have_src) return fail3(h, SPARC_E_STATE, "no source set");
+    const size_t plane = h->nxy;
+    for (int t = t0; t <= t1; t++) {
+        int r = enqueue_step3(h, t);
+        if (r) return r;
+        if (vz_out && t % h->steps == 0) {                      // driver.f90:388-419
+            const int k = t / h->steps - k0;
+            if (k >= 0 && k < nslots)
+                HCHECK3(h, hipMemcpyAsync(vz_out + (size_t)k * plane, h->a + 5 * h->n + (size_t)h->o_rad * plane,
+                                          plane * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+        }
+    }
+    HCHECK3(h, hipStreamSynchronize(h->stream));
+    return SPARC_OK;
+}
+
 int sparc3d_run(sparc3d_handle *h, int maxtime, double *vz_out)
 {
     if (!h || maxtime < 0) return SPARC_E_ARG;
-    if (!h->have_src) return fail3(h, SPARC_E_STATE, "no source set");
-    const size_t plane = h->nxy;
-    int nout = 0;
-    for (int t = 0; t <= maxtime; t++) {
-        int r = enqueue_step3(h, t);
-        if (r) return r;
-        if (t % h->steps == 0) {                                 // driver.f90:388-419
-            if (vz_out)
-                HCHECK3(h, hipMemcpyAsync(vz_out + (size_t)nout * plane, h->a + 5 * h->n + (size_t)h->o_rad * plane,
-                                          plane * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
-            nout++;
-        }
+    return sparc3d_run_range(h, 0, maxtime, vz_out, 0, maxtime / h->steps + 1);
+}
+
+int sparc3d_checkpoint_size(const sparc3d_handle *h, size_t *bytes)
+{
+    if (!h || !bytes) return SPARC_E_ARG;
+    *bytes = (6 * h->n + 2 * h->np) * sizeof(double);
+    return SPARC_OK;
+}
+
+int sparc3d_save_checkpoint(sparc3d_handle *h, void *dev)
+{
+    if (!h || !dev) return SPARC_E_ARG;
+    double *d = (double *)dev;
+    HCHECK3(h, hipMemcpyAsync(d, h->a, 6 * h->n * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+    HCHECK3(h, hipMemcpyAsync(d + 6 * h->n, h->pml, 2 * h->np * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+    return SPARC_OK;
+}
+
+int sparc3d_load_checkpoint(sparc3d_handle *h, const void *dev)
+{
+    if (!h || !dev) return SPARC_E_ARG;
+    const double *d = (const double *)dev;
+    HCHECK3(h, hipMemcpyAsync(h->a, d, 6 * h->n * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+    HCHECK3(h, hipMemcpyAsync(h->pml, d + 6 * h->n, 2 * h->np * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+    return SPARC_OK;
+}
+
+int sparc3d_get_snapshot(sparc3d_handle *h, double *dev)
+{
+    if (!h || !dev) return SPARC_E_ARG;
+    const size_t N = h->n;
+    if (!h->adjoint) {
+        HCHECK3(h, hipMemcpyAsync(dev, h->a, 6 * N * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+        HCHECK3(h, hipMemcpyAsync(dev + 6 * N, h->scr + 3 * N, 3 * N * sizeof(double), hipMemcpyDeviceToDevice,
+                                  h->stream));
+    } else {
+        HCHECK3(h, hipMemcpyAsync(dev, h->a + 3 * N, 3 * N * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
     }
+    return SPARC_OK;
+}
+
+int sparc3d_kernels_begin(sparc3d_handle *h)
+{
+    if (!h) return SPARC_E_ARG;
+    const size_t N = h->n, nxy = h->nxy;
+    HCHECK3(h, hipStreamSynchronize(h->stream));
+    if (!h->K) {
+        HCHECK3(h, h->kern_mem.alloc(&h->K, 5 * N));
+        HCHECK3(h, h->kern_mem.alloc(&h->KD, 15 * N));
+        // kernel_back (initialize.f90:937-979): 10.0**6., 10.0**(-3.0) as REAL
+        const double dimc = (double)1e6f, dimrho = (double)1e-3f;
+        std::vector<double> r(h->nz), c2(N);
+        for (int k = 0; k < h->nz; k++) {
+            const double *row = &h->model[6 * (size_t)k];
+            r[k] = row[2] / dimrho;
+            for (size_t q = 0; q < nxy; q++) {
+                const double c = (h->c3d.empty() ? row[1] : h->c3d[(size_t)k * nxy + q]) / dimc;
+                c2[(size_t)k * nxy + q] = std::pow(c, 2.0);
+            }
+        }
+        HCHECK3(h, upload(h->kern_mem, &h->krho0, r, h->stream));
+        HCHECK3(h, upload(h->kern_mem, &h->kc2, c2, h->stream));
+    }
+    HCHECK3(h, hipMemsetAsync(h->K, 0, 5 * N * sizeof(double), h->stream));
+    HCHECK3(h, hipStreamSynchronize(h->stream));
+    h->kern_pairs = 0;
+    return SPARC_OK;
+}
+
+int sparc3d_kernels_add(sparc3d_handle *h, const double *fwd, const double *adj)
+{
+    if (!h || !fwd || !adj) return SPARC_E_ARG;
+    if (!h->K) return fail3(h, SPARC_E_STATE, "sparc3d_kernels_add before sparc3d_kernels_begin");
+    const size_t N = h->n;
+    const int nyz = h->ny * h->nz, nz = h->nz;
+    double *DX = h->KD, *DY = DX + 5 * N, *DZ = DY + 5 * N;
+    int r;
+    // ddxkern, ddykern (:665-691 and the y analogue) of forward xi_x / xi_y,
+    // forward v and adjoint v_x / v_y
+    if ((r = ddx3(h, fwd, DX, nyz))) return r;
+    if ((r = ddx3(h, fwd + 3 * N, DX + N, 3 * nyz))) return r;
+    if ((r = ddx3(h, adj, DX + 4 * N, nyz))) return r;
+    if ((r = ddy3(h, fwd + N, DY, nz))) return r;
+    if ((r = ddy3(h, fwd + 3 * N, DY + N, 3 * nz))) return r;
+    if ((r = ddy3(h, adj + N, DY + 4 * N, nz))) return r;
+    // ddzkern (:648-661): dbyd2 ibc 1, stretch, FILTER_VAR_Z
+    const double *zin[5] = {fwd + 2 * N, fwd + 3 * N, fwd + 4 * N, fwd + 5 * N, adj + 2 * N};
+    for (int i = 0; i < 5; i++)
+        HCHECK3(h, launch_ddz_cols(zin[i], DZ + i * N, (int)h->nxy, nz, 1, 1, h->d_stretch, h->cf, h->tri[1], h->W,
+                                   1, h->stream));
+    const unsigned blocks = (unsigned)std::min<size_t>((N + 255) / 256, 16384);
+    hipLaunchKernelGGL(k3_kern_accum, dim3(blocks), dim3(256), 0, h->stream, fwd, adj, DX, DY, DZ, h->K, N);
+    HCHECK3(h, hipGetLastError());
+    h->kern_pairs++;
+    return SPARC_OK;
+}
+
+int sparc3d_kernels_end(sparc3d_handle *h, double *kc, double *kvx, double *kvy, double *kvz, double *hess)
+{
+    if (!h || !kc || !kvx || !kvy || !kvz || !hess) return SPARC_E_ARG;
+    if (!h->K) return fail3(h, SPARC_E_STATE, "sparc3d_kernels_end before sparc3d_kernels_begin");
+    const double diml = kRsun3, dimc = (double)1e6f, e8 = (double)1e-8f, e5 = (double)1e-5f;
+    const double dimen = std::pow(diml * e8, -3.0) * std::pow(dimc * e5, -1.0);   // kernels.f90:503
+    const double dimen_c = std::pow(diml * e8, -3.0);                             // :563
+    const int stepskern = (int)std::floor(h->outputcad / h->timestep);
+    const size_t N = h->n;
+    const unsigned blocks = (unsigned)std::min<size_t>((N + 255) / 256, 16384);
+    hipLaunchKernelGGL(k3_kern_final, dim3(blocks), dim3(256), 0, h->stream, h->K, h->krho0, h->kc2, N, h->nxy,
+                       (double)stepskern, h->timestep, dimen, dimen_c, kc, kvx, kvy, kvz, hess);
+    HCHECK3(h, hipGetLastError());
     HCHECK3(h, hipStreamSynchronize(h->stream));
     return SPARC_OK;
 }

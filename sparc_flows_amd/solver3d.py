@@ -87,6 +87,14 @@ def lib() -> ctypes.CDLL:
             "sparc3d_reset": [vp],
             "sparc3d_step": [vp, ip],
             "sparc3d_run": [vp, ip, vp],
+            "sparc3d_run_range": [vp, ip, ip, vp, ip, ip],
+            "sparc3d_checkpoint_size": [vp, ctypes.POINTER(ctypes.c_size_t)],
+            "sparc3d_save_checkpoint": [vp, vp],
+            "sparc3d_load_checkpoint": [vp, vp],
+            "sparc3d_get_snapshot": [vp, vp],
+            "sparc3d_kernels_begin": [vp],
+            "sparc3d_kernels_add": [vp, vp, vp],
+            "sparc3d_kernels_end": [vp, vp, vp, vp, vp, vp],
             "sparc3d_get_state": [vp, vp],
             "sparc3d_set_state": [vp, vp],
             "sparc3d_get_pml": [vp, vp],
@@ -220,6 +228,54 @@ class Sparc3D:
         self.reset()
         return self.run(maxtime)
 
+    def run_range(self, t0: int, t1: int, vz_out: torch.Tensor | None = None, k0: int = 0) -> None:
+        """Steps t0..t1 from the current state; vz of time t goes to slot t/steps - k0."""
+        n = 0 if vz_out is None else int(vz_out.shape[0])
+        self._call("sparc3d_run_range", int(t0), int(t1), _ptr(vz_out), int(k0), n)
+
+    # ---- checkpoints and snapshots ----
+    def checkpoint_size(self) -> int:
+        b = ctypes.c_size_t()
+        self._call("sparc3d_checkpoint_size", ctypes.byref(b))
+        return b.value
+
+    def save_checkpoint(self, buf: torch.Tensor | None = None) -> torch.Tensor:
+        if buf is None:
+            buf = torch.empty(self.checkpoint_size() // 8, dtype=torch.float64, device=self.device)
+        self._call("sparc3d_save_checkpoint", _ptr(buf))
+        return buf
+
+    def load_checkpoint(self, buf: torch.Tensor) -> None:
+        self._call("sparc3d_load_checkpoint", _ptr(buf))
+
+    def snapshot(self, out: torch.Tensor | None = None) -> torch.Tensor:
+        """Forward: [9][nz][ny][nx] (xi, v, acc); adjoint: [3][nz][ny][nx] (v)."""
+        nf = 3 if self.adjoint else 9
+        if out is None:
+            out = torch.empty((nf, self.p.nz, self.p.ny, self.p.nx), dtype=torch.float64, device=self.device)
+        self._call("sparc3d_get_snapshot", _ptr(out))
+        return out
+
+    # ---- kernels (PRODUCE_KERNELS with the y terms) ----
+    def kernels_begin(self) -> None:
+        self._call("sparc3d_kernels_begin")
+
+    def kernels_add(self, fwd_snap: torch.Tensor, adj_snap: torch.Tensor) -> None:
+        self._call("sparc3d_kernels_add", _ptr(fwd_snap.contiguous()), _ptr(adj_snap.contiguous()))
+
+    def kernels_end(self) -> dict:
+        out = {k: self._zeros(self.p.nz, self.p.ny, self.p.nx) for k in ("kc", "kvx", "kvy", "kvz", "hess")}
+        self._call("sparc3d_kernels_end", *[_ptr(out[k]) for k in ("kc", "kvx", "kvy", "kvz", "hess")])
+        return out
+
+    def kernels(self, fwd_snaps: torch.Tensor, adj_snaps: torch.Tensor) -> dict:
+        """All pairs at once: forward snapshot k with adjoint snapshot nt-1-k."""
+        nt = fwd_snaps.shape[0]
+        self.kernels_begin()
+        for k in range(nt):
+            self.kernels_add(fwd_snaps[k], adj_snaps[nt - 1 - k])
+        return self.kernels_end()
+
     # ---- state and operators ----
     def state(self) -> torch.Tensor:
         a = self._zeros(6, self.p.nz, self.p.ny, self.p.nx)
@@ -266,3 +322,85 @@ class Sparc3D:
         self._check(lib().sparc3d_profile(self.h, int(reps), ms), self.h)
         names = ["stage", "ddx", "ddy", "z_p", "z_v", "update", "unused6", "unused7"]
         return {n: ms[i] for i, n in enumerate(names)}
+
+
+def receiver_mask_3d(p: Params3D, xsrc: float, ysrc: float, dmin_mm: float = 5.0, dmax_mm: float = 60.0):
+    """[ny][nx] 1 where the horizontal distance to the source is within [dmin, dmax] Mm."""
+    x = (np.arange(p.nx) / p.nx - 0.5) * p.xlength * 1e-8 - xsrc
+    y = (np.arange(p.ny) / p.ny - 0.5) * p.ylength * 1e-8 - ysrc
+    d = np.sqrt(x[None, :] ** 2 + y[:, None] ** 2)
+    return ((d >= dmin_mm) & (d <= dmax_mm)).astype(np.float64)
+
+
+class Gradient3D:
+    """Forward + misfit + adjoint + kernels of one 3D source with fixed-interval
+    checkpoint-and-recompute: the forward run keeps its state every `interval`
+    steps in HBM; the adjoint run then meets each forward snapshot k (time
+    k*steps) at its own output nt-1-k (kernels.f90:215, 256), recomputing the
+    forward segments last to first and holding one segment's snapshots at a
+    time.  Cost: two forward runs and one adjoint run; memory: the checkpoints
+    plus one segment of snapshots instead of every snapshot of the run.
+
+    The adjoint source is the windowed, time-reversed vz residual over the
+    receivers 5-60 Mm from the source (gradient.waveform_adjoint_source), which
+    needs outputcad == cadforcing."""
+
+    def __init__(self, params: Params3D, model, c3d=None, flow=None, interval: int = 150,
+                 window_s: float = 1800.0):
+        if abs(params.outputcad - params.cadforcing) > 1e-12:
+            raise ValueError("the adjoint source from vz needs outputcad == cadforcing")
+        self.p = params
+        self.interval = int(interval)
+        self.window_s = window_s
+        self.fwd = Sparc3D(params, model, c3d=c3d)
+        self.adj = Sparc3D(params, model, c3d=c3d, adjoint=True)
+        if flow is not None:
+            self.fwd.set_flow(*flow)
+            self.adj.set_flow(*flow)
+
+    def run(self, xsrc: float, ysrc: float, maxtime: int, vz_obs: torch.Tensor | None = None) -> dict:
+        from .gradient import waveform_adjoint_source
+
+        p, I = self.p, self.interval
+        steps = p.steps
+        nt = maxtime // steps + 1
+        fwd, adj = self.fwd, self.adj
+        # forward with checkpoints at t0 = 0, I, 2I, ... (the state before step t0)
+        fwd.set_forward_source(xsrc, ysrc, maxtime)
+        fwd.reset()
+        vz = fwd._zeros(nt, p.ny, p.nx)
+        ckpt = {}
+        for t0 in range(0, maxtime + 1, I):
+            ckpt[t0] = fwd.save_checkpoint()
+            fwd.run_range(t0, min(t0 + I, maxtime + 1) - 1, vz)
+        # adjoint source
+        mask = receiver_mask_3d(p, xsrc, ysrc)[None]
+        misfit, vr = waveform_adjoint_source(vz.reshape(1, nt, -1), None if vz_obs is None else
+                                             vz_obs.reshape(1, nt, -1), mask.reshape(1, -1),
+                                             self.window_s, steps * p.timestep)
+        vr = vr.reshape(nt, p.ny, p.nx)
+        adj.set_adjoint_source(vr, maxtime)
+        adj.reset()
+        adj.kernels_begin()
+        asnap = None
+        t_adj = 0
+        for t0 in sorted(ckpt, reverse=True):
+            t1 = min(t0 + I, maxtime + 1) - 1
+            fwd.load_checkpoint(ckpt[t0])
+            snaps = {}
+            for t in range(t0, t1 + 1):
+                fwd.step(t)
+                if t % steps == 0:
+                    snaps[t // steps] = fwd.snapshot()
+            for k in sorted(snaps, reverse=True):
+                ta = (nt - 1 - k) * steps
+                if ta >= t_adj:
+                    adj.run_range(t_adj, ta)
+                    t_adj = ta + 1
+                asnap = adj.snapshot(asnap)
+                adj.kernels_add(snaps[k], asnap)
+            del snaps
+        K = adj.kernels_end()
+        K["vz"] = vz
+        K["misfit"] = misfit
+        return K

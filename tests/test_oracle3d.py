@@ -151,3 +151,37 @@ def test_short_run_bounded(adjoint):
         o.set_forward_source(0.0, 0.0, 40)
     vz = o.run(40)
     assert np.isfinite(vz).all() and np.abs(vz).max() > 0
+
+
+def test_kernels3d_reduce_to_the_2d_kernels():
+    """y-invariant snapshots with no y components: the 3D kernels are the 2D
+    oracle's (orc_kernels, pinned against the reference's formulas) up to the
+    3D units, dimen3 / dimen2 for the flow kernels and dimen3 / dimen_c for c."""
+    from sparc_flows_amd.params import SparcParams
+    from oracle.oracle3d import kernels3d
+    nx, ny, nz, nt = 16, 8, 40, 4
+    bg = B.polytrope(nz)
+    dt = B.stable_timestep(bg)
+    p3 = Params3D(nx=nx, ny=ny, nz=nz, timestep=dt, xlength=150e8, ylength=80e8)
+    p2 = SparcParams(nx=nx, nz=nz, xlength=150e8, timestep=dt, npmlbot=6, npmltop=6, npmlhor=3)
+    rng = np.random.default_rng(11)
+    f2 = rng.standard_normal((nt, 6, nz, nx))     # xi_x, xi_z, v_x, v_z, acc_x, acc_z
+    a2 = rng.standard_normal((nt, 4, nz, nx))     # v_x, v_z, acc_x, acc_z
+    rep = lambda g: np.repeat(g[:, None, :], ny, axis=1)
+    zero = np.zeros((nz, ny, nx))
+    f3 = np.array([[rep(f[0]), zero, rep(f[1]), rep(f[2]), zero, rep(f[3]), rep(f[4]), zero, rep(f[5])] for f in f2])
+    a3 = np.array([[rep(a[0]), zero, rep(a[1])] for a in a2])
+    k2 = oracle.kernels(p2, bg, f2, a2)
+    k3 = kernels3d(p3, bg, f3, a3)
+    e8, e5 = oracle.oracle3d.f32(1e-8), oracle.oracle3d.f32(1e-5)
+    diml, dimc = oracle.oracle3d.RSUN, oracle.oracle3d.f32(1e6)
+    d3 = (diml * e8) ** -3.0 * (dimc * e5) ** -1.0
+    d2 = (diml * e8 * p2.xlength * e8 * dimc * e5) ** -1.0
+    dc3 = (diml * e8) ** -3.0
+    dc2 = (diml * e8 * p2.xlength * e8) ** -1.0
+    for j in range(ny):
+        assert np.allclose(k3["hess"][:, j], k2["hess"], rtol=1e-13, atol=0)
+        assert np.allclose(k3["kvx"][:, j] * d2 / d3, k2["kvx"], rtol=1e-11, atol=1e-13 * np.abs(k2["kvx"]).max())
+        assert np.allclose(k3["kvz"][:, j] * d2 / d3, k2["kvz"], rtol=1e-11, atol=1e-13 * np.abs(k2["kvz"]).max())
+        assert np.allclose(k3["kc"][:, j] * dc2 / dc3, k2["kc"], rtol=1e-11, atol=1e-13 * np.abs(k2["kc"]).max())
+    assert np.all(k3["kvy"] == 0)
