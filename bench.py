@@ -127,6 +127,32 @@ def forward3d_rate(nsteps=20, warm=2, nx=256, ny=256, nz=300):
                          "algorithmic_bytes_per_point": bpp[dom]}}
 
 
+def gradient3d_rate(maxtime=449, interval=150, nx=256, ny=256, nz=300):
+    """3D forward + adjoint + kernels of one source at 256x256x300 with
+    in-HBM checkpoints every `interval` steps and recompute (two forward runs,
+    one adjoint run, one kernel accumulation per output pair)."""
+    import torch
+    from sparc_flows_amd import background as B
+    from sparc_flows_amd.solver3d import Gradient3D, Params3D
+
+    bg = B.polytrope(nz)
+    dt = B.stable_timestep(bg)
+    p = Params3D(nx=nx, ny=ny, nz=nz, timestep=dt, npmlbot=12, npmltop=12)
+    p.outputcad = p.cadforcing = 30.0
+    c3d = B.sound_speed_bump_3d(nx, ny, p.xlength, p.ylength, bg)
+    flow = B.supergranule_flow_3d(nx, ny, p.xlength, p.ylength, bg)
+    g = Gradient3D(p, bg, c3d=c3d, flow=flow, interval=interval)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    K = g.run(0.0, 0.0, maxtime)
+    torch.cuda.synchronize()
+    sec = time.perf_counter() - t0
+    finite = bool(all(torch.isfinite(K[k]).all() for k in ("kc", "kvx", "kvy", "kvz")))
+    return {"s_per_gradient": sec, "time_steps": maxtime + 1, "checkpoint_interval": interval,
+            "grid": [nx, ny, nz], "output_pairs": maxtime // p.steps + 1, "finite": finite,
+            "Mpoint_steps_per_s_fwd_adj": nx * ny * nz * (maxtime + 1) * 2 / sec / 1e6}
+
+
 def cpu_baseline(p, bg, psi, seconds=12.0):
     """Time the C oracle (one core) on a bounded sample of the same forward run."""
     import oracle
@@ -183,7 +209,8 @@ def main():
         print(json.dumps({"copy_probe_bytes": n * 8, "launches": 10}))
         return
     if args.only_3d:
-        print(json.dumps({"forward3d_256x256x300": forward3d_rate()}))
+        print(json.dumps({"forward3d_256x256x300": forward3d_rate(),
+                          "gradient3d_256x256x300": gradient3d_rate()}))
         return
     p, bg, psi = make_config()
     maxtime = args.nsteps - 1
@@ -263,7 +290,8 @@ def main():
     if not args.no_configs:
         others = {"plain_forward_128x150": forward_rate(128, 150, 1, 500, False, False, reps=3),
                   "batched16_forward_128x150": forward_rate(128, 150, 16, 4000, True, True),
-                  "forward3d_256x256x300": forward3d_rate()}
+                  "forward3d_256x256x300": forward3d_rate(),
+                  "gradient3d_256x256x300": gradient3d_rate()}
         log("forward configurations done")
     cpu = None if args.no_cpu else cpu_baseline(p, bg, psi, args.cpu_seconds)
     log("cpu baseline done")
