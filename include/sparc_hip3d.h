@@ -131,9 +131,9 @@ int sparc3d_query_double(const sparc3d_handle *h, const char *name, double *v);
 
 /* Instrumentation: run `reps` steps with HIP events around every kernel and
  * return the mean milliseconds per launch of each pass kind in ms_out[8]:
- * stage input (a + beta scr), x-derivative, y-derivative, z pass of xi_z
- * (div, p, rho'), z pass of v_x, v_y, v_z and p (the RHS), step update + z
- * filter, and two unused slots. */
+ * (unused), x-derivative, y-derivative, z pass of xi_z (div, p, rho'), z
+ * pass of v_x, v_y, v_z and p (the RHS and the next stage's input), z filter
+ * (every other step), and two unused slots. */
 int sparc3d_profile(sparc3d_handle *h, int reps, double *ms_out);
 
 #ifdef __cplusplus

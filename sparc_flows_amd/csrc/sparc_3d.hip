@@ -2,7 +2,6 @@
 // solver with flows, option 16 (MP_FLOWS_PML_3D_DISPL, displacement.f90:377-477).
 //
 // One RK stage (step.f90:48-106) runs as
-//   stage input  T = a + beta_s scr, psi = pmlvars + beta_s scrpml (s > 1)
 //   x pass       d/dx of xi_x, v_x, v_y, v_z (FFT rows; x contiguous)
 //   y pass       d/dy of xi_y, v_x, v_y, v_z (FFT lines of stride nx)
 //   z pass Zp    d xi_z/dz (ibc 1) with the z-PML, div, p and rho' g / rho0
@@ -10,8 +9,10 @@
 //   z pass Zv    d v_x/dz, d v_y/dz, d v_z/dz (ibc 1) and d p/dz (ibc 0, z-PML)
 //                as four chains of one lane; RHS v_i = -2 v0 . grad v_i
 //                - d_i p / rho0, RHS v_z - rho' g / rho0 + source, RHS xi_i = v_i
-// and after the fifth stage one pass adds deltat * scr to a and applies
-// FILTER_VARS_3D_HORIZ_PML (physics.f90:836-882) on every other step.
+// where the z passes write the next stage's input T = a + beta * RHS (and
+// its z-PML variables) instead of the RHS, and the fifth stage's the updated
+// state a + deltat * RHS; every other step one pass applies
+// FILTER_VARS_3D_HORIZ_PML (physics.f90:836-882).
 //
 // Layout: fields [nz][ny][nx], x fastest.  A z pass gives each lane one (x, y)
 // column and streams it bottom to top and back (the Thomas sweeps of dbyd2 /
@@ -34,17 +35,6 @@
 
 namespace sparc {
 namespace d3 {
-
-// out = a + beta * b over n doubles (n even; pointers 16-byte aligned).
-__global__ __launch_bounds__(256) void k3_axpy(double *out, const double *a, const double *b, double beta,
-                                               size_t n2)
-{
-    const size_t stride = (size_t)gridDim.x * 256;
-    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n2; i += stride) {
-        const double2 x = ((const double2 *)a)[i], y = ((const double2 *)b)[i];
-        ((double2 *)out)[i] = make_double2(x.x + beta * y.x, x.y + beta * y.y);
-    }
-}
 
 // d/dy (ddxyz, derivatives.f90:224-247) of nplanes [NY][nx] planes: a team of
 // N1 lanes transforms two neighbouring columns (x0, x0 + 1) as one complex
@@ -90,8 +80,14 @@ struct Z3 {
     Tri tr1, tr0;               // dbyd2 factors for ibc 1 and ibc 0
     const double *rhoinv;       // [nz] 1/rho0
     const double *az, *bz;      // [npml][nxy]
-    const double *psi;          // z-PML variables [2][npml][nxy]: d xi_z/dz, d p/dz
-    double *rpsi;               // their RHS
+    const double *psi;          // z-PML variables of this stage [2][npml][nxy]: d xi_z/dz, d p/dz
+    // the next stage's input (step.f90:60-62) or, after the fifth stage, the
+    // updated state (:107-108): out = base + beta * RHS, for the fields and
+    // the z-PML variables; the fifth stage also keeps the RHS in rhs
+    const double *a, *pml;      // base: state a [6][nz][nxy], pmlvars [2][npml][nxy]
+    double *tnext, *pnext;      // T of the next stage (or a), its PML variables (or pmlvars)
+    double beta;
+    int last;
     // Zp
     const double *dxi, *dyi;    // d xi_x/dx, d xi_y/dy
     const double *c2rho0, *gp0, *gr0;   // [nz][nxy], PML factor applied
@@ -184,7 +180,8 @@ __global__ __launch_bounds__(256) void k3_zp(Z3 z)
             if (pj >= 0) {
                 const size_t qp = c + (size_t)pj * s;
                 const double ps = z.psi[qp];
-                z.rpsi[qp] = z.az[qp] * dz + z.bz[qp] * ps;
+                const double rp = z.az[qp] * dz + z.bz[qp] * ps;
+                z.pnext[qp] = z.pml[qp] + z.beta * rp;
                 dz = dz + ps;
             }
             const double div = (dxi[u] + dyi[u]) + dz;
@@ -259,7 +256,8 @@ __global__ __launch_bounds__(256) void k3_zv(Z3 z)
             if (pj >= 0) {
                 const size_t qa = c + (size_t)pj * s, qp = qa + (size_t)(z.nb + z.nt) * s;
                 const double ps = z.psi[qp];
-                z.rpsi[qp] = z.az[qa] * gpz + z.bz[qa] * ps;
+                const double rp = z.az[qa] * gpz + z.bz[qa] * ps;
+                z.pnext[qp] = z.pml[qp] + z.beta * rp;
                 gpz = gpz + ps;
             }
             const bool edge = j == 0 || j == nz - 1;
@@ -268,44 +266,42 @@ __global__ __launch_bounds__(256) void k3_zv(Z3 z)
 #pragma unroll
             for (int i = 0; i < 3; i++)
                 adv[i] = -2.0 * ((u0[0][u] * dx[i][u] + u0[1][u] * dy[i][u]) + u0[2][u] * (d[i] * st));
-            r[q + 3 * N] = edge ? 0.0 : adv[0] - ri * dxp[u];
-            r[q + 4 * N] = edge ? 0.0 : adv[1] - ri * dyp[u];
+            double rhs[6];
+            rhs[3] = edge ? 0.0 : adv[0] - ri * dxp[u];
+            rhs[4] = edge ? 0.0 : adv[1] - ri * dyp[u];
             double rz = adv[2] - ri * gpz;
             rz = rz - rg[u];
             const int dk = j - z.e_rad;
             if (dk >= -4 && dk <= 4) rz = rz + forcing * z.srcdep[j];
-            r[q + 5 * N] = edge ? 0.0 : rz;
+            rhs[5] = edge ? 0.0 : rz;
 #pragma unroll
-            for (int i = 0; i < 3; i++) r[q + i * N] = edge ? 0.0 : v[i][u];
+            for (int i = 0; i < 3; i++) rhs[i] = edge ? 0.0 : v[i][u];
+            double base[6];
+#pragma unroll
+            for (int f = 0; f < 6; f++) base[f] = z.a[q + f * N];
+#pragma unroll
+            for (int f = 0; f < 6; f++) z.tnext[q + f * N] = base[f] + z.beta * rhs[f];
+            if (z.last) {
+#pragma unroll
+                for (int f = 0; f < 6; f++) r[q + f * N] = rhs[f];
+            }
         }
     }
 }
 
-// a += deltat * scr (step.f90:107) for the six fields, then, when filt is set,
-// FILTER_VARS_3D_HORIZ_PML (physics.f90:855-878) on rows npmlbot+1 ..
-// nz-npmltop from a register window of the updated, unfiltered column.
-__global__ __launch_bounds__(256) void k3_update(double *a, const double *scr, int nxy, int nz, int nb, int nt,
-                                                 double dt, int filt)
+// FILTER_VARS_3D_HORIZ_PML (physics.f90:855-878) on the six fields, rows
+// npmlbot+1 .. nz-npmltop, in place from a register window of the unfiltered
+// column.
+__global__ __launch_bounds__(256) void k3_filter(double *a, int nxy, int nz, int nb, int nt)
 {
     const long id = (long)blockIdx.x * 256 + threadIdx.x;
     if (id >= 6L * nxy) return;
     const int f = (int)(id / nxy), c = (int)(id - (long)f * nxy);
     const size_t s = (size_t)nxy;
     double *A = a + (size_t)f * nz * s + c;
-    const double *S = scr + (size_t)f * nz * s + c;
-    if (!filt) {
-        for (int j = 0; j < nz; j++) A[j * s] = A[j * s] + dt * S[j * s];
-        return;
-    }
-    for (int j = 0; j < nb - 5; j++) A[j * s] = A[j * s] + dt * S[j * s];
-    double w[11];                       // w[k] = u(j - 5 + k) at filtered row j
+    double w[11];                       // w[k] = a(j - 5 + k) at filtered row j
 #pragma unroll
-    for (int k = 0; k < 11; k++) {
-        const size_t q = (size_t)(nb - 5 + k) * s;
-        w[k] = A[q] + dt * S[q];
-    }
-#pragma unroll
-    for (int k = 0; k < 5; k++) A[(size_t)(nb - 5 + k) * s] = w[k];
+    for (int k = 0; k < 11; k++) w[k] = A[(size_t)(nb - 5 + k) * s];
     const double c0 = 0.75390625, c1 = 0.41015625, c2 = -0.234375, c3 = 0.087890625,
                  c4 = -0.01953125, c5 = 0.001953125;
     for (int j = nb; j < nz - nt; j++) {
@@ -315,15 +311,12 @@ __global__ __launch_bounds__(256) void k3_update(double *a, const double *scr, i
         v = v + 0.5 * c4 * (w[1] + w[9]);
         v = v + 0.5 * c5 * (w[0] + w[10]);
         const int jn = j + 6;
-        const double un = jn < nz ? A[(size_t)jn * s] + dt * S[(size_t)jn * s] : 0.0;
+        const double un = jn < nz ? A[(size_t)jn * s] : 0.0;
         A[(size_t)j * s] = v;
 #pragma unroll
         for (int k = 0; k < 10; k++) w[k] = w[k + 1];
         w[10] = un;
     }
-#pragma unroll
-    for (int k = 0; k < 5; k++) A[(size_t)(nz - nt + k) * s] = w[5 + k];
-    for (int j = nz - nt + 5; j < nz; j++) A[j * s] = A[j * s] + dt * S[j * s];
 }
 
 // One snapshot pair of PRODUCE_KERNELS with its y terms (kernels.f90:
@@ -379,15 +372,6 @@ __global__ __launch_bounds__(256) void k3_kern_final(const double *__restrict__ 
     default: return hipErrorInvalidValue; \
     }
 
-static hipError_t launch_axpy(double *out, const double *a, const double *b, double beta, size_t n,
-                              hipStream_t st)
-{
-    const size_t n2 = n / 2;
-    const unsigned blocks = (unsigned)std::min<size_t>((n2 + 255) / 256, 8192);
-    hipLaunchKernelGGL(k3_axpy, dim3(blocks), dim3(256), 0, st, out, a, b, beta, n2);
-    return hipGetLastError();
-}
-
 static hipError_t launch_ddy(const double *in, double *out, int nplanes, int nx, int ny, const double2 *tw,
                              const double *dmul, double stretch, hipStream_t st)
 {
@@ -415,12 +399,10 @@ static hipError_t launch_zv(const Z3 &z, hipStream_t st)
     return hipGetLastError();
 }
 
-static hipError_t launch_update(double *a, const double *scr, int nxy, int nz, int nb, int nt, double dt,
-                                int filt, hipStream_t st)
+static hipError_t launch_filter(double *a, int nxy, int nz, int nb, int nt, hipStream_t st)
 {
     const long n = 6L * nxy;
-    hipLaunchKernelGGL(k3_update, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a, scr, nxy, nz, nb, nt,
-                       dt, filt);
+    hipLaunchKernelGGL(k3_filter, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a, nxy, nz, nb, nt);
     return hipGetLastError();
 }
 
@@ -455,7 +437,7 @@ struct sparc3d_handle {
                  *d_g = nullptr, *d_srcdep = nullptr;
     double *c2rho0 = nullptr, *gp0 = nullptr, *gr0 = nullptr, *az = nullptr, *bz = nullptr;
     double *v0x = nullptr, *v0y = nullptr, *v0z = nullptr;
-    double *a = nullptr, *scr = nullptr, *tmp = nullptr, *pml = nullptr, *scrpml = nullptr, *psit = nullptr;
+    double *a = nullptr, *scr = nullptr, *tmp = nullptr, *pml = nullptr, *psit = nullptr;
     double *DXI = nullptr, *DYI = nullptr, *DX3 = nullptr, *DY3 = nullptr, *P = nullptr, *RG = nullptr,
            *DXP = nullptr, *DYP = nullptr, *W = nullptr;
     Src src{};
@@ -553,13 +535,13 @@ static int enqueue_stage(sparc3d_handle *h, int s, int t)
 {
     const size_t N = h->n;
     const int nyz = h->ny * h->nz, nz = h->nz;
-    const double *T = h->a, *PS = h->pml;
-    if (s > 0) {                                                  // step.f90:60-62
-        RUN3(0, launch_axpy(h->tmp, h->a, h->scr, h->betas[s], 6 * N, h->stream));
-        RUN3(0, launch_axpy(h->psit, h->pml, h->scrpml, h->betas[s], 2 * h->np, h->stream));
-        T = h->tmp;
-        PS = h->psit;
-    }
+    // stage 1 reads a and pmlvars (step.f90:50-52); the z passes of stage s
+    // write stage s+1's input T = a + beta * RHS (:60-62) into tmp / psit, and
+    // those of stage 5 the update a += deltat * RHS (:107-108) into a / pmlvars
+    const double *T = s == 0 ? h->a : h->tmp, *PS = s == 0 ? h->pml : h->psit;
+    const bool last = s == 4;
+    double *tnext = last ? h->a : h->tmp, *pnext = last ? h->pml : h->psit;
+    const double beta = last ? h->deltat : h->betas[s + 1];
     int r;
     // ddxyz(xi_x, xi_y, xi_z) and the x, y derivatives of the velocity
     if ((r = ddx3(h, T, h->DXI, nyz))) return r;
@@ -569,7 +551,9 @@ static int enqueue_stage(sparc3d_handle *h, int s, int t)
     {
         Z3 z = zargs(h, T + 2 * N);
         z.psi = PS;
-        z.rpsi = h->scrpml;
+        z.pml = h->pml;
+        z.pnext = pnext;
+        z.beta = beta;
         z.dxi = h->DXI;
         z.dyi = h->DYI;
         z.c2rho0 = h->c2rho0;
@@ -588,7 +572,12 @@ static int enqueue_stage(sparc3d_handle *h, int s, int t)
         z.P = h->P;
         z.RG = h->RG;
         z.psi = PS;
-        z.rpsi = h->scrpml;
+        z.pml = h->pml;
+        z.pnext = pnext;
+        z.a = h->a;
+        z.tnext = tnext;
+        z.beta = beta;
+        z.last = last;
         z.dxv = h->DX3;
         z.dyv = h->DY3;
         z.v0x = h->v0x;
@@ -613,8 +602,7 @@ static int enqueue_step3(sparc3d_handle *h, int t)
         if (r) return r;
     }
     const int filt = (t + 1) % 2 == 0;                           // step.f90:114
-    RUN3(5, launch_update(h->a, h->scr, (int)h->nxy, h->nz, h->nb, h->nt, h->deltat, filt, h->stream));
-    RUN3(0, launch_axpy(h->pml, h->pml, h->scrpml, h->deltat, 2 * h->np, h->stream));
+    if (filt) RUN3(5, launch_filter(h->a, (int)h->nxy, h->nz, h->nb, h->nt, h->stream));
     return SPARC_OK;
 }
 
@@ -805,7 +793,6 @@ static int setup3(sparc3d_handle *h)
     HCHECK3(h, m.alloc(&h->scr, 6 * N));
     HCHECK3(h, m.alloc(&h->tmp, 6 * N));
     HCHECK3(h, m.alloc(&h->pml, 2 * h->np));
-    HCHECK3(h, m.alloc(&h->scrpml, 2 * h->np));
     HCHECK3(h, m.alloc(&h->psit, 2 * h->np));
     HCHECK3(h, m.alloc(&h->DXI, N));
     HCHECK3(h, m.alloc(&h->DYI, N));
@@ -819,7 +806,6 @@ static int setup3(sparc3d_handle *h)
     HCHECK3(h, hipMemsetAsync(h->a, 0, 6 * N * sizeof(double), h->stream));
     HCHECK3(h, hipMemsetAsync(h->scr, 0, 6 * N * sizeof(double), h->stream));
     HCHECK3(h, hipMemsetAsync(h->pml, 0, 2 * h->np * sizeof(double), h->stream));
-    HCHECK3(h, hipMemsetAsync(h->scrpml, 0, 2 * h->np * sizeof(double), h->stream));
     HCHECK3(h, hipStreamSynchronize(h->stream));
     (void)nx;
     (void)ny;
@@ -1040,7 +1026,6 @@ int sparc3d_reset(sparc3d_handle *h)
     HCHECK3(h, hipMemsetAsync(h->a, 0, 6 * h->n * sizeof(double), h->stream));
     HCHECK3(h, hipMemsetAsync(h->scr, 0, 6 * h->n * sizeof(double), h->stream));
     HCHECK3(h, hipMemsetAsync(h->pml, 0, 2 * h->np * sizeof(double), h->stream));
-    HCHECK3(h, hipMemsetAsync(h->scrpml, 0, 2 * h->np * sizeof(double), h->stream));
     HCHECK3(h, hipStreamSynchronize(h->stream));
     return SPARC_OK;
 }

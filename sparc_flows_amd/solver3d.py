@@ -320,7 +320,7 @@ class Sparc3D:
         """Mean ms per launch of each pass kind over `reps` steps (HIP events)."""
         ms = (ctypes.c_double * 8)()
         self._check(lib().sparc3d_profile(self.h, int(reps), ms), self.h)
-        names = ["stage", "ddx", "ddy", "z_p", "z_v", "update", "unused6", "unused7"]
+        names = ["unused0", "ddx", "ddy", "z_p", "z_v", "filter", "unused6", "unused7"]
         return {n: ms[i] for i, n in enumerate(names)}
 
 
