@@ -38,18 +38,27 @@ namespace d3 {
 
 // d/dy (ddxyz, derivatives.f90:224-247) of nplanes [NY][nx] planes: a team of
 // N1 lanes transforms two neighbouring columns (x0, x0 + 1) as one complex
-// line, so each lane moves 16 bytes per row and a wave covers 2 * TEAMS
-// consecutive columns.
+// line, so each lane moves 16 bytes per row; a block covers 2 * TEAMS
+// consecutive columns (128 bytes, one cache line, of each row at NY = 256).  Blocks
+// are renumbered so that the eight XCDs (which take the hardware block order
+// round robin) each work on contiguous stretches of columns and keep their
+// partial cache lines in their own L2.
+// Block width per line length: the teams' LDS stays within 64 KiB.
+template <int NY> struct DdyBlock { static constexpr int n = NY <= 128 ? 256 : (NY <= 512 ? 128 : 64); };
+
 template <int NY>
-__global__ __launch_bounds__(64) void k3_ddy(const double *__restrict__ in, double *__restrict__ out, int nplanes,
-                                             int nx, const double2 *__restrict__ tw,
-                                             const double *__restrict__ dmul, double stretch)
+__global__ __launch_bounds__(256) void k3_ddy(const double *__restrict__ in, double *__restrict__ out, int nplanes,
+                                              int nx, const double2 *__restrict__ tw,
+                                              const double *__restrict__ dmul, double stretch)
 {
-    constexpr int N1 = FftShape<NY>::N1, N2 = FftShape<NY>::N2, TEAMS = 64 / N1;
+    constexpr int N1 = FftShape<NY>::N1, N2 = FftShape<NY>::N2, TEAMS = DdyBlock<NY>::n / N1;
     __shared__ double2 lds[TEAMS][FftLds<N1, N2>::n];
     const int team = threadIdx.x / N1, t = threadIdx.x % N1;
     const int hx = nx >> 1;
-    const long pair = (long)blockIdx.x * TEAMS + team;
+    int bid = blockIdx.x;
+    const int nb = gridDim.x;
+    if ((nb & 7) == 0) bid = (bid & 7) * (nb >> 3) + (bid >> 3);
+    const long pair = (long)bid * TEAMS + team;
     const bool ok = pair < (long)nplanes * hx;
     const long pl = ok ? pair / hx : 0;
     const int x0 = ok ? 2 * (int)(pair - pl * hx) : 0;
@@ -377,9 +386,9 @@ static hipError_t launch_ddy(const double *in, double *out, int nplanes, int nx,
 {
 #define L(NYV)                                                                                     \
     {                                                                                              \
-        constexpr int TEAMS = 64 / FftShape<NYV>::N1;                                              \
+        constexpr int BLK = DdyBlock<NYV>::n, TEAMS = BLK / FftShape<NYV>::N1;                     \
         const long pairs = (long)nplanes * (nx / 2);                                               \
-        hipLaunchKernelGGL(k3_ddy<NYV>, dim3((unsigned)((pairs + TEAMS - 1) / TEAMS)), dim3(64), 0, \
+        hipLaunchKernelGGL(k3_ddy<NYV>, dim3((unsigned)((pairs + TEAMS - 1) / TEAMS)), dim3(BLK), 0, \
                            st, in, out, nplanes, nx, tw, dmul, stretch);                           \
     }
     SPARC3D_DISPATCH(ny, L)
