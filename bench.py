@@ -83,7 +83,7 @@ def forward_rate(nx, nz, nsrc, nsteps, flow, dc, reps=1):
             "column_pass_shape": {k: s.query(k) for k in ("colseg_tc", "colseg_ms", "colseg_p", "colseg_split")}}
 
 
-def forward3d_rate(nsteps=20, warm=2, nx=256, ny=256, nz=300):
+def forward3d_rate(nsteps=20, warm=2, nx=256, ny=256, nz=300, cpu_seconds=8.0):
     """3D forward (option 16) at 256x256x300 with the axisymmetric supergranule
     flow and the radially symmetric sound-speed bump: Mpoint-steps/s, the
     per-pass launch times (HIP events on the solver's stream) and the roofline
@@ -120,12 +120,34 @@ def forward3d_rate(nsteps=20, warm=2, nx=256, ny=256, nz=300):
     dom = max(per_step, key=per_step.get)
     npts = nx * ny * nz
     achieved = bpp[dom] * npts / (ms[dom] * 1e-3) / 1e9
+    cpu = cpu_baseline_3d(p, bg) if cpu_seconds > 0 else None
     return {"Mpoint_steps_per_s": npts / sec / 1e6, "ms_per_step": sec * 1e3, "grid": [nx, ny, nz],
+            "cpu_baseline": cpu,
             "time_steps_timed": nsteps, "flow": "axisymmetric supergranule 240 m/s",
             "dc": "radial bump 5%", "ms_per_launch": ms, "ms_per_step_by_pass": per_step,
             "roofline": {"kernel": dom, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                          "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                          "algorithmic_bytes_per_point": bpp[dom]}}
+
+
+def cpu_baseline_3d(p, bg, seconds=8.0, nxy=64):
+    """The 3D oracle (numpy, one process) stepping a 64x64 column subset of the
+    3D workload's grid (same nz, PML and time step) for about `seconds`."""
+    import dataclasses
+    from oracle.oracle3d import Oracle3D, Params3D as OP
+
+    q = dataclasses.replace(p, nx=nxy, ny=nxy, xlength=p.xlength * nxy / p.nx, ylength=p.ylength * nxy / p.ny)
+    o = Oracle3D(OP(**dataclasses.asdict(q)), bg)
+    o.set_forward_source(0.0, 0.0, 100000)
+    n = 0
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < seconds:
+        o.step(n)
+        n += 1
+    dt = time.perf_counter() - t0
+    return {"value": nxy * nxy * p.nz * n / dt / 1e6, "unit": "Mpoint-steps/s", "cores": 1, "kind": "port",
+            "sample": f"{n} forward time steps of a {nxy}x{nxy}x{p.nz} grid (same dz, PML and time step), "
+                      f"{dt:.1f} s, numpy oracle"}
 
 
 def gradient3d_rate(maxtime=449, interval=150, nx=256, ny=256, nz=300):
