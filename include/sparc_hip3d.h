@@ -60,11 +60,12 @@ int sparc3d_set_stream(sparc3d_handle *h, void *stream);
  * adjoint handle reverses it (driver.f90:144-147).  NULL, NULL, NULL: no flow. */
 int sparc3d_set_flow(sparc3d_handle *h, const double *vx, const double *vy, const double *vz);
 
-/* Forward source at (xsrc, ysrc) Mm: forcing(x, y) = vr(x, y) * fwdsource(t+1)
- * with vr the radial form of the 2D profile (driver.f90:536-553) and
- * fwdsource from FORWARD_SOURCE (driver.f90:1120-1145), injected as
- * forcing * source_dep(k) on rows |k - e_rad| <= 4 (displacement.f90:459-461).
- * Sets maxtime. */
+/* Forward source: forcing(x, y) = vr(x, y) * fwdsource(t+1), injected as
+ * forcing * source_dep(k) on rows |k - e_rad| <= 4 (displacement.f90:459-461),
+ * with fwdsource from FORWARD_SOURCE (driver.f90:1120-1145).  ysrc_mm NaN: vr
+ * as DETERMINE_STATUS builds it, the x profile at xsrc repeated in every y row
+ * (driver.f90:536-553, a line source); otherwise the same profile in the
+ * horizontal distance to (xsrc, ysrc) (a point source).  Sets maxtime. */
 int sparc3d_set_forward_source(sparc3d_handle *h, double xsrc_mm, double ysrc_mm, int maxtime);
 /* Adjoint source (DETERMINE_STATUS adjoint branch + LAGRANGE_INTERP,
  * driver.f90:584-627, physics.f90:678-737): device [nt][ny][nx] slices at the

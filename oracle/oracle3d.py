@@ -25,8 +25,9 @@ Three points where the reference cannot run this path as written are fixed
 here, and the GPU implementation follows the same fixes:
 - the forward source: kernel-mode forward runs never set `forcing`
   (driver.f90:358-360); here forcing(x, y) = vr(x, y) * fwdsource(time+1),
-  with vr the radial form of the 2D source profile (driver.f90:536-553, the
-  x distance replaced by the horizontal distance) and fwdsource from
+  with vr as DETERMINE_STATUS builds it (driver.f90:536-553: the x profile
+  repeated along y, a line source) or, for a point source at (xsrc, ysrc),
+  the same profile in the horizontal distance, and fwdsource from
   FORWARD_SOURCE (driver.f90:1120-1145);
 - the flow: the reference derives only v0_x, v0_z from a 2D psi
   (driver.f90:94-126); here (v0_x, v0_y, v0_z) are given (in cm/s), as the
@@ -199,18 +200,28 @@ class Oracle3D:
         return d * self.stretch.reshape((nz,) + (1,) * (f.ndim - 1))
 
     # ---- sources ----
-    def set_forward_source(self, xsrc_mm: float, ysrc_mm: float, maxtime: int):
+    def set_forward_source(self, xsrc_mm: float, ysrc_mm, maxtime: int):
+        """ysrc_mm None: the reference's vr, the x profile repeated along y
+        (DETERMINE_STATUS, driver.f90:536-553, `vr(:,j,1)` for every j: a line
+        source); a number: the same profile in the horizontal distance to
+        (xsrc, ysrc) (a point source)."""
         p = self.p
         e8 = f32(1e-8)
         xd = (self.x - 0.5) * p.xlength * e8 - xsrc_mm
-        yd = (self.y - 0.5) * p.ylength * e8 - ysrc_mm
         dx = (self.x[1] - self.x[0]) * p.xlength * e8
-        rd = np.sqrt(yd[:, None] ** 2 + xd[None, :] ** 2)
+        if ysrc_mm is None:
+            rd = np.broadcast_to(xd[None, :], (p.ny, p.nx))
+            imin = None
+        else:
+            yd = (self.y - 0.5) * p.ylength * e8 - ysrc_mm
+            rd = np.sqrt(yd[:, None] ** 2 + xd[None, :] ** 2)
         k43 = f32(0.43) * PI
-        imin = int(np.argmin(rd))
         with np.errstate(invalid="ignore", divide="ignore"):
             sinc = np.sin(k43 * rd) / (k43 * rd)
-        sinc.flat[imin] = 1.0
+        if ysrc_mm is None:
+            sinc[:, int(np.argmin(np.abs(xd)))] = 1.0
+        else:
+            sinc.flat[int(np.argmin(rd))] = 1.0
         self.vr = np.exp(-(rd * rd) / (400.0 * (dx * dx))) * sinc
         self.fwd = forward_source_series(p.timestep, p.nupeak, p.nuwidth, maxtime + 8)
 

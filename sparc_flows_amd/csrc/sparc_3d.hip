@@ -973,19 +973,25 @@ int sparc3d_set_forward_source(sparc3d_handle *h, double xsrc, double ysrc, int 
     const int nx = h->nx, ny = h->ny;
     const double e8 = (double)1e-8f, k43 = (double)0.43f * kPi3;
     const double dx = (1.0 / (double)nx) * h->xlength * e8;
+    // NaN ysrc: the reference's vr, the x profile in every y row (a line
+    // source); otherwise the profile in the horizontal distance (a point source)
+    const bool line = std::isnan(ysrc);
     std::vector<double> rd(h->nxy), vr(h->nxy);
     size_t imin = 0;
+    int ixmin = 0;
     for (int j = 0; j < ny; j++) {
-        const double yd = ((double)j / (double)ny - 0.5) * h->ylength * e8 - ysrc;
+        const double yd = line ? 0.0 : ((double)j / (double)ny - 0.5) * h->ylength * e8 - ysrc;
         for (int i = 0; i < nx; i++) {
             const double xd = ((double)i / (double)nx - 0.5) * h->xlength * e8 - xsrc;
             const size_t q = (size_t)j * nx + i;
-            rd[q] = std::sqrt(yd * yd + xd * xd);
-            if (rd[q] < rd[imin]) imin = q;
+            rd[q] = line ? xd : std::sqrt(yd * yd + xd * xd);
+            if (std::fabs(rd[q]) < std::fabs(rd[imin])) imin = q;
+            if (j == 0 && std::fabs(xd) < std::fabs(rd[ixmin])) ixmin = i;
         }
     }
     for (size_t q = 0; q < h->nxy; q++) {
-        const double sinc = q == imin ? 1.0 : std::sin(k43 * rd[q]) / (k43 * rd[q]);
+        const bool peak = line ? (int)(q % nx) == ixmin : q == imin;
+        const double sinc = peak ? 1.0 : std::sin(k43 * rd[q]) / (k43 * rd[q]);
         vr[q] = std::exp(-(rd[q] * rd[q]) / (400.0 * (dx * dx))) * sinc;
     }
     h->src_mem.release();

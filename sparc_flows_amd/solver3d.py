@@ -193,8 +193,11 @@ class Sparc3D:
                 for v in (vx, vy, vz)]
         self._call("sparc3d_set_flow", *[a.ctypes.data for a in arrs])
 
-    def set_forward_source(self, xsrc_mm: float, ysrc_mm: float, maxtime: int) -> None:
-        self._call("sparc3d_set_forward_source", float(xsrc_mm), float(ysrc_mm), int(maxtime))
+    def set_forward_source(self, xsrc_mm: float, ysrc_mm, maxtime: int) -> None:
+        """ysrc_mm None: the reference's line source at x = xsrc (uniform in y);
+        a number: a point source at (xsrc, ysrc)."""
+        y = float("nan") if ysrc_mm is None else float(ysrc_mm)
+        self._call("sparc3d_set_forward_source", float(xsrc_mm), y, int(maxtime))
         self.maxtime = int(maxtime)
 
     def set_adjoint_source(self, vr: torch.Tensor, maxtime: int) -> None:
@@ -218,7 +221,7 @@ class Sparc3D:
         self._call("sparc3d_run", maxtime, _ptr(vz))
         return vz
 
-    def forward(self, xsrc_mm: float, ysrc_mm: float, maxtime: int) -> torch.Tensor:
+    def forward(self, xsrc_mm: float, ysrc_mm, maxtime: int) -> torch.Tensor:
         self.set_forward_source(xsrc_mm, ysrc_mm, maxtime)
         self.reset()
         return self.run(maxtime)
@@ -324,11 +327,12 @@ class Sparc3D:
         return {n: ms[i] for i, n in enumerate(names)}
 
 
-def receiver_mask_3d(p: Params3D, xsrc: float, ysrc: float, dmin_mm: float = 5.0, dmax_mm: float = 60.0):
-    """[ny][nx] 1 where the horizontal distance to the source is within [dmin, dmax] Mm."""
+def receiver_mask_3d(p: Params3D, xsrc: float, ysrc, dmin_mm: float = 5.0, dmax_mm: float = 60.0):
+    """[ny][nx] 1 where the horizontal distance to the source (to the line
+    x = xsrc for a line source, ysrc None) is within [dmin, dmax] Mm."""
     x = (np.arange(p.nx) / p.nx - 0.5) * p.xlength * 1e-8 - xsrc
-    y = (np.arange(p.ny) / p.ny - 0.5) * p.ylength * 1e-8 - ysrc
-    d = np.sqrt(x[None, :] ** 2 + y[:, None] ** 2)
+    y = (np.arange(p.ny) / p.ny - 0.5) * p.ylength * 1e-8 - (0.0 if ysrc is None else ysrc)
+    d = np.sqrt(x[None, :] ** 2 + (0.0 if ysrc is None else 1.0) * y[:, None] ** 2)
     return ((d >= dmin_mm) & (d <= dmax_mm)).astype(np.float64)
 
 
@@ -358,7 +362,7 @@ class Gradient3D:
             self.fwd.set_flow(*flow)
             self.adj.set_flow(*flow)
 
-    def run(self, xsrc: float, ysrc: float, maxtime: int, vz_obs: torch.Tensor | None = None) -> dict:
+    def run(self, xsrc: float, ysrc, maxtime: int, vz_obs: torch.Tensor | None = None) -> dict:
         from .gradient import waveform_adjoint_source
 
         p, I = self.p, self.interval

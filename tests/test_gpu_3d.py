@@ -212,3 +212,19 @@ def test_checkpointed_gradient_matches_oracle():
     K1 = Gradient3D(p, bg, c3d=c3d, flow=fl, interval=maxtime + 1, window_s=window).run(5.0, -3.0, maxtime)
     for k in ("kc", "kvx", "kvy", "kvz", "hess"):
         assert torch.equal(K1[k], K[k]), k
+
+
+def test_line_source_matches_oracle_and_is_y_invariant():
+    """The reference's vr (the x profile in every y row): with no flow and no
+    sound-speed bump the whole run stays uniform in y."""
+    p, bg, c3d, fl = case(flow=False, dc=False)
+    o, s = pair(p, bg, c3d, fl)
+    maxtime = 100
+    o.set_forward_source(5.0, None, maxtime)
+    assert np.allclose(o.vr, o.vr[:1])
+    ovz = o.run(maxtime)
+    vz = s.forward(5.0, None, maxtime).cpu().numpy()
+    assert rel_l2(vz, ovz) < 1e-10
+    a = s.state().cpu().numpy()
+    assert np.abs(a - a[:, :, :1]).max() <= 1e-14 * np.abs(a).max()
+    assert np.abs(a[1]).max() <= 1e-14 * np.abs(a).max() and np.abs(a[4]).max() <= 1e-14 * np.abs(a).max()

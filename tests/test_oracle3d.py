@@ -185,3 +185,17 @@ def test_kernels3d_reduce_to_the_2d_kernels():
         assert np.allclose(k3["kvz"][:, j] * d2 / d3, k2["kvz"], rtol=1e-11, atol=1e-13 * np.abs(k2["kvz"]).max())
         assert np.allclose(k3["kc"][:, j] * dc2 / dc3, k2["kc"], rtol=1e-11, atol=1e-13 * np.abs(k2["kc"]).max())
     assert np.all(k3["kvy"] == 0)
+
+
+def test_line_source_is_the_reference_x_profile():
+    """ysrc None: vr is the 2D x profile (orc_set_forward_source's) in every y row."""
+    from sparc_flows_amd.params import SparcParams
+    p, bg, _, _ = make(nx=32, ny=8, flow=False, dc=False)
+    o = Oracle3D(p, bg)
+    o.set_forward_source(7.5, None, 40)
+    p2 = SparcParams(nx=32, nz=p.nz, xlength=p.xlength, timestep=p.timestep, npmlbot=6, npmltop=6, npmlhor=3)
+    o2 = oracle.Oracle(p2, bg)
+    o2.set_forward_source(7.5, 40)
+    vr2 = o2.array("vr")[:32]
+    for j in range(p.ny):
+        assert np.allclose(o.vr[j], vr2, rtol=1e-14, atol=1e-16)   # numpy vs libm sin, exp
