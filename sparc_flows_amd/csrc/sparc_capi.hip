@@ -83,7 +83,7 @@ struct sparc_handle {
     SegTab seg{};
     int seg_tc = 16, seg_ms = 20;
     int seg_split = 0;                   // 1: one lane per (column, segment, field)
-    int row_wide = 0;                    // 1: row pass on 32 lanes per row (nx = 256)
+    int row_wide = 0;                    // row pass at nx = 256 on 16 (0), 32 (1) or 64 (2) lanes per row
     bool tuned = false;
 };
 
@@ -719,7 +719,10 @@ static int setup_seg(sparc_handle *h)
     h->seg_split = 0;
     if (const char *e = std::getenv("SPARC_COLSEG_SPLIT")) h->seg_split = std::atoi(e) == 1;
     h->row_wide = 0;
-    if (const char *e = std::getenv("SPARC_ROW_WIDE")) h->row_wide = std::atoi(e) == 1 && h->nx == 256;
+    if (const char *e = std::getenv("SPARC_ROW_WIDE")) {
+        const int v = std::atoi(e);
+        h->row_wide = (v == 1 || v == 2) && h->nx == 256 ? v : 0;
+    }
     h->seg_tc = 1;
     for (int tc : {2, 4, 8, 16}) {
         const bool fits = h->seg_split ? (tc * sg.P + 63) / 64 <= 2 : tc * sg.P <= 256;
@@ -1202,25 +1205,28 @@ int sparc_step(sparc_handle *h, int t)
     return SPARC_OK;
 }
 
-// Row-pass team width for nx = 256: 32 lanes per row (two rows per wave)
-// when the 16-lane form would leave compute units without a wave (one
-// 64-lane wave per 4 rows), 16 lanes otherwise.  Measured: 10 % faster
-// gradients for one source, 2 % slower for 16.  (Timing a few steps with
-// each width at set-up chose the 32-lane form for 16 sources too.)
-// SPARC_ROW_WIDE (0 or 1) forces the choice.
+// Row-pass team width for nx = 256: the narrowest form whose waves still
+// reach every compute unit -- 64 lanes per row (one row per wave) when the
+// 32-lane form (two rows per wave) would leave compute units idle, else 32
+// lanes when the 16-lane form (four rows per wave) would, else 16 lanes.
+// Measured for one source: 32 lanes 10 % and 64 lanes a further 3 % faster
+// gradients; for 16 sources the 16-lane form is 2 % faster than 32 lanes.
+// (Timing a few steps with each width at set-up chose the 32-lane form for
+// 16 sources too.)  SPARC_ROW_WIDE (0, 1 or 2) forces the choice.
 static int tune_rows(sparc_handle *h)
 {
     h->row_wide = 0;
     if (h->nx != 256) return SPARC_OK;
     if (const char *e = std::getenv("SPARC_ROW_WIDE")) {
         const int v = std::atoi(e);
-        if (v == 0 || v == 1) { h->row_wide = v; return SPARC_OK; }
+        if (v >= 0 && v <= 2) { h->row_wide = v; return SPARC_OK; }
     }
     int dev = 0, ncu = 0;
     HCHECK(h, hipGetDevice(&dev));
     HCHECK(h, hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-    const long waves16 = ((long)h->S * h->nz + 3) / 4;
-    h->row_wide = waves16 < ncu ? 1 : 0;
+    const long rows = (long)h->S * h->nz;
+    const long waves16 = (rows + 3) / 4, waves32 = (rows + 1) / 2;
+    h->row_wide = waves32 < ncu ? 2 : (waves16 < ncu ? 1 : 0);
     return SPARC_OK;
 }
 

@@ -367,8 +367,138 @@ __device__ __forceinline__ void fft_deriv_team_256x32(double (&re)[8], double (&
 }
 
 // LDS slots (double2) a team needs for the derivative of an nx-point row.
+// The same derivative for nx = 256 on 64 lanes (one row per wave), in four
+// radix-4 stages: with x = t + 64 m and k = k2 + 4p + 16r + 64g,
+//   S1 (lane t):            DFT over m, twiddle W_256^(t k2)      -> A[k2][t]
+//   S2 (lane (k2, t'), t = t' + 16 v): DFT over v, W_64^(t' p)     -> B[k2][t'][p]
+//   S3 (lane (k2, p, s), t' = s + 4 w): DFT over w, W_16^(s r)    -> A[k2][p][s][r]
+//   S4 (lane (k2, p, r)):   DFT over s -> X[k]; times i*dmul[k]; inverse over g
+// and the inverse stages back to lane t holding x = t + 64 m.  A and B are the
+// wave's two LDS buffers of 256 slots (one pad slot per 32), used alternately
+// so that each exchange needs one barrier.
+__device__ __forceinline__ int fft64_ix(int e) { return e + (e >> 5); }
+
+__device__ __forceinline__ void fft_deriv_team_256x64(double (&re)[4], double (&im)[4], int t, double2 *A,
+                                                      double2 *B, const double2 *__restrict__ tw,
+                                                      const double *__restrict__ dmul, double stretchx)
+{
+    // S1
+    dft_reg<4, -1>(re, im);
+#pragma unroll
+    for (int k2 = 0; k2 < 4; k2++) {
+        const double2 w = tw[(t * k2) & 255];
+        A[fft64_ix(k2 * 64 + t)] = make_double2(re[k2] * w.x - im[k2] * w.y, re[k2] * w.y + im[k2] * w.x);
+    }
+    __syncthreads();
+    const int k2 = t >> 4;
+    {   // S2
+        const int tp = t & 15;
+        double yr[4], yi[4];
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const double2 a = A[fft64_ix(k2 * 64 + tp + 16 * v)];
+            yr[v] = a.x; yi[v] = a.y;
+        }
+        dft_reg<4, -1>(yr, yi);
+#pragma unroll
+        for (int p = 0; p < 4; p++) {
+            const double2 w = tw[(4 * tp * p) & 255];
+            B[fft64_ix(k2 * 64 + tp * 4 + p)] = make_double2(yr[p] * w.x - yi[p] * w.y, yr[p] * w.y + yi[p] * w.x);
+        }
+    }
+    __syncthreads();
+    const int p = (t >> 2) & 3, s = t & 3;     // S3 and IS3: (k2, p, s); S4: (k2, p, r = s)
+    {   // S3
+        double yr[4], yi[4];
+#pragma unroll
+        for (int w4 = 0; w4 < 4; w4++) {
+            const double2 a = B[fft64_ix(k2 * 64 + (s + 4 * w4) * 4 + p)];
+            yr[w4] = a.x; yi[w4] = a.y;
+        }
+        dft_reg<4, -1>(yr, yi);
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const double2 w = tw[(16 * s * r) & 255];
+            A[fft64_ix(k2 * 64 + p * 16 + s * 4 + r)] =
+                make_double2(yr[r] * w.x - yi[r] * w.y, yr[r] * w.y + yi[r] * w.x);
+        }
+    }
+    __syncthreads();
+    {   // S4, the derivative, inverse over g
+        const int r = s;
+        double xr[4], xi[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const double2 a = A[fft64_ix(k2 * 64 + p * 16 + q * 4 + r)];
+            xr[q] = a.x; xi[q] = a.y;
+        }
+        dft_reg<4, -1>(xr, xi);
+#pragma unroll
+        for (int g = 0; g < 4; g++) {
+            const double w = dmul[k2 + 4 * p + 16 * r + 64 * g];
+            const double x0 = xr[g];
+            xr[g] = -(xi[g] * w);
+            xi[g] = x0 * w;
+        }
+        dft_reg<4, +1>(xr, xi);
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const double2 w = tw[(16 * q * r) & 255];   // conj: exp(+2 pi i q r / 16)
+            B[fft64_ix(k2 * 64 + p * 16 + q * 4 + r)] =
+                make_double2(xr[q] * w.x + xi[q] * w.y, xi[q] * w.x - xr[q] * w.y);
+        }
+    }
+    __syncthreads();
+    {   // IS3: inverse over r at (k2, p, s) -> t' = s + 4 w
+        double yr[4], yi[4];
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const double2 a = B[fft64_ix(k2 * 64 + p * 16 + s * 4 + r)];
+            yr[r] = a.x; yi[r] = a.y;
+        }
+        dft_reg<4, +1>(yr, yi);
+#pragma unroll
+        for (int w4 = 0; w4 < 4; w4++) {
+            const int tp = s + 4 * w4;
+            const double2 w = tw[(4 * tp * p) & 255];
+            A[fft64_ix(k2 * 64 + tp * 4 + p)] = make_double2(yr[w4] * w.x + yi[w4] * w.y, yi[w4] * w.x - yr[w4] * w.y);
+        }
+    }
+    __syncthreads();
+    {   // IS2: inverse over p at (k2, t') -> t = t' + 16 v
+        const int tp = t & 15;
+        double yr[4], yi[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const double2 a = A[fft64_ix(k2 * 64 + tp * 4 + q)];
+            yr[q] = a.x; yi[q] = a.y;
+        }
+        dft_reg<4, +1>(yr, yi);
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const int tt = tp + 16 * v;
+            const double2 w = tw[(tt * k2) & 255];
+            B[fft64_ix(k2 * 64 + tt)] = make_double2(yr[v] * w.x + yi[v] * w.y, yi[v] * w.x - yr[v] * w.y);
+        }
+    }
+    __syncthreads();
+    // IS1: inverse over k2 at lane t -> x = t + 64 m
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const double2 a = B[fft64_ix(q * 64 + t)];
+        re[q] = a.x; im[q] = a.y;
+    }
+    dft_reg<4, +1>(re, im);
+#pragma unroll
+    for (int m = 0; m < 4; m++) {
+        re[m] = re[m] * stretchx;
+        im[m] = im[m] * stretchx;
+    }
+}
+
 template <int N1, int N2> struct FftLds { static constexpr int n = N2 * (N1 + 1); };
 template <> struct FftLds<32, 8> { static constexpr int n = 2 * 8 * 33; };
+template <> struct FftLds<64, 4> { static constexpr int n = 2 * 264; };
 
 // Factorization N1 x N2 of the supported row lengths (N1 >= N2).
 template <int NX> struct FftShape;

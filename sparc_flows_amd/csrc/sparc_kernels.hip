@@ -82,6 +82,8 @@ __global__ __launch_bounds__(64) void k_ddz_cols(const double *in, double *out,
     do {                                                                                       \
         if constexpr (N1 == 32 && N2 == 8)                                                     \
             fft_deriv_team_256x32(RE, IM, T, LDS, (LDS) + 8 * 33, TW, DM, SX);                 \
+        else if constexpr (N1 == 64 && N2 == 4)                                                \
+            fft_deriv_team_256x64(RE, IM, T, LDS, (LDS) + 264, TW, DM, SX);                    \
         else                                                                                   \
             fft_deriv_team<N1, N2>(RE, IM, T, LDS, TW, DM, SX);                                \
     } while (0)
@@ -121,8 +123,9 @@ __global__ __launch_bounds__(64) void k_ddx_rows(const double *__restrict__ in,
 
 
 // N1 lanes per row: FftShape's by default; nx = 256 also runs on 32 lanes
-// (two rows per wave, three-stage transform), which is faster when the
-// pass is latency-bound (one source) and slower when it is bandwidth-bound.
+// (two rows per wave, three-stage transform) or 64 lanes (one row per wave,
+// four radix-4 stages), which are faster when the pass is latency-bound (one
+// source) and slower when it is bandwidth-bound.
 template <int NX, int MODE, int N1 = FftShape<NX>::N1>
 __global__ __launch_bounds__(64) void k_row(Dims d, Bg bg, State st, RowArgs ra, Clock clk, Sink sink)
 {
@@ -453,6 +456,13 @@ hipError_t launch_ddx_rows(const double *in, double *out, int nrows, int nx, con
 hipError_t launch_row(int mode, const Dims &d, const Bg &bg, const State &s, const RowArgs &ra,
                       const Clock &c, const Sink &sk, hipStream_t st, int wide)
 {
+    if (wide == 2 && d.nx == 256) {
+        dim3 grid(d.S * d.nz);
+        if (mode == 0) hipLaunchKernelGGL((k_row<256, 0, 64>), grid, dim3(64), 0, st, d, bg, s, ra, c, sk);
+        else if (mode == 1) hipLaunchKernelGGL((k_row<256, 1, 64>), grid, dim3(64), 0, st, d, bg, s, ra, c, sk);
+        else hipLaunchKernelGGL((k_row<256, 2, 64>), grid, dim3(64), 0, st, d, bg, s, ra, c, sk);
+        return hipGetLastError();
+    }
     if (wide && d.nx == 256) {
         dim3 grid((d.S * d.nz + 1) / 2);
         if (mode == 0) hipLaunchKernelGGL((k_row<256, 0, 32>), grid, dim3(64), 0, st, d, bg, s, ra, c, sk);

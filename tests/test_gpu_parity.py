@@ -122,10 +122,10 @@ def test_steps_match_oracle(flow, dc, split, monkeypatch):
     assert rel_l2(px[:, 0].cpu().numpy(), opx) < 1e-10
 
 
-@pytest.mark.parametrize("wide", ["0", "1"])
+@pytest.mark.parametrize("wide", ["0", "1", "2"])
 def test_steps_match_oracle_256_columns(wide, monkeypatch):
-    """40 full steps at nx = 256 with both row-pass team widths (16 lanes per
-    row, or 32 lanes and the three-stage transform)."""
+    """40 full steps at nx = 256 with each row-pass team width: 16 lanes per
+    row, 32 lanes (three-stage transform), 64 lanes (four radix-4 stages)."""
     monkeypatch.setenv("SPARC_ROW_WIDE", wide)
     p, bg, c2d, psi = small_case(nx=256, nz=60, flow=True, dc=True)
     maxtime = 60
