@@ -13,7 +13,9 @@ accumulation) with all inputs resident in HBM.
 Prints one JSON line: value = Mpoint-steps/s (nx*nz*time-steps*(forward +
 adjoint)*sources / wall-s / 1e6), plus the batched 16-source gradient, the
 roofline of the dominant kernel (HIP events) and the CPU baseline (the C
-oracle on the host, one core, bounded sample).
+oracle on the host, one core, bounded sample).  other_configurations holds
+the plain and batched 2D forward runs and the 3D (option 16) forward and
+checkpointed gradient at 256x256x300; `--only-3d` prints just those two.
 """
 from __future__ import annotations
 
