@@ -130,7 +130,8 @@ int sparc3d_ddz(sparc3d_handle *h, const double *in, double *out, int nfields, i
 int sparc3d_query_int(const sparc3d_handle *h, const char *name, int *v);
 int sparc3d_query_double(const sparc3d_handle *h, const char *name, double *v);
 
-/* Instrumentation: run `reps` steps with HIP events around every kernel and
+/* Instrumentation: reset the state, run `reps` steps from t = 0 with HIP
+ * events around every kernel (the state is left after those steps) and
  * return the mean milliseconds per launch of each pass kind in ms_out[8]:
  * (unused), x-derivative, y-derivative, z pass of xi_z (div, p, rho'), z
  * pass of v_x, v_y, v_z and p (the RHS and the next stage's input), z filter

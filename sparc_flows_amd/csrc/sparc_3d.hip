@@ -1283,7 +1283,8 @@ int sparc3d_profile(sparc3d_handle *h, int reps, double *ms_out)
 {
     if (!h || reps < 1 || !ms_out) return SPARC_E_ARG;
     if (!h->have_src) return fail3(h, SPARC_E_STATE, "no source set");
-    HCHECK3(h, hipStreamSynchronize(h->stream));
+    int r0 = sparc3d_reset(h);
+    if (r0) return r0;
     h->profiling = true;
     int r = SPARC_OK;
     for (int t = 0; t < reps && !r; t++) r = enqueue_step3(h, t);

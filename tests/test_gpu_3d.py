@@ -228,3 +228,18 @@ def test_line_source_matches_oracle_and_is_y_invariant():
     a = s.state().cpu().numpy()
     assert np.abs(a - a[:, :, :1]).max() <= 1e-14 * np.abs(a).max()
     assert np.abs(a[1]).max() <= 1e-14 * np.abs(a).max() and np.abs(a[4]).max() <= 1e-14 * np.abs(a).max()
+
+
+def test_profile_times_every_pass():
+    p, bg, c3d, fl = case()
+    s = Sparc3D(p, bg, c3d=c3d)
+    s.set_forward_source(0.0, 0.0, 10)
+    ms = s.profile(2)
+    for k in ("ddx", "ddy", "z_p", "z_v", "filter"):
+        assert ms[k] > 0, k
+    # the profile starts from a zeroed state: two steps from rest match two plain steps
+    a = s.state()
+    s.reset()
+    s.step(0)
+    s.step(1)
+    assert torch.equal(a, s.state())
