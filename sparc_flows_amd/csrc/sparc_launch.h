@@ -9,7 +9,8 @@ hipError_t launch_ddz_cols(const double *in, double *out, int ncols, int nz, int
                            int filt, hipStream_t st);
 hipError_t launch_ddx_rows(const double *in, double *out, int nrows, int nx, const double2 *tw,
                            const double *eyekx, double stretchx, hipStream_t st);
-// Row pass; wide = 1 runs nx = 256 on 32 lanes per row (two rows per wave).
+// Row pass; at nx = 256 wide = 1 runs 32 lanes per row (two rows per wave),
+// wide = 2 runs 64 lanes per row (one row per wave).
 hipError_t launch_row(int mode, const Dims &d, const Bg &bg, const State &s, const RowArgs &ra,
                       const Clock &c, const Sink &sk, hipStream_t st, int wide);
 // Segmented column pass; tc in {1, 2, 4, 8, 16} columns per block, ms = rows
