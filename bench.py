@@ -209,7 +209,9 @@ def main():
                     help="skip the plain-forward (128x150, 500 steps) and batched 16-source forward "
                          "(128x150 with flow and sound-speed bump, 4000 steps) lines")
     ap.add_argument("--only-3d", action="store_true",
-                    help="only run the 3D forward line (256x256x300) and print it")
+                    help="only run the 3D forward and gradient lines (256x256x300) and print them")
+    ap.add_argument("--gradient3d-steps", type=int, default=450,
+                    help="time steps of the 3D checkpointed gradient line (configuration 5: 3000)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a plain device-to-device copy of this many MiB ten times "
                          "(calibrates the FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -235,7 +237,7 @@ def main():
         return
     if args.only_3d:
         print(json.dumps({"forward3d_256x256x300": forward3d_rate(),
-                          "gradient3d_256x256x300": gradient3d_rate()}))
+                          "gradient3d_256x256x300": gradient3d_rate(maxtime=args.gradient3d_steps - 1)}))
         return
     p, bg, psi = make_config()
     maxtime = args.nsteps - 1
@@ -316,7 +318,7 @@ def main():
         others = {"plain_forward_128x150": forward_rate(128, 150, 1, 500, False, False, reps=3),
                   "batched16_forward_128x150": forward_rate(128, 150, 16, 4000, True, True),
                   "forward3d_256x256x300": forward3d_rate(),
-                  "gradient3d_256x256x300": gradient3d_rate()}
+                  "gradient3d_256x256x300": gradient3d_rate(maxtime=args.gradient3d_steps - 1)}
         log("forward configurations done")
     cpu = None if args.no_cpu else cpu_baseline(p, bg, psi, args.cpu_seconds)
     log("cpu baseline done")
