@@ -639,7 +639,7 @@ static int setup3(sparc3d_handle *h)
     const size_t nxy = h->nxy, N = h->n;
     const double dimc = 1e6, dimrho = 1e-3, diml = kRsun3, dimc2 = std::pow(dimc, 2.0);
     // x, y spectral tables (initialize.f90:194-225): exp(-2 pi i k/n) and i k
-    auto tables = [&](int n, double length, const double2 **tw, const double **dm) -> int {
+    auto tables = [&](int n, const double2 **tw, const double **dm) -> int {
         std::vector<double> t(2 * (size_t)n), m(n);
         for (int k = 0; k < n; k++) {
             const double ang = -2.0 * kPi3 * (double)k / (double)n;
@@ -660,12 +660,11 @@ static int setup3(sparc3d_handle *h)
         *tw = (const double2 *)p;
         HCHECK3(h, upload(h->bg_mem, &p, m, h->stream));
         *dm = p;
-        (void)length;
         return SPARC_OK;
     };
     int r;
-    if ((r = tables(nx, h->xlength, &h->twx, &h->dmx))) return r;
-    if ((r = tables(ny, h->ylength, &h->twy, &h->dmy))) return r;
+    if ((r = tables(nx, &h->twx, &h->dmx))) return r;
+    if ((r = tables(ny, &h->twy, &h->dmy))) return r;
     h->stretchx = kRsun3 / h->xlength;
     h->stretchy = kRsun3 / h->ylength;
     h->cf = make_coef(nz);
@@ -816,8 +815,6 @@ static int setup3(sparc3d_handle *h)
     HCHECK3(h, hipMemsetAsync(h->scr, 0, 6 * N * sizeof(double), h->stream));
     HCHECK3(h, hipMemsetAsync(h->pml, 0, 2 * h->np * sizeof(double), h->stream));
     HCHECK3(h, hipStreamSynchronize(h->stream));
-    (void)nx;
-    (void)ny;
     return SPARC_OK;
 }
 
