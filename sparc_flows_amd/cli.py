@@ -13,6 +13,13 @@ initialize.f90:984-1144, DETERMINE_STATUS driver.f90:480-631):
 * ``params.i`` (working directory, or $SPARC_PARAMS) gives the grid and
   cadences; the data directory is $SPARC_DIRECTORY, else the ``directory``
   parameter of params.i; the model file is ``file_data``.
+* A params.i with ny > 1 runs the 3D solver (option 16, sparc_flows_amd.
+  solver3d) through the same three phases: the forward source is the
+  reference's vr (the x profile at x_src in every y row), vz_cc.fits and the
+  snapshots have axes (nx, ny, n), the adjoint reads adjoint_srcSS/source.fits
+  with axes (nx, ny, nt), and the kernel step writes kernel_c, kernel_vx,
+  kernel_vy, kernel_vz and hessian (PRODUCE_KERNELS with its y terms); no
+  adjoint source is computed from a 3D forward run.
 * Forward writes ``forward_srcSS_lsLL/{vz_cc.bin, vz_cc.fits, timeline,
   kernel_info, rms_hist}`` and the partial snapshots; adjoint writes
   ``adjoint_srcSS/{timeline, kernel_info}`` and its partial snapshots; the
@@ -58,6 +65,8 @@ def _label(t: int) -> str:
 
 FWD_FIELDS = ("xix", "xiz", "vx", "vz", "acc_x", "acc_z")
 ADJ_FIELDS = ("vx", "vz", "acc_x", "acc_z")
+FWD_FIELDS_3D = ("xix", "xiy", "xiz", "vx", "vy", "vz", "acc_x", "acc_y", "acc_z")
+ADJ_FIELDS_3D = ("vx", "vy", "vz")
 
 
 def main(argv=None) -> int:
@@ -75,6 +84,8 @@ def main(argv=None) -> int:
         model_path = cwd / model_path
     bg = B.read_model_file(model_path, p.nz)
     maxtime = _maxtime(p)
+    if p.ny > 1:
+        return _main3d(p, bg, ddir, contrib, jobno, maxtime, cwd)
     instr = cwd / f"Instruction_src{contrib}_ls{jobno}"
     compute_data = (cwd / "compute_data").exists()
     linesearch = (cwd / "linesearch").exists()
@@ -275,6 +286,141 @@ def _kernels(p, bg, c2d, ddir, contrib, jobno) -> int:
     for key, name in (("kc", "kernel_c"), ("kpsi", "kernel_psi"), ("kvx", "kernel_vx"),
                       ("kvz", "kernel_vz"), ("hess", "hessian")):
         write_fits(kdir / f"{name}_{contrib}.fits", K[key][0].cpu().numpy()[:, None, :])
+    for d in (fdir, adir):                               # kernels.f90:591-597
+        for f in d.glob("*partial*"):
+            f.unlink()
+    return 0
+
+
+# ---------------------------------------------------------------------------
+# 3D builds (ny > 1)
+
+def _params3d(p):
+    from .solver3d import Params3D
+    return Params3D(nx=p.nx, ny=p.ny, nz=p.nz, xlength=p.xlength, ylength=p.ylength or p.xlength,
+                    timestep=p.timestep, cadforcing=p.cadforcing, outputcad=p.outputcad, obsheight=p.obsheight,
+                    excitdep=p.excitdep, npmlbot=p.npmlbot, npmltop=p.npmltop, nupeak=p.nupeak, nuwidth=p.nuwidth)
+
+
+def _main3d(p, bg, ddir, contrib, jobno, maxtime, cwd) -> int:
+    instr = cwd / f"Instruction_src{contrib}_ls{jobno}"
+    q = _params3d(p)
+    c_path = ddir / f"model_c_ls{jobno}.fits"
+    c3d = None
+    if c_path.exists():
+        cm = read_fits(c_path).reshape(p.nz, p.ny, p.nx)
+        if cm.sum() != 0:
+            c3d = cm
+    flow = None
+    if (cwd / "compute_data").exists() and all((cwd / f"true_{k}.fits").exists() for k in ("vx", "vy", "vz")):
+        # true flows in m/s (driver.f90:65-73), here with their y component
+        flow = tuple(read_fits(cwd / f"true_{k}.fits").reshape(p.nz, p.ny, p.nx) * 1e2 for k in ("vx", "vy", "vz"))
+    if instr.exists():
+        lines = instr.read_text().split()
+        kind = lines[0].strip().lower() if lines else ""
+        instr.unlink()                                  # driver.f90:243
+        if kind == "spectral":
+            snaps = not any((cwd / f).exists() for f in ("linesearch", "compute_synth", "compute_data"))
+            return _forward3d(p, q, bg, c3d, flow, ddir, contrib, jobno, maxtime, snaps)
+        if kind == "adjoint":
+            return _adjoint3d(p, q, bg, c3d, flow, ddir, contrib, maxtime)
+        print(f"unknown instruction '{kind}'", file=sys.stderr)
+        return 1
+    return _kernels3d(q, bg, c3d, ddir, contrib, jobno)
+
+
+def _forward3d(p, q, bg, c3d, flow, ddir, contrib, jobno, maxtime, snapshots) -> int:
+    import torch
+
+    from .solver3d import Sparc3D
+
+    out = ddir / f"forward_src{contrib}_ls{jobno}"
+    out.mkdir(parents=True, exist_ok=True)
+    s = Sparc3D(q, bg, c3d=c3d)
+    if flow is not None:
+        s.set_flow(*flow)
+    s.set_forward_source(_xsrc(ddir, contrib), None, maxtime)
+    s.reset()
+    steps = q.steps
+    nt = maxtime // steps + 1
+    vz = torch.zeros((nt, q.ny, q.nx), dtype=torch.float64, device=s.device)
+    for t in range(maxtime + 1):
+        s.run_range(t, t, vz)
+        if snapshots and t % steps == 0:
+            a = s.snapshot().cpu().numpy()
+            for fi, name in enumerate(FWD_FIELDS_3D):
+                write_fits(out / f"{name}_{_label(t)}_partial.fits", a[fi], time=t)
+    v = vz.cpu().numpy()
+    write_direct_records(out / "vz_cc.bin", v.reshape(nt, -1))
+    write_fits(out / "vz_cc.fits", v, time=maxtime)
+    _timeline(out, q, maxtime, -p.solartime * 3600.0)
+    with open(out / "kernel_info", "w") as f:
+        f.write(f"{0:12d}\n{maxtime:12d}\n")
+    print("3D forward: no adjoint source computed (ADJOINT_SOURCE_FILT is 2D)")
+    return 0
+
+
+def _timeline(out, q, maxtime, start) -> None:
+    timeline = start
+    with open(out / "timeline", "w") as f:
+        for t in range(0, maxtime + 1):
+            timeline += q.timestep
+            if t % q.steps == 0:
+                f.write(f"{t:12d} {timeline:24.15E}\n")
+
+
+def _adjoint3d(p, q, bg, c3d, flow, ddir, contrib, maxtime) -> int:
+    import torch
+
+    from .solver3d import Sparc3D
+
+    out = ddir / f"adjoint_src{contrib}"
+    out.mkdir(parents=True, exist_ok=True)
+    cfs = int(q.cadforcing // q.timestep)
+    fl = maxtime // cfs + 1
+    vr = read_fits(out / "source.fits").reshape(-1, q.ny, q.nx)[:fl]
+    s = Sparc3D(q, bg, c3d=c3d, adjoint=True)
+    if flow is not None:
+        s.set_flow(*flow)
+    s.set_adjoint_source(torch.as_tensor(np.ascontiguousarray(vr), device=s.device), maxtime)
+    s.reset()
+    for t in range(maxtime + 1):
+        s.run_range(t, t)
+        if t % q.steps == 0:
+            a = s.snapshot().cpu().numpy()
+            for fi, name in enumerate(ADJ_FIELDS_3D):
+                write_fits(out / f"{name}_{_label(t)}_partial.fits", a[fi], time=t)
+    _timeline(out, q, maxtime, p.solartime * 3600.0 - (fl - 1.0) * cfs * q.timestep)
+    with open(out / "kernel_info", "w") as f:            # driver.f90:283-292
+        f.write(f"{maxtime // cfs + 1:12d}\n{(maxtime // cfs) * cfs:12d}\n")
+    return 0
+
+
+def _kernels3d(q, bg, c3d, ddir, contrib, jobno) -> int:
+    import torch
+
+    from .solver3d import Sparc3D
+
+    fdir = ddir / f"forward_src{contrib}_ls{jobno}"
+    adir = ddir / f"adjoint_src{contrib}"
+    nt_kern, adj_end = [int(v) for v in (adir / "kernel_info").read_text().split()[:2]]
+    fwd_start = int((fdir / "kernel_info").read_text().split()[0])
+    steps = q.steps
+    s = Sparc3D(q, bg, c3d=c3d, adjoint=True)
+    s.kernels_begin()
+    shape = (q.nz, q.ny, q.nx)
+    for k in range(nt_kern):                             # kernels.f90:215, 256: reversed-time adjoint
+        lf = _label(k * steps + fwd_start)
+        la = _label(adj_end - k * steps)
+        f = np.stack([read_fits(fdir / f"{n}_{lf}_partial.fits").reshape(shape) for n in FWD_FIELDS_3D])
+        a = np.stack([read_fits(adir / f"{n}_{la}_partial.fits").reshape(shape) for n in ADJ_FIELDS_3D])
+        s.kernels_add(torch.as_tensor(f, device=s.device), torch.as_tensor(a, device=s.device))
+    K = s.kernels_end()
+    kdir = ddir / "kernel"
+    kdir.mkdir(parents=True, exist_ok=True)
+    for key, name in (("kc", "kernel_c"), ("kvx", "kernel_vx"), ("kvy", "kernel_vy"), ("kvz", "kernel_vz"),
+                      ("hess", "hessian")):
+        write_fits(kdir / f"{name}_{contrib}.fits", K[key].cpu().numpy())
     for d in (fdir, adir):                               # kernels.f90:591-597
         for f in d.glob("*partial*"):
             f.unlink()

@@ -29,7 +29,9 @@ def f32(v: float) -> float:
 class SparcParams:
     nx: int = 512
     nz: int = 300
+    ny: int = 1                       # > 1: a 3D build (option 16, sparc_flows_amd.solver3d)
     xlength: float = 800.0e8          # cm (params.i:28)
+    ylength: float = 0.0              # cm; 0: = xlength (params.i:28)
     timestep: float = 2.0             # s
     cadforcing: float = 30.0          # s
     outputcad: float = 30.0           # s
@@ -106,7 +108,7 @@ def read_params_i(path: str | Path) -> SparcParams:
                 k, v = part.split("=", 1)
                 vals[k.strip().lower()] = v.strip()
     p = SparcParams()
-    ints = {"nx": "nx", "nz": "nz", "npmlbot": "npmlbot", "npmltop": "npmltop", "npmlhor": "npmlhor"}
+    ints = {"nx": "nx", "ny": "ny", "nz": "nz", "npmlbot": "npmlbot", "npmltop": "npmltop", "npmlhor": "npmlhor"}
     for key, attr in ints.items():
         if key in vals:
             setattr(p, attr, int(vals[key]))
@@ -117,6 +119,8 @@ def read_params_i(path: str | Path) -> SparcParams:
             setattr(p, attr, _fortran_number(vals[key]))
     if "xlength" in vals:
         p.xlength = _fortran_number(vals["xlength"].split(",")[0])
+    if "ylength" in vals and vals["ylength"].strip().lower() != "xlength":
+        p.ylength = _fortran_number(vals["ylength"])
     for key in ("nupeak", "nuwidth"):
         if key in vals:
             setattr(p, key, f32(_fortran_number(vals[key])))
@@ -135,8 +139,9 @@ def write_params_i(p: SparcParams, path: str | Path, directory: str = "") -> Non
     lines = [
         f"character (LEN = *), PARAMETER :: directory = '{directory}'",
         f"character (LEN = *), PARAMETER :: file_data = '{p.file_data}'",
-        f"parameter (nx = {p.nx}, ny = 1, nz = {p.nz})",
-        f"parameter (xlength = {d(p.xlength)}, ylength = xlength, timestep = {d(p.timestep)})",
+        f"parameter (nx = {p.nx}, ny = {p.ny}, nz = {p.nz})",
+        f"parameter (xlength = {d(p.xlength)}, ylength = "
+        f"{'xlength' if not p.ylength or p.ylength == p.xlength else d(p.ylength)}, timestep = {d(p.timestep)})",
         f"parameter( cadforcing = {d(p.cadforcing)})",
         f"parameter(nupeak={repr(float(p.nupeak))}, nuwidth = {repr(float(p.nuwidth))})",
         f"parameter (npmlhor = {p.npmlhor})",

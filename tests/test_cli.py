@@ -2,9 +2,11 @@
 line, instruction and flag files, FITS outputs and exit code.
 
 The CPU tests cover the parameter file and the argument/instruction
-handling; the GPU test runs the three phases of full.py:17-61 (forward,
-adjoint, kernel) as child processes in a scratch directory and compares the
-files they write with the same computation through the Python API.
+handling; the GPU tests run the three phases of full.py:17-61 (forward,
+adjoint, kernel) as child processes in a scratch directory, for a 2D and a
+3D params.i, and compare the files they write with the same computation
+through the Python API (whose kernel call pairs forward output k with
+adjoint output nt-1-k, as kernels.f90:215, 256 read the partial files).
 """
 import os
 import subprocess
@@ -39,6 +41,16 @@ def _setup(tmp: Path, nx=64, nz=60):
 def test_params_file_round_trip(tmp_path):
     p, _, _, _ = _setup(tmp_path)
     assert read_params_i(tmp_path / "params.i") == p
+
+
+def test_params_file_round_trip_3d(tmp_path):
+    p = SparcParams(nx=32, ny=16, nz=40, xlength=100e8, ylength=50e8, npmlbot=6, npmltop=6)
+    write_params_i(p, tmp_path / "params.i", str(tmp_path) + "/")
+    assert read_params_i(tmp_path / "params.i") == p
+    q = SparcParams(nx=32, ny=32, nz=40)                 # ylength = xlength, written symbolically
+    write_params_i(q, tmp_path / "params.i", str(tmp_path) + "/")
+    assert "ylength = xlength" in (tmp_path / "params.i").read_text()
+    assert read_params_i(tmp_path / "params.i") == q
 
 
 def test_usage_and_unknown_instruction(tmp_path, monkeypatch, capsys):
@@ -118,3 +130,77 @@ def test_forward_adjoint_kernel_phases(tmp_path):
         ref = ref[0].cpu().numpy()
         assert np.abs(ref).max() > 0
         assert np.linalg.norm(got - ref) <= 1e-12 * np.linalg.norm(ref)
+
+
+@pytest.mark.gpu
+def test_forward_adjoint_kernel_phases_3d(tmp_path):
+    """A params.i with ny > 1: the three phases on the 3D solver (line source
+    at master.pixels' x, a 3D model_c, a given adjoint source.fits) against
+    the same runs through sparc_flows_amd.solver3d."""
+    torch = pytest.importorskip("torch")
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from sparc_flows_amd.solver3d import Params3D, Sparc3D
+    nx, ny, nz = 32, 16, 40
+    bg = B.polytrope(nz)
+    p = SparcParams(nx=nx, ny=ny, nz=nz, xlength=100e8, ylength=50e8, timestep=B.stable_timestep(bg),
+                    npmlbot=6, npmltop=6, file_data="poly.dat")
+    data = tmp_path / "data"
+    data.mkdir()
+    B.write_model_file(tmp_path / "poly.dat", bg)
+    write_params_i(p, tmp_path / "params.i", str(data) + "/")
+    (data / "master.pixels").write_text("-12.0\n")
+    x = (np.arange(nx) / nx - 0.5)[None, None, :]
+    y = (np.arange(ny) / ny - 0.5)[None, :, None]
+    zz = (np.arange(nz) / nz)[:, None, None]
+    c3d = bg[:, 1][:, None, None] * (1 + 0.05 * np.exp(-(x ** 2 + y ** 2) / 0.03 - (zz - 0.7) ** 2 / 0.01))
+    write_fits(data / "model_c_ls00.fits", c3d)
+    q = Params3D(nx=nx, ny=ny, nz=nz, xlength=100e8, ylength=50e8, timestep=p.timestep, npmlbot=6, npmltop=6,
+                 cadforcing=p.cadforcing, outputcad=p.outputcad, obsheight=p.obsheight, excitdep=p.excitdep,
+                 nupeak=p.nupeak, nuwidth=p.nuwidth)
+    steps = q.steps
+    maxtime = 12 * steps
+    # forward
+    (tmp_path / "Instruction_src01_ls00").write_text("spectral\n")
+    assert _cli(tmp_path, data, maxtime, "01", "00") == 0
+    fdir = data / "forward_src01_ls00"
+    vz = read_fits(fdir / "vz_cc.fits")
+    s = Sparc3D(q, bg, c3d=c3d)
+    vz_ref = s.forward(-12.0, None, maxtime).cpu().numpy()
+    assert vz.shape == (maxtime // steps + 1, ny, nx) and np.abs(vz_ref).max() > 0
+    assert np.array_equal(vz, vz_ref)
+    assert np.array_equal(np.fromfile(fdir / "vz_cc.bin", dtype="<f8").reshape(vz.shape), vz)
+    # adjoint from a given source.fits, (nx, ny, nt) at the forcing cadence
+    cfs = int(q.cadforcing // q.timestep)
+    fl = maxtime // cfs + 1
+    rng = np.random.default_rng(3)
+    vr = rng.standard_normal((fl, ny, nx)) * np.exp(-(x ** 2) / 0.02)[0]
+    adir = data / "adjoint_src01"
+    adir.mkdir()
+    write_fits(adir / "source.fits", vr)
+    (tmp_path / "Instruction_src01_ls00").write_text("adjoint\n")
+    assert _cli(tmp_path, data, maxtime, "01", "00") == 0
+    assert (adir / "kernel_info").exists() and (adir / f"vy_{0:06d}_partial.fits").exists()
+    # kernels
+    assert _cli(tmp_path, data, maxtime, "01", "00") == 0
+    assert not list(fdir.glob("*partial*")) and not list(adir.glob("*partial*"))
+    # the same through the Python API: snapshots every `steps` steps
+    fs, asn = [], []
+    s.set_forward_source(-12.0, None, maxtime)
+    s.reset()
+    sa = Sparc3D(q, bg, c3d=c3d, adjoint=True)
+    sa.set_adjoint_source(torch.as_tensor(vr, device="cuda"), maxtime)
+    sa.reset()
+    for t in range(maxtime + 1):
+        s.run_range(t, t)
+        sa.run_range(t, t)
+        if t % steps == 0:
+            fs.append(s.snapshot())
+            asn.append(sa.snapshot())
+    K = sa.kernels(torch.stack(fs), torch.stack(asn))
+    for key, name in (("kc", "kernel_c"), ("kvx", "kernel_vx"), ("kvy", "kernel_vy"), ("kvz", "kernel_vz"),
+                      ("hess", "hessian")):
+        got = read_fits(data / "kernel" / f"{name}_01.fits")
+        ref = K[key].cpu().numpy()
+        assert np.abs(ref).max() > 0, key
+        assert np.array_equal(got, ref), key
