@@ -154,8 +154,9 @@ def cpu_baseline_3d(p, bg, seconds=8.0, nxy=64):
 
 def gradient3d_rate(maxtime=449, interval=150, nx=256, ny=256, nz=300):
     """3D forward + adjoint + kernels of one source at 256x256x300 with
-    in-HBM checkpoints every `interval` steps and recompute (two forward runs,
-    one adjoint run, one kernel accumulation per output pair)."""
+    in-HBM checkpoints every `interval` steps and recompute (a forward run, a
+    second run of every segment but the last, one adjoint run, one kernel
+    accumulation per output pair)."""
     import torch
     from sparc_flows_amd import background as B
     from sparc_flows_amd.solver3d import Gradient3D, Params3D

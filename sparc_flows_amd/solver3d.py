@@ -342,8 +342,10 @@ class Gradient3D:
     steps in HBM; the adjoint run then meets each forward snapshot k (time
     k*steps) at its own output nt-1-k (kernels.f90:215, 256), recomputing the
     forward segments last to first and holding one segment's snapshots at a
-    time.  Cost: two forward runs and one adjoint run; memory: the checkpoints
-    plus one segment of snapshots instead of every snapshot of the run.
+    time.  The last segment keeps its snapshots from the forward run itself,
+    so it is never recomputed.  Cost: one forward run, a second run of every
+    segment but the last, and one adjoint run; memory: the checkpoints plus one
+    segment of snapshots instead of every snapshot of the run.
 
     The adjoint source is the windowed, time-reversed vz residual over the
     receivers 5-60 Mm from the source (gradient.waveform_adjoint_source), which
@@ -374,9 +376,16 @@ class Gradient3D:
         fwd.reset()
         vz = fwd._zeros(nt, p.ny, p.nx)
         ckpt = {}
-        for t0 in range(0, maxtime + 1, I):
+        last = (maxtime // I) * I
+        kept = {}
+        for t0 in range(0, last, I):
             ckpt[t0] = fwd.save_checkpoint()
-            fwd.run_range(t0, min(t0 + I, maxtime + 1) - 1, vz)
+            fwd.run_range(t0, t0 + I - 1, vz)
+        for t in range(last, maxtime + 1):                # the last segment, snapshots kept
+            fwd.run_range(t, t, vz)
+            if t % steps == 0:
+                kept[t // steps] = fwd.snapshot()
+        ckpt[last] = None
         # adjoint source
         mask = receiver_mask_3d(p, xsrc, ysrc)[None]
         misfit, vr = waveform_adjoint_source(vz.reshape(1, nt, -1), None if vz_obs is None else
@@ -389,13 +398,15 @@ class Gradient3D:
         asnap = None
         t_adj = 0
         for t0 in sorted(ckpt, reverse=True):
-            t1 = min(t0 + I, maxtime + 1) - 1
-            fwd.load_checkpoint(ckpt[t0])
-            snaps = {}
-            for t in range(t0, t1 + 1):
-                fwd.step(t)
-                if t % steps == 0:
-                    snaps[t // steps] = fwd.snapshot()
+            if ckpt[t0] is None:
+                snaps, kept = kept, None
+            else:
+                fwd.load_checkpoint(ckpt[t0])
+                snaps = {}
+                for t in range(t0, t0 + I):
+                    fwd.step(t)
+                    if t % steps == 0:
+                        snaps[t // steps] = fwd.snapshot()
             for k in sorted(snaps, reverse=True):
                 ta = (nt - 1 - k) * steps
                 if ta >= t_adj:
