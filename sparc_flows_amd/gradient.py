@@ -77,29 +77,44 @@ class GradientPipeline:
         return K
 
 
-def forward_with_checkpoints(fwd: Sparc, xsrc, maxtime: int, interval: int):
+def forward_with_checkpoints(fwd: Sparc, xsrc, maxtime: int, interval: int, keep_last: bool = False):
     """Forward run keeping the vz traces and a full-state checkpoint every
     `interval` steps (checkpoint j holds the state at time index j*interval).
-    Returns (vz [nsrc][nt][nx], checkpoints [nseg][numel])."""
+    Returns (vz [nsrc][nt][nx], checkpoints [nseg][numel]); with keep_last,
+    (vz, checkpoints, last) where `last` holds the snapshots of the last
+    segment's outputs [nsrc][n][6][nz][nx] (None if it has none) and that
+    segment's checkpoint is not written: it need never be recomputed."""
     steps = fwd.steps
+    p = fwd.params
     nt = maxtime // steps + 1
     nseg = (maxtime + 1 + interval - 1) // interval
     fwd.set_forward_sources(xsrc, maxtime)
     fwd.reset()
     ck = fwd.checkpoint_buffer(nseg)
-    vz = torch.zeros(fwd.nsrc, nt, fwd.params.nx, dtype=torch.float64, device=fwd.device)
+    vz = torch.zeros(fwd.nsrc, nt, p.nx, dtype=torch.float64, device=fwd.device)
+    last = None
     for j in range(nseg):
+        t0, t1 = j * interval, min(maxtime, j * interval + interval - 1)
+        ka, kb = math.ceil(t0 / steps), t1 // steps + 1
+        if keep_last and j == nseg - 1 and kb > ka:
+            last = torch.empty(fwd.nsrc, kb - ka, 6, p.nz, p.nx, dtype=torch.float64, device=fwd.device)
+            vzl = torch.zeros(fwd.nsrc, kb - ka, p.nx, dtype=torch.float64, device=fwd.device)
+            fwd.run_range(t0, t1, vz=vzl, snaps=last, k0=ka)
+            vz[:, ka:kb] = vzl
+            break
         fwd.save_checkpoint(ck[j])
-        fwd.run_range(j * interval, min(maxtime, j * interval + interval - 1), vz=vz, k0=0)
-    return vz, ck
+        fwd.run_range(t0, t1, vz=vz, k0=0)
+    return (vz, ck, last) if keep_last else (vz, ck)
 
 
 def kernels_from_checkpoints(fwd: Sparc, adj: Sparc, ck: torch.Tensor, vr: torch.Tensor, maxtime: int,
-                             interval: int) -> dict:
+                             interval: int, last: torch.Tensor | None = None) -> dict:
     """Adjoint run driven by vr with the forward snapshots recomputed segment
     by segment, in reverse, from the checkpoints; forward output k is
     correlated with adjoint output nt-1-k (kernels.f90:200-360) as soon as
-    both exist.  Returns the kernels of sparc_kernels_end."""
+    both exist.  `last`: the last segment's snapshots kept by
+    forward_with_checkpoints(keep_last=True), used instead of a recompute.
+    Returns the kernels of sparc_kernels_end."""
     p, C, steps = fwd.params, interval, fwd.steps
     nt = maxtime // steps + 1
     nseg = ck.shape[0]
@@ -117,9 +132,12 @@ def kernels_from_checkpoints(fwd: Sparc, adj: Sparc, ck: torch.Tensor, vr: torch
         n = kb - ka
         if n <= 0:
             continue
-        fs = fbuf[:, :n].contiguous() if n < kmax else fbuf
-        fwd.load_checkpoint(ck[j])
-        fwd.run_range(t0, t1, snaps=fs, k0=ka)
+        if j == nseg - 1 and last is not None:
+            fs = last
+        else:
+            fs = fbuf[:, :n].contiguous() if n < kmax else fbuf
+            fwd.load_checkpoint(ck[j])
+            fwd.run_range(t0, t1, snaps=fs, k0=ka)
         tend = (nt - 1 - ka) * steps                        # adjoint outputs nt-kb .. nt-1-ka
         as_ = abuf[:, :n].contiguous() if n < kmax else abuf
         adj.run_range(ta, tend, snaps=as_, k0=nt - kb)
@@ -137,7 +155,8 @@ class CheckpointedGradient(GradientPipeline):
     kernels.f90:200-280).  Here the forward run keeps only a full-state
     checkpoint every `interval` steps (plus the vz traces); the adjoint then
     advances in its own time while the forward segments are recomputed from
-    their checkpoints in reverse order, and each recomputed segment's
+    their checkpoints in reverse order (all but the last, whose snapshots the
+    forward run keeps), and each segment's
     snapshots are correlated with the matching adjoint snapshots
     (forward output k with adjoint output nt-1-k) as soon as both exist.
     Forward snapshot memory drops from nt to one segment's worth.
@@ -151,10 +170,10 @@ class CheckpointedGradient(GradientPipeline):
         self.interval = int(interval)
 
     def run(self, xsrc, maxtime: int, vz_obs: torch.Tensor | None = None) -> dict:
-        vz, ck = forward_with_checkpoints(self.fwd, xsrc, maxtime, self.interval)
+        vz, ck, last = forward_with_checkpoints(self.fwd, xsrc, maxtime, self.interval, keep_last=True)
         mask = receiver_mask(self.p, xsrc)
         misfit, vr = waveform_adjoint_source(vz, vz_obs, mask, self.window_s, self.fwd.steps * self.p.timestep)
-        K = kernels_from_checkpoints(self.fwd, self.adj, ck, vr, maxtime, self.interval)
+        K = kernels_from_checkpoints(self.fwd, self.adj, ck, vr, maxtime, self.interval, last)
         K["vz"] = vz
         K["misfit"] = misfit
         return K
