@@ -152,11 +152,11 @@ def cpu_baseline_3d(p, bg, seconds=8.0, nxy=64):
                       f"{dt:.1f} s, numpy oracle"}
 
 
-def gradient3d_rate(maxtime=449, interval=150, nx=256, ny=256, nz=300):
+def gradient3d_rate(maxtime=449, interval="auto", nx=256, ny=256, nz=300):
     """3D forward + adjoint + kernels of one source at 256x256x300 with
-    in-HBM checkpoints every `interval` steps and recompute (a forward run, a
-    second run of every segment but the last, one adjoint run, one kernel
-    accumulation per output pair)."""
+    in-HBM checkpoint-and-recompute (a forward run, a second run of all but
+    the last `interval` steps, one adjoint run, one kernel accumulation per
+    output pair); "auto" sizes the interval to half the free HBM."""
     import torch
     from sparc_flows_amd import background as B
     from sparc_flows_amd.solver3d import Gradient3D, Params3D
@@ -174,7 +174,7 @@ def gradient3d_rate(maxtime=449, interval=150, nx=256, ny=256, nz=300):
     torch.cuda.synchronize()
     sec = time.perf_counter() - t0
     finite = bool(all(torch.isfinite(K[k]).all() for k in ("kc", "kvx", "kvy", "kvz")))
-    return {"s_per_gradient": sec, "time_steps": maxtime + 1, "checkpoint_interval": interval,
+    return {"s_per_gradient": sec, "time_steps": maxtime + 1, "checkpoint_interval": K["interval"],
             "grid": [nx, ny, nz], "output_pairs": maxtime // p.steps + 1, "finite": finite,
             "Mpoint_steps_per_s_fwd_adj": nx * ny * nz * (maxtime + 1) * 2 / sec / 1e6}
 

@@ -338,25 +338,31 @@ def receiver_mask_3d(p: Params3D, xsrc: float, ysrc, dmin_mm: float = 5.0, dmax_
 
 class Gradient3D:
     """Forward + misfit + adjoint + kernels of one 3D source with fixed-interval
-    checkpoint-and-recompute: the forward run keeps its state every `interval`
-    steps in HBM; the adjoint run then meets each forward snapshot k (time
-    k*steps) at its own output nt-1-k (kernels.f90:215, 256), recomputing the
-    forward segments last to first and holding one segment's snapshots at a
-    time.  The last segment keeps its snapshots from the forward run itself,
-    so it is never recomputed.  Cost: one forward run, a second run of every
-    segment but the last, and one adjoint run; memory: the checkpoints plus one
-    segment of snapshots instead of every snapshot of the run.
+    checkpoint-and-recompute: the run is cut into segments of `interval` steps
+    counted back from its end (the first segment takes the remainder); the
+    forward run keeps its state at each segment start in HBM and the last
+    segment's snapshots themselves.  The adjoint run then meets each forward
+    snapshot k (time k*steps) at its own output nt-1-k (kernels.f90:215, 256),
+    taking the kept snapshots first and then recomputing the earlier segments
+    last to first, holding one segment's snapshots at a time.  Cost: one
+    forward run, a second run of all but the last `interval` steps, and one
+    adjoint run; memory: the checkpoints plus one segment of snapshots.
+    interval "auto" (auto_interval) sizes the segment to the free HBM, so that
+    a run whose snapshots all fit is never recomputed.
 
     The adjoint source is the windowed, time-reversed vz residual over the
     receivers 5-60 Mm from the source (gradient.waveform_adjoint_source), which
     needs outputcad == cadforcing."""
 
-    def __init__(self, params: Params3D, model, c3d=None, flow=None, interval: int = 150,
-                 window_s: float = 1800.0):
+    def __init__(self, params: Params3D, model, c3d=None, flow=None, interval: int | str = "auto",
+                 window_s: float = 1800.0, hbm_fraction: float = 0.5):
         if abs(params.outputcad - params.cadforcing) > 1e-12:
             raise ValueError("the adjoint source from vz needs outputcad == cadforcing")
         self.p = params
-        self.interval = int(interval)
+        self.interval = interval if interval == "auto" else int(interval)
+        if self.interval != "auto" and self.interval < 1:
+            raise ValueError("checkpoint interval must be positive")
+        self.hbm_fraction = float(hbm_fraction)
         self.window_s = window_s
         self.fwd = Sparc3D(params, model, c3d=c3d)
         self.adj = Sparc3D(params, model, c3d=c3d, adjoint=True)
@@ -364,23 +370,40 @@ class Gradient3D:
             self.fwd.set_flow(*flow)
             self.adj.set_flow(*flow)
 
+    def auto_interval(self, maxtime: int) -> int:
+        """The longest segment whose forward snapshots (9 fields each) fit in
+        hbm_fraction of the free device memory, in whole output cadences; the
+        whole run (no recompute) when all of them fit."""
+        p = self.p
+        free, _ = torch.cuda.mem_get_info(self.fwd.device)
+        per = 9 * p.nz * p.ny * p.nx * 8
+        nsnap = max(1, int(self.hbm_fraction * free // per))
+        if nsnap >= p.nt_out(maxtime):
+            return maxtime + 1
+        return nsnap * p.steps
+
     def run(self, xsrc: float, ysrc, maxtime: int, vz_obs: torch.Tensor | None = None) -> dict:
         from .gradient import waveform_adjoint_source
 
-        p, I = self.p, self.interval
+        p = self.p
         steps = p.steps
+        I = self.interval if self.interval != "auto" else self.auto_interval(maxtime)
         nt = maxtime // steps + 1
         fwd, adj = self.fwd, self.adj
-        # forward with checkpoints at t0 = 0, I, 2I, ... (the state before step t0)
         fwd.set_forward_source(xsrc, ysrc, maxtime)
         fwd.reset()
         vz = fwd._zeros(nt, p.ny, p.nx)
+        # segments aligned on the end of the run: the last one, [last, maxtime],
+        # is a whole interval and keeps its snapshots; the ones before it start
+        # at 0 (the remainder) and at last - j*I
+        last = max(0, maxtime + 1 - I)
+        starts = sorted({0} | set(range(last, 0, -I)))
+        ends = {t0: t1 - 1 for t0, t1 in zip(starts, starts[1:] + [maxtime + 1])}
         ckpt = {}
-        last = (maxtime // I) * I
         kept = {}
-        for t0 in range(0, last, I):
+        for t0 in starts[:-1]:
             ckpt[t0] = fwd.save_checkpoint()
-            fwd.run_range(t0, t0 + I - 1, vz)
+            fwd.run_range(t0, ends[t0], vz)
         for t in range(last, maxtime + 1):                # the last segment, snapshots kept
             fwd.run_range(t, t, vz)
             if t % steps == 0:
@@ -403,7 +426,7 @@ class Gradient3D:
             else:
                 fwd.load_checkpoint(ckpt[t0])
                 snaps = {}
-                for t in range(t0, t0 + I):
+                for t in range(t0, ends[t0] + 1):
                     fwd.step(t)
                     if t % steps == 0:
                         snaps[t // steps] = fwd.snapshot()
@@ -418,4 +441,5 @@ class Gradient3D:
         K = adj.kernels_end()
         K["vz"] = vz
         K["misfit"] = misfit
+        K["interval"] = I
         return K

@@ -208,10 +208,15 @@ def test_checkpointed_gradient_matches_oracle():
         e = rel_l2(K[k].cpu().numpy(), ref[k])
         print("3D gradient", k, e)
         assert np.abs(ref[k]).max() > 0 and e < 1e-9, k
-    # the same run with one segment (every snapshot held) is bit for bit equal
-    K1 = Gradient3D(p, bg, c3d=c3d, flow=fl, interval=maxtime + 1, window_s=window).run(5.0, -3.0, maxtime)
-    for k in ("kc", "kvx", "kvy", "kvz", "hess"):
-        assert torch.equal(K1[k], K[k]), k
+    # the same run with one segment (every snapshot held), with segments that
+    # do not fall on output times (35 steps; a remainder segment at the start)
+    # and with the HBM-sized interval (one segment here) is bit for bit equal
+    for interval in (maxtime + 1, 35, "auto"):
+        K1 = Gradient3D(p, bg, c3d=c3d, flow=fl, interval=interval, window_s=window).run(5.0, -3.0, maxtime)
+        if interval == "auto":
+            assert K1["interval"] == maxtime + 1
+        for k in ("kc", "kvx", "kvy", "kvz", "hess"):
+            assert torch.equal(K1[k], K[k]), (interval, k)
 
 
 def test_line_source_matches_oracle_and_is_y_invariant():
