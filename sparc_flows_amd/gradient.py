@@ -77,9 +77,19 @@ class GradientPipeline:
         return K
 
 
+def checkpoint_segments(maxtime: int, interval: int) -> list:
+    """Segments (t0, t1) of time indices 0..maxtime, `interval` steps each
+    counted back from maxtime, so that the last one is whole and the first
+    one takes the remainder."""
+    last = max(0, maxtime + 1 - interval)
+    starts = sorted({0} | set(range(last, 0, -interval)))
+    return [(t0, t1 - 1) for t0, t1 in zip(starts, starts[1:] + [maxtime + 1])]
+
+
 def forward_with_checkpoints(fwd: Sparc, xsrc, maxtime: int, interval: int, keep_last: bool = False):
-    """Forward run keeping the vz traces and a full-state checkpoint every
-    `interval` steps (checkpoint j holds the state at time index j*interval).
+    """Forward run keeping the vz traces and a full-state checkpoint at the
+    start of each segment of checkpoint_segments (checkpoint j holds the state
+    at the start time of segment j).
     Returns (vz [nsrc][nt][nx], checkpoints [nseg][numel]); with keep_last,
     (vz, checkpoints, last) where `last` holds the snapshots of the last
     segment's outputs [nsrc][n][6][nz][nx] (None if it has none) and that
@@ -87,14 +97,14 @@ def forward_with_checkpoints(fwd: Sparc, xsrc, maxtime: int, interval: int, keep
     steps = fwd.steps
     p = fwd.params
     nt = maxtime // steps + 1
-    nseg = (maxtime + 1 + interval - 1) // interval
+    segs = checkpoint_segments(maxtime, interval)
+    nseg = len(segs)
     fwd.set_forward_sources(xsrc, maxtime)
     fwd.reset()
     ck = fwd.checkpoint_buffer(nseg)
     vz = torch.zeros(fwd.nsrc, nt, p.nx, dtype=torch.float64, device=fwd.device)
     last = None
-    for j in range(nseg):
-        t0, t1 = j * interval, min(maxtime, j * interval + interval - 1)
+    for j, (t0, t1) in enumerate(segs):
         ka, kb = math.ceil(t0 / steps), t1 // steps + 1
         if keep_last and j == nseg - 1 and kb > ka:
             last = torch.empty(fwd.nsrc, kb - ka, 6, p.nz, p.nx, dtype=torch.float64, device=fwd.device)
@@ -126,8 +136,11 @@ def kernels_from_checkpoints(fwd: Sparc, adj: Sparc, ck: torch.Tensor, vr: torch
     kmax = (C + steps - 1) // steps + 1
     fbuf = torch.empty(fwd.nsrc, kmax, 6, p.nz, p.nx, dtype=torch.float64, device=dev)
     abuf = torch.empty(fwd.nsrc, kmax, 4, p.nz, p.nx, dtype=torch.float64, device=dev)
+    segs = checkpoint_segments(maxtime, C)
+    if len(segs) != nseg:
+        raise ValueError("checkpoints and interval do not match")
     for j in reversed(range(nseg)):
-        t0, t1 = j * C, min(maxtime, j * C + C - 1)
+        t0, t1 = segs[j]
         ka, kb = math.ceil(t0 / steps), t1 // steps + 1      # forward outputs in this segment
         n = kb - ka
         if n <= 0:

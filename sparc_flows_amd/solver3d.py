@@ -383,7 +383,7 @@ class Gradient3D:
         return nsnap * p.steps
 
     def run(self, xsrc: float, ysrc, maxtime: int, vz_obs: torch.Tensor | None = None) -> dict:
-        from .gradient import waveform_adjoint_source
+        from .gradient import checkpoint_segments, waveform_adjoint_source
 
         p = self.p
         steps = p.steps
@@ -393,12 +393,11 @@ class Gradient3D:
         fwd.set_forward_source(xsrc, ysrc, maxtime)
         fwd.reset()
         vz = fwd._zeros(nt, p.ny, p.nx)
-        # segments aligned on the end of the run: the last one, [last, maxtime],
-        # is a whole interval and keeps its snapshots; the ones before it start
-        # at 0 (the remainder) and at last - j*I
-        last = max(0, maxtime + 1 - I)
-        starts = sorted({0} | set(range(last, 0, -I)))
-        ends = {t0: t1 - 1 for t0, t1 in zip(starts, starts[1:] + [maxtime + 1])}
+        # segments counted back from the end: the last one, [last, maxtime], is
+        # a whole interval and keeps its snapshots
+        segs = checkpoint_segments(maxtime, I)
+        starts, ends = [a for a, _ in segs], dict(segs)
+        last = starts[-1]
         ckpt = {}
         kept = {}
         for t0 in starts[:-1]:
