@@ -95,11 +95,14 @@ class _Solve(torch.autograd.Function):
     def forward(ctx, psi, c, pipe, xsrc, maxtime):
         pipe.set_model(c, psi)
         if pipe.interval:
-            vz, ctx.ck = forward_with_checkpoints(pipe.fwd, xsrc, maxtime, pipe.interval)
+            # the last segment's snapshots stay from the forward run: the
+            # backward pass recomputes only the segments before it
+            vz, ctx.ck, ctx.last = forward_with_checkpoints(pipe.fwd, xsrc, maxtime, pipe.interval,
+                                                            keep_last=True)
             ctx.fsn = None
         else:
             vz, ctx.fsn = pipe.fwd.forward(xsrc, maxtime, snapshots=True)
-            ctx.ck = None
+            ctx.ck = ctx.last = None
         ctx.pipe, ctx.maxtime, ctx.model_id = pipe, maxtime, pipe.model_id
         ctx.save_for_backward(psi, c)
         return vz
@@ -113,11 +116,11 @@ class _Solve(torch.autograd.Function):
         vr = pipe.adjoint_source_from_grad(gvz)
         if ctx.ck is not None:
             # the forward model of this solve: checkpoints were taken with it
-            K = kernels_from_checkpoints(pipe.fwd, pipe.adj, ctx.ck, vr, ctx.maxtime, pipe.interval)
+            K = kernels_from_checkpoints(pipe.fwd, pipe.adj, ctx.ck, vr, ctx.maxtime, pipe.interval, ctx.last)
         else:
             _, asn = pipe.adj.adjoint(vr, ctx.maxtime, snapshots=True)
             K = pipe.adj.kernels(ctx.fsn, asn)
-        ctx.fsn = ctx.ck = None
+        ctx.fsn = ctx.ck = ctx.last = None
         gpsi, gc = kernel_to_gradient(pipe.p, pipe.model, K, psi, c)
         return gpsi.to(psi.device), gc.to(c.device), None, None, None
 

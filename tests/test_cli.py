@@ -93,7 +93,9 @@ def test_forward_adjoint_kernel_phases(tmp_path):
     vz = read_fits(fdir / "vz_cc.fits")[:, 0, :]
     s = Sparc(p, bg, psi=psi)
     vz_ref, _ = s.forward([-12.0], maxtime)
-    assert np.array_equal(vz, vz_ref[0].cpu().numpy())
+    # another process's solver, which times its own column-pass shape: equal to rounding
+    vr0 = vz_ref[0].cpu().numpy()
+    assert np.linalg.norm(vz - vr0) <= 1e-12 * np.linalg.norm(vr0)
     assert np.array_equal(np.fromfile(fdir / "vz_cc.bin", dtype="<f8").reshape(vz.shape), vz)
     assert (fdir / "kernel_info").exists() and (fdir / f"xix_{0:06d}_partial.fits").exists()
     from sparc_flows_amd.adjsrc import adjoint_source_filt

@@ -152,3 +152,28 @@ def test_gradient_near_top_pml_is_approximate():
     xs = list(B.source_positions(8, seed=4))
     err, signs = _check(p, bg, c2d, psi, xs, 3999, ndir=8, seed=2)
     assert signs >= 6 and err["psi"] < 0.8 and err["c"] < 0.5
+
+
+@pytest.mark.parametrize("interval", [0, 37, 400])
+def test_checkpointed_backward_equals_stored_snapshots(interval):
+    """The backward pass from checkpoints (segments counted back from the end,
+    the last one's snapshots kept from the forward run) gives the gradient of
+    the one from every stored snapshot to 1e-12 relative: the kernel sums are
+    grouped by segment, and each solver times its own column-pass shape, whose
+    segmentation changes the rounding (interval 0: two solvers without
+    checkpoints; 400: one segment)."""
+    p, bg, c2d, psi = small_case(nx=128, nz=150)
+    p = dataclasses.replace(_cadenced(p, 10.0), obsheight=-3e8, excitdep=-5e8)
+    c2d = np.asarray(c2d, dtype=np.float64)
+    maxtime, xsrc = 300, [-20.0, 15.0]
+    grads = []
+    for iv in (0, interval):
+        ds = DifferentiableSolver(p, bg, xsrc, maxtime, checkpoint_interval=iv)
+        ps = torch.as_tensor(psi).clone().requires_grad_(True)
+        cc = torch.as_tensor(c2d).clone().requires_grad_(True)
+        vz = ds.solve(ps, cc)
+        (vz ** 2).sum().backward()
+        grads.append((ps.grad.clone(), cc.grad.clone()))
+    assert grads[0][0].abs().max() > 0
+    for a, b in zip(grads[0], grads[1]):
+        assert float(torch.linalg.norm(a - b) / torch.linalg.norm(a)) < 1e-12

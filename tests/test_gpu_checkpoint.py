@@ -53,7 +53,8 @@ def test_checkpointed_gradient_equals_stored_snapshots(interval):
     xs = [-15.0, 10.0]
     ref = GradientPipeline(p, bg, nsrc=2, c2d=c2d, psi=psi).run(xs, maxtime)
     got = CheckpointedGradient(p, bg, nsrc=2, c2d=c2d, psi=psi, interval=interval).run(xs, maxtime)
-    assert torch.equal(got["vz"], ref["vz"])
+    # two solvers, each with its own timed column-pass shape: equal to rounding
+    assert rel_l2(got["vz"].cpu().numpy(), ref["vz"].cpu().numpy()) < 1e-12
     for k in ("kc", "kpsi", "kvx", "kvz", "hess"):
         e = rel_l2(got[k].cpu().numpy(), ref[k].cpu().numpy())
         print(interval, k, e)
