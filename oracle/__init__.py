@@ -41,41 +41,73 @@ def build(force: bool = False) -> None:
 _lib = None
 
 
+def _bind(path) -> ctypes.CDLL:
+    L = ctypes.CDLL(str(path))
+    L.orc_create.restype = ctypes.c_void_p
+    L.orc_create.argtypes = [ctypes.POINTER(OrcParams), _dp, _dp, _dp]
+    L.orc_destroy.argtypes = [ctypes.c_void_p]
+    L.orc_set_flow.argtypes = [ctypes.c_void_p, _dp, _dp]
+    L.orc_set_forward_source.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_int]
+    L.orc_set_adjoint_source.argtypes = [ctypes.c_void_p, _dp, ctypes.c_int, ctypes.c_int]
+    L.orc_step.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    L.orc_run.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp]
+    L.orc_get_state.argtypes = [ctypes.c_void_p, _dp]
+    L.orc_set_state.argtypes = [ctypes.c_void_p, _dp]
+    L.orc_get_pml.argtypes = [ctypes.c_void_p, _dp, _dp]
+    L.orc_get_scr.argtypes = [ctypes.c_void_p, _dp]
+    L.orc_info_int.argtypes = [ctypes.c_void_p, ctypes.c_char_p]
+    L.orc_info_double.argtypes = [ctypes.c_void_p, ctypes.c_char_p]
+    L.orc_info_double.restype = ctypes.c_double
+    L.orc_get_array.argtypes = [ctypes.c_void_p, ctypes.c_char_p, _dp, ctypes.c_int]
+    L.orc_dbyd2.argtypes = [_dp, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    L.orc_ddx.argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_int]
+    L.orc_ddz.argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_int]
+    L.orc_rhs.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _dp]
+    L.orc_filter_vars_2d.argtypes = [ctypes.c_void_p, _dp]
+    L.orc_filter_var_z.argtypes = [ctypes.c_int, ctypes.c_int, _dp]
+    L.orc_lagrange.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp]
+    L.orc_forward_source_series.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                            ctypes.c_int, _dp]
+    L.orc_fft.argtypes = [_dp, _dp, ctypes.c_int, ctypes.c_int]
+    L.orc_kernels.argtypes = [ctypes.POINTER(OrcParams), _dp, _dp, ctypes.c_int, _dp, _dp,
+                              _dp, _dp, _dp, _dp, _dp]
+    return L
+
+
 def lib() -> ctypes.CDLL:
     global _lib
     if _lib is None:
         if not LIB.exists():
             build()
-        L = ctypes.CDLL(str(LIB))
-        L.orc_create.restype = ctypes.c_void_p
-        L.orc_create.argtypes = [ctypes.POINTER(OrcParams), _dp, _dp, _dp]
-        L.orc_destroy.argtypes = [ctypes.c_void_p]
-        L.orc_set_forward_source.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_int]
-        L.orc_set_adjoint_source.argtypes = [ctypes.c_void_p, _dp, ctypes.c_int, ctypes.c_int]
-        L.orc_step.argtypes = [ctypes.c_void_p, ctypes.c_int]
-        L.orc_run.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp]
-        L.orc_get_state.argtypes = [ctypes.c_void_p, _dp]
-        L.orc_set_state.argtypes = [ctypes.c_void_p, _dp]
-        L.orc_get_pml.argtypes = [ctypes.c_void_p, _dp, _dp]
-        L.orc_get_scr.argtypes = [ctypes.c_void_p, _dp]
-        L.orc_info_int.argtypes = [ctypes.c_void_p, ctypes.c_char_p]
-        L.orc_info_double.argtypes = [ctypes.c_void_p, ctypes.c_char_p]
-        L.orc_info_double.restype = ctypes.c_double
-        L.orc_get_array.argtypes = [ctypes.c_void_p, ctypes.c_char_p, _dp, ctypes.c_int]
-        L.orc_dbyd2.argtypes = [_dp, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
-        L.orc_ddx.argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_int]
-        L.orc_ddz.argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_int]
-        L.orc_rhs.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _dp]
-        L.orc_filter_vars_2d.argtypes = [ctypes.c_void_p, _dp]
-        L.orc_filter_var_z.argtypes = [ctypes.c_int, ctypes.c_int, _dp]
-        L.orc_lagrange.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp]
-        L.orc_forward_source_series.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_double,
-                                                ctypes.c_int, _dp]
-        L.orc_fft.argtypes = [_dp, _dp, ctypes.c_int, ctypes.c_int]
-        L.orc_kernels.argtypes = [ctypes.POINTER(OrcParams), _dp, _dp, ctypes.c_int, _dp, _dp,
-                                  _dp, _dp, _dp, _dp, _dp]
-        _lib = L
+        _lib = _bind(LIB)
     return _lib
+
+
+class using:
+    """Context manager: route every oracle call to another build of
+    sparc_oracle.c (the mutant copies of tests/test_oracle_mutants.py)."""
+
+    def __init__(self, path):
+        self.path = path
+
+    def __enter__(self):
+        global _lib
+        self.prev = lib()
+        _lib = _bind(self.path)
+        return _lib
+
+    def __exit__(self, *exc):
+        global _lib
+        _lib = self.prev
+        return False
+
+
+def build_variant(mutant: int, outdir) -> Path:
+    """Compile sparc_oracle.c with ORC_MUTANT=mutant into outdir."""
+    out = Path(outdir) / f"liboracle_m{mutant}.so"
+    subprocess.check_call(["gcc", "-O2", "-fPIC", "-ffp-contract=off", "-fno-fast-math", "-shared",
+                           f"-DORC_MUTANT={int(mutant)}", "-o", str(out), str(HERE / "sparc_oracle.c"), "-lm"])
+    return out
 
 
 def _ptr(a: np.ndarray | None):
@@ -102,7 +134,8 @@ class Oracle:
         self._c2d = None if c2d is None else np.ascontiguousarray(c2d, dtype=np.float64)
         self._psi = None if psi is None else np.ascontiguousarray(psi, dtype=np.float64)
         self.adjoint = adjoint
-        self.h = lib().orc_create(ctypes.byref(self._op), _ptr(self._bg), _ptr(self._c2d),
+        self._L = lib()
+        self.h = self._L.orc_create(ctypes.byref(self._op), _ptr(self._bg), _ptr(self._c2d),
                                   _ptr(self._psi))
         if not self.h:
             raise ValueError("orc_create rejected the configuration")
@@ -110,77 +143,83 @@ class Oracle:
     def __del__(self):
         h = getattr(self, "h", None)
         if h:
-            lib().orc_destroy(h)
+            self._L.orc_destroy(h)
             self.h = None
 
     # ---- info ----
     def info(self, name: str):
-        v = lib().orc_info_int(self.h, name.encode())
+        v = self._L.orc_info_int(self.h, name.encode())
         if v != -1:
             return v
-        return lib().orc_info_double(self.h, name.encode())
+        return self._L.orc_info_double(self.h, name.encode())
 
     def array(self, name: str) -> np.ndarray:
         cap = 4 * self.nx * self.nz + 1_000_000
         out = np.zeros(cap)
-        n = lib().orc_get_array(self.h, name.encode(), _ptr(out), cap)
+        n = self._L.orc_get_array(self.h, name.encode(), _ptr(out), cap)
         if n < 0:
             raise KeyError(name)
         return out[:n].copy()
 
+    def set_flow(self, vx_ms: np.ndarray, vz_ms: np.ndarray) -> None:
+        """compute_data flow (true_vx / true_vz in m/s), replacing the psi flow."""
+        self._vx = np.ascontiguousarray(vx_ms, dtype=np.float64)
+        self._vz = np.ascontiguousarray(vz_ms, dtype=np.float64)
+        assert self._L.orc_set_flow(self.h, _ptr(self._vx), _ptr(self._vz)) == 0
+
     # ---- sources ----
     def set_forward_source(self, xloc: float, maxtime: int) -> None:
-        assert lib().orc_set_forward_source(self.h, float(xloc), int(maxtime)) == 0
+        assert self._L.orc_set_forward_source(self.h, float(xloc), int(maxtime)) == 0
 
     def set_adjoint_source(self, vr: np.ndarray, maxtime: int) -> None:
         vr = np.ascontiguousarray(vr, dtype=np.float64)
-        assert lib().orc_set_adjoint_source(self.h, _ptr(vr), vr.shape[0], int(maxtime)) == 0
+        assert self._L.orc_set_adjoint_source(self.h, _ptr(vr), vr.shape[0], int(maxtime)) == 0
 
     # ---- stepping ----
     def step(self, t: int) -> None:
-        lib().orc_step(self.h, int(t))
+        self._L.orc_step(self.h, int(t))
 
     def run(self, maxtime: int, snapshots: bool = False):
         nout = maxtime // self.p.steps + 1
         vz = np.zeros((nout, self.nx))
         nf = 4 if self.adjoint else 6
         snaps = np.zeros((nout, nf, self.nz, self.nx)) if snapshots else None
-        n = lib().orc_run(self.h, int(maxtime), _ptr(vz), _ptr(snaps))
+        n = self._L.orc_run(self.h, int(maxtime), _ptr(vz), _ptr(snaps))
         assert n == nout
         return (vz, snaps) if snapshots else vz
 
     def state(self) -> np.ndarray:
         a = np.zeros((4, self.nz, self.nx))
-        lib().orc_get_state(self.h, _ptr(a))
+        self._L.orc_get_state(self.h, _ptr(a))
         return a
 
     def set_state(self, a: np.ndarray) -> None:
         a = np.ascontiguousarray(a, dtype=np.float64)
-        lib().orc_set_state(self.h, _ptr(a))
+        self._L.orc_set_state(self.h, _ptr(a))
 
     def pml(self):
         nzp = self.p.npmlbot + self.p.npmltop
         pv = np.zeros((2, nzp, self.nx))
         px = np.zeros((2, self.nz, 2 * self.p.npmlhor))
-        lib().orc_get_pml(self.h, _ptr(pv), _ptr(px))
+        self._L.orc_get_pml(self.h, _ptr(pv), _ptr(px))
         return pv, px
 
     def scr(self) -> np.ndarray:
         s = np.zeros((4, self.nz, self.nx))
-        lib().orc_get_scr(self.h, _ptr(s))
+        self._L.orc_get_scr(self.h, _ptr(s))
         return s
 
     # ---- operators ----
     def ddx(self, f: np.ndarray) -> np.ndarray:
         f = np.ascontiguousarray(f, dtype=np.float64)
         out = np.zeros_like(f)
-        lib().orc_ddx(self.h, _ptr(f), _ptr(out), f.size // self.nx)
+        self._L.orc_ddx(self.h, _ptr(f), _ptr(out), f.size // self.nx)
         return out
 
     def ddz(self, f: np.ndarray, ibc: int, preset: np.ndarray | None = None) -> np.ndarray:
         f = np.ascontiguousarray(f, dtype=np.float64)
         out = np.zeros_like(f) if preset is None else np.ascontiguousarray(preset, dtype=np.float64).copy()
-        lib().orc_ddz(self.h, _ptr(f), _ptr(out), int(ibc))
+        self._L.orc_ddz(self.h, _ptr(f), _ptr(out), int(ibc))
         return out
 
     def rhs(self, t: int, temp4, psipml=None, psipmlx=None):
@@ -192,18 +231,18 @@ class Oracle:
         scr = np.zeros_like(temp4)
         scrpml = np.zeros_like(psipml)
         scrpmlx = np.zeros_like(psipmlx)
-        lib().orc_rhs(self.h, int(t), _ptr(temp4), _ptr(psipml), _ptr(psipmlx), _ptr(scr),
+        self._L.orc_rhs(self.h, int(t), _ptr(temp4), _ptr(psipml), _ptr(psipmlx), _ptr(scr),
                       _ptr(scrpml), _ptr(scrpmlx))
         return scr, scrpml, scrpmlx
 
     def filter_vars_2d(self, a4: np.ndarray) -> np.ndarray:
         a4 = np.ascontiguousarray(a4, dtype=np.float64).copy()
-        lib().orc_filter_vars_2d(self.h, _ptr(a4))
+        self._L.orc_filter_vars_2d(self.h, _ptr(a4))
         return a4
 
     def lagrange(self, t: int) -> np.ndarray:
         out = np.zeros(self.nx)
-        lib().orc_lagrange(self.h, int(t), _ptr(out))
+        self._L.orc_lagrange(self.h, int(t), _ptr(out))
         return out
 
 

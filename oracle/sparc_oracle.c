@@ -16,9 +16,21 @@
  *   orc_create           - flows against the closed form of a separable
  *                          stream function; PML coefficient structure and
  *                          the absorption of an outgoing pulse;
- *   orc_rhs / orc_step   - linearity, symmetry and energy invariants.
+ *   orc_rhs / orc_step   - the RK polynomial, linearity, mirror symmetry,
+ *                          and closed forms (tests/test_oracle_physics.py):
+ *                          a uniform xi_z in hydrostatic balance is force
+ *                          free, a horizontal xi_x mode gives c^2 d2/dx2 and
+ *                          (1 - Gamma) g div, flow increments -2 v0 . grad v,
+ *                          the PML auxiliary equations, the top closure;
+ *   orc_kernels          - a numpy restatement on the reference-built dbyd2,
+ *                          the psi <-> v0 chain rule, and central finite
+ *                          differences of a linear functional of the traces.
  * End-to-end traces against the full Fortran build are "parity unpinned":
  * the reference needs FFTW3, which this image does not have.
+ *
+ * ORC_MUTANT (compile time, default 0) plants one known mistake in a copy of
+ * this library so that tests/test_oracle_mutants.py can show that each pin
+ * fails on the mistake it is meant to catch.  Every other build has 0.
  */
 #include "sparc_oracle.h"
 
@@ -28,6 +40,11 @@
 
 #define RSUN 6.95989467700e10           /* initialize.f90:37 */
 #define PI 3.141592653589793            /* initialize.f90:37 */
+
+#ifndef ORC_MUTANT
+#define ORC_MUTANT 0
+#endif
+#define MUT(n) (ORC_MUTANT == (n))
 
 struct orc_model {
     orc_params p;
@@ -510,6 +527,21 @@ int orc_set_adjoint_source(orc_model *m, const double *vr, int nt, int maxtime)
     return 0;
 }
 
+/* compute_data branch of the flow set-up: v0_x, v0_z read in m/s from
+ * true_vx.fits / true_vz.fits and scaled by 1e2/dimc (driver.f90:65-73),
+ * in place of the flow from psi.  An adjoint model reverses it, as the psi
+ * flow is reversed (driver.f90:144-147). */
+int orc_set_flow(orc_model *m, const double *vx_ms, const double *vz_ms)
+{
+    size_t n2 = (size_t)m->nx * m->nz;
+    double sgn = m->p.adjoint ? -1.0 : 1.0;
+    for (size_t q = 0; q < n2; q++) {
+        m->v0x[q] = sgn * (vx_ms[q] * 1e2 / m->dimc);
+        m->v0z[q] = sgn * (vz_ms[q] * 1e2 / m->dimc);
+    }
+    return 0;
+}
+
 /* LAGRANGE_INTERP: physics.f90:678-737. vr slice j (1-based) is node time
  * (j-1)*cadforcing. */
 void orc_lagrange(orc_model *m, int time, double *forcing)
@@ -569,15 +601,17 @@ void orc_rhs(orc_model *m, int time, const double *temp4, const double *psipml,
 
     orc_ddx(m, xi_x, dxixdx, nz);                                  /* :953 */
     for (int i = 0; i < nx; i++)                                   /* :954 */
-        dxizdz[(size_t)(nz - 1) * nx + i] = -dxixdx[(size_t)(nz - 1) * nx + i] * m->unstretch[nz - 1];
+        dxizdz[(size_t)(nz - 1) * nx + i] = (MUT(14) ? 1.0 : -1.0) *
+                                            dxixdx[(size_t)(nz - 1) * nx + i] * m->unstretch[nz - 1];
     orc_ddz(m, xi_z, dxizdz, 3);                                   /* :955 */
     int pmlindex = 0;                                              /* :957-967 */
     for (int k = 0; k < nz; k++) {
         if (k < m->p.npmlbot || k >= nz - m->p.npmltop) {
             for (int i = 0; i < nx; i++) {
                 size_t q = (size_t)k * nx + i, qp = (size_t)pmlindex * nx + i;
+                if (MUT(6)) dxizdz[q] = dxizdz[q] + psivz[qp];
                 RHSpsivz[qp] = m->az[q] * dxizdz[q] + m->bzpml[q] * psivz[qp];
-                dxizdz[q] = dxizdz[q] + psivz[qp];
+                if (!MUT(6)) dxizdz[q] = dxizdz[q] + psivz[qp];
             }
             pmlindex++;
         }
@@ -585,14 +619,17 @@ void orc_rhs(orc_model *m, int time, const double *temp4, const double *psipml,
     for (int k = 0; k < nz; k++)                                   /* :970-986 */
         for (int i = 0; i < nh; i++) {
             size_t q = (size_t)k * nx + i, qx = (size_t)k * nph + i;
+            if (MUT(5)) RHSpsivx[qx] = m->axpml[qx] * dxixdx[q] + m->bxpml[qx] * psivx[qx];
             dxixdx[q] = dxixdx[q] + psivx[qx];
-            RHSpsivx[qx] = m->axpml[qx] * dxixdx[q] + m->bxpml[qx] * psivx[qx];
+            if (!MUT(5)) RHSpsivx[qx] = m->axpml[qx] * dxixdx[q] + m->bxpml[qx] * psivx[qx];
             size_t q2 = (size_t)k * nx + nx - nh + i, qx2 = qx + nh;
+            if (MUT(5)) RHSpsivx[qx2] = m->axpml[qx2] * dxixdx[q2] + m->bxpml[qx2] * psivx[qx2];
             dxixdx[q2] = dxixdx[q2] + psivx[qx2];
-            RHSpsivx[qx2] = m->axpml[qx2] * dxixdx[q2] + m->bxpml[qx2] * psivx[qx2];
+            if (!MUT(5)) RHSpsivx[qx2] = m->axpml[qx2] * dxixdx[q2] + m->bxpml[qx2] * psivx[qx2];
         }
     for (size_t q = 0; q < n2; q++) div[q] = dxixdx[q] + dxizdz[q];              /* :990 */
-    for (size_t q = 0; q < n2; q++) p[q] = -(m->c2rho0[q] * div[q]) - m->gradp0_z[q] * xi_z[q]; /* :998 */
+    for (size_t q = 0; q < n2; q++)                                               /* :998 */
+        p[q] = -(m->c2rho0[q] * div[q]) + (MUT(2) ? 1.0 : -1.0) * m->gradp0_z[q] * xi_z[q];
     orc_ddx(m, p, gradp_x, nz);                                    /* :1000 */
     for (int k = 0; k < nz; k++)                                   /* :1003-1020 */
         for (int i = 0; i < nh; i++) {
@@ -616,11 +653,12 @@ void orc_rhs(orc_model *m, int time, const double *temp4, const double *psipml,
         }
     }
     for (size_t q = 0; q < n2; q++) RHSv_x[q] = -m->rhoinv[q] * gradp_x[q];       /* :1039 */
-    for (size_t q = 0; q < n2; q++) rho[q] = -(m->rho0[q] * div[q]) - m->gradrho0_z[q] * xi_z[q]; /* :1043 */
+    for (size_t q = 0; q < n2; q++)                                /* :1043 */
+        rho[q] = -(m->rho0[q] * div[q]) + (MUT(7) ? 1.0 : -1.0) * m->gradrho0_z[q] * xi_z[q];
     for (int k = 0; k < nz; k++)                                   /* :1044-1046 */
         for (int i = 0; i < nx; i++) {
             size_t q = (size_t)k * nx + i;
-            RHSv_z[q] = -m->rhoinv[q] * (gradp_z[q] + rho[q] * m->g[k]);
+            RHSv_z[q] = -m->rhoinv[q] * (gradp_z[q] + (MUT(1) ? -1.0 : 1.0) * rho[q] * m->g[k]);
         }
     memcpy(RHSxi_x, v_x, n2 * sizeof(double));                     /* :1048-1049 */
     memcpy(RHSxi_z, v_z, n2 * sizeof(double));
@@ -637,14 +675,16 @@ void orc_rhs(orc_model *m, int time, const double *temp4, const double *psipml,
         }
     }
     /* flows, :1065-1079; dxixdx is the reference's scratch here */
+    const double two = MUT(3) ? 1.0 : 2.0;
+    const double *fx = MUT(4) ? m->v0z : m->v0x, *fz = MUT(4) ? m->v0x : m->v0z;
     orc_ddx(m, v_x, dxixdx, nz);
-    for (size_t q = 0; q < n2; q++) RHSv_x[q] = RHSv_x[q] - 2.0 * m->v0x[q] * dxixdx[q];
+    for (size_t q = 0; q < n2; q++) RHSv_x[q] = RHSv_x[q] - two * fx[q] * dxixdx[q];
     orc_ddz(m, v_x, dxixdx, 1);
-    for (size_t q = 0; q < n2; q++) RHSv_x[q] = RHSv_x[q] - 2.0 * m->v0z[q] * dxixdx[q];
+    for (size_t q = 0; q < n2; q++) RHSv_x[q] = RHSv_x[q] - two * fz[q] * dxixdx[q];
     orc_ddx(m, v_z, dxixdx, nz);
-    for (size_t q = 0; q < n2; q++) RHSv_z[q] = RHSv_z[q] - 2.0 * m->v0x[q] * dxixdx[q];
+    for (size_t q = 0; q < n2; q++) RHSv_z[q] = RHSv_z[q] - two * fx[q] * dxixdx[q];
     orc_ddz(m, v_z, dxixdx, 1);
-    for (size_t q = 0; q < n2; q++) RHSv_z[q] = RHSv_z[q] - 2.0 * m->v0z[q] * dxixdx[q];
+    for (size_t q = 0; q < n2; q++) RHSv_z[q] = RHSv_z[q] - two * fz[q] * dxixdx[q];
     for (int i = 0; i < nx; i++) {                                 /* :1083-1087 */
         RHSv_x[i] = 0.0; RHSv_z[i] = 0.0; RHSxi_x[i] = 0.0; RHSxi_z[i] = 0.0;
     }
@@ -879,7 +919,7 @@ int orc_kernels(const orc_params *p, const double *bg, const double *c2d, int nt
     double *kv1 = dalloc(n2), *kv3 = dalloc(n2), *kcc = dalloc(n2), *hh = dalloc(n2);
     double *d11 = dalloc(n2), *d13 = dalloc(n2), *d31 = dalloc(n2), *d33 = dalloc(n2);
 #define DDXK(in, out) ddx_rows(nx, eyekx, stretchx, fre, fim, in, out, nz)
-#define DDZK(in, out, bc) do { ddz_cols(nx, nz, stk, in, out, bc); orc_filter_var_z(nx, nz, out); } while (0)
+#define DDZK(in, out, bc) do { ddz_cols(nx, nz, stk, in, out, bc); if (!MUT(10)) orc_filter_var_z(nx, nz, out); } while (0)
     for (int k = 0; k < nt_kern; k++) {
         const double *f = fwd + (size_t)k * 6 * n2;
         const double *a = adj + (size_t)(nt_kern - 1 - k) * 4 * n2;
@@ -887,7 +927,7 @@ int orc_kernels(const orc_params *p, const double *bg, const double *c2d, int nt
         const double *faccx = f + 4 * n2, *faccz = f + 5 * n2;
         const double *avx = a, *avz = a + n2;
         for (size_t q = 0; q < n2; q++)            /* kernels.f90:293-294 */
-            hh[q] = hh[q] + faccz[q] * faccz[q] + faccx[q] * faccx[q] + 0.0 * 0.0;
+            hh[q] = hh[q] + faccz[q] * faccz[q] + (MUT(13) ? 0.0 : faccx[q] * faccx[q]) + 0.0 * 0.0;
         DDXK(fxix, t2); DDZK(fxiz, t1, 1);         /* compute_div(f_xi) */
         for (size_t q = 0; q < n2; q++) t2[q] = t2[q] + t1[q];
         DDXK(avx, t3); DDZK(avz, t1, 1);           /* compute_div(a_vel) */
@@ -903,18 +943,19 @@ int orc_kernels(const orc_params *p, const double *bg, const double *c2d, int nt
     memcpy(hess, hh, n2 * sizeof(double));
     double e8 = (double)1e-8f, e5 = (double)1e-5f;
     double dimen = pow(diml * e8 * p->xlength * e8 * dimc * e5, -1.0) * 1.0; /* :503 */
-    DDZK(kv1, t1, 0);                              /* curl_y = ddz(Kx) - ddx(Kz) */
+    DDZK(kv1, t1, MUT(9) ? 1 : 0);                 /* curl_y = ddz(Kx) - ddx(Kz) */
     DDXK(kv3, t2);
     for (size_t q = 0; q < n2; q++) {
         double kp = t1[q] - t2[q];                 /* :510-513 */
         kpsi[q] = 2.0 * kp * dimen * 1.0 * (rho0[q] * sqrt(c2[q])) * (double)stepskern *
-                  p->timestep * dimen;
-        kvx[q] = -2.0 * kv1[q] * rho0[q] * (double)stepskern * p->timestep * dimen; /* :520-522 */
-        kvz[q] = -2.0 * kv3[q] * rho0[q] * (double)stepskern * p->timestep * dimen;
+                  p->timestep * (MUT(8) ? 1.0 : dimen);
+        const double sv = MUT(12) ? 2.0 : -2.0;    /* :520-522 */
+        kvx[q] = sv * kv1[q] * rho0[q] * (double)stepskern * p->timestep * dimen;
+        kvz[q] = sv * kv3[q] * rho0[q] * (double)stepskern * p->timestep * dimen;
     }
     double dimen_c = pow(diml * e8 * p->xlength * e8, -1.0) * 1.0; /* :563-566 */
     for (size_t q = 0; q < n2; q++)                /* :579 */
-        kc[q] = -kcc[q] * rho0[q] * c2[q] * (double)stepskern * p->timestep * dimen_c * 2.0;
+        kc[q] = -kcc[q] * rho0[q] * c2[q] * (double)stepskern * p->timestep * dimen_c * (MUT(11) ? 1.0 : 2.0);
 #undef DDXK
 #undef DDZK
     free(zk); free(unst); free(stk); free(c2); free(rho0); free(eyekx); free(fre); free(fim);

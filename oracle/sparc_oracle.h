@@ -50,6 +50,10 @@ orc_model *orc_create(const orc_params *p, const double *bg, const double *c2d,
                       const double *psi);
 void orc_destroy(orc_model *m);
 
+/* Flow in m/s [nz][nx] as the compute_data branch reads it (true_vx/true_vz,
+ * driver.f90:65-73), replacing the flow from psi; reversed for an adjoint. */
+int orc_set_flow(orc_model *m, const double *vx_ms, const double *vz_ms);
+
 /* Forward source at x_src (Mm), fwdsource table of length maxtime+8. */
 int orc_set_forward_source(orc_model *m, double xloc, int maxtime);
 /* Adjoint source vr[nt][nx], nt = forcing_length slices. */
