@@ -82,7 +82,7 @@ def forward_rate(nx, nz, nsrc, nsteps, flow, dc, reps=1):
     sec = (time.perf_counter() - t0) / reps
     return {"Mpoint_steps_per_s": nx * nz * nsteps * nsrc / sec / 1e6, "s_per_run": sec,
             "grid": [nx, nz], "sources": nsrc, "time_steps": nsteps, "flow": flow, "dc": dc,
-            "column_pass_shape": {k: s.query(k) for k in ("colseg_tc", "colseg_ms", "colseg_p", "colseg_split")}}
+            "column_pass_shape": {k: s.query(k) for k in ("colseg_tc", "colseg_ms", "colseg_p")}}
 
 
 def forward3d_rate(nsteps=20, warm=2, nx=256, ny=256, nz=300, cpu_seconds=8.0):
@@ -296,8 +296,7 @@ def main():
                    "speedup_vs_sequential_plain": (sec * args.batch) / bsec,
                    "roofline_col": {"achieved": bach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                     "frac": bach / HBM_PEAK_GBS, "ms_per_launch": bms},
-                   "shape": {k: bpipe.fwd.query(k) for k in ("colseg_tc", "colseg_ms", "colseg_split",
-                                                             "row_wide")}}
+                   "shape": {k: bpipe.fwd.query(k) for k in ("colseg_tc", "colseg_ms", "row_wide")}}
         del bpipe
 
     # roofline of the dominant kernel: HIP events on the solver's stream
@@ -306,7 +305,7 @@ def main():
                    "row_first": ms["row_first"], "row_last": ms["row_last"]}
     dom = max(per_step_ms, key=per_step_ms.get)
     achieved = bytes_per_pt[dom] * npts / (ms[dom] * 1e-3) / 1e9
-    colk = "k_colsplit" if pipe.fwd.query("colseg_split") else "k_colseg"
+    colk = "k_colsplit"
     traffic = (measured_traffic(colk, pipe.fwd.query("colseg_ms"), pipe.fwd.query("colseg_p"))
                if dom == "col" else None)
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -336,7 +335,7 @@ def main():
         "batched": batched,
         "roofline": roofline,
         "column_pass_shape": {k: pipe.fwd.query(k) for k in ("colseg_tc", "colseg_ms", "colseg_p", "colseg_hw",
-                                                             "colseg_split", "row_wide")},
+                                                             "row_wide")},
         "cpu_baseline": cpu,
         "other_configurations": others,
     }

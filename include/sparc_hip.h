@@ -93,6 +93,15 @@ int sparc_set_flow(sparc_handle *h, const double *vx_host, const double *vz_host
  * FORWARD_SOURCE (driver.f90:1120-1145) for x positions xsrc (Mm, one per
  * source as in master.pixels), for time indices 0..maxtime. */
 int sparc_set_forward_sources(sparc_handle *h, const double *xsrc_host, int maxtime);
+/* A separable forward source in place of FORWARD_SOURCE's one-row form
+ * (physics2d.f90:1056-1059 injects vr * fwdsource(time+1) * delta_width /
+ * rho0 at row e_rad only): RHS v_z(x, z) += vr[s][x] * series[time] *
+ * zprof[z] for time <= forcing_length * cadforcing_step, with zprof already
+ * holding the 1/rho0 and any z normalisation.  All arrays are host:
+ * vr [nsrc][nx], zprof [nz], series [nseries] (0 beyond its end).  Used for
+ * a force distributed in z, such as a Gaussian. */
+int sparc_set_forward_profile(sparc_handle *h, const double *vr_host, const double *zprof_host,
+                              const double *series_host, int nseries, int maxtime);
 /* Adjoint sources: DETERMINE_STATUS adjoint branch (driver.f90:584-594).
  * vr: device [nsrc][nt][nx] (source.fits of each source), nt slices at
  * cadence cadforcing, interpolated by LAGRANGE_INTERP (physics.f90:678-737). */

@@ -48,6 +48,7 @@ def _bind(path) -> ctypes.CDLL:
     L.orc_destroy.argtypes = [ctypes.c_void_p]
     L.orc_set_flow.argtypes = [ctypes.c_void_p, _dp, _dp]
     L.orc_set_forward_source.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_int]
+    L.orc_set_forward_profile.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, ctypes.c_int, ctypes.c_int]
     L.orc_set_adjoint_source.argtypes = [ctypes.c_void_p, _dp, ctypes.c_int, ctypes.c_int]
     L.orc_step.argtypes = [ctypes.c_void_p, ctypes.c_int]
     L.orc_run.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp]
@@ -170,6 +171,12 @@ class Oracle:
     # ---- sources ----
     def set_forward_source(self, xloc: float, maxtime: int) -> None:
         assert self._L.orc_set_forward_source(self.h, float(xloc), int(maxtime)) == 0
+
+    def set_forward_profile(self, vr, zprof, series, maxtime: int) -> None:
+        """Separable forward source vr[nx] * series[t] * zprof[nz]."""
+        self._pv = [np.ascontiguousarray(a, dtype=np.float64) for a in (vr, zprof, series)]
+        assert self._L.orc_set_forward_profile(self.h, *[_ptr(a) for a in self._pv],
+                                               self._pv[2].size, int(maxtime)) == 0
 
     def set_adjoint_source(self, vr: np.ndarray, maxtime: int) -> None:
         vr = np.ascontiguousarray(vr, dtype=np.float64)

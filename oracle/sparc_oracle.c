@@ -69,6 +69,7 @@ struct orc_model {
     double *fwdsource;
     int nfwd;
     double *forcing;
+    double *zprof;                      /* profile source: [nz], NULL for the one-row form */
     int maxtime, steps, cadforcing_step, forcing_length, num_steps;
 };
 
@@ -453,7 +454,7 @@ void orc_destroy(orc_model *m)
                        &m->temp, &m->scr, &m->pmlvars, &m->psipml, &m->scrpml, &m->pmlvarsx,
                        &m->psipmlx, &m->scrpmlx, &m->dxixdx, &m->dxizdz, &m->pp, &m->rho,
                        &m->gradp_x, &m->gradp_z, &m->div, &m->fre, &m->fim, &m->vr,
-                       &m->fwdsource, &m->forcing};
+                       &m->fwdsource, &m->forcing, &m->zprof};
     for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); i++) free(*ptrs[i]);
     free(m);
 }
@@ -493,6 +494,8 @@ int orc_set_forward_source(orc_model *m, double xloc, int maxtime)
     m->nfwd = maxtime + 8;
     m->fwdsource = dalloc(m->nfwd);
     orc_forward_source_series(m->p.timestep, m->p.nupeak, m->p.nuwidth, m->nfwd, m->fwdsource);
+    free(m->zprof);
+    m->zprof = NULL;
     free(m->vr);
     m->vr = dalloc(nx);
     m->nvr = 1;
@@ -510,6 +513,29 @@ int orc_set_forward_source(orc_model *m, double xloc, int maxtime)
         m->vr[i] = exp(-(xd[i] * xd[i]) / (400.0 * (dx * dx))) * sincx;
     }
     free(xd);
+    return 0;
+}
+
+/* A separable forward source in place of the one-row form of
+ * physics2d.f90:1056-1059: RHS v_z(x, z) += vr[x] * series[time] * zprof[z]
+ * (zprof holds 1/rho0 and the z normalisation), for time <=
+ * forcing_length * cadforcing_step as in :1051. */
+int orc_set_forward_profile(orc_model *m, const double *vr, const double *zprof,
+                            const double *series, int nseries, int maxtime)
+{
+    if (m->p.adjoint || nseries < 1) return -1;
+    set_times(m, maxtime);
+    free(m->fwdsource);
+    m->nfwd = nseries;
+    m->fwdsource = dalloc(nseries);
+    memcpy(m->fwdsource, series, (size_t)nseries * sizeof(double));
+    free(m->vr);
+    m->vr = dalloc(m->nx);
+    m->nvr = 1;
+    memcpy(m->vr, vr, (size_t)m->nx * sizeof(double));
+    free(m->zprof);
+    m->zprof = dalloc(m->nz);
+    memcpy(m->zprof, zprof, (size_t)m->nz * sizeof(double));
     return 0;
 }
 
@@ -667,6 +693,12 @@ void orc_rhs(orc_model *m, int time, const double *temp4, const double *psipml,
             int r = m->o_rad;
             for (int i = 0; i < nx; i++)
                 RHSv_z[(size_t)r * nx + i] += m->forcing[i] * m->delta_width / m->rho0[(size_t)r * nx + i];
+        } else if (m->fwdsource && m->zprof) {
+            double fs = (time >= 0 && time < m->nfwd) ? m->fwdsource[time] : 0.0;
+            for (int k = 0; k < nz; k++)
+                if (m->zprof[k] != 0.0)
+                    for (int i = 0; i < nx; i++)
+                        RHSv_z[(size_t)k * nx + i] += (m->vr[i] * fs) * m->zprof[k];
         } else if (m->fwdsource) {
             int r = m->e_rad;
             double fs = (time >= 0 && time < m->nfwd) ? m->fwdsource[time] : 0.0;

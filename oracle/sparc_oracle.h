@@ -56,6 +56,9 @@ int orc_set_flow(orc_model *m, const double *vx_ms, const double *vz_ms);
 
 /* Forward source at x_src (Mm), fwdsource table of length maxtime+8. */
 int orc_set_forward_source(orc_model *m, double xloc, int maxtime);
+/* Separable forward source: RHS v_z += vr[x] * series[t] * zprof[z]. */
+int orc_set_forward_profile(orc_model *m, const double *vr, const double *zprof,
+                            const double *series, int nseries, int maxtime);
 /* Adjoint source vr[nt][nx], nt = forcing_length slices. */
 int orc_set_adjoint_source(orc_model *m, const double *vr, int nt, int maxtime);
 

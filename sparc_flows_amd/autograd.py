@@ -36,12 +36,13 @@ percent, not to rounding.
 """
 from __future__ import annotations
 
-import math
 
 import numpy as np
 import torch
 
-from .gradient import forward_with_checkpoints, kernels_from_checkpoints
+from .gradient import (forward_binomial, forward_with_checkpoints, kernels_binomial,
+                       kernels_from_checkpoints)
+from .misfit import travel_time_misfit, travel_times, window_weights  # noqa: F401  (public names)
 from .params import SparcParams
 from .solver import Sparc
 
@@ -90,11 +91,22 @@ def kernel_to_gradient(p: SparcParams, model: np.ndarray, K: dict, psi: np.ndarr
     return gpsi, gc
 
 
+def _take(obj, name):
+    """obj.name, leaving None in its place."""
+    v = getattr(obj, name)
+    setattr(obj, name, None)
+    return v
+
+
 class _Solve(torch.autograd.Function):
     @staticmethod
     def forward(ctx, psi, c, pipe, xsrc, maxtime):
         pipe.set_model(c, psi)
-        if pipe.interval:
+        ctx.binomial = pipe.slots is not None
+        if ctx.binomial:
+            vz, ctx.ck = forward_binomial(pipe.fwd, xsrc, maxtime, pipe.slots)
+            ctx.fsn = ctx.last = None
+        elif pipe.interval:
             # the last segment's snapshots stay from the forward run: the
             # backward pass recomputes only the segments before it
             vz, ctx.ck, ctx.last = forward_with_checkpoints(pipe.fwd, xsrc, maxtime, pipe.interval,
@@ -114,9 +126,14 @@ class _Solve(torch.autograd.Function):
         if pipe.model_id != ctx.model_id:        # another solve changed the model since
             pipe.set_model(c, psi)
         vr = pipe.adjoint_source_from_grad(gvz)
-        if ctx.ck is not None:
+        if ctx.binomial:
+            K = kernels_binomial(pipe.fwd, pipe.adj, ctx.ck, vr, ctx.maxtime, pipe.slots)
+        elif ctx.ck is not None:
             # the forward model of this solve: checkpoints were taken with it
-            K = kernels_from_checkpoints(pipe.fwd, pipe.adj, ctx.ck, vr, ctx.maxtime, pipe.interval, ctx.last)
+            # ctx gives up `last` here, so the kept segment is freed before the
+            # recompute buffer is allocated (kernels_from_checkpoints)
+            K = kernels_from_checkpoints(pipe.fwd, pipe.adj, ctx.ck, vr, ctx.maxtime, pipe.interval,
+                                         _take(ctx, "last"))
         else:
             _, asn = pipe.adj.adjoint(vr, ctx.maxtime, snapshots=True)
             K = pipe.adj.kernels(ctx.fsn, asn)
@@ -131,12 +148,18 @@ class DifferentiableSolver:
     ``solve(psi, c)`` returns vz [nsrc][nout][nx] (non-dimensional, the
     vz_cc records) and is differentiable with respect to psi [nz][nx]
     (model_psi) and c [nz][nx] (cm/s).  With ``checkpoint_interval`` n > 0
-    the forward pass keeps a full-state checkpoint every n steps instead of
-    the snapshots, and the backward pass recomputes them segment by segment.
+    the forward pass keeps a full-state checkpoint every n steps and the
+    snapshots of the last n steps only (that segment is never recomputed);
+    the backward pass correlates that segment first, releases it, and
+    recomputes the others segment by segment, so forward snapshots peak at
+    one segment from the forward pass until the end of the backward pass.
+    With ``checkpoint_slots`` c (not None) the backward pass uses binomial
+    checkpointing instead (gradient.binomial_schedule): at most c full-state
+    checkpoints and one snapshot per run, whatever the run length.
     """
 
     def __init__(self, params: SparcParams, model: np.ndarray, xsrc, maxtime: int,
-                 checkpoint_interval: int = 0):
+                 checkpoint_interval: int = 0, checkpoint_slots: int | None = None):
         if params.steps != params.cadforcing_step:
             raise ValueError("the adjoint source is sampled at the output cadence: "
                              "outputcad and cadforcing must give the same step count")
@@ -152,6 +175,9 @@ class DifferentiableSolver:
         # 0: keep every forward snapshot; n > 0: checkpoint every n steps and
         # recompute the snapshots during the backward pass
         self.interval = int(checkpoint_interval)
+        self.slots = None if checkpoint_slots is None else int(checkpoint_slots)
+        if self.slots is not None and self.slots < 0:
+            raise ValueError("checkpoint_slots must be >= 0")
         self.model_id = 0
 
     def set_model(self, c, psi) -> None:
@@ -168,35 +194,3 @@ class DifferentiableSolver:
 
     def solve(self, psi: torch.Tensor, c: torch.Tensor) -> torch.Tensor:
         return _Solve.apply(psi, c, self, self.xsrc, self.maxtime)
-
-
-def window_weights(nt: int, cadence_s: float, window_s: float, device=None) -> torch.Tensor:
-    """1 over the first window_s seconds, cosine taper over its last 10 %."""
-    t = torch.arange(nt, dtype=torch.float64, device=device) * cadence_s
-    w = torch.ones(nt, dtype=torch.float64, device=device)
-    taper = 0.1 * window_s
-    w = torch.where(t > window_s, torch.zeros_like(w), w)
-    ramp = (t > window_s - taper) & (t <= window_s)
-    return torch.where(ramp, 0.5 * (1 + torch.cos(math.pi * (t - (window_s - taper)) / taper)), w)
-
-
-def travel_times(vz: torch.Tensor, vz_ref: torch.Tensor, mask: torch.Tensor, w: torch.Tensor,
-                 cadence_s: float, floor: float = 1e-2) -> tuple[torch.Tensor, torch.Tensor]:
-    """Linearised cross-correlation travel times (Gizon & Birch 2002, eq. 13):
-    tau = -int W dref/dt (vz - vref) dt / int W (dref/dt)^2 dt, so that
-    vz(t) = vref(t - tau) gives tau > 0.  vz, vz_ref: [nsrc][nt][nx];
-    mask: [nsrc][nx] receivers; w: [nt] window.  Receivers whose windowed
-    reference energy is below `floor` times the source's largest are dropped
-    (tau is ill-conditioned where the wave has not arrived).
-    Returns (tau [nsrc][nx], the receiver mask used)."""
-    dref = torch.gradient(vz_ref, spacing=cadence_s, dim=1)[0]
-    num = (w[None, :, None] * dref * (vz - vz_ref)).sum(1)
-    den = (w[None, :, None] * dref * dref).sum(1)
-    m = mask * (den * mask > floor * (den * mask).amax(dim=1, keepdim=True))
-    return -num / torch.where(m > 0, den, torch.ones_like(den)) * m, m
-
-
-def travel_time_misfit(vz, vz_ref, tau_obs, mask, w, cadence_s, floor: float = 1e-2):
-    """0.5 * sum over sources and receivers of (tau - tau_obs)^2."""
-    tau, m = travel_times(vz, vz_ref, mask, w, cadence_s, floor)
-    return 0.5 * ((tau - tau_obs) * m).pow(2).sum()

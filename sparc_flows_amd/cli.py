@@ -126,14 +126,15 @@ def _forward(p, bg, psi, ddir, contrib, jobno, maxtime, compute_data, linesearch
 
     out = ddir / f"forward_src{contrib}_ls{jobno}"
     out.mkdir(parents=True, exist_ok=True)
-    v0 = None
     if compute_data:
-        # true flows in m/s (driver.f90:65-73): build psi-free flows is not
-        # supported by the C ABI, so the stream function of the true model
-        # must be given as model_psi of this iterate instead.
-        if (Path.cwd() / "true_psi.fits").exists():
-            v0 = read_fits(Path.cwd() / "true_psi.fits").reshape(p.nz, p.nx)
-    s = Sparc(p, bg, psi=v0 if v0 is not None else psi)
+        # the true flow, v0_x and v0_z in m/s from true_vx.fits / true_vz.fits
+        # in the working directory, in place of the flow from psi (driver.f90:65-73)
+        s = Sparc(p, bg)
+        s.set_flow(read_fits(Path.cwd() / "true_vx.fits").reshape(p.nz, p.nx),
+                   read_fits(Path.cwd() / "true_vz.fits").reshape(p.nz, p.nx))
+    else:
+        s = Sparc(p, bg, psi=psi)
+    # partial snapshots only for a plain forward run (driver.f90:412-414)
     no_snapshots = linesearch or compute_synth or compute_data
     vz, sn = s.forward([_xsrc(ddir, contrib)], maxtime, snapshots=not no_snapshots)
     vz = vz[0].cpu().numpy()

@@ -12,6 +12,8 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -82,17 +84,15 @@ struct sparc_handle {
     // segmented column pass: tables and launch shape (chosen by autotune)
     SegTab seg{};
     int seg_tc = 16, seg_ms = 20;
-    int seg_split = 0;                   // 1: one lane per (column, segment, field)
     int row_wide = 0;                    // row pass at nx = 256 on 16 (0), 32 (1) or 64 (2) lanes per row
     bool tuned = false;
 };
 
-// One launch of the column pass in the handle's chosen form.
-static hipError_t launch_col(const sparc_handle *h, int split, int tc, int ms, const SegTab &sg,
-                             const ColArgs &ca, const Clock &clk, const Sink &sk)
+// One launch of the column pass in the handle's chosen shape.
+static hipError_t launch_col(const sparc_handle *h, int tc, int ms, const SegTab &sg, const ColArgs &ca,
+                             const Clock &clk, const Sink &sk)
 {
-    return split ? launch_colsplit(tc, ms, h->d, h->bg, h->cf, sg, h->st, ca, h->src, clk, sk, h->stream)
-                 : launch_colseg(tc, ms, h->d, h->bg, h->cf, sg, h->st, ca, h->src, clk, sk, h->stream);
+    return launch_colsplit(tc, ms, h->d, h->bg, h->cf, sg, h->st, ca, h->src, clk, sk, h->stream);
 }
 
 static thread_local std::string g_create_err;
@@ -714,10 +714,7 @@ static int setup_seg(sparc_handle *h)
     h->seg = sg;
     h->seg_ms = m;
     h->seg_cands.push_back(sg);
-    // untuned form and block width: SPARC_COLSEG_SPLIT=1 selects one lane
-    // per (column, segment, field); the widest block that fits
-    h->seg_split = 0;
-    if (const char *e = std::getenv("SPARC_COLSEG_SPLIT")) h->seg_split = std::atoi(e) == 1;
+    // untuned block width: the widest block that fits
     h->row_wide = 0;
     if (const char *e = std::getenv("SPARC_ROW_WIDE")) {
         const int v = std::atoi(e);
@@ -725,7 +722,7 @@ static int setup_seg(sparc_handle *h)
     }
     h->seg_tc = 1;
     for (int tc : {2, 4, 8, 16}) {
-        const bool fits = h->seg_split ? (tc * sg.P + 63) / 64 <= 2 : tc * sg.P <= 256;
+        const bool fits = (tc * sg.P + 63) / 64 <= 2;
         if (h->nx % tc == 0 && fits) h->seg_tc = tc;
     }
     return SPARC_OK;
@@ -832,7 +829,7 @@ static int enqueue_step(sparc_handle *h, int toff, bool filt_vx, const Sink &sk)
         ca.last = stg == 5;
         ca.bnext = stg == 5 ? h->deltat : b[stg];
         int r = timed_launch(h, stg == 5 ? 1 : 0, [&] {
-            HCHECK(h, launch_col(h, h->seg_split, h->seg_tc, h->seg_ms, h->seg, ca, clk, sk));
+            HCHECK(h, launch_col(h, h->seg_tc, h->seg_ms, h->seg, ca, clk, sk));
             return SPARC_OK;
         });
         if (r) return r;
@@ -988,7 +985,6 @@ int sparc_query_int(const sparc_handle *h, const char *name, int *v)
     else if (n == "nzpml") *v = h->d.nzpml;
     else if (n == "nph") *v = h->d.nph;
     else if (n == "colseg_tc") *v = h->seg_tc;
-    else if (n == "colseg_split") *v = h->seg_split;
     else if (n == "row_wide") *v = h->row_wide;
     else if (n == "colseg_ms") *v = h->seg_ms;
     else if (n == "colseg_p") *v = h->seg.P;
@@ -1134,6 +1130,7 @@ int sparc_set_forward_sources(sparc_handle *h, const double *xsrc, int maxtime)
     HCHECK(h, upload(h->src_mem, &dfw, h->fwdsource, h->stream));
     HCHECK(h, hipStreamSynchronize(h->stream));
     Src &s = h->src;
+    s = Src{};
     s.mode = 1;
     s.row = h->e_rad;
     s.limit = h->forcing_length * h->cfs;
@@ -1145,6 +1142,52 @@ int sparc_set_forward_sources(sparc_handle *h, const double *xsrc, int maxtime)
     s.timestep = h->timestep;
     s.cadforcing = h->cadforcing;
     s.num_steps = h->num_steps;
+    h->have_src = true;
+    return SPARC_OK;
+}
+
+int sparc_set_forward_profile(sparc_handle *h, const double *vr, const double *zprof, const double *series,
+                              int nseries, int maxtime)
+{
+    if (!h || !vr || !zprof || !series || nseries < 1 || maxtime < 0) return SPARC_E_ARG;
+    if (h->adjoint) return fail(h, SPARC_E_STATE, "forward sources on an adjoint handle");
+    int zlo = -1, zhi = -1;
+    for (int k = 0; k < h->nz; k++) {
+        if (!std::isfinite(zprof[k])) return fail(h, SPARC_E_ARG, "non-finite source z profile");
+        if (zprof[k] != 0.0) {
+            if (zlo < 0) zlo = k;
+            zhi = k;
+        }
+    }
+    for (size_t q = 0; q < (size_t)h->S * h->nx; q++)
+        if (!std::isfinite(vr[q])) return fail(h, SPARC_E_ARG, "non-finite source x profile");
+    for (int k = 0; k < nseries; k++)
+        if (!std::isfinite(series[k])) return fail(h, SPARC_E_ARG, "non-finite source time series");
+    HCHECK(h, hipStreamSynchronize(h->stream));
+    set_times(h, maxtime);
+    h->fwdsource.assign(series, series + nseries);
+    h->src_mem.release();
+    double *dvr = nullptr, *dfw = nullptr, *dzp = nullptr;
+    HCHECK(h, upload(h->src_mem, &dvr, std::vector<double>(vr, vr + (size_t)h->S * h->nx), h->stream));
+    HCHECK(h, upload(h->src_mem, &dfw, h->fwdsource, h->stream));
+    HCHECK(h, upload(h->src_mem, &dzp, std::vector<double>(zprof, zprof + h->nz), h->stream));
+    HCHECK(h, hipStreamSynchronize(h->stream));
+    Src &s = h->src;
+    s = Src{};
+    s.mode = zlo >= 0 ? 3 : 0;
+    s.row = -1;
+    s.limit = h->forcing_length * h->cfs;
+    s.delta_width = h->delta_width;
+    s.vr = dvr;
+    s.nvr = 1;
+    s.fwd = dfw;
+    s.nfwd = nseries;
+    s.timestep = h->timestep;
+    s.cadforcing = h->cadforcing;
+    s.num_steps = h->num_steps;
+    s.zprof = dzp;
+    s.zlo = zlo;
+    s.zhi = zhi;
     h->have_src = true;
     return SPARC_OK;
 }
@@ -1167,6 +1210,7 @@ int sparc_set_adjoint_sources(sparc_handle *h, const double *vr_dev, int nt, int
                                  (size_t)ncopy * nx * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
     HCHECK(h, hipStreamSynchronize(h->stream));
     Src &s = h->src;
+    s = Src{};
     s.mode = 2;
     s.row = h->o_rad;
     s.limit = h->forcing_length * h->cfs;
@@ -1230,29 +1274,63 @@ static int tune_rows(sparc_handle *h)
     return SPARC_OK;
 }
 
-// Pick the segment length, column-block width and form (one lane per column
-// and segment, or per column, segment and field) of the column pass for this
+// Column-pass shapes already timed in this process, by device and grid
+// (nx, nz, sources, layer widths): later handles of the same grid take the
+// same shape, so identical runs round identically across handles.
+namespace {
+struct TuneKey {
+    int v[7];
+    bool operator<(const TuneKey &o) const { return std::lexicographical_compare(v, v + 7, o.v, o.v + 7); }
+};
+struct TuneChoice { int ms, tc; };
+std::mutex g_tune_mu;
+std::map<TuneKey, TuneChoice> g_tuned;
+}  // namespace
+
+// Pick the segment length and column-block width of the column pass for this
 // grid and batch by timing each candidate on the handle's stream (state is
-// reset afterwards).  SPARC_COLSEG_MS, SPARC_COLSEG_TC and SPARC_COLSEG_SPLIT
-// (0 or 1) in the environment force a choice.
+// reset afterwards), or take the shape an earlier handle of the same grid
+// chose in this process.  SPARC_COLSEG_MS and SPARC_COLSEG_TC in the
+// environment force a choice.
 static int autotune(sparc_handle *h)
 {
     if (h->tuned) return SPARC_OK;
     h->tuned = true;
-    int force_tc = 0, force_split = -1;
+    const TuneKey key{{h->device, h->nx, h->nz, h->S, h->nb, h->nt, h->nh}};
+    const bool forced = h->seg_forced || std::getenv("SPARC_COLSEG_TC");
+    auto fits = [&](int tc, const SegTab &sg) { return h->nx % tc == 0 && (tc * sg.P + 63) / 64 <= 2; };
+    if (!forced) {
+        TuneChoice c{0, 0};
+        {
+            std::lock_guard<std::mutex> lk(g_tune_mu);
+            auto it = g_tuned.find(key);
+            if (it != g_tuned.end()) c = it->second;
+        }
+        if (c.tc) {
+            SegTab sg{};
+            bool have = false;
+            for (const SegTab &t : h->seg_cands)
+                if (t.m == c.ms) { sg = t; have = true; }
+            if (!have) {
+                const int r = build_seg(h, c.ms, sg);
+                if (r) return r;
+                h->seg_cands.push_back(sg);
+            }
+            h->seg = sg;
+            h->seg_ms = c.ms;
+            h->seg_tc = c.tc;
+            return tune_rows(h);
+        }
+    }
+    int force_tc = 0;
     if (const char *e = std::getenv("SPARC_COLSEG_TC")) {
         const int v = std::atoi(e);
         if (v == 1 || v == 2 || v == 4 || v == 8 || v == 16) force_tc = v;
     }
-    if (const char *e = std::getenv("SPARC_COLSEG_SPLIT")) {
-        const int v = std::atoi(e);
-        if (v == 0 || v == 1) force_split = v;
-    }
     // candidate segment lengths (only the forced one under SPARC_COLSEG_MS)
     std::vector<int> ms = h->seg_forced ? std::vector<int>{h->seg.m} : seg_lengths(h->nz);
-    if (force_tc && force_split >= 0 && ms.size() == 1) {
-        if (h->nx % force_tc == 0 && force_tc * h->seg.P <= 256) h->seg_tc = force_tc;
-        h->seg_split = force_split;
+    if (force_tc && ms.size() == 1) {
+        if (fits(force_tc, h->seg)) h->seg_tc = force_tc;
         return tune_rows(h);
     }
     Sink sk = empty_sink(h);
@@ -1267,7 +1345,7 @@ static int autotune(sparc_handle *h)
     HCHECK(h, hipEventCreate(&e1));
     float best = 1e30f;
     SegTab best_sg = h->seg;
-    int best_tc = 0, best_split = 0;
+    int best_tc = 0;
     for (int m : ms) {
         SegTab sg{};
         bool have = false;
@@ -1278,22 +1356,20 @@ static int autotune(sparc_handle *h)
             if (r) return r;
             h->seg_cands.push_back(sg);
         }
-        for (int split : {0, 1})
         for (int tc : {1, 2, 4, 8, 16}) {
             if (force_tc && tc != force_tc) continue;
-            if (force_split >= 0 && split != force_split) continue;
-            if (h->nx % tc || tc * sg.P > 256) continue;
-            if (launch_col(h, split, tc, m, sg, ca, clk, sk) != hipSuccess) {
+            if (!fits(tc, sg)) continue;
+            if (launch_col(h, tc, m, sg, ca, clk, sk) != hipSuccess) {
                 (void)hipGetLastError();     // shape not launchable here (LDS): skip it
                 continue;
             }
             HCHECK(h, hipEventRecord(e0, h->stream));
-            for (int r = 0; r < 10; r++) HCHECK(h, launch_col(h, split, tc, m, sg, ca, clk, sk));
+            for (int r = 0; r < 10; r++) HCHECK(h, launch_col(h, tc, m, sg, ca, clk, sk));
             HCHECK(h, hipEventRecord(e1, h->stream));
             HCHECK(h, hipEventSynchronize(e1));
             float t = 0.f;
             HCHECK(h, hipEventElapsedTime(&t, e0, e1));
-            if (t < best) { best = t; best_sg = sg; best_tc = tc; best_split = split; }
+            if (t < best) { best = t; best_sg = sg; best_tc = tc; }
         }
     }
     (void)hipEventDestroy(e0);
@@ -1302,7 +1378,10 @@ static int autotune(sparc_handle *h)
     h->seg = best_sg;
     h->seg_ms = best_sg.m;
     h->seg_tc = best_tc;
-    h->seg_split = best_split;
+    if (!forced) {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        g_tuned.emplace(key, TuneChoice{best_sg.m, best_tc});
+    }
     // the timed launches wrote only stage temporaries: restore the z-PML
     // stage copy, which equals its base between steps
     const size_t pz = (size_t)2 * h->S * h->d.nzpml * h->nx;

@@ -7,13 +7,14 @@
 // rows and columns, the RK stage update and, after the last stage,
 // FILTER_VARS_2D (physics2d.f90:509-556).
 //
-// Every column is split into P <= kSegMax segments of exactly MS rows, held in
-// registers by one lane each; a block covers TC columns (blockDim = P*TC).
-// Each z-derivative is: dbyd2 right-hand side from a 7-row window, local
-// Thomas solve of the segment, exchange of the segment's end values through
-// LDS, and the exact correction y - L_{s-1} V - F_{s+1} W.  Loads use clamped
-// row indices and stores go to a discard slot for padding rows, so the
-// unrolled row loops are straight-line code the scheduler can batch.
+// Every column is split into P <= kSegMax segments of exactly MS rows; one
+// lane holds one segment of one field of one column, and a block covers TC
+// columns (k_colsplit below).  Each z-derivative is: dbyd2 right-hand side
+// from a 7-row window, local Thomas solve of the segment, exchange of the
+// segment's end values through LDS, and the exact correction
+// y - L_{s-1} V - F_{s+1} W.  Loads use clamped row indices and stores go to a
+// discard slot for padding rows, so the unrolled row loops are straight-line
+// code the scheduler can batch.
 #include <type_traits>
 
 #include "sparc_device.h"
@@ -118,366 +119,8 @@ inline size_t colseg_lds_bytes(int tc, int P, int hw, int nzp)
     return sizeof(double) * (size_t)(3 * 2 * P * (4 * hw + 4) + 4 * tc * 2 * P + kZL * nzp);
 }
 
-template <int MS, int LAST>
-__global__ __launch_bounds__(256) void k_colseg(Dims d, Bg bg, DbyCoef cf, SegTab sg, State st,
-                                                ColArgs ca, Src src, Clock clk, Sink sink)
-{
-    extern __shared__ double lds_dyn[];
-    const int P = sg.P, n2P = 2 * P, nzp = sg.nzp;
-    const int TC = blockDim.x / P;
-    const int nband = n2P * (4 * sg.hw + 4);    // one banded inverse
-    double *const mv = lds_dyn;
-    double *const yb0 = lds_dyn + 3 * nband;
-    double *const yb[4] = {yb0, yb0 + TC * n2P, yb0 + 2 * TC * n2P, yb0 + 3 * TC * n2P};
-    double *const zl = yb0 + 4 * TC * n2P;
-    const int tid = threadIdx.x;
-    // Interface inverses and z tables into LDS; first read after the
-    // exchange barrier of the first derivatives, so the loads overlap the
-    // window loads issued before this is called.
-    auto stage_tables = [&]() {
-        // eight loads in flight per thread and round
-        const int nb3 = 3 * nband, stride = blockDim.x;
-        for (int q0 = tid; q0 < nb3; q0 += 8 * stride) {
-            double v[8];
-#pragma unroll
-            for (int u = 0; u < 8; u++) v[u] = sg.Minv[min(q0 + u * stride, nb3 - 1)];
-#pragma unroll
-            for (int u = 0; u < 8; u++)
-                if (q0 + u * stride < nb3) mv[q0 + u * stride] = v[u];
-        }
-        const int nzl = kZL * nzp;
-        for (int q0 = tid; q0 < nzl; q0 += 8 * stride) {
-            double v[8];
-#pragma unroll
-            for (int u = 0; u < 8; u++) {
-                const int q = min(q0 + u * stride, nzl - 1), r = q / nzp;
-                v[u] = sg.ztab[(size_t)lrow_src(r) * nzp + (q - r * nzp)];
-            }
-#pragma unroll
-            for (int u = 0; u < 8; u++)
-                if (q0 + u * stride < nzl) zl[q0 + u * stride] = v[u];
-        }
-    };
-    const int c = tid % TC, s = tid / TC;
-    // XCD-aware block order: workgroups are dealt to the 8 XCDs in turn, so
-    // workgroup id is remapped to give each XCD a contiguous range of column
-    // groups; neighbouring groups share 128-byte lines, now in one L2.
-    int bx = blockIdx.x, by = blockIdx.y;
-    {
-        const int n = gridDim.x * gridDim.y;
-        if ((n & 7) == 0) {
-            const int id = blockIdx.x + gridDim.x * blockIdx.y;
-            const int l = (id & 7) * (n >> 3) + (id >> 3);
-            bx = l % gridDim.x;
-            by = l / gridDim.x;
-        }
-    }
-    const int x = bx * TC + c, b = by;
-    const int nx = d.nx, nz = d.nz;
-    const bool act = s < P;
-    const int sc = act ? s : 0;
-    const int a = sc * MS;                  // first row of the segment
-    const bool first = act && s == 0;
-    const size_t col = (size_t)b * nz * nx + x;
-    const int time = clock_t_(clk);
-    const int nb = d.npmlbot, ntop = d.npmltop, nzpml = d.nzpml;
-    const size_t pz0 = (size_t)b * nzpml * nx + x;
-    const size_t pz1 = (size_t)(d.S + b) * nzpml * nx + x;
-    const double *zt = sg.ztab + a;         // zt[row * nzp + i] = table(row, a + i)
-    const double *zs = zl + a;              // zs[r * nzp + i] = staged table r at row a + i
-    double *const dump = st.discard + tid;  // destination of padding-row stores
-
-    auto rowq = [&](int i) -> size_t { return col + (size_t)min(a + i, nz - 1) * nx; };
-    auto valid = [&](int i) -> bool { return act && a + i < nz; };
-    auto gload = [&](const double *F) {
-        return [=](int k) -> double {
-            const int j = min(max(a - 3 + k, 0), nz - 1);
-            return F[col + (size_t)j * nx];
-        };
-    };
-    // z-PML row index of row j (clamped into the table for other rows)
-    auto pmlrow = [&](int j) -> int {
-        const int jc = min(j, nz - 1);
-        return jc < nb ? jc : min(nzpml - 1, nb + max(jc - (nz - ntop), 0));
-    };
-    auto is_pml = [&](int j) -> bool { return j < nb || (j >= nz - ntop && j < nz); };
-
-    // Own rows of xi_z, v_z, v_x are kept from the right-hand-side windows for
-    // short segments; long segments reload them (register budget).
-    constexpr bool KEEP = MS <= 12;
-    constexpr int KO = KEEP ? MS : 1;
-    double Yz[MS], Yv[MS], Yx[MS];          // d xi_z/dz, d v_z/dz, d v_x/dz (unstretched)
-    double oz[KO], ov[KO], ox[KO];          // own rows of xi_z, v_z, v_x
-    double w[MS];                           // RHS v_z without the pressure gradient
-    const bool segpml = a < nb || a + MS > nz - ntop;   // segment touches PML rows
-    auto load_win = [&](const double *F, double(&win)[MS + 6]) {
-#pragma unroll
-        for (int k = 0; k < MS + 6; k++) win[k] = F[col + (size_t)min(max(a - 3 + k, 0), nz - 1) * nx];
-    };
-    auto rhs_win = [&](const double(&win)[MS + 6], double(&R)[MS], double *own, int ibc, double pre0,
-                       double preN) {
-        seg_rhs<MS>([&](int k) { return win[k]; }, R, a, nz, first, ibc, pre0, preN, cf);
-        if (KEEP && own) {
-#pragma unroll
-            for (int i = 0; i < MS; i++) own[i] = win[3 + i];
-        }
-    };
-
-    // --- B: d xi_z/dz (ibc 3, top row preset, physics2d.f90:954-955),
-    //        d v_z/dz (ibc 1, :1076) and d v_x/dz (ibc 1, :1070): three
-    //        independent solves, one exchange of segment end values
-    double wz[MS + 6], wv[MS + 6], wx[MS + 6];
-    load_win(ca.T[1], wz);
-    load_win(ca.T[3], wv);
-    load_win(ca.T[2], wx);
-    const double topv = st.topbc[(size_t)b * nx + x];
-    stage_tables();
-    // source value of this column at its injection row (physics2d.f90:1051-1061)
-    double srcv = 0.0;
-    const int isrc = src.row - a;
-    if (act && isrc >= 0 && isrc < MS && time <= src.limit) {
-        if (src.mode == 1) {
-            const double fs = (time >= 0 && time < src.nfwd) ? src.fwd[time] : 0.0;
-            srcv = src.vr[(size_t)b * nx + x] * fs * src.delta_width / bg.rho0_src[x];
-        } else if (src.mode == 2) {
-            srcv = lagrange_forcing(src, b, x, nx, time) * src.delta_width / bg.rho0_src[x];
-        }
-    }
-
-    rhs_win(wz, Yz, oz, 3, 0.0, topv);
-    seg_solve<MS>(Yz, zt + zrow(2, 0) * nzp, zt + zrow(2, 1) * nzp, zt + zrow(2, 2) * nzp);
-    rhs_win(wv, Yv, ov, 1, 0.0, 0.0);
-    seg_solve<MS>(Yv, zt + zrow(0, 0) * nzp, zt + zrow(0, 1) * nzp, zt + zrow(0, 2) * nzp);
-    rhs_win(wx, Yx, ox, 1, 0.0, 0.0);
-    seg_solve<MS>(Yx, zt + zrow(0, 0) * nzp, zt + zrow(0, 1) * nzp, zt + zrow(0, 2) * nzp);
-    if (act) {
-        yb[0][c * n2P + 2 * s] = Yz[0]; yb[0][c * n2P + 2 * s + 1] = Yz[MS - 1];
-        yb[1][c * n2P + 2 * s] = Yv[0]; yb[1][c * n2P + 2 * s + 1] = Yv[MS - 1];
-        yb[2][c * n2P + 2 * s] = Yx[0]; yb[2][c * n2P + 2 * s + 1] = Yx[MS - 1];
-    }
-    __syncthreads();
-    if (act) {
-        seg_correct<MS>(Yz, s, P, sg.hw, yb[0] + c * n2P, mv + 2 * nband, zt + zrow(2, 3) * nzp,
-                        zt + zrow(2, 4) * nzp);
-        seg_correct<MS>(Yv, s, P, sg.hw, yb[1] + c * n2P, mv, zt + zrow(0, 3) * nzp, zt + zrow(0, 4) * nzp);
-        seg_correct<MS>(Yx, s, P, sg.hw, yb[2] + c * n2P, mv, zt + zrow(0, 3) * nzp, zt + zrow(0, 4) * nzp);
-    }
-    // z-PML (:957-967), divergence, p and rho' (:990-998, :1043), the flow
-    // terms of RHS v_z (:1073-1077) and the flow part of RHS v_x (:1067-1071)
-    // Every row's loads of a chunk are issued before the chunk's first store:
-    // stores may alias later rows' inputs (T and TN are one buffer after the
-    // first stage), so the compiler keeps source order, and a load after a
-    // store costs one memory round trip.  PML segments load twice as many
-    // values and use two chunks.
-    auto epi_b = [&](auto pml_c) {
-        constexpr bool PML = decltype(pml_c)::value;
-        constexpr int CH = PML ? (MS + 1) / 2 : MS;
-#pragma unroll
-        for (int c0 = 0; c0 < MS; c0 += CH) {
-            double ldix[CH], ldvz[CH], ldvx[CH], lxiz[CH], lc2r[CH], lv0x[CH], lv0z[CH];
-            double lpsi[CH], laz[CH], lbz[CH], lpb[CH], lfz[CH];
-#pragma unroll
-            for (int k = 0; k < CH; k++) {
-                const int i = c0 + k;
-                if (i >= MS) break;
-                const size_t qj = rowq(i), gq = qj - (size_t)b * nz * nx;
-                ldix[k] = st.dxixdx[qj];
-                ldvz[k] = st.dxvz[qj];
-                ldvx[k] = st.dxvx[qj];
-                lxiz[k] = KEEP ? oz[KEEP ? i : 0] : ca.T[1][qj];
-                lc2r[k] = bg.c2rho0[gq];
-                lv0x[k] = bg.v0x[gq];
-                lv0z[k] = bg.v0z[gq];
-                if (PML) {
-                    const int pj = pmlrow(a + i);
-                    const size_t qp = pz0 + (size_t)pj * nx, qb = (size_t)pj * nx + x;
-                    lpsi[k] = st.pz_tmp[qp];
-                    lpb[k] = st.pz_base[qp];
-                    laz[k] = bg.az[qb];
-                    lbz[k] = bg.bz[qb];
-                    lfz[k] = bg.fz[qb];
-                }
-            }
-#pragma unroll
-            for (int k = 0; k < CH; k++) {
-                const int i = c0 + k;
-                if (i >= MS) break;
-                const int j = a + i;
-                const size_t qj = rowq(i);
-                const double str = zs[L_STR * nzp + i], r0 = zs[L_RHO * nzp + i];
-                const double rinv = zs[L_RINV * nzp + i], gz = zs[L_G * nzp + i];
-                double dz = Yz[i] * str;
-                double fdec = 1.0;
-                if (PML) {
-                    const bool pm = is_pml(j);
-                    // functiondecay scales d p0/dz and d rho0/dz in the PML rows (physics2d.f90:303-318)
-                    fdec = pm ? lfz[k] : 1.0;
-                    const double prhs = laz[k] * dz + lbz[k] * lpsi[k];
-                    dz = pm ? dz + lpsi[k] : dz;
-                    const double pnew = lpb[k] + ca.bnext * prhs;
-                    if (pm && act) {
-                        const size_t qp = pz0 + (size_t)pmlrow(j) * nx;
-                        st.pz_tmp[qp] = pnew;
-                        if (LAST) st.pz_base[qp] = pnew;
-                    }
-                }
-                const double gp0 = zs[L_GP * nzp + i] * fdec, gr0 = zs[L_GRHO * nzp + i] * fdec;
-                const double xiz = lxiz[k];
-                const double div = ldix[k] + dz;
-                const double pv = -(lc2r[k] * div) - gp0 * xiz;
-                *(valid(i) ? &st.p[qj] : dump) = pv;
-                const double rho = -(r0 * div) - gr0 * xiz;
-                double bzv = -rinv * (rho * gz);
-                bzv = (i == isrc) ? bzv + srcv : bzv;
-                const double dzvz = Yv[i] * str;
-                bzv = bzv - 2.0 * lv0x[k] * ldvz[k];
-                w[i] = bzv - 2.0 * lv0z[k] * dzvz;
-                double rvx = -(2.0 * lv0x[k] * ldvx[k]) - 2.0 * lv0z[k] * (Yx[i] * str);
-                rvx = (j == 0 || x == 0 || x == nx - 1) ? 0.0 : rvx;
-                *(valid(i) ? &st.rvx[qj] : dump) = rvx;
-            }
-        }
-    };
-    if (segpml) epi_b(std::true_type{});
-    else epi_b(std::false_type{});
-
-    // --- A4: d p/dz (ibc 2, bottom row zero, :1023) with the z-PML (:1025-1035),
-    //         RHS v_z, zeroed rows/columns (:1083-1093), RK stage update.
-    // p of the whole column was written above; the block barrier makes the
-    // neighbouring segments' rows visible.
-    __syncthreads();
-    double (&Y)[MS] = Yz;
-    {
-        double wp[MS + 6];
-        load_win(st.p, wp);
-        rhs_win(wp, Y, nullptr, 2, 0.0, 0.0);
-    }
-    seg_solve<MS>(Y, zs, zs + nzp, zs + 2 * nzp);   // staged rows 0..4 = zrow(1, 0..4)
-    if (act) { yb[3][c * n2P + 2 * s] = Y[0]; yb[3][c * n2P + 2 * s + 1] = Y[MS - 1]; }
-    __syncthreads();
-    if (act) seg_correct<MS>(Y, s, P, sg.hw, yb[3] + c * n2P, mv + nband, zs + 3 * nzp, zs + 4 * nzp);
-    const int sslot = (LAST && sink.snap) ? out_slot(time, clk.steps, sink.k0, sink.nsnap) : -1;
-    const bool snap = sslot >= 0;
-    const size_t slot = snap ? ((size_t)b * sink.nsnap + sslot) * sink.nfield : 0;
-    auto epi_4 = [&](auto pml_c) {
-        constexpr bool PML = decltype(pml_c)::value;
-        constexpr int CH = PML ? (MS + 1) / 2 : MS;
-#pragma unroll
-        for (int c0 = 0; c0 < MS; c0 += CH) {
-            double la0[CH], la1[CH], la3[CH], lvx[CH], lvz[CH], lpsi[CH], laz[CH], lbz[CH], lpb[CH];
-#pragma unroll
-            for (int k = 0; k < CH; k++) {
-                const int i = c0 + k;
-                if (i >= MS) break;
-                const size_t qj = rowq(i);
-                lvx[k] = KEEP ? ox[KEEP ? i : 0] : ca.T[2][qj];
-                lvz[k] = KEEP ? ov[KEEP ? i : 0] : ca.T[3][qj];
-                la0[k] = st.a[0][qj];
-                la1[k] = st.a[1][qj];
-                la3[k] = st.a[3][qj];
-                if (PML) {
-                    const int pj = pmlrow(a + i);
-                    const size_t qp = pz1 + (size_t)pj * nx, qb = (size_t)pj * nx + x;
-                    lpsi[k] = st.pz_tmp[qp];
-                    lpb[k] = st.pz_base[qp];
-                    laz[k] = bg.az[qb];
-                    lbz[k] = bg.bz[qb];
-                }
-            }
-#pragma unroll
-            for (int k = 0; k < CH; k++) {
-                const int i = c0 + k;
-                if (i >= MS) break;
-                const int j = a + i;
-                const size_t qj = rowq(i);
-                const double rinv = zs[L_RINV * nzp + i];
-                double gpz = Y[i] * zs[L_STR * nzp + i];
-                if (PML) {
-                    const bool pm = is_pml(j);
-                    const double prhs = laz[k] * gpz + lbz[k] * lpsi[k];
-                    gpz = pm ? gpz + lpsi[k] : gpz;
-                    const double pnew = lpb[k] + ca.bnext * prhs;
-                    if (pm && act) {
-                        const size_t qp = pz1 + (size_t)pmlrow(j) * nx;
-                        st.pz_tmp[qp] = pnew;
-                        if (LAST) st.pz_base[qp] = pnew;
-                    }
-                }
-                double rvz = -rinv * gpz + w[i];
-                double rxix = lvx[k], rxiz = lvz[k];
-                const bool zero = j == 0 || x == 0 || x == nx - 1;
-                rvz = zero ? 0.0 : rvz;
-                rxix = zero ? 0.0 : rxix;
-                rxiz = zero ? 0.0 : rxiz;
-                const bool v = valid(i);
-                *(v ? &ca.TN[0][qj] : dump) = la0[k] + ca.bnext * rxix;
-                *(v ? &ca.TN[1][qj] : dump) = la1[k] + ca.bnext * rxiz;
-                *(v ? &ca.TN[3][qj] : dump) = la3[k] + ca.bnext * rvz;
-                if (snap && v)
-                    sink.snap[((slot + (sink.adjoint ? 3 : 5)) * nz + j) * nx + x] = rvz;
-            }
-        }
-    };
-    if (segpml) epi_4(std::true_type{});
-    else epi_4(std::false_type{});
-    if (LAST) {
-        // FILTER_VARS_2D (physics2d.f90:509-556) on xi_x, xi_z, v_z of the
-        // column: each field's rows a-5 .. a+MS+4 are read once (neighbour
-        // segments' rows were written before the barrier) and filtered in
-        // registers.
-        __syncthreads();
-        const int oslot = sink.vz_out ? out_slot(time, clk.steps, sink.k0, sink.nout) : -1;
-        const bool out = oslot >= 0;
-        const int nh = d.nh;
-        const bool xin = x >= nh - 1 && x <= nx - nh;
-        // the three windows are loaded before the first store (one memory
-        // round) when they fit the register budget, else field by field
-        constexpr int FB = MS <= 16 ? 3 : 1;
-#pragma unroll
-        for (int f0 = 0; f0 < 3; f0 += FB) {
-            double w[FB][MS + 10];
-#pragma unroll
-            for (int u = 0; u < FB; u++) {
-                const double *F = ca.TN[f0 + u == 2 ? 3 : f0 + u];
-#pragma unroll
-                for (int q = 0; q < MS + 10; q++) w[u][q] = F[col + (size_t)min(max(a - 5 + q, 0), nz - 1) * nx];
-            }
-            asm volatile("" ::: "memory");
-#pragma unroll
-            for (int u = 0; u < FB; u++) {
-                const int f = f0 + u, fi = f == 2 ? 3 : f;
-#pragma unroll
-                for (int i = 0; i < MS; i++) {
-                    const int j = a + i;
-                    const int q = i + 5;   // w[u][q] = row j
-                    const double *wf = w[u];
-                    const double f11 = 0.75390625 * wf[q] + 0.5 * 0.41015625 * (wf[q - 1] + wf[q + 1]) +
-                                       0.5 * -0.234375 * (wf[q - 2] + wf[q + 2]) +
-                                       0.5 * 0.087890625 * (wf[q - 3] + wf[q + 3]) +
-                                       0.5 * -0.01953125 * (wf[q - 4] + wf[q + 4]) +
-                                       0.5 * 0.001953125 * (wf[q - 5] + wf[q + 5]);
-                    const double f2 = 0.5 * (wf[q + 1] + wf[q - 1]);
-                    double v = wf[q];
-                    if (xin && j >= 5 && j <= nz - 2) v = (j <= nz - 6) ? f11 : f2;
-                    const bool ok = valid(i);
-                    const size_t qj = rowq(i);
-                    *(ok ? &st.a[fi][qj] : dump) = v;
-                    if (f == 2 && ok && out && j == sink.orow)
-                        sink.vz_out[((size_t)b * sink.nout + oslot) * nx + x] = v;
-                    if (ok && snap) {
-                        const size_t o = (size_t)j * nx + x;
-                        const int fs = sink.adjoint ? (f == 2 ? 1 : -1) : (f == 2 ? 3 : f);
-                        if (fs >= 0) sink.snap[(slot + fs) * nz * nx + o] = v;
-                    }
-                }
-            }
-            asm volatile("" ::: "memory");
-        }
-    }
-}
-
 // ---------------------------------------------------------------------------
-// The same column pass with one lane per (column, segment, field): the three
+// The column pass, one lane per (column, segment, field): the three
 // independent first derivatives (xi_z, v_z, v_x) run in three lane groups of
 // G = roundup(TC*P, 64) <= 128 threads each (blockDim = 3G, so every wave
 // holds one field), which splits the window loads, solves, corrections and epilogue
@@ -489,7 +132,8 @@ __global__ __launch_bounds__(256) void k_colseg(Dims d, Bg bg, DbyCoef cf, SegTa
 // window reads of T[1] are complete (T and TN are one buffer after the first
 // stage); v_z (TN[3]) after the third.
 //
-// Dynamic LDS: as k_colseg, plus the group-1 part of RHS v_z [TC][nzp].
+// Dynamic LDS: the banded inverses, the end-value exchange buffers and the
+// staged z tables (colseg_lds_bytes), plus the parts of RHS v_z [2][TC][nzp].
 inline size_t colsplit_lds_bytes(int tc, int P, int hw, int nzp)
 {
     return colseg_lds_bytes(tc, P, hw, nzp) + sizeof(double) * (size_t)2 * tc * nzp;
@@ -585,6 +229,7 @@ __global__ __launch_bounds__(384) void k_colsplit(Dims d, Bg bg, DbyCoef cf, Seg
             srcv = lagrange_forcing(src, b, x, nx, time) * src.delta_width / bg.rho0_src[x];
         }
     }
+    const double pamp = (f == 0 && act) ? profile_amp(src, b, x, nx, a, MS, time) : 0.0;
     const int slot_f = f == 0 ? 2 : 0;                 // dbyd2 table set: ibc 3 or ibc 1
     double Y[MS], own[MS];
     seg_rhs<MS>([&](int k) { return win[k]; }, Y, a, nz, first, f == 0 ? 3 : 1, 0.0, topv, cf);
@@ -662,7 +307,8 @@ __global__ __launch_bounds__(384) void k_colsplit(Dims d, Bg bg, DbyCoef cf, Seg
                 const double pv = -(pb[i] * div) - gp0 * xiz;
                 *(valid(i) ? &st.p[qj] : dump) = pv;
                 const double rho = -(r0 * div) - gr0 * xiz;
-                const double bzv = -rinv * (rho * gz);
+                double bzv = -rinv * (rho * gz);
+                if (src.mode == 3) bzv = bzv + pamp * src.zprof[min(j, nz - 1)];
                 if (act) w0[i] = (i == isrc) ? bzv + srcv : bzv;
             }
             }
@@ -820,11 +466,7 @@ hipError_t colseg_init()
     if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && v > 0)
         g_lds_max = v;
 #define A(MSV)                                                                                          \
-    if (hipFuncSetAttribute((const void *)k_colseg<MSV, 0>, hipFuncAttributeMaxDynamicSharedMemorySize, \
-                            g_lds_max) != hipSuccess ||                                                 \
-        hipFuncSetAttribute((const void *)k_colseg<MSV, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, \
-                            g_lds_max) != hipSuccess ||                                                 \
-        hipFuncSetAttribute((const void *)k_colsplit<MSV, 0>, hipFuncAttributeMaxDynamicSharedMemorySize, \
+    if (hipFuncSetAttribute((const void *)k_colsplit<MSV, 0>, hipFuncAttributeMaxDynamicSharedMemorySize, \
                             g_lds_max) != hipSuccess ||                                                 \
         hipFuncSetAttribute((const void *)k_colsplit<MSV, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, \
                             g_lds_max) != hipSuccess) {                                                 \
@@ -834,26 +476,6 @@ hipError_t colseg_init()
     SPARC_COLSEG_SIZES(A)
 #undef A
     return hipSuccess;
-}
-
-hipError_t launch_colseg(int tc, int ms, const Dims &d, const Bg &bg, const DbyCoef &cf, const SegTab &sg,
-                         const State &s, const ColArgs &ca, const Src &src, const Clock &c, const Sink &sk,
-                         hipStream_t st)
-{
-    if ((tc != 1 && tc != 2 && tc != 4 && tc != 8 && tc != 16) || d.nx % tc || sg.m != ms || sg.P > kSegMax || tc * sg.P > 256)
-        return hipErrorInvalidValue;
-    dim3 grid(d.nx / tc, d.S), block(tc * sg.P);
-    const size_t shm = colseg_lds_bytes(tc, sg.P, sg.hw, sg.nzp);
-    if (shm > (size_t)g_lds_max) return hipErrorInvalidValue;
-#define L(MSV)                                                                                         \
-    if (ms == MSV) {                                                                                   \
-        if (ca.last) hipLaunchKernelGGL((k_colseg<MSV, 1>), grid, block, shm, st, d, bg, cf, sg, s, ca, src, c, sk); \
-        else hipLaunchKernelGGL((k_colseg<MSV, 0>), grid, block, shm, st, d, bg, cf, sg, s, ca, src, c, sk);         \
-        return hipGetLastError();                                                                      \
-    }
-    SPARC_COLSEG_SIZES(L)
-#undef L
-    return hipErrorInvalidValue;
 }
 
 }  // namespace sparc

@@ -79,8 +79,8 @@ struct State {
 
 // Source term of the RHS (physics2d.f90:1051-1061).
 struct Src {
-    int mode;               // 0 none, 1 forward, 2 adjoint
-    int row;                // e_rad (forward) or o_rad (adjoint), 0-based
+    int mode;               // 0 none, 1 forward, 2 adjoint, 3 forward with a z profile
+    int row;                // e_rad (forward) or o_rad (adjoint), 0-based; -1 in mode 3
     int limit;              // forcing_length * cadforcing_step
     double delta_width;
     const double *vr;       // forward: [S][nx]; adjoint: [S][nvr][nx]
@@ -89,7 +89,21 @@ struct Src {
     int nfwd;
     double timestep, cadforcing;
     int num_steps;
+    // mode 3: RHS v_z(x, z) += vr[s][x] * fwd[t] * zprof[z] on rows zlo..zhi
+    const double *zprof;    // [nz], zero outside zlo..zhi
+    int zlo, zhi;
 };
+
+// Mode-3 source amplitude vr * fwd(t) of one column, for a segment of rows
+// a .. a+ms-1; 0 when the segment misses the profile's rows or the source
+// has ended.
+__device__ __forceinline__ double profile_amp(const Src &src, int b, int x, int nx, int a, int ms,
+                                              int time)
+{
+    if (src.mode != 3 || a > src.zhi || a + ms - 1 < src.zlo || time > src.limit) return 0.0;
+    const double fs = (time >= 0 && time < src.nfwd) ? src.fwd[time] : 0.0;
+    return src.vr[(size_t)b * nx + x] * fs;
+}
 
 // Per-launch time information. t = *tctr + toff.
 struct Clock {

@@ -13,14 +13,11 @@ hipError_t launch_ddx_rows(const double *in, double *out, int nrows, int nx, con
 // wide = 2 runs 64 lanes per row (one row per wave).
 hipError_t launch_row(int mode, const Dims &d, const Bg &bg, const State &s, const RowArgs &ra,
                       const Clock &c, const Sink &sk, hipStream_t st, int wide);
-// Segmented column pass; tc in {1, 2, 4, 8, 16} columns per block, ms = rows
-// per segment, one of SPARC_COLSEG_SIZES.  launch_colsplit runs the same pass
-// with one lane per (column, segment, field).
+// Segmented column pass, one lane per (column, segment, field); tc in
+// {1, 2, 4, 8, 16} columns per block, ms = rows per segment, one of
+// SPARC_COLSEG_SIZES.
 #define SPARC_COLSEG_SIZES(X) X(4) X(6) X(8) X(10) X(12) X(16) X(19) X(24) X(32)
 hipError_t colseg_init();
-hipError_t launch_colseg(int tc, int ms, const Dims &d, const Bg &bg, const DbyCoef &cf, const SegTab &sg,
-                         const State &s, const ColArgs &ca, const Src &src, const Clock &c, const Sink &sk,
-                         hipStream_t st);
 hipError_t launch_colsplit(int tc, int ms, const Dims &d, const Bg &bg, const DbyCoef &cf, const SegTab &sg,
                            const State &s, const ColArgs &ca, const Src &src, const Clock &c, const Sink &sk,
                            hipStream_t st);

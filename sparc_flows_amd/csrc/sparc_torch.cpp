@@ -111,6 +111,16 @@ struct Solver : torch::CustomClassHolder {
         at::Tensor x = cpu_f64(xsrc, nsrc, "xsrc");
         check(sparc_set_forward_sources(h, x.data_ptr<double>(), (int)maxtime), h, "sparc_set_forward_sources");
     }
+    void set_forward_profile(at::Tensor vr, at::Tensor zprof, at::Tensor series, int64_t maxtime)
+    {
+        at::Tensor v = cpu_f64(vr, nsrc * nx, "vr");
+        at::Tensor z = cpu_f64(zprof, nz, "zprof");
+        TORCH_CHECK(series.numel() > 0, "series must not be empty");
+        at::Tensor t = cpu_f64(series, series.numel(), "series");
+        check(sparc_set_forward_profile(h, v.data_ptr<double>(), z.data_ptr<double>(), t.data_ptr<double>(),
+                                        (int)t.numel(), (int)maxtime),
+              h, "sparc_set_forward_profile");
+    }
     void set_adjoint_sources(at::Tensor vr, int64_t maxtime)
     {
         check_dev(vr, "vr");
@@ -284,6 +294,7 @@ TORCH_LIBRARY(sparc, m)
         .def("kernels_add", &Solver::kernels_add)
         .def("kernels_end", &Solver::kernels_end)
         .def("set_forward_sources", &Solver::set_forward_sources)
+        .def("set_forward_profile", &Solver::set_forward_profile)
         .def("set_adjoint_sources", &Solver::set_adjoint_sources)
         .def("run", &Solver::run)
         .def("reset", &Solver::reset)

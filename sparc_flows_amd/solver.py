@@ -71,11 +71,25 @@ class Sparc:
             self._s = Solver(p.nx, p.nz, p.npmlbot, p.npmltop, p.npmlhor, self.nsrc, int(adjoint), scalars,
                              _as_tensor(model), _as_tensor(c2d), _as_tensor(psi), self.device.index)
         self.maxtime = None
+        self._own_stream = True
 
     # ---- plumbing ----
     def _stream(self):
-        self._s.set_stream(torch.cuda.current_stream(self.device).cuda_stream)
+        """The binding, queuing on torch's current stream.  On torch's default
+        stream the handle queues on a stream of its own instead (a HIP stream
+        that orders with the default stream but not with other handles'), so
+        then every call that queues device work waits for it (_done): a
+        handle's outputs are complete, and its inputs free, when the call
+        returns, whichever handle or torch op uses them next."""
+        cs = torch.cuda.current_stream(self.device).cuda_stream
+        self._own_stream = cs == 0
+        self._s.set_stream(cs)
         return self._s
+
+    def _done(self, out=None):
+        if self._own_stream:
+            torch.cuda.synchronize(self.device)
+        return out
 
     def query(self, name: str):
         try:
@@ -106,20 +120,38 @@ class Sparc:
         if x.size != self.nsrc:
             raise ValueError(f"{x.size} source positions for a batch of {self.nsrc}")
         self._stream().set_forward_sources(torch.as_tensor(x), int(maxtime))
+        self._done()
         self.maxtime = int(maxtime)
+
+    def set_forward_profile(self, vr, zprof, series, maxtime: int) -> None:
+        """Separable forward source RHS v_z += vr[s][x] * series[t] * zprof[z]
+        (sources.gaussian_force builds one) in place of the reference's
+        one-row form."""
+        vr = np.asarray(vr, dtype=np.float64).reshape(self.nsrc, self.params.nx)
+        self._stream().set_forward_profile(torch.as_tensor(vr), _as_tensor(zprof), _as_tensor(series),
+                                           int(maxtime))
+        self._done()
+        self.maxtime = int(maxtime)
+
+    def forward_profile(self, vr, zprof, series, maxtime: int, snapshots: bool = False):
+        if self.is_adjoint:
+            raise RuntimeError("forward_profile() on an adjoint solver")
+        self.set_forward_profile(vr, zprof, series, maxtime)
+        return self.run(snapshots)
 
     def set_adjoint_sources(self, vr: torch.Tensor, maxtime: int) -> None:
         vr = torch.as_tensor(vr, dtype=torch.float64, device=self.device).contiguous()
         if vr.dim() == 2:
             vr = vr.unsqueeze(0)
         self._stream().set_adjoint_sources(vr, int(maxtime))
+        self._done()
         self.maxtime = int(maxtime)
 
     def run(self, snapshots: bool = False):
         """TIMESTEPPING for time indices 0..maxtime; returns (vz, snaps)."""
         if self.maxtime is None:
             raise RuntimeError("set sources before run()")
-        vz, sn = self._stream().run(self.maxtime, bool(snapshots))
+        vz, sn = self._done(self._stream().run(self.maxtime, bool(snapshots)))
         return vz, (sn if snapshots else None)
 
     def forward(self, xsrc, maxtime: int, snapshots: bool = False):
@@ -141,6 +173,7 @@ class Sparc:
         empty = torch.empty(0, dtype=torch.float64, device=self.device)
         self._stream().run_range(int(t0), int(t1), empty if vz is None else vz,
                                  empty if snaps is None else snaps, int(k0))
+        self._done()
 
     def checkpoint_buffer(self, n: int = 1) -> torch.Tensor:
         """Device storage for n full-state checkpoints, [n][numel]."""
@@ -148,48 +181,55 @@ class Sparc:
 
     def save_checkpoint(self, buf: torch.Tensor) -> None:
         self._stream().save_checkpoint(buf)
+        self._done()
 
     def load_checkpoint(self, buf: torch.Tensor) -> None:
         self._stream().load_checkpoint(buf)
+        self._done()
 
     def kernels_begin(self) -> None:
         self._stream().kernels_begin()
+        self._done()
 
     def kernels_add(self, fwd_snaps: torch.Tensor, adj_snaps: torch.Tensor) -> None:
         self._stream().kernels_add(fwd_snaps.contiguous(), adj_snaps.contiguous())
+        self._done()
 
     def kernels_end(self) -> dict:
-        return dict(zip(("kc", "kpsi", "kvx", "kvz", "hess"), self._stream().kernels_end()))
+        return dict(zip(("kc", "kpsi", "kvx", "kvz", "hess"), self._done(self._stream().kernels_end())))
 
     def reset(self) -> None:
         self._stream().reset()
+        self._done()
 
     def step(self, t: int) -> None:
         self._stream().step(int(t))
+        self._done()
 
     def state(self) -> torch.Tensor:
         """[4][nsrc][nz][nx]: xi_x, xi_z, v_x, v_z."""
-        return self._stream().get_state()
+        return self._done(self._stream().get_state())
 
     def set_state(self, a: torch.Tensor) -> None:
         self._stream().set_state(torch.as_tensor(a, dtype=torch.float64, device=self.device).contiguous())
+        self._done()
 
     def pml(self):
-        return self._stream().get_pml()
+        return self._done(self._stream().get_pml())
 
     # ---- operators ----
     def ddx(self, f: torch.Tensor) -> torch.Tensor:
-        return self._stream().ddx(torch.as_tensor(f, dtype=torch.float64, device=self.device).contiguous())
+        return self._done(self._stream().ddx(torch.as_tensor(f, dtype=torch.float64, device=self.device).contiguous()))
 
     def ddz(self, f: torch.Tensor, ibc: int, preset: torch.Tensor | None = None) -> torch.Tensor:
         f = torch.as_tensor(f, dtype=torch.float64, device=self.device).contiguous()
         pre = torch.empty(0, dtype=torch.float64, device=self.device) if preset is None else \
             torch.as_tensor(preset, dtype=torch.float64, device=self.device).contiguous()
-        return self._stream().ddz(f, int(ibc), pre)
+        return self._done(self._stream().ddz(f, int(ibc), pre))
 
     def ddz_kern(self, f: torch.Tensor, ibc: int) -> torch.Tensor:
         f = torch.as_tensor(f, dtype=torch.float64, device=self.device).contiguous()
-        return self._stream().ddz_kern(f, int(ibc))
+        return self._done(self._stream().ddz_kern(f, int(ibc)))
 
     def profile_kernels(self, reps: int = 200) -> dict:
         """Average ms per launch of each pass over `reps` real time steps (HIP
@@ -199,5 +239,5 @@ class Sparc:
 
     def kernels(self, fwd_snaps: torch.Tensor, adj_snaps: torch.Tensor) -> dict:
         """PRODUCE_KERNELS from in-HBM snapshots; arrays [nsrc][nz][nx]."""
-        o = self._stream().kernels(fwd_snaps.contiguous(), adj_snaps.contiguous())
+        o = self._done(self._stream().kernels(fwd_snaps.contiguous(), adj_snaps.contiguous()))
         return dict(zip(("kc", "kpsi", "kvx", "kvz", "hess"), o))

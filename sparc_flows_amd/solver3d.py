@@ -336,6 +336,21 @@ def receiver_mask_3d(p: Params3D, xsrc: float, ysrc, dmin_mm: float = 5.0, dmax_
     return ((d >= dmin_mm) & (d <= dmax_mm)).astype(np.float64)
 
 
+def choose_interval(budget: float, snap_bytes: int, ckpt_bytes: int, nt: int, steps: int, maxtime: int) -> int:
+    """Checkpoint interval (time steps) of a gradient over outputs 0..nt-1:
+    the whole run (maxtime + 1, no recompute) when all nt snapshots fit the
+    budget, else the largest n outputs per segment with
+    n * snap_bytes + (ceil(nt / n) - 1) * ckpt_bytes <= budget (one segment of
+    snapshots plus the checkpoints of the others), at least one output."""
+    if nt * snap_bytes <= budget:
+        return maxtime + 1
+    best = 1
+    for n in range(1, nt):
+        if n * snap_bytes + (-(-nt // n) - 1) * ckpt_bytes <= budget:
+            best = n
+    return best * steps
+
+
 class Gradient3D:
     """Forward + misfit + adjoint + kernels of one 3D source with fixed-interval
     checkpoint-and-recompute: the run is cut into segments of `interval` steps
@@ -371,16 +386,16 @@ class Gradient3D:
             self.adj.set_flow(*flow)
 
     def auto_interval(self, maxtime: int) -> int:
-        """The longest segment whose forward snapshots (9 fields each) fit in
-        hbm_fraction of the free device memory, in whole output cadences; the
-        whole run (no recompute) when all of them fit."""
+        """The longest segment whose forward snapshots (9 fields each), with the
+        checkpoints of the other segments, fit in hbm_fraction of the free
+        device memory (what the device reports free plus the blocks torch's
+        allocator holds unused); see choose_interval."""
+        dev = self.fwd.device
+        free, _ = torch.cuda.mem_get_info(dev)
+        free += torch.cuda.memory_reserved(dev) - torch.cuda.memory_allocated(dev)
         p = self.p
-        free, _ = torch.cuda.mem_get_info(self.fwd.device)
-        per = 9 * p.nz * p.ny * p.nx * 8
-        nsnap = max(1, int(self.hbm_fraction * free // per))
-        if nsnap >= p.nt_out(maxtime):
-            return maxtime + 1
-        return nsnap * p.steps
+        return choose_interval(self.hbm_fraction * free, 9 * p.nz * p.ny * p.nx * 8,
+                               self.fwd.checkpoint_size(), p.nt_out(maxtime), p.steps, maxtime)
 
     def run(self, xsrc: float, ysrc, maxtime: int, vz_obs: torch.Tensor | None = None) -> dict:
         from .gradient import checkpoint_segments, waveform_adjoint_source

@@ -135,6 +135,95 @@ def test_forward_adjoint_kernel_phases(tmp_path):
 
 
 @pytest.mark.gpu
+def test_flag_file_branches(tmp_path):
+    """The working-directory flag files of a forward run (READ_PARAMETERS,
+    initialize.f90:1036-1043):
+    * compute_data: the flow is v0_x, v0_z in m/s from true_vx.fits /
+      true_vz.fits (driver.f90:65-73), no partial snapshots (:412-414) and no
+      adjoint source or misfit (:274);
+    * linesearch: no snapshots; travel times in the windows a plain run wrote
+      to forward_srcSS_ls00 (:2023-2029); source.fits and information are
+      left alone (:2194, :2210); the misfit file is written;
+    * compute_synth: no snapshots and no source.fits (:2194); the windows and
+      information are written as in a plain run.
+    Traces are compared with the Python API (another process's solver:
+    rounding), files by presence and content."""
+    torch = pytest.importorskip("torch")
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from sparc_flows_amd.solver import Sparc
+    p, bg, psi, data = _setup(tmp_path)
+    maxtime = 60 * p.steps
+    s0 = Sparc(p, bg)
+    vz_obs, _ = s0.forward([-12.0], maxtime)
+    (data / "data").mkdir(exist_ok=True)
+    write_fits(data / "data" / "01.fits", vz_obs[0].cpu().numpy()[:, None, :])
+    (data / "params.8").write_text("3.0\n40.0\n10.0\n")
+    for ls in ("01", "02", "03"):
+        write_fits(data / f"model_psi_ls{ls}.fits", (0.8 * psi)[:, None, :])
+    def forward(ls):
+        (tmp_path / f"Instruction_src01_ls{ls}").write_text("spectral\n")
+        assert _cli(tmp_path, data, maxtime, "01", ls) == 0
+        return data / f"forward_src01_ls{ls}"
+
+    def partials(d):
+        return sorted(x.name for x in d.glob("*_partial.fits"))
+
+    def api_vz(**kw):
+        s = Sparc(p, bg, psi=kw.get("psi"))
+        if "flow" in kw:
+            s.set_flow(*kw["flow"])
+        v, _ = s.forward([-12.0], maxtime)
+        return v[0].cpu().numpy()
+
+    def close(a, b):
+        return np.linalg.norm(a - b) <= 1e-12 * np.linalg.norm(b)
+
+    # a plain forward run: snapshots, windows in ls00, source.fits, information
+    f0 = forward("00")
+    adir = data / "adjoint_src01"
+    assert partials(f0) and (f0 / "windows.8").exists() and (adir / "source.fits").exists()
+    src0, info0 = (adir / "source.fits").read_bytes(), (adir / "information").read_text()
+    win0 = (f0 / "windows.8").read_text()
+    # compute_data: the true flow from true_vx / true_vz (m/s)
+    sp = Sparc(p, bg, psi=psi)
+    to_ms = 1e6 / 1e2
+    vx = 1.5 * sp.background("v0x").reshape(p.nz, p.nx) * to_ms
+    vz_t = 1.5 * sp.background("v0z").reshape(p.nz, p.nx) * to_ms
+    write_fits(tmp_path / "true_vx.fits", vx[:, None, :])
+    write_fits(tmp_path / "true_vz.fits", vz_t[:, None, :])
+    (tmp_path / "compute_data").touch()
+    fd = forward("01")
+    (tmp_path / "compute_data").unlink()
+    vz_d = read_fits(fd / "vz_cc.fits")[:, 0, :]
+    assert close(vz_d, api_vz(flow=(vx, vz_t)))
+    assert not close(vz_d, api_vz(psi=0.8 * psi))            # not the model_psi flow of ls01
+    assert not partials(fd) and not list(fd.glob("ttdiff.*"))
+    assert not (data / "kernel" / "misfit_01_01").exists()
+    assert (adir / "source.fits").read_bytes() == src0
+    # linesearch: ls00's windows, no source.fits / information, misfit written
+    (tmp_path / "linesearch").touch()
+    fl = forward("02")
+    (tmp_path / "linesearch").unlink()
+    assert close(read_fits(fl / "vz_cc.fits")[:, 0, :], api_vz(psi=0.8 * psi))
+    assert not partials(fl)
+    nwin = win0.count("\n")
+    assert (fl / "ttdiff.8").read_text().count("\n") == nwin
+    assert (f0 / "windows.8").read_text() == win0
+    assert (adir / "source.fits").read_bytes() == src0 and (adir / "information").read_text() == info0
+    assert (data / "kernel" / "misfit_01_02").exists()
+    # compute_synth: no snapshots, no source.fits; windows and information written
+    (tmp_path / "compute_synth").touch()
+    fs = forward("03")
+    (tmp_path / "compute_synth").unlink()
+    assert not partials(fs) and (fs / "ttdiff.8").exists()
+    assert (adir / "source.fits").read_bytes() == src0
+    assert (f0 / "windows.8").exists() and (adir / "information").exists()
+    assert (data / "kernel" / "misfit_01_03").exists()
+    assert not any(tmp_path.glob("Instruction_*"))
+
+
+@pytest.mark.gpu
 def test_forward_adjoint_kernel_phases_3d(tmp_path):
     """A params.i with ny > 1: the three phases on the 3D solver (line source
     at master.pixels' x, a 3D model_c, a given adjoint source.fits) against

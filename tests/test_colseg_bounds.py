@@ -1,0 +1,163 @@
+"""CPU emulation of the index arithmetic of the column-pass kernel
+(sparc_flows_amd/csrc/sparc_colseg.hip: k_colsplit) for every launch shape
+the autotuner can choose: segment lengths MS of SPARC_COLSEG_SIZES with
+P = ceil(nz/MS) <= 32 and a last segment of >= 3 real rows, block widths TC in
+{1, 2, 4, 8, 16} with 3*roundup(TC*P, 64) <= 384, both segment-count extremes
+of every MS, and nz from 16 to 1024.  Every global index (loads, stores, the
+discard slot) must fall inside its allocation and every LDS index inside the
+dynamic LDS the launcher requests (colsplit_lds_bytes).
+
+The expressions below are the kernels' own, evaluated for every thread of a
+block and every unrolled row; blocks are checked at the grid's extremes
+(first and last column group and source), where the bounds are tightest."""
+import numpy as np
+import pytest
+
+SIZES = (4, 6, 8, 10, 12, 16, 19, 24, 32)
+KSEGMAX, KZT, KZL = 32, 21, 11
+L_ROWS = {"str": 5, "g": 6, "rho": 7, "grho": 8, "gp": 9, "rinv": 10}
+
+
+def seg_lengths(nz):
+    out = []
+    for v in SIZES:
+        p = -(-nz // v)
+        if p <= KSEGMAX and nz - (p - 1) * v >= 3:
+            out.append(v)
+    return out
+
+
+def nz_cases():
+    """nz 16..64 in full, then the two ends of every (MS, P) pair up to 1024."""
+    cases = set(range(16, 65))
+    for ms in SIZES:
+        for p in range(1, KSEGMAX + 1):
+            for nz in ((p - 1) * ms + 3, p * ms):
+                if 16 <= nz <= 1024:
+                    cases.add(nz)
+    return sorted(cases)
+
+
+class Check:
+    def __init__(self, what):
+        self.what = what
+
+    def idx(self, name, v, size, mask=None):
+        v = np.asarray(v)
+        if mask is not None:
+            v = v[np.broadcast_to(mask, v.shape)]
+        if v.size:
+            assert v.min() >= 0 and v.max() < size, f"{self.what}: {name} in [{v.min()}, {v.max()}], size {size}"
+
+
+def emulate(nz, ms, tc, split, nx, S, nb, nt, hw=3):
+    """All indices of one launch of k_colsplit (split 1; split 0 is the
+    lane layout of one lane per (column, segment) for all fields)."""
+    P = -(-nz // ms)
+    nzp, n2P = P * ms, 2 * P
+    nzpml = nb + nt
+    nband = n2P * (4 * hw + 4)
+    if split:
+        G = -(-tc * P // 64) * 64
+        block = 3 * G
+        lds = 3 * nband + 4 * tc * n2P + KZL * nzp + 2 * tc * nzp
+    else:
+        block = tc * P
+        lds = 3 * nband + 4 * tc * n2P + KZL * nzp
+    chk = Check(f"nz={nz} MS={ms} TC={tc} split={split} nx={nx} S={S}")
+    tid = np.arange(block)[:, None]
+    if split:
+        f = tid // G
+        l_ = tid - f * G
+    else:
+        f = np.zeros_like(tid)
+        l_ = tid
+    c, s = l_ % tc, l_ // tc
+    act = s < P
+    a = np.where(act, s, 0) * ms
+    i = np.arange(ms)[None, :]
+    gridx = nx // tc
+    n = S * nzp * 0 + nz * nx
+    for bx in (0, gridx - 1):
+        for b in (0, S - 1):
+            x = bx * tc + c
+            col = b * nz * nx + x
+            rows = np.minimum(a + i, nz - 1)
+            qj = col + rows * nx
+            valid = act & (a + i < nz)
+            chk.idx("x", x, nx)
+            chk.idx("row loads (rowq)", qj, S * n)
+            chk.idx("x-varying coefficient (gq)", qj - b * nz * nx, n)
+            k6 = np.arange(ms + 6)[None, :]
+            chk.idx("rhs window", col + np.clip(a - 3 + k6, 0, nz - 1) * nx, S * n)
+            k10 = np.arange(ms + 10)[None, :]
+            chk.idx("filter window", col + np.clip(a - 5 + k10, 0, nz - 1) * nx, S * n)
+            chk.idx("topbc", b * nx + x, S * nx)
+            # z tables: zt = ztab + a, rows zrow(slot, k) = 2 + 5 slot + k and 17..20
+            for row in [2 + 5 * sl + k for sl in range(3) for k in range(5)] + [0, 1, 17, 18, 19, 20]:
+                chk.idx("ztab", row * nzp + a + i, KZT * nzp)
+            # z-PML rows
+            j = a + i
+            jc = np.minimum(j, nz - 1)
+            pj = np.where(jc < nb, jc, np.minimum(nzpml - 1, nb + np.maximum(jc - (nz - nt), 0)))
+            chk.idx("pmlrow", pj, nzpml)
+            pz0 = b * nzpml * nx + x
+            pz1 = (S + b) * nzpml * nx + x
+            chk.idx("pz (psivz)", pz0 + pj * nx, 2 * S * nzpml * nx)
+            chk.idx("pz (psip)", pz1 + pj * nx, 2 * S * nzpml * nx)
+            chk.idx("az/bz/fz", pj * nx + x, nzpml * nx)
+            # stores: padding rows go to the discard slot
+            dump = (tid & 255) if split else tid
+            chk.idx("discard slot", dump, 256)
+            chk.idx("valid-row stores", qj, S * n, valid)
+            # snapshots and vz output at their extreme slots
+            for nsnap, nfield in ((3, 6), (5, 4)):
+                slot = (b * nsnap + (nsnap - 1)) * nfield
+                for fs in range(nfield):
+                    chk.idx("snapshot", ((slot + fs) * nz + j) * nx + x, S * nsnap * nfield * n, valid)
+            # LDS: interface inverses, end-value exchange, staged tables, wl
+            chk.idx("yb", np.where(act, 3 * tc * n2P + c * n2P + 2 * s + 1, 0) + 3 * nband, lds)
+            for k in (2 * s - 1, 2 * s + 2):
+                kk = np.clip(k, 0, n2P - 1)
+                for base in (0, nband, 2 * nband):
+                    chk.idx("banded inverse", base + kk * (4 * hw + 4) + np.arange(4 * hw + 4)[None, :], lds,
+                            act)
+            zl = 3 * nband + 4 * tc * n2P
+            for r in list(range(5)) + list(L_ROWS.values()):
+                chk.idx("staged z table", zl + r * nzp + a + i, lds)
+            if split:
+                wl = zl + KZL * nzp
+                chk.idx("wl group 1", wl + c * nzp + a + i, lds, act)
+                chk.idx("wl group 0 (w0)", wl + (tc + c) * nzp + a + i, lds, act)
+    # staging loops of the tables (every thread, 8 loads per round)
+    for q in (np.arange(0, 3 * nband), np.arange(0, KZL * nzp)):
+        pass
+    rsrc = [2 + 5 * 1 + k for k in range(5)] + [0, 1, 17, 18, 19, 20]
+    assert max(rsrc) < KZT and len(rsrc) == KZL
+    return P
+
+
+def test_every_autotuner_shape_stays_in_bounds():
+    split = 1
+    count = 0
+    for nz in nz_cases():
+        for ms in seg_lengths(nz):
+            P = -(-nz // ms)
+            for tc in (1, 2, 4, 8, 16):
+                if split and 3 * (-(-tc * P // 64) * 64) > 384:
+                    continue
+                if not split and tc * P > 256:
+                    continue
+                for nx, S in ((16, 1), (1024, 3)):
+                    if nx % tc:
+                        continue
+                    npml = min(12, (nz - 1) // 2 - 1)
+                    emulate(nz, ms, tc, split, nx, S, npml, npml)
+                    count += 1
+    assert count > 1000
+
+
+def test_segmentation_covers_every_nz_but_the_documented_gaps():
+    """nz without a usable segmentation: exactly the 18 values the library rejects."""
+    gaps = [nz for nz in range(16, 1025) if not seg_lengths(nz)]
+    assert len(gaps) == 18 and min(gaps) == 673

@@ -56,6 +56,8 @@ def _check(p, bg, c2d, psi, xsrc, maxtime, ndir, seed, checkpoint_interval=0):
     directions of that parameter, and the number of directions whose signs
     agree."""
     ds = DifferentiableSolver(p, bg, xsrc, maxtime, checkpoint_interval=checkpoint_interval)
+    print("column-pass shapes: forward", {k: ds.fwd.query(k) for k in ("colseg_ms", "colseg_tc")},
+          flush=True)
     mask = torch.as_tensor(receiver_mask(p, xsrc, 5.0, 60.0), device="cuda")
     nout = p.nt_out(maxtime)
     cad = p.steps * p.timestep
@@ -63,6 +65,9 @@ def _check(p, bg, c2d, psi, xsrc, maxtime, ndir, seed, checkpoint_interval=0):
     # reference traces: the model without flow
     with torch.no_grad():
         vref = ds.solve(torch.zeros_like(torch.as_tensor(psi)), torch.as_tensor(c2d)).clone()
+    print("after the first solve: shapes", {k: (ds.fwd.query(k), ds.adj.query(k))
+                                            for k in ("colseg_ms", "colseg_tc", "row_wide")},
+          "vref finite", bool(torch.isfinite(vref).all()), flush=True)
     tau_obs = torch.zeros(len(xsrc), p.nx, dtype=torch.float64, device="cuda")
 
     def J(ps, cc):
@@ -177,3 +182,25 @@ def test_checkpointed_backward_equals_stored_snapshots(interval):
     assert grads[0][0].abs().max() > 0
     for a, b in zip(grads[0], grads[1]):
         assert float(torch.linalg.norm(a - b) / torch.linalg.norm(a)) < 1e-12
+
+
+def test_binomial_backward_equals_stored_snapshots():
+    """Binomial checkpointing (3 slots) in the autograd backward pass gives
+    the gradient of the one from every stored snapshot to 1e-12."""
+    p, bg, c2d, psi = small_case(nx=128, nz=150)
+    p = dataclasses.replace(_cadenced(p, 10.0), obsheight=-3e8, excitdep=-5e8)
+    c2d = np.asarray(c2d, dtype=np.float64)
+    maxtime, xsrc = 300, [-20.0, 15.0]
+    grads = []
+    for kw in ({}, {"checkpoint_slots": 3}):
+        ds = DifferentiableSolver(p, bg, xsrc, maxtime, **kw)
+        ps = torch.as_tensor(psi).clone().requires_grad_(True)
+        cc = torch.as_tensor(c2d).clone().requires_grad_(True)
+        vz = ds.solve(ps, cc)
+        (vz ** 2).sum().backward()
+        grads.append((ps.grad.clone(), cc.grad.clone()))
+    assert grads[0][0].abs().max() > 0
+    for a, b in zip(grads[0], grads[1]):
+        e = float(torch.linalg.norm(a - b) / torch.linalg.norm(a))
+        print("binomial autograd", e)
+        assert e < 1e-12

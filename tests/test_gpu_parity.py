@@ -93,17 +93,12 @@ def test_background_matches_oracle():
         assert abs(s.query(name) - o.info(name)) <= 1e-15 * abs(o.info(name))
 
 
-@pytest.mark.parametrize("split", ["0", "1"])
 @pytest.mark.parametrize("flow,dc", [(False, False), (True, True)])
-def test_steps_match_oracle(flow, dc, split, monkeypatch):
-    """Full STEPs (RK + PML + filter + source) against the oracle, with both
-    forms of the column pass (one lane per column and segment, or per
-    column, segment and field)."""
-    monkeypatch.setenv("SPARC_COLSEG_SPLIT", split)
+def test_steps_match_oracle(flow, dc):
+    """Full STEPs (RK + PML + filter + source) against the oracle."""
     p, bg, c2d, psi = small_case(nx=64, nz=60, flow=flow, dc=dc)
     maxtime = 60
     s = Sparc(p, bg, c2d=c2d, psi=psi)
-    assert s.query("colseg_split") == int(split)
     o = oracle.Oracle(p, bg, c2d=c2d, psi=psi)
     s.set_forward_sources([3.0], maxtime)
     o.set_forward_source(3.0, maxtime)
@@ -162,8 +157,7 @@ def test_forward_trace_1000_steps():
     assert e < 1e-10 and es < 1e-10
 
 
-@pytest.mark.parametrize("split", ["0", "1"])
-def test_batched_sources_equal_single_runs(split, monkeypatch):
+def test_batched_sources_equal_single_runs(monkeypatch):
     p, bg, c2d, psi = small_case(nx=64, nz=60)
     xs = [-10.0, 0.0, 7.5]
     maxtime = 200
@@ -171,7 +165,6 @@ def test_batched_sources_equal_single_runs(split, monkeypatch):
     # batch is bit for bit the single runs
     monkeypatch.setenv("SPARC_COLSEG_MS", "6")
     monkeypatch.setenv("SPARC_COLSEG_TC", "2")
-    monkeypatch.setenv("SPARC_COLSEG_SPLIT", split)
     sb = Sparc(p, bg, nsrc=3, c2d=c2d, psi=psi)
     vzb, _ = sb.forward(xs, maxtime)
     for i, x in enumerate(xs):
@@ -181,7 +174,6 @@ def test_batched_sources_equal_single_runs(split, monkeypatch):
     # the autotuner may segment a batch differently: equal to rounding
     monkeypatch.delenv("SPARC_COLSEG_MS")
     monkeypatch.delenv("SPARC_COLSEG_TC")
-    monkeypatch.delenv("SPARC_COLSEG_SPLIT")
     sb = Sparc(p, bg, nsrc=3, c2d=c2d, psi=psi)
     vzb, _ = sb.forward(xs, maxtime)
     for i, x in enumerate(xs):
@@ -190,9 +182,7 @@ def test_batched_sources_equal_single_runs(split, monkeypatch):
         assert rel_l2(vzb[i].cpu().numpy(), vz1[0].cpu().numpy()) < 1e-13, i
 
 
-@pytest.mark.parametrize("split", ["0", "1"])
-def test_adjoint_and_kernels_match_oracle(split, monkeypatch):
-    monkeypatch.setenv("SPARC_COLSEG_SPLIT", split)
+def test_adjoint_and_kernels_match_oracle():
     p, bg, c2d, psi = small_case(nx=64, nz=60)
     maxtime = 4 * p.steps * 5  # multiple of the cadence
     nt = maxtime // p.cadforcing_step + 1
@@ -254,3 +244,55 @@ def test_unsupported_sizes_are_rejected():
         p = dataclasses.replace(p, nx=nx, nz=nz, timestep=B.stable_timestep(bg))
         with pytest.raises((ValueError, RuntimeError)):
             Sparc(p, bg)
+
+
+def _shapes(nz):
+    sizes = (4, 6, 8, 10, 12, 16, 19, 24, 32)
+    for ms in sizes:
+        P = -(-nz // ms)
+        if P > 32 or nz - (P - 1) * ms < 3:
+            continue
+        for tc in (1, 2, 4, 8, 16):
+            if 3 * (-(-tc * P // 64) * 64) <= 384:
+                yield ms, tc
+
+
+@pytest.mark.parametrize("nz", [150, 300])
+def test_every_column_pass_shape_matches_oracle(nz, monkeypatch):
+    """Every segment length and block width the autotuner can choose at
+    256 x nz with 8 sources in one batch (flow and sound-speed bump, 60
+    steps): every source against its own single-source run with the same
+    shape, and two sources against the oracle, so that no timing outcome can
+    select a shape whose results differ."""
+    p, bg, c2d, psi = small_case(nx=256, nz=nz, npml=12, nph=9)
+    maxtime, nsteps = 80, 60
+    xs = [-60.0, -45.0, -30.0, -10.0, 0.0, 12.0, 33.0, 55.0]
+    refs = {}
+    for i in (0, 6):
+        o = oracle.Oracle(p, bg, c2d=c2d, psi=psi)
+        o.set_forward_source(xs[i], maxtime)
+        for t in range(nsteps):
+            o.step(t)
+        refs[i] = o.state()
+    bad = []
+    for ms, tc in _shapes(p.nz):
+        monkeypatch.setenv("SPARC_COLSEG_MS", str(ms))
+        monkeypatch.setenv("SPARC_COLSEG_TC", str(tc))
+        s = Sparc(p, bg, nsrc=8, c2d=c2d, psi=psi)
+        s.set_forward_sources(xs, maxtime)
+        s.reset()
+        s.run_range(0, nsteps - 1)               # applies the forced shape (autotuner entry)
+        assert (s.query("colseg_ms"), s.query("colseg_tc")) == (ms, tc)
+        got = s.state().cpu().numpy()
+        e_or = max(rel_l2(got[:, i], refs[i]) for i in refs)
+        e_b = 0.0
+        for i in range(8):
+            s1 = Sparc(p, bg, nsrc=1, c2d=c2d, psi=psi)
+            s1.set_forward_sources([xs[i]], maxtime)
+            s1.reset()
+            s1.run_range(0, nsteps - 1)
+            e_b = max(e_b, rel_l2(got[:, i], s1.state()[:, 0].cpu().numpy()))
+        print("nz", nz, "shape MS", ms, "TC", tc, "vs oracle", e_or, "batch vs single", e_b, flush=True)
+        if not (np.isfinite(e_or) and e_or < 1e-11 and e_b < 1e-13):
+            bad.append((ms, tc, e_or, e_b))
+    assert not bad, bad
