@@ -154,6 +154,16 @@ int sparc_ddx(sparc_handle *h, const double *in_dev, double *out_dev, int nrows)
 int sparc_ddz(sparc_handle *h, const double *in_dev, double *out_dev, int nbatch, int ibc);
 /* ddzkern (kernels.f90:648-661): ddz followed by FILTER_VAR_Z. */
 int sparc_ddz_kern(sparc_handle *h, const double *in_dev, double *out_dev, int nbatch, int ibc);
+/* No counterpart in the reference.  sparc_ddz (filt 0) or sparc_ddz_kern
+ * (filt 1) without any allocation, for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops): the Thomas work
+ * buffer is a library-owned device array of 1048576 doubles (8 MiB) per GPU,
+ * kept until the process ends.  nbatch*nz*nx above that returns SPARC_E_ARG.
+ * The work is queued on the handle's stream and not waited for.  Every call
+ * on one GPU shares that array: calls that overlap on one GPU (other streams,
+ * other handles, other threads) race. */
+int sparc_ddz_noalloc(sparc_handle *h, const double *in_dev, double *out_dev, int nbatch, int ibc,
+                      int filt);
 
 /* PRODUCE_KERNELS (kernels.f90:13-612) for every source of a pair of runs:
  * fwd_snaps device [nsrc][nt][6][nz][nx] and adj_snaps [nsrc][nt][4][nz][nx]

@@ -31,6 +31,36 @@ constexpr double kPi = 3.141592653589793;    // initialize.f90:37
 
 }  // namespace
 
+// Work buffer of sparc_ddz_noalloc: the Thomas sweep's per-column scratch
+// (k_ddz_cols) in a file-scope device array, so that entry point allocates
+// nothing.  Every GPU the library runs on holds its own copy, 8 MiB, until the
+// process ends.  Host code never uses the name `scratch` as an address: the
+// per-device address comes from hipGetSymbolAddress (scratch_address).
+constexpr size_t kScratchDoubles = (size_t)1 << 20;
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+namespace {
+std::mutex g_scratch_mu;
+std::vector<double *> g_scratch_dev;   // device address of scratch, by device id
+}
+
+// This device's address of scratch, resolved once per device with that device
+// current (called from sparc_create).
+static hipError_t scratch_address(int dev, double **p)
+{
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (dev < 0) return hipErrorInvalidDevice;
+    if ((int)g_scratch_dev.size() <= dev) g_scratch_dev.resize(dev + 1, nullptr);
+    if (!g_scratch_dev[dev]) {
+        void *q = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&q, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch_dev[dev] = (double *)q;
+    }
+    *p = g_scratch_dev[dev];
+    return hipSuccess;
+}
+
 struct sparc_handle {
     // configuration
     int nx = 0, nz = 0, S = 0, nb = 0, nt = 0, nh = 0, adjoint = 0;
@@ -43,6 +73,7 @@ struct sparc_handle {
     hipStream_t stream = nullptr;
     bool own_stream = false;
     int device = 0;
+    double *scratch_dev = nullptr;   // this device's scratch (sparc_ddz_noalloc)
 
     // host background (kept for queries)
     std::vector<double> x, eyekx, z, stretch, unstretch, g, c2, rho0, p0, rhoinv, c2rho0, gp0, gr0,
@@ -919,6 +950,14 @@ int sparc_create(const sparc_config *cfg, sparc_handle **out)
         return SPARC_E_HIP;
     }
     (void)hipGetDevice(&h->device);
+    {
+        const hipError_t e = scratch_address(h->device, &h->scratch_dev);
+        if (e != hipSuccess) {
+            g_create_err = std::string("sparc_create: hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+            delete h;
+            return SPARC_E_HIP;
+        }
+    }
     if (hipStreamCreate(&h->stream) != hipSuccess) {
         g_create_err = "sparc_create: hipStreamCreate failed";
         delete h;
@@ -1574,6 +1613,21 @@ int sparc_ddz(sparc_handle *h, const double *in, double *out, int nbatch, int ib
 int sparc_ddz_kern(sparc_handle *h, const double *in, double *out, int nbatch, int ibc)
 {
     return ddz_impl(h, in, out, nbatch, ibc, 1);
+}
+
+int sparc_ddz_noalloc(sparc_handle *h, const double *in, double *out, int nbatch, int ibc, int filt)
+{
+    if (!h || !in || !out || nbatch < 1 || ibc < 0 || ibc > 3 || (filt != 0 && filt != 1))
+        return SPARC_E_ARG;
+    const size_t need = (size_t)nbatch * h->nz * h->nx;
+    if (need > kScratchDoubles)
+        return fail(h, SPARC_E_ARG,
+                    "sparc_ddz_noalloc: nbatch*nz*nx = " + std::to_string(need) +
+                        " doubles exceeds the library scratch of " + std::to_string(kScratchDoubles));
+    if (!h->scratch_dev) return fail(h, SPARC_E_STATE, "sparc_ddz_noalloc: scratch address unresolved");
+    HCHECK(h, launch_ddz_cols(in, out, h->nx, h->nz, nbatch, ibc, h->bg.stretch, h->cf, h->tri[ibc],
+                              h->scratch_dev, filt, h->stream));
+    return SPARC_OK;
 }
 
 int sparc_kernels_begin(sparc_handle *h)
