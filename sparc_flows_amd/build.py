@@ -24,6 +24,8 @@ TORCH_SO = LIB / "libsparc_torch.so"
 ARCH = os.environ.get("SPARC_OFFLOAD_ARCH", "gfx950")
 
 HIP_SOURCES = ["sparc_kernels.hip", "sparc_colseg.hip", "sparc_capi.hip", "sparc_3d.hip"]
+# plain C++ compiled for the host (also built into the CPU check of the column pass)
+HOST_SOURCES = ["sparc_segtab.cpp"]
 HIP_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared",
     # packed FP32 instructions are kept out of generated code on this platform
@@ -47,9 +49,13 @@ def build_hip() -> Path:
     objdir.mkdir(exist_ok=True)
     cflags = [f for f in HIP_FLAGS if f != "-shared"]
     objs = [objdir / (Path(s).stem + ".o") for s in HIP_SOURCES]
-    with ThreadPoolExecutor(len(HIP_SOURCES)) as ex:
-        list(ex.map(lambda so: _run(["hipcc", *cflags, "-c", str(CSRC / so[0]), "-o", str(so[1])]),
-                    zip(HIP_SOURCES, objs)))
+    hobjs = [objdir / (Path(s).stem + ".host.o") for s in HOST_SOURCES]
+    jobs = [["hipcc", *cflags, "-c", str(CSRC / s), "-o", str(o)] for s, o in zip(HIP_SOURCES, objs)]
+    jobs += [["g++", "-O2", "-std=c++17", "-fPIC", "-c", str(CSRC / s), "-o", str(o)]
+             for s, o in zip(HOST_SOURCES, hobjs)]
+    with ThreadPoolExecutor(len(jobs)) as ex:
+        list(ex.map(_run, jobs))
+    objs += hobjs
     tmp = HIP_SO.with_suffix(".so.tmp")
     _run(["hipcc", f"--offload-arch={ARCH}", "-shared", "-fPIC", *[str(o) for o in objs], "-o", str(tmp)])
     os.replace(tmp, HIP_SO)

@@ -21,6 +21,7 @@
 #include "sparc_device.h"
 #include "sparc_launch.h"
 #include "sparc_host.h"
+#include "sparc_seg.h"
 
 using namespace sparc;
 
@@ -148,22 +149,13 @@ static int fail(sparc_handle *h, int code, const std::string &msg)
 DbyCoef make_coef(int N2)
 {
     DbyCoef c{};
-    const float d2i = (float)(N2 - 1);
-    c.c1 = (double)(d2i * 7.0f / 9.0f);
-    c.c2 = (double)(d2i / 36.0f);
-    c.b1 = (double)(-(43.f / 96.f * d2i));
-    c.b2 = (double)(-(5.f / 6.f * d2i));
-    c.b3 = (double)(9.f / 8.f * d2i);
-    c.b4 = (double)(1.f / 6.f * d2i);
-    c.b5 = (double)(-(1.f / 96.f * d2i));
-    c.f1 = (double)(-(10.0f / 3.f * d2i));
-    c.f2 = (double)(-(3.0f * d2i));
-    c.f3 = (double)(6.f * d2i);
-    c.f4 = (double)(1.f / 3.f * d2i);
-    const float alpha2 = 3.f / 4.f, gamma2 = 1.f / 8.f, beta1 = 3.f;
-    c.alpha2 = alpha2;
-    c.beta1 = beta1;
-    c.cnst = (double)(1.f / (alpha2 - beta1 * gamma2));
+    double cb[kCB];
+    dbyd2_coefs(N2, c.c1, c.c2, cb);
+    c.f1 = cb[CB_F]; c.f2 = cb[CB_F + 1]; c.f3 = cb[CB_F + 2]; c.f4 = cb[CB_F + 3];
+    c.b1 = cb[CB_B]; c.b2 = cb[CB_B + 1]; c.b3 = cb[CB_B + 2]; c.b4 = cb[CB_B + 3]; c.b5 = cb[CB_B + 4];
+    c.alpha2 = cb[CB_ALPHA2];
+    c.beta1 = cb[CB_BETA1];
+    c.cnst = cb[CB_CNST];
     return c;
 }
 
@@ -192,55 +184,6 @@ void make_tri(int N2, int ibc, std::vector<double> &lowr, std::vector<double> &g
         betad[j] = 1.0 - lowr[j] * gam[j];
     }
     for (int j = 0; j < N2; j++) rbet[j] = 1.0 / betad[j];
-}
-
-// The dbyd2 matrix of one ibc: sub-diagonal lowr, unit diagonal, uppr.
-static void make_mat(int N2, int ibc, std::vector<double> &lowr, std::vector<double> &uppr)
-{
-    const float alpha = 1.f / 3.f, alpha2 = 3.f / 4.f, gamma2 = 1.f / 8.f;
-    const float alpha1 = 6.0f, beta1 = 3.f;
-    const float cnst = 1.f / (alpha2 - beta1 * gamma2);
-    const float up1 = (alpha1 * alpha2 - beta1) * cnst;
-    uppr.assign(N2, 0.0);
-    lowr.assign(N2, 0.0);
-    for (int j = 2; j <= N2 - 3; j++) uppr[j] = alpha;
-    uppr[0] = (ibc % 2 == 0) ? 0.0 : (double)up1;
-    uppr[1] = alpha2;
-    uppr[N2 - 2] = gamma2;
-    for (int j = 0; j < N2; j++) lowr[j] = uppr[N2 - 1 - j];
-    lowr[N2 - 1] = (ibc != 1 && ibc != 2) ? 0.0 : (double)up1;
-    lowr[0] = 0.0;
-    uppr[N2 - 1] = 0.0;
-}
-
-// Invert a dense n x n matrix (Gauss-Jordan, partial pivoting).
-static bool invert(std::vector<double> A, int n, std::vector<double> &inv)
-{
-    inv.assign((size_t)n * n, 0.0);
-    for (int i = 0; i < n; i++) inv[(size_t)i * n + i] = 1.0;
-    for (int c = 0; c < n; c++) {
-        int pr = c;
-        for (int r = c + 1; r < n; r++)
-            if (std::fabs(A[(size_t)r * n + c]) > std::fabs(A[(size_t)pr * n + c])) pr = r;
-        if (A[(size_t)pr * n + c] == 0.0) return false;
-        if (pr != c)
-            for (int k = 0; k < n; k++) {
-                std::swap(A[(size_t)pr * n + k], A[(size_t)c * n + k]);
-                std::swap(inv[(size_t)pr * n + k], inv[(size_t)c * n + k]);
-            }
-        const double d = 1.0 / A[(size_t)c * n + c];
-        for (int k = 0; k < n; k++) { A[(size_t)c * n + k] *= d; inv[(size_t)c * n + k] *= d; }
-        for (int r = 0; r < n; r++) {
-            if (r == c) continue;
-            const double f = A[(size_t)r * n + c];
-            if (f == 0.0) continue;
-            for (int k = 0; k < n; k++) {
-                A[(size_t)r * n + k] -= f * A[(size_t)c * n + k];
-                inv[(size_t)r * n + k] -= f * inv[(size_t)c * n + k];
-            }
-        }
-    }
-    return true;
 }
 
 // Upload a host vector into a new device buffer of the arena.
@@ -632,7 +575,11 @@ static int build_seg(sparc_handle *h, int m, SegTab &sg)
     sg.P = P;
     sg.m = m;
     sg.nzp = nzp;
-    std::vector<double> ztab((size_t)kZT * nzp, 0.0);
+    std::vector<double> ztab((size_t)kZT * nzp + kCB, 0.0);
+    {
+        double c1, c2;
+        dbyd2_coefs(N, c1, c2, ztab.data() + (size_t)kZT * nzp);
+    }
     // z profiles; rho0, d rho0/dz, d p0/dz and 1/rho0 come from the 1D model
     // and are the same in every column
     for (int j = 0; j < N; j++) {
@@ -648,73 +595,11 @@ static int build_seg(sparc_handle *h, int m, SegTab &sg)
                 h->gr0[q + i] != h->gr0z[j] * h->fdec[q + i] || h->gp0[q + i] != h->gp0z[j] * h->fdec[q + i])
                 return fail(h, SPARC_E_ARG, "background density / pressure vary along x");
     }
-    const int n2 = 2 * P;
-    std::vector<double> minv_all;
-    int hw = 0;
-    const int ibcs[3] = {1, 2, 3};
-    for (int slot = 0; slot < 3; slot++) {
-        std::vector<double> lo0, up0;
-        make_mat(N, ibcs[slot], lo0, up0);
-        std::vector<double> lowr(nzp, 0.0), uppr(nzp, 0.0);
-        for (int j = 0; j < N; j++) { lowr[j] = lo0[j]; uppr[j] = up0[j]; }
-        std::vector<double> rbl(nzp, 1.0), gl(nzp, 0.0), V(nzp, 0.0), W(nzp, 0.0), bet(nzp, 1.0);
-        for (int s = 0; s < P; s++) {
-            const int a0 = s * m;
-            for (int j = a0 + 1; j < a0 + m; j++) {
-                gl[j] = uppr[j - 1] / bet[j - 1];
-                bet[j] = 1.0 - lowr[j] * gl[j];
-                rbl[j] = 1.0 / bet[j];
-            }
-            auto lsolve = [&](std::vector<double> r, double *out) {
-                r[0] = r[0] * rbl[a0];
-                for (int i = 1; i < m; i++) r[i] = (r[i] - lowr[a0 + i] * r[i - 1]) * rbl[a0 + i];
-                for (int i = m - 2; i >= 0; i--) r[i] = r[i] - gl[a0 + i + 1] * r[i + 1];
-                for (int i = 0; i < m; i++) out[a0 + i] = r[i];
-            };
-            std::vector<double> r(m, 0.0);
-            if (s > 0) { r[0] = lowr[a0]; lsolve(r, V.data()); }
-            std::fill(r.begin(), r.end(), 0.0);
-            if (s < P - 1) { r[m - 1] = uppr[a0 + m - 1]; lsolve(r, W.data()); }
-        }
-        std::vector<double> M((size_t)n2 * n2, 0.0), Minv;
-        for (int s = 0; s < P; s++) {
-            const int f = s * m, l = s * m + m - 1;
-            M[(size_t)(2 * s) * n2 + 2 * s] = 1.0;
-            M[(size_t)(2 * s + 1) * n2 + 2 * s + 1] = 1.0;
-            if (s > 0) {
-                M[(size_t)(2 * s) * n2 + 2 * s - 1] += V[f];
-                M[(size_t)(2 * s + 1) * n2 + 2 * s - 1] += V[l];
-            }
-            if (s < P - 1) {
-                M[(size_t)(2 * s) * n2 + 2 * s + 2] += W[f];
-                M[(size_t)(2 * s + 1) * n2 + 2 * s + 2] += W[l];
-            }
-        }
-        if (!invert(M, n2, Minv)) return fail(h, SPARC_E_ARG, "singular interface system");
-        // window half-width: entries below 1e-24 of their row maximum are dropped
-        for (int k = 0; k < n2; k++) {
-            double rm = 0.0;
-            for (int q = 0; q < n2; q++) rm = std::max(rm, std::fabs(Minv[(size_t)k * n2 + q]));
-            for (int q = 0; q < n2; q++)
-                if (std::fabs(Minv[(size_t)k * n2 + q]) > 1e-24 * rm) hw = std::max(hw, std::abs(q / 2 - k / 2));
-        }
-        minv_all.insert(minv_all.end(), Minv.begin(), Minv.end());   // banded below, once hw is known
-        const std::vector<double> *tabs[5] = {&lowr, &rbl, &gl, &V, &W};
-        for (int k = 0; k < 5; k++)
-            std::copy(tabs[k]->begin(), tabs[k]->end(), ztab.begin() + (size_t)(2 + 5 * slot + k) * nzp);
-    }
-    sg.hw = hw;
-    // band storage: row k keeps the entries q = 2 (k/2 - hw) + w, w < 4 hw + 4
-    // (the window around its own segment), zero outside 0..2P-1
-    const int W = 4 * hw + 4;
-    std::vector<double> band((size_t)3 * n2 * W, 0.0);
-    for (int slot = 0; slot < 3; slot++)
-        for (int k = 0; k < n2; k++)
-            for (int w = 0; w < W; w++) {
-                const int q = 2 * (k / 2 - hw) + w;
-                if (q >= 0 && q < n2)
-                    band[((size_t)slot * n2 + k) * W + w] = minv_all[((size_t)slot * n2 + k) * n2 + q];
-            }
+    SegTables t;
+    if (!build_seg_tables(N, m, t)) return fail(h, SPARC_E_ARG, "singular interface system");
+    std::copy(t.slots.begin(), t.slots.end(), ztab.begin() + (size_t)2 * nzp);   // rows 2-16
+    sg.hw = t.hw;
+    const std::vector<double> &band = t.band;
     double *p = nullptr;
     HCHECK(h, upload(h->bg_mem, &p, ztab, h->stream));
     sg.ztab = p;
@@ -923,6 +808,11 @@ int sparc_create(const sparc_config *cfg, sparc_handle **out)
     if (nz < 16 || cfg->npmlbot < 1 || cfg->npmltop < 1 || cfg->npmlbot + cfg->npmltop >= nz ||
         cfg->npmlhor < 2 || 2 * cfg->npmlhor >= nx || cfg->nsrc < 1 || nz < 6 + 5) {
         g_create_err = "sparc_create: grid / PML sizes out of range";
+        return SPARC_E_ARG;
+    }
+    if ((double)cfg->nsrc * nz * nx >= (double)(1 << 27)) {
+        // the column pass addresses [2][S][nz][nx] arrays with 32-bit byte offsets
+        g_create_err = "sparc_create: nsrc * nz * nx must stay below 2^27 points";
         return SPARC_E_ARG;
     }
     if (!(cfg->timestep > 0) || !(cfg->xlength > 0) || cfg->outputcad < cfg->timestep ||

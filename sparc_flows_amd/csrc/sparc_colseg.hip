@@ -12,13 +12,13 @@
 // columns (k_colsplit below).  Each z-derivative is: dbyd2 right-hand side
 // from a 7-row window, local Thomas solve of the segment, exchange of the
 // segment's end values through LDS, and the exact correction
-// y - L_{s-1} V - F_{s+1} W.  Loads use clamped row indices and stores go to a
-// discard slot for padding rows, so the unrolled row loops are straight-line
-// code the scheduler can batch.
+// y - L_{s-1} V - F_{s+1} W.  Loads use clamped row indices and 32-bit byte
+// offsets from wave-uniform bases; stores of padding rows are masked off.
 #include <type_traits>
 
 #include "sparc_device.h"
 #include "sparc_launch.h"
+#include "sparc_seg.h"
 
 namespace sparc {
 
@@ -26,78 +26,6 @@ namespace {
 
 enum ZRow { Z_STRETCH = 0, Z_G = 1, Z_RHO = 17, Z_GRHO = 18, Z_GP = 19, Z_RINV = 20 };
 __device__ __forceinline__ int zrow(int slot, int k) { return 2 + 5 * slot + k; }  // k: 0 lowr 1 rbl 2 gl 3 V 4 W
-
-// dbyd2 right-hand side (dbyd2.f:91-172) of rows j = a..a+MS-1; ld(k) returns
-// B(a - 3 + k) for any k in [0, MS + 6) (values outside the column are never
-// used by the formula of a real row).  Padding rows (j >= nz) get 0.
-template <int MS, typename Loader>
-__device__ __forceinline__ void seg_rhs(Loader ld, double (&R)[MS], int a, int nz, bool first,
-                                        int ibc, double preset0, double presetN, const DbyCoef &cf)
-{
-    double w0 = ld(0), w1 = ld(1), w2 = ld(2), w3 = ld(3), w4 = ld(4), w5 = ld(5);
-#pragma unroll
-    for (int i = 0; i < MS; i++) {
-        const double w6 = ld(i + 6);  // w0..w6 = B(j-3..j+3), j = a + i
-        const int j = a + i;
-        double r = cf.c1 * (w4 - w2) + cf.c2 * (w5 - w1);
-        if (i == 0 && first) {
-            r = (ibc == 1 || ibc == 3) ? cf.f1 * w3 + cf.f2 * w4 + cf.f3 * w5 + cf.f4 * w6
-                                       : (ibc == 0 ? 0.0 : preset0);
-        }
-        if (i == 1 && first) r = cf.b1 * w2 + cf.b2 * w3 + cf.b3 * w4 + cf.b4 * w5 + cf.b5 * w6;
-        const double rm2 = -(cf.b1 * w4 + cf.b2 * w3 + cf.b3 * w2 + cf.b4 * w1 + cf.b5 * w0);
-        const double rm1 = (ibc == 1 || ibc == 2) ? -(cf.f1 * w3 + cf.f2 * w2 + cf.f3 * w1 + cf.f4 * w0)
-                                                  : (ibc == 0 ? 0.0 : presetN);
-        r = (j == nz - 2) ? rm2 : r;
-        r = (j == nz - 1) ? rm1 : r;
-        r = (j >= nz) ? 0.0 : r;
-        R[i] = r;
-        w0 = w1; w1 = w2; w2 = w3; w3 = w4; w4 = w5; w5 = w6;
-    }
-    if (first && (ibc & 1)) R[0] = (cf.alpha2 * R[0] - cf.beta1 * R[1]) * cf.cnst;  // dbyd2.f:163-167
-    if (ibc == 1 || ibc == 2) {                                                     // dbyd2.f:168-172
-#pragma unroll
-        for (int i = 1; i < MS; i++)
-            if (a + i == nz - 1) R[i] = (cf.alpha2 * R[i] - cf.beta1 * R[i - 1]) * cf.cnst;
-    }
-}
-
-// Local Thomas solve of the segment (factorization restarted at its start).
-template <int MS>
-__device__ __forceinline__ void seg_solve(double (&Y)[MS], const double *__restrict__ lowr,
-                                          const double *__restrict__ rbl, const double *__restrict__ gl)
-{
-    Y[0] = Y[0] * rbl[0];
-#pragma unroll
-    for (int i = 1; i < MS; i++) Y[i] = (Y[i] - lowr[i] * Y[i - 1]) * rbl[i];
-#pragma unroll
-    for (int i = MS - 2; i >= 0; i--) Y[i] = Y[i] - gl[i + 1] * Y[i + 1];
-}
-
-// y <- y - L_{s-1} V - F_{s+1} W; the interface values are rows of the
-// banded inverse (row k holds entries q = 2 (k/2 - hw) + w, w < 4 hw + 4)
-// against the segment end values y.
-template <int MS>
-__device__ __forceinline__ void seg_correct(double (&Y)[MS], int s, int P, int hw, const double *y,
-                                            const double *band, const double *__restrict__ V,
-                                            const double *__restrict__ W)
-{
-    const int n2P = 2 * P, nw = 4 * hw + 4;
-    auto dot = [&](int k) -> double {
-        const double *row = band + k * nw;
-        const int qb = 2 * (k / 2 - hw);
-        double acc0 = 0.0, acc1 = 0.0;
-        for (int w = 0; w < nw; w += 2) {   // nw is even; entries outside 0..2P-1 are zero
-            acc0 += row[w] * y[min(max(qb + w, 0), n2P - 1)];
-            acc1 += row[w + 1] * y[min(max(qb + w + 1, 0), n2P - 1)];
-        }
-        return acc0 + acc1;
-    };
-    const double Lp = s > 0 ? dot(2 * s - 1) : 0.0;
-    const double Fn = s < P - 1 ? dot(2 * s + 2) : 0.0;
-#pragma unroll
-    for (int i = 0; i < MS; i++) Y[i] = Y[i] - Lp * V[i] - Fn * W[i];
-}
 
 }  // namespace
 
@@ -112,11 +40,12 @@ __device__ __forceinline__ int lrow_src(int r)
 }
 
 // Dynamic LDS of one block: the three banded interface inverses
-// [3][2P][4hw+4], the four end-value exchange buffers [4][TC][2P], then the
-// staged z tables [kZL][nzp].
+// [3][2P][4hw+4], the four end-value exchange buffers [4][TC][2P + 4hw + 2]
+// (each column's 2P end values between 2hw and 2hw+2 zeros), then the staged
+// z tables [kZL][nzp].
 inline size_t colseg_lds_bytes(int tc, int P, int hw, int nzp)
 {
-    return sizeof(double) * (size_t)(3 * 2 * P * (4 * hw + 4) + 4 * tc * 2 * P + kZL * nzp);
+    return sizeof(double) * (size_t)(3 * 2 * P * (4 * hw + 4) + 4 * tc * yb_len(P, hw) + kZL * nzp);
 }
 
 // ---------------------------------------------------------------------------
@@ -139,6 +68,29 @@ inline size_t colsplit_lds_bytes(int tc, int P, int hw, int nzp)
     return colseg_lds_bytes(tc, P, hw, nzp) + sizeof(double) * (size_t)2 * tc * nzp;
 }
 
+// Loads and stores at a 32-bit byte offset from a wave-uniform base address:
+// the compiler then uses the SGPR-base + VGPR-offset addressing form, with no
+// 64-bit address arithmetic per access.  The host checks that every array
+// indexed this way spans less than 2^31 bytes.
+#define SPARC_GLOBAL __attribute__((address_space(1)))
+__device__ __forceinline__ double ldb(const double *base, unsigned ob)
+{
+    return *(const SPARC_GLOBAL double *)((const SPARC_GLOBAL char *)base + ob);
+}
+__device__ __forceinline__ void stb(double *base, unsigned ob, double v)
+{
+    *(SPARC_GLOBAL double *)((SPARC_GLOBAL char *)base + ob) = v;
+}
+// A value every lane of the wave holds, moved to a scalar register.
+template <typename T>
+__device__ __forceinline__ T *wave_ptr(T *p)
+{
+    const unsigned long long u = (unsigned long long)p;
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)u);
+    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(u >> 32));
+    return (T *)(((unsigned long long)hi << 32) | lo);
+}
+
 template <int MS, int LAST>
 __global__ __launch_bounds__(384) void k_colsplit(Dims d, Bg bg, DbyCoef cf, SegTab sg, State st,
                                                   ColArgs ca, Src src, Clock clk, Sink sink, int TC)
@@ -147,17 +99,20 @@ __global__ __launch_bounds__(384) void k_colsplit(Dims d, Bg bg, DbyCoef cf, Seg
     const int P = sg.P, n2P = 2 * P, nzp = sg.nzp;
     const int nband = n2P * (4 * sg.hw + 4);
     double *const mv = lds_dyn;
+    const int LY = n2P + 4 * sg.hw + 2, hw2 = 2 * sg.hw;
     double *const yb0 = lds_dyn + 3 * nband;
-    double *const yb[4] = {yb0, yb0 + TC * n2P, yb0 + 2 * TC * n2P, yb0 + 3 * TC * n2P};
-    double *const zl = yb0 + 4 * TC * n2P;
+    double *const yb[4] = {yb0, yb0 + TC * LY, yb0 + 2 * TC * LY, yb0 + 3 * TC * LY};
+    double *const zl = yb0 + 4 * TC * LY;
     double *const wl = zl + kZL * nzp;                 // [TC][nzp] group-1 part of RHS v_z
     const int tid = threadIdx.x, G = blockDim.x / 3;
-    const int f = tid / G, l = tid - f * G;            // f is wave-uniform
+    const int f = __builtin_amdgcn_readfirstlane(tid / G);   // the field group: one per wave
+    const int l = tid - f * G;
     const int c = l % TC, s = l / TC;
     const bool act = s < P;
     const int sc = act ? s : 0;
     const int a = sc * MS;
     const bool first = act && s == 0;
+    const bool lastseg = a + MS > d.nz - 2;            // holds rows nz-2 and nz-1
     int bx = blockIdx.x, by = blockIdx.y;
     {
         const int n = gridDim.x * gridDim.y;
@@ -170,16 +125,19 @@ __global__ __launch_bounds__(384) void k_colsplit(Dims d, Bg bg, DbyCoef cf, Seg
     }
     const int x = bx * TC + c, b = by;
     const int nx = d.nx, nz = d.nz;
-    const size_t col = (size_t)b * nz * nx + x;
+    const int nxb = nx * 8;                            // bytes per row
+    const int colb = (b * nz * nx + x) * 8;            // byte offset of (b, row 0, x) in an [S][nz][nx] array
+    const int gcolb = x * 8;                           // ... of (row 0, x) in an [nz][nx] array
+    const int lastb = (nz - 1) * nxb;
     const int time = clock_t_(clk);
     const int nb = d.npmlbot, ntop = d.npmltop, nzpml = d.nzpml;
-    const size_t pz0 = (size_t)b * nzpml * nx + x;
-    const size_t pz1 = (size_t)(d.S + b) * nzpml * nx + x;
-    const double *zt = sg.ztab + a;
+    const int pz0b = (b * nzpml * nx + x) * 8;
+    const int pz1b = ((d.S + b) * nzpml * nx + x) * 8;
+    const unsigned ztb = a * 8;                        // byte offset of row a in a z table
     const double *zs = zl + a;
-    double *const dump = st.discard + (tid & 255);
 
-    auto rowq = [&](int i) -> size_t { return col + (size_t)min(a + i, nz - 1) * nx; };
+    // byte offset of row r of this column, r clamped to the column
+    auto rowc = [&](int r) -> unsigned { return (unsigned)(min(max(r * nxb, 0), lastb)); };
     auto valid = [&](int i) -> bool { return act && a + i < nz; };
     auto pmlrow = [&](int j) -> int {
         const int jc = min(j, nz - 1);
@@ -190,11 +148,24 @@ __global__ __launch_bounds__(384) void k_colsplit(Dims d, Bg bg, DbyCoef cf, Seg
 
     // --- first derivatives: group 0 d xi_z/dz (ibc 3, top row preset),
     //     group 1 d v_z/dz (ibc 1), group 2 d v_x/dz (ibc 1)
-    const double *F = f == 0 ? ca.T[1] : (f == 1 ? ca.T[3] : ca.T[2]);
+    const double *F = wave_ptr(f == 0 ? ca.T[1] : (f == 1 ? ca.T[3] : ca.T[2]));
     double win[MS + 6];
 #pragma unroll
-    for (int k = 0; k < MS + 6; k++) win[k] = F[col + (size_t)min(max(a - 3 + k, 0), nz - 1) * nx];
-    const double topv = f == 0 ? st.topbc[(size_t)b * nx + x] : 0.0;
+    for (int k = 0; k < MS + 6; k++) win[k] = ldb(F, colb + rowc(a - 3 + k));
+    const double topv = f == 0 ? ldb(st.topbc, (b * nx + x) * 8) : 0.0;
+    double e[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    if (lastseg) {
+#pragma unroll
+        for (int k = 0; k < 5; k++) e[k] = ldb(F, colb + (nz - 5 + k) * nxb);
+    }
+    {
+        // zero the pads of the end-value buffers
+        const int npad = 4 * TC, padw = LY - n2P;
+        for (int q = tid; q < npad * padw; q += blockDim.x) {
+            const int r = q / padw, u = q - r * padw;
+            yb0[r * LY + (u < hw2 ? u : n2P + u)] = 0.0;
+        }
+    }
     {
         const int nb3 = 3 * nband, stride = blockDim.x;
         for (int q0 = tid; q0 < nb3; q0 += 8 * stride) {
@@ -211,7 +182,7 @@ __global__ __launch_bounds__(384) void k_colsplit(Dims d, Bg bg, DbyCoef cf, Seg
 #pragma unroll
             for (int u = 0; u < 8; u++) {
                 const int q = min(q0 + u * stride, nzl - 1), r = q / nzp;
-                v[u] = sg.ztab[(size_t)lrow_src(r) * nzp + (q - r * nzp)];
+                v[u] = sg.ztab[lrow_src(r) * nzp + (q - r * nzp)];
             }
 #pragma unroll
             for (int u = 0; u < 8; u++)
@@ -224,7 +195,7 @@ __global__ __launch_bounds__(384) void k_colsplit(Dims d, Bg bg, DbyCoef cf, Seg
     if (f == 0 && act && isrc >= 0 && isrc < MS && time <= src.limit) {
         if (src.mode == 1) {
             const double fs = (time >= 0 && time < src.nfwd) ? src.fwd[time] : 0.0;
-            srcv = src.vr[(size_t)b * nx + x] * fs * src.delta_width / bg.rho0_src[x];
+            srcv = src.vr[b * nx + x] * fs * src.delta_width / bg.rho0_src[x];
         } else if (src.mode == 2) {
             srcv = lagrange_forcing(src, b, x, nx, time) * src.delta_width / bg.rho0_src[x];
         }
@@ -232,11 +203,15 @@ __global__ __launch_bounds__(384) void k_colsplit(Dims d, Bg bg, DbyCoef cf, Seg
     const double pamp = (f == 0 && act) ? profile_amp(src, b, x, nx, a, MS, time) : 0.0;
     const int slot_f = f == 0 ? 2 : 0;                 // dbyd2 table set: ibc 3 or ibc 1
     double Y[MS], own[MS];
-    seg_rhs<MS>([&](int k) { return win[k]; }, Y, a, nz, first, f == 0 ? 3 : 1, 0.0, topv, cf);
+    const double *const cbt = sg.ztab + kZT * nzp;
+    seg_rhs<MS>([&](int k) { return win[k]; }, e, Y, a, nz, first, lastseg, f == 0 ? 3 : 1, 0.0, topv, cf.c1,
+                cf.c2, cbt);
 #pragma unroll
     for (int i = 0; i < MS; i++) own[i] = win[3 + i];
-    seg_solve<MS>(Y, zt + zrow(slot_f, 0) * nzp, zt + zrow(slot_f, 1) * nzp, zt + zrow(slot_f, 2) * nzp);
-    if (act) { yb[f][c * n2P + 2 * s] = Y[0]; yb[f][c * n2P + 2 * s + 1] = Y[MS - 1]; }
+    const double *const ztab_f = wave_ptr(sg.ztab + zrow(slot_f, 0) * nzp);   // this group's table set
+    auto gtab = [&](int t, int i) { return ldb(ztab_f, ztb + (t * nzp + i) * 8); };
+    seg_solve<MS>(Y, gtab);
+    if (act) { yb[f][c * LY + hw2 + 2 * s] = Y[0]; yb[f][c * LY + hw2 + 2 * s + 1] = Y[MS - 1]; }
     // the epilogue's inputs that no lane of this kernel writes, loaded
     // before the barrier so that they arrive during the exchange and the
     // correction: group 0 d xi_x/dx, c^2 rho0 and the v_z base; groups 1, 2
@@ -245,16 +220,15 @@ __global__ __launch_bounds__(384) void k_colsplit(Dims d, Bg bg, DbyCoef cf, Seg
     if (f == 0) {
 #pragma unroll
         for (int i = 0; i < MS; i++) {
-            const size_t qj = rowq(i), gq = qj - (size_t)b * nz * nx;
-            pa[i] = st.dxixdx[qj];
-            pb[i] = bg.c2rho0[gq];
-            pc[i] = st.a[3][qj];
+            const unsigned r = rowc(a + i);
+            pa[i] = ldb(st.dxixdx, colb + r);
+            pb[i] = ldb(bg.c2rho0, gcolb + r);
+            pc[i] = ldb(st.a[3], colb + r);
         }
     }
     __syncthreads();
     if (act)
-        seg_correct<MS>(Y, s, P, sg.hw, yb[f] + c * n2P, mv + (f == 0 ? 2 * nband : 0),
-                        zt + zrow(slot_f, 3) * nzp, zt + zrow(slot_f, 4) * nzp);
+        seg_correct<MS>(Y, s, P, sg.hw, yb[f] + c * LY, mv + (f == 0 ? 2 * nband : 0), gtab);
 
     double *const w0 = wl + (TC + c) * nzp + a;       // group 0: -rho'/rho0 g + source
     if (f == 0) {
@@ -271,12 +245,12 @@ __global__ __launch_bounds__(384) void k_colsplit(Dims d, Bg bg, DbyCoef cf, Seg
                 if (i >= MS) break;
                 if (PML) {
                     const int pj = pmlrow(a + i);
-                    const size_t qp = pz0 + (size_t)pj * nx, qb = (size_t)pj * nx + x;
-                    lpsi[k] = st.pz_tmp[qp];
-                    lpb[k] = st.pz_base[qp];
-                    laz[k] = bg.az[qb];
-                    lbz[k] = bg.bz[qb];
-                    lfz[k] = bg.fz[qb];
+                    const unsigned qp = pz0b + pj * nxb, qb = gcolb + pj * nxb;
+                    lpsi[k] = ldb(st.pz_tmp, qp);
+                    lpb[k] = ldb(st.pz_base, qp);
+                    laz[k] = ldb(bg.az, qb);
+                    lbz[k] = ldb(bg.bz, qb);
+                    lfz[k] = ldb(bg.fz, qb);
                 }
             }
 #pragma unroll
@@ -284,7 +258,6 @@ __global__ __launch_bounds__(384) void k_colsplit(Dims d, Bg bg, DbyCoef cf, Seg
                 const int i = c0 + k;
                 if (i >= MS) break;
                 const int j = a + i;
-                const size_t qj = rowq(i);
                 const double str = zs[L_STR * nzp + i], r0 = zs[L_RHO * nzp + i];
                 const double rinv = zs[L_RINV * nzp + i], gz = zs[L_G * nzp + i];
                 double dz = Y[i] * str;
@@ -296,16 +269,16 @@ __global__ __launch_bounds__(384) void k_colsplit(Dims d, Bg bg, DbyCoef cf, Seg
                     dz = pm ? dz + lpsi[k] : dz;
                     const double pnew = lpb[k] + ca.bnext * prhs;
                     if (pm && act) {
-                        const size_t qp = pz0 + (size_t)pmlrow(j) * nx;
-                        st.pz_tmp[qp] = pnew;
-                        if (LAST) st.pz_base[qp] = pnew;
+                        const unsigned qp = pz0b + pmlrow(j) * nxb;
+                        stb(st.pz_tmp, qp, pnew);
+                        if (LAST) stb(st.pz_base, qp, pnew);
                     }
                 }
                 const double gp0 = zs[L_GP * nzp + i] * fdec, gr0 = zs[L_GRHO * nzp + i] * fdec;
                 const double xiz = own[i];
                 const double div = pa[i] + dz;
                 const double pv = -(pb[i] * div) - gp0 * xiz;
-                *(valid(i) ? &st.p[qj] : dump) = pv;
+                if (valid(i)) stb(st.p, colb + j * nxb, pv);
                 const double rho = -(r0 * div) - gr0 * xiz;
                 double bzv = -rinv * (rho * gz);
                 if (src.mode == 3) bzv = bzv + pamp * src.zprof[min(j, nz - 1)];
@@ -318,30 +291,35 @@ __global__ __launch_bounds__(384) void k_colsplit(Dims d, Bg bg, DbyCoef cf, Seg
     } else {
         // group 1: flow part of RHS v_z (:1073-1077) to LDS, xi_z update;
         // group 2: RHS v_x flow part (:1067-1071), xi_x update
+        const double *D = wave_ptr(f == 1 ? st.dxvz : st.dxvx);
+        const double *B0 = wave_ptr(st.a[f == 1 ? 1 : 0]);
         double l1[MS], l2[MS], l3[MS], l4[MS];
 #pragma unroll
         for (int i = 0; i < MS; i++) {
-            const size_t qj = rowq(i), gq = qj - (size_t)b * nz * nx;
-            l1[i] = (f == 1 ? st.dxvz : st.dxvx)[qj];
-            l2[i] = bg.v0x[gq];
-            l3[i] = bg.v0z[gq];
-            l4[i] = st.a[f == 1 ? 1 : 0][qj];
+            const unsigned r = rowc(a + i);
+            l1[i] = ldb(D, colb + r);
+            l2[i] = ldb(bg.v0x, gcolb + r);
+            l3[i] = ldb(bg.v0z, gcolb + r);
+            l4[i] = ldb(B0, colb + r);
         }
+        double *const X1 = ca.TN[1], *const X0 = ca.TN[0], *const RV = st.rvx;
 #pragma unroll
         for (int i = 0; i < MS; i++) {
             const int j = a + i;
-            const size_t qj = rowq(i);
+            const unsigned q = colb + j * nxb;
             const bool zero = j == 0 || x == 0 || x == nx - 1;
             const double dzf = Y[i] * zs[L_STR * nzp + i];
             const bool v = valid(i);
             if (f == 1) {
                 if (act) wl[c * nzp + a + i] = -(2.0 * l2[i] * l1[i]) - 2.0 * l3[i] * dzf;
-                *(v ? &ca.TN[1][qj] : dump) = l4[i] + ca.bnext * (zero ? 0.0 : own[i]);
+                if (v) stb(X1, q, l4[i] + ca.bnext * (zero ? 0.0 : own[i]));
             } else {
                 double rvx = -(2.0 * l2[i] * l1[i]) - 2.0 * l3[i] * dzf;
                 rvx = zero ? 0.0 : rvx;
-                *(v ? &st.rvx[qj] : dump) = rvx;
-                *(v ? &ca.TN[0][qj] : dump) = l4[i] + ca.bnext * (zero ? 0.0 : own[i]);
+                if (v) {
+                    stb(RV, q, rvx);
+                    stb(X0, q, l4[i] + ca.bnext * (zero ? 0.0 : own[i]));
+                }
             }
         }
     }
@@ -352,14 +330,21 @@ __global__ __launch_bounds__(384) void k_colsplit(Dims d, Bg bg, DbyCoef cf, Seg
     if (f == 0) {
         double wp[MS + 6];
 #pragma unroll
-        for (int k = 0; k < MS + 6; k++) wp[k] = st.p[col + (size_t)min(max(a - 3 + k, 0), nz - 1) * nx];
-        seg_rhs<MS>([&](int k) { return wp[k]; }, Y, a, nz, first, 2, 0.0, 0.0, cf);
-        seg_solve<MS>(Y, zs, zs + nzp, zs + 2 * nzp);
-        if (act) { yb[3][c * n2P + 2 * s] = Y[0]; yb[3][c * n2P + 2 * s + 1] = Y[MS - 1]; }
+        for (int k = 0; k < MS + 6; k++) wp[k] = ldb(st.p, colb + rowc(a - 3 + k));
+        double ep[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+        if (lastseg) {
+#pragma unroll
+            for (int k = 0; k < 5; k++) ep[k] = ldb(st.p, colb + (nz - 5 + k) * nxb);
+        }
+        seg_rhs<MS>([&](int k) { return wp[k]; }, ep, Y, a, nz, first, lastseg, 2, 0.0, 0.0, cf.c1, cf.c2, cbt);
+        auto ltab = [&](int t, int i) { return zs[t * nzp + i]; };
+        seg_solve<MS>(Y, ltab);
+        if (act) { yb[3][c * LY + hw2 + 2 * s] = Y[0]; yb[3][c * LY + hw2 + 2 * s + 1] = Y[MS - 1]; }
     }
     __syncthreads();
     if (f == 0) {
-        if (act) seg_correct<MS>(Y, s, P, sg.hw, yb[3] + c * n2P, mv + nband, zs + 3 * nzp, zs + 4 * nzp);
+        auto ltab = [&](int t, int i) { return zs[t * nzp + i]; };
+        if (act) seg_correct<MS>(Y, s, P, sg.hw, yb[3] + c * LY, mv + nband, ltab);
         const int sslot = (LAST && sink.snap) ? out_slot(time, clk.steps, sink.k0, sink.nsnap) : -1;
         const bool snap = sslot >= 0;
         const size_t slot = snap ? ((size_t)b * sink.nsnap + sslot) * sink.nfield : 0;
@@ -375,11 +360,11 @@ __global__ __launch_bounds__(384) void k_colsplit(Dims d, Bg bg, DbyCoef cf, Seg
                 if (i >= MS) break;
                 if (PML) {
                     const int pj = pmlrow(a + i);
-                    const size_t qp = pz1 + (size_t)pj * nx, qb = (size_t)pj * nx + x;
-                    lpsi[k] = st.pz_tmp[qp];
-                    lpb[k] = st.pz_base[qp];
-                    laz[k] = bg.az[qb];
-                    lbz[k] = bg.bz[qb];
+                    const unsigned qp = pz1b + pj * nxb, qb = gcolb + pj * nxb;
+                    lpsi[k] = ldb(st.pz_tmp, qp);
+                    lpb[k] = ldb(st.pz_base, qp);
+                    laz[k] = ldb(bg.az, qb);
+                    lbz[k] = ldb(bg.bz, qb);
                 }
             }
 #pragma unroll
@@ -387,7 +372,6 @@ __global__ __launch_bounds__(384) void k_colsplit(Dims d, Bg bg, DbyCoef cf, Seg
                 const int i = c0 + k;
                 if (i >= MS) break;
                 const int j = a + i;
-                const size_t qj = rowq(i);
                 double gpz = Y[i] * zs[L_STR * nzp + i];
                 if (PML) {
                     const bool pm = is_pml(j);
@@ -395,17 +379,17 @@ __global__ __launch_bounds__(384) void k_colsplit(Dims d, Bg bg, DbyCoef cf, Seg
                     gpz = pm ? gpz + lpsi[k] : gpz;
                     const double pnew = lpb[k] + ca.bnext * prhs;
                     if (pm && act) {
-                        const size_t qp = pz1 + (size_t)pmlrow(j) * nx;
-                        st.pz_tmp[qp] = pnew;
-                        if (LAST) st.pz_base[qp] = pnew;
+                        const unsigned qp = pz1b + pmlrow(j) * nxb;
+                        stb(st.pz_tmp, qp, pnew);
+                        if (LAST) stb(st.pz_base, qp, pnew);
                     }
                 }
                 double rvz = -zs[L_RINV * nzp + i] * gpz + (w0[i] + wl[c * nzp + a + i]);
                 rvz = (j == 0 || x == 0 || x == nx - 1) ? 0.0 : rvz;
-                const bool v = valid(i);
-                *(v ? &ca.TN[3][qj] : dump) = pc[i] + ca.bnext * rvz;
-                if (snap && v)
-                    sink.snap[((slot + (sink.adjoint ? 3 : 5)) * nz + j) * nx + x] = rvz;
+                if (valid(i)) {
+                    stb(ca.TN[3], colb + j * nxb, pc[i] + ca.bnext * rvz);
+                    if (snap) sink.snap[((slot + (sink.adjoint ? 3 : 5)) * nz + j) * nx + x] = rvz;
+                }
             }
             }
         };
@@ -424,10 +408,11 @@ __global__ __launch_bounds__(384) void k_colsplit(Dims d, Bg bg, DbyCoef cf, Seg
         const bool out = oslot >= 0;
         const int nh = d.nh;
         const bool xin = x >= nh - 1 && x <= nx - nh;
-        const double *Fw = ca.TN[fi];
+        const double *Fw = wave_ptr((const double *)ca.TN[fi]);
+        double *const Aw = wave_ptr(st.a[fi]);
         double w[MS + 10];
 #pragma unroll
-        for (int q = 0; q < MS + 10; q++) w[q] = Fw[col + (size_t)min(max(a - 5 + q, 0), nz - 1) * nx];
+        for (int q = 0; q < MS + 10; q++) w[q] = ldb(Fw, colb + rowc(a - 5 + q));
         // snapshot field: forward xi_x 0, xi_z 1, v_z 3; adjoint v_z 1
         const int fs = sink.adjoint ? (fi == 3 ? 1 : -1) : fi;
 #pragma unroll
@@ -442,12 +427,12 @@ __global__ __launch_bounds__(384) void k_colsplit(Dims d, Bg bg, DbyCoef cf, Seg
             const double f2 = 0.5 * (w[q + 1] + w[q - 1]);
             double v = w[q];
             if (xin && j >= 5 && j <= nz - 2) v = (j <= nz - 6) ? f11 : f2;
-            const bool ok = valid(i);
-            const size_t qj = rowq(i);
-            *(ok ? &st.a[fi][qj] : dump) = v;
-            if (fi == 3 && ok && out && j == sink.orow)
-                sink.vz_out[((size_t)b * sink.nout + oslot) * nx + x] = v;
-            if (ok && snap && fs >= 0) sink.snap[(slot + fs) * nz * nx + (size_t)j * nx + x] = v;
+            if (valid(i)) {
+                stb(Aw, colb + j * nxb, v);
+                if (fi == 3 && out && j == sink.orow)
+                    sink.vz_out[((size_t)b * sink.nout + oslot) * nx + x] = v;
+                if (snap && fs >= 0) sink.snap[(slot + fs) * nz * nx + (size_t)j * nx + x] = v;
+            }
         }
     }
 }

@@ -57,10 +57,10 @@ struct Tri {
 // for slot 0 (ibc 1), 1 (ibc 2), 2 (ibc 3): lowr, rbl, gl, V, W (rows 2-16),
 // then the x-independent profiles rho0, d rho0/dz, d p0/dz, 1/rho0 (17-20).
 constexpr int kSegMax = 32;
-constexpr int kZT = 21;
+constexpr int kZT = 21;     // then the kCB boundary-row coefficients (sparc_seg.h)
 struct SegTab {
     int P, m, hw, nzp;              // segments, rows per segment, Minv window, P*m
-    const double *ztab;             // [kZT][nzp]
+    const double *ztab;             // [kZT][nzp], then [kCB]
     const double *Minv;             // [3][2P][4hw+4]: row k of each inverse, entries 2(k/2-hw)+w
 };
 

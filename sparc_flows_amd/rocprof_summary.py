@@ -1,7 +1,7 @@
 """Summaries of rocprofv3 CSV output (``-f csv``), small enough to keep as text.
 
-    python3 -m sparc_flows_amd.rocprof_summary stats DIR [--prune]
-    python3 -m sparc_flows_amd.rocprof_summary pmc DIR [--prune]
+    python3 -m sparc_flows_amd.rocprof_summary stats DIR [--prune] [--by-grid]
+    python3 -m sparc_flows_amd.rocprof_summary pmc DIR [--prune] [--by-grid]
 
 ``stats`` prints the per-kernel rows of ``*kernel_stats.csv`` (calls, total,
 average, min and max duration) and, when a ``*kernel_trace.csv`` is present,
@@ -10,7 +10,10 @@ the idle gaps between back-to-back dispatches.
 ``pmc`` averages every counter of ``*counter_collection.csv`` per kernel over
 its dispatches (the value is per dispatch, e.g. KiB for FETCH_SIZE and
 WRITE_SIZE).  ``--prune`` deletes the per-dispatch CSV files and database files
-afterwards, which are large for long runs.
+afterwards, which are large for long runs.  ``--by-grid`` keeps dispatches of
+one kernel with different grid sizes apart (e.g. a one-source and a
+16-source batch of the same instantiation); ``stats`` then reads the
+per-dispatch ``*kernel_trace.csv``.
 """
 from __future__ import annotations
 
@@ -25,6 +28,30 @@ import sys
 def _find(d: str, pat: str):
     f = sorted(glob.glob(os.path.join(d, "**", pat), recursive=True))
     return f[0] if f else None
+
+
+def _grid(r) -> str:
+    for k in ("Grid_Size", "Grid_Size_X"):
+        if k in r:
+            return "x".join(r[c] for c in ("Grid_Size_X", "Grid_Size_Y", "Grid_Size_Z") if c in r) \
+                if k == "Grid_Size_X" else r[k]
+    return "?"
+
+
+def stats_by_grid(d: str, out=sys.stdout, top: int = 24):
+    t = _find(d, "*kernel_trace.csv")
+    if t is None:
+        raise SystemExit(f"no kernel_trace.csv under {d}")
+    agg = collections.defaultdict(list)
+    with open(t) as fh:
+        for r in csv.DictReader(fh):
+            agg[(r["Kernel_Name"][:72], _grid(r))].append(int(r["End_Timestamp"]) - int(r["Start_Timestamp"]))
+    rows = sorted(agg.items(), key=lambda kv: -sum(kv[1]))
+    total = sum(sum(v) for _, v in rows)
+    for (k, g), v in rows[:top]:
+        print(f"{k:72s} grid {g} calls {len(v)} total_ms {sum(v) / 1e6:.3f} avg_us {sum(v) / len(v) / 1e3:.2f} "
+              f"min_us {min(v) / 1e3:.2f} max_us {max(v) / 1e3:.2f} pct {100.0 * sum(v) / total:.2f}", file=out)
+    return t
 
 
 def stats(d: str, out=sys.stdout, top: int = 16):
@@ -55,14 +82,15 @@ def stats(d: str, out=sys.stdout, top: int = 16):
     return t
 
 
-def pmc(d: str, out=sys.stdout):
+def pmc(d: str, out=sys.stdout, by_grid: bool = False):
     f = _find(d, "*counter_collection.csv")
     if f is None:
         raise SystemExit(f"no counter_collection.csv under {d}")
     agg = collections.defaultdict(list)
     with open(f) as fh:
         for r in csv.DictReader(fh):
-            agg[(r["Kernel_Name"][:72], r["Counter_Name"])].append(float(r["Counter_Value"]))
+            name = r["Kernel_Name"][:72] + (f" grid {_grid(r)}" if by_grid else "")
+            agg[(name, r["Counter_Name"])].append(float(r["Counter_Value"]))
     for (k, c), v in sorted(agg.items(), key=lambda kv: -sum(kv[1])):
         print(f"{k:72s} {c} dispatches {len(v)} mean {sum(v) / len(v):.1f} min {min(v):.1f} max {max(v):.1f}",
               file=out)
@@ -75,8 +103,12 @@ def main(argv=None):
     ap.add_argument("dir")
     ap.add_argument("--prune", action="store_true",
                     help="delete the per-dispatch CSV and database files afterwards")
+    ap.add_argument("--by-grid", action="store_true", help="one line per kernel and grid size")
     a = ap.parse_args(argv)
-    big = stats(a.dir) if a.mode == "stats" else pmc(a.dir)
+    if a.mode == "stats":
+        big = stats_by_grid(a.dir) if a.by_grid else stats(a.dir)
+    else:
+        big = pmc(a.dir, by_grid=a.by_grid)
     if a.prune:
         for f in ([big] if big else []) + glob.glob(os.path.join(a.dir, "**", "*.db"), recursive=True):
             os.remove(f)

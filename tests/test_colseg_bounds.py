@@ -3,8 +3,8 @@
 the autotuner can choose: segment lengths MS of SPARC_COLSEG_SIZES with
 P = ceil(nz/MS) <= 32 and a last segment of >= 3 real rows, block widths TC in
 {1, 2, 4, 8, 16} with 3*roundup(TC*P, 64) <= 384, both segment-count extremes
-of every MS, and nz from 16 to 1024.  Every global index (loads, stores, the
-discard slot) must fall inside its allocation and every LDS index inside the
+of every MS, and nz from 16 to 1024.  Every global index (loads and the
+stores of real rows) must fall inside its allocation and every LDS index inside the
 dynamic LDS the launcher requests (colsplit_lds_bytes).
 
 The expressions below are the kernels' own, evaluated for every thread of a
@@ -14,7 +14,7 @@ import numpy as np
 import pytest
 
 SIZES = (4, 6, 8, 10, 12, 16, 19, 24, 32)
-KSEGMAX, KZT, KZL = 32, 21, 11
+KSEGMAX, KZT, KZL, KCB = 32, 21, 11, 12
 L_ROWS = {"str": 5, "g": 6, "rho": 7, "grho": 8, "gp": 9, "rinv": 10}
 
 
@@ -28,8 +28,8 @@ def seg_lengths(nz):
 
 
 def nz_cases():
-    """nz 16..64 in full, then the two ends of every (MS, P) pair up to 1024."""
-    cases = set(range(16, 65))
+    """nz 16..40 in full, then the two ends of every (MS, P) pair up to 1024."""
+    cases = set(range(16, 41))
     for ms in SIZES:
         for p in range(1, KSEGMAX + 1):
             for nz in ((p - 1) * ms + 3, p * ms):
@@ -57,13 +57,14 @@ def emulate(nz, ms, tc, split, nx, S, nb, nt, hw=3):
     nzp, n2P = P * ms, 2 * P
     nzpml = nb + nt
     nband = n2P * (4 * hw + 4)
+    LY, hw2 = n2P + 4 * hw + 2, 2 * hw
     if split:
         G = -(-tc * P // 64) * 64
         block = 3 * G
-        lds = 3 * nband + 4 * tc * n2P + KZL * nzp + 2 * tc * nzp
+        lds = 3 * nband + 4 * tc * LY + KZL * nzp + 2 * tc * nzp
     else:
         block = tc * P
-        lds = 3 * nband + 4 * tc * n2P + KZL * nzp
+        lds = 3 * nband + 4 * tc * LY + KZL * nzp
     chk = Check(f"nz={nz} MS={ms} TC={tc} split={split} nx={nx} S={S}")
     tid = np.arange(block)[:, None]
     if split:
@@ -90,12 +91,14 @@ def emulate(nz, ms, tc, split, nx, S, nb, nt, hw=3):
             chk.idx("x-varying coefficient (gq)", qj - b * nz * nx, n)
             k6 = np.arange(ms + 6)[None, :]
             chk.idx("rhs window", col + np.clip(a - 3 + k6, 0, nz - 1) * nx, S * n)
+            chk.idx("last rows (e)", col + (nz - 5 + np.arange(5)[None, :]) * nx, S * n)
             k10 = np.arange(ms + 10)[None, :]
             chk.idx("filter window", col + np.clip(a - 5 + k10, 0, nz - 1) * nx, S * n)
             chk.idx("topbc", b * nx + x, S * nx)
             # z tables: zt = ztab + a, rows zrow(slot, k) = 2 + 5 slot + k and 17..20
             for row in [2 + 5 * sl + k for sl in range(3) for k in range(5)] + [0, 1, 17, 18, 19, 20]:
-                chk.idx("ztab", row * nzp + a + i, KZT * nzp)
+                chk.idx("ztab", row * nzp + a + i, KZT * nzp + KCB)
+            chk.idx("boundary coefficients", KZT * nzp + np.arange(KCB), KZT * nzp + KCB)
             # z-PML rows
             j = a + i
             jc = np.minimum(j, nz - 1)
@@ -106,9 +109,7 @@ def emulate(nz, ms, tc, split, nx, S, nb, nt, hw=3):
             chk.idx("pz (psivz)", pz0 + pj * nx, 2 * S * nzpml * nx)
             chk.idx("pz (psip)", pz1 + pj * nx, 2 * S * nzpml * nx)
             chk.idx("az/bz/fz", pj * nx + x, nzpml * nx)
-            # stores: padding rows go to the discard slot
-            dump = (tid & 255) if split else tid
-            chk.idx("discard slot", dump, 256)
+            # stores: only real rows store (padding rows are masked off)
             chk.idx("valid-row stores", qj, S * n, valid)
             # snapshots and vz output at their extreme slots
             for nsnap, nfield in ((3, 6), (5, 4)):
@@ -116,13 +117,30 @@ def emulate(nz, ms, tc, split, nx, S, nb, nt, hw=3):
                 for fs in range(nfield):
                     chk.idx("snapshot", ((slot + fs) * nz + j) * nx + x, S * nsnap * nfield * n, valid)
             # LDS: interface inverses, end-value exchange, staged tables, wl
-            chk.idx("yb", np.where(act, 3 * tc * n2P + c * n2P + 2 * s + 1, 0) + 3 * nband, lds)
+            for fb in range(4):
+                ybase = 3 * nband + fb * tc * LY + c * LY
+                chk.idx("yb end values", ybase + hw2 + 2 * s + np.array([[0, 1]]), 3 * nband + 4 * tc * LY, act)
+                for k, on in ((2 * s - 1, s > 0), (2 * s + 2, s < P - 1)):
+                    rd = ybase + 2 * (k // 2) + np.arange(4 * hw + 4)[None, :]
+                    chk.idx("yb window of a band row", rd, 3 * nband + 4 * tc * LY, act & on)
+                    # the window covers q = 2 (k/2 - hw) .. + 4hw+3, stored at q + 2hw
+                    q = rd - ybase - hw2
+                    inside = (q >= 0) & (q < n2P)
+                    assert np.all(np.where(np.broadcast_to(act & on, q.shape) & inside,
+                                           rd == ybase + hw2 + q, True))
+            padw = LY - n2P
+            qq = np.arange(4 * tc * padw)
+            r, u = qq // padw, qq % padw
+            pad = r * LY + np.where(u < hw2, u, n2P + u)
+            chk.idx("yb pads", 3 * nband + pad, 3 * nband + 4 * tc * LY)
+            assert len(set(pad.tolist())) == pad.size
+            assert not set(pad.tolist()) & {f * LY + hw2 + v for f in range(4 * tc) for v in range(n2P)}
             for k in (2 * s - 1, 2 * s + 2):
                 kk = np.clip(k, 0, n2P - 1)
                 for base in (0, nband, 2 * nband):
                     chk.idx("banded inverse", base + kk * (4 * hw + 4) + np.arange(4 * hw + 4)[None, :], lds,
                             act)
-            zl = 3 * nband + 4 * tc * n2P
+            zl = 3 * nband + 4 * tc * LY
             for r in list(range(5)) + list(L_ROWS.values()):
                 chk.idx("staged z table", zl + r * nzp + a + i, lds)
             if split:
@@ -152,7 +170,8 @@ def test_every_autotuner_shape_stays_in_bounds():
                     if nx % tc:
                         continue
                     npml = min(12, (nz - 1) // 2 - 1)
-                    emulate(nz, ms, tc, split, nx, S, npml, npml)
+                    hw = (0, 3, P)[count % 3]        # interface-inverse half-widths in turn
+                    emulate(nz, ms, tc, split, nx, S, npml, npml, hw)
                     count += 1
     assert count > 1000
 
