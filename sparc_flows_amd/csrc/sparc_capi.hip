@@ -565,6 +565,21 @@ static std::vector<int> seg_lengths(int N)
     return out;
 }
 
+// Segment lengths the autotuner times for nz rows: the usable ones up to
+// kTuneMaxMS, whose column-pass kernels hold their state in registers (longer
+// segments spill to scratch), or the shortest usable one when none is that
+// short.  SPARC_COLSEG_MS still forces any usable length.
+constexpr int kTuneMaxMS = 12;
+static std::vector<int> tune_lengths(int N)
+{
+    const std::vector<int> all = seg_lengths(N);
+    std::vector<int> out;
+    for (int v : all)
+        if (v <= kTuneMaxMS) out.push_back(v);
+    if (out.empty() && !all.empty()) out.push_back(all[0]);
+    return out;
+}
+
 // Segmented solve tables (SegTab) of segment length m for the column pass:
 // P segments of m rows, the last padded with identity rows.
 static int build_seg(sparc_handle *h, int m, SegTab &sg)
@@ -605,6 +620,12 @@ static int build_seg(sparc_handle *h, int m, SegTab &sg)
     sg.ztab = p;
     HCHECK(h, upload(h->bg_mem, &p, band, h->stream));
     sg.Minv = p;
+    std::vector<double> stage(band);
+    for (int r = 0; r < kZL; r++)
+        stage.insert(stage.end(), ztab.begin() + (size_t)stage_zrow(r) * nzp,
+                     ztab.begin() + (size_t)(stage_zrow(r) + 1) * nzp);
+    HCHECK(h, upload(h->bg_mem, &p, stage, h->stream));
+    sg.stage = p;
     return SPARC_OK;
 }
 
@@ -1257,7 +1278,7 @@ static int autotune(sparc_handle *h)
         if (v == 1 || v == 2 || v == 4 || v == 8 || v == 16) force_tc = v;
     }
     // candidate segment lengths (only the forced one under SPARC_COLSEG_MS)
-    std::vector<int> ms = h->seg_forced ? std::vector<int>{h->seg.m} : seg_lengths(h->nz);
+    std::vector<int> ms = h->seg_forced ? std::vector<int>{h->seg.m} : tune_lengths(h->nz);
     if (force_tc && ms.size() == 1) {
         if (fits(force_tc, h->seg)) h->seg_tc = force_tc;
         return tune_rows(h);

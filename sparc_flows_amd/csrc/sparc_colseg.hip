@@ -29,23 +29,29 @@ __device__ __forceinline__ int zrow(int slot, int k) { return 2 + 5 * slot + k; 
 
 }  // namespace
 
-// z tables staged in LDS: the ones read after the first global stores of a
-// lane (the d p/dz solve tables and the epilogue profiles), so that their
-// reads can move ahead of those stores.
-enum LRow { L_STR = 5, L_G, L_RHO, L_GRHO, L_GP, L_RINV, kZL };
-__device__ __forceinline__ int lrow_src(int r)
-{
-    return r < 5 ? zrow(1, r) : (r == L_STR ? Z_STRETCH : r == L_G ? Z_G : r == L_RHO ? Z_RHO
-                                 : r == L_GRHO ? Z_GRHO : r == L_GP ? Z_GP : Z_RINV);
-}
+// z tables staged in LDS, kZL rows of nzp (stage_zrow): the segment tables
+// of the three dbyd2 table sets (ibc 1, 2, 3; rows 5 slot + k, k: lowr, rbl,
+// gl, V, W), then the six epilogue profiles.
+enum LRow { L_STR = 15, L_G, L_RHO, L_GRHO, L_GP, L_RINV };
+static_assert(L_RINV + 1 == kZL, "staged z-table rows");
+// The block's z-PML data staged in LDS, [kPL][nzpml][TC]: az, bz, fz and the
+// two auxiliary fields (psivz, psip), stage value and base.
+enum PRow { P_AZ = 0, P_BZ, P_FZ, P_T0, P_B0, P_T1, P_B1, kPL };
 
-// Dynamic LDS of one block: the three banded interface inverses
-// [3][2P][4hw+4], the four end-value exchange buffers [4][TC][2P + 4hw + 2]
-// (each column's 2P end values between 2hw and 2hw+2 zeros), then the staged
-// z tables [kZL][nzp].
-inline size_t colseg_lds_bytes(int tc, int P, int hw, int nzp)
+// Dynamic LDS of one block, in this order: the staged region (the three
+// banded interface inverses [3][2P][4hw+4], the z tables [kZL][nzp], the
+// z-PML data [kPL][nzpml][TC]; filled in one pass at the start), the four
+// end-value exchange buffers [4][TC][2P + 4hw + 2] (each column's 2P end
+// values between 2hw and 2hw+2 zeros), and three [TC][nzp] row buffers: the
+// group-1 and group-0 parts of RHS v_z and p.
+inline size_t colsplit_staged(int tc, int P, int hw, int nzp, int nzpml)
 {
-    return sizeof(double) * (size_t)(3 * 2 * P * (4 * hw + 4) + 4 * tc * yb_len(P, hw) + kZL * nzp);
+    return (size_t)3 * 2 * P * (4 * hw + 4) + (size_t)kZL * nzp + (size_t)kPL * nzpml * tc;
+}
+inline size_t colsplit_lds_bytes(int tc, int P, int hw, int nzp, int nzpml)
+{
+    return sizeof(double) *
+           (colsplit_staged(tc, P, hw, nzp, nzpml) + (size_t)4 * tc * yb_len(P, hw) + (size_t)3 * tc * nzp);
 }
 
 // ---------------------------------------------------------------------------
@@ -60,13 +66,8 @@ inline size_t colseg_lds_bytes(int tc, int P, int hw, int nzp)
 // xi_z (TN[1]) is written only after the first barrier, when every lane's
 // window reads of T[1] are complete (T and TN are one buffer after the first
 // stage); v_z (TN[3]) after the third.
-//
-// Dynamic LDS: the banded inverses, the end-value exchange buffers and the
-// staged z tables (colseg_lds_bytes), plus the parts of RHS v_z [2][TC][nzp].
-inline size_t colsplit_lds_bytes(int tc, int P, int hw, int nzp)
-{
-    return colseg_lds_bytes(tc, P, hw, nzp) + sizeof(double) * (size_t)2 * tc * nzp;
-}
+// Every table the pass reads comes from LDS, so no lane waits on global
+// memory for them once the staging barrier has passed.
 
 // Loads and stores at a 32-bit byte offset from a wave-uniform base address:
 // the compiler then uses the SGPR-base + VGPR-offset addressing form, with no
@@ -98,14 +99,20 @@ __global__ __launch_bounds__(384) void k_colsplit(Dims d, Bg bg, DbyCoef cf, Seg
     extern __shared__ double lds_dyn[];
     const int P = sg.P, n2P = 2 * P, nzp = sg.nzp;
     const int nband = n2P * (4 * sg.hw + 4);
-    double *const mv = lds_dyn;
+    const int nzpml = d.nzpml;
+    double *const mv = lds_dyn;                        // [3][nband] banded interface inverses
+    double *const zl = mv + 3 * nband;                 // [kZL][nzp] z tables
+    double *const pl = zl + kZL * nzp;                 // [kPL][nzpml][TC] z-PML data
+    const int nstaged = 3 * nband + kZL * nzp + kPL * nzpml * TC;
     const int LY = n2P + 4 * sg.hw + 2, hw2 = 2 * sg.hw;
-    double *const yb0 = lds_dyn + 3 * nband;
-    double *const yb[4] = {yb0, yb0 + TC * LY, yb0 + 2 * TC * LY, yb0 + 3 * TC * LY};
-    double *const zl = yb0 + 4 * TC * LY;
-    double *const wl = zl + kZL * nzp;                 // [TC][nzp] group-1 part of RHS v_z
+    double *const yb0 = lds_dyn + nstaged;             // [4][TC][LY] end values: field groups 0-2, d p/dz
+    double *const wl = yb0 + 4 * TC * LY;              // [TC][nzp] group-1 part of RHS v_z,
+                                                       // then [TC][nzp] group 0's part, [TC][nzp] p
     const int tid = threadIdx.x, G = blockDim.x / 3;
     const int f = __builtin_amdgcn_readfirstlane(tid / G);   // the field group: one per wave
+    // end-value buffers by offset from the LDS base (an array of pointers
+    // indexed by f would turn these accesses into flat ones)
+    double *const ybf = yb0 + f * TC * LY, *const yb3 = yb0 + 3 * TC * LY;
     const int l = tid - f * G;
     const int c = l % TC, s = l / TC;
     const bool act = s < P;
@@ -130,11 +137,11 @@ __global__ __launch_bounds__(384) void k_colsplit(Dims d, Bg bg, DbyCoef cf, Seg
     const int gcolb = x * 8;                           // ... of (row 0, x) in an [nz][nx] array
     const int lastb = (nz - 1) * nxb;
     const int time = clock_t_(clk);
-    const int nb = d.npmlbot, ntop = d.npmltop, nzpml = d.nzpml;
+    const int nb = d.npmlbot, ntop = d.npmltop;
     const int pz0b = (b * nzpml * nx + x) * 8;
     const int pz1b = ((d.S + b) * nzpml * nx + x) * 8;
-    const unsigned ztb = a * 8;                        // byte offset of row a in a z table
-    const double *zs = zl + a;
+    const double *zs = zl + a;                         // row a of the staged z tables
+    auto plv = [&](int k, int pj) -> double { return pl[(k * nzpml + pj) * TC + c]; };
 
     // byte offset of row r of this column, r clamped to the column
     auto rowc = [&](int r) -> unsigned { return (unsigned)(min(max(r * nxb, 0), lastb)); };
@@ -167,28 +174,49 @@ __global__ __launch_bounds__(384) void k_colsplit(Dims d, Bg bg, DbyCoef cf, Seg
         }
     }
     {
-        const int nb3 = 3 * nband, stride = blockDim.x;
-        for (int q0 = tid; q0 < nb3; q0 += 8 * stride) {
-            double v[8];
+        // the staged region in one pass for the usual shapes: the inverses
+        // and z tables are one contiguous copy of sg.stage (kStage loads per
+        // thread), the z-PML data come from their own arrays (two elements per
+        // thread and kind); every load of a round is issued before its stores
+        constexpr int kStage = 32;
+        const int nA = 3 * nband + kZL * nzp, npl = nzpml * TC, stride = blockDim.x;
+        const int x0 = bx * TC;
+        const int rounds = max((nA + kStage * stride - 1) / (kStage * stride), (npl + 2 * stride - 1) / (2 * stride));
+        const double *const stg = wave_ptr(sg.stage);
+        for (int r0 = 0; r0 < rounds; r0++) {
+            double va[kStage], vb[kPL][2];
+            const int qa0 = r0 * kStage * stride + tid, qb0 = r0 * 2 * stride + tid;
 #pragma unroll
-            for (int u = 0; u < 8; u++) v[u] = sg.Minv[min(q0 + u * stride, nb3 - 1)];
+            for (int u = 0; u < kStage; u++) va[u] = ldb(stg, (unsigned)min(qa0 + u * stride, nA - 1) * 8u);
 #pragma unroll
-            for (int u = 0; u < 8; u++)
-                if (q0 + u * stride < nb3) mv[q0 + u * stride] = v[u];
-        }
-        const int nzl = kZL * nzp;
-        for (int q0 = tid; q0 < nzl; q0 += 8 * stride) {
-            double v[8];
-#pragma unroll
-            for (int u = 0; u < 8; u++) {
-                const int q = min(q0 + u * stride, nzl - 1), r = q / nzp;
-                v[u] = sg.ztab[lrow_src(r) * nzp + (q - r * nzp)];
+            for (int h = 0; h < 2; h++) {
+                const int q = min(qb0 + h * stride, npl - 1);
+                const int pj = q / TC, xc = x0 + (q - pj * TC);
+                const unsigned gq = (unsigned)(pj * nx + xc) * 8u;
+                const unsigned s0 = (unsigned)((b * nzpml + pj) * nx + xc) * 8u;
+                const unsigned s1 = (unsigned)(((d.S + b) * nzpml + pj) * nx + xc) * 8u;
+                vb[P_AZ][h] = ldb(bg.az, gq);
+                vb[P_BZ][h] = ldb(bg.bz, gq);
+                vb[P_FZ][h] = ldb(bg.fz, gq);
+                vb[P_T0][h] = ldb(st.pz_tmp, s0);
+                vb[P_B0][h] = ldb(st.pz_base, s0);
+                vb[P_T1][h] = ldb(st.pz_tmp, s1);
+                vb[P_B1][h] = ldb(st.pz_base, s1);
             }
 #pragma unroll
-            for (int u = 0; u < 8; u++)
-                if (q0 + u * stride < nzl) zl[q0 + u * stride] = v[u];
+            for (int u = 0; u < kStage; u++)
+                if (qa0 + u * stride < nA) lds_dyn[qa0 + u * stride] = va[u];
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                const int q = qb0 + h * stride;
+                if (q < npl) {
+#pragma unroll
+                    for (int k = 0; k < kPL; k++) pl[k * npl + q] = vb[k][h];
+                }
+            }
         }
     }
+    __syncthreads();
     // source value of this column at its injection row (physics2d.f90:1051-1061)
     double srcv = 0.0;
     const int isrc = src.row - a;
@@ -208,10 +236,10 @@ __global__ __launch_bounds__(384) void k_colsplit(Dims d, Bg bg, DbyCoef cf, Seg
                 cf.c2, cbt);
 #pragma unroll
     for (int i = 0; i < MS; i++) own[i] = win[3 + i];
-    const double *const ztab_f = wave_ptr(sg.ztab + zrow(slot_f, 0) * nzp);   // this group's table set
-    auto gtab = [&](int t, int i) { return ldb(ztab_f, ztb + (t * nzp + i) * 8); };
+    const double *const zt_f = zs + 5 * slot_f * nzp;   // this group's table set
+    auto gtab = [&](int t, int i) { return zt_f[t * nzp + i]; };
     seg_solve<MS>(Y, gtab);
-    if (act) { yb[f][c * LY + hw2 + 2 * s] = Y[0]; yb[f][c * LY + hw2 + 2 * s + 1] = Y[MS - 1]; }
+    if (act) { ybf[c * LY + hw2 + 2 * s] = Y[0]; ybf[c * LY + hw2 + 2 * s + 1] = Y[MS - 1]; }
     // the epilogue's inputs that no lane of this kernel writes, loaded
     // before the barrier so that they arrive during the exchange and the
     // correction: group 0 d xi_x/dx, c^2 rho0 and the v_z base; groups 1, 2
@@ -228,35 +256,25 @@ __global__ __launch_bounds__(384) void k_colsplit(Dims d, Bg bg, DbyCoef cf, Seg
     }
     __syncthreads();
     if (act)
-        seg_correct<MS>(Y, s, P, sg.hw, yb[f] + c * LY, mv + (f == 0 ? 2 * nband : 0), gtab);
+        seg_correct<MS>(Y, s, P, sg.hw, ybf + c * LY, mv + (f == 0 ? 2 * nband : 0), gtab);
 
     double *const w0 = wl + (TC + c) * nzp + a;       // group 0: -rho'/rho0 g + source
+    double *const pcol = wl + (2 * TC + c) * nzp;     // p of this column (group 0)
     if (f == 0) {
         // z-PML (:957-967), divergence, p and rho' (:990-998, :1043)
         auto epi = [&](auto pml_c) {
             constexpr bool PML = decltype(pml_c)::value;
-            constexpr int CH = PML ? (MS + 1) / 2 : MS;   // rows whose loads go out together
 #pragma unroll
-            for (int c0 = 0; c0 < MS; c0 += CH) {
-            double lpsi[CH], laz[CH], lbz[CH], lpb[CH], lfz[CH];
-#pragma unroll
-            for (int k = 0; k < CH; k++) {
-                const int i = c0 + k;
-                if (i >= MS) break;
+            for (int i = 0; i < MS; i++) {
+                double lpsi = 0.0, laz = 0.0, lbz = 0.0, lpb = 0.0, lfz = 0.0;
                 if (PML) {
                     const int pj = pmlrow(a + i);
-                    const unsigned qp = pz0b + pj * nxb, qb = gcolb + pj * nxb;
-                    lpsi[k] = ldb(st.pz_tmp, qp);
-                    lpb[k] = ldb(st.pz_base, qp);
-                    laz[k] = ldb(bg.az, qb);
-                    lbz[k] = ldb(bg.bz, qb);
-                    lfz[k] = ldb(bg.fz, qb);
+                    lpsi = plv(P_T0, pj);
+                    lpb = plv(P_B0, pj);
+                    laz = plv(P_AZ, pj);
+                    lbz = plv(P_BZ, pj);
+                    lfz = plv(P_FZ, pj);
                 }
-            }
-#pragma unroll
-            for (int k = 0; k < CH; k++) {
-                const int i = c0 + k;
-                if (i >= MS) break;
                 const int j = a + i;
                 const double str = zs[L_STR * nzp + i], r0 = zs[L_RHO * nzp + i];
                 const double rinv = zs[L_RINV * nzp + i], gz = zs[L_G * nzp + i];
@@ -264,10 +282,10 @@ __global__ __launch_bounds__(384) void k_colsplit(Dims d, Bg bg, DbyCoef cf, Seg
                 double fdec = 1.0;
                 if (PML) {
                     const bool pm = is_pml(j);
-                    fdec = pm ? lfz[k] : 1.0;
-                    const double prhs = laz[k] * dz + lbz[k] * lpsi[k];
-                    dz = pm ? dz + lpsi[k] : dz;
-                    const double pnew = lpb[k] + ca.bnext * prhs;
+                    fdec = pm ? lfz : 1.0;
+                    const double prhs = laz * dz + lbz * lpsi;
+                    dz = pm ? dz + lpsi : dz;
+                    const double pnew = lpb + ca.bnext * prhs;
                     if (pm && act) {
                         const unsigned qp = pz0b + pmlrow(j) * nxb;
                         stb(st.pz_tmp, qp, pnew);
@@ -278,12 +296,14 @@ __global__ __launch_bounds__(384) void k_colsplit(Dims d, Bg bg, DbyCoef cf, Seg
                 const double xiz = own[i];
                 const double div = pa[i] + dz;
                 const double pv = -(pb[i] * div) - gp0 * xiz;
-                if (valid(i)) stb(st.p, colb + j * nxb, pv);
+                if (valid(i)) {
+                    stb(st.p, colb + j * nxb, pv);
+                    pcol[j] = pv;
+                }
                 const double rho = -(r0 * div) - gr0 * xiz;
                 double bzv = -rinv * (rho * gz);
                 if (src.mode == 3) bzv = bzv + pamp * src.zprof[min(j, nz - 1)];
                 if (act) w0[i] = (i == isrc) ? bzv + srcv : bzv;
-            }
             }
         };
         if (segpml) epi(std::true_type{});
@@ -330,54 +350,43 @@ __global__ __launch_bounds__(384) void k_colsplit(Dims d, Bg bg, DbyCoef cf, Seg
     if (f == 0) {
         double wp[MS + 6];
 #pragma unroll
-        for (int k = 0; k < MS + 6; k++) wp[k] = ldb(st.p, colb + rowc(a - 3 + k));
+        for (int k = 0; k < MS + 6; k++) wp[k] = pcol[min(max(a - 3 + k, 0), nz - 1)];
         double ep[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
         if (lastseg) {
 #pragma unroll
-            for (int k = 0; k < 5; k++) ep[k] = ldb(st.p, colb + (nz - 5 + k) * nxb);
+            for (int k = 0; k < 5; k++) ep[k] = pcol[nz - 5 + k];
         }
         seg_rhs<MS>([&](int k) { return wp[k]; }, ep, Y, a, nz, first, lastseg, 2, 0.0, 0.0, cf.c1, cf.c2, cbt);
-        auto ltab = [&](int t, int i) { return zs[t * nzp + i]; };
+        auto ltab = [&](int t, int i) { return zs[(5 + t) * nzp + i]; };   // ibc 2 set
         seg_solve<MS>(Y, ltab);
-        if (act) { yb[3][c * LY + hw2 + 2 * s] = Y[0]; yb[3][c * LY + hw2 + 2 * s + 1] = Y[MS - 1]; }
+        if (act) { yb3[c * LY + hw2 + 2 * s] = Y[0]; yb3[c * LY + hw2 + 2 * s + 1] = Y[MS - 1]; }
     }
     __syncthreads();
     if (f == 0) {
-        auto ltab = [&](int t, int i) { return zs[t * nzp + i]; };
-        if (act) seg_correct<MS>(Y, s, P, sg.hw, yb[3] + c * LY, mv + nband, ltab);
+        auto ltab = [&](int t, int i) { return zs[(5 + t) * nzp + i]; };   // ibc 2 set
+        if (act) seg_correct<MS>(Y, s, P, sg.hw, yb3 + c * LY, mv + nband, ltab);
         const int sslot = (LAST && sink.snap) ? out_slot(time, clk.steps, sink.k0, sink.nsnap) : -1;
         const bool snap = sslot >= 0;
         const size_t slot = snap ? ((size_t)b * sink.nsnap + sslot) * sink.nfield : 0;
         auto epi = [&](auto pml_c) {
             constexpr bool PML = decltype(pml_c)::value;
-            constexpr int CH = PML ? (MS + 1) / 2 : MS;
 #pragma unroll
-            for (int c0 = 0; c0 < MS; c0 += CH) {
-            double lpsi[CH], laz[CH], lbz[CH], lpb[CH];
-#pragma unroll
-            for (int k = 0; k < CH; k++) {
-                const int i = c0 + k;
-                if (i >= MS) break;
+            for (int i = 0; i < MS; i++) {
+                double lpsi = 0.0, laz = 0.0, lbz = 0.0, lpb = 0.0;
                 if (PML) {
                     const int pj = pmlrow(a + i);
-                    const unsigned qp = pz1b + pj * nxb, qb = gcolb + pj * nxb;
-                    lpsi[k] = ldb(st.pz_tmp, qp);
-                    lpb[k] = ldb(st.pz_base, qp);
-                    laz[k] = ldb(bg.az, qb);
-                    lbz[k] = ldb(bg.bz, qb);
+                    lpsi = plv(P_T1, pj);
+                    lpb = plv(P_B1, pj);
+                    laz = plv(P_AZ, pj);
+                    lbz = plv(P_BZ, pj);
                 }
-            }
-#pragma unroll
-            for (int k = 0; k < CH; k++) {
-                const int i = c0 + k;
-                if (i >= MS) break;
                 const int j = a + i;
                 double gpz = Y[i] * zs[L_STR * nzp + i];
                 if (PML) {
                     const bool pm = is_pml(j);
-                    const double prhs = laz[k] * gpz + lbz[k] * lpsi[k];
-                    gpz = pm ? gpz + lpsi[k] : gpz;
-                    const double pnew = lpb[k] + ca.bnext * prhs;
+                    const double prhs = laz * gpz + lbz * lpsi;
+                    gpz = pm ? gpz + lpsi : gpz;
+                    const double pnew = lpb + ca.bnext * prhs;
                     if (pm && act) {
                         const unsigned qp = pz1b + pmlrow(j) * nxb;
                         stb(st.pz_tmp, qp, pnew);
@@ -390,7 +399,6 @@ __global__ __launch_bounds__(384) void k_colsplit(Dims d, Bg bg, DbyCoef cf, Seg
                     stb(ca.TN[3], colb + j * nxb, pc[i] + ca.bnext * rvz);
                     if (snap) sink.snap[((slot + (sink.adjoint ? 3 : 5)) * nz + j) * nx + x] = rvz;
                 }
-            }
             }
         };
         if (segpml) epi(std::true_type{});
@@ -476,7 +484,7 @@ hipError_t launch_colsplit(int tc, int ms, const Dims &d, const Bg &bg, const Db
         sg.P > kSegMax || 3 * G > 384)
         return hipErrorInvalidValue;
     dim3 grid(d.nx / tc, d.S), block(3 * G);
-    const size_t shm = colsplit_lds_bytes(tc, sg.P, sg.hw, sg.nzp);
+    const size_t shm = colsplit_lds_bytes(tc, sg.P, sg.hw, sg.nzp, d.nzpml);
     if (shm > (size_t)g_lds_max) return hipErrorInvalidValue;
 #define L(MSV)                                                                                          \
     if (ms == MSV) {                                                                                    \

@@ -58,10 +58,16 @@ struct Tri {
 // then the x-independent profiles rho0, d rho0/dz, d p0/dz, 1/rho0 (17-20).
 constexpr int kSegMax = 32;
 constexpr int kZT = 21;     // then the kCB boundary-row coefficients (sparc_seg.h)
+// The column pass keeps kZL z-table rows in LDS, in this order: the segment
+// tables of the three dbyd2 sets (ztab rows 2-16), then the six epilogue
+// profiles stretch, g, rho0, d rho0/dz, d p0/dz, 1/rho0 (ztab rows 0, 1, 17-20).
+constexpr int kZL = 21;
+__host__ __device__ inline int stage_zrow(int r) { return r < 15 ? 2 + r : (r == 15 ? 0 : (r == 16 ? 1 : r)); }
 struct SegTab {
     int P, m, hw, nzp;              // segments, rows per segment, Minv window, P*m
     const double *ztab;             // [kZT][nzp], then [kCB]
     const double *Minv;             // [3][2P][4hw+4]: row k of each inverse, entries 2(k/2-hw)+w
+    const double *stage;            // the column pass's LDS image: Minv, then [kZL][nzp] ztab rows by stage_zrow
 };
 
 // Prognostic state and work arrays of a batch (device pointers).

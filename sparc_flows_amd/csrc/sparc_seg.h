@@ -85,18 +85,26 @@ template <int MS, typename Tab>
 SPARC_HD void seg_correct(double (&Y)[MS], int s, int P, int hw, const double *yp, const double *band, Tab tab)
 {
     const int nw = 4 * hw + 4;
-    auto dot = [&](int k) -> double {
-        const double *row = band + k * nw;
-        const double *y = yp + 2 * (k / 2);       // y[2 (k/2 - hw) + w] = yp[2 (k/2) + w]
-        double acc0 = 0.0, acc1 = 0.0;
-        for (int w = 0; w < nw; w += 2) {
-            acc0 += row[w] * y[w];
-            acc1 += row[w + 1] * y[w + 1];
-        }
-        return acc0 + acc1;
-    };
-    const double Lp = s > 0 ? dot(2 * s - 1) : 0.0;
-    const double Fn = s < P - 1 ? dot(2 * s + 2) : 0.0;
+    // both rows in one loop, four entries of each per iteration, so that
+    // 16 reads are issued before the first product needs one; rows outside
+    // 0..2P-1 are clamped (their windows stay inside the buffer) and weighted 0
+    const int k1 = s > 0 ? 2 * s - 1 : 0, k2 = s < P - 1 ? 2 * s + 2 : 2 * P - 1;
+    const double *r1 = band + k1 * nw, *y1 = yp + 2 * (k1 / 2);   // y[2 (k/2 - hw) + w] = yp[2 (k/2) + w]
+    const double *r2 = band + k2 * nw, *y2 = yp + 2 * (k2 / 2);
+    double a0 = 0.0, a1 = 0.0, b0 = 0.0, b1 = 0.0;
+#pragma unroll 2
+    for (int w = 0; w < nw; w += 4) {
+        const double p0 = r1[w], p1 = r1[w + 1], p2 = r1[w + 2], p3 = r1[w + 3];
+        const double u0 = y1[w], u1 = y1[w + 1], u2 = y1[w + 2], u3 = y1[w + 3];
+        const double q0 = r2[w], q1 = r2[w + 1], q2 = r2[w + 2], q3 = r2[w + 3];
+        const double v0 = y2[w], v1 = y2[w + 1], v2 = y2[w + 2], v3 = y2[w + 3];
+        a0 += p0 * u0 + p2 * u2;
+        a1 += p1 * u1 + p3 * u3;
+        b0 += q0 * v0 + q2 * v2;
+        b1 += q1 * v1 + q3 * v3;
+    }
+    const double Lp = s > 0 ? a0 + a1 : 0.0;
+    const double Fn = s < P - 1 ? b0 + b1 : 0.0;
 #pragma unroll
     for (int i = 0; i < MS; i++) Y[i] = Y[i] - Lp * tab(3, i) - Fn * tab(4, i);
 }

@@ -14,8 +14,17 @@ import numpy as np
 import pytest
 
 SIZES = (4, 6, 8, 10, 12, 16, 19, 24, 32)
-KSEGMAX, KZT, KZL, KCB = 32, 21, 11, 12
-L_ROWS = {"str": 5, "g": 6, "rho": 7, "grho": 8, "gp": 9, "rinv": 10}
+KSEGMAX, KZT, KZL, KCB, KPL = 32, 21, 21, 12, 7
+L_ROWS = {"str": 15, "g": 16, "rho": 17, "grho": 18, "gp": 19, "rinv": 20}
+Z_PROFILE_ROWS = [0, 1, 17, 18, 19, 20]   # ztab rows of the six epilogue profiles
+
+
+def lrow_src(r):
+    """ztab row of staged z-table row r (the kernel's lrow_src)."""
+    return 2 + 5 * (r // 5) + r % 5 if r < 15 else Z_PROFILE_ROWS[r - 15]
+
+
+LROW_SRC = np.array([lrow_src(r) for r in range(KZL)])
 
 
 def seg_lengths(nz):
@@ -58,21 +67,17 @@ def emulate(nz, ms, tc, split, nx, S, nb, nt, hw=3):
     nzpml = nb + nt
     nband = n2P * (4 * hw + 4)
     LY, hw2 = n2P + 4 * hw + 2, 2 * hw
-    if split:
-        G = -(-tc * P // 64) * 64
-        block = 3 * G
-        lds = 3 * nband + 4 * tc * LY + KZL * nzp + 2 * tc * nzp
-    else:
-        block = tc * P
-        lds = 3 * nband + 4 * tc * LY + KZL * nzp
+    assert split, "only the field-split kernel exists"
+    G = -(-tc * P // 64) * 64
+    block = 3 * G
+    nstaged = 3 * nband + KZL * nzp + KPL * nzpml * tc
+    lds = nstaged + 4 * tc * LY + 3 * tc * nzp
+    zl, pl, yb0 = 3 * nband, 3 * nband + KZL * nzp, nstaged
+    wl = yb0 + 4 * tc * LY
     chk = Check(f"nz={nz} MS={ms} TC={tc} split={split} nx={nx} S={S}")
     tid = np.arange(block)[:, None]
-    if split:
-        f = tid // G
-        l_ = tid - f * G
-    else:
-        f = np.zeros_like(tid)
-        l_ = tid
+    f = tid // G
+    l_ = tid - f * G
     c, s = l_ % tc, l_ // tc
     act = s < P
     a = np.where(act, s, 0) * ms
@@ -109,6 +114,8 @@ def emulate(nz, ms, tc, split, nx, S, nb, nt, hw=3):
             chk.idx("pz (psivz)", pz0 + pj * nx, 2 * S * nzpml * nx)
             chk.idx("pz (psip)", pz1 + pj * nx, 2 * S * nzpml * nx)
             chk.idx("az/bz/fz", pj * nx + x, nzpml * nx)
+            for k in range(KPL):
+                chk.idx("staged z-PML data", pl + (k * nzpml + pj) * tc + c, lds)
             # stores: only real rows store (padding rows are masked off)
             chk.idx("valid-row stores", qj, S * n, valid)
             # snapshots and vz output at their extreme slots
@@ -117,41 +124,56 @@ def emulate(nz, ms, tc, split, nx, S, nb, nt, hw=3):
                 for fs in range(nfield):
                     chk.idx("snapshot", ((slot + fs) * nz + j) * nx + x, S * nsnap * nfield * n, valid)
             # LDS: interface inverses, end-value exchange, staged tables, wl
+            yend = yb0 + 4 * tc * LY
             for fb in range(4):
-                ybase = 3 * nband + fb * tc * LY + c * LY
-                chk.idx("yb end values", ybase + hw2 + 2 * s + np.array([[0, 1]]), 3 * nband + 4 * tc * LY, act)
-                for k, on in ((2 * s - 1, s > 0), (2 * s + 2, s < P - 1)):
+                ybase = yb0 + fb * tc * LY + c * LY
+                chk.idx("yb end values", ybase + hw2 + 2 * s + np.array([[0, 1]]), yend, act)
+                # seg_correct reads rows k1, k2 (clamped to 0 / 2P-1 at the ends) for every active lane
+                k1 = np.where(s > 0, 2 * s - 1, 0)
+                k2 = np.where(s < P - 1, 2 * s + 2, n2P - 1)
+                for k, on in ((k1, s > 0), (k2, s < P - 1)):
                     rd = ybase + 2 * (k // 2) + np.arange(4 * hw + 4)[None, :]
-                    chk.idx("yb window of a band row", rd, 3 * nband + 4 * tc * LY, act & on)
+                    chk.idx("yb window of a band row", rd, yend, act)
+                    chk.idx("yb window of a band row (lower end)", rd - yb0, 4 * tc * LY, act)
                     # the window covers q = 2 (k/2 - hw) .. + 4hw+3, stored at q + 2hw
                     q = rd - ybase - hw2
                     inside = (q >= 0) & (q < n2P)
                     assert np.all(np.where(np.broadcast_to(act & on, q.shape) & inside,
                                            rd == ybase + hw2 + q, True))
+            for k in (k1, k2):
+                for base in (0, nband, 2 * nband):
+                    chk.idx("banded inverse", base + k * (4 * hw + 4) + np.arange(4 * hw + 4)[None, :],
+                            base + nband, act)
             padw = LY - n2P
             qq = np.arange(4 * tc * padw)
             r, u = qq // padw, qq % padw
             pad = r * LY + np.where(u < hw2, u, n2P + u)
-            chk.idx("yb pads", 3 * nband + pad, 3 * nband + 4 * tc * LY)
+            chk.idx("yb pads", yb0 + pad, yend)
             assert len(set(pad.tolist())) == pad.size
             assert not set(pad.tolist()) & {f * LY + hw2 + v for f in range(4 * tc) for v in range(n2P)}
-            for k in (2 * s - 1, 2 * s + 2):
-                kk = np.clip(k, 0, n2P - 1)
-                for base in (0, nband, 2 * nband):
-                    chk.idx("banded inverse", base + kk * (4 * hw + 4) + np.arange(4 * hw + 4)[None, :], lds,
-                            act)
-            zl = 3 * nband + 4 * tc * LY
-            for r in list(range(5)) + list(L_ROWS.values()):
-                chk.idx("staged z table", zl + r * nzp + a + i, lds)
-            if split:
-                wl = zl + KZL * nzp
-                chk.idx("wl group 1", wl + c * nzp + a + i, lds, act)
-                chk.idx("wl group 0 (w0)", wl + (tc + c) * nzp + a + i, lds, act)
-    # staging loops of the tables (every thread, 8 loads per round)
-    for q in (np.arange(0, 3 * nband), np.arange(0, KZL * nzp)):
-        pass
-    rsrc = [2 + 5 * 1 + k for k in range(5)] + [0, 1, 17, 18, 19, 20]
-    assert max(rsrc) < KZT and len(rsrc) == KZL
+            for r in range(KZL):
+                chk.idx("staged z table", zl + r * nzp + a + i, pl)
+            chk.idx("wl group 1", wl + c * nzp + a + i, lds, act)
+            chk.idx("wl group 0 (w0)", wl + (tc + c) * nzp + a + i, lds, act)
+            chk.idx("p column (store)", wl + (2 * tc + c) * nzp + j, lds, valid)
+            k6 = np.arange(ms + 6)[None, :]
+            chk.idx("p column (window)", wl + (2 * tc + c) * nzp + np.clip(a - 3 + k6, 0, nz - 1), lds, act)
+            chk.idx("p column (last rows)", wl + (2 * tc + c) * nzp + nz - 5 + np.arange(5)[None, :], lds, act)
+    # staging, for the last column group and source (every index grows with
+    # both): the inverses and z tables are element q < nA of the contiguous
+    # LDS image sg.stage, the z-PML data element q < nzpml*TC of kind k
+    bx, b = gridx - 1, S - 1
+    nA, npl = 3 * nband + KZL * nzp, nzpml * tc
+    assert nA + KPL * npl == nstaged
+    q = np.arange(nA)
+    chk.idx("staging: image", q, nA)
+    q = np.arange(npl)
+    pj2, xc = q // tc, bx * tc + q % tc
+    chk.idx("staging: az/bz/fz", pj2 * nx + xc, nzpml * nx)
+    chk.idx("staging: pz slot 0", (b * nzpml + pj2) * nx + xc, 2 * S * nzpml * nx)
+    chk.idx("staging: pz slot 1", ((S + b) * nzpml + pj2) * nx + xc, 2 * S * nzpml * nx)
+    chk.idx("staging: LDS", nA + np.arange(KPL)[:, None] * npl + q[None, :], nstaged)
+    assert max(lrow_src(r) for r in range(KZL)) < KZT
     return P
 
 

@@ -247,11 +247,13 @@ def test_unsupported_sizes_are_rejected():
 
 
 def _shapes(nz):
+    """The autotuner's candidates (sparc_capi.hip: tune_lengths): usable
+    segment lengths up to 12, or the shortest usable one."""
     sizes = (4, 6, 8, 10, 12, 16, 19, 24, 32)
-    for ms in sizes:
+    usable = [ms for ms in sizes if -(-nz // ms) <= 32 and nz - (-(-nz // ms) - 1) * ms >= 3]
+    tuned = [ms for ms in usable if ms <= 12] or usable[:1]
+    for ms in tuned:
         P = -(-nz // ms)
-        if P > 32 or nz - (P - 1) * ms < 3:
-            continue
         for tc in (1, 2, 4, 8, 16):
             if 3 * (-(-tc * P // 64) * 64) <= 384:
                 yield ms, tc
