@@ -184,6 +184,30 @@ int sparc_kernels_add(sparc_handle *h, int n, const double *fwd_snaps_dev, const
 int sparc_kernels_end(sparc_handle *h, double *kc_dev, double *kpsi_dev, double *kvx_dev,
                       double *kvz_dev, double *hess_dev);
 
+/* Exact discrete adjoint.  No counterpart in the reference: its adjoint
+ * (driver.f90:144-147, kernels.f90:286-355) is the continuous recipe, a
+ * time-reversed run with -v0 correlated with the forward snapshots; this is
+ * the transpose of the discrete step itself (step.f90:43-119 with
+ * MP_QUIET_PML_2D_DISPL, physics2d.f90:943-1094, and FILTER_VARS_2D,
+ * :509-556), for gradients that finite differences of the solver reproduce.
+ * For a forward handle whose sources are set for maxtime: the gradient of
+ * J = sum_{s,k,x} gvz[s][k][x] vz[s][k][x] (vz the traces sparc_run writes,
+ * nt = maxtime/steps + 1 outputs; gvz device [S][nt][nx], e.g. dL/dvz of any
+ * misfit L) over the run from the zero state.  The forward trajectory is
+ * recomputed from a checkpoint every `interval` steps (0: one segment, every
+ * state of the run kept).  Outputs (device [S][nz][nx] per source, each may
+ * be NULL): gpsi = dJ/dpsi (model_psi units; needs the flow from psi),
+ * gc = dJ/dc (cm/s); gcoef [3][S][nz][nx] = dJ/d(c^2 rho0), dJ/dv0x,
+ * dJ/dv0z in code units.  The c-dependence of the PML coefficients (rows 1
+ * and nz-5, column 1) is not part of gc. */
+int sparc_discrete_gradient(sparc_handle *h, int maxtime, const double *gvz_dev, int nt, int interval,
+                            double *gpsi_dev, double *gc_dev, double *gcoef_dev);
+/* The transpose of sparc_ddz with ibc 1..3 (kept rows at 0), for nbatch
+ * [nz][nx] fields; for ibc 3, presetN_bar (device [nbatch][nx], may be NULL)
+ * gets the adjoint of the kept last row. */
+int sparc_ddz_T(sparc_handle *h, const double *in_dev, double *out_dev, int nbatch, int ibc,
+                double *presetN_bar_dev);
+
 /* Instrumentation (no reference counterpart; the reference prints per-step
  * MPI_WTIME, driver.f90:355-381): average device time in ms of one launch of
  * each pass, timed with HIP events recorded around every launch of reps real

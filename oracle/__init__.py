@@ -72,6 +72,11 @@ def _bind(path) -> ctypes.CDLL:
     L.orc_fft.argtypes = [_dp, _dp, ctypes.c_int, ctypes.c_int]
     L.orc_kernels.argtypes = [ctypes.POINTER(OrcParams), _dp, _dp, ctypes.c_int, _dp, _dp,
                               _dp, _dp, _dp, _dp, _dp]
+    L.orc_ddz_T.argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_int, _dp]
+    L.orc_ddx_T.argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_int]
+    L.orc_rhs_T.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
+    L.orc_filter_vars_2d_T.argtypes = [ctypes.c_void_p, _dp]
+    L.orc_discrete_gradient.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp, _dp, _dp, _dp]
     return L
 
 
@@ -251,6 +256,65 @@ class Oracle:
         out = np.zeros(self.nx)
         self._L.orc_lagrange(self.h, int(t), _ptr(out))
         return out
+
+    # ---- exact discrete adjoint (transposes of the operators above) ----
+    def ddz_T(self, f: np.ndarray, ibc: int):
+        """Dz_ibc^T f; for ibc 3 also the adjoint of the kept last row [nx]."""
+        f = np.ascontiguousarray(f, dtype=np.float64)
+        out = np.zeros_like(f)
+        pb = np.zeros(self.nx)
+        self._L.orc_ddz_T(self.h, _ptr(f), _ptr(out), int(ibc), _ptr(pb))
+        return (out, pb) if ibc == 3 else out
+
+    def ddx_T(self, f: np.ndarray) -> np.ndarray:
+        f = np.ascontiguousarray(f, dtype=np.float64)
+        out = np.zeros_like(f)
+        self._L.orc_ddx_T(self.h, _ptr(f), _ptr(out), f.size // self.nx)
+        return out
+
+    def rhs_T(self, Rb4, Rbpz, Rbpx):
+        """Transpose of rhs() without its source: (Yb4, Ybpz, Ybpx, pbar)."""
+        Rb4, Rbpz, Rbpx = (np.ascontiguousarray(a, dtype=np.float64) for a in (Rb4, Rbpz, Rbpx))
+        Yb4, Ybpz, Ybpx = np.zeros_like(Rb4), np.zeros_like(Rbpz), np.zeros_like(Rbpx)
+        pb = np.zeros((self.nz, self.nx))
+        self._L.orc_rhs_T(self.h, _ptr(Rb4), _ptr(Rbpz), _ptr(Rbpx), _ptr(Yb4), _ptr(Ybpz), _ptr(Ybpx),
+                          _ptr(pb))
+        return Yb4, Ybpz, Ybpx, pb
+
+    def filter_vars_2d_T(self, a4: np.ndarray) -> np.ndarray:
+        a4 = np.ascontiguousarray(a4, dtype=np.float64).copy()
+        self._L.orc_filter_vars_2d_T(self.h, _ptr(a4))
+        return a4
+
+    def discrete_gradient(self, maxtime: int, gvz: np.ndarray):
+        """dJ/d(c^2 rho0), dJ/dv0x, dJ/dv0z [nz][nx] (code units) and the
+        adjoint of the initial state, for J = sum_k <gvz[k], vz_out[k]>."""
+        gvz = np.ascontiguousarray(gvz, dtype=np.float64)
+        assert gvz.shape == (maxtime // self.p.steps + 1, self.nx)
+        out = [np.zeros((self.nz, self.nx)) for _ in range(3)]
+        lam0 = np.zeros((4, self.nz, self.nx))
+        r = self._L.orc_discrete_gradient(self.h, int(maxtime), _ptr(gvz), *[_ptr(a) for a in out], _ptr(lam0))
+        assert r == 0
+        return out[0], out[1], out[2], lam0
+
+    def model_gradient(self, maxtime: int, gvz: np.ndarray, c: np.ndarray, psi: np.ndarray | None):
+        """dJ/dpsi and dJ/dc [nz][nx] for psi in the model_psi units and c in
+        cm/s, through c^2 rho0 = (c/dimc)^2 rho0 and the flow map
+        rho0 v0 = curl(rho0 c psi / (Rsun/1e8) y) (driver.f90:94-126).  The
+        PML coefficients' dependence on c (rows 1 and nz-5, column 1) is not
+        included: directions that vanish in the layers see none of it."""
+        g2, gx, gz, _ = self.discrete_gradient(maxtime, gvz)
+        dimc, diml = self.info("dimc"), self.info("diml")
+        rho0 = self.array("rho0").reshape(self.nz, self.nx)
+        cc = np.asarray(c, dtype=np.float64) / dimc
+        # pv = rho0 c psi / (diml/1e8); v0x = -Dz1(pv)/rho0, v0z = Dx(pv)/rho0
+        gpv = -self.ddz_T(gx / rho0, 1) + self.ddx_T(gz / rho0)
+        scale = diml / 1e8
+        gpsi = gpv * rho0 * cc / scale
+        gc = g2 * 2.0 * cc * rho0 / dimc
+        if psi is not None:
+            gc = gc + gpv * rho0 * np.asarray(psi, dtype=np.float64) / scale / dimc
+        return gpsi, gc
 
 
 def dbyd2(B: np.ndarray, ibc: int, A: np.ndarray | None = None) -> np.ndarray:

@@ -71,6 +71,10 @@ struct orc_model {
     double *forcing;
     double *zprof;                      /* profile source: [nz], NULL for the one-row form */
     int maxtime, steps, cadforcing_step, forcing_length, num_steps;
+    /* discrete adjoint: the z-derivative of ibc 1, 2, 3 as dense matrices
+     * [3][nz][nz] (output row, input row; stretch included) and the ibc-3
+     * response to its kept last row [nz]; built on first use */
+    double *dzm, *dzmN;
 };
 
 static double *dalloc(size_t n) { return (double *)calloc(n ? n : 1, sizeof(double)); }
@@ -454,7 +458,7 @@ void orc_destroy(orc_model *m)
                        &m->temp, &m->scr, &m->pmlvars, &m->psipml, &m->scrpml, &m->pmlvarsx,
                        &m->psipmlx, &m->scrpmlx, &m->dxixdx, &m->dxizdz, &m->pp, &m->rho,
                        &m->gradp_x, &m->gradp_z, &m->div, &m->fre, &m->fim, &m->vr,
-                       &m->fwdsource, &m->forcing, &m->zprof};
+                       &m->fwdsource, &m->forcing, &m->zprof, &m->dzm, &m->dzmN};
     for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); i++) free(*ptrs[i]);
     free(m);
 }
@@ -666,7 +670,12 @@ void orc_rhs(orc_model *m, int time, const double *temp4, const double *psipml,
             gradp_x[q2] = gradp_x[q2] + psi_gradp_x[qx2];
             RHS_psi_gradp_x[qx2] = m->axpml[qx2] * gradp_x[q2] + m->bxpml[qx2] * psi_gradp_x[qx2];
         }
-    orc_ddz(m, p, gradp_z, 2);                                     /* :1023 (stale row 1) */
+    /* :1023 with ibc 2 keeps row 1 of gradp_z as it is: the previous call's
+     * value, which stays 0 along every run (psi_p of that row starts at 0 and
+     * its RHS is then bz psi_p); it is set to 0 here so that one evaluation
+     * depends on its arguments only, as in the HIP column pass */
+    for (int i = 0; i < nx; i++) gradp_z[i] = 0.0;
+    orc_ddz(m, p, gradp_z, 2);
     pmlindex = 0;                                                  /* :1025-1035 */
     for (int k = 0; k < nz; k++) {
         if (k < m->p.npmlbot || k >= nz - m->p.npmltop) {
@@ -883,6 +892,9 @@ double orc_info_double(const orc_model *m, const char *name)
     if (!strcmp(name, "beta2")) return m->betas[2];
     if (!strcmp(name, "beta3")) return m->betas[3];
     if (!strcmp(name, "beta4")) return m->betas[4];
+    if (!strcmp(name, "dimc")) return m->dimc;
+    if (!strcmp(name, "diml")) return m->diml;
+    if (!strcmp(name, "dimrho")) return m->dimrho;
     return NAN;
 }
 
@@ -993,5 +1005,287 @@ int orc_kernels(const orc_params *p, const double *bg, const double *c2d, int nt
     free(zk); free(unst); free(stk); free(c2); free(rho0); free(eyekx); free(fre); free(fim);
     free(t1); free(t2); free(t3); free(kv1); free(kv3); free(kcc); free(hh);
     free(d11); free(d13); free(d31); free(d33);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Exact discrete adjoint of the option-17 step: the transpose of orc_step's
+ * linear map (5-stage low-storage RK of orc_rhs, z filter, PML auxiliaries)
+ * and the gradient of a functional of the vz traces with respect to the
+ * point coefficients c^2 rho0, v0x, v0z.  Not a routine of the reference,
+ * whose adjoint is the continuous recipe (a time-reversed run with -v0,
+ * driver.f90:144-147); this is its transpose, statement by statement. */
+
+static void dz_matrices(orc_model *m)
+{
+    if (m->dzm) return;
+    int nz = m->nz;
+    m->dzm = dalloc((size_t)3 * nz * nz);
+    m->dzmN = dalloc(nz);
+    double *e = dalloc(nz), *y = dalloc(nz);
+    for (int ibc = 1; ibc <= 3; ibc++) {
+        double *M = m->dzm + (size_t)(ibc - 1) * nz * nz;
+        for (int j = 0; j < nz; j++) {
+            memset(e, 0, nz * sizeof(double));
+            memset(y, 0, nz * sizeof(double));       /* kept rows (ibc 2 / 3) at 0 */
+            e[j] = 1.0;
+            ddz_cols(1, nz, m->stretch, e, y, ibc);
+            for (int i = 0; i < nz; i++) M[(size_t)i * nz + j] = y[i];
+        }
+    }
+    memset(e, 0, nz * sizeof(double));
+    memset(m->dzmN, 0, nz * sizeof(double));
+    m->dzmN[nz - 1] = 1.0;                           /* the kept last row of ibc 3 */
+    ddz_cols(1, nz, m->stretch, e, m->dzmN, 3);
+    free(e); free(y);
+}
+
+/* out = Dz_ibc^T in, column by column; for ibc 3 also presetN_bar[x] =
+ * sum_k dzmN[k] in[k][x], the adjoint of the kept last row. */
+void orc_ddz_T(orc_model *m, const double *in, double *out, int ibc, double *presetN_bar)
+{
+    dz_matrices(m);
+    int nx = m->nx, nz = m->nz;
+    const double *M = m->dzm + (size_t)(ibc - 1) * nz * nz;
+    memset(out, 0, (size_t)nx * nz * sizeof(double));
+    for (int i = 0; i < nz; i++)
+        for (int j = 0; j < nz; j++) {
+            double c = M[(size_t)i * nz + j];
+            if (c == 0.0) continue;
+            for (int x = 0; x < nx; x++) out[(size_t)j * nx + x] += c * in[(size_t)i * nx + x];
+        }
+    if (presetN_bar) {
+        for (int x = 0; x < nx; x++) presetN_bar[x] = 0.0;
+        for (int k = 0; k < nz; k++)
+            for (int x = 0; x < nx; x++) presetN_bar[x] += m->dzmN[k] * in[(size_t)k * nx + x];
+    }
+}
+
+/* Dx^T = -Dx: the multiplier i k is odd in k and the Nyquist bin is dropped. */
+void orc_ddx_T(const orc_model *m, const double *in, double *out, int nrows)
+{
+    orc_ddx(m, in, out, nrows);
+    for (size_t q = 0; q < (size_t)nrows * m->nx; q++) out[q] = -out[q];
+}
+
+/* Transpose of orc_rhs without its source term (physics2d.f90:943-1094 in
+ * reverse).  Rb* are the adjoints of the outputs (scr4, scrpml, scrpmlx),
+ * Yb* receive the adjoints of the inputs (temp4, psipml, psipmlx); pbar
+ * (may be NULL) receives the adjoint of p. */
+void orc_rhs_T(orc_model *m, const double *Rb4, const double *Rbpz, const double *Rbpx,
+               double *Yb4, double *Ybpz, double *Ybpx, double *pbar)
+{
+    int nx = m->nx, nz = m->nz, nh = m->p.npmlhor, nph = m->nph, nzpml = m->nzpml;
+    size_t n2 = (size_t)nx * nz;
+    double *R = dalloc(4 * n2), *t1 = dalloc(n2), *t2 = dalloc(n2);
+    double *gpz = dalloc(n2), *gpx = dalloc(n2), *rhob = dalloc(n2), *pb = dalloc(n2);
+    double *divb = dalloc(n2), *dxb = dalloc(n2), *dzb = dalloc(n2), *presetb = dalloc(nx);
+    memcpy(R, Rb4, 4 * n2 * sizeof(double));
+    memset(Yb4, 0, 4 * n2 * sizeof(double));
+    memset(Ybpz, 0, (size_t)2 * nzpml * nx * sizeof(double));
+    memset(Ybpx, 0, (size_t)2 * nz * nph * sizeof(double));
+    double *Rvx = R + 2 * n2, *Rvz = R + 3 * n2;
+    double *xxb = Yb4, *xzb = Yb4 + n2, *vxb = Yb4 + 2 * n2, *vzb = Yb4 + 3 * n2;
+    const double *Rpsivz = Rbpz, *Rpsip = Rbpz + (size_t)nzpml * nx;
+    double *psivzb = Ybpz, *psipb = Ybpz + (size_t)nzpml * nx;
+    const double *Rpsivx = Rbpx, *Rpsigpx = Rbpx + (size_t)nz * nph;
+    double *psivxb = Ybpx, *psigpxb = Ybpx + (size_t)nz * nph;
+    /* :1083-1092 zeroed rows and columns */
+    for (int f = 0; f < 4; f++) {
+        for (int i = 0; i < nx; i++) R[f * n2 + i] = 0.0;
+        for (int k = 0; k < nz; k++) {
+            R[f * n2 + (size_t)k * nx] = 0.0;
+            R[f * n2 + (size_t)k * nx + nx - 1] = 0.0;
+        }
+    }
+    /* :1065-1079 flows: RHS v -= 2 v0x dv/dx + 2 v0z dv/dz */
+    for (int c = 0; c < 2; c++) {
+        const double *Rv = c ? Rvz : Rvx;
+        double *vb = c ? vzb : vxb;
+        for (size_t q = 0; q < n2; q++) t1[q] = -2.0 * m->v0x[q] * Rv[q];
+        orc_ddx_T(m, t1, t2, nz);
+        for (size_t q = 0; q < n2; q++) vb[q] += t2[q];
+        for (size_t q = 0; q < n2; q++) t1[q] = -2.0 * m->v0z[q] * Rv[q];
+        orc_ddz_T(m, t1, t2, 1, NULL);
+        for (size_t q = 0; q < n2; q++) vb[q] += t2[q];
+    }
+    /* :1048-1049 RHS xi = v */
+    for (size_t q = 0; q < n2; q++) { vxb[q] += R[q]; vzb[q] += R[n2 + q]; }
+    /* :1039, :1044-1046 */
+    for (int k = 0; k < nz; k++)
+        for (int i = 0; i < nx; i++) {
+            size_t q = (size_t)k * nx + i;
+            gpz[q] = -m->rhoinv[q] * Rvz[q];
+            rhob[q] = -m->rhoinv[q] * m->g[k] * Rvz[q];
+            gpx[q] = -m->rhoinv[q] * Rvx[q];
+        }
+    /* :1025-1035 z-PML of d p/dz */
+    int pmlindex = 0;
+    for (int k = 0; k < nz; k++) {
+        if (k < m->p.npmlbot || k >= nz - m->p.npmltop) {
+            for (int i = 0; i < nx; i++) {
+                size_t q = (size_t)k * nx + i, qp = (size_t)pmlindex * nx + i;
+                psipb[qp] = gpz[q] + m->bzpml[q] * Rpsip[qp];
+                gpz[q] += m->az[q] * Rpsip[qp];
+            }
+            pmlindex++;
+        }
+    }
+    orc_ddz_T(m, gpz, pb, 2, NULL);                                /* :1023 */
+    /* :1003-1020 x-PML of d p/dx */
+    for (int k = 0; k < nz; k++)
+        for (int i = 0; i < nph; i++) {
+            size_t q = (size_t)k * nx + (i < nh ? i : nx - nh + (i - nh)), qx = (size_t)k * nph + i;
+            double tot = gpx[q] + m->axpml[qx] * Rpsigpx[qx];
+            psigpxb[qx] = tot + m->bxpml[qx] * Rpsigpx[qx];
+            gpx[q] = tot;
+        }
+    orc_ddx_T(m, gpx, t2, nz);                                     /* :1000 */
+    for (size_t q = 0; q < n2; q++) pb[q] += t2[q];
+    /* :998, :1043 p and rho' */
+    for (size_t q = 0; q < n2; q++) {
+        divb[q] = -m->rho0[q] * rhob[q] - m->c2rho0[q] * pb[q];
+        xzb[q] += -m->gradrho0_z[q] * rhob[q] - m->gradp0_z[q] * pb[q];
+    }
+    /* :990 div = dxixdx + dxizdz; :970-986 x-PML of dxi_x/dx */
+    memcpy(dxb, divb, n2 * sizeof(double));
+    memcpy(dzb, divb, n2 * sizeof(double));
+    for (int k = 0; k < nz; k++)
+        for (int i = 0; i < nph; i++) {
+            size_t q = (size_t)k * nx + (i < nh ? i : nx - nh + (i - nh)), qx = (size_t)k * nph + i;
+            double tot = dxb[q] + m->axpml[qx] * Rpsivx[qx];
+            psivxb[qx] = tot + m->bxpml[qx] * Rpsivx[qx];
+            dxb[q] = tot;
+        }
+    /* :957-967 z-PML of dxi_z/dz */
+    pmlindex = 0;
+    for (int k = 0; k < nz; k++) {
+        if (k < m->p.npmlbot || k >= nz - m->p.npmltop) {
+            for (int i = 0; i < nx; i++) {
+                size_t q = (size_t)k * nx + i, qp = (size_t)pmlindex * nx + i;
+                psivzb[qp] = dzb[q] + m->bzpml[q] * Rpsivz[qp];
+                dzb[q] += m->az[q] * Rpsivz[qp];
+            }
+            pmlindex++;
+        }
+    }
+    /* :955 ibc 3 with the kept last row :954 = -dxi_x/dx unstretch */
+    orc_ddz_T(m, dzb, t2, 3, presetb);
+    for (size_t q = 0; q < n2; q++) xzb[q] += t2[q];
+    for (int i = 0; i < nx; i++) dxb[(size_t)(nz - 1) * nx + i] += -m->unstretch[nz - 1] * presetb[i];
+    orc_ddx_T(m, dxb, t2, nz);                                     /* :953 */
+    for (size_t q = 0; q < n2; q++) xxb[q] += t2[q];
+    if (pbar) memcpy(pbar, pb, n2 * sizeof(double));
+    free(R); free(t1); free(t2); free(gpz); free(gpx); free(rhob); free(pb);
+    free(divb); free(dxb); free(dzb); free(presetb);
+}
+
+/* Transpose of FILTER_VARS_2D (physics2d.f90:509-556) on the 4 fields. */
+void orc_filter_vars_2d_T(const orc_model *m, double *a4)
+{
+    int nx = m->nx, nz = m->nz, nh = m->p.npmlhor;
+    size_t n2 = (size_t)nx * nz;
+    double *out = dalloc(n2);
+    for (int f = 0; f < 4; f++) {
+        double *a = a4 + f * n2;
+        memcpy(out, a, n2 * sizeof(double));
+        for (int i = nh - 1; i <= nx - nh; i++) {
+            for (int k = 5; k <= nz - 2; k++) out[(size_t)k * nx + i] = 0.0;   /* replaced rows */
+            for (int k = 5; k <= nz - 6; k++) {
+                double l = a[(size_t)k * nx + i];
+                out[(size_t)k * nx + i] += FC[0] * l;
+                for (int d = 1; d <= 5; d++) {
+                    out[(size_t)(k - d) * nx + i] += 0.5 * FC[d] * l;
+                    out[(size_t)(k + d) * nx + i] += 0.5 * FC[d] * l;
+                }
+            }
+            for (int k = nz - 5; k <= nz - 2; k++) {
+                double l = a[(size_t)k * nx + i];
+                out[(size_t)(k - 1) * nx + i] += 0.5 * l;
+                out[(size_t)(k + 1) * nx + i] += 0.5 * l;
+            }
+        }
+        memcpy(a, out, n2 * sizeof(double));
+    }
+    free(out);
+}
+
+/* Gradient of J = sum_k <gvz[k], vz_out[k]> (vz_out as orc_run records it,
+ * after step k*steps) over a forward run of maxtime+1 steps from the zero
+ * state, by the exact transpose of every step.  The model must be a forward
+ * model (adjoint = 0) with its source set.  Outputs [nz][nx]: dJ/d(c^2 rho0),
+ * dJ/dv0x, dJ/dv0z (code units), and (may be NULL) the adjoint of the
+ * initial state lam0 [4][nz][nx].  Every forward state is kept in memory.
+ * Returns 0, or -1 on an adjoint model. */
+int orc_discrete_gradient(orc_model *m, int maxtime, const double *gvz, double *g_c2rho0,
+                          double *g_v0x, double *g_v0z, double *lam0)
+{
+    if (m->p.adjoint) return -1;
+    int nx = m->nx, nz = m->nz;
+    size_t n2 = (size_t)nx * nz, n4 = 4 * n2;
+    size_t npz = (size_t)2 * m->nzpml * nx, npx = (size_t)2 * nz * m->nph;
+    size_t nst = n4 + npz + npx;
+    if (m->maxtime != maxtime) set_times(m, maxtime);
+    /* forward run keeping every state U_t (before step t) */
+    memset(m->a, 0, n4 * sizeof(double));
+    memset(m->pmlvars, 0, npz * sizeof(double));
+    memset(m->pmlvarsx, 0, npx * sizeof(double));
+    double *U = dalloc((size_t)(maxtime + 1) * nst);
+    for (int t = 0; t <= maxtime; t++) {
+        double *u = U + (size_t)t * nst;
+        memcpy(u, m->a, n4 * sizeof(double));
+        memcpy(u + n4, m->pmlvars, npz * sizeof(double));
+        memcpy(u + n4 + npz, m->pmlvarsx, npx * sizeof(double));
+        orc_step(m, t);
+    }
+    memset(g_c2rho0, 0, n2 * sizeof(double));
+    memset(g_v0x, 0, n2 * sizeof(double));
+    memset(g_v0z, 0, n2 * sizeof(double));
+    /* stage inputs and the forward quantities the gradient pairs with */
+    double *Y = dalloc(5 * nst), *R = dalloc(nst);
+    double *Dd = dalloc(5 * 5 * n2);   /* [stage][div, dvx/dx, dvx/dz, dvz/dx, dvz/dz] */
+    double *lam = dalloc(nst), *acc = dalloc(nst), *Rb = dalloc(nst), *Yb = dalloc(nst);
+    double *pb = dalloc(n2);
+    for (int t = maxtime; t >= 0; t--) {
+        if (t % m->steps == 0) {                 /* output k after step t */
+            int k = t / m->steps;
+            for (int i = 0; i < nx; i++) lam[3 * n2 + (size_t)m->o_rad * nx + i] += gvz[(size_t)k * nx + i];
+        }
+        const double *u = U + (size_t)t * nst;
+        for (int s = 0; s < 5; s++) {            /* orc_step's stages */
+            double *y = Y + (size_t)s * nst;
+            for (size_t q = 0; q < nst; q++) y[q] = s == 0 ? u[q] : u[q] + m->betas[s] * R[q];
+            orc_rhs(m, t, y, y + n4, y + n4 + npz, R, R + n4, R + n4 + npz);
+            double *D = Dd + (size_t)s * 5 * n2;
+            memcpy(D, m->div, n2 * sizeof(double));
+            orc_ddx(m, y + 2 * n2, D + n2, nz);
+            orc_ddz(m, y + 2 * n2, D + 2 * n2, 1);
+            orc_ddx(m, y + 3 * n2, D + 3 * n2, nz);
+            orc_ddz(m, y + 3 * n2, D + 4 * n2, 1);
+        }
+        /* back through a_new = filter(a + deltat scr), pml += deltat scrpml */
+        orc_filter_vars_2d_T(m, lam);
+        memcpy(acc, lam, nst * sizeof(double));
+        for (size_t q = 0; q < nst; q++) Rb[q] = m->deltat * lam[q];
+        for (int s = 4; s >= 0; s--) {
+            orc_rhs_T(m, Rb, Rb + n4, Rb + n4 + npz, Yb, Yb + n4, Yb + n4 + npz, pb);
+            const double *D = Dd + (size_t)s * 5 * n2;
+            for (int k = 0; k < nz; k++)
+                for (int i = 0; i < nx; i++) {
+                    size_t q = (size_t)k * nx + i;
+                    int z = (k == 0 || i == 0 || i == nx - 1);       /* zeroed RHS entries */
+                    double rvx = z ? 0.0 : Rb[2 * n2 + q], rvz = z ? 0.0 : Rb[3 * n2 + q];
+                    g_c2rho0[q] += -pb[q] * D[q];
+                    g_v0x[q] += -2.0 * (rvx * D[n2 + q] + rvz * D[3 * n2 + q]);
+                    g_v0z[q] += -2.0 * (rvx * D[2 * n2 + q] + rvz * D[4 * n2 + q]);
+                }
+            for (size_t q = 0; q < nst; q++) acc[q] += Yb[q];
+            if (s > 0)
+                for (size_t q = 0; q < nst; q++) Rb[q] = m->betas[s] * Yb[q];
+        }
+        memcpy(lam, acc, nst * sizeof(double));
+    }
+    if (lam0) memcpy(lam0, lam, n4 * sizeof(double));
+    free(U); free(Y); free(R); free(Dd); free(lam); free(acc); free(Rb); free(Yb); free(pb);
     return 0;
 }

@@ -104,6 +104,22 @@ void orc_forward_source_series(double timestep, double nupeak, double nuwidth,
 /* Complex FFT used by ddx (sign -1 forward, +1 inverse, unnormalized). */
 void orc_fft(double *re, double *im, int n, int sign);
 
+/* Exact discrete adjoint (transpose of orc_step; not a reference routine).
+ * orc_ddz_T: out = Dz_ibc^T in (stretch included); for ibc 3, presetN_bar[nx]
+ * (may be NULL) gets the adjoint of the kept last row.  orc_ddx_T = -orc_ddx.
+ * orc_rhs_T: transpose of orc_rhs without its source; pbar (may be NULL)
+ * gets the adjoint of p.  orc_filter_vars_2d_T: transpose of the z filter.
+ * orc_discrete_gradient: dJ/d(c^2 rho0), dJ/dv0x, dJ/dv0z [nz][nx] of
+ * J = sum_k <gvz[k], vz_out[k]> over a forward run from the zero state;
+ * lam0 (may be NULL) gets the adjoint of the initial state [4][nz][nx]. */
+void orc_ddz_T(orc_model *m, const double *in, double *out, int ibc, double *presetN_bar);
+void orc_ddx_T(const orc_model *m, const double *in, double *out, int nrows);
+void orc_rhs_T(orc_model *m, const double *Rb4, const double *Rbpz, const double *Rbpx,
+               double *Yb4, double *Ybpz, double *Ybpx, double *pbar);
+void orc_filter_vars_2d_T(const orc_model *m, double *a4);
+int orc_discrete_gradient(orc_model *m, int maxtime, const double *gvz, double *g_c2rho0,
+                          double *g_v0x, double *g_v0z, double *lam0);
+
 /* PRODUCE_KERNELS (kernels.f90:13-612) for one source.
  * fwd: [nt_kern][6][nz][nx] forward snapshots ordered by label 0,steps,...
  * adj: [nt_kern][4][nz][nx] adjoint snapshots ordered by label 0,steps,...

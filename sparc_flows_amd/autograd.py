@@ -33,6 +33,13 @@ These are the continuous-adjoint identities; the discrete solver matches them
 up to its boundary closures, the z filter and the sampling of the time
 integral at the output cadence, so finite differences agree to about a
 percent, not to rounding.
+
+With ``adjoint="discrete"`` the backward pass is instead the exact transpose
+of the discrete solver (Sparc.discrete_gradient, sparc_discrete.hip): every
+RK stage, the PMLs, the z filter and the compact-FD and FFT derivatives are
+transposed, and dL/dvz enters at the output steps as it is, so the gradient
+is the derivative of the computed traces themselves and finite differences
+reproduce it to their own accuracy.
 """
 from __future__ import annotations
 
@@ -103,6 +110,14 @@ class _Solve(torch.autograd.Function):
     def forward(ctx, psi, c, pipe, xsrc, maxtime):
         pipe.set_model(c, psi)
         ctx.binomial = pipe.slots is not None
+        ctx.pipe, ctx.maxtime, ctx.model_id, ctx.xsrc = pipe, maxtime, pipe.model_id, xsrc
+        if pipe.adjoint == "discrete":
+            # the backward pass recomputes the trajectory from its own
+            # checkpoints: nothing but the traces is kept here
+            vz, _ = pipe.fwd.forward(xsrc, maxtime)
+            ctx.fsn = ctx.ck = ctx.last = None
+            ctx.save_for_backward(psi, c)
+            return vz.clone()
         if ctx.binomial:
             vz, ctx.ck = forward_binomial(pipe.fwd, xsrc, maxtime, pipe.slots)
             ctx.fsn = ctx.last = None
@@ -125,6 +140,11 @@ class _Solve(torch.autograd.Function):
         pipe = ctx.pipe
         if pipe.model_id != ctx.model_id:        # another solve changed the model since
             pipe.set_model(c, psi)
+        if pipe.adjoint == "discrete":
+            pipe.fwd.set_forward_sources(ctx.xsrc, ctx.maxtime)
+            gpsi, gc, _ = pipe.fwd.discrete_gradient(gvz.to(torch.float64).contiguous(),
+                                                     interval=pipe.discrete_interval())
+            return gpsi.sum(0).to(psi.device), gc.sum(0).to(c.device), None, None, None
         vr = pipe.adjoint_source_from_grad(gvz)
         if ctx.binomial:
             K = kernels_binomial(pipe.fwd, pipe.adj, ctx.ck, vr, ctx.maxtime, pipe.slots)
@@ -156,10 +176,18 @@ class DifferentiableSolver:
     With ``checkpoint_slots`` c (not None) the backward pass uses binomial
     checkpointing instead (gradient.binomial_schedule): at most c full-state
     checkpoints and one snapshot per run, whatever the run length.
+    ``adjoint``: "sparc" (default) runs the reference's recipe above;
+    "discrete" differentiates the discrete solver exactly
+    (Sparc.discrete_gradient), recomputing the forward trajectory from a
+    checkpoint every ``checkpoint_interval`` steps (100 when 0).
     """
 
     def __init__(self, params: SparcParams, model: np.ndarray, xsrc, maxtime: int,
-                 checkpoint_interval: int = 0, checkpoint_slots: int | None = None):
+                 checkpoint_interval: int = 0, checkpoint_slots: int | None = None,
+                 adjoint: str = "sparc"):
+        if adjoint not in ("sparc", "discrete"):
+            raise ValueError(f"unknown adjoint {adjoint!r}: 'sparc' (the reference's recipe) or 'discrete'")
+        self.adjoint = adjoint
         if params.steps != params.cadforcing_step:
             raise ValueError("the adjoint source is sampled at the output cadence: "
                              "outputcad and cadforcing must give the same step count")
@@ -169,7 +197,7 @@ class DifferentiableSolver:
         self.maxtime = int(maxtime)
         n = len(self.xsrc)
         self.fwd = Sparc(params, model, nsrc=n, adjoint=False)
-        self.adj = Sparc(params, model, nsrc=n, adjoint=True)
+        self.adj = Sparc(params, model, nsrc=n, adjoint=True) if adjoint == "sparc" else None
         self.dx, _ = grid_cells(params, model)
         self.dT = params.steps * params.timestep * DIMC / RSUN
         # 0: keep every forward snapshot; n > 0: checkpoint every n steps and
@@ -180,11 +208,17 @@ class DifferentiableSolver:
             raise ValueError("checkpoint_slots must be >= 0")
         self.model_id = 0
 
+    def discrete_interval(self) -> int:
+        """Checkpoint interval of the discrete adjoint's trajectory recompute:
+        checkpoint_interval when set, else 100 steps."""
+        return self.interval if self.interval > 0 else min(100, self.maxtime + 1)
+
     def set_model(self, c, psi) -> None:
         cn = c.detach().cpu().numpy() if torch.is_tensor(c) else np.asarray(c)
         pn = psi.detach().cpu().numpy() if torch.is_tensor(psi) else np.asarray(psi)
         self.fwd.set_model(cn, pn)
-        self.adj.set_model(cn, pn)
+        if self.adj is not None:
+            self.adj.set_model(cn, pn)
         self.model_id += 1
 
     def adjoint_source_from_grad(self, gvz: torch.Tensor) -> torch.Tensor:

@@ -23,7 +23,7 @@ HIP_SO = LIB / "libsparc_hip.so"
 TORCH_SO = LIB / "libsparc_torch.so"
 ARCH = os.environ.get("SPARC_OFFLOAD_ARCH", "gfx950")
 
-HIP_SOURCES = ["sparc_kernels.hip", "sparc_colseg.hip", "sparc_capi.hip", "sparc_3d.hip"]
+HIP_SOURCES = ["sparc_kernels.hip", "sparc_colseg.hip", "sparc_capi.hip", "sparc_3d.hip", "sparc_discrete.hip"]
 # plain C++ compiled for the host (also built into the CPU check of the column pass)
 HOST_SOURCES = ["sparc_segtab.cpp"]
 HIP_FLAGS = [

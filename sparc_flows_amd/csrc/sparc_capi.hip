@@ -1625,6 +1625,123 @@ int sparc_kernels(sparc_handle *h, int nt, const double *fwd, const double *adj,
     return r;
 }
 
+// ---------------------------------------------------------------------------
+// Exact discrete adjoint (sparc_discrete.hip).
+
+// The dense z-derivative matrices of ibc 1, 2, 3 (stretch included; kept rows
+// at 0), their transposes and ibc 3's response to its kept last row, built by
+// the library's compact-FD solve of an identity block.
+static int discrete_matrices(sparc_handle *h, DevArena &ar, DiscreteCtx &c)
+{
+    const int nz = h->nz;
+    const size_t nn = (size_t)nz * nz;
+    std::vector<double> eye(nn, 0.0), m;
+    for (int i = 0; i < nz; i++) eye[(size_t)i * nz + i] = 1.0;
+    for (int ibc = 1; ibc <= 3; ibc++) {
+        int r = host_ddz(h, eye, m, nz, 1, ibc, true, ibc);
+        if (r) return r;
+        std::vector<double> mt(nn);
+        for (int i = 0; i < nz; i++)
+            for (int j = 0; j < nz; j++) mt[(size_t)j * nz + i] = m[(size_t)i * nz + j];
+        double *p = nullptr;
+        HCHECK(h, upload(ar, &p, m, h->stream));
+        c.M[ibc - 1] = p;
+        HCHECK(h, upload(ar, &p, mt, h->stream));
+        c.MT[ibc - 1] = p;
+    }
+    // ibc 3 with a zero input and the kept last row 1
+    double *di = nullptr, *dout = nullptr, *dw = nullptr;
+    HCHECK(h, ar.alloc(&di, nz));
+    HCHECK(h, ar.alloc(&dout, nz));
+    HCHECK(h, ar.alloc(&dw, nz));
+    HCHECK(h, launch_fill(di, nz, 0.0, h->stream));
+    HCHECK(h, launch_fill(dout, nz - 1, 0.0, h->stream));
+    HCHECK(h, launch_fill(dout + nz - 1, 1, 1.0, h->stream));
+    HCHECK(h, launch_ddz_cols(di, dout, 1, nz, 1, 3, h->bg.stretch, h->cf, h->tri[3], dw, 0, h->stream));
+    c.vN3 = dout;
+    return SPARC_OK;
+}
+
+int sparc_discrete_gradient(sparc_handle *h, int maxtime, const double *gvz_dev, int nt, int interval,
+                            double *gpsi_dev, double *gc_dev, double *gcoef_dev)
+{
+    if (!h || !gvz_dev || maxtime < 0 || interval < 0) return SPARC_E_ARG;
+    if (h->adjoint) return fail(h, SPARC_E_STATE, "the discrete adjoint differentiates a forward handle");
+    if (!h->have_src || h->maxtime != maxtime || (h->src.mode != 1 && h->src.mode != 3))
+        return fail(h, SPARC_E_STATE, "forward sources for this maxtime not set");
+    if (nt != maxtime / h->steps + 1) return fail(h, SPARC_E_ARG, "gvz must hold maxtime/steps + 1 outputs");
+    if (gpsi_dev && !h->flowx.empty())
+        return fail(h, SPARC_E_STATE, "d/dpsi needs the flow from psi, not an explicit flow");
+    const int C = interval ? std::min(interval, maxtime + 1) : maxtime + 1;
+    DevArena ar;
+    DiscreteCtx c{};
+    c.d = h->d;
+    c.bg = h->bg;
+    c.src = h->src;
+    c.stream = h->stream;
+    for (int s = 0; s < 5; s++) c.betas[s] = h->betas[s];
+    c.deltat = h->deltat;
+    c.unstretch_last = h->unstretch[h->nz - 1];
+    c.steps = h->steps;
+    c.orow = h->o_rad;
+    auto done = [&](int r) { (void)hipStreamSynchronize(h->stream); ar.release(); return r; };
+    int r = discrete_matrices(h, ar, c);
+    if (r) return done(r);
+    DiscreteWork w{};
+    const size_t S = h->S;
+    w.SN = S * h->nz * h->nx;
+    w.PZ1 = S * h->d.nzpml * h->nx;
+    w.PX1 = S * h->nz * h->d.nph;
+    w.NST = 4 * w.SN + 2 * w.PZ1 + 2 * w.PX1;
+    const int nseg = (maxtime + C) / C;
+    struct B { double **p; size_t n; } bufs[] = {
+        {&w.U, w.NST}, {&w.Unext, w.NST}, {&w.Y, w.NST}, {&w.R, w.NST}, {&w.acc, w.NST}, {&w.Rb, w.NST},
+        {&w.Yb, w.NST}, {&w.lam, w.NST}, {&w.ck, (size_t)nseg * w.NST}, {&w.seg, (size_t)C * w.NST},
+        {&w.Dst, 25 * w.SN}, {&w.Dscr, 5 * w.SN}, {&w.T0, w.SN}, {&w.T1, w.SN}, {&w.T2, w.SN}, {&w.T3, w.SN},
+        {&w.GZ, w.SN}, {&w.DGZ, w.SN}, {&w.RHOB, w.SN}, {&w.XZP, w.SN}, {&w.DXB, w.SN}, {&w.DZB, w.SN},
+        {&w.DZ3B, w.SN}, {&w.DDXB, w.SN}, {&w.FX, 3 * w.SN}, {&w.DFX, 3 * w.SN}, {&w.FZ, 2 * w.SN},
+        {&w.DFZ, 2 * w.SN}, {&w.PB, S * h->nx}};
+    for (const B &b : bufs) {
+        hipError_t e = ar.alloc(b.p, b.n);
+        if (e != hipSuccess) { h->err = std::string("discrete adjoint work: ") + hipGetErrorString(e); return done(SPARC_E_HIP); }
+    }
+    double *G = gcoef_dev;
+    if (!G) {
+        hipError_t e = ar.alloc(&G, 3 * w.SN);
+        if (e != hipSuccess) { h->err = hipGetErrorString(e); return done(SPARC_E_HIP); }
+    }
+    hipError_t e = discrete_gradient(c, w, maxtime, C, gvz_dev, nt, G);
+    if (e == hipSuccess && (gpsi_dev || gc_dev)) {
+        const size_t n2 = (size_t)h->nx * h->nz;
+        std::vector<double> psi(h->psi.empty() ? std::vector<double>() : h->psi);
+        double *dc2 = nullptr, *dpsi = nullptr;
+        std::vector<double> c2(h->c2.begin(), h->c2.begin() + n2);
+        int rr = upload(ar, &dc2, c2, h->stream) == hipSuccess ? SPARC_OK : SPARC_E_HIP;
+        if (!rr && !psi.empty()) rr = upload(ar, &dpsi, psi, h->stream) == hipSuccess ? SPARC_OK : SPARC_E_HIP;
+        if (rr) return done(rr);
+        const double dimc = 1e6, scale = kRsun / 1.0e8;
+        e = discrete_flow_chain(c, w, G, h->bg.rho0, dc2, dpsi, scale, dimc, gpsi_dev, gc_dev);
+    }
+    if (e != hipSuccess) { h->err = std::string("discrete adjoint: ") + hipGetErrorString(e); return done(SPARC_E_HIP); }
+    return done(SPARC_OK);
+}
+
+int sparc_ddz_T(sparc_handle *h, const double *in_dev, double *out_dev, int nbatch, int ibc, double *presetN_bar_dev)
+{
+    if (!h || !in_dev || !out_dev || nbatch < 1 || ibc < 1 || ibc > 3) return SPARC_E_ARG;
+    DevArena ar;
+    DiscreteCtx c{};
+    c.d = h->d;
+    c.stream = h->stream;
+    int r = discrete_matrices(h, ar, c);
+    hipError_t e = r ? hipSuccess : discrete_apply(c, in_dev, out_dev, nbatch, ibc + 2, presetN_bar_dev);
+    (void)hipStreamSynchronize(h->stream);
+    ar.release();
+    if (r) return r;
+    HCHECK(h, e);
+    return SPARC_OK;
+}
+
 int sparc_profile_kernels(sparc_handle *h, int reps, double *ms_out)
 {
     if (!h || reps < 1 || !ms_out) return SPARC_E_ARG;

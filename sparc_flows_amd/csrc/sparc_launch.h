@@ -37,4 +37,40 @@ hipError_t launch_kern_final(int S, size_t n2, const double *rho0, const double 
                              const double *KC, const double *KX, const double *KZ, double *kc,
                              double *kpsi, double *kvx, double *kvz, hipStream_t st);
 
+// Exact discrete adjoint (sparc_discrete.hip).  Full states are
+// [4][S][nz][nx] fields, then [2][S][nzpml][nx] (psivz, psip), then
+// [2][S][nz][nph] (psivx, psi_gradp_x).
+struct DiscreteCtx {
+    Dims d;
+    Bg bg;
+    Src src;
+    hipStream_t stream;
+    const double *M[3], *MT[3];    // the z-derivative of ibc 1, 2, 3 as nz x nz matrices, and their transposes
+    const double *vN3;             // ibc 3's response to its kept last row [nz]
+    double betas[5], deltat, unstretch_last;
+    int steps, orow;
+};
+struct DiscreteWork {
+    size_t SN, PZ1, PX1, NST;      // one [S][nz][nx] field, one pz field, one px field, a full state
+    double *U, *Unext, *Y, *R, *acc, *Rb, *Yb, *lam;          // NST each
+    double *ck, *seg;                                         // segment starts, one segment of states
+    double *Dst, *Dscr;                                       // [5][5][SN], [5][SN]
+    double *T0, *T1, *T2, *T3, *GZ, *DGZ, *RHOB, *XZP, *DXB, *DZB, *DZ3B, *DDXB;   // SN each
+    double *FX, *DFX, *FZ, *DFZ;                              // 3, 3, 2, 2 SN
+    double *PB;                                               // [S][nx]
+};
+// dJ/d(c^2 rho0), dJ/dv0x, dJ/dv0z into G [3][S][nz][nx] for J = sum <gvz, vz>
+// (gvz [S][nt][nx]) over a forward run of maxtime+1 steps from the zero
+// state, recomputing segments of `interval` steps from checkpoints.
+hipError_t discrete_gradient(const DiscreteCtx &c, const DiscreteWork &w, int maxtime, int interval,
+                             const double *gvz, int nt, double *G);
+// dJ/dpsi, dJ/dc [S][nz][nx] from G through c^2 rho0 and the flow map.
+hipError_t discrete_flow_chain(const DiscreteCtx &c, const DiscreteWork &w, const double *G, const double *rho0,
+                               const double *c2, const double *psi, double scale, double dimc, double *gpsi,
+                               double *gc);
+// out = Dz_ibc in (which 0..2) or Dz_ibc^T in (which 3..5) for nbatch fields;
+// for which 5 also the adjoint of the kept last row presetN_bar [nbatch][nx].
+hipError_t discrete_apply(const DiscreteCtx &c, const double *in, double *out, int nbatch, int which,
+                          double *presetN_bar);
+
 }  // namespace sparc

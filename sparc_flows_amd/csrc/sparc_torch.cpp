@@ -247,6 +247,31 @@ struct Solver : torch::CustomClassHolder {
     at::Tensor ddz(at::Tensor in, int64_t ibc, at::Tensor preset) { return ddz_impl(in, ibc, preset, false); }
     at::Tensor ddz_kern(at::Tensor in, int64_t ibc) { return ddz_impl(in, ibc, at::Tensor(), true); }
 
+    // exact discrete adjoint: gvz [nsrc][nt][nx] -> gpsi, gc, gcoef ([3][nsrc][nz][nx])
+    std::vector<at::Tensor> discrete_gradient(int64_t maxtime, at::Tensor gvz, int64_t interval, bool want_psi)
+    {
+        check_dev(gvz, "gvz");
+        TORCH_CHECK(gvz.dim() == 3 && gvz.size(0) == nsrc && gvz.size(2) == nx, "gvz must be [nsrc][nt][nx]");
+        at::Tensor gpsi = at::empty({nsrc, nz, nx}, opts()), gc = at::empty({nsrc, nz, nx}, opts());
+        at::Tensor gcoef = at::empty({3, nsrc, nz, nx}, opts());
+        check(sparc_discrete_gradient(h, (int)maxtime, gvz.data_ptr<double>(), (int)gvz.size(1), (int)interval,
+                                      want_psi ? gpsi.data_ptr<double>() : nullptr, gc.data_ptr<double>(),
+                                      gcoef.data_ptr<double>()),
+              h, "sparc_discrete_gradient");
+        return {gpsi, gc, gcoef};
+    }
+    // transpose of ddz (ibc 1..3); returns out and, for ibc 3, the adjoint of the kept last row
+    std::vector<at::Tensor> ddz_T(at::Tensor in, int64_t ibc)
+    {
+        check_dev(in, "in");
+        TORCH_CHECK(in.numel() % (nx * nz) == 0, "in must hold whole [nz][nx] fields");
+        const int64_t nb = in.numel() / (nx * nz);
+        at::Tensor out = at::empty_like(in), pb = at::zeros({nb, nx}, opts());
+        check(sparc_ddz_T(h, in.data_ptr<double>(), out.data_ptr<double>(), (int)nb, (int)ibc, pb.data_ptr<double>()),
+              h, "sparc_ddz_T");
+        return {out, pb};
+    }
+
     std::vector<double> profile_kernels(int64_t reps)
     {
         std::vector<double> ms(5, 0.0);
@@ -306,5 +331,7 @@ TORCH_LIBRARY(sparc, m)
         .def("ddz", &Solver::ddz)
         .def("ddz_kern", &Solver::ddz_kern)
         .def("kernels", &Solver::kernels)
-        .def("profile_kernels", &Solver::profile_kernels);
+        .def("profile_kernels", &Solver::profile_kernels)
+        .def("discrete_gradient", &Solver::discrete_gradient)
+        .def("ddz_T", &Solver::ddz_T);
 }

@@ -227,6 +227,28 @@ class Sparc:
             torch.as_tensor(preset, dtype=torch.float64, device=self.device).contiguous()
         return self._done(self._stream().ddz(f, int(ibc), pre))
 
+    def ddz_T(self, f: torch.Tensor, ibc: int):
+        """Transpose of ddz (ibc 1..3, kept rows 0); for ibc 3 also the
+        adjoint of the kept last row [nbatch][nx]."""
+        f = torch.as_tensor(f, dtype=torch.float64, device=self.device).contiguous()
+        out, pb = self._done(self._stream().ddz_T(f, int(ibc)))
+        return (out, pb) if ibc == 3 else out
+
+    def discrete_gradient(self, gvz: torch.Tensor, interval: int = 0, psi: bool = True):
+        """Exact gradient of J = sum(gvz * vz) over the forward run of the
+        sources set for self.maxtime, by the transpose of the discrete step
+        (sparc_discrete_gradient): dJ/dpsi, dJ/dc and [dJ/d(c^2 rho0),
+        dJ/dv0x, dJ/dv0z], each [nsrc][nz][nx].  interval: checkpoint
+        interval of the trajectory recompute (0: keep every state)."""
+        if self.is_adjoint:
+            raise ValueError("the discrete adjoint differentiates a forward solver")
+        if self.maxtime is None:
+            raise ValueError("set the forward sources first")
+        g = torch.as_tensor(gvz, dtype=torch.float64, device=self.device).contiguous()
+        gpsi, gc, gcoef = self._done(self._stream().discrete_gradient(int(self.maxtime), g, int(interval),
+                                                                      bool(psi)))
+        return (gpsi if psi else None), gc, gcoef
+
     def ddz_kern(self, f: torch.Tensor, ibc: int) -> torch.Tensor:
         f = torch.as_tensor(f, dtype=torch.float64, device=self.device).contiguous()
         return self._done(self._stream().ddz_kern(f, int(ibc)))

@@ -50,12 +50,12 @@ def smooth_direction(rng, nz, nx, npml, nph, width=3.0):
     return f / np.abs(f).max()
 
 
-def _check(p, bg, c2d, psi, xsrc, maxtime, ndir, seed, checkpoint_interval=0):
-    """Directional derivatives from the adjoint and from central differences;
-    returns {"psi": err, "c": err} with err = |ad - fd| / |fd| over the
-    directions of that parameter, and the number of directions whose signs
-    agree."""
-    ds = DifferentiableSolver(p, bg, xsrc, maxtime, checkpoint_interval=checkpoint_interval)
+def _check(p, bg, c2d, psi, xsrc, maxtime, ndir, seed, checkpoint_interval=0, adjoint="sparc", h=1e-6):
+    """Directional derivatives from the adjoint and from central differences
+    (step h times the parameter's maximum); returns {"psi": err, "c": err}
+    with err = |ad - fd| / |fd| over the directions of that parameter, and
+    the number of directions whose signs agree."""
+    ds = DifferentiableSolver(p, bg, xsrc, maxtime, checkpoint_interval=checkpoint_interval, adjoint=adjoint)
     print("column-pass shapes: forward", {k: ds.fwd.query(k) for k in ("colseg_ms", "colseg_tc")},
           flush=True)
     mask = torch.as_tensor(receiver_mask(p, xsrc, 5.0, 60.0), device="cuda")
@@ -65,8 +65,7 @@ def _check(p, bg, c2d, psi, xsrc, maxtime, ndir, seed, checkpoint_interval=0):
     # reference traces: the model without flow
     with torch.no_grad():
         vref = ds.solve(torch.zeros_like(torch.as_tensor(psi)), torch.as_tensor(c2d)).clone()
-    print("after the first solve: shapes", {k: (ds.fwd.query(k), ds.adj.query(k))
-                                            for k in ("colseg_ms", "colseg_tc", "row_wide")},
+    print("after the first solve: shapes", {k: ds.fwd.query(k) for k in ("colseg_ms", "colseg_tc", "row_wide")},
           "vref finite", bool(torch.isfinite(vref).all()), flush=True)
     tau_obs = torch.zeros(len(xsrc), p.nx, dtype=torch.float64, device="cuda")
 
@@ -82,7 +81,6 @@ def _check(p, bg, c2d, psi, xsrc, maxtime, ndir, seed, checkpoint_interval=0):
     L.backward()
     gpsi, gc = ps.grad.numpy(), cc.grad.numpy()
     rng = np.random.default_rng(seed)
-    h = 1e-6
     res = {"psi": ([], []), "c": ([], [])}
     for i in range(ndir):
         d = smooth_direction(rng, p.nz, p.nx, p.npmlbot, p.npmlhor)
@@ -204,3 +202,22 @@ def test_binomial_backward_equals_stored_snapshots():
         e = float(torch.linalg.norm(a - b) / torch.linalg.norm(a))
         print("binomial autograd", e)
         assert e < 1e-12
+
+
+def test_discrete_adjoint_matches_finite_differences_reference_heights():
+    """adjoint="discrete" (the transpose of the discrete solver) at the
+    reference's heights (receivers at +200 km in the first row of the top
+    PML, source at -150 km) and its 30 s cadence, where the recipe is off by
+    tens of percent: 256 x 300, flow and sound-speed bump, 4 sources, 4000
+    steps, trajectory checkpoints every 100 steps, 8 directions, central
+    differences with a 1e-6 relative step; agreement to 1e-6."""
+    bg = B.polytrope(300)
+    from sparc_flows_amd.params import SparcParams
+    p = SparcParams(nx=256, nz=300, xlength=200e8, timestep=B.stable_timestep(bg), npmlbot=12,
+                    npmltop=12, npmlhor=9)
+    assert (p.obsheight, p.excitdep, p.outputcad) == (2e7, -1.5e7, 30.0)
+    psi = B.supergranule_psi(p.nx, p.xlength, bg)
+    c2d = np.asarray(B.sound_speed_bump(p.nx, p.xlength, bg), dtype=np.float64)
+    xs = list(B.source_positions(4, seed=4))
+    err, signs = _check(p, bg, c2d, psi, xs, 3999, ndir=8, seed=2, checkpoint_interval=100, adjoint="discrete")
+    assert err["psi"] < 1e-6 and err["c"] < 1e-6 and signs == 8
