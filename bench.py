@@ -22,7 +22,8 @@ concurrently; bounded samples).  other_configurations holds
   * the batched 16-source forward at 128x150 (flow and sound-speed bump,
     Gaussian force sources, 4000 steps) beside 16 sequential single-source runs;
   * the 8-source autograd gradient at 256x300 (travel-time misfit, 4000 steps,
-    checkpoints every 100 steps);
+    checkpoints every 100 steps), with the reference's adjoint recipe and
+    with the exact discrete adjoint;
   * the 3D (option 16) forward and checkpointed gradient at 256x256x300
     (3000 steps, checkpoints every 150 steps).
 `--only-3d` prints just the two 3D lines.
@@ -291,12 +292,13 @@ def cpu_baseline(p, bg, psi, seconds=12.0, instances=1):
     return out
 
 
-def autograd_rate(nsrc=8, nsteps=4000, interval=100):
+def autograd_rate(nsrc=8, nsteps=4000, interval=100, adjoint="sparc"):
     """Seconds per autograd gradient of 8 sources at 256x300 (flow and
     sound-speed bump): DifferentiableSolver with a full-state checkpoint every
     `interval` steps, the travel-time misfit against reference traces of the
     model without flow (computed before the timed region), L.backward()
-    giving dL/dpsi and dL/dc on the grid.  The timed gradient is the second
+    giving dL/dpsi and dL/dc on the grid; `adjoint` "sparc" (the reference's
+    recipe) or "discrete" (the exact transpose of the solver).  The timed gradient is the second
     one (the first fixes the autotuned shapes and the allocator's pools)."""
     import torch
     from sparc_flows_amd import background as B
@@ -307,7 +309,7 @@ def autograd_rate(nsrc=8, nsteps=4000, interval=100):
     c2d = B.sound_speed_bump(p.nx, p.xlength, bg)
     xs = list(B.source_positions(nsrc, seed=0))
     maxtime = nsteps - 1
-    ds = DifferentiableSolver(p, bg, xs, maxtime, checkpoint_interval=interval)
+    ds = DifferentiableSolver(p, bg, xs, maxtime, checkpoint_interval=interval, adjoint=adjoint)
     mask = torch.as_tensor(receiver_mask(p, xs), device="cuda")
     cad = p.steps * p.timestep
     w = window_weights(p.nt_out(maxtime), cad, 1800.0, device="cuda")
@@ -333,9 +335,11 @@ def autograd_rate(nsrc=8, nsteps=4000, interval=100):
     finite = bool(np.isfinite(L) and torch.isfinite(gp).all() and torch.isfinite(gc).all()
                   and float(gp.abs().max()) > 0 and float(gc.abs().max()) > 0)
     return {"s_per_gradient": sec, "sources": nsrc, "grid": [p.nx, p.nz], "time_steps": nsteps,
-            "checkpoint_interval": interval, "misfit": L, "finite": finite,
+            "checkpoint_interval": interval, "misfit": L, "finite": finite, "adjoint": adjoint,
             "Mpoint_steps_per_s_fwd_adj": p.nx * p.nz * nsteps * 2 * nsrc / sec / 1e6,
-            "runs": "forward with checkpoints, recompute of all but the last segment, adjoint"}
+            "runs": ("forward with checkpoints, recompute of all but the last segment, adjoint"
+                     if adjoint == "sparc" else
+                     "forward, trajectory recompute from checkpoints, transposed step (exact discrete adjoint)")}
 
 
 def reference_traces(p, bg, xs, maxtime):
@@ -484,6 +488,8 @@ def main():
         log("batched forward 128x150 done")
         others["autograd8_gradient_256x300"] = autograd_rate(8, 4000, 100)
         log("autograd gradient done")
+        others["autograd8_discrete_gradient_256x300"] = autograd_rate(8, 4000, 100, adjoint="discrete")
+        log("discrete-adjoint gradient done")
         others["forward3d_256x256x300"] = forward3d_rate(cpu_seconds=0.0 if args.no_cpu else 8.0)
         log("3D forward done")
         others["gradient3d_256x256x300"] = gradient3d_rate(maxtime=args.gradient3d_steps - 1,
