@@ -47,19 +47,22 @@ HBM_PEAK_GBS = 8000.0
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "colseg_traffic_256x300.md")
 
 
-def measured_traffic(kernel: str, ms: int, nseg: int):
+def measured_traffic(kernel: str, ms: int, nseg: int, tc: int, nsrc: int):
     """HBM bytes per launch of the column pass from the committed rocprofv3
-    --pmc FETCH_SIZE / WRITE_SIZE summary, when it was taken with this kernel
-    and segmentation (the line `traffic: <kernel><MS,0> P=.. fetch_kib=..
-    write_kib=.. fetch_scale=.. write_scale=..`); None otherwise."""
+    --pmc FETCH_SIZE / WRITE_SIZE summary, when one of its lines was taken
+    with this kernel, batch size and launch shape (`traffic: <kernel><MS,0>
+    S=.. TC=.. P=.. fetch_kib=.. write_kib=.. fetch_scale=.. write_scale=..`);
+    None otherwise."""
     import re
     if not os.path.exists(TRAFFIC_FILE):
         return None
-    m = re.search(r"^traffic: (k_col\w+)<(\d+),0> P=(\d+) fetch_kib=([\d.]+) write_kib=([\d.]+) "
-                  r"fetch_scale=([\d.]+) write_scale=([\d.]+)", open(TRAFFIC_FILE).read(), re.M)
-    if not m or m.group(1) != kernel or int(m.group(2)) != ms or int(m.group(3)) != nseg:
-        return None
-    return (float(m.group(4)) * float(m.group(6)) + float(m.group(5)) * float(m.group(7))) * 1024.0
+    for m in re.finditer(r"^traffic: (k_col\w+)<(\d+),0> S=(\d+) TC=(\d+) P=(\d+) fetch_kib=([\d.]+) "
+                         r"write_kib=([\d.]+) fetch_scale=([\d.]+) write_scale=([\d.]+)",
+                         open(TRAFFIC_FILE).read(), re.M):
+        if (m.group(1), int(m.group(2)), int(m.group(3)), int(m.group(4)), int(m.group(5))) == \
+                (kernel, ms, nsrc, tc, nseg):
+            return (float(m.group(6)) * float(m.group(8)) + float(m.group(7)) * float(m.group(9))) * 1024.0
+    return None
 
 
 def make_config(nx=256, nz=300, npml=12):
@@ -457,11 +460,14 @@ def main():
         # times the single-source bytes, more than the 256 MB MALL holds
         bms = bpipe.fwd.profile_kernels(200)
         bach = bytes_per_pt["col"] * npts * args.batch / (bms["col"] * 1e-3) / 1e9
+        btraffic = measured_traffic("k_colsplit", bpipe.fwd.query("colseg_ms"), bpipe.fwd.query("colseg_p"),
+                                    bpipe.fwd.query("colseg_tc"), args.batch)
         batched = {"sources": args.batch, "s_per_gradient": bsec, "finite": bfinite,
                    "Mpoint_steps_per_s": pts * args.batch / bsec / 1e6,
                    "speedup_vs_sequential_plain": (sec * args.batch) / bsec,
                    "roofline_col": {"achieved": bach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                                    "frac": bach / HBM_PEAK_GBS, "ms_per_launch": bms},
+                                    "frac": bach / HBM_PEAK_GBS, "ms_per_launch": bms, "traffic": btraffic,
+                                    "algorithmic_bytes_per_launch": bytes_per_pt["col"] * npts * args.batch},
                    "shape": {k: bpipe.fwd.query(k) for k in ("colseg_tc", "colseg_ms", "colseg_p", "row_wide")}}
         del bpipe, bref
 
@@ -471,8 +477,8 @@ def main():
                    "row_first": ms["row_first"], "row_last": ms["row_last"]}
     dom = max(per_step_ms, key=per_step_ms.get)
     achieved = bytes_per_pt[dom] * npts / (ms[dom] * 1e-3) / 1e9
-    traffic = (measured_traffic("k_colsplit", pipe.fwd.query("colseg_ms"), pipe.fwd.query("colseg_p"))
-               if dom == "col" else None)
+    traffic = (measured_traffic("k_colsplit", pipe.fwd.query("colseg_ms"), pipe.fwd.query("colseg_p"),
+                                pipe.fwd.query("colseg_tc"), 1) if dom == "col" else None)
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": dom,
                 "ms_per_launch": ms, "algorithmic_bytes_per_point": bytes_per_pt[dom]}
