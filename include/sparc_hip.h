@@ -132,6 +132,12 @@ int sparc_run_range(sparc_handle *h, int t0, int t1, double *vz_out_dev, double 
  * checkpoint-and-recompute; they replace the reference's full-state dumps
  * (WRITE_OUT_FULL_STATE, all_modules.f90:623-694).  A checkpoint taken after
  * sparc_run / sparc_run_range ended at time index t - 1 restarts at t. */
+/* TIMESTEPPING's convergence statistic (driver.f90:364, written to rms_hist
+ * every step at :384): for the runs that follow, norm2_dev (device
+ * [nsrc][n]) gets sum(v_z^2) over the grid after each step t in t0 .. t0+n-1
+ * (slot t - t0; nor1 is its square root).  NULL switches it off.  The array
+ * must outlive those runs. */
+int sparc_set_norm_output(sparc_handle *h, double *norm2_dev, int t0, int n);
 int sparc_checkpoint_size(const sparc_handle *h, size_t *bytes);
 int sparc_save_checkpoint(sparc_handle *h, void *dev);
 int sparc_load_checkpoint(sparc_handle *h, const void *dev);

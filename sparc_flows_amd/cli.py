@@ -22,7 +22,7 @@ initialize.f90:984-1144, DETERMINE_STATUS driver.f90:480-631):
   adjoint source is computed from a 3D forward run.
 * Forward writes ``forward_srcSS_lsLL/{vz_cc.bin, vz_cc.fits, timeline,
   kernel_info, rms_hist}`` and the partial snapshots; adjoint writes
-  ``adjoint_srcSS/{timeline, kernel_info}`` and its partial snapshots; the
+  ``adjoint_srcSS/{timeline, kernel_info, rms_hist}`` and its partial snapshots; the
   kernel step writes ``kernel/{kernel_c,kernel_psi,kernel_vx,kernel_vz,
   hessian}_SS.fits`` and removes the partial snapshots (kernels.f90:591-597).
 
@@ -119,6 +119,15 @@ def _xsrc(ddir: Path, contrib: str) -> float:
     return vals[int(contrib) - 1]
 
 
+def _write_rms_hist(out, norm2) -> None:
+    """rms_hist: one line per time step with nor1 = sqrt(sum v_z^2) over the
+    grid after the step (driver.f90:247-248, 281-282 open it, :364 and :384
+    compute and write it)."""
+    with open(out / "rms_hist", "w") as f:
+        for v in np.sqrt(norm2.cpu().numpy()):
+            f.write(f"{v:26.16E}\n")
+
+
 def _forward(p, bg, psi, ddir, contrib, jobno, maxtime, compute_data, linesearch, compute_synth) -> int:
     import torch
 
@@ -136,7 +145,9 @@ def _forward(p, bg, psi, ddir, contrib, jobno, maxtime, compute_data, linesearch
         s = Sparc(p, bg, psi=psi)
     # partial snapshots only for a plain forward run (driver.f90:412-414)
     no_snapshots = linesearch or compute_synth or compute_data
+    s.record_norms(maxtime + 1)
     vz, sn = s.forward([_xsrc(ddir, contrib)], maxtime, snapshots=not no_snapshots)
+    _write_rms_hist(out, s.norm2[0])
     vz = vz[0].cpu().numpy()
     write_direct_records(out / "vz_cc.bin", vz)
     write_fits(out / "vz_cc.fits", vz[:, None, :], time=maxtime)
@@ -241,8 +252,10 @@ def _adjoint(p, bg, psi, ddir, contrib, maxtime) -> int:
     fl = p.forcing_length(maxtime)
     vr = read_fits(out / "source.fits").reshape(-1, p.nx)[:fl]
     s = Sparc(p, bg, adjoint=True, psi=psi)
+    s.record_norms(maxtime + 1)
     _, sn = s.adjoint(torch.as_tensor(vr[None], dtype=torch.float64, device="cuda"), maxtime,
                       snapshots=True)
+    _write_rms_hist(out, s.norm2[0])
     steps = s.steps
     cfs = p.cadforcing_step
     a = sn[0].cpu().numpy()

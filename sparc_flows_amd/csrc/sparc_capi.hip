@@ -118,6 +118,9 @@ struct sparc_handle {
     int seg_tc = 16, seg_ms = 20;
     int row_wide = 0;                    // row pass at nx = 256 on 16 (0), 32 (1) or 64 (2) lanes per row
     bool tuned = false;
+    // per-step v_z norm output of the runs (sparc_set_norm_output)
+    double *norm2 = nullptr;
+    int nnorm = 0, tnorm0 = 0;
 };
 
 // One launch of the column pass in the handle's chosen shape.
@@ -794,6 +797,8 @@ static int enqueue_step(sparc_handle *h, int toff, bool filt_vx, const Sink &sk)
         }
         if (r) return r;
     }
+    if (sk.norm2 && !h->profiling)
+        HCHECK(h, launch_vz_norm2(s.a[3], d.S, (size_t)d.nz * d.nx, clk, sk, h->stream));
     return SPARC_OK;
 }
 
@@ -803,6 +808,9 @@ static Sink empty_sink(sparc_handle *h)
     sk.orow = h->o_rad;
     sk.adjoint = h->adjoint;
     sk.nfield = h->adjoint ? 4 : 6;
+    sk.norm2 = h->norm2;
+    sk.nnorm = h->nnorm;
+    sk.tnorm0 = h->tnorm0;
     return sk;
 }
 
@@ -1426,6 +1434,15 @@ int sparc_run_range(sparc_handle *h, int t0, int t1, double *vz_out, double *sna
     sk.snap = snaps;
     sk.nsnap = snaps ? nslots : 0;
     return run_steps(h, t0, t1, sk);
+}
+
+int sparc_set_norm_output(sparc_handle *h, double *norm2_dev, int t0, int n)
+{
+    if (!h || (norm2_dev && (n < 1 || t0 < 0))) return SPARC_E_ARG;
+    h->norm2 = norm2_dev;
+    h->nnorm = norm2_dev ? n : 0;
+    h->tnorm0 = t0;
+    return SPARC_OK;
 }
 
 int sparc_checkpoint_size(const sparc_handle *h, size_t *bytes)

@@ -127,6 +127,8 @@ struct Sink {
     int orow;               // o_rad
     int adjoint;
     int k0;                 // output index held by slot 0 of vz_out / snap
+    double *norm2;          // [S][nnorm] (may be null): sum of v_z^2 over the grid after step t,
+    int nnorm, tnorm0;      //   in slot t - tnorm0 (TIMESTEPPING's nor1 squared, driver.f90:364)
 };
 
 // Slot of time index t in an output buffer of n slots starting at output

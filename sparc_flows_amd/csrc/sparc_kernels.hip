@@ -437,6 +437,34 @@ hipError_t launch_ddz_cols(const double *in, double *out, int ncols, int nz, int
     return hipGetLastError();
 }
 
+// The sum of v_z^2 over the grid after the step of clock c (TIMESTEPPING's
+// convergence statistic nor1 = sqrt(sum a(:,:,:,6)^2), driver.f90:364, which
+// it writes to rms_hist, :384): one block per source, a fixed summation order.
+__global__ __launch_bounds__(1024) void k_vz_norm2(const double *__restrict__ vz, size_t n2, Clock c,
+                                                   double *__restrict__ out, int nnorm, int t0)
+{
+    __shared__ double red[16];
+    const int b = blockIdx.x;
+    const double *v = vz + (size_t)b * n2;
+    double s = 0.0;
+    for (size_t q = threadIdx.x; q < n2; q += blockDim.x) s += v[q] * v[q];
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); w++) t += red[w];
+        const int k = clock_t_(c) - t0;
+        if (k >= 0 && k < nnorm) out[(size_t)b * nnorm + k] = t;
+    }
+}
+
+hipError_t launch_vz_norm2(const double *vz, int S, size_t n2, const Clock &c, const Sink &sk, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_vz_norm2, dim3(S), dim3(1024), 0, st, vz, n2, c, sk.norm2, sk.nnorm, sk.tnorm0);
+    return hipGetLastError();
+}
+
 hipError_t launch_ddx_rows(const double *in, double *out, int nrows, int nx, const double2 *tw,
                            const double *dmul, double stretchx, hipStream_t st)
 {

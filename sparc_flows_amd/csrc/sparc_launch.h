@@ -24,6 +24,8 @@ hipError_t launch_colsplit(int tc, int ms, const Dims &d, const Bg &bg, const Db
 hipError_t launch_finalize_vx(const Dims &d, const State &s, const Clock &c, const Sink &sk,
                               hipStream_t st);
 hipError_t launch_tick(int *tctr, int inc, hipStream_t st);
+// sk.norm2 slot of the step at clock c: the sum of v_z^2 of each source.
+hipError_t launch_vz_norm2(const double *vz, int S, size_t n2, const Clock &c, const Sink &sk, hipStream_t st);
 hipError_t launch_set_int(int *p, int v, hipStream_t st);
 hipError_t launch_fill(double *p, size_t n, double v, hipStream_t st);
 

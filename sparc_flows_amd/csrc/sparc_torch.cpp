@@ -247,6 +247,19 @@ struct Solver : torch::CustomClassHolder {
     at::Tensor ddz(at::Tensor in, int64_t ibc, at::Tensor preset) { return ddz_impl(in, ibc, preset, false); }
     at::Tensor ddz_kern(at::Tensor in, int64_t ibc) { return ddz_impl(in, ibc, at::Tensor(), true); }
 
+    // per-step sum of v_z^2 into norm2 [nsrc][n] from time t0 on (empty: off)
+    void set_norm_output(at::Tensor norm2, int64_t t0)
+    {
+        if (!norm2.numel()) {
+            check(sparc_set_norm_output(h, nullptr, 0, 0), h, "sparc_set_norm_output");
+            return;
+        }
+        check_dev(norm2, "norm2");
+        TORCH_CHECK(norm2.dim() == 2 && norm2.size(0) == nsrc, "norm2 must be [nsrc][n]");
+        check(sparc_set_norm_output(h, norm2.data_ptr<double>(), (int)t0, (int)norm2.size(1)), h,
+              "sparc_set_norm_output");
+    }
+
     // exact discrete adjoint: gvz [nsrc][nt][nx] -> gpsi, gc, gcoef ([3][nsrc][nz][nx])
     std::vector<at::Tensor> discrete_gradient(int64_t maxtime, at::Tensor gvz, int64_t interval, bool want_psi)
     {
@@ -333,5 +346,6 @@ TORCH_LIBRARY(sparc, m)
         .def("kernels", &Solver::kernels)
         .def("profile_kernels", &Solver::profile_kernels)
         .def("discrete_gradient", &Solver::discrete_gradient)
+        .def("set_norm_output", &Solver::set_norm_output)
         .def("ddz_T", &Solver::ddz_T);
 }

@@ -227,6 +227,18 @@ class Sparc:
             torch.as_tensor(preset, dtype=torch.float64, device=self.device).contiguous()
         return self._done(self._stream().ddz(f, int(ibc), pre))
 
+    def record_norms(self, n: int | None, t0: int = 0):
+        """From the next run on, record sum(v_z^2) over the grid after every
+        step t0 .. t0+n-1 (TIMESTEPPING's nor1 squared, driver.f90:364) into
+        self.norm2 [nsrc][n]; None switches it off."""
+        if n is None:
+            self._s.set_norm_output(torch.empty(0, dtype=torch.float64, device=self.device), 0)
+            self.norm2 = None
+            return None
+        self.norm2 = torch.zeros(self.nsrc, int(n), dtype=torch.float64, device=self.device)
+        self._s.set_norm_output(self.norm2, int(t0))
+        return self.norm2
+
     def ddz_T(self, f: torch.Tensor, ibc: int):
         """Transpose of ddz (ibc 1..3, kept rows 0); for ibc 3 also the
         adjoint of the kept last row [nbatch][nx]."""

@@ -72,6 +72,40 @@ def _cli(tmp: Path, data: Path, maxtime: int, *args):
 
 
 @pytest.mark.gpu
+def _check_text_outputs(p, bg, psi, d, maxtime, adjoint, vr=None):
+    """timeline, kernel_info and rms_hist of a forward or adjoint phase:
+    * timeline: time and solar time of every output step (driver.f90:386-390),
+      from -solartime (forward) or from final_time - (forcing_length - 1)
+      cadforcing_step timestep (adjoint);
+    * kernel_info: init and maxtime (forward, :249-252), the number of adjoint
+      source slices and the last forcing time (adjoint, :283-292);
+    * rms_hist: nor1 = sqrt(sum v_z^2) after every step (:364, :384), against
+      the oracle stepping the same run."""
+    import oracle
+    steps, dt, cfs = p.steps, p.timestep, p.cadforcing_step
+    tl = np.loadtxt(d / "timeline", ndmin=2)
+    outs = np.arange(0, maxtime + 1, steps)
+    assert np.array_equal(tl[:, 0].astype(int), outs)
+    start = (p.solartime * 3600.0 - (p.forcing_length(maxtime) - 1.0) * cfs * dt) if adjoint \
+        else -p.solartime * 3600.0
+    assert np.allclose(tl[:, 1], start + (outs + 1) * dt, rtol=0, atol=1e-9 * p.solartime * 3600.0)
+    ki = [int(v) for v in (d / "kernel_info").read_text().split()]
+    assert ki == ([maxtime // cfs + 1, (maxtime // cfs) * cfs] if adjoint else [0, maxtime])
+    rms = np.loadtxt(d / "rms_hist")
+    assert rms.shape == (maxtime + 1,)
+    o = oracle.Oracle(p, bg, psi=psi, adjoint=adjoint)
+    if adjoint:
+        o.set_adjoint_source(vr[:p.forcing_length(maxtime)], maxtime)      # the slices the phase reads
+    else:
+        o.set_forward_source(-12.0, maxtime)
+    ref = np.empty(maxtime + 1)
+    for t in range(maxtime + 1):
+        o.step(t)
+        ref[t] = np.sqrt((o.state()[3] ** 2).sum())
+    assert np.abs(ref).max() > 0
+    assert np.abs(rms - ref).max() <= 1e-9 * np.abs(ref).max()
+
+
 def test_forward_adjoint_kernel_phases(tmp_path):
     torch = pytest.importorskip("torch")
     if not torch.cuda.is_available():
@@ -98,6 +132,7 @@ def test_forward_adjoint_kernel_phases(tmp_path):
     assert np.linalg.norm(vz - vr0) <= 1e-12 * np.linalg.norm(vr0)
     assert np.array_equal(np.fromfile(fdir / "vz_cc.bin", dtype="<f8").reshape(vz.shape), vz)
     assert (fdir / "kernel_info").exists() and (fdir / f"xix_{0:06d}_partial.fits").exists()
+    _check_text_outputs(p, bg, psi, fdir, maxtime, adjoint=False)
     from sparc_flows_amd.adjsrc import adjoint_source_filt
     r = adjoint_source_filt(vz, vz_obs[0].cpu().numpy(), np.arange(p.nx) / p.nx, -12.0, p.xlength,
                             p.outputcad, p.steps * p.timestep / 60.0, {8: (3.0, 40.0, 10.0)})
@@ -118,6 +153,7 @@ def test_forward_adjoint_kernel_phases(tmp_path):
     (tmp_path / "Instruction_src01_ls00").write_text("adjoint\n")
     assert _cli(tmp_path, data, maxtime, "01", "00") == 0
     assert (adir / "kernel_info").exists()
+    _check_text_outputs(p, bg, psi, adir, maxtime, adjoint=True, vr=vr)
     # kernels (no instruction file)
     assert _cli(tmp_path, data, maxtime, "01", "00") == 0
     kc = read_fits(data / "kernel" / "kernel_c_01.fits")[:, 0, :]
