@@ -103,12 +103,15 @@ int sparc3d_get_snapshot(sparc3d_handle *h, double *dev);
  * false): begin zeroes the sums; add accumulates one forward snapshot (9
  * fields, device) with the adjoint snapshot (3 fields, device) it pairs with
  * (the caller pairs forward output k with adjoint output nt-1-k, :215, :256);
- * end writes Kc, Kvx, Kvy, Kvz and the hessian, [nz][ny][nx] device, in the
- * reference's units (:498-522, 563, 579) on kernel_back's background
- * (initialize.f90:937-979).  Uses the handle's 3D sound speed for c2. */
+ * end writes Kc, K_psi, Kvx, Kvy, Kvz and the hessian, [nz][ny][nx] device,
+ * in the reference's units (:498-522, 563, 579) on kernel_back's background
+ * (initialize.f90:937-979); K_psi is the y component of CURL_KERN
+ * (:815-836, bcz = 0) of (Kvx, 0, Kvz) scaled as :511-513.  Uses the
+ * handle's 3D sound speed for c2. */
 int sparc3d_kernels_begin(sparc3d_handle *h);
 int sparc3d_kernels_add(sparc3d_handle *h, const double *fwd, const double *adj);
-int sparc3d_kernels_end(sparc3d_handle *h, double *kc, double *kvx, double *kvy, double *kvz, double *hess);
+int sparc3d_kernels_end(sparc3d_handle *h, double *kc, double *kpsi, double *kvx, double *kvy, double *kvz,
+                        double *hess);
 
 /* State a [6][nz][ny][nx] (xi_x, xi_y, xi_z, v_x, v_y, v_z), device. */
 int sparc3d_get_state(sparc3d_handle *h, double *a_dev);

@@ -352,7 +352,10 @@ def kernels3d(p: Params3D, model: np.ndarray, fwd: np.ndarray, adj: np.ndarray, 
     snapshot k meets adjoint snapshot nt-1-k.  The background is kernel_back's
     (initialize.f90:937-979): rho0 = rho / 10.0**(-3.0), c2 = (c / 10.0**6.)**2.0,
     with c from c3d when given (model_c, kernels.f90:35-42).  ddzkern is dbyd2
-    (ibc 1) times stretch followed by FILTER_VAR_Z (:648-661)."""
+    (ibc 1) times stretch followed by FILTER_VAR_Z (:648-661).  The psi kernel
+    is the y component of CURL_KERN (:815-836) of (K_vx, 0, K_vz) before their
+    unit scaling, ddzkern with bcz = 0 minus ddxkern, in the units of
+    :511-513 (UNKNOWN_FACTOR 1)."""
     nx, ny, nz = p.nx, p.ny, p.nz
     m = np.asarray(model, dtype=np.float64)[:nz]
     dimc, dimrho, diml = f32(1e6), f32(1e-3), RSUN
@@ -363,8 +366,8 @@ def kernels3d(p: Params3D, model: np.ndarray, fwd: np.ndarray, adj: np.ndarray, 
     rho0 = (m[:, 2] / dimrho)[:, None, None]
     sx, sy = RSUN / p.xlength, RSUN / p.ylength
 
-    def ddzk(f):
-        d = _dbyd2(f.reshape(nz, -1), 1) * stretch[:, None]
+    def ddzk(f, ibc=1):
+        d = _dbyd2(f.reshape(nz, -1), ibc) * stretch[:, None]
         return _filter_var_z(d).reshape(f.shape)
 
     nt = fwd.shape[0]
@@ -393,6 +396,8 @@ def kernels3d(p: Params3D, model: np.ndarray, fwd: np.ndarray, adj: np.ndarray, 
     e8, e5 = f32(1e-8), f32(1e-5)
     dimen = (diml * e8) ** (-3.0) * (dimc * e5) ** (-1.0)                  # :503
     out = {"hess": H}
+    curl_y = ddzk(KV[0], 0) - _deriv_fft(KV[2], -1, sx)                    # CURL_KERN :832-834
+    out["kpsi"] = 2.0 * curl_y * dimen * 1.0 * (rho0 * c2 ** 0.5) * steps * p.timestep * dimen   # :511-513
     for i, name in enumerate(("kvx", "kvy", "kvz")):                       # :520-522
         out[name] = -2.0 * KV[i] * rho0 * steps * p.timestep * dimen
     dimen_c = (diml * e8) ** (-3.0)                                        # :563

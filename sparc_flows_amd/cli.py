@@ -432,8 +432,8 @@ def _kernels3d(q, bg, c3d, ddir, contrib, jobno) -> int:
     K = s.kernels_end()
     kdir = ddir / "kernel"
     kdir.mkdir(parents=True, exist_ok=True)
-    for key, name in (("kc", "kernel_c"), ("kvx", "kernel_vx"), ("kvy", "kernel_vy"), ("kvz", "kernel_vz"),
-                      ("hess", "hessian")):
+    for key, name in (("kc", "kernel_c"), ("kpsi", "kernel_psi"), ("kvx", "kernel_vx"), ("kvy", "kernel_vy"),
+                      ("kvz", "kernel_vz"), ("hess", "hessian")):
         write_fits(kdir / f"{name}_{contrib}.fits", K[key].cpu().numpy())
     for d in (fdir, adir):                               # kernels.f90:591-597
         for f in d.glob("*partial*"):

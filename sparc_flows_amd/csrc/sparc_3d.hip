@@ -352,15 +352,21 @@ __global__ __launch_bounds__(256) void k3_kern_accum(const double *__restrict__ 
     }
 }
 
-// Units (kernels.f90:498-522, 563, 579): K = [Kc, Kvx, Kvy, Kvz, H].
-__global__ __launch_bounds__(256) void k3_kern_final(const double *__restrict__ K, const double *__restrict__ rho0z,
+// Units (kernels.f90:498-522, 563, 579): K = [Kc, Kvx, Kvy, Kvz, H]; T1, T2 the
+// two halves of the y component of CURL_KERN (:832-834) of (Kvx, 0, Kvz):
+// ddzkern(Kvx) with bcz = 0 and ddxkern(Kvz).
+__global__ __launch_bounds__(256) void k3_kern_final(const double *__restrict__ K, const double *__restrict__ T1,
+                                                     const double *__restrict__ T2, const double *__restrict__ rho0z,
                                                      const double *__restrict__ c2, size_t N, size_t nxy,
                                                      double steps, double dt, double dimen, double dimen_c,
-                                                     double *kc, double *kvx, double *kvy, double *kvz, double *hess)
+                                                     double *kc, double *kpsi, double *kvx, double *kvy, double *kvz,
+                                                     double *hess)
 {
     const size_t stride = (size_t)gridDim.x * 256;
     for (size_t q = (size_t)blockIdx.x * 256 + threadIdx.x; q < N; q += stride) {
         const double r = rho0z[q / nxy];
+        const double kp = T1[q] - T2[q];
+        kpsi[q] = 2.0 * kp * dimen * 1.0 * (r * sqrt(c2[q])) * steps * dt * dimen;   // :511-513
         kvx[q] = -2.0 * K[q + N] * r * steps * dt * dimen;
         kvy[q] = -2.0 * K[q + 2 * N] * r * steps * dt * dimen;
         kvz[q] = -2.0 * K[q + 3 * N] * r * steps * dt * dimen;
@@ -1172,18 +1178,25 @@ int sparc3d_kernels_add(sparc3d_handle *h, const double *fwd, const double *adj)
     return SPARC_OK;
 }
 
-int sparc3d_kernels_end(sparc3d_handle *h, double *kc, double *kvx, double *kvy, double *kvz, double *hess)
+int sparc3d_kernels_end(sparc3d_handle *h, double *kc, double *kpsi, double *kvx, double *kvy, double *kvz,
+                        double *hess)
 {
-    if (!h || !kc || !kvx || !kvy || !kvz || !hess) return SPARC_E_ARG;
+    if (!h || !kc || !kpsi || !kvx || !kvy || !kvz || !hess) return SPARC_E_ARG;
     if (!h->K) return fail3(h, SPARC_E_STATE, "sparc3d_kernels_end before sparc3d_kernels_begin");
     const double diml = kRsun3, dimc = (double)1e6f, e8 = (double)1e-8f, e5 = (double)1e-5f;
     const double dimen = std::pow(diml * e8, -3.0) * std::pow(dimc * e5, -1.0);   // kernels.f90:503
     const double dimen_c = std::pow(diml * e8, -3.0);                             // :563
     const int stepskern = (int)std::floor(h->outputcad / h->timestep);
     const size_t N = h->n;
+    // the psi kernel's curl (CURL_KERN :832-834, bcz = 0) into the derivative scratch
+    double *T1 = h->KD, *T2 = h->KD + N;
+    HCHECK3(h, launch_ddz_cols(h->K + N, T1, (int)h->nxy, h->nz, 1, 0, h->d_stretch, h->cf, h->tri[0], h->W, 1,
+                               h->stream));
+    int r;
+    if ((r = ddx3(h, h->K + 3 * N, T2, h->ny * h->nz))) return r;
     const unsigned blocks = (unsigned)std::min<size_t>((N + 255) / 256, 16384);
-    hipLaunchKernelGGL(k3_kern_final, dim3(blocks), dim3(256), 0, h->stream, h->K, h->krho0, h->kc2, N, h->nxy,
-                       (double)stepskern, h->timestep, dimen, dimen_c, kc, kvx, kvy, kvz, hess);
+    hipLaunchKernelGGL(k3_kern_final, dim3(blocks), dim3(256), 0, h->stream, h->K, T1, T2, h->krho0, h->kc2, N,
+                       h->nxy, (double)stepskern, h->timestep, dimen, dimen_c, kc, kpsi, kvx, kvy, kvz, hess);
     HCHECK3(h, hipGetLastError());
     HCHECK3(h, hipStreamSynchronize(h->stream));
     return SPARC_OK;

@@ -94,7 +94,7 @@ def lib() -> ctypes.CDLL:
             "sparc3d_get_snapshot": [vp, vp],
             "sparc3d_kernels_begin": [vp],
             "sparc3d_kernels_add": [vp, vp, vp],
-            "sparc3d_kernels_end": [vp, vp, vp, vp, vp, vp],
+            "sparc3d_kernels_end": [vp, vp, vp, vp, vp, vp, vp],
             "sparc3d_get_state": [vp, vp],
             "sparc3d_set_state": [vp, vp],
             "sparc3d_get_pml": [vp, vp],
@@ -267,8 +267,9 @@ class Sparc3D:
         self._call("sparc3d_kernels_add", _ptr(fwd_snap.contiguous()), _ptr(adj_snap.contiguous()))
 
     def kernels_end(self) -> dict:
-        out = {k: self._zeros(self.p.nz, self.p.ny, self.p.nx) for k in ("kc", "kvx", "kvy", "kvz", "hess")}
-        self._call("sparc3d_kernels_end", *[_ptr(out[k]) for k in ("kc", "kvx", "kvy", "kvz", "hess")])
+        keys = ("kc", "kpsi", "kvx", "kvy", "kvz", "hess")
+        out = {k: self._zeros(self.p.nz, self.p.ny, self.p.nx) for k in keys}
+        self._call("sparc3d_kernels_end", *[_ptr(out[k]) for k in keys])
         return out
 
     def kernels(self, fwd_snaps: torch.Tensor, adj_snaps: torch.Tensor) -> dict:

@@ -325,8 +325,8 @@ def test_forward_adjoint_kernel_phases_3d(tmp_path):
             fs.append(s.snapshot())
             asn.append(sa.snapshot())
     K = sa.kernels(torch.stack(fs), torch.stack(asn))
-    for key, name in (("kc", "kernel_c"), ("kvx", "kernel_vx"), ("kvy", "kernel_vy"), ("kvz", "kernel_vz"),
-                      ("hess", "hessian")):
+    for key, name in (("kc", "kernel_c"), ("kpsi", "kernel_psi"), ("kvx", "kernel_vx"), ("kvy", "kernel_vy"),
+                      ("kvz", "kernel_vz"), ("hess", "hessian")):
         got = read_fits(data / "kernel" / f"{name}_01.fits")
         ref = K[key].cpu().numpy()
         assert np.abs(ref).max() > 0, key

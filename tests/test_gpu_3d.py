@@ -158,7 +158,7 @@ def test_kernels_match_oracle():
     a = rng.standard_normal((nt, 3, p.nz, p.ny, p.nx))
     K = s.kernels(torch.as_tensor(f, device="cuda"), torch.as_tensor(a, device="cuda"))
     ref = kernels3d(OParams(**dataclasses.asdict(p)), bg, f, a, c3d=c3d)
-    for k in ("kc", "kvx", "kvy", "kvz", "hess"):
+    for k in ("kc", "kpsi", "kvx", "kvy", "kvz", "hess"):
         e = rel_l2(K[k].cpu().numpy(), ref[k])
         print("3D kernel", k, e)
         assert e < 1e-11, k
@@ -204,7 +204,7 @@ def test_checkpointed_gradient_matches_oracle():
     K = g.run(5.0, -3.0, maxtime)
     ref, ovz = _oracle_gradient(p, bg, c3d, fl, 5.0, -3.0, maxtime, window)
     assert rel_l2(K["vz"].cpu().numpy(), ovz) < 1e-10
-    for k in ("kc", "kvx", "kvy", "kvz", "hess"):
+    for k in ("kc", "kpsi", "kvx", "kvy", "kvz", "hess"):
         e = rel_l2(K[k].cpu().numpy(), ref[k])
         print("3D gradient", k, e)
         assert np.abs(ref[k]).max() > 0 and e < 1e-9, k
@@ -215,7 +215,7 @@ def test_checkpointed_gradient_matches_oracle():
         K1 = Gradient3D(p, bg, c3d=c3d, flow=fl, interval=interval, window_s=window).run(5.0, -3.0, maxtime)
         if interval == "auto":
             assert K1["interval"] == maxtime + 1
-        for k in ("kc", "kvx", "kvy", "kvz", "hess"):
+        for k in ("kc", "kpsi", "kvx", "kvy", "kvz", "hess"):
             assert torch.equal(K1[k], K[k]), (interval, k)
 
 

@@ -184,6 +184,9 @@ def test_kernels3d_reduce_to_the_2d_kernels():
         assert np.allclose(k3["kvx"][:, j] * d2 / d3, k2["kvx"], rtol=1e-11, atol=1e-13 * np.abs(k2["kvx"]).max())
         assert np.allclose(k3["kvz"][:, j] * d2 / d3, k2["kvz"], rtol=1e-11, atol=1e-13 * np.abs(k2["kvz"]).max())
         assert np.allclose(k3["kc"][:, j] * dc2 / dc3, k2["kc"], rtol=1e-11, atol=1e-13 * np.abs(k2["kc"]).max())
+        # K_psi carries dimen twice (kernels.f90:511-513)
+        assert np.allclose(k3["kpsi"][:, j] * (d2 / d3) ** 2, k2["kpsi"], rtol=1e-11,
+                           atol=1e-13 * np.abs(k2["kpsi"]).max())
     assert np.all(k3["kvy"] == 0)
 
 
