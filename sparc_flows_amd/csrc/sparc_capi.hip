@@ -1722,6 +1722,21 @@ int sparc_discrete_gradient(sparc_handle *h, int maxtime, const double *gvz_dev,
         hipError_t e = ar.alloc(b.p, b.n);
         if (e != hipSuccess) { h->err = std::string("discrete adjoint work: ") + hipGetErrorString(e); return done(SPARC_E_HIP); }
     }
+    {
+        // the stage quantities of one segment, kept by its recompute so that
+        // the transposed steps need not run their forward step again, when
+        // they take at most half of the free memory (SPARC_DISCRETE_KEEP_STAGES=0
+        // recomputes them in every transposed step instead)
+        const size_t need = (size_t)C * 25 * w.SN * sizeof(double);
+        const char *ke = std::getenv("SPARC_DISCRETE_KEEP_STAGES");
+        const bool keep = !(ke && std::atoi(ke) == 0);
+        size_t fr = 0, tot = 0;
+        if (keep && hipMemGetInfo(&fr, &tot) == hipSuccess && need <= fr / 2 &&
+            ar.alloc(&w.Dseg, (size_t)C * 25 * w.SN) != hipSuccess) {
+            (void)hipGetLastError();
+            w.Dseg = nullptr;
+        }
+    }
     double *G = gcoef_dev;
     if (!G) {
         hipError_t e = ar.alloc(&G, 3 * w.SN);

@@ -151,7 +151,12 @@ __global__ __launch_bounds__(384) void k_colsplit(Dims d, Bg bg, DbyCoef cf, Seg
         return jc < nb ? jc : min(nzpml - 1, nb + max(jc - (nz - ntop), 0));
     };
     auto is_pml = [&](int j) -> bool { return j < nb || (j >= nz - ntop && j < nz); };
-    const bool segpml = a < nb || a + MS > nz - ntop;
+    // The epilogues take their z-PML form for the whole wave when any lane's
+    // segment meets a PML layer: that form leaves rows outside the layers as
+    // the plain form does (the PML terms are selected off and the auxiliary
+    // stores masked), and a per-lane choice would run both forms one after
+    // the other in every wave that holds a layer.
+    const bool segpml = __ballot(a < nb || a + MS > nz - ntop) != 0;
 
     // --- first derivatives: group 0 d xi_z/dz (ibc 3, top row preset),
     //     group 1 d v_z/dz (ibc 1), group 2 d v_x/dz (ibc 1)

@@ -469,14 +469,20 @@ static hipError_t fwd_step(const DiscreteCtx &c, const DiscreteWork &w, const do
 }
 
 // Transpose of one step at time t: lam (adjoint of the state after the step)
-// -> adjoint of U_t, accumulating the gradient into G.
+// -> adjoint of U_t, accumulating the gradient into G.  Dkept: the step's
+// stage quantities [5][5][SN] kept by the segment recompute, or null to
+// recompute them from U here.
 static hipError_t bwd_step(const DiscreteCtx &c, const DiscreteWork &w, const double *U, int t, double *lam,
-                           double *G)
+                           double *G, const double *Dkept)
 {
     const Dims &d = c.d;
     const Views v{w.SN, w.PZ1, w.PX1};
     const int S = d.S;
-    DCHK(fwd_step(c, w, U, t, w.Unext, w.Dst));              // the stage quantities of this step
+    const double *Dst = Dkept;
+    if (!Dst) {
+        DCHK(fwd_step(c, w, U, t, w.Unext, w.Dst));          // the stage quantities of this step
+        Dst = w.Dst;
+    }
     hipLaunchKernelGGL(k_filter_T, grid1(w.NST), dim3(256), 0, c.stream, d, v, lam, w.acc);
     DCHK(hipGetLastError());
     hipLaunchKernelGGL(k_axpy, grid1(w.NST), dim3(256), 0, c.stream, w.acc, w.acc, c.deltat - 1.0, w.Rb, w.NST);
@@ -489,7 +495,7 @@ static hipError_t bwd_step(const DiscreteCtx &c, const DiscreteWork &w, const do
         DCHK(dz_dense(c, w.FZ, w.DFZ, 2 * S, c.MT[0], nullptr, nullptr, 0, 0.0));
         DCHK(dz_dense(c, w.GZ, w.DGZ, S, c.MT[1], nullptr, nullptr, 0, 0.0));
         hipLaunchKernelGGL(k_adj_b, grid1(w.SN), dim3(256), 0, c.stream, d, c.bg, v, w.Rb, w.DFX, w.DFZ, w.DGZ,
-                           w.RHOB, w.Dst + (size_t)s * 5 * w.SN, w.Yb, w.XZP, w.DXB, w.DZB, G);
+                           w.RHOB, Dst + (size_t)s * 5 * w.SN, w.Yb, w.XZP, w.DXB, w.DZB, G);
         DCHK(hipGetLastError());
         hipLaunchKernelGGL(k_colsum_w, grid1((size_t)S * d.nx), dim3(256), 0, c.stream, w.DZB, c.vN3, w.PB, d.nz,
                            d.nx, S);
@@ -531,15 +537,21 @@ hipError_t discrete_gradient(const DiscreteCtx &c, const DiscreteWork &w, int ma
         // states U_t of the segment
         DCHK(hipMemcpyAsync(w.seg, w.ck + (size_t)j * w.NST, w.NST * sizeof(double), hipMemcpyDeviceToDevice,
                             c.stream));
-        for (int t = t0; t < t1; t++)
-            DCHK(fwd_step(c, w, w.seg + (size_t)(t - t0) * w.NST, t, w.seg + (size_t)(t - t0 + 1) * w.NST, nullptr));
+        // states U_t0..U_t1 and, when w.Dseg holds a segment, the stage
+        // quantities of steps t0..t1 (the state after step t1 is not needed)
+        const size_t D1 = (size_t)25 * w.SN;
+        for (int t = t0; t < t1 + (w.Dseg ? 1 : 0); t++)
+            DCHK(fwd_step(c, w, w.seg + (size_t)(t - t0) * w.NST, t,
+                          t < t1 ? w.seg + (size_t)(t - t0 + 1) * w.NST : w.Unext,
+                          w.Dseg ? w.Dseg + (size_t)(t - t0) * D1 : nullptr));
         for (int t = t1; t >= t0; t--) {
             if (t % c.steps == 0) {                            // output k after step t
                 hipLaunchKernelGGL(k_inject, grid1((size_t)S * d.nx), dim3(256), 0, c.stream, w.lam, gvz,
                                    t / c.steps, nt, c.orow, d.nz, d.nx, S, w.SN);
                 DCHK(hipGetLastError());
             }
-            DCHK(bwd_step(c, w, w.seg + (size_t)(t - t0) * w.NST, t, w.lam, G));
+            DCHK(bwd_step(c, w, w.seg + (size_t)(t - t0) * w.NST, t, w.lam, G,
+                          w.Dseg ? w.Dseg + (size_t)(t - t0) * D1 : nullptr));
         }
     }
     return hipSuccess;

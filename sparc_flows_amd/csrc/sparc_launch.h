@@ -57,6 +57,7 @@ struct DiscreteWork {
     double *U, *Unext, *Y, *R, *acc, *Rb, *Yb, *lam;          // NST each
     double *ck, *seg;                                         // segment starts, one segment of states
     double *Dst, *Dscr;                                       // [5][5][SN], [5][SN]
+    double *Dseg;                                             // [interval][5][5][SN] or null
     double *T0, *T1, *T2, *T3, *GZ, *DGZ, *RHOB, *XZP, *DXB, *DZB, *DZ3B, *DDXB;   // SN each
     double *FX, *DFX, *FZ, *DFZ;                              // 3, 3, 2, 2 SN
     double *PB;                                               // [S][nx]

@@ -79,10 +79,12 @@ def test_discrete_gradient_matches_oracle(flow, dc):
 
 
 @pytest.mark.parametrize("interval", [1, 37, 120])
-def test_checkpointed_trajectory_equals_kept_states(interval):
+def test_checkpointed_trajectory_equals_kept_states(interval, monkeypatch):
     """Recomputing the trajectory from checkpoints every `interval` steps
     gives the gradient of the run that keeps every state, bit for bit (the
-    same kernels in the same order)."""
+    same kernels in the same order), and so does recomputing each step's
+    stage quantities in its transposed step instead of keeping them from the
+    segment's recompute."""
     p, bg, c2d, psi = small_case(nx=64, nz=60)
     maxtime = 300
     s = Sparc(p, bg, nsrc=1, c2d=c2d, psi=psi)
@@ -90,9 +92,11 @@ def test_checkpointed_trajectory_equals_kept_states(interval):
     gvz = _dev(np.random.default_rng(7).standard_normal((1, p.nt_out(maxtime), p.nx)))
     a = s.discrete_gradient(gvz, interval=0)
     b = s.discrete_gradient(gvz, interval=interval)
-    for x, y in zip(a[:2], b[:2]):
-        assert torch.equal(x, y)
-    assert torch.equal(a[2], b[2])
+    monkeypatch.setenv("SPARC_DISCRETE_KEEP_STAGES", "0")
+    c = s.discrete_gradient(gvz, interval=interval)
+    for x, y, z in zip(a[:2], b[:2], c[:2]):
+        assert torch.equal(x, y) and torch.equal(x, z)
+    assert torch.equal(a[2], b[2]) and torch.equal(a[2], c[2])
 
 
 def test_batch_equals_single_sources():
