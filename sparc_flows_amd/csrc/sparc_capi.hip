@@ -624,9 +624,17 @@ static int build_seg(sparc_handle *h, int m, SegTab &sg)
     HCHECK(h, upload(h->bg_mem, &p, band, h->stream));
     sg.Minv = p;
     std::vector<double> stage(band);
-    for (int r = 0; r < kZL; r++)
-        stage.insert(stage.end(), ztab.begin() + (size_t)stage_zrow(r) * nzp,
-                     ztab.begin() + (size_t)(stage_zrow(r) + 1) * nzp);
+    {
+        // the segment tables of the first, an interior and the last segment,
+        // then the profiles
+        std::vector<double> kt;
+        if (!seg_kind_tables(t, kt)) return fail(h, SPARC_E_ARG, "interior segment tables differ");
+        static_assert(kZS == 15, "segment-table rows");
+        stage.insert(stage.end(), kt.begin(), kt.end());
+        for (int r = 0; r < kZF; r++)
+            stage.insert(stage.end(), ztab.begin() + (size_t)stage_prow(r) * nzp,
+                         ztab.begin() + (size_t)(stage_prow(r) + 1) * nzp);
+    }
     HCHECK(h, upload(h->bg_mem, &p, stage, h->stream));
     sg.stage = p;
     return SPARC_OK;

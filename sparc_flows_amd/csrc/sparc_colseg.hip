@@ -29,29 +29,32 @@ __device__ __forceinline__ int zrow(int slot, int k) { return 2 + 5 * slot + k; 
 
 }  // namespace
 
-// z tables staged in LDS, kZL rows of nzp (stage_zrow): the segment tables
-// of the three dbyd2 table sets (ibc 1, 2, 3; rows 5 slot + k, k: lowr, rbl,
-// gl, V, W), then the six epilogue profiles.
-enum LRow { L_STR = 15, L_G, L_RHO, L_GRHO, L_GP, L_RINV };
-static_assert(L_RINV + 1 == kZL, "staged z-table rows");
+// z tables staged in LDS: the segment tables of the three dbyd2 table sets
+// (ibc 1, 2, 3; rows 5 slot + k, k: lowr, rbl, gl, V, W) of the first, an
+// interior and the last segment, [kZS][3][MS], then the six epilogue
+// profiles [kZF][nzp] in this order (stage_prow).
+enum PfRow { F_STR = 0, F_G, F_RHO, F_GRHO, F_GP, F_RINV };
+static_assert(F_RINV + 1 == kZF, "staged z profiles");
 // The block's z-PML data staged in LDS, [kPL][nzpml][TC]: az, bz, fz and the
 // two auxiliary fields (psivz, psip), stage value and base.
 enum PRow { P_AZ = 0, P_BZ, P_FZ, P_T0, P_B0, P_T1, P_B1, kPL };
 
 // Dynamic LDS of one block, in this order: the staged region (the three
-// banded interface inverses [3][2P][4hw+4], the z tables [kZL][nzp], the
-// z-PML data [kPL][nzpml][TC]; filled in one pass at the start), the four
+// banded interface inverses [3][2P][4hw+4], the segment tables [kZS][3][ms],
+// the z profiles [kZF][nzp], the z-PML data [kPL][nzpml][TC], the mode-3 source z profile [nzp] (zero on
+// padding rows; only staged in that mode); filled in one pass at the start), the four
 // end-value exchange buffers [4][TC][2P + 4hw + 2] (each column's 2P end
 // values between 2hw and 2hw+2 zeros), and three [TC][nzp] row buffers: the
 // group-1 and group-0 parts of RHS v_z and p.
-inline size_t colsplit_staged(int tc, int P, int hw, int nzp, int nzpml)
+inline size_t colsplit_staged(int tc, int P, int hw, int nzp, int nzpml, int ms)
 {
-    return (size_t)3 * 2 * P * (4 * hw + 4) + (size_t)kZL * nzp + (size_t)kPL * nzpml * tc;
+    return (size_t)3 * 2 * P * (4 * hw + 4) + (size_t)kZS * 3 * ms + (size_t)kZF * nzp + (size_t)kPL * nzpml * tc +
+           (size_t)nzp;
 }
-inline size_t colsplit_lds_bytes(int tc, int P, int hw, int nzp, int nzpml)
+inline size_t colsplit_lds_bytes(int tc, int P, int hw, int nzp, int nzpml, int ms)
 {
     return sizeof(double) *
-           (colsplit_staged(tc, P, hw, nzp, nzpml) + (size_t)4 * tc * yb_len(P, hw) + (size_t)3 * tc * nzp);
+           (colsplit_staged(tc, P, hw, nzp, nzpml, ms) + (size_t)4 * tc * yb_len(P, hw) + (size_t)3 * tc * nzp);
 }
 
 // ---------------------------------------------------------------------------
@@ -101,9 +104,11 @@ __global__ __launch_bounds__(384) void k_colsplit(Dims d, Bg bg, DbyCoef cf, Seg
     const int nband = n2P * (4 * sg.hw + 4);
     const int nzpml = d.nzpml;
     double *const mv = lds_dyn;                        // [3][nband] banded interface inverses
-    double *const zl = mv + 3 * nband;                 // [kZL][nzp] z tables
-    double *const pl = zl + kZL * nzp;                 // [kPL][nzpml][TC] z-PML data
-    const int nstaged = 3 * nband + kZL * nzp + kPL * nzpml * TC;
+    double *const zl = mv + 3 * nband;                 // [kZS][3][MS] segment tables
+    double *const zf = zl + kZS * 3 * MS;              // [kZF][nzp] z profiles
+    double *const pl = zf + kZF * nzp;                 // [kPL][nzpml][TC] z-PML data
+    double *const zpl = pl + kPL * nzpml * TC;         // [nzp] mode-3 source z profile
+    const int nstaged = 3 * nband + kZS * 3 * MS + kZF * nzp + kPL * nzpml * TC + nzp;
     const int LY = n2P + 4 * sg.hw + 2, hw2 = 2 * sg.hw;
     double *const yb0 = lds_dyn + nstaged;             // [4][TC][LY] end values: field groups 0-2, d p/dz
     double *const wl = yb0 + 4 * TC * LY;              // [TC][nzp] group-1 part of RHS v_z,
@@ -140,7 +145,10 @@ __global__ __launch_bounds__(384) void k_colsplit(Dims d, Bg bg, DbyCoef cf, Seg
     const int nb = d.npmlbot, ntop = d.npmltop;
     const int pz0b = (b * nzpml * nx + x) * 8;
     const int pz1b = ((d.S + b) * nzpml * nx + x) * 8;
-    const double *zs = zl + a;                         // row a of the staged z tables
+    const double *zs = zf + a;                         // row a of the staged z profiles
+    // this segment's tables: those of the first, an interior or the last
+    // segment (every interior segment's are the same), row r at zt[r * 3 MS + i]
+    const double *zt = zl + seg_kind(sc, P) * MS;
     auto plv = [&](int k, int pj) -> double { return pl[(k * nzpml + pj) * TC + c]; };
 
     // byte offset of row r of this column, r clamped to the column
@@ -165,6 +173,7 @@ __global__ __launch_bounds__(384) void k_colsplit(Dims d, Bg bg, DbyCoef cf, Seg
 #pragma unroll
     for (int k = 0; k < MS + 6; k++) win[k] = ldb(F, colb + rowc(a - 3 + k));
     const double topv = f == 0 ? ldb(st.topbc, (b * nx + x) * 8) : 0.0;
+    const double r0src = f == 0 ? ldb(bg.rho0_src, gcolb) : 1.0;   // the source lanes' rho0 at the source row
     double e[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
     if (lastseg) {
 #pragma unroll
@@ -184,12 +193,14 @@ __global__ __launch_bounds__(384) void k_colsplit(Dims d, Bg bg, DbyCoef cf, Seg
         // thread), the z-PML data come from their own arrays (two elements per
         // thread and kind); every load of a round is issued before its stores
         constexpr int kStage = 32;
-        const int nA = 3 * nband + kZL * nzp, npl = nzpml * TC, stride = blockDim.x;
+        const int nA = 3 * nband + kZS * 3 * MS + kZF * nzp, npl = nzpml * TC, stride = blockDim.x;
         const int x0 = bx * TC;
-        const int rounds = max((nA + kStage * stride - 1) / (kStage * stride), (npl + 2 * stride - 1) / (2 * stride));
+        const int rounds = max((nA + kStage * stride - 1) / (kStage * stride),
+                               (max(npl, nzp) + 2 * stride - 1) / (2 * stride));
+        const bool zsrc = src.mode == 3;
         const double *const stg = wave_ptr(sg.stage);
         for (int r0 = 0; r0 < rounds; r0++) {
-            double va[kStage], vb[kPL][2];
+            double va[kStage], vb[kPL][2], vz[2] = {0.0, 0.0};
             const int qa0 = r0 * kStage * stride + tid, qb0 = r0 * 2 * stride + tid;
 #pragma unroll
             for (int u = 0; u < kStage; u++) va[u] = ldb(stg, (unsigned)min(qa0 + u * stride, nA - 1) * 8u);
@@ -208,6 +219,10 @@ __global__ __launch_bounds__(384) void k_colsplit(Dims d, Bg bg, DbyCoef cf, Seg
                 vb[P_T1][h] = ldb(st.pz_tmp, s1);
                 vb[P_B1][h] = ldb(st.pz_base, s1);
             }
+            if (zsrc) {
+#pragma unroll
+                for (int h = 0; h < 2; h++) vz[h] = ldb(src.zprof, (unsigned)min(qb0 + h * stride, nz - 1) * 8u);
+            }
 #pragma unroll
             for (int u = 0; u < kStage; u++)
                 if (qa0 + u * stride < nA) lds_dyn[qa0 + u * stride] = va[u];
@@ -218,6 +233,7 @@ __global__ __launch_bounds__(384) void k_colsplit(Dims d, Bg bg, DbyCoef cf, Seg
 #pragma unroll
                     for (int k = 0; k < kPL; k++) pl[k * npl + q] = vb[k][h];
                 }
+                if (zsrc && q < nzp) zpl[q] = q < nz ? vz[h] : 0.0;
             }
         }
     }
@@ -228,9 +244,9 @@ __global__ __launch_bounds__(384) void k_colsplit(Dims d, Bg bg, DbyCoef cf, Seg
     if (f == 0 && act && isrc >= 0 && isrc < MS && time <= src.limit) {
         if (src.mode == 1) {
             const double fs = (time >= 0 && time < src.nfwd) ? src.fwd[time] : 0.0;
-            srcv = src.vr[b * nx + x] * fs * src.delta_width / bg.rho0_src[x];
+            srcv = src.vr[b * nx + x] * fs * src.delta_width / r0src;
         } else if (src.mode == 2) {
-            srcv = lagrange_forcing(src, b, x, nx, time) * src.delta_width / bg.rho0_src[x];
+            srcv = lagrange_forcing(src, b, x, nx, time) * src.delta_width / r0src;
         }
     }
     const double pamp = (f == 0 && act) ? profile_amp(src, b, x, nx, a, MS, time) : 0.0;
@@ -241,8 +257,8 @@ __global__ __launch_bounds__(384) void k_colsplit(Dims d, Bg bg, DbyCoef cf, Seg
                 cf.c2, cbt);
 #pragma unroll
     for (int i = 0; i < MS; i++) own[i] = win[3 + i];
-    const double *const zt_f = zs + 5 * slot_f * nzp;   // this group's table set
-    auto gtab = [&](int t, int i) { return zt_f[t * nzp + i]; };
+    const double *const zt_f = zt + 5 * slot_f * 3 * MS;   // this group's table set
+    auto gtab = [&](int t, int i) { return zt_f[t * 3 * MS + i]; };
     seg_solve<MS>(Y, gtab);
     if (act) { ybf[c * LY + hw2 + 2 * s] = Y[0]; ybf[c * LY + hw2 + 2 * s + 1] = Y[MS - 1]; }
     // the epilogue's inputs that no lane of this kernel writes, loaded
@@ -281,8 +297,8 @@ __global__ __launch_bounds__(384) void k_colsplit(Dims d, Bg bg, DbyCoef cf, Seg
                     lfz = plv(P_FZ, pj);
                 }
                 const int j = a + i;
-                const double str = zs[L_STR * nzp + i], r0 = zs[L_RHO * nzp + i];
-                const double rinv = zs[L_RINV * nzp + i], gz = zs[L_G * nzp + i];
+                const double str = zs[F_STR * nzp + i], r0 = zs[F_RHO * nzp + i];
+                const double rinv = zs[F_RINV * nzp + i], gz = zs[F_G * nzp + i];
                 double dz = Y[i] * str;
                 double fdec = 1.0;
                 if (PML) {
@@ -297,7 +313,7 @@ __global__ __launch_bounds__(384) void k_colsplit(Dims d, Bg bg, DbyCoef cf, Seg
                         if (LAST) stb(st.pz_base, qp, pnew);
                     }
                 }
-                const double gp0 = zs[L_GP * nzp + i] * fdec, gr0 = zs[L_GRHO * nzp + i] * fdec;
+                const double gp0 = zs[F_GP * nzp + i] * fdec, gr0 = zs[F_GRHO * nzp + i] * fdec;
                 const double xiz = own[i];
                 const double div = pa[i] + dz;
                 const double pv = -(pb[i] * div) - gp0 * xiz;
@@ -307,7 +323,7 @@ __global__ __launch_bounds__(384) void k_colsplit(Dims d, Bg bg, DbyCoef cf, Seg
                 }
                 const double rho = -(r0 * div) - gr0 * xiz;
                 double bzv = -rinv * (rho * gz);
-                if (src.mode == 3) bzv = bzv + pamp * src.zprof[min(j, nz - 1)];
+                if (src.mode == 3) bzv = bzv + pamp * zpl[j];
                 if (act) w0[i] = (i == isrc) ? bzv + srcv : bzv;
             }
         };
@@ -333,7 +349,7 @@ __global__ __launch_bounds__(384) void k_colsplit(Dims d, Bg bg, DbyCoef cf, Seg
             const int j = a + i;
             const unsigned q = colb + j * nxb;
             const bool zero = j == 0 || x == 0 || x == nx - 1;
-            const double dzf = Y[i] * zs[L_STR * nzp + i];
+            const double dzf = Y[i] * zs[F_STR * nzp + i];
             const bool v = valid(i);
             if (f == 1) {
                 if (act) wl[c * nzp + a + i] = -(2.0 * l2[i] * l1[i]) - 2.0 * l3[i] * dzf;
@@ -362,13 +378,13 @@ __global__ __launch_bounds__(384) void k_colsplit(Dims d, Bg bg, DbyCoef cf, Seg
             for (int k = 0; k < 5; k++) ep[k] = pcol[nz - 5 + k];
         }
         seg_rhs<MS>([&](int k) { return wp[k]; }, ep, Y, a, nz, first, lastseg, 2, 0.0, 0.0, cf.c1, cf.c2, cbt);
-        auto ltab = [&](int t, int i) { return zs[(5 + t) * nzp + i]; };   // ibc 2 set
+        auto ltab = [&](int t, int i) { return zt[(5 + t) * 3 * MS + i]; };   // ibc 2 set
         seg_solve<MS>(Y, ltab);
         if (act) { yb3[c * LY + hw2 + 2 * s] = Y[0]; yb3[c * LY + hw2 + 2 * s + 1] = Y[MS - 1]; }
     }
     __syncthreads();
     if (f == 0) {
-        auto ltab = [&](int t, int i) { return zs[(5 + t) * nzp + i]; };   // ibc 2 set
+        auto ltab = [&](int t, int i) { return zt[(5 + t) * 3 * MS + i]; };   // ibc 2 set
         if (act) seg_correct<MS>(Y, s, P, sg.hw, yb3 + c * LY, mv + nband, ltab);
         const int sslot = (LAST && sink.snap) ? out_slot(time, clk.steps, sink.k0, sink.nsnap) : -1;
         const bool snap = sslot >= 0;
@@ -386,7 +402,7 @@ __global__ __launch_bounds__(384) void k_colsplit(Dims d, Bg bg, DbyCoef cf, Seg
                     lbz = plv(P_BZ, pj);
                 }
                 const int j = a + i;
-                double gpz = Y[i] * zs[L_STR * nzp + i];
+                double gpz = Y[i] * zs[F_STR * nzp + i];
                 if (PML) {
                     const bool pm = is_pml(j);
                     const double prhs = laz * gpz + lbz * lpsi;
@@ -398,7 +414,7 @@ __global__ __launch_bounds__(384) void k_colsplit(Dims d, Bg bg, DbyCoef cf, Seg
                         if (LAST) stb(st.pz_base, qp, pnew);
                     }
                 }
-                double rvz = -zs[L_RINV * nzp + i] * gpz + (w0[i] + wl[c * nzp + a + i]);
+                double rvz = -zs[F_RINV * nzp + i] * gpz + (w0[i] + wl[c * nzp + a + i]);
                 rvz = (j == 0 || x == 0 || x == nx - 1) ? 0.0 : rvz;
                 if (valid(i)) {
                     stb(ca.TN[3], colb + j * nxb, pc[i] + ca.bnext * rvz);
@@ -489,7 +505,7 @@ hipError_t launch_colsplit(int tc, int ms, const Dims &d, const Bg &bg, const Db
         sg.P > kSegMax || 3 * G > 384)
         return hipErrorInvalidValue;
     dim3 grid(d.nx / tc, d.S), block(3 * G);
-    const size_t shm = colsplit_lds_bytes(tc, sg.P, sg.hw, sg.nzp, d.nzpml);
+    const size_t shm = colsplit_lds_bytes(tc, sg.P, sg.hw, sg.nzp, d.nzpml, ms);
     if (shm > (size_t)g_lds_max) return hipErrorInvalidValue;
 #define L(MSV)                                                                                          \
     if (ms == MSV) {                                                                                    \

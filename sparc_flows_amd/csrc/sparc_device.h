@@ -58,16 +58,20 @@ struct Tri {
 // then the x-independent profiles rho0, d rho0/dz, d p0/dz, 1/rho0 (17-20).
 constexpr int kSegMax = 32;
 constexpr int kZT = 21;     // then the kCB boundary-row coefficients (sparc_seg.h)
-// The column pass keeps kZL z-table rows in LDS, in this order: the segment
-// tables of the three dbyd2 sets (ztab rows 2-16), then the six epilogue
-// profiles stretch, g, rho0, d rho0/dz, d p0/dz, 1/rho0 (ztab rows 0, 1, 17-20).
-constexpr int kZL = 21;
-__host__ __device__ inline int stage_zrow(int r) { return r < 15 ? 2 + r : (r == 15 ? 0 : (r == 16 ? 1 : r)); }
+// The column pass keeps in LDS the kZS segment-table rows of the three dbyd2
+// sets (ztab rows 2-16) for three segments only, [kZS][3][m]: the first, an
+// interior one and the last.  Every interior segment's tables are the same
+// (the coefficients are constant away from the boundary rows and each
+// segment's factorization restarts at its start; the host checks it).  Then
+// the kZF epilogue profiles stretch, g, rho0, d rho0/dz, d p0/dz, 1/rho0,
+// [kZF][nzp] (ztab rows stage_prow(r)).
+constexpr int kZS = 15, kZF = 6;
+__host__ __device__ inline int stage_prow(int r) { return r == 0 ? 0 : (r == 1 ? 1 : 15 + r); }
 struct SegTab {
     int P, m, hw, nzp;              // segments, rows per segment, Minv window, P*m
     const double *ztab;             // [kZT][nzp], then [kCB]
     const double *Minv;             // [3][2P][4hw+4]: row k of each inverse, entries 2(k/2-hw)+w
-    const double *stage;            // the column pass's LDS image: Minv, then [kZL][nzp] ztab rows by stage_zrow
+    const double *stage;            // the column pass's LDS image: Minv, [kZS][3][m] segment tables, [kZF][nzp] profiles
 };
 
 // Prognostic state and work arrays of a batch (device pointers).
@@ -627,6 +631,24 @@ __device__ __forceinline__ double lagrange_forcing(const Src &src, int b, int x,
 #pragma unroll
     for (int q = 0; q < 7; q++) xs[q] = (double)(time1 - 4 + q) * cad;
     const double *vr = src.vr + (size_t)b * src.nvr * nx + x;
+    // the seven slices' values (clamped slice indices), all loaded before the
+    // first is used so that they cost one memory round trip: the empty asm
+    // takes all seven at one point, which keeps the compiler from turning the
+    // zeroing of slices outside 1..nvr below into branches that wait for each
+    // load in turn
+    double vs[7];
+#pragma unroll
+    for (int q = 0; q < 7; q++) {
+        const int slice = time1 - 3 + q;  // 1-based vr slice
+        const int sl = min(max(slice, 1), max(src.nvr, 1));
+        vs[q] = vr[(size_t)(sl - 1) * nx];
+    }
+    asm volatile("" : "+v"(vs[0]), "+v"(vs[1]), "+v"(vs[2]), "+v"(vs[3]), "+v"(vs[4]), "+v"(vs[5]), "+v"(vs[6]));
+#pragma unroll
+    for (int q = 0; q < 7; q++) {
+        const int slice = time1 - 3 + q;
+        vs[q] = (slice >= 1 && slice <= src.nvr) ? vs[q] : 0.0;
+    }
     double f = 0.0;
 #pragma unroll
     for (int q = 0; q < 7; q++) {
@@ -636,7 +658,6 @@ __device__ __forceinline__ double lagrange_forcing(const Src &src, int b, int x,
         if (q == 6) ok = time1 < src.num_steps - 1;
         double LC = 0.0;
         if (ok) {
-            const int slice = time1 - 3 + q;  // 1-based vr slice
             double den = 1.0;
             bool first = true;
 #pragma unroll
@@ -645,8 +666,7 @@ __device__ __forceinline__ double lagrange_forcing(const Src &src, int b, int x,
                 if (first) { den = xs[q] - xs[r]; first = false; }
                 else den = den * (xs[q] - xs[r]);
             }
-            const double v = (slice >= 1 && slice <= src.nvr) ? vr[(size_t)(slice - 1) * nx] : 0.0;
-            LC = v / den;
+            LC = vs[q] / den;
         }
         double tq = LC;
 #pragma unroll

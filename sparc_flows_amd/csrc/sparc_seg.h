@@ -137,4 +137,14 @@ struct SegTables {
 // false if the interface system is singular.
 bool build_seg_tables(int nz, int m, SegTables &t);
 
+// Segment kind of segment s of P: 0 the first, 2 the last, 1 any other.
+// Every interior segment has the same tables (the dbyd2 coefficients are
+// constant away from the boundary rows, and each segment's factorization
+// restarts at its start), so the column pass keeps three segments' worth.
+SPARC_HD int seg_kind(int s, int P) { return s == 0 ? 0 : (s == P - 1 ? 2 : 1); }
+// The tables of the first, an interior and the last segment, [15][3][m] (row
+// r = 5 slot + k of t.slots, then the segment kind); false if some interior
+// segment's tables differ from the first interior one's.
+bool seg_kind_tables(const SegTables &t, std::vector<double> &out);
+
 }  // namespace sparc

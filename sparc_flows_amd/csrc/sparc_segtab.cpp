@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
+#include <cstring>
 #include <vector>
 
 #include "sparc_seg.h"
@@ -156,6 +157,21 @@ bool build_seg_tables(int N, int m, SegTables &t)
                 if (q >= 0 && q < n2)
                     t.band[((size_t)slot * n2 + k) * W + w] = minv_all[((size_t)slot * n2 + k) * n2 + q];
             }
+    return true;
+}
+
+bool seg_kind_tables(const SegTables &t, std::vector<double> &out)
+{
+    const int P = t.P, m = t.m, nzp = t.nzp;
+    const int sk[3] = {0, P > 1 ? 1 : 0, P - 1};
+    out.clear();
+    out.reserve((size_t)15 * 3 * m);
+    for (int r = 0; r < 15; r++) {
+        const double *row = t.slots.data() + (size_t)r * nzp;
+        for (int s = 2; s < P - 1; s++)
+            if (std::memcmp(row + (size_t)s * m, row + m, (size_t)m * sizeof(double))) return false;
+        for (int k = 0; k < 3; k++) out.insert(out.end(), row + (size_t)sk[k] * m, row + (size_t)(sk[k] + 1) * m);
+    }
     return true;
 }
 

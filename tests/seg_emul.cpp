@@ -21,6 +21,9 @@ static int run(int nz, int ibc, const double *B, double preset0, double presetN,
     const int slot = ibc - 1;                     // table sets for ibc 1, 2, 3
     double c1, c2, cb[kCB];
     dbyd2_coefs(nz, c1, c2, cb);
+    // the tables as the kernel holds them: three segments' worth, by kind
+    std::vector<double> kt;
+    if (!seg_kind_tables(t, kt)) return -4;
     std::vector<double> yp(yb_len(P, hw), 0.0);
     std::vector<double> Ys((size_t)P * MS);
     const double *band = t.band.data() + (size_t)slot * n2P * nw;
@@ -32,7 +35,7 @@ static int run(int nz, int ibc, const double *B, double preset0, double presetN,
             r = r < 0 ? 0 : (r > nz - 1 ? nz - 1 : r);
             return B[r];
         };
-        auto tab = [&](int tt, int i) { return t.slots[((size_t)slot * 5 + tt) * nzp + a + i]; };
+        auto tab = [&](int tt, int i) { return kt[((size_t)(slot * 5 + tt) * 3 + seg_kind(s, P)) * MS + i]; };
         double e[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
         if (lastseg)
             for (int k = 0; k < 5; k++) e[k] = B[nz - 5 + k];
@@ -45,7 +48,7 @@ static int run(int nz, int ibc, const double *B, double preset0, double presetN,
     }
     for (int s = 0; s < P; s++) {
         const int a = s * MS;
-        auto tab = [&](int tt, int i) { return t.slots[((size_t)slot * 5 + tt) * nzp + a + i]; };
+        auto tab = [&](int tt, int i) { return kt[((size_t)(slot * 5 + tt) * 3 + seg_kind(s, P)) * MS + i]; };
         double Y[MS];
         std::memcpy(Y, &Ys[(size_t)s * MS], sizeof Y);
         seg_correct<MS>(Y, s, P, hw, yp.data(), band, tab);

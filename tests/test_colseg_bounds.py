@@ -14,17 +14,13 @@ import numpy as np
 import pytest
 
 SIZES = (4, 6, 8, 10, 12, 16, 19, 24, 32)
-KSEGMAX, KZT, KZL, KCB, KPL = 32, 21, 21, 12, 7
-L_ROWS = {"str": 15, "g": 16, "rho": 17, "grho": 18, "gp": 19, "rinv": 20}
+KSEGMAX, KZT, KZS, KZF, KCB, KPL = 32, 21, 15, 6, 12, 7
 Z_PROFILE_ROWS = [0, 1, 17, 18, 19, 20]   # ztab rows of the six epilogue profiles
 
 
-def lrow_src(r):
-    """ztab row of staged z-table row r (the kernel's lrow_src)."""
-    return 2 + 5 * (r // 5) + r % 5 if r < 15 else Z_PROFILE_ROWS[r - 15]
-
-
-LROW_SRC = np.array([lrow_src(r) for r in range(KZL)])
+def stage_prow(r):
+    """ztab row of staged profile row r (stage_prow in sparc_device.h)."""
+    return 0 if r == 0 else (1 if r == 1 else 15 + r)
 
 
 def seg_lengths(nz):
@@ -70,9 +66,11 @@ def emulate(nz, ms, tc, split, nx, S, nb, nt, hw=3):
     assert split, "only the field-split kernel exists"
     G = -(-tc * P // 64) * 64
     block = 3 * G
-    nstaged = 3 * nband + KZL * nzp + KPL * nzpml * tc
+    nstaged = 3 * nband + KZS * 3 * ms + KZF * nzp + KPL * nzpml * tc + nzp
     lds = nstaged + 4 * tc * LY + 3 * tc * nzp
-    zl, pl, yb0 = 3 * nband, 3 * nband + KZL * nzp, nstaged
+    zl = 3 * nband
+    zf = zl + KZS * 3 * ms
+    pl, yb0 = zf + KZF * nzp, nstaged
     wl = yb0 + 4 * tc * LY
     chk = Check(f"nz={nz} MS={ms} TC={tc} split={split} nx={nx} S={S}")
     tid = np.arange(block)[:, None]
@@ -100,6 +98,13 @@ def emulate(nz, ms, tc, split, nx, S, nb, nt, hw=3):
             k10 = np.arange(ms + 10)[None, :]
             chk.idx("filter window", col + np.clip(a - 5 + k10, 0, nz - 1) * nx, S * n)
             chk.idx("topbc", b * nx + x, S * nx)
+            chk.idx("rho0 at the source row", x, nx)
+            # adjoint-source slices of LAGRANGE_INTERP, loaded with clamped slice
+            # numbers for every time1 (nvr slices of [S][nvr][nx])
+            nvr = 9
+            for time1 in range(-6, nvr + 6):
+                sl = np.clip(time1 - 3 + np.arange(7), 1, max(nvr, 1))
+                chk.idx("adjoint source slices", (b * nvr + sl[None, :] - 1) * nx + x, S * nvr * nx)
             # z tables: zt = ztab + a, rows zrow(slot, k) = 2 + 5 slot + k and 17..20
             for row in [2 + 5 * sl + k for sl in range(3) for k in range(5)] + [0, 1, 17, 18, 19, 20]:
                 chk.idx("ztab", row * nzp + a + i, KZT * nzp + KCB)
@@ -116,6 +121,9 @@ def emulate(nz, ms, tc, split, nx, S, nb, nt, hw=3):
             chk.idx("az/bz/fz", pj * nx + x, nzpml * nx)
             for k in range(KPL):
                 chk.idx("staged z-PML data", pl + (k * nzpml + pj) * tc + c, lds)
+            zpl = pl + KPL * nzpml * tc
+            chk.idx("staged source z profile (read)", zpl + j, nstaged)
+            chk.idx("staged source z profile (read, lower end)", j, nzp)
             # stores: only real rows store (padding rows are masked off)
             chk.idx("valid-row stores", qj, S * n, valid)
             # snapshots and vz output at their extreme slots
@@ -151,8 +159,13 @@ def emulate(nz, ms, tc, split, nx, S, nb, nt, hw=3):
             chk.idx("yb pads", yb0 + pad, yend)
             assert len(set(pad.tolist())) == pad.size
             assert not set(pad.tolist()) & {f * LY + hw2 + v for f in range(4 * tc) for v in range(n2P)}
-            for r in range(KZL):
-                chk.idx("staged z table", zl + r * nzp + a + i, pl)
+            sc = np.where(act, s, 0)
+            zt = zl + np.where(sc == 0, 0, np.where(sc == P - 1, 2, 1)) * ms   # seg_kind
+            for r in range(KZS):
+                chk.idx("staged segment table", zt + r * 3 * ms + i, zf)
+                chk.idx("staged segment table (lower end)", zt + r * 3 * ms + i - zl, KZS * 3 * ms)
+            for r in range(KZF):
+                chk.idx("staged z profile", zf + r * nzp + a + i, pl)
             chk.idx("wl group 1", wl + c * nzp + a + i, lds, act)
             chk.idx("wl group 0 (w0)", wl + (tc + c) * nzp + a + i, lds, act)
             chk.idx("p column (store)", wl + (2 * tc + c) * nzp + j, lds, valid)
@@ -163,8 +176,8 @@ def emulate(nz, ms, tc, split, nx, S, nb, nt, hw=3):
     # both): the inverses and z tables are element q < nA of the contiguous
     # LDS image sg.stage, the z-PML data element q < nzpml*TC of kind k
     bx, b = gridx - 1, S - 1
-    nA, npl = 3 * nband + KZL * nzp, nzpml * tc
-    assert nA + KPL * npl == nstaged
+    nA, npl = 3 * nband + KZS * 3 * ms + KZF * nzp, nzpml * tc
+    assert nA + KPL * npl + nzp == nstaged
     q = np.arange(nA)
     chk.idx("staging: image", q, nA)
     q = np.arange(npl)
@@ -173,7 +186,11 @@ def emulate(nz, ms, tc, split, nx, S, nb, nt, hw=3):
     chk.idx("staging: pz slot 0", (b * nzpml + pj2) * nx + xc, 2 * S * nzpml * nx)
     chk.idx("staging: pz slot 1", ((S + b) * nzpml + pj2) * nx + xc, 2 * S * nzpml * nx)
     chk.idx("staging: LDS", nA + np.arange(KPL)[:, None] * npl + q[None, :], nstaged)
-    assert max(lrow_src(r) for r in range(KZL)) < KZT
+    # the source z profile: element q < nzp, loaded from row min(q, nz-1) of [nz]
+    q = np.arange(nzp)
+    chk.idx("staging: source z profile", np.minimum(q, nz - 1), nz)
+    chk.idx("staging: source z profile (LDS)", nA + KPL * npl + q, nstaged)
+    assert [stage_prow(r) for r in range(KZF)] == Z_PROFILE_ROWS
     return P
 
 

@@ -298,3 +298,25 @@ def test_every_column_pass_shape_matches_oracle(nz, monkeypatch):
         if not (np.isfinite(e_or) and e_or < 1e-11 and e_b < 1e-13):
             bad.append((ms, tc, e_or, e_b))
     assert not bad, bad
+
+
+@pytest.mark.parametrize("nz", [700, 1000])
+def test_steps_match_oracle_deep_columns(nz):
+    """Columns deeper than 384 rows, where the column pass takes segments of
+    16-32 rows (the only lengths with at most 32 segments) and its LDS holds
+    the segment tables of three segments only: 20 full steps with flow and a
+    sound-speed bump against the oracle."""
+    p, bg, c2d, psi = small_case(nx=32, nz=nz, npml=10, nph=4)
+    maxtime = 30
+    s = Sparc(p, bg, c2d=c2d, psi=psi)
+    o = oracle.Oracle(p, bg, c2d=c2d, psi=psi)
+    s.set_forward_sources([3.0], maxtime)
+    o.set_forward_source(3.0, maxtime)
+    s.reset()
+    for t in range(20):
+        s.step(t)
+        o.step(t)
+    print("deep column nz", nz, "MS", s.query("colseg_ms"), "TC", s.query("colseg_tc"), flush=True)
+    err = rel_l2(s.state()[:, 0].cpu().numpy(), o.state())
+    print("20 steps, nz", nz, err)
+    assert err < 1e-11

@@ -100,3 +100,26 @@ def test_each_boundary_row_is_exercised(emul):
                 got, _ = seg_ddz(emul, B, ms, ibc, 0.0, 0.0)
                 ref = reference(B, ibc, 0.0, 0.0)
                 assert np.abs(got - ref).max() < 1e-12 * max(np.abs(ref).max(), 1.0), (r, ibc)
+
+
+def test_column_pass_fits_lds_up_to_nz_1024(emul):
+    """Some segment length the autotuner can choose fits the column pass's
+    LDS (colsplit_lds_bytes, TC = 1) in gfx950's 160 KB for every nz up to
+    1024: the segment tables are held for three segments only, and the
+    profiles, p, the RHS v_z parts and the source profile per row."""
+    kzs, kzf, kpl, nzpml = 15, 6, 7, 40
+    worst = 0
+    for nz in list(range(16, 1025, 7)) + [1023, 1024]:
+        best = None
+        for ms in [m for m in SIZES if usable(nz, m)]:
+            _, hw = seg_ddz(emul, np.ones(nz), ms, 1, 0.0, 0.0)
+            P = -(-nz // ms)
+            nzp = P * ms
+            staged = 3 * 2 * P * (4 * hw + 4) + kzs * 3 * ms + kzf * nzp + kpl * nzpml + nzp
+            lds = 8 * (staged + 4 * (2 * P + 4 * hw + 2) + 3 * nzp)
+            best = lds if best is None else min(best, lds)
+        if best is None:
+            continue                         # one of the segmentation gaps (test_colseg_bounds)
+        assert best <= 160 * 1024, (nz, best)
+        worst = max(worst, best)
+    print(f"largest minimum LDS {worst / 1024:.1f} KB")
