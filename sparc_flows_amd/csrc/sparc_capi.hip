@@ -668,11 +668,10 @@ static int setup_seg(sparc_handle *h)
         const int v = std::atoi(e);
         h->row_wide = (v == 1 || v == 2) && h->nx == 256 ? v : 0;
     }
-    h->seg_tc = 1;
-    for (int tc : {2, 4, 8, 16}) {
-        const bool fits = (tc * sg.P + 63) / 64 <= 2;
-        if (h->nx % tc == 0 && fits) h->seg_tc = tc;
-    }
+    h->seg_tc = 0;
+    for (int tc : {1, 2, 4, 8, 16})
+        if (colsplit_fits(tc, h->d, sg)) h->seg_tc = tc;
+    if (!h->seg_tc) return fail(h, SPARC_E_ARG, "no column-pass block shape fits this grid");
     return SPARC_OK;
 }
 
@@ -1264,7 +1263,7 @@ static int autotune(sparc_handle *h)
     h->tuned = true;
     const TuneKey key{{h->device, h->nx, h->nz, h->S, h->nb, h->nt, h->nh}};
     const bool forced = h->seg_forced || std::getenv("SPARC_COLSEG_TC");
-    auto fits = [&](int tc, const SegTab &sg) { return h->nx % tc == 0 && (tc * sg.P + 63) / 64 <= 2; };
+    auto fits = [&](int tc, const SegTab &sg) { return colsplit_fits(tc, h->d, sg); };
     if (!forced) {
         TuneChoice c{0, 0};
         {

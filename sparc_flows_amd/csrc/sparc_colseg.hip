@@ -496,6 +496,12 @@ hipError_t colseg_init()
 
 namespace sparc {
 
+bool colsplit_fits(int tc, const Dims &d, const SegTab &sg)
+{
+    return d.nx % tc == 0 && (tc * sg.P + 63) / 64 <= 2 &&
+           colsplit_lds_bytes(tc, sg.P, sg.hw, sg.nzp, d.nzpml, sg.m) <= (size_t)g_lds_max;
+}
+
 hipError_t launch_colsplit(int tc, int ms, const Dims &d, const Bg &bg, const DbyCoef &cf, const SegTab &sg,
                            const State &s, const ColArgs &ca, const Src &src, const Clock &c, const Sink &sk,
                            hipStream_t st)

@@ -56,7 +56,7 @@ struct Tri {
 // interface system.  ztab rows (each nzp = P*m long): 0 stretch, 1 g, then
 // for slot 0 (ibc 1), 1 (ibc 2), 2 (ibc 3): lowr, rbl, gl, V, W (rows 2-16),
 // then the x-independent profiles rho0, d rho0/dz, d p0/dz, 1/rho0 (17-20).
-constexpr int kSegMax = 32;
+constexpr int kSegMax = 64;
 constexpr int kZT = 21;     // then the kCB boundary-row coefficients (sparc_seg.h)
 // The column pass keeps in LDS the kZS segment-table rows of the three dbyd2
 // sets (ztab rows 2-16) for three segments only, [kZS][3][m]: the first, an

@@ -16,8 +16,11 @@ hipError_t launch_row(int mode, const Dims &d, const Bg &bg, const State &s, con
 // Segmented column pass, one lane per (column, segment, field); tc in
 // {1, 2, 4, 8, 16} columns per block, ms = rows per segment, one of
 // SPARC_COLSEG_SIZES.
-#define SPARC_COLSEG_SIZES(X) X(4) X(6) X(8) X(10) X(12) X(16) X(19) X(24) X(32)
+#define SPARC_COLSEG_SIZES(X) X(4) X(6) X(8) X(10) X(12) X(16) X(19) X(20) X(24)
 hipError_t colseg_init();
+// Whether a column-pass block of tc columns fits: its threads (3 lane groups
+// of at most 128) and its dynamic LDS (after colseg_init, the device maximum).
+bool colsplit_fits(int tc, const Dims &d, const SegTab &sg);
 hipError_t launch_colsplit(int tc, int ms, const Dims &d, const Bg &bg, const DbyCoef &cf, const SegTab &sg,
                            const State &s, const ColArgs &ca, const Src &src, const Clock &c, const Sink &sk,
                            hipStream_t st);

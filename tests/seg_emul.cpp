@@ -71,6 +71,7 @@ extern "C" int seg_ddz_column(int nz, int ms, int ibc, const double *B, double p
     case 12: return run<12>(nz, ibc, B, preset0, presetN, out, hw);
     case 16: return run<16>(nz, ibc, B, preset0, presetN, out, hw);
     case 19: return run<19>(nz, ibc, B, preset0, presetN, out, hw);
+    case 20: return run<20>(nz, ibc, B, preset0, presetN, out, hw);
     case 24: return run<24>(nz, ibc, B, preset0, presetN, out, hw);
     case 32: return run<32>(nz, ibc, B, preset0, presetN, out, hw);
     }

@@ -1,7 +1,7 @@
 """CPU emulation of the index arithmetic of the column-pass kernel
 (sparc_flows_amd/csrc/sparc_colseg.hip: k_colsplit) for every launch shape
 the autotuner can choose: segment lengths MS of SPARC_COLSEG_SIZES with
-P = ceil(nz/MS) <= 32 and a last segment of >= 3 real rows, block widths TC in
+P = ceil(nz/MS) <= 64 and a last segment of >= 3 real rows, block widths TC in
 {1, 2, 4, 8, 16} with 3*roundup(TC*P, 64) <= 384, both segment-count extremes
 of every MS, and nz from 16 to 1024.  Every global index (loads and the
 stores of real rows) must fall inside its allocation and every LDS index inside the
@@ -13,8 +13,8 @@ block and every unrolled row; blocks are checked at the grid's extremes
 import numpy as np
 import pytest
 
-SIZES = (4, 6, 8, 10, 12, 16, 19, 24, 32)
-KSEGMAX, KZT, KZS, KZF, KCB, KPL = 32, 21, 15, 6, 12, 7
+SIZES = (4, 6, 8, 10, 12, 16, 19, 20, 24)
+KSEGMAX, KZT, KZS, KZF, KCB, KPL = 64, 21, 15, 6, 12, 7
 Z_PROFILE_ROWS = [0, 1, 17, 18, 19, 20]   # ztab rows of the six epilogue profiles
 
 
@@ -215,7 +215,7 @@ def test_every_autotuner_shape_stays_in_bounds():
     assert count > 1000
 
 
-def test_segmentation_covers_every_nz_but_the_documented_gaps():
-    """nz without a usable segmentation: exactly the 18 values the library rejects."""
+def test_segmentation_covers_every_nz():
+    """Every nz from 16 to 1024 has a usable segmentation."""
     gaps = [nz for nz in range(16, 1025) if not seg_lengths(nz)]
-    assert len(gaps) == 18 and min(gaps) == 673
+    assert gaps == []

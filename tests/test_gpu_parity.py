@@ -236,9 +236,10 @@ def test_explicit_flow_equals_psi_flow():
 
 def test_unsupported_sizes_are_rejected():
     """nx must be a power of two in [16, 1024] (the parameter check says so
-    before any device work); nz needs a segmentation of the column pass (673
-    has none), which the library rejects at creation."""
-    for nx, nz in ((100, 60), (64, 673)):
+    before any device work); nz needs a segmentation of the column pass into
+    at most 64 segments (1600 has none), which the library rejects at
+    creation."""
+    for nx, nz in ((100, 60), (64, 1600)):
         p, bg, _, _ = small_case(nx=64, nz=min(nz, 300), flow=False, dc=False)
         bg = B.polytrope(nz)
         p = dataclasses.replace(p, nx=nx, nz=nz, timestep=B.stable_timestep(bg))
@@ -249,8 +250,8 @@ def test_unsupported_sizes_are_rejected():
 def _shapes(nz):
     """The autotuner's candidates (sparc_capi.hip: tune_lengths): usable
     segment lengths up to 12, or the shortest usable one."""
-    sizes = (4, 6, 8, 10, 12, 16, 19, 24, 32)
-    usable = [ms for ms in sizes if -(-nz // ms) <= 32 and nz - (-(-nz // ms) - 1) * ms >= 3]
+    sizes = (4, 6, 8, 10, 12, 16, 19, 20, 24)
+    usable = [ms for ms in sizes if -(-nz // ms) <= 64 and nz - (-(-nz // ms) - 1) * ms >= 3]
     tuned = [ms for ms in usable if ms <= 12] or usable[:1]
     for ms in tuned:
         P = -(-nz // ms)

@@ -22,12 +22,12 @@ import oracle
 
 ROOT = Path(__file__).resolve().parents[1]
 CSRC = ROOT / "sparc_flows_amd" / "csrc"
-SIZES = (4, 6, 8, 10, 12, 16, 19, 24, 32)
+SIZES = (4, 6, 8, 10, 12, 16, 19, 20, 24)
 
 
 def usable(nz, ms):
     p = -(-nz // ms)
-    return p <= 32 and nz - (p - 1) * ms >= 3
+    return p <= 64 and nz - (p - 1) * ms >= 3
 
 
 @pytest.fixture(scope="module")
