@@ -618,6 +618,26 @@ __device__ __forceinline__ double filter_vars_2d_at(const double *__restrict__ a
     return 0.5 * (a[q + nx] + a[q - nx]);
 }
 
+// The same value with all eleven rows loaded (row indices clamped to the
+// column) and the form selected afterwards, so that a wave whose lanes take
+// different forms issues one set of loads instead of one per form.
+__device__ __forceinline__ double filter_vars_2d_sel(const double *__restrict__ a, size_t q, int k,
+                                                     int x, int nx, int nz, int nh)
+{
+    const size_t q0 = q - (size_t)k * nx;       // row 0 of this column
+    double w[11];
+#pragma unroll
+    for (int u = 0; u < 11; u++) w[u] = a[q0 + (size_t)min(max(k - 5 + u, 0), nz - 1) * nx];
+    const double f11 = 0.75390625 * w[5] + 0.5 * 0.41015625 * (w[4] + w[6]) +
+                       0.5 * -0.234375 * (w[3] + w[7]) +
+                       0.5 * 0.087890625 * (w[2] + w[8]) +
+                       0.5 * -0.01953125 * (w[1] + w[9]) +
+                       0.5 * 0.001953125 * (w[0] + w[10]);
+    const double f2 = 0.5 * (w[6] + w[4]);
+    const bool keep = k < 5 || k > nz - 2 || x < nh - 1 || x > nx - nh;
+    return keep ? w[5] : (k <= nz - 6 ? f11 : f2);
+}
+
 __device__ __forceinline__ int clock_t_(const Clock &c) { return c.tctr[0] + c.toff; }
 
 // LAGRANGE_INTERP (physics.f90:678-737) for column x of source b.

@@ -135,10 +135,24 @@ __global__ __launch_bounds__(64) void k_row(Dims d, Bg bg, State st, RowArgs ra,
     // wait behind this pass's global stores
     __shared__ double2 stw[NX];
     __shared__ double sdm[NX];
+    {
+        constexpr int NQ = (NX + 63) / 64;
+        double2 tv[NQ];
+        double dv[NQ];
 #pragma unroll
-    for (int q = threadIdx.x; q < NX; q += 64) {
-        stw[q] = bg.tw[q];
-        sdm[q] = bg.dmul[q];
+        for (int u = 0; u < NQ; u++) {
+            const int q = min((int)threadIdx.x + 64 * u, NX - 1);
+            tv[u] = bg.tw[q];
+            dv[u] = bg.dmul[q];
+        }
+#pragma unroll
+        for (int u = 0; u < NQ; u++) {
+            const int q = threadIdx.x + 64 * u;
+            if (NX % 64 == 0 || q < NX) {      // a guard only where it can fail, which keeps the loads above it
+                stw[q] = tv[u];
+                sdm[q] = dv[u];
+            }
+        }
     }
     const int team = threadIdx.x / N1, t = threadIdx.x % N1;
     const int r = blockIdx.x * TEAMS + team;
@@ -152,17 +166,88 @@ __global__ __launch_bounds__(64) void k_row(Dims d, Bg bg, State st, RowArgs ra,
     const size_t pxq1 = ((size_t)(d.S + b) * nz + k) * nph;     // psi_gradp_x row
     double *const dump = st.discard + threadIdx.x;              // target of invalid-row stores
     double re[N2], im[N2], vx[N2];
+    // x-PML auxiliary data (psi stage value and base, a_x, b_x) of the lane's
+    // columns in the first (j2 = 0) and last (j2 = N2-1) column block, the
+    // only ones that hold PML columns when the layer is at most N1 wide:
+    // loaded here, unconditionally at clamped indices, so that their latency
+    // overlaps the row loads instead of following the FFTs (set 0:
+    // psi_gradp_x of the v_x update, set 1: psivx of d xi_x/dx).  Only for
+    // teams of at least 32 lanes (N2 <= 8): the 16-lane form holds 16
+    // columns per lane, and the extra registers cost it more than the waits.
+    constexpr bool kPre = N2 <= 8;
+    auto pml_index = [&](int x) { return x < nh ? x : min(max(x - NX + 2 * nh, 0), nph - 1); };
+    double pre[2][2][4] = {};
+#pragma unroll
+    for (int e = 0; e < (kPre ? 2 : 0); e++) {
+        const int pi = pml_index(t + (e ? N1 * (N2 - 1) : 0));
+#pragma unroll
+        for (int set = 0; set < 2; set++) {
+            const size_t q = (set == 0 ? pxq1 : pxq0) + pi;
+            pre[set][e][0] = (MODE == 0 && set == 0) || (MODE == 2 && set == 1) ? 0.0 : st.px_tmp[q];
+            pre[set][e][1] = (MODE == 0 && set == 0) || (MODE == 2 && set == 1) ? 0.0 : st.px_base[q];
+            pre[set][e][2] = bg.ax[(size_t)k * nph + pi];
+            pre[set][e][3] = bg.bx[(size_t)k * nph + pi];
+        }
+    }
+    // x-PML data of column block j2: prefetched for the end blocks, loaded
+    // here for an inner block (a layer wider than N1)
+    auto pml_data = [&](int set, int j2, int pi, double &psi, double &pb, double &axv, double &bxv) {
+        if (kPre && N1 >= nh && (j2 == 0 || j2 == N2 - 1)) {
+            const int e = j2 == 0 ? 0 : 1;
+            psi = pre[set][e][0];
+            pb = pre[set][e][1];
+            axv = pre[set][e][2];
+            bxv = pre[set][e][3];
+        } else {
+            const size_t q = (set == 0 ? pxq1 : pxq0) + pi;
+            psi = st.px_tmp[q];
+            pb = st.px_base[q];
+            axv = bg.ax[(size_t)k * nph + pi];
+            bxv = bg.bx[(size_t)k * nph + pi];
+        }
+    };
+    const double unst = bg.unstretch[nz - 1];
+    // takes the prefetched values at one point (before the first FFT, after
+    // the row loads are issued), so that the compiler cannot sink their loads
+    // into the PML branches after the FFTs
+    auto pin_pre = [&]() {
+        if (!kPre) return;
+        asm volatile("" : "+v"(pre[0][0][0]), "+v"(pre[0][0][1]), "+v"(pre[0][0][2]), "+v"(pre[0][0][3]),
+                     "+v"(pre[0][1][0]), "+v"(pre[0][1][1]), "+v"(pre[0][1][2]), "+v"(pre[0][1][3]),
+                     "+v"(pre[1][0][0]), "+v"(pre[1][0][1]), "+v"(pre[1][0][2]), "+v"(pre[1][0][3]),
+                     "+v"(pre[1][1][0]), "+v"(pre[1][1][1]), "+v"(pre[1][1][2]), "+v"(pre[1][1][3]));
+    };
 
     if (MODE == 0) {
         // v_x of stage 1 (= a_vx after the pending filter), xi_x, v_z
+        if (kPre) {
+            // the choice is uniform: one loop per form, every load of a loop
+            // issued before the first is used
+            if (ra.filt_vx) {
 #pragma unroll
-        for (int j2 = 0; j2 < N2; j2++) {
-            const int x = t + N1 * j2;
-            const double v = ra.filt_vx ? filter_vars_2d_at(st.vx_unf, rowq + x, k, x, NX, nz, nh)
-                                        : st.a[2][rowq + x];
-            vx[j2] = v;
-            re[j2] = ra.xi_x[rowq + x];
-            im[j2] = v;
+                for (int j2 = 0; j2 < N2; j2++) {
+                    const int x = t + N1 * j2;
+                    vx[j2] = filter_vars_2d_sel(st.vx_unf, rowq + x, k, x, NX, nz, nh);
+                }
+            } else {
+#pragma unroll
+                for (int j2 = 0; j2 < N2; j2++) vx[j2] = st.a[2][rowq + t + N1 * j2];
+            }
+#pragma unroll
+            for (int j2 = 0; j2 < N2; j2++) {
+                re[j2] = ra.xi_x[rowq + t + N1 * j2];
+                im[j2] = vx[j2];
+            }
+        } else {
+#pragma unroll
+            for (int j2 = 0; j2 < N2; j2++) {
+                const int x = t + N1 * j2;
+                const double v = ra.filt_vx ? filter_vars_2d_at(st.vx_unf, rowq + x, k, x, NX, nz, nh)
+                                            : st.a[2][rowq + x];
+                vx[j2] = v;
+                re[j2] = ra.xi_x[rowq + x];
+                im[j2] = v;
+            }
         }
         if (ra.filt_vx) {
             const int tp = time - 1;
@@ -177,6 +262,7 @@ __global__ __launch_bounds__(64) void k_row(Dims d, Bg bg, State st, RowArgs ra,
                     sink.snap[((((size_t)b * sink.nsnap + sslot) * sink.nfield + fslot) * nz + k) * NX + x] = vx[j2];
             }
         }
+        pin_pre();
         __syncthreads();   // staged twiddles visible
         SPARC_FFT_DERIV(re, im, t, lds[team], stw, sdm, bg.stretchx);
         // re = dxi_x/dx, im = dv_x/dx
@@ -209,6 +295,7 @@ __global__ __launch_bounds__(64) void k_row(Dims d, Bg bg, State st, RowArgs ra,
             avx[j2] = st.a[2][rowq + x];
             rinv[j2] = bg.rhoinv_z[k];
         }
+        pin_pre();
         __syncthreads();   // staged twiddles visible
         SPARC_FFT_DERIV(re, im, t, lds[team], stw, sdm, bg.stretchx);
         // x-PML on grad_x p (physics2d.f90:1003-1020), RHS v_x (:1039 + flows)
@@ -220,9 +307,9 @@ __global__ __launch_bounds__(64) void k_row(Dims d, Bg bg, State st, RowArgs ra,
             double gpx = re[j2];
             if (N1 * j2 < nh || N1 * (j2 + 1) > NX - nh) {     // team-uniform: this j2 can hold PML columns
                 const bool pml = x < nh || x >= NX - nh;
-                const int pi = x < nh ? x : min(max(x - NX + 2 * nh, 0), nph - 1);
-                const double psi = st.px_tmp[pxq1 + pi], pb = st.px_base[pxq1 + pi];
-                const double axv = bg.ax[(size_t)k * nph + pi], bxv = bg.bx[(size_t)k * nph + pi];
+                const int pi = pml_index(x);
+                double psi, pb, axv, bxv;
+                pml_data(0, j2, pi, psi, pb, axv, bxv);
                 const double g2 = gpx + psi;
                 const double rhs = axv * g2 + bxv * psi;
                 const double nb = pb + ra.bx2 * rhs;
@@ -267,12 +354,12 @@ __global__ __launch_bounds__(64) void k_row(Dims d, Bg bg, State st, RowArgs ra,
     for (int j2 = 0; j2 < N2; j2++) {
         const int x = t + N1 * j2;
         double dx = re[j2];
-        if (k == nz - 1 && valid) st.topbc[(size_t)b * NX + x] = -dx * bg.unstretch[nz - 1];
+        if (k == nz - 1 && valid) st.topbc[(size_t)b * NX + x] = -dx * unst;
         if (N1 * j2 < nh || N1 * (j2 + 1) > NX - nh) {
             const bool pml = x < nh || x >= NX - nh;
-            const int pi = x < nh ? x : min(max(x - NX + 2 * nh, 0), nph - 1);
-            const double psi = st.px_tmp[pxq0 + pi], pb = st.px_base[pxq0 + pi];
-            const double axv = bg.ax[(size_t)k * nph + pi], bxv = bg.bx[(size_t)k * nph + pi];
+            const int pi = pml_index(x);
+            double psi, pb, axv, bxv;
+            pml_data(1, j2, pi, psi, pb, axv, bxv);
             const double d2 = dx + psi;
             const double rhs = axv * d2 + bxv * psi;
             const double nb = pb + ra.bx1 * rhs;
