@@ -634,6 +634,7 @@ static int build_seg(sparc_handle *h, int m, SegTab &sg)
         for (int r = 0; r < kZF; r++)
             stage.insert(stage.end(), ztab.begin() + (size_t)stage_prow(r) * nzp,
                          ztab.begin() + (size_t)(stage_prow(r) + 1) * nzp);
+        stage.insert(stage.end(), ztab.begin() + (size_t)kZT * nzp, ztab.begin() + (size_t)kZT * nzp + kCB);
     }
     HCHECK(h, upload(h->bg_mem, &p, stage, h->stream));
     sg.stage = p;

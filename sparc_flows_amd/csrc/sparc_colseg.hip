@@ -41,15 +41,15 @@ enum PRow { P_AZ = 0, P_BZ, P_FZ, P_T0, P_B0, P_T1, P_B1, kPL };
 
 // Dynamic LDS of one block, in this order: the staged region (the three
 // banded interface inverses [3][2P][4hw+4], the segment tables [kZS][3][ms],
-// the z profiles [kZF][nzp], the z-PML data [kPL][nzpml][TC], the mode-3 source z profile [nzp] (zero on
+// the z profiles [kZF][nzp], dbyd2's kCB boundary-row coefficients, the z-PML data [kPL][nzpml][TC], the mode-3 source z profile [nzp] (zero on
 // padding rows; only staged in that mode); filled in one pass at the start), the four
 // end-value exchange buffers [4][TC][2P + 4hw + 2] (each column's 2P end
 // values between 2hw and 2hw+2 zeros), and three [TC][nzp] row buffers: the
 // group-1 and group-0 parts of RHS v_z and p.
 inline size_t colsplit_staged(int tc, int P, int hw, int nzp, int nzpml, int ms)
 {
-    return (size_t)3 * 2 * P * (4 * hw + 4) + (size_t)kZS * 3 * ms + (size_t)kZF * nzp + (size_t)kPL * nzpml * tc +
-           (size_t)nzp;
+    return (size_t)3 * 2 * P * (4 * hw + 4) + (size_t)kZS * 3 * ms + (size_t)kZF * nzp + (size_t)kCB +
+           (size_t)kPL * nzpml * tc + (size_t)nzp;
 }
 inline size_t colsplit_lds_bytes(int tc, int P, int hw, int nzp, int nzpml, int ms)
 {
@@ -106,9 +106,10 @@ __global__ __launch_bounds__(384) void k_colsplit(Dims d, Bg bg, DbyCoef cf, Seg
     double *const mv = lds_dyn;                        // [3][nband] banded interface inverses
     double *const zl = mv + 3 * nband;                 // [kZS][3][MS] segment tables
     double *const zf = zl + kZS * 3 * MS;              // [kZF][nzp] z profiles
-    double *const pl = zf + kZF * nzp;                 // [kPL][nzpml][TC] z-PML data
+    double *const cbl = zf + kZF * nzp;                // [kCB] dbyd2 boundary-row coefficients
+    double *const pl = cbl + kCB;                      // [kPL][nzpml][TC] z-PML data
     double *const zpl = pl + kPL * nzpml * TC;         // [nzp] mode-3 source z profile
-    const int nstaged = 3 * nband + kZS * 3 * MS + kZF * nzp + kPL * nzpml * TC + nzp;
+    const int nstaged = 3 * nband + kZS * 3 * MS + kZF * nzp + kCB + kPL * nzpml * TC + nzp;
     const int LY = n2P + 4 * sg.hw + 2, hw2 = 2 * sg.hw;
     double *const yb0 = lds_dyn + nstaged;             // [4][TC][LY] end values: field groups 0-2, d p/dz
     double *const wl = yb0 + 4 * TC * LY;              // [TC][nzp] group-1 part of RHS v_z,
@@ -193,7 +194,7 @@ __global__ __launch_bounds__(384) void k_colsplit(Dims d, Bg bg, DbyCoef cf, Seg
         // thread), the z-PML data come from their own arrays (two elements per
         // thread and kind); every load of a round is issued before its stores
         constexpr int kStage = 32;
-        const int nA = 3 * nband + kZS * 3 * MS + kZF * nzp, npl = nzpml * TC, stride = blockDim.x;
+        const int nA = 3 * nband + kZS * 3 * MS + kZF * nzp + kCB, npl = nzpml * TC, stride = blockDim.x;
         const int x0 = bx * TC;
         const int rounds = max((nA + kStage * stride - 1) / (kStage * stride),
                                (max(npl, nzp) + 2 * stride - 1) / (2 * stride));
@@ -243,8 +244,12 @@ __global__ __launch_bounds__(384) void k_colsplit(Dims d, Bg bg, DbyCoef cf, Seg
     const int isrc = src.row - a;
     if (f == 0 && act && isrc >= 0 && isrc < MS && time <= src.limit) {
         if (src.mode == 1) {
-            const double fs = (time >= 0 && time < src.nfwd) ? src.fwd[time] : 0.0;
-            srcv = src.vr[b * nx + x] * fs * src.delta_width / r0src;
+            // both loads issued before either is used (clamped time index)
+            double fsr = src.fwd[min(max(time, 0), max(src.nfwd - 1, 0))];
+            double vrx = src.vr[b * nx + x];
+            asm volatile("" : "+v"(fsr), "+v"(vrx));
+            const double fs = (time >= 0 && time < src.nfwd) ? fsr : 0.0;
+            srcv = vrx * fs * src.delta_width / r0src;
         } else if (src.mode == 2) {
             srcv = lagrange_forcing(src, b, x, nx, time) * src.delta_width / r0src;
         }
@@ -252,7 +257,7 @@ __global__ __launch_bounds__(384) void k_colsplit(Dims d, Bg bg, DbyCoef cf, Seg
     const double pamp = (f == 0 && act) ? profile_amp(src, b, x, nx, a, MS, time) : 0.0;
     const int slot_f = f == 0 ? 2 : 0;                 // dbyd2 table set: ibc 3 or ibc 1
     double Y[MS], own[MS];
-    const double *const cbt = sg.ztab + kZT * nzp;
+    const double *const cbt = cbl;                      // LDS: read only by the end segments' lanes
     seg_rhs<MS>([&](int k) { return win[k]; }, e, Y, a, nz, first, lastseg, f == 0 ? 3 : 1, 0.0, topv, cf.c1,
                 cf.c2, cbt);
 #pragma unroll

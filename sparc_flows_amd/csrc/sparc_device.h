@@ -71,7 +71,7 @@ struct SegTab {
     int P, m, hw, nzp;              // segments, rows per segment, Minv window, P*m
     const double *ztab;             // [kZT][nzp], then [kCB]
     const double *Minv;             // [3][2P][4hw+4]: row k of each inverse, entries 2(k/2-hw)+w
-    const double *stage;            // the column pass's LDS image: Minv, [kZS][3][m] segment tables, [kZF][nzp] profiles
+    const double *stage;            // the column pass's LDS image: Minv, [kZS][3][m] segment tables, [kZF][nzp] profiles, [kCB]
 };
 
 // Prognostic state and work arrays of a batch (device pointers).

@@ -33,13 +33,15 @@ def seg_lengths(nz):
 
 
 def nz_cases():
-    """nz 16..40 in full, then the two ends of every (MS, P) pair up to 1024."""
-    cases = set(range(16, 41))
+    """(nz, MS) pairs: nz 16..40 in full with every usable MS, then the two
+    ends of every (MS, P) pair up to nz = 1024 with that MS (where the
+    indices of that segmentation are tightest)."""
+    cases = {(nz, ms) for nz in range(16, 41) for ms in seg_lengths(nz)}
     for ms in SIZES:
         for p in range(1, KSEGMAX + 1):
             for nz in ((p - 1) * ms + 3, p * ms):
-                if 16 <= nz <= 1024:
-                    cases.add(nz)
+                if 16 <= nz <= 1024 and ms in seg_lengths(nz):
+                    cases.add((nz, ms))
     return sorted(cases)
 
 
@@ -66,11 +68,12 @@ def emulate(nz, ms, tc, split, nx, S, nb, nt, hw=3):
     assert split, "only the field-split kernel exists"
     G = -(-tc * P // 64) * 64
     block = 3 * G
-    nstaged = 3 * nband + KZS * 3 * ms + KZF * nzp + KPL * nzpml * tc + nzp
+    nstaged = 3 * nband + KZS * 3 * ms + KZF * nzp + KCB + KPL * nzpml * tc + nzp
     lds = nstaged + 4 * tc * LY + 3 * tc * nzp
     zl = 3 * nband
     zf = zl + KZS * 3 * ms
-    pl, yb0 = zf + KZF * nzp, nstaged
+    cbl = zf + KZF * nzp
+    pl, yb0 = cbl + KCB, nstaged
     wl = yb0 + 4 * tc * LY
     chk = Check(f"nz={nz} MS={ms} TC={tc} split={split} nx={nx} S={S}")
     tid = np.arange(block)[:, None]
@@ -165,7 +168,8 @@ def emulate(nz, ms, tc, split, nx, S, nb, nt, hw=3):
                 chk.idx("staged segment table", zt + r * 3 * ms + i, zf)
                 chk.idx("staged segment table (lower end)", zt + r * 3 * ms + i - zl, KZS * 3 * ms)
             for r in range(KZF):
-                chk.idx("staged z profile", zf + r * nzp + a + i, pl)
+                chk.idx("staged z profile", zf + r * nzp + a + i, cbl)
+            chk.idx("staged boundary coefficients", cbl + np.arange(KCB), pl)
             chk.idx("wl group 1", wl + c * nzp + a + i, lds, act)
             chk.idx("wl group 0 (w0)", wl + (tc + c) * nzp + a + i, lds, act)
             chk.idx("p column (store)", wl + (2 * tc + c) * nzp + j, lds, valid)
@@ -176,7 +180,7 @@ def emulate(nz, ms, tc, split, nx, S, nb, nt, hw=3):
     # both): the inverses and z tables are element q < nA of the contiguous
     # LDS image sg.stage, the z-PML data element q < nzpml*TC of kind k
     bx, b = gridx - 1, S - 1
-    nA, npl = 3 * nband + KZS * 3 * ms + KZF * nzp, nzpml * tc
+    nA, npl = 3 * nband + KZS * 3 * ms + KZF * nzp + KCB, nzpml * tc
     assert nA + KPL * npl + nzp == nstaged
     q = np.arange(nA)
     chk.idx("staging: image", q, nA)
@@ -197,21 +201,20 @@ def emulate(nz, ms, tc, split, nx, S, nb, nt, hw=3):
 def test_every_autotuner_shape_stays_in_bounds():
     split = 1
     count = 0
-    for nz in nz_cases():
-        for ms in seg_lengths(nz):
-            P = -(-nz // ms)
-            for tc in (1, 2, 4, 8, 16):
-                if split and 3 * (-(-tc * P // 64) * 64) > 384:
+    for nz, ms in nz_cases():
+        P = -(-nz // ms)
+        for tc in (1, 2, 4, 8, 16):
+            if split and 3 * (-(-tc * P // 64) * 64) > 384:
+                continue
+            if not split and tc * P > 256:
+                continue
+            for nx, S in ((16, 1), (1024, 3)):
+                if nx % tc:
                     continue
-                if not split and tc * P > 256:
-                    continue
-                for nx, S in ((16, 1), (1024, 3)):
-                    if nx % tc:
-                        continue
-                    npml = min(12, (nz - 1) // 2 - 1)
-                    hw = (0, 3, P)[count % 3]        # interface-inverse half-widths in turn
-                    emulate(nz, ms, tc, split, nx, S, npml, npml, hw)
-                    count += 1
+                npml = min(12, (nz - 1) // 2 - 1)
+                hw = (0, 3, P)[count % 3]        # interface-inverse half-widths in turn
+                emulate(nz, ms, tc, split, nx, S, npml, npml, hw)
+                count += 1
     assert count > 1000
 
 

@@ -115,7 +115,7 @@ def test_column_pass_fits_lds_up_to_nz_1024(emul):
             _, hw = seg_ddz(emul, np.ones(nz), ms, 1, 0.0, 0.0)
             P = -(-nz // ms)
             nzp = P * ms
-            staged = 3 * 2 * P * (4 * hw + 4) + kzs * 3 * ms + kzf * nzp + kpl * nzpml + nzp
+            staged = 3 * 2 * P * (4 * hw + 4) + kzs * 3 * ms + kzf * nzp + 12 + kpl * nzpml + nzp
             lds = 8 * (staged + 4 * (2 * P + 4 * hw + 2) + 3 * nzp)
             best = lds if best is None else min(best, lds)
         if best is None:
