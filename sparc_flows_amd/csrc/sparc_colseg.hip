@@ -48,8 +48,11 @@ enum PRow { P_AZ = 0, P_BZ, P_FZ, P_T0, P_B0, P_T1, P_B1, kPL };
 // group-1 and group-0 parts of RHS v_z and p.
 inline size_t colsplit_staged(int tc, int P, int hw, int nzp, int nzpml, int ms)
 {
-    return (size_t)3 * 2 * P * (4 * hw + 4) + (size_t)kZS * 3 * ms + (size_t)kZF * nzp + (size_t)kCB +
-           (size_t)kPL * nzpml * tc + (size_t)nzp;
+    // rounded up to an even count, so that the end-value buffers after it
+    // start on 16 bytes (the interface correction reads them in pairs)
+    const size_t n = (size_t)3 * 2 * P * (4 * hw + 4) + (size_t)kZS * 3 * ms + (size_t)kZF * nzp + (size_t)kCB +
+                     (size_t)kPL * nzpml * tc + (size_t)nzp;
+    return (n + 1) & ~(size_t)1;
 }
 inline size_t colsplit_lds_bytes(int tc, int P, int hw, int nzp, int nzpml, int ms)
 {
@@ -109,7 +112,7 @@ __global__ __launch_bounds__(384) void k_colsplit(Dims d, Bg bg, DbyCoef cf, Seg
     double *const cbl = zf + kZF * nzp;                // [kCB] dbyd2 boundary-row coefficients
     double *const pl = cbl + kCB;                      // [kPL][nzpml][TC] z-PML data
     double *const zpl = pl + kPL * nzpml * TC;         // [nzp] mode-3 source z profile
-    const int nstaged = 3 * nband + kZS * 3 * MS + kZF * nzp + kCB + kPL * nzpml * TC + nzp;
+    const int nstaged = (3 * nband + kZS * 3 * MS + kZF * nzp + kCB + kPL * nzpml * TC + nzp + 1) & ~1;
     const int LY = n2P + 4 * sg.hw + 2, hw2 = 2 * sg.hw;
     double *const yb0 = lds_dyn + nstaged;             // [4][TC][LY] end values: field groups 0-2, d p/dz
     double *const wl = yb0 + 4 * TC * LY;              // [TC][nzp] group-1 part of RHS v_z,

@@ -77,10 +77,19 @@ SPARC_HD void seg_solve(double (&Y)[MS], Tab tab)
     for (int i = MS - 2; i >= 0; i--) Y[i] = Y[i] - tab(2, i + 1) * Y[i + 1];
 }
 
+// Two consecutive doubles read as one 16-byte access (ds_read_b128 from
+// LDS); the pointer must be 16-byte aligned.
+struct alignas(16) Dbl2 {
+    double x, y;
+} __attribute__((may_alias));
+SPARC_HD Dbl2 ld2(const double *p) { return *reinterpret_cast<const Dbl2 *>(p); }
+
 // y <- y - L_{s-1} V - F_{s+1} W; the interface values are rows of the
 // banded inverse (row k holds entries q = 2 (k/2 - hw) + w, w < 4 hw + 4,
 // zero where q is outside 0..2P-1) against the segment end values y, stored
-// with 2 hw zeros before and 2 hw + 2 after (y[q] at yp[q + 2 hw]).
+// with 2 hw zeros before and 2 hw + 2 after (y[q] at yp[q + 2 hw]).  band
+// and yp must be 16-byte aligned (every row and window then starts on an
+// even element: 4 hw + 4 is a multiple of 4, the windows start at 2 (k/2)).
 template <int MS, typename Tab>
 SPARC_HD void seg_correct(double (&Y)[MS], int s, int P, int hw, const double *yp, const double *band, Tab tab)
 {
@@ -94,14 +103,12 @@ SPARC_HD void seg_correct(double (&Y)[MS], int s, int P, int hw, const double *y
     double a0 = 0.0, a1 = 0.0, b0 = 0.0, b1 = 0.0;
 #pragma unroll 2
     for (int w = 0; w < nw; w += 4) {
-        const double p0 = r1[w], p1 = r1[w + 1], p2 = r1[w + 2], p3 = r1[w + 3];
-        const double u0 = y1[w], u1 = y1[w + 1], u2 = y1[w + 2], u3 = y1[w + 3];
-        const double q0 = r2[w], q1 = r2[w + 1], q2 = r2[w + 2], q3 = r2[w + 3];
-        const double v0 = y2[w], v1 = y2[w + 1], v2 = y2[w + 2], v3 = y2[w + 3];
-        a0 += p0 * u0 + p2 * u2;
-        a1 += p1 * u1 + p3 * u3;
-        b0 += q0 * v0 + q2 * v2;
-        b1 += q1 * v1 + q3 * v3;
+        const Dbl2 pA = ld2(r1 + w), pB = ld2(r1 + w + 2), uA = ld2(y1 + w), uB = ld2(y1 + w + 2);
+        const Dbl2 qA = ld2(r2 + w), qB = ld2(r2 + w + 2), vA = ld2(y2 + w), vB = ld2(y2 + w + 2);
+        a0 += pA.x * uA.x + pB.x * uB.x;
+        a1 += pA.y * uA.y + pB.y * uB.y;
+        b0 += qA.x * vA.x + qB.x * vB.x;
+        b1 += qA.y * vA.y + qB.y * vB.y;
     }
     const double Lp = s > 0 ? a0 + a1 : 0.0;
     const double Fn = s < P - 1 ? b0 + b1 : 0.0;

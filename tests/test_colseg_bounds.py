@@ -68,7 +68,7 @@ def emulate(nz, ms, tc, split, nx, S, nb, nt, hw=3):
     assert split, "only the field-split kernel exists"
     G = -(-tc * P // 64) * 64
     block = 3 * G
-    nstaged = 3 * nband + KZS * 3 * ms + KZF * nzp + KCB + KPL * nzpml * tc + nzp
+    nstaged = (3 * nband + KZS * 3 * ms + KZF * nzp + KCB + KPL * nzpml * tc + nzp + 1) & ~1
     lds = nstaged + 4 * tc * LY + 3 * tc * nzp
     zl = 3 * nband
     zf = zl + KZS * 3 * ms
@@ -181,7 +181,10 @@ def emulate(nz, ms, tc, split, nx, S, nb, nt, hw=3):
     # LDS image sg.stage, the z-PML data element q < nzpml*TC of kind k
     bx, b = gridx - 1, S - 1
     nA, npl = 3 * nband + KZS * 3 * ms + KZF * nzp + KCB, nzpml * tc
-    assert nA + KPL * npl + nzp == nstaged
+    assert (nA + KPL * npl + nzp + 1) & ~1 == nstaged
+    # the interface correction reads band rows and end-value windows in
+    # 16-byte pairs: every row, buffer and window starts on an even element
+    assert nband % 2 == 0 and (4 * hw + 4) % 2 == 0 and yb0 % 2 == 0 and LY % 2 == 0
     q = np.arange(nA)
     chk.idx("staging: image", q, nA)
     q = np.arange(npl)
