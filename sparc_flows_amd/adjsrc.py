@@ -245,6 +245,8 @@ def adjoint_source_filt(acc: np.ndarray, dat: np.ndarray, x: np.ndarray, x00: fl
                 loc = int(np.argmax(seg)) + t0              # 1-based
                 lef, rig = max(loc - halftime, 1), min(loc + halftime, nt)
             else:
+                if not wlist:
+                    break                                   # the stored list is exhausted
                 lef, rig = wlist.pop(0)
                 loc, timest, timefin = 0, 0, 0
             tau = tt_gizon_birch(accf[:, i].real, datf[:, i].real, dt_min, lef - 1, rig - 1)
@@ -361,3 +363,142 @@ def read_mode_params(directory: Path) -> tuple[dict, dict]:
         if f.exists():
             freqs[key] = tuple(float(t) for t in f.read_text().split()[:2])
     return modes, freqs
+
+
+def adjoint_source_filt_torch(acc, dat, x: np.ndarray, x00: float, xlength_cm: float, outputcad: float,
+                              dt_min: float, mode_params: dict, freq_params: dict | None = None,
+                              windows: dict | None = None, device="cuda") -> AdjointSource:
+    """adjoint_source_filt on a torch device (the GPU by default), every
+    receiver of a mode at once.
+
+    The same measurements as the loop above: filtered traces by 2D FFTs,
+    the window of each receiver around the largest filtered synthetic sample
+    in its search range (first maximum), Gizon & Birch travel times with the
+    time derivative by a batched real FFT and SIMPSON_REGULAR weights, and
+    the adjoint source.  The one change of order: the per-receiver terms
+    Re[F^-1(filter * c_i(nu) e^{-i k x_i})] * con_i of a mode are summed
+    before the inverse transform, as filter * (C E) with C[nu][i] = con_i c_i(nu)
+    and E[i][k] = e^{-i k x_i} (one matrix product and one inverse FFT per
+    mode instead of one inverse FFT per receiver); that changes rounding
+    only.  acc, dat: [nt][nx] arrays or tensors; the filters and the
+    per-receiver search ranges are built on the host as above."""
+    import torch
+
+    dev = torch.device(device)
+    acc_t = torch.as_tensor(np.asarray(acc, dtype=np.float64) if not torch.is_tensor(acc) else acc,
+                            dtype=torch.float64, device=dev)
+    dat_t = torch.as_tensor(np.asarray(dat, dtype=np.float64) if not torch.is_tensor(dat) else dat,
+                            dtype=torch.float64, device=dev)
+    nt, nx = acc_t.shape
+    freq_params = freq_params or {}
+    distances = np.abs((x - 0.5) * xlength_cm * f32(1e-8) - x00)
+    eyekh = torch.as_tensor(1j * distmat(nx) * 2.0 * PI, device=dev)
+    freqnu = torch.as_tensor(distmat(nt) / (nt * dt_min * 60.0), device=dev)
+    filt = np.ones(nt)
+    if "allmodes" in freq_params:
+        filt = freq_filter(*freq_params["allmodes"], nt, outputcad)
+    filters = {m: ridge_filter(name, nt, nx, xlength_cm, outputcad) * filt[:, None]
+               for m, name in _MODE_NAMES.items()}
+    for m in (0, 1, 2):
+        if m in freq_params:
+            filt = freq_filter(*freq_params[m], nt, outputcad)
+        filters[m] = filters[m] * filt[:, None]
+    scale = nt * nx                                                   # FFTW's unnormalised backward
+    tempout = torch.fft.fft2(acc_t.to(torch.complex128))
+    tempdat = torch.fft.fft2(dat_t.to(torch.complex128))
+    # SIMPSON_REGULAR weights (1, 4, 2, 4, ..., 1 by 1-based parity) and the
+    # time-derivative multiplier of COMPUTE_TT_GIZONBIRCH
+    wts = torch.where(torch.arange(1, nt + 1, device=dev) % 2 == 0, 4.0, 2.0).to(torch.float64)
+    wts[0] = 1.0
+    wts[-1] = 1.0
+    nr = nt // 2 + 1
+    kk = torch.arange(nr, device=dev, dtype=torch.float64)
+    dmul = torch.where(kk < nt // 2, 1j * 2 * PI * kk / (nt * dt_min), torch.zeros_like(kk) * 1j)
+    tt = torch.arange(nt, device=dev)
+    adj = torch.zeros((nt, nx), dtype=torch.float64, device=dev)
+    misfit = 0.0
+    nmeas = 0
+    meas = []
+    for pord in range(9):
+        if pord not in mode_params:
+            continue
+        mindist, maxdist, window = mode_params[pord]
+        halftime = int(math.floor(window / (2.0 * dt_min) + 0.5))
+        fm = torch.as_tensor(filters[pord], device=dev)
+        accf = torch.fft.ifft2(tempout * fm) * scale
+        datf = torch.fft.ifft2(tempdat * fm) * scale
+        ccdot = torch.fft.ifft2(tempout * fm * (1j * freqnu * 2.0 * PI)[:, None]) * scale
+        # receivers, search ranges (1-based inclusive) and windows on the host
+        rec, lefs, rigs, locs, tsts, tfins = [], [], [], [], [], []
+        wlist = list(windows.get(pord, [])) if windows else None
+        cand = [i for i in range(nx) if distances[i] > mindist and distances[i] < maxdist]
+        if wlist is None:
+            t0s, t1s, ts_, tf_ = [], [], [], []
+            for i in cand:
+                if pord != 8:
+                    timest = int(math.floor(distances[i] / f32(_VEL[pord]) * 1.0 / dt_min))
+                    timefin = timest + 40
+                else:
+                    dist = distances[i]
+                    timest = int(math.floor((f32(-0.000417147774671) * dist ** 2 + f32(0.313350998096) * dist
+                                             + f32(17.9609631186)) * 1.0 / dt_min))
+                    timefin = int(math.floor((f32(-0.00028361249034) * dist ** 2 + f32(0.29270337114) * dist
+                                              + f32(36.4398734578)) * 1.0 / dt_min))
+                t0, t1 = max(timest, 1), min(timefin, nt)
+                if t0 > t1:
+                    continue
+                rec.append(i)
+                t0s.append(t0)
+                t1s.append(t1)
+                ts_.append(timest)
+                tf_.append(timefin)
+            if rec:
+                ri = torch.as_tensor(rec, device=dev)
+                a0 = torch.as_tensor(t0s, device=dev)[None, :]
+                a1 = torch.as_tensor(t1s, device=dev)[None, :]
+                inrange = (tt[:, None] + 1 >= a0) & (tt[:, None] + 1 <= a1)
+                mag = torch.where(inrange, accf[:, ri].real.abs(), torch.full_like(accf[:, ri].real, -1.0))
+                loc = (torch.argmax(mag, dim=0) + 1).cpu().tolist()       # first maximum, 1-based
+                for j, i in enumerate(rec):
+                    lefs.append(max(loc[j] - halftime, 1))
+                    rigs.append(min(loc[j] + halftime, nt))
+                    locs.append(loc[j])
+                    tsts.append(ts_[j])
+                    tfins.append(tf_[j])
+        else:
+            for i in cand:
+                if not wlist:
+                    break
+                lef, rig = wlist.pop(0)
+                rec.append(i)
+                lefs.append(lef)
+                rigs.append(rig)
+                locs.append(0)
+                tsts.append(0)
+                tfins.append(0)
+        if not rec:
+            continue
+        ri = torch.as_tensor(rec, device=dev)
+        lt = torch.as_tensor(lefs, device=dev)[None, :]
+        rt = torch.as_tensor(rigs, device=dev)[None, :]
+        win = ((tt[:, None] + 1 >= lt) & (tt[:, None] + 1 <= rt)).to(torch.float64)     # [nt][nrec]
+        u0 = accf[:, ri].real
+        u = datf[:, ri].real
+        u0dot = torch.fft.irfft(torch.fft.rfft(u0, dim=0) * dmul[:, None], n=nt, dim=0)
+        num = (wts[:, None] * (win * u0dot * (u0 - u))).sum(0) * dt_min / 3.0
+        den = (wts[:, None] * (win * u0dot ** 2)).sum(0) * dt_min / 3.0
+        tau = num / den                                                   # [nrec], minutes
+        cc = ccdot[:, ri]
+        ccdotone = torch.fft.fft(cc * win, dim=0)                          # [nt][nrec]
+        con = -tau / ((cc * cc * win).sum(0) * dt_min)                     # complex, as the reference's sum
+        xr = torch.as_tensor(x[rec], device=dev)
+        E = torch.exp(-eyekh[None, :] * xr[:, None])                      # [nrec][nx]
+        spec = fm * ((ccdotone * con[None, :]) @ E)
+        adj = adj + torch.flip((torch.fft.ifft2(spec) * scale).real, dims=[0])
+        tau_h = tau.cpu().tolist()
+        for j, i in enumerate(rec):
+            meas.append(Measurement(pord, i, tau_h[j] * 60.0, lefs[j], rigs[j], locs[j], tsts[j], tfins[j]))
+            misfit += tau_h[j] ** 2.0
+            nmeas += 1
+    adj = adj * nt / 60.0
+    return AdjointSource(adj=adj.cpu().numpy(), misfit=0.5 * misfit, nmeas=nmeas, measurements=meas)

@@ -182,7 +182,7 @@ def _adjoint_source(p, ddir, contrib, jobno, vz, cadence_s, linesearch, compute_
     (driver.f90:271-276): travel times against data/SS.fits, ttdiff.<mode>,
     windows.<mode>, the adjoint source, the misfit file and the per-mode,
     per-band misfit_all file with its windows.all.MM.B."""
-    from .adjsrc import MISFIT_BANDS, adjoint_source_filt, misfit_all, read_mode_params
+    from .adjsrc import MISFIT_BANDS, adjoint_source_filt_torch, misfit_all, read_mode_params
 
     data = ddir / "data" / f"{contrib}.fits"
     if not data.exists():
@@ -200,8 +200,8 @@ def _adjoint_source(p, ddir, contrib, jobno, vz, cadence_s, linesearch, compute_
             if f.exists():
                 v = [int(t) for t in f.read_text().split()]
                 windows[m] = list(zip(v[0::2], v[1::2]))
-    r = adjoint_source_filt(vz, dat, np.arange(p.nx) / p.nx, _xsrc(ddir, contrib), p.xlength, p.outputcad,
-                            cadence_s / 60.0, modes, freqs, windows)
+    r = adjoint_source_filt_torch(vz, dat, np.arange(p.nx) / p.nx, _xsrc(ddir, contrib), p.xlength,
+                                  p.outputcad, cadence_s / 60.0, modes, freqs, windows, device="cuda")
     out = ddir / f"forward_src{contrib}_ls{jobno}"
     for m in modes:
         rows = [x for x in r.measurements if x.mode == m]

@@ -15,6 +15,7 @@ tests pin the pieces by what the mathematics fixes:
   directions.
 """
 import numpy as np
+import pytest
 
 from sparc_flows_amd import adjsrc as A
 
@@ -170,3 +171,47 @@ def test_misfit_all_file_reads_with_loadtxt(tmp_path):
     for b in range(3):
         assert np.allclose(v[b], ma.bands[b][3], rtol=1e-15)
     assert np.all(v[:, 1:8] == 0) and np.all(v[:, 9:] == 0)
+
+
+def _compare_torch(device):
+    """adjoint_source_filt_torch against adjoint_source_filt: the same
+    measurements (receivers, windows, search ranges exactly; travel times to
+    1e-12) and the same adjoint source to 1e-11, for the f mode, p1 and all
+    p modes with frequency filters, with picked and with stored windows
+    (including a list that runs out)."""
+    dt_min = 0.5
+    acc, dat, x, t = _traces(nt=240, nx=64, dt_min=dt_min, delay=0.05)
+    modes = {0: (3.0, 40.0, 20.0), 1: (5.0, 50.0, 25.0), 8: (5.0, 60.0, 30.0)}
+    fp = {"allmodes": (1.0, 7.0), 1: (2.0, 6.0)}
+    cases = [dict(), dict(freq_params=fp)]
+    ref0 = A.adjoint_source_filt(acc, dat, x, 1.5, 200e8, dt_min * 60.0, dt_min, modes)
+    wins = {m: [(q.lef, q.rig) for q in ref0.measurements if q.mode == m] for m in modes}
+    wins[8] = wins[8][:-3]                                # a stored list shorter than the receivers
+    cases.append(dict(windows=wins))
+    for kw in cases:
+        kwn = {k: ({m: list(v) for m, v in w.items()} if k == "windows" else w) for k, w in kw.items()}
+        kwt = {k: ({m: list(v) for m, v in w.items()} if k == "windows" else w) for k, w in kw.items()}
+        ref = A.adjoint_source_filt(acc, dat, x, 1.5, 200e8, dt_min * 60.0, dt_min, modes, **kwn)
+        got = A.adjoint_source_filt_torch(acc, dat, x, 1.5, 200e8, dt_min * 60.0, dt_min, modes, device=device,
+                                          **kwt)
+        assert ref.nmeas > 10 and got.nmeas == ref.nmeas
+        for a, b in zip(ref.measurements, got.measurements):
+            assert (a.mode, a.pixel, a.lef, a.rig, a.loc, a.timest, a.timefin) == \
+                (b.mode, b.pixel, b.lef, b.rig, b.loc, b.timest, b.timefin)
+            assert abs(a.tau_s - b.tau_s) <= 1e-12 * max(abs(a.tau_s), 1e-3)
+        assert abs(got.misfit - ref.misfit) <= 1e-12 * ref.misfit
+        err = np.abs(got.adj - ref.adj).max() / np.abs(ref.adj).max()
+        print(device, sorted(kw), "adjoint source max rel", err)
+        assert err < 1e-11
+
+
+def test_torch_adjoint_source_matches_numpy_on_cpu():
+    _compare_torch("cpu")
+
+
+@pytest.mark.gpu
+def test_torch_adjoint_source_matches_numpy_on_gpu():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    _compare_torch("cuda")

@@ -71,7 +71,6 @@ def _cli(tmp: Path, data: Path, maxtime: int, *args):
     return r.returncode
 
 
-@pytest.mark.gpu
 def _check_text_outputs(p, bg, psi, d, maxtime, adjoint, vr=None):
     """timeline, kernel_info and rms_hist of a forward or adjoint phase:
     * timeline: time and solar time of every output step (driver.f90:386-390),
@@ -106,6 +105,7 @@ def _check_text_outputs(p, bg, psi, d, maxtime, adjoint, vr=None):
     assert np.abs(rms - ref).max() <= 1e-9 * np.abs(ref).max()
 
 
+@pytest.mark.gpu
 def test_forward_adjoint_kernel_phases(tmp_path):
     torch = pytest.importorskip("torch")
     if not torch.cuda.is_available():
@@ -133,13 +133,16 @@ def test_forward_adjoint_kernel_phases(tmp_path):
     assert np.array_equal(np.fromfile(fdir / "vz_cc.bin", dtype="<f8").reshape(vz.shape), vz)
     assert (fdir / "kernel_info").exists() and (fdir / f"xix_{0:06d}_partial.fits").exists()
     _check_text_outputs(p, bg, psi, fdir, maxtime, adjoint=False)
-    from sparc_flows_amd.adjsrc import adjoint_source_filt
-    r = adjoint_source_filt(vz, vz_obs[0].cpu().numpy(), np.arange(p.nx) / p.nx, -12.0, p.xlength,
-                            p.outputcad, p.steps * p.timestep / 60.0, {8: (3.0, 40.0, 10.0)})
+    from sparc_flows_amd.adjsrc import adjoint_source_filt, adjoint_source_filt_torch
+    args = (vz, vz_obs[0].cpu().numpy(), np.arange(p.nx) / p.nx, -12.0, p.xlength, p.outputcad,
+            p.steps * p.timestep / 60.0, {8: (3.0, 40.0, 10.0)})
+    r = adjoint_source_filt_torch(*args, device="cuda")
     assert r.nmeas > 0
     adir = data / "adjoint_src01"
     vr = read_fits(adir / "source.fits")[:, 0, :]
-    assert np.array_equal(vr, r.adj)
+    assert np.array_equal(vr, r.adj)                   # the GPU routine the program ran
+    rn = adjoint_source_filt(*args)                    # the numpy restatement
+    assert rn.nmeas == r.nmeas and np.abs(vr - rn.adj).max() <= 1e-11 * np.abs(rn.adj).max()
     assert (fdir / "ttdiff.8").read_text().count("\n") == r.nmeas
     assert (data / "kernel" / "misfit_01_00").exists() and (adir / "information").exists()
     # MISFIT_ALL: three bands of 11 half misfits, and the band windows it picked
