@@ -2,13 +2,15 @@
 (grad.py:1-433): the step between the kernels of all sources and the
 line-search models of the next iteration.
 
-    python -m sparc_flows_amd.update [algo=sd|cg|LBFGS] [eps1 eps2 ...]
+    python -m sparc_flows_amd.update [algo=sd|cg|LBFGS] [device=cuda] [eps1 eps2 ...]
 
 run in the directory that holds ``params.i`` (or with SPARC_DIRECTORY set),
 as ``python grad.py`` is.  Arrays here are numpy (nz, nx) -- the reference
 transposes its FITS arrays to (nx, 1, nz) (grad.py:12-27); every function
-states its axis.  The work is a few passes over nx*nz doubles per iteration,
-so it runs on the host.
+states its axis.  The work is a few passes over nx*nz doubles per iteration;
+it runs on the host with numpy, or on a torch device (device=cuda: the *_t
+functions, the same steps on GPU tensors, for an inversion loop whose
+kernels are already on the GPU).
 
 What it does, in the reference's order:
 
@@ -216,6 +218,140 @@ def linesearch_model(model: np.ndarray, update: np.ndarray, eps: float, kind: st
 
 
 # --------------------------------------------------------------------------
+# The same steps on a torch device (the GPU in the inversion loop, where the
+# kernels already are): every function above with a torch tensor in and out,
+# the same arithmetic in the same order of operations except for the order
+# of the sums (rounding only).
+
+
+def _t(a, device):
+    import torch
+    return torch.as_tensor(np.asarray(a, dtype=np.float64) if not torch.is_tensor(a) else a,
+                           dtype=torch.float64, device=device)
+
+
+def filterx_t(kern, nk: float = 30.0):
+    import torch
+    nx = kern.shape[-1]
+    k = torch.arange(nx // 2 + 1, dtype=torch.float64, device=kern.device)
+    filt = torch.exp(-k ** 2 / (2.0 * nk ** 2))
+    return torch.fft.irfft(torch.fft.rfft(kern, dim=-1) * filt, n=nx, dim=-1)
+
+
+def antisymmetrize_t(a):
+    return 0.5 * (a - a.flip(-1))
+
+
+def symmetrize_t(a):
+    return 0.5 * (a + a.flip(-1))
+
+
+def gaussian_filter_z_t(a, sigma: float, truncate: float = 4.0):
+    import torch
+    r = int(truncate * float(sigma) + 0.5)
+    t = np.arange(-r, r + 1, dtype=np.float64)
+    w = np.exp(-0.5 * t ** 2 / float(sigma) ** 2)
+    w /= w.sum()
+    nz = a.shape[0]
+    m = np.mod(np.arange(-r, nz + r), 2 * nz)
+    src = torch.as_tensor(np.where(m < nz, m, 2 * nz - 1 - m), device=a.device)
+    pad = a[src]
+    out = torch.zeros_like(a)
+    for q in range(2 * r + 1):
+        out = out + float(w[q]) * pad[q:q + nz]
+    return out
+
+
+def smooth_z_t(a):
+    c = _SMOOTH
+    nz = a.shape[0]
+    temp = a.new_zeros(a.shape)
+    k = slice(5, nz - 5)
+    sh = lambda d: a[5 + d:nz - 5 + d]
+    temp[k] = (c[0] * sh(0) + 0.5 * c[1] * (sh(-1) + sh(1)) + 0.5 * c[2] * (sh(-2) + sh(2))
+               + 0.5 * c[3] * (sh(-3) + sh(3)) + 0.5 * c[4] * (sh(-4) + sh(4))
+               + 0.5 * c[5] * (sh(-5) + sh(5)))
+    temp[nz - 5:nz - 1] = 0.5 * (a[nz - 4:] + a[nz - 6:nz - 2])
+    temp[1:4] = 0.5 * (a[:3] + a[2:5])
+    out = a.clone()
+    out[1:nz] = temp[1:nz]
+    return out
+
+
+def filter_and_symmetrize_t(totkern, hess, sym: str | None = None, z_filt_algo: str = "gaussian",
+                            z_filt_pix: float = 0.3):
+    kern = filterx_t(totkern / hess, 30.0)
+    if sym == "sym":
+        kern = symmetrize_t(kern)
+    elif sym == "asym":
+        kern = antisymmetrize_t(kern)
+    if z_filt_algo == "gaussian":
+        return gaussian_filter_z_t(kern, z_filt_pix)
+    if z_filt_algo == "smooth":
+        return smooth_z_t(kern)
+    raise ValueError(f"z filter {z_filt_algo!r} is not available (gaussian, smooth)")
+
+
+def preconditioner_t(hessians, rho0, floor: float = 5e-3):
+    import torch
+    h = torch.zeros_like(hessians[0])
+    for H in hessians:
+        h = h + H.abs()
+    h = h * rho0[:, None]
+    h = h / h.abs().max()
+    return torch.where(h < floor, torch.full_like(h, floor), h)
+
+
+def gradient_psi_t(kernels_psi, hessians, rho0, device="cuda"):
+    """gradient_psi on a torch device; inputs arrays or tensors."""
+    import torch
+    K = [_t(k, device) for k in kernels_psi]
+    tot = torch.zeros_like(K[0])
+    for k in K:
+        tot = tot + k
+    return filter_and_symmetrize_t(tot, preconditioner_t([_t(h, device) for h in hessians], _t(rho0, device)),
+                                   sym="asym", z_filt_algo="gaussian", z_filt_pix=2.0)
+
+
+def cg_update_t(grad, lastgrad, lastupdate):
+    beta = (grad * (grad - lastgrad)).sum() / (lastgrad ** 2).sum()
+    return -grad + beta * lastupdate
+
+
+def lbfgs_direction_t(grad, models, grads, m: int = 4):
+    s_list, y_list = [], []
+    for i in range(max(len(models) - 1 - m, 0), len(models) - 1):
+        s_list.append(models[i + 1] - models[i])
+        y_list.append(grads[i + 1] - grads[i])
+    q = grad.clone()
+    alphas, rhos = [], []
+    for s, y in zip(reversed(s_list), reversed(y_list)):
+        rho = 1.0 / (y * s).sum()
+        a = rho * (s * q).sum()
+        q = q - a * y
+        alphas.append(a)
+        rhos.append(rho)
+    if s_list:
+        s, y = s_list[-1], y_list[-1]
+        q = q * ((s * y).sum() / (y * y).sum())
+    for (s, y), a, rho in zip(zip(s_list, y_list), reversed(alphas), reversed(rhos)):
+        b = rho * (y * q).sum()
+        q = q + s * (a - b)
+    return -q
+
+
+def linesearch_model_t(model, update, eps: float, kind: str = "linear"):
+    import torch
+    if kind != "linear":
+        raise ValueError("only the linear line-search model is defined (grad.py:350-355)")
+    umax = update.max()
+    u = torch.where(umax != 0, update / torch.where(umax != 0, umax, torch.ones_like(umax)), update)
+    scale = torch.sqrt((model ** 2).sum() / model.numel())
+    scale = torch.where(scale == 0, torch.ones_like(scale), scale)
+    return model + eps * u * scale
+
+
+# --------------------------------------------------------------------------
 # The grad.py driver over the reference's directory layout
 
 
@@ -233,12 +369,24 @@ def iteration_number(datadir: Path) -> int:
     return len(fnmatch.filter(os.listdir(datadir / "update"), "misfit_[0-9][0-9]")) - 1
 
 
-def run(datadir, rho0: np.ndarray, algo: str = "conjugate gradient", eps=None, m: int = 4) -> dict:
+def run(datadir, rho0: np.ndarray, algo: str = "conjugate gradient", eps=None, m: int = 4,
+        device=None) -> dict:
     """One pass of grad.py over `datadir`: reads kernel/kernel_psi_SS.fits,
     kernel/hessian_SS.fits, master.pixels and update/model_psi_NN.fits (and
     the previous gradient/update for CG, all previous models/gradients for
     L-BFGS); writes update/gradient_psi_NN.fits, update/update_psi_NN.fits,
-    update/test_psi_N.fits and epslist.npz.  Returns what it wrote."""
+    update/test_psi_N.fits and epslist.npz.  Returns what it wrote (numpy
+    arrays).  device: a torch device to do the array work on (the *_t
+    functions), or None for numpy."""
+    if device is None:
+        grad_fn, cg_fn, lb_fn, ls_fn = gradient_psi, cg_update, lbfgs_direction, linesearch_model
+        conv = back = (lambda a: a)
+    else:
+        def grad_fn(K, H, r):
+            return gradient_psi_t(K, H, r, device)
+        cg_fn, lb_fn, ls_fn = cg_update_t, lbfgs_direction_t, linesearch_model_t
+        conv = (lambda a: _t(a, device))
+        back = (lambda a: a.cpu().numpy())
     datadir = Path(datadir)
     upd = datadir / "update"
     it = iteration_number(datadir)
@@ -249,26 +397,27 @@ def run(datadir, rho0: np.ndarray, algo: str = "conjugate gradient", eps=None, m
     eps = [float(e) for e in eps]
     kern = [_rd(datadir / "kernel" / f"kernel_psi_{s:02d}.fits") for s in range(1, nsrc + 1)]
     hess = [_rd(datadir / "kernel" / f"hessian_{s:02d}.fits") for s in range(1, nsrc + 1)]
-    g = gradient_psi(kern, hess, rho0)
-    _wr(upd / f"gradient_psi_{it:02d}.fits", g)
+    g = grad_fn(kern, hess, rho0)
+    _wr(upd / f"gradient_psi_{it:02d}.fits", back(g))
     sd = algo in ("sd", "steepest descent")
     cg = algo in ("cg", "conjugate gradient")
     if it == 0 or sd:
-        u = sd_update(g)
+        u = -g
     elif cg:
-        u = cg_update(g, _rd(upd / f"gradient_psi_{it - 1:02d}.fits"), _rd(upd / f"update_psi_{it - 1:02d}.fits"))
+        u = cg_fn(g, conv(_rd(upd / f"gradient_psi_{it - 1:02d}.fits")),
+                  conv(_rd(upd / f"update_psi_{it - 1:02d}.fits")))
     elif algo == "LBFGS":
         lo = max(it - m, 0)
-        models = [_rd(upd / f"model_psi_{i:02d}.fits") for i in range(lo, it + 1)]
-        grads = [_rd(upd / f"gradient_psi_{i:02d}.fits") for i in range(lo, it + 1)]
-        u = lbfgs_direction(g, models, grads, m)
+        models = [conv(_rd(upd / f"model_psi_{i:02d}.fits")) for i in range(lo, it + 1)]
+        grads = [conv(_rd(upd / f"gradient_psi_{i:02d}.fits")) for i in range(lo, it + 1)]
+        u = lb_fn(g, models, grads, m)
     else:
         raise ValueError(f"unknown algorithm {algo!r}")
-    _wr(upd / f"update_psi_{it:02d}.fits", u)
-    model = _rd(upd / f"model_psi_{it:02d}.fits")
+    _wr(upd / f"update_psi_{it:02d}.fits", back(u))
+    model = conv(_rd(upd / f"model_psi_{it:02d}.fits"))
     tests = []
     for i, e in enumerate(eps):
-        t = linesearch_model(model, u, e)
+        t = back(ls_fn(model, u, e))
         _wr(upd / f"test_psi_{i + 1}.fits", t)
         tests.append(t)
     # epslist.npz: per iteration a 4 x nls array, row 0 the eps of the newest
@@ -289,7 +438,7 @@ def run(datadir, rho0: np.ndarray, algo: str = "conjugate gradient", eps=None, m
         arr[0] = eps
     lists[key] = arr
     np.savez(path, **lists)
-    return {"iteration": it, "gradient": g, "update": u, "tests": tests, "eps": eps}
+    return {"iteration": it, "gradient": back(g), "update": back(u), "tests": tests, "eps": eps}
 
 
 def main(argv=None) -> int:
@@ -299,9 +448,12 @@ def main(argv=None) -> int:
     argv = sys.argv[1:] if argv is None else argv
     algo = "conjugate gradient"
     eps = []
+    device = None
     for a in argv:
         if a.startswith("algo="):
             algo = a.split("=", 1)[1]
+        elif a.startswith("device="):
+            device = a.split("=", 1)[1]          # e.g. device=cuda: the array work on the GPU
         else:
             try:
                 eps.append(float(a))
@@ -310,7 +462,7 @@ def main(argv=None) -> int:
     p = read_params_i("params.i")
     datadir = Path(os.environ.get("SPARC_DIRECTORY") or p.directory or ".")
     bg = read_model_file(p.file_data, p.nz)
-    r = run(datadir, bg[:, 2], algo=algo, eps=eps)
+    r = run(datadir, bg[:, 2], algo=algo, eps=eps, device=device)
     print(f"iteration {r['iteration']}: gradient, update and {len(r['tests'])} line-search models written")
     return 0
 

@@ -161,3 +161,75 @@ def test_directory_driver(tmp_path):
     assert np.allclose(r1["update"], -g1 + beta * u)
     with pytest.raises(ValueError):
         U.run(tmp_path, rho, algo="newton", eps=[0.01])
+
+
+def _compare_device(device, tmp_path):
+    """The torch versions (*_t) and the driver on a torch device against the
+    numpy functions: same results to 1e-13 (order of sums only), for the psi
+    gradient, both z filters, CG, L-BFGS, the line-search models and two
+    iterations of the directory driver with each algorithm."""
+    import torch
+    rng = np.random.default_rng(11)
+    nz, nx = 40, 64
+    K = [rng.standard_normal((nz, nx)) for _ in range(3)]
+    H = [rng.standard_normal((nz, nx)) for _ in range(3)]
+    rho = np.linspace(2.0, 0.2, nz)
+    g = U.gradient_psi(K, H, rho)
+    gt = U.gradient_psi_t(K, H, rho, device)
+    assert gt.device.type == torch.device(device).type
+    close = lambda a, b: np.abs(a.cpu().numpy() - b).max() <= 1e-13 * np.abs(b).max()
+    assert close(gt, g)
+    a = rng.standard_normal((nz, nx))
+    at = torch.as_tensor(a, device=device)
+    assert close(U.smooth_z_t(at), U.smooth_z(a))
+    assert close(U.filter_and_symmetrize_t(at, at.abs() + 1.0, "sym", "smooth"),
+                 U.filter_and_symmetrize(a, np.abs(a) + 1.0, "sym", "smooth"))
+    g0, u0 = rng.standard_normal((nz, nx)), rng.standard_normal((nz, nx))
+    T = lambda v: torch.as_tensor(v, device=device)
+    assert close(U.cg_update_t(T(g), T(g0), T(u0)), U.cg_update(g, g0, u0))
+    models = [rng.standard_normal((nz, nx)) for _ in range(6)]
+    grads = [rng.standard_normal((nz, nx)) for _ in range(6)]
+    assert close(U.lbfgs_direction_t(T(g), [T(v) for v in models], [T(v) for v in grads], 4),
+                 U.lbfgs_direction(g, models, grads, 4))
+    assert close(U.linesearch_model_t(T(a), T(g), 0.02), U.linesearch_model(a, g, 0.02))
+    # the directory driver, numpy and device runs in two copies of one layout
+    for algo in ("cg", "LBFGS"):
+        outs = []
+        for dev in (None, device):
+            d = tmp_path / f"{algo}_{dev}"
+            (d / "kernel").mkdir(parents=True)
+            (d / "update").mkdir()
+            (d / "master.pixels").write_text("-10.0\n15.0\n")
+            r2 = np.random.default_rng(5)
+            for s in (1, 2):
+                write_fits(d / "kernel" / f"kernel_psi_{s:02d}.fits", r2.standard_normal((nz, 1, nx)))
+                write_fits(d / "kernel" / f"hessian_{s:02d}.fits", r2.standard_normal((nz, 1, nx)))
+            model = r2.standard_normal((nz, nx))
+            write_fits(d / "update" / "model_psi_00.fits", model[:, None, :])
+            (d / "update" / "misfit_00").write_text("0\n")
+            U.run(d, rho, algo=algo, eps=[0.01, 0.02], device=dev)
+            for s in (1, 2):                         # the next iteration's kernels
+                write_fits(d / "kernel" / f"kernel_psi_{s:02d}.fits", r2.standard_normal((nz, 1, nx)))
+            (d / "update" / "misfit_01").write_text("0\n")
+            # a model step that is not orthogonal to the gradient change (a
+            # constant step makes s.y ~ 0 and the L-BFGS scaling ill-conditioned)
+            write_fits(d / "update" / "model_psi_01.fits",
+                       (model + 0.1 * r2.standard_normal((nz, nx)))[:, None, :])
+            outs.append(U.run(d, rho, algo=algo, eps=[0.01, 0.02], device=dev))
+        ref, got = outs
+        for key in ("gradient", "update"):
+            assert np.abs(got[key] - ref[key]).max() <= 1e-12 * np.abs(ref[key]).max(), (algo, key)
+        for t0, t1 in zip(ref["tests"], got["tests"]):
+            assert np.abs(t1 - t0).max() <= 1e-12 * np.abs(t0).max()
+
+
+def test_torch_update_matches_numpy_on_cpu(tmp_path):
+    _compare_device("cpu", tmp_path)
+
+
+@pytest.mark.gpu
+def test_torch_update_matches_numpy_on_gpu(tmp_path):
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    _compare_device("cuda", tmp_path)
