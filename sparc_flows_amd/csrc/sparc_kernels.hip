@@ -242,7 +242,7 @@ __global__ __launch_bounds__(64) void k_row(Dims d, Bg bg, State st, RowArgs ra,
 #pragma unroll
             for (int j2 = 0; j2 < N2; j2++) {
                 const int x = t + N1 * j2;
-                const double v = ra.filt_vx ? filter_vars_2d_at(st.vx_unf, rowq + x, k, x, NX, nz, nh)
+                const double v = ra.filt_vx ? filter_vars_2d_sel(st.vx_unf, rowq + x, k, x, NX, nz, nh)
                                             : st.a[2][rowq + x];
                 vx[j2] = v;
                 re[j2] = ra.xi_x[rowq + x];
