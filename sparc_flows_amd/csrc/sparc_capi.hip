@@ -1309,9 +1309,11 @@ static int autotune(sparc_handle *h)
     hipEvent_t e0, e1;
     HCHECK(h, hipEventCreate(&e0));
     HCHECK(h, hipEventCreate(&e1));
-    float best = 1e30f;
-    SegTab best_sg = h->seg;
-    int best_tc = 0;
+    // the launchable shapes, then three timing rounds over all of them (a
+    // shape's time is its fastest round, so that a shape timed while the
+    // device warms up or another queue runs is not passed over)
+    struct Cand { SegTab sg; int tc; float t; };
+    std::vector<Cand> cands;
     for (int m : ms) {
         SegTab sg{};
         bool have = false;
@@ -1329,15 +1331,24 @@ static int autotune(sparc_handle *h)
                 (void)hipGetLastError();     // shape not launchable here (LDS): skip it
                 continue;
             }
+            cands.push_back(Cand{sg, tc, 1e30f});
+        }
+    }
+    for (int round = 0; round < 3; round++)
+        for (Cand &cd : cands) {
             HCHECK(h, hipEventRecord(e0, h->stream));
-            for (int r = 0; r < 10; r++) HCHECK(h, launch_col(h, tc, m, sg, ca, clk, sk));
+            for (int r = 0; r < 10; r++) HCHECK(h, launch_col(h, cd.tc, cd.sg.m, cd.sg, ca, clk, sk));
             HCHECK(h, hipEventRecord(e1, h->stream));
             HCHECK(h, hipEventSynchronize(e1));
             float t = 0.f;
             HCHECK(h, hipEventElapsedTime(&t, e0, e1));
-            if (t < best) { best = t; best_sg = sg; best_tc = tc; }
+            cd.t = std::min(cd.t, t);
         }
-    }
+    float best = 1e30f;
+    SegTab best_sg = h->seg;
+    int best_tc = 0;
+    for (const Cand &cd : cands)
+        if (cd.t < best) { best = cd.t; best_sg = cd.sg; best_tc = cd.tc; }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (!best_tc) return fail(h, SPARC_E_ARG, "no column-pass block shape fits this grid");
