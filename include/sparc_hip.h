@@ -40,7 +40,7 @@ extern "C" {
  * initialize.f90:743-747) and the optional 2D sound speed and stream function
  * (model_c_lsLL.fits / model_psi_lsLL.fits, driver.f90:94-126). */
 typedef struct sparc_config {
-    int nx, nz;               /* nx: power of two in [16, 1024] */
+    int nx, nz;               /* nx in [16, 1024]: FFT rows for a power of two, the circulant form otherwise */
     int npmlbot, npmltop, npmlhor;
     int nsrc;                 /* sources advanced together in one batch */
     int adjoint;              /* 0: forward ('spectral'), 1: adjoint ('adjoint') */

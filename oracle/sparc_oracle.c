@@ -9,7 +9,8 @@
  * Parity pins (tests/test_oracle.py):
  *   orc_dbyd2 / mtridag  - bit-compared against the reference dbyd2.f +
  *                          mtridag.f compiled from source (oracle/_ref);
- *   orc_ddx / orc_fft    - closed forms (Fourier modes, Nyquist, constants);
+ *   orc_ddx / orc_fft    - closed forms (Fourier modes, Nyquist, constants) and
+ *                          numpy's FFT, at power-of-two and other lengths;
  *   filters, RK, Lagrange - brute force / closed forms;
  *   orc_forward_source_series - the series' spectrum against the closed-form
  *                          transform of the continuous pulse;
@@ -194,8 +195,73 @@ void orc_dbyd2(double *A, const double *B, int N1, int N2, int ibc)
 /* Radix-2 complex FFT, sum_j x_j exp(sign*2*pi*i*j*k/n), unnormalized.
  * Stands in for FFTW 3.2's guru r2c/c2r plans (initialize.f90:325-328),
  * which compute the same DFT up to rounding. */
+/* DFT for n that is not a power of two (FFTW plans any n; the transform is
+ * the same sum): mixed-radix Cooley-Tukey, decimation in time, splitting
+ * off the smallest prime factor p of n = p m:
+ *   X[k + m q] = sum_{r<p} W_n^{r (k + m q)} Y_r[k],  Y_r = DFT_m(x[r + p j]),
+ * with a prime n summed directly.  W_N^e for a sub-transform of length N
+ * is the table's W_n0^{e n0/N}; every exponent is reduced exactly. */
+static void fft_mixed(const double *xr, const double *xi, int n, int stride, const double *wr,
+                      const double *wi, int n0, double *yr, double *yi)
+{
+    if (n == 1) { yr[0] = xr[0]; yi[0] = xi[0]; return; }
+    int p = 2;
+    while (p * p <= n && n % p) p++;
+    if (n % p) p = n;                               /* n is prime */
+    int m = n / p, sc = n0 / n;
+    if (p == n) {
+        for (int k = 0; k < n; k++) {
+            double ar = 0.0, ai = 0.0;
+            for (int j = 0, e = 0; j < n; j++) {
+                double c = wr[e * sc], sn = wi[e * sc];
+                ar += xr[j * stride] * c - xi[j * stride] * sn;
+                ai += xr[j * stride] * sn + xi[j * stride] * c;
+                e += k; if (e >= n) e -= n;
+            }
+            yr[k] = ar; yi[k] = ai;
+        }
+        return;
+    }
+    double tr[n], ti[n];
+    for (int r = 0; r < p; r++)
+        fft_mixed(xr + r * stride, xi + r * stride, m, stride * p, wr, wi, n0, tr + r * m, ti + r * m);
+    for (int q = 0; q < p; q++)
+        for (int k = 0; k < m; k++) {
+            int kk = k + m * q;
+            double ar = 0.0, ai = 0.0;
+            for (int r = 0, e = 0; r < p; r++) {
+                double c = wr[e * sc], sn = wi[e * sc];
+                ar += tr[r * m + k] * c - ti[r * m + k] * sn;
+                ai += tr[r * m + k] * sn + ti[r * m + k] * c;
+                e += kk; if (e >= n) e -= n;       /* r kk mod n, kk < n */
+            }
+            yr[kk] = ar; yi[kk] = ai;
+        }
+}
+
+static void dft_any(double *re, double *im, int n, int sign)
+{
+    /* W_n^e = exp(sign 2 pi i e / n), e < n, cached per (n, sign) */
+    static int tn = 0, ts = 0;
+    static double *wr = NULL, *wi = NULL;
+    if (tn != n || ts != sign) {
+        free(wr); free(wi);
+        wr = dalloc(n); wi = dalloc(n);
+        for (int e = 0; e < n; e++) {
+            double ang = 2.0 * PI * (double)e / (double)n;
+            wr[e] = cos(ang); wi[e] = (sign < 0 ? -1.0 : 1.0) * sin(ang);
+        }
+        tn = n; ts = sign;
+    }
+    double yr[n], yi[n];
+    fft_mixed(re, im, n, 1, wr, wi, n, yr, yi);
+    memcpy(re, yr, (size_t)n * sizeof(double));
+    memcpy(im, yi, (size_t)n * sizeof(double));
+}
+
 void orc_fft(double *re, double *im, int n, int sign)
 {
+    if (n & (n - 1)) { dft_any(re, im, n, sign); return; }
     /* twiddle table w_k = exp(-2 pi i k / n), k < n/2, cached per n */
     static int tn = 0;
     static double *twr = NULL, *twi = NULL;
@@ -233,8 +299,9 @@ void orc_fft(double *re, double *im, int n, int sign)
 }
 
 /* ddx: derivatives.f90:28-58 (PERIODIC and USE_FFT branch).  r2c, multiply
- * bins 0..nx/2 by eyekx = i*2*pi*k/nx, c2r (imaginary parts of the DC and
- * Nyquist bins ignored, as FFTW's c2r does), times stretchx. */
+ * bins 0..nx/2 by eyekx = i*2*pi*k/nx, c2r (imaginary parts of the DC and,
+ * for even nx, Nyquist bins ignored, as FFTW's c2r does; for odd nx bin
+ * nx/2 = (nx-1)/2 is an ordinary bin), times stretchx. */
 static void ddx_rows(int nx, const double *eyekx, double stretchx, double *fre, double *fim,
                      const double *var, double *dvar, int nrows)
 {
@@ -243,14 +310,16 @@ static void ddx_rows(int nx, const double *eyekx, double stretchx, double *fre, 
         const double *v = var + (size_t)r * nx;
         for (int i = 0; i < nx; i++) { fre[i] = v[i]; fim[i] = 0.0; }
         orc_fft(fre, fim, nx, -1);
-        fim[0] = 0.0; fim[h] = 0.0;                /* r2c: DC/Nyquist are real */
+        fim[0] = 0.0;                              /* r2c: DC (and Nyquist) are real */
+        if (nx % 2 == 0) fim[h] = 0.0;
         for (int k = 0; k <= h; k++) {             /* derivatives.f90:38-44 */
             double ar = fre[k], ai = fim[k], w = eyekx[k];
             fre[k] = ar * 0.0 - ai * w;
             fim[k] = ar * w + ai * 0.0;
         }
-        fim[0] = 0.0; fim[h] = 0.0;                /* c2r ignores these */
-        for (int k = 1; k < h; k++) { fre[nx - k] = fre[k]; fim[nx - k] = -fim[k]; }
+        fim[0] = 0.0;                              /* c2r ignores these */
+        if (nx % 2 == 0) fim[h] = 0.0;
+        for (int k = 1; k < nx - k; k++) { fre[nx - k] = fre[k]; fim[nx - k] = -fim[k]; }
         orc_fft(fre, fim, nx, +1);                 /* derivatives.f90:46 */
         double *d = dvar + (size_t)r * nx;
         for (int i = 0; i < nx; i++) d[i] = fre[i] * stretchx; /* :56 */
@@ -283,7 +352,7 @@ orc_model *orc_create(const orc_params *p, const double *bg, const double *c2d,
                       const double *psi)
 {
     int nx = p->nx, nz = p->nz;
-    if (nx < 8 || (nx & (nx - 1)) || nz < 16 || p->npmlbot + p->npmltop >= nz ||
+    if (nx < 8 || nz < 16 || p->npmlbot + p->npmltop >= nz ||
         2 * p->npmlhor >= nx)
         return NULL;
     orc_model *m = (orc_model *)calloc(1, sizeof(orc_model));
@@ -934,7 +1003,7 @@ int orc_kernels(const orc_params *p, const double *bg, const double *c2d, int nt
 {
     int nx = p->nx, nz = p->nz;
     size_t n2 = (size_t)nx * nz;
-    if (nx < 8 || (nx & (nx - 1)) || nz < 16 || nt_kern < 1) return -1;
+    if (nx < 8 || nz < 16 || nt_kern < 1) return -1;
     double dimc = (double)1e6f;                    /* 10.0**6. */
     double dimrho = (double)1e-3f;                 /* 10.0**(-3.0) */
     double diml = RSUN;

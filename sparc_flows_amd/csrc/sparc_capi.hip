@@ -228,7 +228,7 @@ static int host_ddx(sparc_handle *h, const std::vector<double> &in, std::vector<
     HCHECK(h, hipMalloc(&di, n * sizeof(double)));
     HCHECK(h, hipMalloc(&dout, n * sizeof(double)));
     HCHECK(h, hipMemcpyAsync(di, in.data(), n * sizeof(double), hipMemcpyHostToDevice, h->stream));
-    HCHECK(h, launch_ddx_rows(di, dout, nrows, h->nx, h->bg.tw, h->bg.dmul, h->stretchx, h->stream));
+    HCHECK(h, launch_ddx_rows(di, dout, nrows, h->nx, h->bg.tw, h->bg.dmul, h->bg.cdx, h->stretchx, h->stream));
     out.resize(n);
     HCHECK(h, hipMemcpyAsync(out.data(), dout, n * sizeof(double), hipMemcpyDeviceToHost, h->stream));
     HCHECK(h, hipStreamSynchronize(h->stream));
@@ -239,13 +239,55 @@ static int host_ddx(sparc_handle *h, const std::vector<double> &in, std::vector<
 
 // Per-bin multiplier of the spectral derivative over the full complex
 // spectrum: eyekx(k) below the Nyquist bin, zero at it (FFTW's c2r drops its
-// imaginary part, derivatives.f90:46), -eyekx(nx-k) above.
+// imaginary part, derivatives.f90:46; only an even nx has one), -eyekx(nx-k)
+// above.
 static std::vector<double> dmul_table(const sparc_handle *h)
 {
     const int n = h->nx;
     std::vector<double> m(n, 0.0);
-    for (int k = 0; k < n; k++) m[k] = k < n / 2 ? h->eyekx[k] : (k == n / 2 ? 0.0 : -h->eyekx[n - k]);
+    for (int k = 0; k < n; k++) m[k] = 2 * k < n ? h->eyekx[k] : (2 * k == n ? 0.0 : -h->eyekx[n - k]);
     return m;
+}
+
+// The x-derivative's tables: twiddles exp(-2 pi i k / nx) (from the exact
+// octant values where they exist) for the FFT form, and for any other nx the
+// map's first column c[m] = sum_k i dmul[k] exp(2 pi i k m / nx)
+// = -sum_k dmul[k] sin(2 pi (k m mod nx) / nx) (real: dmul is odd in k; the
+// inverse transform's 1/nx is inside eyekx, as in ddx), stored twice over
+// (circ_deriv_row indexes [0, 2 nx) without a wrap).
+static int upload_x_tables(sparc_handle *h)
+{
+    const int nx = h->nx;
+    std::vector<double> tw(2 * (size_t)nx);
+    for (int k = 0; k < nx; k++) {
+        const double ang = -2.0 * kPi * (double)k / (double)nx;
+        double c = std::cos(ang), s = std::sin(ang);
+        if (4 * k == nx || 4 * k == 3 * nx) c = 0.0;
+        if (2 * k == nx) s = 0.0;
+        tw[2 * k] = c;
+        tw[2 * k + 1] = s;
+    }
+    const std::vector<double> dm = dmul_table(h);
+    std::vector<double> cc(2 * (size_t)nx, 0.0);
+    if (!nx_has_fft(nx)) {
+        std::vector<long double> sn(nx);
+        for (int q = 0; q < nx; q++) sn[q] = std::sin(2.0L * (long double)kPi * q / nx);
+        for (int m = 0; m < nx; m++) {
+            long double acc = 0.0L;
+            for (int k = 0; k < nx; k++) acc -= (long double)dm[k] * sn[(int)(((long)k * m) % nx)];
+            cc[m] = cc[m + nx] = (double)acc;
+        }
+    }
+    double *p = nullptr;
+    HCHECK(h, upload(h->bg_mem, &p, tw, h->stream));
+    h->bg.tw = (const double2 *)p;
+    HCHECK(h, upload(h->bg_mem, &p, h->eyekx, h->stream));
+    h->bg.eyekx = p;
+    HCHECK(h, upload(h->bg_mem, &p, dm, h->stream));
+    h->bg.dmul = p;
+    HCHECK(h, upload(h->bg_mem, &p, cc, h->stream));
+    h->bg.cdx = p;
+    return SPARC_OK;
 }
 
 // Grid, derivative tables (initialize.f90:194-225, 466-474).
@@ -258,24 +300,8 @@ static int setup_grid(sparc_handle *h)
     for (int i = 0; i < nx; i++) h->x[i] = (double)i / (double)nx;
     for (int k = 0; k <= nx / 2; k++) h->eyekx[k] = (double)k * normx * 2.0 * kPi;
     h->stretchx = kRsun / h->xlength;
-    std::vector<double> tw(2 * (size_t)nx);
-    for (int k = 0; k < nx; k++) {
-        // exp(-2 pi i k / nx) from the exact octant values where they exist
-        const double ang = -2.0 * kPi * (double)k / (double)nx;
-        double c = std::cos(ang), s = std::sin(ang);
-        if (4 * k == nx || 4 * k == 3 * nx) c = 0.0;
-        if (2 * k == nx) s = 0.0;
-        tw[2 * k] = c;
-        tw[2 * k + 1] = s;
-    }
-    double *dtw = nullptr;
-    HCHECK(h, upload(h->bg_mem, &dtw, tw, h->stream));
-    h->bg.tw = (const double2 *)dtw;
+    if (int r = upload_x_tables(h)) return r;
     double *p = nullptr;
-    HCHECK(h, upload(h->bg_mem, &p, h->eyekx, h->stream));
-    h->bg.eyekx = p;
-    HCHECK(h, upload(h->bg_mem, &p, dmul_table(h), h->stream));
-    h->bg.dmul = p;
     h->bg.stretchx = h->stretchx;
 
     h->cf = make_coef(nz);
@@ -481,19 +507,7 @@ static int upload_background(sparc_handle *h)
     // setup_background's caller order; re-upload everything here
     const int nz = h->nz;
     double *p = nullptr;
-    std::vector<double> tw(2 * (size_t)h->nx);
-    for (int k = 0; k < h->nx; k++) {
-        const double ang = -2.0 * kPi * (double)k / (double)h->nx;
-        double c = std::cos(ang), s = std::sin(ang);
-        if (4 * k == h->nx || 4 * k == 3 * h->nx) c = 0.0;
-        if (2 * k == h->nx) s = 0.0;
-        tw[2 * k] = c;
-        tw[2 * k + 1] = s;
-    }
-    HCHECK(h, upload(h->bg_mem, &p, tw, h->stream));
-    h->bg.tw = (const double2 *)p;
-    HCHECK(h, upload(h->bg_mem, &p, h->eyekx, h->stream)); h->bg.eyekx = p;
-    HCHECK(h, upload(h->bg_mem, &p, dmul_table(h), h->stream)); h->bg.dmul = p;
+    if (int r = upload_x_tables(h)) return r;
     HCHECK(h, upload(h->bg_mem, &p, h->stretch, h->stream)); h->bg.stretch = p;
     HCHECK(h, upload(h->bg_mem, &p, h->unstretch, h->stream)); h->bg.unstretch = p;
     for (int ibc = 0; ibc < 4; ibc++) {
@@ -838,8 +852,8 @@ int sparc_create(const sparc_config *cfg, sparc_handle **out)
         return SPARC_E_ARG;
     }
     const int nx = cfg->nx, nz = cfg->nz;
-    if (nx < 16 || nx > 1024 || (nx & (nx - 1))) {
-        g_create_err = "sparc_create: nx must be a power of two in [16, 1024]";
+    if (nx < 16 || nx > 1024) {
+        g_create_err = "sparc_create: nx must be in [16, 1024]";
         return SPARC_E_ARG;
     }
     if (nz < 16 || cfg->npmlbot < 1 || cfg->npmltop < 1 || cfg->npmlbot + cfg->npmltop >= nz ||
@@ -1536,7 +1550,7 @@ int sparc_get_pml(sparc_handle *h, double *pv, double *px)
 int sparc_ddx(sparc_handle *h, const double *in, double *out, int nrows)
 {
     if (!h || !in || !out || nrows < 1) return SPARC_E_ARG;
-    HCHECK(h, launch_ddx_rows(in, out, nrows, h->nx, h->bg.tw, h->bg.dmul, h->stretchx, h->stream));
+    HCHECK(h, launch_ddx_rows(in, out, nrows, h->nx, h->bg.tw, h->bg.dmul, h->bg.cdx, h->stretchx, h->stream));
     return SPARC_OK;
 }
 
@@ -1618,7 +1632,7 @@ int sparc_kernels_add(sparc_handle *h, int n_pairs, const double *fwd, const dou
         const size_t per = (size_t)Kc * n;
         double *G = h->kpair, *Dx = G + 8 * per, *Dz = Dx + 4 * per, *work = Dz + 4 * per;
         HCHECK(h, launch_gather_pairs(fwd, adj, S, nt, k0, Kc, n2, G, h->stream));
-        HCHECK(h, launch_ddx_rows(G, Dx, 4 * Kc * S * h->nz, h->nx, h->bg.tw, h->bg.dmul, h->stretchx,
+        HCHECK(h, launch_ddx_rows(G, Dx, 4 * Kc * S * h->nz, h->nx, h->bg.tw, h->bg.dmul, h->bg.cdx, h->stretchx,
                                   h->stream));
         HCHECK(h, launch_ddz_cols(G + 4 * per, Dz, h->nx, h->nz, 4 * Kc * S, 1, h->bg.stretch, h->cf, tr1,
                                   work, 1, h->stream));
@@ -1639,7 +1653,7 @@ int sparc_kernels_end(sparc_handle *h, double *kc, double *kpsi, double *kvx, do
     // curl (bcz = 0, kernels.f90:832) and unit scalings (:498-579)
     HCHECK(h, launch_ddz_cols(KX, t1, h->nx, h->nz, S, 0, h->bg.stretch, h->cf, h->tri[0], work, 1,
                               h->stream));
-    HCHECK(h, launch_ddx_rows(KZ, t2, S * h->nz, h->nx, h->bg.tw, h->bg.dmul, h->stretchx, h->stream));
+    HCHECK(h, launch_ddx_rows(KZ, t2, S * h->nz, h->nx, h->bg.tw, h->bg.dmul, h->bg.cdx, h->stretchx, h->stream));
     const double diml = kRsun, dimc = (double)1e6f, e8 = (double)1e-8f, e5 = (double)1e-5f;
     const double dimen = std::pow(diml * e8 * h->xlength * e8 * dimc * e5, -1.0) * 1.0;
     const double dimen_c = std::pow(diml * e8 * h->xlength * e8, -1.0) * 1.0;

@@ -29,6 +29,7 @@ struct Bg {
     const double *eyekx;                                   // [nx/2+1]
     const double *dmul;                                    // [nx] bin multiplier (see fft_deriv_team)
     const double2 *tw;                                     // [nx] exp(-2 pi i k/nx)
+    const double *cdx;                                     // [2 nx] circulant derivative column, twice (circ_deriv_row)
     double stretchx;
 };
 
@@ -522,11 +523,57 @@ __device__ __forceinline__ void fft_deriv_team_256x64(double (&re)[4], double (&
     }
 }
 
+// The same derivative for any nx (the form for an nx that is not a power of
+// two): the linear map that fft_deriv_team applies, multiplying bin k by
+// i*dmul[k] (for odd nx there is no Nyquist bin to drop), written as the
+// circulant product (D z)[x] = sum_y c[(x - y) mod nx] z[y] with c the map's
+// first column, then times stretchx.  O(nx^2) per row pair against the
+// FFT's O(nx log nx): a general-grid fallback, not a tuned path.
+//
+// A team of 64 lanes (one row pair z = f + i g per block); lane t holds
+// z[t + 64 j2], j2 < N2, 64 N2 >= nx; slots with t + 64 j2 >= nx carry
+// nothing and get garbage.  cc: c repeated, cc[i] = c[i mod nx] for
+// i < 2 nx (padded to 128 N2 entries), so that x - y + nx, in [1, 2 nx) for
+// a real slot, indexes it without a wrap.  z: at least 64 N2 double2.
+// Every lane of the block must call this (it contains block barriers).
+template <int N2>
+__device__ __forceinline__ void circ_deriv_row(double (&re)[N2], double (&im)[N2], int t, double2 *z,
+                                               const double *cc, int nx, double stretchx)
+{
+#pragma unroll
+    for (int j2 = 0; j2 < N2; j2++) {
+        const int x = t + 64 * j2;
+        if (x < nx) z[x] = make_double2(re[j2], im[j2]);
+    }
+    __syncthreads();
+    double ar[N2], ai[N2];
+#pragma unroll
+    for (int j2 = 0; j2 < N2; j2++) ar[j2] = ai[j2] = 0.0;
+#pragma unroll 2
+    for (int y = 0; y < nx; y++) {
+        const double2 v = z[y];            // the same address on every lane: a broadcast
+#pragma unroll
+        for (int j2 = 0; j2 < N2; j2++) {
+            const double c = cc[t + 64 * j2 - y + nx];   // < 64 N2 + nx <= 128 N2
+            ar[j2] += c * v.x;
+            ai[j2] += c * v.y;
+        }
+    }
+    __syncthreads();   // z free for the next call
+#pragma unroll
+    for (int j2 = 0; j2 < N2; j2++) {
+        re[j2] = ar[j2] * stretchx;
+        im[j2] = ai[j2] * stretchx;
+    }
+}
+
 template <int N1, int N2> struct FftLds { static constexpr int n = N2 * (N1 + 1); };
 template <> struct FftLds<32, 8> { static constexpr int n = 2 * 8 * 33; };
 template <> struct FftLds<64, 4> { static constexpr int n = 2 * 264; };
 
-// Factorization N1 x N2 of the supported row lengths (N1 >= N2).
+// Factorization N1 x N2 of the power-of-two row lengths (N1 >= N2).  Any
+// other nx runs the circulant form, a kernel template argument NX = -N2
+// (rows of at most 64 N2 on 64 lanes; any_nx_slots picks N2).
 template <int NX> struct FftShape;
 template <> struct FftShape<16> { static constexpr int N1 = 4, N2 = 4; };
 template <> struct FftShape<32> { static constexpr int N1 = 8, N2 = 4; };
@@ -535,6 +582,8 @@ template <> struct FftShape<128> { static constexpr int N1 = 16, N2 = 8; };
 template <> struct FftShape<256> { static constexpr int N1 = 16, N2 = 16; };
 template <> struct FftShape<512> { static constexpr int N1 = 32, N2 = 16; };
 template <> struct FftShape<1024> { static constexpr int N1 = 32, N2 = 32; };
+__host__ __device__ inline bool nx_has_fft(int nx) { return nx >= 16 && nx <= 1024 && (nx & (nx - 1)) == 0; }
+__host__ __device__ inline int any_nx_slots(int nx) { return nx <= 128 ? 2 : (nx <= 256 ? 4 : (nx <= 512 ? 8 : 16)); }
 
 // ---------------------------------------------------------------------------
 // Compact-FD forward elimination (dbyd2.f + mtridag.f:96-105) of one column,

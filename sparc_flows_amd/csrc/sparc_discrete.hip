@@ -424,7 +424,7 @@ static hipError_t dz_dense(const DiscreteCtx &c, const double *in, double *out, 
 
 static hipError_t ddx(const DiscreteCtx &c, const double *in, double *out, int nrows)
 {
-    return launch_ddx_rows(in, out, nrows, c.d.nx, c.bg.tw, c.bg.dmul, c.bg.stretchx, c.stream);
+    return launch_ddx_rows(in, out, nrows, c.d.nx, c.bg.tw, c.bg.dmul, c.bg.cdx, c.bg.stretchx, c.stream);
 }
 
 // One forward RHS evaluation of the stage state Y at time t: R (full state)

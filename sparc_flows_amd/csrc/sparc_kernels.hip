@@ -78,9 +78,13 @@ __global__ __launch_bounds__(64) void k_ddz_cols(const double *in, double *out,
 // The team derivative for the factorization N1 x N2, dispatched in place
 // (a wrapper function around the two forms costs the 16 x 16 row kernels
 // some 30 registers).
+// The enclosing kernel defines kAny, N1, N2, nxr and sdm (the circulant
+// column of the any-nx form, staged in LDS).
 #define SPARC_FFT_DERIV(RE, IM, T, LDS, TW, DM, SX)                                            \
     do {                                                                                       \
-        if constexpr (N1 == 32 && N2 == 8)                                                     \
+        if constexpr (kAny)                                                                    \
+            circ_deriv_row<N2>(RE, IM, T, LDS, sdm, nxr, SX);                                  \
+        else if constexpr (N1 == 32 && N2 == 8)                                                \
             fft_deriv_team_256x32(RE, IM, T, LDS, (LDS) + 8 * 33, TW, DM, SX);                 \
         else if constexpr (N1 == 64 && N2 == 4)                                                \
             fft_deriv_team_256x64(RE, IM, T, LDS, (LDS) + 264, TW, DM, SX);                    \
@@ -88,31 +92,42 @@ __global__ __launch_bounds__(64) void k_ddz_cols(const double *in, double *out,
             fft_deriv_team<N1, N2>(RE, IM, T, LDS, TW, DM, SX);                                \
     } while (0)
 
-// Generic x-derivative (ddx) of nrows real rows of length NX; two rows per
-// FFT team.
-template <int NX>
+// Generic x-derivative (ddx) of nrows real rows of length nx; two rows per
+// FFT team.  NX = nx for a power of two; NX = -N2 for the any-nx form (64
+// lanes, rows up to 64 N2, circ_deriv_row).
+template <int NX, int N1 = (NX < 0 ? 64 : FftShape<(NX < 0 ? 16 : NX)>::N1)>
 __global__ __launch_bounds__(64) void k_ddx_rows(const double *__restrict__ in,
                                                  double *__restrict__ out, int nrows,
                                                  const double2 *__restrict__ tw,
-                                                 const double *__restrict__ dmul, double stretchx)
+                                                 const double *__restrict__ dmul, double stretchx,
+                                                 const double *__restrict__ cdx, int nx)
 {
-    constexpr int N1 = FftShape<NX>::N1, N2 = FftShape<NX>::N2, TEAMS = 64 / N1;
+    constexpr bool kAny = NX < 0;
+    constexpr int N2 = kAny ? -NX : NX / N1, TEAMS = 64 / N1;
+    const int nxr = kAny ? nx : NX;
     __shared__ double2 lds[TEAMS][FftLds<N1, N2>::n];
+    __shared__ double sdm[kAny ? 128 * N2 : 1];
+    if constexpr (kAny) {
+        // visible after circ_deriv_row's first barrier
+        for (int q = threadIdx.x; q < 128 * N2; q += 64) sdm[q] = q < 2 * nxr ? cdx[q] : 0.0;
+    }
     const int team = threadIdx.x / N1, t = threadIdx.x % N1;
     const int r0 = 2 * (blockIdx.x * TEAMS + team), r1 = r0 + 1;
     double re[N2], im[N2];
 #pragma unroll
     for (int j2 = 0; j2 < N2; j2++) {
         const int x = t + N1 * j2;
-        re[j2] = r0 < nrows ? in[(size_t)r0 * NX + x] : 0.0;
-        im[j2] = r1 < nrows ? in[(size_t)r1 * NX + x] : 0.0;
+        const bool in_row = !kAny || x < nxr;
+        re[j2] = r0 < nrows && in_row ? in[(size_t)r0 * nxr + x] : 0.0;
+        im[j2] = r1 < nrows && in_row ? in[(size_t)r1 * nxr + x] : 0.0;
     }
     SPARC_FFT_DERIV(re, im, t, lds[team], tw, dmul, stretchx);
 #pragma unroll
     for (int j2 = 0; j2 < N2; j2++) {
         const int x = t + N1 * j2;
-        if (r0 < nrows) out[(size_t)r0 * NX + x] = re[j2];
-        if (r1 < nrows) out[(size_t)r1 * NX + x] = im[j2];
+        if (kAny && x >= nxr) continue;
+        if (r0 < nrows) out[(size_t)r0 * nxr + x] = re[j2];
+        if (r1 < nrows) out[(size_t)r1 * nxr + x] = im[j2];
     }
 }
 
@@ -126,16 +141,25 @@ __global__ __launch_bounds__(64) void k_ddx_rows(const double *__restrict__ in,
 // (two rows per wave, three-stage transform) or 64 lanes (one row per wave,
 // four radix-4 stages), which are faster when the pass is latency-bound (one
 // source) and slower when it is bandwidth-bound.
-template <int NX, int MODE, int N1 = FftShape<NX>::N1>
+//
+// NX = -N2: the any-nx form, one row per block of 64 lanes, rows up to
+// 64 N2 (circ_deriv_row); lane slots past nx load column nx-1 and store
+// nothing.
+template <int NX, int MODE, int N1 = (NX < 0 ? 64 : FftShape<(NX < 0 ? 16 : NX)>::N1)>
 __global__ __launch_bounds__(64) void k_row(Dims d, Bg bg, State st, RowArgs ra, Clock clk, Sink sink)
 {
-    constexpr int N2 = NX / N1, TEAMS = 64 / N1;
+    constexpr bool kAny = NX < 0;
+    static_assert(!kAny || N1 == 64, "the any-nx form runs one row on 64 lanes");
+    constexpr int N2 = kAny ? -NX : NX / N1, TEAMS = 64 / N1;
+    const int nxr = kAny ? d.nx : NX;
     __shared__ double2 lds[TEAMS][FftLds<N1, N2>::n];
-    // twiddles and bin multipliers staged in LDS: their reads then never
-    // wait behind this pass's global stores
-    __shared__ double2 stw[NX];
-    __shared__ double sdm[NX];
-    {
+    // twiddles and bin multipliers (any nx: the circulant column) staged in
+    // LDS: their reads then never wait behind this pass's global stores
+    __shared__ double2 stw[kAny ? 1 : NX];
+    __shared__ double sdm[kAny ? 128 * N2 : NX];
+    if constexpr (kAny) {
+        for (int q = threadIdx.x; q < 128 * N2; q += 64) sdm[q] = q < 2 * nxr ? bg.cdx[q] : 0.0;
+    } else {
         constexpr int NQ = (NX + 63) / 64;
         double2 tv[NQ];
         double dv[NQ];
@@ -161,10 +185,14 @@ __global__ __launch_bounds__(64) void k_row(Dims d, Bg bg, State st, RowArgs ra,
     const int b = rc / d.nz, k = rc % d.nz;
     const int nz = d.nz, nh = d.nh, nph = d.nph;
     const int time = clock_t_(clk);
-    const size_t rowq = ((size_t)b * nz + k) * NX;
+    const size_t rowq = ((size_t)b * nz + k) * nxr;
     const size_t pxq0 = ((size_t)b * nz + k) * nph;             // psivx row
     const size_t pxq1 = ((size_t)(d.S + b) * nz + k) * nph;     // psi_gradp_x row
     double *const dump = st.discard + threadIdx.x;              // target of invalid-row stores
+    // column of lane slot j2 (any nx: slots past the row load column nx-1)
+    // and whether the slot's stores land (a real row and, any nx, a real column)
+    auto colx = [&](int j2) { const int x = t + N1 * j2; return kAny ? min(x, nxr - 1) : x; };
+    auto ok = [&](int j2) { return valid && (!kAny || t + N1 * j2 < nxr); };
     double re[N2], im[N2], vx[N2];
     // x-PML auxiliary data (psi stage value and base, a_x, b_x) of the lane's
     // columns in the first (j2 = 0) and last (j2 = N2-1) column block, the
@@ -174,8 +202,8 @@ __global__ __launch_bounds__(64) void k_row(Dims d, Bg bg, State st, RowArgs ra,
     // psi_gradp_x of the v_x update, set 1: psivx of d xi_x/dx).  Only for
     // teams of at least 32 lanes (N2 <= 8): the 16-lane form holds 16
     // columns per lane, and the extra registers cost it more than the waits.
-    constexpr bool kPre = N2 <= 8;
-    auto pml_index = [&](int x) { return x < nh ? x : min(max(x - NX + 2 * nh, 0), nph - 1); };
+    constexpr bool kPre = !kAny && N2 <= 8;
+    auto pml_index = [&](int x) { return x < nh ? x : min(max(x - nxr + 2 * nh, 0), nph - 1); };
     double pre[2][2][4] = {};
 #pragma unroll
     for (int e = 0; e < (kPre ? 2 : 0); e++) {
@@ -226,23 +254,23 @@ __global__ __launch_bounds__(64) void k_row(Dims d, Bg bg, State st, RowArgs ra,
             if (ra.filt_vx) {
 #pragma unroll
                 for (int j2 = 0; j2 < N2; j2++) {
-                    const int x = t + N1 * j2;
-                    vx[j2] = filter_vars_2d_sel(st.vx_unf, rowq + x, k, x, NX, nz, nh);
+                    const int x = colx(j2);
+                    vx[j2] = filter_vars_2d_sel(st.vx_unf, rowq + x, k, x, nxr, nz, nh);
                 }
             } else {
 #pragma unroll
-                for (int j2 = 0; j2 < N2; j2++) vx[j2] = st.a[2][rowq + t + N1 * j2];
+                for (int j2 = 0; j2 < N2; j2++) vx[j2] = st.a[2][rowq + colx(j2)];
             }
 #pragma unroll
             for (int j2 = 0; j2 < N2; j2++) {
-                re[j2] = ra.xi_x[rowq + t + N1 * j2];
+                re[j2] = ra.xi_x[rowq + colx(j2)];
                 im[j2] = vx[j2];
             }
         } else {
 #pragma unroll
             for (int j2 = 0; j2 < N2; j2++) {
-                const int x = t + N1 * j2;
-                const double v = ra.filt_vx ? filter_vars_2d_sel(st.vx_unf, rowq + x, k, x, NX, nz, nh)
+                const int x = colx(j2);
+                const double v = ra.filt_vx ? filter_vars_2d_sel(st.vx_unf, rowq + x, k, x, nxr, nz, nh)
                                             : st.a[2][rowq + x];
                 vx[j2] = v;
                 re[j2] = ra.xi_x[rowq + x];
@@ -256,10 +284,10 @@ __global__ __launch_bounds__(64) void k_row(Dims d, Bg bg, State st, RowArgs ra,
             const int fslot = sink.adjoint ? 0 : 2;
 #pragma unroll
             for (int j2 = 0; j2 < N2; j2++) {
-                const int x = t + N1 * j2;
-                *(valid ? &st.a[2][rowq + x] : dump) = vx[j2];
-                if (snap && valid)
-                    sink.snap[((((size_t)b * sink.nsnap + sslot) * sink.nfield + fslot) * nz + k) * NX + x] = vx[j2];
+                const int x = colx(j2);
+                *(ok(j2) ? &st.a[2][rowq + x] : dump) = vx[j2];
+                if (snap && ok(j2))
+                    sink.snap[((((size_t)b * sink.nsnap + sslot) * sink.nfield + fslot) * nz + k) * nxr + x] = vx[j2];
             }
         }
         pin_pre();
@@ -269,28 +297,28 @@ __global__ __launch_bounds__(64) void k_row(Dims d, Bg bg, State st, RowArgs ra,
         double zr[N2], zi[N2];
 #pragma unroll
         for (int j2 = 0; j2 < N2; j2++) {
-            zr[j2] = ra.v_z[rowq + t + N1 * j2];
+            zr[j2] = ra.v_z[rowq + colx(j2)];
             zi[j2] = 0.0;
         }
         SPARC_FFT_DERIV(zr, zi, t, lds[team], stw, sdm, bg.stretchx);
 #pragma unroll
         for (int j2 = 0; j2 < N2; j2++) {
-            const int x = t + N1 * j2;
-            *(valid ? &st.dxvx[rowq + x] : dump) = im[j2];
-            *(valid ? &st.dxvz[rowq + x] : dump) = zr[j2];
+            const int x = colx(j2);
+            *(ok(j2) ? &st.dxvx[rowq + x] : dump) = im[j2];
+            *(ok(j2) ? &st.dxvz[rowq + x] : dump) = zr[j2];
         }
     } else {
         // dx p (stage s) paired with dx xi_x (stage s+1, or nothing in MODE 2)
 #pragma unroll
         for (int j2 = 0; j2 < N2; j2++) {
-            const int x = t + N1 * j2;
+            const int x = colx(j2);
             re[j2] = st.p[rowq + x];
             im[j2] = MODE == 1 ? ra.xi_x[rowq + x] : 0.0;
         }
         double rvx0[N2], avx[N2], rinv[N2];
 #pragma unroll
         for (int j2 = 0; j2 < N2; j2++) {
-            const int x = t + N1 * j2;
+            const int x = colx(j2);
             rvx0[j2] = st.rvx[rowq + x];
             avx[j2] = st.a[2][rowq + x];
             rinv[j2] = bg.rhoinv_z[k];
@@ -303,30 +331,30 @@ __global__ __launch_bounds__(64) void k_row(Dims d, Bg bg, State st, RowArgs ra,
         const bool snap = sslot >= 0;
 #pragma unroll
         for (int j2 = 0; j2 < N2; j2++) {
-            const int x = t + N1 * j2;
+            const int x = colx(j2);
             double gpx = re[j2];
-            if (N1 * j2 < nh || N1 * (j2 + 1) > NX - nh) {     // team-uniform: this j2 can hold PML columns
-                const bool pml = x < nh || x >= NX - nh;
+            if (N1 * j2 < nh || N1 * (j2 + 1) > nxr - nh) {     // team-uniform: this j2 can hold PML columns
+                const bool pml = x < nh || x >= nxr - nh;
                 const int pi = pml_index(x);
                 double psi, pb, axv, bxv;
                 pml_data(0, j2, pi, psi, pb, axv, bxv);
                 const double g2 = gpx + psi;
                 const double rhs = axv * g2 + bxv * psi;
                 const double nb = pb + ra.bx2 * rhs;
-                if (pml && valid) {
+                if (pml && ok(j2)) {
                     st.px_tmp[pxq1 + pi] = nb;
                     if (ra.last2) st.px_base[pxq1 + pi] = nb;
                 }
                 gpx = pml ? g2 : gpx;
             }
             double rvx = -rinv[j2] * gpx + rvx0[j2];
-            rvx = (k == 0 || x == 0 || x == NX - 1) ? 0.0 : rvx;
+            rvx = (k == 0 || x == 0 || x == nxr - 1) ? 0.0 : rvx;
             const double v = avx[j2] + ra.bx2 * rvx;
             vx[j2] = v;
-            *(valid ? &ra.v_x_out[rowq + x] : dump) = v;
-            if (snap && valid)
+            *(ok(j2) ? &ra.v_x_out[rowq + x] : dump) = v;
+            if (snap && ok(j2))
                 sink.snap[((((size_t)b * sink.nsnap + sslot) * sink.nfield +
-                            (sink.adjoint ? 2 : 4)) * nz + k) * NX + x] = rvx;
+                            (sink.adjoint ? 2 : 4)) * nz + k) * nxr + x] = rvx;
         }
         if (MODE == 1) {
             // im holds dxi_x/dx of stage s+1; now dv_x/dx, dv_z/dx of stage s+1
@@ -334,14 +362,14 @@ __global__ __launch_bounds__(64) void k_row(Dims d, Bg bg, State st, RowArgs ra,
 #pragma unroll
             for (int j2 = 0; j2 < N2; j2++) {
                 wr[j2] = vx[j2];
-                wi[j2] = ra.v_z[rowq + t + N1 * j2];
+                wi[j2] = ra.v_z[rowq + colx(j2)];
             }
             SPARC_FFT_DERIV(wr, wi, t, lds[team], stw, sdm, bg.stretchx);
 #pragma unroll
             for (int j2 = 0; j2 < N2; j2++) {
-                const int x = t + N1 * j2;
-                *(valid ? &st.dxvx[rowq + x] : dump) = wr[j2];
-                *(valid ? &st.dxvz[rowq + x] : dump) = wi[j2];
+                const int x = colx(j2);
+                *(ok(j2) ? &st.dxvx[rowq + x] : dump) = wr[j2];
+                *(ok(j2) ? &st.dxvz[rowq + x] : dump) = wi[j2];
             }
 #pragma unroll
             for (int j2 = 0; j2 < N2; j2++) re[j2] = im[j2];
@@ -352,24 +380,24 @@ __global__ __launch_bounds__(64) void k_row(Dims d, Bg bg, State st, RowArgs ra,
     // (physics2d.f90:954, before the x-PML), then the x-PML (:970-986)
 #pragma unroll
     for (int j2 = 0; j2 < N2; j2++) {
-        const int x = t + N1 * j2;
+        const int x = colx(j2);
         double dx = re[j2];
-        if (k == nz - 1 && valid) st.topbc[(size_t)b * NX + x] = -dx * unst;
-        if (N1 * j2 < nh || N1 * (j2 + 1) > NX - nh) {
-            const bool pml = x < nh || x >= NX - nh;
+        if (k == nz - 1 && ok(j2)) st.topbc[(size_t)b * nxr + x] = -dx * unst;
+        if (N1 * j2 < nh || N1 * (j2 + 1) > nxr - nh) {
+            const bool pml = x < nh || x >= nxr - nh;
             const int pi = pml_index(x);
             double psi, pb, axv, bxv;
             pml_data(1, j2, pi, psi, pb, axv, bxv);
             const double d2 = dx + psi;
             const double rhs = axv * d2 + bxv * psi;
             const double nb = pb + ra.bx1 * rhs;
-            if (pml && valid) {
+            if (pml && ok(j2)) {
                 st.px_tmp[pxq0 + pi] = nb;
                 if (ra.last1) st.px_base[pxq0 + pi] = nb;
             }
             dx = pml ? d2 : dx;
         }
-        *(valid ? &st.dxixdx[rowq + x] : dump) = dx;
+        *(ok(j2) ? &st.dxixdx[rowq + x] : dump) = dx;
     }
 }
 
@@ -553,15 +581,26 @@ hipError_t launch_vz_norm2(const double *vz, int S, size_t n2, const Clock &c, c
 }
 
 hipError_t launch_ddx_rows(const double *in, double *out, int nrows, int nx, const double2 *tw,
-                           const double *dmul, double stretchx, hipStream_t st)
+                           const double *dmul, const double *cdx, double stretchx, hipStream_t st)
 {
+    const int npairs = (nrows + 1) / 2;
+    if (!nx_has_fft(nx)) {
+        if (nx < 16 || nx > 1024) return hipErrorInvalidValue;
+        // any nx: one row pair per 64-lane block
+        const int n2 = any_nx_slots(nx);
+        dim3 grid(npairs);
+        if (n2 == 2) hipLaunchKernelGGL((k_ddx_rows<-2>), grid, dim3(64), 0, st, in, out, nrows, tw, dmul, stretchx, cdx, nx);
+        else if (n2 == 4) hipLaunchKernelGGL((k_ddx_rows<-4>), grid, dim3(64), 0, st, in, out, nrows, tw, dmul, stretchx, cdx, nx);
+        else if (n2 == 8) hipLaunchKernelGGL((k_ddx_rows<-8>), grid, dim3(64), 0, st, in, out, nrows, tw, dmul, stretchx, cdx, nx);
+        else hipLaunchKernelGGL((k_ddx_rows<-16>), grid, dim3(64), 0, st, in, out, nrows, tw, dmul, stretchx, cdx, nx);
+        return hipGetLastError();
+    }
 #define L(NXV)                                                                              \
     {                                                                                       \
         constexpr int TEAMS = 64 / FftShape<NXV>::N1;                                      \
-        const int npairs = (nrows + 1) / 2;                                                 \
         dim3 grid((npairs + TEAMS - 1) / TEAMS);                                            \
         hipLaunchKernelGGL(k_ddx_rows<NXV>, grid, dim3(64), 0, st, in, out, nrows, tw, dmul, \
-                           stretchx);                                                       \
+                           stretchx, cdx, nx);                                              \
     }
     SPARC_DISPATCH_NX(nx, L)
 #undef L
@@ -583,6 +622,23 @@ hipError_t launch_row(int mode, const Dims &d, const Bg &bg, const State &s, con
         if (mode == 0) hipLaunchKernelGGL((k_row<256, 0, 32>), grid, dim3(64), 0, st, d, bg, s, ra, c, sk);
         else if (mode == 1) hipLaunchKernelGGL((k_row<256, 1, 32>), grid, dim3(64), 0, st, d, bg, s, ra, c, sk);
         else hipLaunchKernelGGL((k_row<256, 2, 32>), grid, dim3(64), 0, st, d, bg, s, ra, c, sk);
+        return hipGetLastError();
+    }
+    if (!nx_has_fft(d.nx)) {
+        if (d.nx < 16 || d.nx > 1024) return hipErrorInvalidValue;
+        // any nx: one row per 64-lane block
+        dim3 grid(d.S * d.nz);
+#define A(N2V)                                                                                   \
+        if (mode == 0) hipLaunchKernelGGL((k_row<-N2V, 0, 64>), grid, dim3(64), 0, st, d, bg, s, ra, c, sk); \
+        else if (mode == 1) hipLaunchKernelGGL((k_row<-N2V, 1, 64>), grid, dim3(64), 0, st, d, bg, s, ra, c, sk); \
+        else hipLaunchKernelGGL((k_row<-N2V, 2, 64>), grid, dim3(64), 0, st, d, bg, s, ra, c, sk);
+        switch (any_nx_slots(d.nx)) {
+        case 2: A(2) break;
+        case 4: A(4) break;
+        case 8: A(8) break;
+        default: A(16) break;
+        }
+#undef A
         return hipGetLastError();
     }
 #define L(NXV)                                                                                   \

@@ -7,8 +7,10 @@ namespace sparc {
 hipError_t launch_ddz_cols(const double *in, double *out, int ncols, int nz, int nbatch, int ibc,
                            const double *stretch, const DbyCoef &cf, const Tri &tr, double *work,
                            int filt, hipStream_t st);
+// x-derivative of nrows rows of length nx: the FFT form for a power of two,
+// the circulant product (cdx, Bg::cdx's layout) for any other nx in [16, 1024].
 hipError_t launch_ddx_rows(const double *in, double *out, int nrows, int nx, const double2 *tw,
-                           const double *eyekx, double stretchx, hipStream_t st);
+                           const double *dmul, const double *cdx, double stretchx, hipStream_t st);
 // Row pass; at nx = 256 wide = 1 runs 32 lanes per row (two rows per wave),
 // wide = 2 runs 64 lanes per row (one row per wave).
 hipError_t launch_row(int mode, const Dims &d, const Bg &bg, const State &s, const RowArgs &ra,

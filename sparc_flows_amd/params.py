@@ -68,8 +68,8 @@ class SparcParams:
         return maxtime // self.steps + 1
 
     def validate(self) -> None:
-        if self.nx < 16 or self.nx & (self.nx - 1) or self.nx > 1024:
-            raise ValueError(f"nx={self.nx}: the FFT x-derivative needs a power of two in [16, 1024]")
+        if self.nx < 16 or self.nx > 1024:
+            raise ValueError(f"nx={self.nx}: the x-derivative takes nx in [16, 1024]")
         if self.nz < 16 or self.npmlbot + self.npmltop >= self.nz:
             raise ValueError(f"nz={self.nz} too small for npml {self.npmlbot}+{self.npmltop}")
         if 2 * self.npmlhor >= self.nx:

@@ -95,7 +95,7 @@ def _model(nx=64, nz=40):
 
 def test_fft_matches_numpy():
     rng = np.random.default_rng(0)
-    for n in (8, 64, 256):
+    for n in (8, 64, 256, 12, 75, 300, 73, 511, 1000):   # radix 2, mixed radix, prime lengths
         re, im = rng.standard_normal(n), rng.standard_normal(n)
         fr, fi = oracle.fft(re, im, -1)
         ref = np.fft.fft(re + 1j * im)
@@ -104,19 +104,25 @@ def test_fft_matches_numpy():
         assert np.allclose(br + 1j * bi, np.fft.ifft(re + 1j * im) * n, rtol=0, atol=1e-12 * n)
 
 
-@pytest.mark.parametrize("nx", [16, 64, 256])
+@pytest.mark.parametrize("nx", [16, 64, 256, 48, 100, 75])
 def test_ddx_fourier_modes(nx):
+    # every mode below the Nyquist bin is differentiated exactly; for odd nx
+    # the last bin, (nx-1)/2, is such a mode (only an even nx has a Nyquist
+    # bin, which c2r drops)
     p, bg, o = _model(nx=nx)
     i = np.arange(nx)
     sx = o.info("stretchx")
-    for m in (1, 3, nx // 2 - 1):
-        f = np.tile(np.sin(2 * np.pi * m * i / nx), (p.nz, 1))
-        d = o.ddx(f)
-        exact = np.tile(2 * np.pi * m * np.cos(2 * np.pi * m * i / nx) * sx, (p.nz, 1))
-        assert rel_l2(d, exact) < 1e-13, m
-    nyq = np.tile((-1.0) ** i, (p.nz, 1))
-    assert np.max(np.abs(o.ddx(nyq))) == 0.0 or np.max(np.abs(o.ddx(nyq))) < 1e-12 * sx
-    assert np.max(np.abs(o.ddx(np.full((p.nz, nx), 3.0)))) < 1e-13 * sx
+    for m in (1, 3, (nx - 1) // 2):
+        for f0, d0 in ((np.sin, np.cos), (np.cos, lambda a: -np.sin(a))):
+            f = np.tile(f0(2 * np.pi * m * i / nx), (p.nz, 1))
+            d = o.ddx(f)
+            exact = np.tile(2 * np.pi * m * d0(2 * np.pi * m * i / nx) * sx, (p.nz, 1))
+            assert rel_l2(d, exact) < 1e-13, m
+    if nx % 2 == 0:
+        nyq = np.tile((-1.0) ** i, (p.nz, 1))
+        assert np.max(np.abs(o.ddx(nyq))) < (1e-12 if nx & (nx - 1) == 0 else 1e-13 * nx) * sx
+    # a constant: exactly 0 through radix 2, rounding of the direct DFT's sums otherwise
+    assert np.max(np.abs(o.ddx(np.full((p.nz, nx), 3.0)))) < (1e-13 if nx & (nx - 1) == 0 else 5e-14 * nx) * sx
 
 
 # --------------------------------------------------------------- filters ----
