@@ -30,7 +30,7 @@ extern "C" {
  * frequencies), the background model file rows (solar_data,
  * initialize.f90:743-747) and an optional 3D sound speed. */
 typedef struct sparc3d_config {
-    int nx, ny, nz;           /* nx, ny: powers of two in [16, 1024]; nz >= 16 */
+    int nx, ny, nz;           /* nx even, ny any, both in [16, 1024] (FFT for powers of two, circulant form otherwise); nz >= 16 */
     int npmlbot, npmltop;     /* z-PML rows, each >= 5 (the z filter reaches 5 rows) */
     int adjoint;              /* 0: forward, 1: adjoint */
     double xlength, ylength;  /* cm */

@@ -78,6 +78,37 @@ __global__ __launch_bounds__(256) void k3_ddy(const double *__restrict__ in, dou
     }
 }
 
+// The y-derivative for any ny that is not a power of two: circ_deriv_row
+// along y on one column pair (x0, x0 + 1) per 64-lane block (nx even, so
+// every pair is 16-byte aligned); cdy holds the derivative's circulant
+// column twice over.  Grid: exactly nplanes * nx/2 blocks.
+template <int N2>
+__global__ __launch_bounds__(64) void k3_ddy_any(const double *__restrict__ in, double *__restrict__ out,
+                                                 int nx, int ny, const double *__restrict__ cdy, double stretch)
+{
+    __shared__ double2 z[64 * N2];
+    __shared__ double cc[128 * N2];
+    for (int q = threadIdx.x; q < 128 * N2; q += 64) cc[q] = q < 2 * ny ? cdy[q] : 0.0;
+    const int t = threadIdx.x, hx = nx >> 1;
+    const long pl = (long)blockIdx.x / hx;
+    const int x0 = 2 * (int)((long)blockIdx.x - pl * hx);
+    const size_t base = (size_t)pl * ny * nx + x0;
+    double re[N2], im[N2];
+#pragma unroll
+    for (int j2 = 0; j2 < N2; j2++) {
+        const int y = min(t + 64 * j2, ny - 1);
+        const double2 v = *(const double2 *)(in + base + (size_t)y * nx);
+        re[j2] = v.x;
+        im[j2] = v.y;
+    }
+    circ_deriv_row<N2>(re, im, t, z, cc, ny, stretch);
+#pragma unroll
+    for (int j2 = 0; j2 < N2; j2++) {
+        const int y = t + 64 * j2;
+        if (y < ny) *(double2 *)(out + base + (size_t)y * nx) = make_double2(re[j2], im[j2]);
+    }
+}
+
 // Arguments of the two z passes.
 struct Z3 {
     const double *B;            // Zp: xi_z; Zv: v_x (v_y, v_z follow at +N, +2N)
@@ -388,8 +419,19 @@ __global__ __launch_bounds__(256) void k3_kern_final(const double *__restrict__ 
     }
 
 static hipError_t launch_ddy(const double *in, double *out, int nplanes, int nx, int ny, const double2 *tw,
-                             const double *dmul, double stretch, hipStream_t st)
+                             const double *dmul, const double *cdy, double stretch, hipStream_t st)
 {
+    if (!nx_has_fft(ny)) {
+        if (ny < 16 || ny > 1024 || (nx & 1)) return hipErrorInvalidValue;
+        const dim3 grid((unsigned)((long)nplanes * (nx / 2)));
+        switch (any_nx_slots(ny)) {
+        case 2: hipLaunchKernelGGL((k3_ddy_any<2>), grid, dim3(64), 0, st, in, out, nx, ny, cdy, stretch); break;
+        case 4: hipLaunchKernelGGL((k3_ddy_any<4>), grid, dim3(64), 0, st, in, out, nx, ny, cdy, stretch); break;
+        case 8: hipLaunchKernelGGL((k3_ddy_any<8>), grid, dim3(64), 0, st, in, out, nx, ny, cdy, stretch); break;
+        default: hipLaunchKernelGGL((k3_ddy_any<16>), grid, dim3(64), 0, st, in, out, nx, ny, cdy, stretch); break;
+        }
+        return hipGetLastError();
+    }
 #define L(NYV)                                                                                     \
     {                                                                                              \
         constexpr int BLK = DdyBlock<NYV>::n, TEAMS = BLK / FftShape<NYV>::N1;                     \
@@ -448,6 +490,7 @@ struct sparc3d_handle {
     Tri tri[4]{};
     DevArena bg_mem, st_mem, src_mem;
     const double2 *twx = nullptr, *twy = nullptr;
+    const double *cdx = nullptr, *cdy = nullptr;      // circulant columns [2n] (an n that is not a power of two)
     const double *dmx = nullptr, *dmy = nullptr, *d_stretch = nullptr, *d_rhoinv = nullptr, *d_rho0 = nullptr,
                  *d_g = nullptr, *d_srcdep = nullptr;
     double *c2rho0 = nullptr, *gp0 = nullptr, *gr0 = nullptr, *az = nullptr, *bz = nullptr;
@@ -486,7 +529,6 @@ static int fail3(sparc3d_handle *h, int code, const std::string &msg)
     return code;
 }
 
-static bool pow2_in(int n, int lo, int hi) { return n >= lo && n <= hi && (n & (n - 1)) == 0; }
 
 // Launch with HIP events around it when profiling (kinds as in sparc3d_profile).
 template <class F>
@@ -512,17 +554,16 @@ static int timed3(sparc3d_handle *h, int kind, F launch)
         if (r_) return r_;                                     \
     } while (0)
 
-// ddx of nrows rows, ddy of nplanes planes (device).  3D grids are powers of
-// two (checked at create), so the FFT form runs and no circulant table is needed.
+// ddx of nrows rows, ddy of nplanes planes (device).
 static int ddx3(sparc3d_handle *h, const double *in, double *out, int nrows)
 {
-    RUN3(1, launch_ddx_rows(in, out, nrows, h->nx, h->twx, h->dmx, nullptr, h->stretchx, h->stream));
+    RUN3(1, launch_ddx_rows(in, out, nrows, h->nx, h->twx, h->dmx, h->cdx, h->stretchx, h->stream));
     return SPARC_OK;
 }
 
 static int ddy3(sparc3d_handle *h, const double *in, double *out, int nplanes)
 {
-    RUN3(2, launch_ddy(in, out, nplanes, h->nx, h->ny, h->twy, h->dmy, h->stretchy, h->stream));
+    RUN3(2, launch_ddy(in, out, nplanes, h->nx, h->ny, h->twy, h->dmy, h->cdy, h->stretchy, h->stream));
     return SPARC_OK;
 }
 
@@ -645,8 +686,11 @@ static int setup3(sparc3d_handle *h)
     const int nx = h->nx, ny = h->ny, nz = h->nz;
     const size_t nxy = h->nxy, N = h->n;
     const double dimc = 1e6, dimrho = 1e-3, diml = kRsun3, dimc2 = std::pow(dimc, 2.0);
-    // x, y spectral tables (initialize.f90:194-225): exp(-2 pi i k/n) and i k
-    auto tables = [&](int n, const double2 **tw, const double **dm) -> int {
+    // x, y spectral tables (initialize.f90:194-225): exp(-2 pi i k/n), i k
+    // (zero at the Nyquist bin of an even n), and for an n that is not a
+    // power of two the derivative's circulant column c[m] = -sum_k dm[k]
+    // sin(2 pi (k m mod n)/n), twice over (circ_deriv_row)
+    auto tables = [&](int n, const double2 **tw, const double **dm, const double **cd) -> int {
         std::vector<double> t(2 * (size_t)n), m(n);
         for (int k = 0; k < n; k++) {
             const double ang = -2.0 * kPi3 * (double)k / (double)n;
@@ -658,20 +702,32 @@ static int setup3(sparc3d_handle *h)
         }
         const double norm = 1.0 / (double)n;
         for (int k = 0; k < n; k++) {
-            const int kk = k < n / 2 ? k : (k == n / 2 ? -1 : n - k);
+            const int kk = 2 * k < n ? k : (2 * k == n ? -1 : n - k);
             const double e = kk < 0 ? 0.0 : (double)kk * norm * 2.0 * kPi3;
-            m[k] = k <= n / 2 ? e : -e;
+            m[k] = 2 * k <= n ? e : -e;
+        }
+        std::vector<double> c(2 * (size_t)n, 0.0);
+        if (!nx_has_fft(n)) {
+            std::vector<long double> sn(n);
+            for (int q = 0; q < n; q++) sn[q] = std::sin(2.0L * (long double)kPi3 * q / n);
+            for (int j = 0; j < n; j++) {
+                long double acc = 0.0L;
+                for (int k = 0; k < n; k++) acc -= (long double)m[k] * sn[(int)(((long)k * j) % n)];
+                c[j] = c[j + n] = (double)acc;
+            }
         }
         double *p = nullptr;
         HCHECK3(h, upload(h->bg_mem, &p, t, h->stream));
         *tw = (const double2 *)p;
         HCHECK3(h, upload(h->bg_mem, &p, m, h->stream));
         *dm = p;
+        HCHECK3(h, upload(h->bg_mem, &p, c, h->stream));
+        *cd = p;
         return SPARC_OK;
     };
     int r;
-    if ((r = tables(nx, &h->twx, &h->dmx))) return r;
-    if ((r = tables(ny, &h->twy, &h->dmy))) return r;
+    if ((r = tables(nx, &h->twx, &h->dmx, &h->cdx))) return r;
+    if ((r = tables(ny, &h->twy, &h->dmy, &h->cdy))) return r;
     h->stretchx = kRsun3 / h->xlength;
     h->stretchy = kRsun3 / h->ylength;
     h->cf = make_coef(nz);
@@ -852,8 +908,8 @@ int sparc3d_create(const sparc3d_config *cfg, sparc3d_handle **out)
         return SPARC_E_ARG;
     }
     *out = nullptr;
-    if (!pow2_in(cfg->nx, 16, 1024) || !pow2_in(cfg->ny, 16, 1024)) {
-        g_create_err3 = "nx and ny must be powers of two in [16, 1024]";
+    if (cfg->nx < 16 || cfg->nx > 1024 || (cfg->nx & 1) || cfg->ny < 16 || cfg->ny > 1024) {
+        g_create_err3 = "nx must be even and in [16, 1024], ny in [16, 1024]";
         return SPARC_E_ARG;
     }
     if (cfg->nz < 16 || cfg->npmlbot < 5 || cfg->npmltop < 5 || cfg->npmlbot + cfg->npmltop + 1 > cfg->nz) {

@@ -50,8 +50,10 @@ class Params3D:
 
     def validate(self) -> None:
         for n, v in (("nx", self.nx), ("ny", self.ny)):
-            if v < 16 or v > 1024 or v & (v - 1):
-                raise ValueError(f"{n}={v}: the FFT derivatives need a power of two in [16, 1024]")
+            if v < 16 or v > 1024:
+                raise ValueError(f"{n}={v}: the horizontal derivatives take [16, 1024]")
+        if self.nx % 2:
+            raise ValueError(f"nx={self.nx}: the y-derivative pairs columns, nx must be even")
         if self.npmlbot < 5 or self.npmltop < 5 or self.npmlbot + self.npmltop + 1 > self.nz:
             raise ValueError("z-PML rows of at least 5 each inside nz are needed")
 

@@ -50,8 +50,11 @@ def pair(p, bg, c3d, fl, adjoint=False):
     return o, s
 
 
-def test_derivatives_match_oracle():
-    p, bg, c3d, fl = case()
+# (48, 21), (20, 75), (24, 300), (18, 600): the circulant forms for an nx or
+# ny that is not a power of two (every slot count of the y form; nx even)
+@pytest.mark.parametrize("nx,ny", [(32, 16), (48, 21), (20, 75), (24, 300), (18, 600)])
+def test_derivatives_match_oracle(nx, ny):
+    p, bg, c3d, fl = case(nx=nx, ny=ny)
     o, s = pair(p, bg, c3d, fl)
     rng = np.random.default_rng(1)
     f = rng.standard_normal((2, p.nz, p.ny, p.nx))
@@ -91,9 +94,9 @@ def test_one_step_matches_oracle(t):
     assert rel_l2(s.pml().cpu().numpy(), o.pmlvars) < 1e-11
 
 
-@pytest.mark.parametrize("flow,dc", [(False, False), (True, True)])
-def test_forward_run_matches_oracle(flow, dc):
-    p, bg, c3d, fl = case(flow=flow, dc=dc)
+@pytest.mark.parametrize("flow,dc,nx,ny", [(False, False, 32, 16), (True, True, 32, 16), (True, True, 48, 21)])
+def test_forward_run_matches_oracle(flow, dc, nx, ny):
+    p, bg, c3d, fl = case(flow=flow, dc=dc, nx=nx, ny=ny)
     o, s = pair(p, bg, c3d, fl)
     maxtime = 200
     o.set_forward_source(5.0, 0.0, maxtime)
@@ -248,3 +251,10 @@ def test_profile_times_every_pass():
     s.step(0)
     s.step(1)
     assert torch.equal(a, s.state())
+
+
+def test_odd_nx_is_rejected():
+    """The y-derivative pairs columns (16-byte loads), so 3D nx must be even."""
+    p, bg, c3d, fl = case(nx=33, ny=16, flow=False, dc=False)
+    with pytest.raises(ValueError):
+        Sparc3D(p, bg)
