@@ -375,6 +375,10 @@ def main():
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a plain device-to-device copy of this many MiB ten times "
                          "(calibrates the FETCH_SIZE / WRITE_SIZE counters) and exit")
+    ap.add_argument("--forward-grid", default="",
+                    help="only time a one-source forward run (flow and sound-speed bump, --nsteps steps) "
+                         "on this NXxNZ grid, e.g. 300x300 (an nx that is not a power of two runs the "
+                         "circulant row pass), print it and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1",
@@ -398,6 +402,10 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         print(json.dumps({"copy_probe_bytes": n * 8, "launches": 10}))
+        return
+    if args.forward_grid:
+        nx, nz = (int(v) for v in args.forward_grid.lower().split("x"))
+        print(json.dumps({f"forward_{nx}x{nz}": forward_rate(nx, nz, 1, args.nsteps, True, True)}))
         return
     if args.only_3d:
         f3 = forward3d_rate()

@@ -425,10 +425,11 @@ static hipError_t launch_ddy(const double *in, double *out, int nplanes, int nx,
         if (ny < 16 || ny > 1024 || (nx & 1)) return hipErrorInvalidValue;
         const dim3 grid((unsigned)((long)nplanes * (nx / 2)));
         switch (any_nx_slots(ny)) {
-        case 2: hipLaunchKernelGGL((k3_ddy_any<2>), grid, dim3(64), 0, st, in, out, nx, ny, cdy, stretch); break;
-        case 4: hipLaunchKernelGGL((k3_ddy_any<4>), grid, dim3(64), 0, st, in, out, nx, ny, cdy, stretch); break;
-        case 8: hipLaunchKernelGGL((k3_ddy_any<8>), grid, dim3(64), 0, st, in, out, nx, ny, cdy, stretch); break;
-        default: hipLaunchKernelGGL((k3_ddy_any<16>), grid, dim3(64), 0, st, in, out, nx, ny, cdy, stretch); break;
+#define A(N2V)                                                                                           \
+        case N2V: hipLaunchKernelGGL((k3_ddy_any<N2V>), grid, dim3(64), 0, st, in, out, nx, ny, cdy, stretch); break;
+            SPARC_ANY_NX_SLOTS(A)
+#undef A
+        default: return hipErrorInvalidValue;
         }
         return hipGetLastError();
     }

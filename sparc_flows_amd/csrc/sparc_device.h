@@ -583,7 +583,14 @@ template <> struct FftShape<256> { static constexpr int N1 = 16, N2 = 16; };
 template <> struct FftShape<512> { static constexpr int N1 = 32, N2 = 16; };
 template <> struct FftShape<1024> { static constexpr int N1 = 32, N2 = 32; };
 __host__ __device__ inline bool nx_has_fft(int nx) { return nx >= 16 && nx <= 1024 && (nx & (nx - 1)) == 0; }
-__host__ __device__ inline int any_nx_slots(int nx) { return nx <= 128 ? 2 : (nx <= 256 ? 4 : (nx <= 512 ? 8 : 16)); }
+// Slots per lane of the circulant form: the smallest instantiated N2 with
+// 64 N2 >= nx (SPARC_ANY_NX_SLOTS lists them), so that few slots are idle.
+#define SPARC_ANY_NX_SLOTS(X) X(2) X(3) X(4) X(5) X(6) X(8) X(10) X(12) X(16)
+__host__ __device__ inline int any_nx_slots(int nx)
+{
+    const int need = (nx + 63) / 64;
+    return need <= 2 ? 2 : (need <= 6 ? need : (need <= 8 ? 8 : (need <= 10 ? 10 : (need <= 12 ? 12 : 16))));
+}
 
 // ---------------------------------------------------------------------------
 // Compact-FD forward elimination (dbyd2.f + mtridag.f:96-105) of one column,

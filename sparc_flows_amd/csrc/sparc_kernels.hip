@@ -587,12 +587,17 @@ hipError_t launch_ddx_rows(const double *in, double *out, int nrows, int nx, con
     if (!nx_has_fft(nx)) {
         if (nx < 16 || nx > 1024) return hipErrorInvalidValue;
         // any nx: one row pair per 64-lane block
-        const int n2 = any_nx_slots(nx);
         dim3 grid(npairs);
-        if (n2 == 2) hipLaunchKernelGGL((k_ddx_rows<-2>), grid, dim3(64), 0, st, in, out, nrows, tw, dmul, stretchx, cdx, nx);
-        else if (n2 == 4) hipLaunchKernelGGL((k_ddx_rows<-4>), grid, dim3(64), 0, st, in, out, nrows, tw, dmul, stretchx, cdx, nx);
-        else if (n2 == 8) hipLaunchKernelGGL((k_ddx_rows<-8>), grid, dim3(64), 0, st, in, out, nrows, tw, dmul, stretchx, cdx, nx);
-        else hipLaunchKernelGGL((k_ddx_rows<-16>), grid, dim3(64), 0, st, in, out, nrows, tw, dmul, stretchx, cdx, nx);
+        switch (any_nx_slots(nx)) {
+#define A(N2V)                                                                                   \
+        case N2V:                                                                                \
+            hipLaunchKernelGGL((k_ddx_rows<-N2V>), grid, dim3(64), 0, st, in, out, nrows, tw, dmul, \
+                               stretchx, cdx, nx);                                              \
+            break;
+            SPARC_ANY_NX_SLOTS(A)
+#undef A
+        default: return hipErrorInvalidValue;
+        }
         return hipGetLastError();
     }
 #define L(NXV)                                                                              \
@@ -629,14 +634,14 @@ hipError_t launch_row(int mode, const Dims &d, const Bg &bg, const State &s, con
         // any nx: one row per 64-lane block
         dim3 grid(d.S * d.nz);
 #define A(N2V)                                                                                   \
-        if (mode == 0) hipLaunchKernelGGL((k_row<-N2V, 0, 64>), grid, dim3(64), 0, st, d, bg, s, ra, c, sk); \
-        else if (mode == 1) hipLaunchKernelGGL((k_row<-N2V, 1, 64>), grid, dim3(64), 0, st, d, bg, s, ra, c, sk); \
-        else hipLaunchKernelGGL((k_row<-N2V, 2, 64>), grid, dim3(64), 0, st, d, bg, s, ra, c, sk);
+        case N2V:                                                                                \
+            if (mode == 0) hipLaunchKernelGGL((k_row<-N2V, 0, 64>), grid, dim3(64), 0, st, d, bg, s, ra, c, sk); \
+            else if (mode == 1) hipLaunchKernelGGL((k_row<-N2V, 1, 64>), grid, dim3(64), 0, st, d, bg, s, ra, c, sk); \
+            else hipLaunchKernelGGL((k_row<-N2V, 2, 64>), grid, dim3(64), 0, st, d, bg, s, ra, c, sk); \
+            break;
         switch (any_nx_slots(d.nx)) {
-        case 2: A(2) break;
-        case 4: A(4) break;
-        case 8: A(8) break;
-        default: A(16) break;
+            SPARC_ANY_NX_SLOTS(A)
+        default: return hipErrorInvalidValue;
         }
 #undef A
         return hipGetLastError();

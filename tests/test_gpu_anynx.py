@@ -3,7 +3,7 @@ FFTW, derivatives.f90:28-58): the row pass's and the x-derivative's
 circulant form against the oracle, whose ddx takes a direct DFT for such nx
 (pinned by the Fourier-mode closed forms of tests/test_oracle.py, odd nx
 included).  The slot counts 64 N2 >= nx cover every instantiated form:
-N2 = 2 (nx <= 128), 4 (<= 256), 8 (<= 512), 16 (<= 1024).
+N2 = 2 (nx <= 128), 3, 4, 5, 6 (<= 384), 8 (<= 512), 10, 12, 16 (<= 1024).
 
 Tolerances as for the power-of-two grids: single operators 1e-12 (the
 circulant sums round differently from the FFT), multi-step runs 1e-10.
@@ -29,7 +29,7 @@ def _dev(a):
     return torch.as_tensor(np.ascontiguousarray(a), dtype=torch.float64, device="cuda")
 
 
-@pytest.mark.parametrize("nx", [48, 75, 100, 129, 200, 300, 511, 600, 1000])
+@pytest.mark.parametrize("nx", [48, 75, 100, 129, 200, 300, 350, 511, 600, 700, 1000])
 def test_ddx_any_nx_matches_oracle(nx):
     p, bg, _, _ = small_case(nx=nx, nz=20, npml=4, flow=False, dc=False)
     s = Sparc(p, bg)
@@ -59,7 +59,7 @@ def test_forward_trace_any_nx(nx, maxtime):
 
 
 def test_steps_any_nx_wide_rows():
-    """20 full steps at nx = 600 (the 16-slot form) against the oracle."""
+    """20 full steps at nx = 600 (the 10-slot form) against the oracle."""
     p, bg, c2d, psi = small_case(nx=600, nz=40, npml=4, flow=True, dc=True)
     s = Sparc(p, bg, c2d=c2d, psi=psi)
     o = oracle.Oracle(p, bg, c2d=c2d, psi=psi)
